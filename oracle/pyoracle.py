@@ -1,0 +1,171 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the timed CPU baseline.
+The product package (falcon_amd) never imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+HARNESS = os.path.join(HERE, "_ref", "falcon_ref_harness")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        vp, ip, dp = C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double)
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_divide_all.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.orc_amr.argtypes = [vp, ip, C.c_int, dp]
+        L.orc_finalize.argtypes = [vp]
+        L.orc_nblocks.argtypes = [vp]
+        L.orc_nleaf_yin.argtypes = [vp]
+        L.orc_block_info.argtypes = [vp, C.c_int, dp, ip]
+        L.orc_block_centres.argtypes = [vp, C.c_int, dp, dp, dp]
+        L.orc_gc_iblocks.argtypes = [vp, C.c_int, ip]
+        L.orc_ntargets.argtypes = [vp, C.c_int]
+        L.orc_target.argtypes = [vp, C.c_int, C.c_int, ip, dp]
+        L.orc_target_donor.argtypes = [vp, C.c_int, C.c_int]
+        L.orc_nsources.argtypes = [vp, C.c_int]
+        L.orc_source.argtypes = [vp, C.c_int, C.c_int, ip, ip, dp]
+        L.orc_set_background.argtypes = [vp, C.c_int, dp]
+        L.orc_set_state.argtypes = [vp, C.c_int, dp]
+        L.orc_get_state.argtypes = [vp, C.c_int, dp]
+        L.orc_exchange.argtypes = [vp, C.c_int]
+        L.orc_rhs.argtypes = [vp, C.c_int, dp]
+        L.orc_diffusion.argtypes = [vp, C.c_int, dp]
+        L.orc_dt.argtypes = [vp, C.c_double]
+        L.orc_dt.restype = C.c_double
+        L.orc_step.argtypes = [vp, C.c_double]
+        L.orc_step.restype = C.c_double
+        L.orc_set_threads.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class Oracle:
+    """One Yin-Yang layout in the oracle: tree, blocks, tables, state."""
+
+    def __init__(self, layout):
+        L = lib()
+        self.L = L
+        self.layout = layout
+        ni, nj, nk, ng = layout["ni"], layout["nj"], layout["nk"], layout["ng"]
+        self.ni, self.nj, self.nk, self.ng = ni, nj, nk, ng
+        self.pad = (ni + 2 * ng, nj + 2 * ng, nk + 2 * ng)
+        self.h = L.orc_create(ni, nj, nk, ng, layout["r_rsun"][0], layout["r_rsun"][1])
+        for op in layout.get("ops", []):
+            if op[0] == "split":
+                L.orc_divide_all(self.h, *op[1])
+            else:
+                _, flags, glob, rng = op
+                f = np.asarray(flags, dtype=np.int32)
+                r = np.asarray(rng, dtype=np.float64)
+                L.orc_amr(self.h, _p(f, C.c_int32), int(glob), _p(r, C.c_double))
+        self.nblocks = L.orc_finalize(self.h)
+        self.nleaf_yin = L.orc_nleaf_yin(self.h)
+
+    def close(self):
+        if self.h:
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def block_info(self, ib):
+        rng = np.zeros(6)
+        fl = np.zeros(3, dtype=np.int32)
+        self.L.orc_block_info(self.h, ib, _p(rng, C.c_double), _p(fl, C.c_int32))
+        return rng, fl
+
+    def centres(self, ib):
+        xi, xj, xk = (np.zeros(n) for n in self.pad)
+        self.L.orc_block_centres(self.h, ib, _p(xi, C.c_double), _p(xj, C.c_double), _p(xk, C.c_double))
+        return xi, xj, xk
+
+    def gc_iblocks(self, ib):
+        out = np.zeros(self.pad[::-1], dtype=np.int32)  # C view of Fortran (i,j,k): [k][j][i]
+        self.L.orc_gc_iblocks(self.h, ib, _p(out, C.c_int32))
+        return out
+
+    def targets(self, ib):
+        res = []
+        for it in range(self.L.orc_ntargets(self.h, ib)):
+            n = self.L.orc_target(self.h, ib, it, None, None)
+            ijk = np.zeros((3, n), dtype=np.int32)
+            x = np.zeros((3, n))
+            self.L.orc_target(self.h, ib, it, _p(ijk, C.c_int32), _p(x, C.c_double))
+            res.append((self.L.orc_target_donor(self.h, ib, it), ijk, x))
+        return res
+
+    def sources(self, ib):
+        res = []
+        for s in range(self.L.orc_nsources(self.h, ib)):
+            rb = np.zeros(1, dtype=np.int32)
+            n = self.L.orc_source(self.h, ib, s, _p(rb, C.c_int32), None, None)
+            ijk = np.zeros((3, n), dtype=np.int32)
+            x = np.zeros((3, n))
+            self.L.orc_source(self.h, ib, s, _p(rb, C.c_int32), _p(ijk, C.c_int32), _p(x, C.c_double))
+            res.append((int(rb[0]), ijk, x))
+        return res
+
+    # state arrays are numpy arrays shaped (8, NK, NJ, NI): the C view of Fortran (i,j,k,v)
+    def set_background(self, ib, bg):
+        bg = np.ascontiguousarray(bg, dtype=np.float64)  # (8, NI)
+        self.L.orc_set_background(self.h, ib, _p(bg, C.c_double))
+
+    def set_state(self, ib, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        self.L.orc_set_state(self.h, ib, _p(u, C.c_double))
+
+    def get_state(self, ib):
+        out = np.zeros((8,) + self.pad[::-1])
+        self.L.orc_get_state(self.h, ib, _p(out, C.c_double))
+        return out
+
+    def exchange(self, which=0):
+        self.L.orc_exchange(self.h, which)
+
+    def rhs(self, ib):
+        out = np.zeros((8, self.nk, self.nj, self.ni))
+        self.L.orc_rhs(self.h, ib, _p(out, C.c_double))
+        return out
+
+    def diffusion(self, ib):
+        out = np.zeros((8, self.nk, self.nj, self.ni))
+        self.L.orc_diffusion(self.h, ib, _p(out, C.c_double))
+        return out
+
+    def dt(self, cfl):
+        return self.L.orc_dt(self.h, cfl)
+
+    def step(self, cfl):
+        return self.L.orc_step(self.h, cfl)
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))

@@ -1,0 +1,381 @@
+! Test-infrastructure driver for the reference Fortran (never part of the product).
+!
+! Builds the Yin-Yang tree, ghost tables and blocks with the reference's own
+! modules (compiled in place from 3_6/ by oracle/Makefile), loads a background
+! and initial state written by tests/golden/make_golden.py, and advances the
+! state with the reference's physics routines.  Two reference files need MPI and
+! are not compiled; their single-rank control flow is restated here:
+!   * harness_exchange  follows ModCommunication_SendRecvGC_new
+!                       (3_6/ModCommunication.f90:655-795, local-rank branch)
+!                       and the local branch of SetGC_sources (:219-300);
+!   * harness_rk4       follows ModAdvance_rk4 (3_6/ModAdvance.f90:18-112) with
+!                       MPI_AllReduce(MIN) over one rank = identity.
+! Background replication follows ModBlock_InitSSM (3_6/ModBlock.f90:251-264);
+! the 1-D profiles come from the input file because the FreeEOS table that
+! ModStratification_new reads is not in the reference tree.
+!
+! Usage: falcon_ref_harness <input.bin> <output.bin>
+program falcon_ref_harness
+    use ModParameters,  only:   ni,nj,nk,ng,nvar,iGeometry,iEquation,MpiSize,MpiRank,&
+                                if_involve_B,Artificial_heating_ratio,CFL,rLevelInitial,&
+                                nAMRs,AMRs,DivB_option
+    use ModBlock,       only:   BlockType,GC_target,ModBlock_InitGrid,ModBlock_InitPrimitives
+    use ModYinYangTree, only:   YYTree,YinYangTree_InitTree,YinYangTree_LabelNodes,&
+                                YinYangTree_GetLocalBlocksInfo,YinYangTree_if_top_bottom
+    use ModAllocation,  only:   ModAllocation_Init,nLocalLeafNodes,iLocalLeafNodes
+    use ModAMR,         only:   ModAMR_set_grid
+    use ModGC,          only:   ModGC_FindGC_Single,ModGC_GetGC_Targets_single
+    use ModMath,        only:   ModMath_1D3D_interpolate_1D1D
+    use ModYinYang,     only:   ModYinYang_CoordConv_1D,ModYinYang_VecConv_1D
+    use ModEquation,    only:   ModEquation_Dynamo
+    use ModDiffusion,   only:   ModDiffusion_Aritificial_1
+    use ModBoundary,    only:   ModBoundary_Dynamo_primitives
+    use ModWaveSpeed,   only:   ModWaveSpeed_Dynamo
+    use ModTimeStep,    only:   ModTimeStep_Dynamo
+    implicit none
+
+    type(YYTree),target         ::  Tree
+    character(len=512)          ::  fin,fout
+    integer                     ::  ui,uo,iamr,iflag(3),iglob,ib,nsteps,ndump,mode,istep,idump
+    integer,allocatable         ::  dump_steps(:)
+    real(8)                     ::  r_range_cgs(2),rng(3,2),dt
+    real(8),allocatable         ::  rtp_ranges(:,:,:),bg(:,:),st(:,:,:,:)
+    integer                     ::  nstate,npad_i,npad_j,npad_k,v,ig,it
+    type(BlockType),pointer     ::  B
+
+    call get_command_argument(1,fin)
+    call get_command_argument(2,fout)
+    open(newunit=ui,file=trim(fin),access='stream',form='unformatted',status='old')
+    open(newunit=uo,file=trim(fout),access='stream',form='unformatted',status='replace')
+
+    read(ui) ni,nj,nk,ng,nstate
+    read(ui) r_range_cgs
+    read(ui) nAMRs
+    allocate(AMRs(max(nAMRs,1)))
+    do iamr=1,nAMRs
+        read(ui) iflag,iglob,rng
+        AMRs(iamr)%iLevel=iamr
+        AMRs(iamr)%rtp_if_divide=(iflag/=0)
+        AMRs(iamr)%if_global=(iglob/=0)
+        AMRs(iamr)%rtp_range_Rsun=rng
+    end do
+    read(ui) CFL
+    read(ui) nsteps
+    read(ui) ndump
+    allocate(dump_steps(max(ndump,1)))
+    if (ndump>0) read(ui) dump_steps(1:ndump)
+    read(ui) mode
+
+    ! Global parameters as the reference's #GRID/#EQUATION/#MPI setup leaves them.
+    MpiSize=1; MpiRank=0
+    iGeometry=1; iEquation=1; if_involve_B=.true.; DivB_option=0
+    Artificial_heating_ratio=1.0d0
+    nvar=9
+    rLevelInitial=0
+
+    ! Tree (3_6/test1.f90:113-119 with YinYangTree_SetAll's steps split out so the
+    ! background can come from the input file instead of ModStratification_new).
+    call YinYangTree_InitTree(Tree,r_range_cgs)
+    call ModAMR_set_grid(Tree)
+    Tree%NumLeafNodes=0
+    Tree%NumLeafNodes_YinYang=0
+    call YinYangTree_LabelNodes(Tree,Tree%Yin)
+    Tree%NumLeafNodes_YinYang(1)=Tree%NumLeafNodes
+    call YinYangTree_LabelNodes(Tree,Tree%Yang)
+    Tree%NumLeafNodes_YinYang(2)=Tree%NumLeafNodes-Tree%NumLeafNodes_YinYang(1)
+    call ModAllocation_Init(Tree%NumLeafNodes)
+    allocate(rtp_ranges(3,2,nLocalLeafNodes))
+    allocate(Tree%LocalBlocks(nLocalLeafNodes))
+    allocate(Tree%iBlocks(nLocalLeafNodes))
+    Tree%iBlocks=iLocalLeafNodes
+    Tree%nLocalBlocks=nLocalLeafNodes
+    ib=1
+    call YinYangTree_GetLocalBlocksInfo(Tree,Tree%Yin ,rtp_ranges,ib)
+    call YinYangTree_GetLocalBlocksInfo(Tree,Tree%Yang,rtp_ranges,ib)
+
+    npad_i=ni+2*ng; npad_j=nj+2*ng; npad_k=nk+2*ng
+    allocate(bg(-ng+1:ni+ng,8),st(-ng+1:ni+ng,-ng+1:nj+ng,-ng+1:nk+ng,nstate))
+
+    do ib=1,Tree%nLocalBlocks
+        B=>Tree%LocalBlocks(ib)
+        ! ModBlock_Init (3_6/ModBlock.f90:115-186), itype_equation=1, without InitSSM
+        B%iBlock=Tree%iBlocks(ib)
+        B%if_yin=(Tree%iBlocks(ib).le.Tree%NumLeafNodes_YinYang(1))
+        B%itype_equation=1
+        B%if_involve_B=.true.
+        B%if_bottom=.false.; B%if_top=.false.
+        call ModBlock_InitGrid(B,rtp_ranges(:,:,ib))
+        B%nvar=9
+        B%rho1_=1; B%vr_=2; B%vt_=3; B%vp_=4; B%br_=5; B%bt_=6; B%bp_=7; B%s1_=8; B%psi_=9
+        call ModBlock_InitPrimitives(B,.true.)
+        B%if_initialized=.true.
+        call YinYangTree_if_top_bottom(B,Tree%r_range_CGS)
+    end do
+
+    ! Tables (ModCommunication_SetGCAll, 3_6/ModCommunication.f90:22-64, single rank)
+    do ib=1,Tree%nLocalBlocks
+        call ModGC_FindGC_Single(Tree,Tree%LocalBlocks(ib))
+    end do
+    do ib=1,Tree%nLocalBlocks
+        call ModGC_GetGC_Targets_single(Tree,Tree%LocalBlocks(ib))
+    end do
+    call harness_set_sources(Tree)
+
+    ! ---- output: layout and tables
+    write(uo) Tree%nLocalBlocks,Tree%NumLeafNodes_YinYang(1)
+    do ib=1,Tree%nLocalBlocks
+        B=>Tree%LocalBlocks(ib)
+        write(uo) B%xijk_range
+        write(uo) merge(1,0,B%if_yin),merge(1,0,B%if_top),merge(1,0,B%if_bottom)
+        write(uo) B%xi_I,B%xj_I,B%xk_I
+    end do
+    do ib=1,Tree%nLocalBlocks
+        write(uo) Tree%LocalBlocks(ib)%GC_iBlocks_III
+    end do
+    do ib=1,Tree%nLocalBlocks
+        B=>Tree%LocalBlocks(ib)
+        write(uo) B%nGC_targets
+        do it=1,B%nGC_targets
+            write(uo) B%GC_targets(it)%iBlock,B%GC_targets(it)%nGC
+            write(uo) B%GC_targets(it)%ijk_list
+            write(uo) B%GC_targets(it)%xijk_list
+        end do
+    end do
+    do ib=1,Tree%nLocalBlocks
+        B=>Tree%LocalBlocks(ib)
+        write(uo) B%nGC_sources
+        do it=1,B%nGC_sources
+            write(uo) B%GC_sources(it)%iBlock,B%GC_sources(it)%nGC
+            write(uo) B%GC_sources(it)%ijk_list
+            write(uo) B%GC_sources(it)%xijk_list
+        end do
+    end do
+
+    if (mode>=2) then
+        ! ---- input: per block 1-D background (8 profiles) and padded state
+        do ib=1,Tree%nLocalBlocks
+            B=>Tree%LocalBlocks(ib)
+            read(ui) bg
+            read(ui) st
+            call harness_background(B,bg)
+            B%primitive_IV=0.0d0
+            do v=1,min(nstate,8)
+                B%primitive_IV(:,:,:,v)=st(:,:,:,v)
+            end do
+            B%primitive_rk_IV=B%primitive_IV
+            B%primitive=>B%primitive_IV
+        end do
+
+        ! initial ghost fill as 3_6/test1.f90:143-144
+        call harness_exchange(Tree,.false.)
+        call harness_exchange(Tree,.false.)
+        call harness_dump(Tree,uo,0,0.0d0)
+
+        ! single-operator dumps on the exchanged initial state
+        do ib=1,Tree%nLocalBlocks
+            B=>Tree%LocalBlocks(ib)
+            B%primitive=>B%primitive_IV
+            call ModBoundary_Dynamo_primitives(B)
+            B%EQN_update_R_IV=0.0d0
+            call ModEquation_Dynamo(B)
+            write(uo) B%EQN_update_R_IV(:,:,:,1:8)
+        end do
+        do ib=1,Tree%nLocalBlocks
+            B=>Tree%LocalBlocks(ib)
+            B%EQN_update_R_IV=0.0d0
+            call ModDiffusion_Aritificial_1(B,B%EQN_update_R_IV,2,.false.)
+            write(uo) B%EQN_update_R_IV(:,:,:,1:8)
+        end do
+        ! restore the state the stepping starts from (BCs above touched ghosts,
+        ! which ModAdvance_rk4 overwrites at stage 1 anyway)
+        do ib=1,Tree%nLocalBlocks
+            Tree%LocalBlocks(ib)%EQN_update_R_IV=0.0d0
+        end do
+
+        idump=1
+        do istep=1,nsteps
+            call harness_rk4(Tree,dt)
+            if (idump<=ndump) then
+                if (dump_steps(idump)==istep) then
+                    call harness_dump(Tree,uo,istep,dt)
+                    idump=idump+1
+                end if
+            end if
+        end do
+    end if
+    close(ui); close(uo)
+
+contains
+
+    ! Background replication exactly as ModBlock_InitSSM (3_6/ModBlock.f90:252-264).
+    ! bg columns: g, rho0, p0, T0, gamma1, gamma3, heat, xi_rsst.
+    subroutine harness_background(B,bg)
+        type(BlockType),target      ::  B
+        real(8),intent(in)          ::  bg(-ng+1:ni+ng,8)
+        integer                     ::  j,k
+        allocate(B%gamma1_III        (-ng+1:ni+ng,-ng+1:nj+ng,-ng+1:nk+ng))
+        allocate(B%gamma3_minus_1_III(-ng+1:ni+ng,-ng+1:nj+ng,-ng+1:nk+ng))
+        allocate(B%rho0_III          (-ng+1:ni+ng,-ng+1:nj+ng,-ng+1:nk+ng))
+        allocate(B%g_over_rho0_III   (-ng+1:ni+ng,-ng+1:nj+ng,-ng+1:nk+ng))
+        allocate(B%p0_over_rho0_III  (-ng+1:ni+ng,-ng+1:nj+ng,-ng+1:nk+ng))
+        allocate(B%rho0T0_III        (-ng+1:ni+ng,-ng+1:nj+ng,-ng+1:nk+ng))
+        allocate(B%total_heat_III    (-ng+1:ni+ng,-ng+1:nj+ng,-ng+1:nk+ng))
+        allocate(B%p1_III            (-ng+1:ni+ng,-ng+1:nj+ng,-ng+1:nk+ng))
+        allocate(B%Xi_rsst_III       (-ng+1:ni+ng,-ng+1:nj+ng,-ng+1:nk+ng))
+        allocate(B%v_wave_III        (-ng+1:ni+ng,-ng+1:nj+ng,-ng+1:nk+ng))
+        B%if_SSM=.true.
+        do j=-ng+1,nj+ng
+            do k=-ng+1,nk+ng
+                B%gamma1_III(:,j,k)        =bg(:,5)
+                B%gamma3_minus_1_III(:,j,k)=bg(:,6)-1.0
+                B%rho0_III(:,j,k)          =bg(:,2)
+                B%g_over_rho0_III(:,j,k)   =bg(:,1)/bg(:,2)
+                B%p0_over_rho0_III(:,j,k)  =bg(:,3)/bg(:,2)
+                B%rho0T0_III(:,j,k)        =bg(:,2)*bg(:,4)
+                B%total_heat_III(:,j,k)    =bg(:,7)+0.0d0
+                B%Xi_rsst_III(:,j,k)       =bg(:,8)
+            end do
+        end do
+    end subroutine harness_background
+
+    ! Local branch of ModCommunication_GetnGC_sources / SetGC_sources
+    ! (3_6/ModCommunication.f90:66-247): every receiving block's target list is
+    ! appended to the donor's GC_sources, with coordinates moved into the donor's
+    ! panel frame by ModYinYang_CoordConv_1D when the panels differ.
+    subroutine harness_set_sources(Tree)
+        type(YYTree),target         ::  Tree
+        type(BlockType),pointer     ::  Bt,Bs
+        type(GC_target),pointer     ::  tg,sc
+        integer,allocatable         ::  cnt(:)
+        integer                     ::  ib,it
+
+        allocate(cnt(Tree%NumLeafNodes)); cnt=0
+        do ib=1,Tree%nLocalBlocks
+            Bt=>Tree%LocalBlocks(ib)
+            do it=1,Bt%nGC_targets
+                cnt(Bt%GC_targets(it)%iBlock)=cnt(Bt%GC_targets(it)%iBlock)+1
+            end do
+        end do
+        do ib=1,Tree%nLocalBlocks
+            allocate(Tree%LocalBlocks(ib)%GC_sources(cnt(Tree%LocalBlocks(ib)%iBlock)))
+            Tree%LocalBlocks(ib)%nGC_sources=0
+        end do
+        do ib=1,Tree%nLocalBlocks
+            Bt=>Tree%LocalBlocks(ib)
+            do it=1,Bt%nGC_targets
+                tg=>Bt%GC_targets(it)
+                Bs=>Tree%LocalBlocks(tg%iBlock)
+                Bs%nGC_sources=Bs%nGC_sources+1
+                sc=>Bs%GC_sources(Bs%nGC_sources)
+                sc%iRank=0
+                sc%iBlock=Bt%iBlock
+                sc%if_yin=Bt%iBlock.le.Tree%NumLeafNodes_YinYang(1)
+                sc%nGC=tg%nGC
+                allocate(sc%xijk_list(sc%nGC,3),sc%ijk_list(sc%nGC,3),sc%primitive_list(sc%nGC,nvar))
+                sc%ijk_list=tg%ijk_list
+                if (sc%if_yin .eqv. Bs%if_yin) then
+                    sc%xijk_list=tg%xijk_list
+                else
+                    sc%xijk_list=ModYinYang_CoordConv_1D(tg%xijk_list,tg%nGC)
+                end if
+            end do
+        end do
+    end subroutine harness_set_sources
+
+    ! ModCommunication_SendRecvGC_new (3_6/ModCommunication.f90:655-795) on one rank.
+    subroutine harness_exchange(Tree,if_rk)
+        type(YYTree),target         ::  Tree
+        logical,intent(in)          ::  if_rk
+        type(BlockType),pointer     ::  Bs,Bt
+        type(GC_target),pointer     ::  sc
+        real(8),pointer             ::  ps(:,:,:,:),pt(:,:,:,:)
+        integer                     ::  ib,is,ig
+
+        do ib=1,Tree%nLocalBlocks
+            Bs=>Tree%LocalBlocks(ib)
+            if (if_rk) then
+                ps=>Bs%primitive_rk_IV
+            else
+                ps=>Bs%primitive_IV
+            end if
+            do is=1,Bs%nGC_sources
+                sc=>Bs%GC_sources(is)
+                call ModMath_1D3D_interpolate_1D1D(ps,nvar,ni,nj,nk,ng,&
+                    Bs%xi_I,Bs%xj_I,Bs%xk_I,sc%nGC,sc%xijk_list,sc%primitive_list)
+                if (sc%if_yin .neqv. Bs%if_yin) then
+                    sc%primitive_list(:,Bs%br_:Bs%bp_)=ModYinYang_VecConv_1D(sc%xijk_list,&
+                        sc%primitive_list(:,Bs%br_:Bs%bp_),sc%nGC)
+                    sc%primitive_list(:,Bs%vr_:Bs%vp_)=ModYinYang_VecConv_1D(sc%xijk_list,&
+                        sc%primitive_list(:,Bs%vr_:Bs%vp_),sc%nGC)
+                end if
+                Bt=>Tree%LocalBlocks(sc%iBlock)
+                if (if_rk) then
+                    pt=>Bt%primitive_rk_IV
+                else
+                    pt=>Bt%primitive_IV
+                end if
+                do ig=1,sc%nGC
+                    pt(sc%ijk_list(ig,1),sc%ijk_list(ig,2),sc%ijk_list(ig,3),:)=sc%primitive_list(ig,:)
+                end do
+            end do
+        end do
+    end subroutine harness_exchange
+
+    ! ModAdvance_rk4 (3_6/ModAdvance.f90:18-112), iEquation=1, one rank.
+    subroutine harness_rk4(Tree,dt_global)
+        type(YYTree),target         ::  Tree
+        real(8),intent(out)         ::  dt_global
+        type(BlockType),pointer     ::  B
+        integer                     ::  ib,rk_index
+        logical                     ::  if_rk_input,if_rk_output
+        real(8)                     ::  dt_local
+
+        do ib=1,Tree%nLocalBlocks
+            call ModWaveSpeed_Dynamo(Tree%LocalBlocks(ib))
+        end do
+        call ModTimeStep_Dynamo(Tree,CFL,dt_local)
+        dt_global=dt_local
+        do rk_index=1,4
+            do ib=1,Tree%nLocalBlocks
+                B=>Tree%LocalBlocks(ib)
+                if_rk_input=(rk_index>1)
+                if_rk_output=(rk_index<4)
+                if (if_rk_input) then
+                    B%primitive=>B%primitive_rk_IV
+                else
+                    B%primitive=>B%primitive_IV
+                end if
+                call ModBoundary_Dynamo_primitives(B)
+                call ModEquation_Dynamo(B)
+                if (if_rk_output) then
+                    B%primitive_rk_IV(1:ni,1:nj,1:nk,:)=&
+                        B%primitive_IV(1:ni,1:nj,1:nk,:)+dt_global*B%EQN_update_R_IV/(5.0-rk_index)
+                else
+                    B%primitive_IV(1:ni,1:nj,1:nk,:)=&
+                        B%primitive_IV(1:ni,1:nj,1:nk,:)+dt_global*B%EQN_update_R_IV/(5.0-rk_index)
+                end if
+            end do
+            call harness_exchange(Tree,if_rk_output)
+        end do
+        do ib=1,Tree%nLocalBlocks
+            B=>Tree%LocalBlocks(ib)
+            B%EQN_update_R_IV=0.0
+            call ModDiffusion_Aritificial_1(B,B%EQN_update_R_IV,2,.false.)
+            B%primitive_IV(1:ni,1:nj,1:nk,:)=&
+                B%primitive_IV(1:ni,1:nj,1:nk,:)+0.5*dt_global*B%EQN_update_R_IV
+        end do
+        call harness_exchange(Tree,.false.)
+    end subroutine harness_rk4
+
+    subroutine harness_dump(Tree,uo,istep,dt)
+        type(YYTree),target         ::  Tree
+        integer,intent(in)          ::  uo,istep
+        real(8),intent(in)          ::  dt
+        integer                     ::  ib
+        write(uo) istep,dt
+        do ib=1,Tree%nLocalBlocks
+            write(uo) Tree%LocalBlocks(ib)%primitive_IV(:,:,:,1:8)
+        end do
+    end subroutine harness_dump
+
+end program falcon_ref_harness
