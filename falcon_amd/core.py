@@ -1,0 +1,239 @@
+"""ctypes binding of libfalcon_hip.so and the host-side driver objects.
+
+The library is required: importing a compute entry point without the built
+library raises, there is no CPU fallback.  Arrays handed to and from the library
+use the reference's Fortran layout; in numpy they are C-contiguous arrays of
+shape (8, NK, NJ, NI) for a padded block state (the C view of Fortran
+(NI, NJ, NK, 8)) and (8, nk, nj, ni) for an interior field.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libfalcon_hip.so")
+R_SUN = float(np.float32(6.955e10))   # 3_6/ModConst.f90:5 (default real)
+
+_lib = None
+
+
+class FalconError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libfalcon_hip.so (built by `python -m falcon_amd.build`)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FalconError(f"{LIB_PATH} is missing: build it with `python -m falcon_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, ip, dp, i64p = C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_int64)
+    sig = {
+        "falcon_create": [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double],
+        "falcon_destroy": [vp],
+        "falcon_tree_divide_all": [vp, C.c_int, C.c_int, C.c_int],
+        "falcon_tree_amr": [vp, ip, C.c_int, dp],
+        "falcon_tree_set_nodes": [vp, C.c_int, ip, ip, dp],
+        "falcon_nblocks": [vp, ip, ip],
+        "falcon_block_info": [vp, C.c_int, dp, ip, dp, dp, dp],
+        "falcon_set_gc_all": [vp],
+        "falcon_build_tables": [vp],
+        "falcon_gc_iblocks": [vp, C.c_int, ip],
+        "falcon_gc_ntargets": [vp, C.c_int, ip],
+        "falcon_gc_target": [vp, C.c_int, C.c_int, ip, ip, ip],
+        "falcon_gc_entries": [vp, i64p, ip, ip, ip, ip, dp, ip, dp],
+        "falcon_set_background": [vp, C.c_int, dp],
+        "falcon_background_polytrope": [vp, C.c_double, C.c_double],
+        "falcon_set_state": [vp, C.c_int, dp],
+        "falcon_get_state": [vp, C.c_int, dp],
+        "falcon_init_synthetic": [vp, C.c_uint64, C.c_double, C.c_double],
+        "falcon_set_precision": [vp, C.c_int],
+        "falcon_sendrecv_gc": [vp, C.c_int],
+        "falcon_advance_rk4": [vp, C.c_double, dp],
+        "falcon_advance": [vp, C.c_int, C.c_double, C.c_int, dp],
+        "falcon_compute_dt": [vp, C.c_double, dp],
+        "falcon_rhs": [vp, dp],
+        "falcon_diffusion_rhs": [vp, dp],
+        "falcon_profile_step": [vp, C.c_int, C.c_double, dp],
+        "falcon_device_bytes": [vp, i64p],
+        "falcon_synchronize": [vp],
+        "falcon_last_error": [C.c_char_p, C.c_int],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        buf = C.create_string_buffer(1024)
+        _lib.falcon_last_error(buf, 1024)
+        raise FalconError(buf.value.decode())
+
+
+def _p(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class Shell:
+    """One Yin-Yang shell of blocks resident in HBM (one library handle).
+
+    `layout` = {"ni","nj","nk","ng","r_rsun":(lo,hi),"ops":[...]} where each op is
+    ("split", (nr,nt,np)) or ("amr", flags3, if_global, rng6_rsun).
+    """
+
+    def __init__(self, layout):
+        L = load()
+        self.L = L
+        self.layout = layout
+        ni, nj, nk, ng = layout["ni"], layout["nj"], layout["nk"], layout.get("ng", 2)
+        self.ni, self.nj, self.nk, self.ng = ni, nj, nk, ng
+        self.pad = (ni + 2 * ng, nj + 2 * ng, nk + 2 * ng)
+        h = C.c_void_p()
+        r0, r1 = layout["r_rsun"]
+        _check(L.falcon_create(C.byref(h), ni, nj, nk, ng, r0 * R_SUN, r1 * R_SUN))
+        self.h = h
+        for op in layout.get("ops", []):
+            if op[0] == "split":
+                _check(L.falcon_tree_divide_all(h, *[int(x) for x in op[1]]))
+            elif op[0] == "amr":
+                f = np.asarray(op[1], dtype=np.int32)
+                r = np.asarray(op[3], dtype=np.float64)
+                _check(L.falcon_tree_amr(h, _p(f, C.c_int32), int(op[2]), _p(r)))
+            else:
+                raise ValueError(op)
+        nb, ny = C.c_int32(), C.c_int32()
+        _check(L.falcon_nblocks(h, C.byref(nb), C.byref(ny)))
+        self.nblocks, self.nleaf_yin = nb.value, ny.value
+        self.on_device = False
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.falcon_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- layout and tables
+    def block_info(self, ib):
+        rng = np.zeros(6)
+        fl = np.zeros(3, dtype=np.int32)
+        xi, xj, xk = (np.zeros(n) for n in self.pad)
+        _check(self.L.falcon_block_info(self.h, ib, _p(rng), _p(fl, C.c_int32), _p(xi), _p(xj), _p(xk)))
+        return rng, fl, (xi, xj, xk)
+
+    def build_tables(self):
+        _check(self.L.falcon_build_tables(self.h))
+
+    def set_gc_all(self):
+        _check(self.L.falcon_set_gc_all(self.h))
+        self.on_device = True
+
+    def gc_iblocks(self, ib):
+        out = np.zeros(self.pad[::-1], dtype=np.int32)
+        _check(self.L.falcon_gc_iblocks(self.h, ib, _p(out, C.c_int32)))
+        return out
+
+    def gc_targets(self, ib):
+        n = C.c_int32()
+        _check(self.L.falcon_gc_ntargets(self.h, ib, C.byref(n)))
+        res = []
+        for it in range(n.value):
+            d, g = C.c_int32(), C.c_int32()
+            _check(self.L.falcon_gc_target(self.h, ib, it, C.byref(d), C.byref(g), None))
+            ijk = np.zeros((3, g.value), dtype=np.int32)
+            _check(self.L.falcon_gc_target(self.h, ib, it, C.byref(d), C.byref(g), _p(ijk, C.c_int32)))
+            res.append((d.value, ijk))
+        return res
+
+    def gc_entries(self):
+        n = C.c_int64()
+        _check(self.L.falcon_gc_entries(self.h, C.byref(n), None, None, None, None, None, None, None))
+        n = n.value
+        recv = np.zeros(n, np.int32); donor = np.zeros(n, np.int32)
+        gijk = np.zeros((n, 3), np.int32); bijk = np.zeros((n, 3), np.int32)
+        w = np.zeros((n, 3)); yy = np.zeros(n, np.int32); rot = np.zeros((n, 2))
+        nn = C.c_int64()
+        _check(self.L.falcon_gc_entries(self.h, C.byref(nn), _p(recv, C.c_int32), _p(donor, C.c_int32),
+                                        _p(gijk, C.c_int32), _p(bijk, C.c_int32), _p(w), _p(yy, C.c_int32), _p(rot)))
+        return dict(recv=recv, donor=donor, gijk=gijk, bijk=bijk, w=w, yy=yy, rot=rot)
+
+    # ---- data
+    def set_background(self, ib, bg):
+        bg = np.ascontiguousarray(bg, dtype=np.float64)
+        _check(self.L.falcon_set_background(self.h, ib, _p(bg)))
+
+    def background_polytrope(self, contrast=20.0, rho_top=1.0e-2):
+        _check(self.L.falcon_background_polytrope(self.h, contrast, rho_top))
+
+    def set_state(self, ib, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        _check(self.L.falcon_set_state(self.h, ib, _p(u)))
+
+    def get_state(self, ib):
+        out = np.zeros((8,) + self.pad[::-1])
+        _check(self.L.falcon_get_state(self.h, ib, _p(out)))
+        return out
+
+    def init_synthetic(self, seed=12345, s_amp=1e-4, b_ratio=1e-3):
+        _check(self.L.falcon_init_synthetic(self.h, seed, s_amp, b_ratio))
+
+    # ---- stepping
+    def set_precision(self, mode):
+        _check(self.L.falcon_set_precision(self.h, {"fp64": 0, "fp32": 1}.get(mode, mode)))
+
+    def sendrecv_gc(self, which=0):
+        _check(self.L.falcon_sendrecv_gc(self.h, which))
+
+    def advance_rk4(self, cfl=0.5):
+        dt = C.c_double()
+        _check(self.L.falcon_advance_rk4(self.h, cfl, C.byref(dt)))
+        return dt.value
+
+    def advance(self, nsteps, cfl=0.5, graph=True):
+        dt = C.c_double()
+        _check(self.L.falcon_advance(self.h, nsteps, cfl, 1 if graph else 0, C.byref(dt)))
+        return dt.value
+
+    def compute_dt(self, cfl=0.5):
+        dt = C.c_double()
+        _check(self.L.falcon_compute_dt(self.h, cfl, C.byref(dt)))
+        return dt.value
+
+    def rhs(self):
+        out = np.zeros((self.nblocks, 8, self.nk, self.nj, self.ni))
+        _check(self.L.falcon_rhs(self.h, _p(out)))
+        return out
+
+    def diffusion_rhs(self):
+        out = np.zeros((self.nblocks, 8, self.nk, self.nj, self.ni))
+        _check(self.L.falcon_diffusion_rhs(self.h, _p(out)))
+        return out
+
+    def profile_step(self, nsteps, cfl=0.5):
+        out = np.zeros(7)
+        _check(self.L.falcon_profile_step(self.h, nsteps, cfl, _p(out)))
+        keys = ["dt", "boundary", "stage", "exchange_stage", "exchange_state", "diffusion", "step"]
+        return dict(zip(keys, out.tolist()))
+
+    def device_bytes(self):
+        b = C.c_int64()
+        _check(self.L.falcon_device_bytes(self.h, C.byref(b)))
+        return b.value
+
+    def synchronize(self):
+        _check(self.L.falcon_synchronize(self.h))
+
+    @property
+    def interior_cells(self):
+        return self.nblocks * self.ni * self.nj * self.nk

@@ -1,0 +1,65 @@
+// Shared declarations between the kernel file and the C-ABI file.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace falcon {
+
+// Per-block constants, FP64 (3_6/ModBlock.f90:315-317 and the derivative
+// coefficient (1./12./dx) of 3_6/ModDerivative.f90:16,22,28).
+struct BlockMeta {
+    double cr, ct, cp;      // (double)(1.0f/12.0f)/dx
+    double dxi, dxj, dxk;
+};
+
+// Geometry and background tables on the device, one row per block.
+// Radial rows have NI entries, angular rows NJ entries (padded indices).
+struct DevTables {
+    const BlockMeta *meta;
+    // radial (NI per block)
+    const double *xi, *rinv, *r2inv, *rho0, *gp, *gq, *g_over_rho0, *xir2, *heat, *cs, *sqrt_rho0;
+    // theta (NJ per block)
+    const double *sint, *sinv, *cot;
+};
+
+struct Dims {
+    int ni, nj, nk, ng;
+    int NI, NJ, NK;
+    int nblocks;
+    int tiles_j, tiles_k;   // 16x16 (theta, phi) tiles per block
+    long long P;            // cells per variable per block (NI*NJ*NK)
+};
+
+// Ghost-fill tables (device): plain entries first, Yin-Yang entries after.
+struct DevGhosts {
+    const int32_t *dst;     // block*8*P + cell of the ghost (variable 0)
+    const int32_t *src;     // donor*8*P + cell of the interpolation base (variable 0)
+    const double *w;        // 3 weights per entry
+    const double *rot;      // 2 rotation coefficients per Yin-Yang entry
+    int n_plain, n_yy;
+};
+
+struct DevBoundary {
+    const int32_t *list;    // block*2 + side (0 bottom, 1 top)
+    int n;
+};
+
+// launchers (kernels.hip)
+template <typename T, typename TIN, typename TOUT>
+void launch_stage(const Dims &d, const DevTables &t, const TIN *in, const double *U, TOUT *out,
+                  const double *dtp, double div, hipStream_t s);
+template <typename T>
+void launch_diffusion(const Dims &d, const DevTables &t, const double *U, double *out, const double *dtp, hipStream_t s);
+template <typename TS>
+void launch_exchange(const Dims &d, const DevGhosts &g, TS *state, hipStream_t s);
+template <typename TS>
+void launch_boundary(const Dims &d, const DevBoundary &b, TS *state, hipStream_t s);
+void launch_dt(const Dims &d, const DevTables &t, const double *U, double *partial, double *dtp, double cfl, hipStream_t s);
+// single-operator entry points used by parity tests: R only
+template <typename T, typename TIN>
+void launch_rhs_only(const Dims &d, const DevTables &t, const TIN *in, double *R, hipStream_t s);
+template <typename T>
+void launch_diffusion_only(const Dims &d, const DevTables &t, const double *U, double *R, hipStream_t s);
+void launch_copy(const double *src, double *dst, long long n, hipStream_t s);
+
+}  // namespace falcon

@@ -1,0 +1,818 @@
+// Handle and C-ABI of libfalcon_hip.so (declarations in include/falcon_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/falcon_hip.h"
+#include "device.hpp"
+#include "layout.hpp"
+
+namespace falcon {
+void synthetic_state(const Layout &lay, const std::vector<double> &bg, uint64_t seed, double s_amp, double b_ratio,
+                     int nthreads, std::vector<double> &state);
+}
+
+using namespace falcon;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIPCHECK(x)                                                                                   \
+    do {                                                                                              \
+        hipError_t e_ = (x);                                                                          \
+        if (e_ != hipSuccess) throw Error(std::string(#x) + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+template <typename F>
+int guard(F &&f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+int host_threads() {
+    unsigned n = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(n, 32u));
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    void alloc(size_t n) {
+        free();
+        if (n) HIPCHECK(hipMalloc(&p, n));
+        bytes = n;
+    }
+    void free() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T>
+    T *as() const { return (T *)p; }
+};
+
+}  // namespace
+
+struct falcon_handle {
+    Layout lay;
+    bool labelled = false, built = false, tables = false;
+    std::vector<double> bg;   // per block 8 x NI
+    std::vector<char> bg_set;
+    int precision = 0;
+    hipStream_t stream = nullptr;
+    Dims dims{};
+    DevTables dt{};
+    DevGhosts gh{};
+    DevBoundary bnd{};
+    // device memory
+    DevBuf meta, radial, angular, ghost_dst, ghost_src, ghost_w, ghost_rot, bnd_list;
+    DevBuf U[2], S64[2], S32[2], dtp, partial, rbuf;
+    int cur = 0;             // index of the current state buffer in U[]
+    hipGraphExec_t graph = nullptr;
+    int graph_cur = -1, graph_prec = -1;
+    double graph_cfl = -1;
+
+    ~falcon_handle() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        DevBuf *all[] = {&meta, &radial, &angular, &ghost_dst, &ghost_src, &ghost_w, &ghost_rot, &bnd_list,
+                         &U[0], &U[1], &S64[0], &S64[1], &S32[0], &S32[1], &dtp, &partial, &rbuf};
+        for (DevBuf *b : all) b->free();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    int NI() const { return lay.tree.ni + 2 * lay.tree.ng; }
+    int NJ() const { return lay.tree.nj + 2 * lay.tree.ng; }
+    int NK() const { return lay.tree.nk + 2 * lay.tree.ng; }
+
+    void need_label() {
+        if (!labelled) {
+            lay.tree.label();
+            labelled = true;
+            bg.assign((size_t)lay.tree.nleaf * 8 * NI(), 0.0);
+            bg_set.assign(lay.tree.nleaf, 0);
+        }
+    }
+    void need_tables() const {
+        if (!tables) throw Error("falcon_set_gc_all has not been called");
+    }
+    void need_built() const {
+        if (!built) throw Error("falcon_build_tables has not been called");
+    }
+    void check_block(int ib) const {
+        if (ib < 0 || ib >= lay.tree.nleaf) throw Error("block index out of range");
+    }
+};
+
+namespace {
+
+// host Fortran (i fastest) <-> device [v][I][J][K] (K fastest) for one block
+void fortran_to_device(const double *src, double *dst, int NI, int NJ, int NK) {
+    const size_t P = (size_t)NI * NJ * NK;
+    for (int v = 0; v < 8; v++)
+        for (int K = 0; K < NK; K++)
+            for (int J = 0; J < NJ; J++)
+                for (int I = 0; I < NI; I++)
+                    dst[v * P + ((size_t)I * NJ + J) * NK + K] = src[v * P + (size_t)I + (size_t)NI * (J + (size_t)NJ * K)];
+}
+void device_to_fortran(const double *src, double *dst, int NI, int NJ, int NK) {
+    const size_t P = (size_t)NI * NJ * NK;
+    for (int v = 0; v < 8; v++)
+        for (int K = 0; K < NK; K++)
+            for (int J = 0; J < NJ; J++)
+                for (int I = 0; I < NI; I++)
+                    dst[v * P + (size_t)I + (size_t)NI * (J + (size_t)NJ * K)] = src[v * P + ((size_t)I * NJ + J) * NK + K];
+}
+
+void upload_tables(falcon_handle *h) {
+    const Tree &t = h->lay.tree;
+    const int nb = t.nleaf, NI = h->NI(), NJ = h->NJ(), NK = h->NK(), ng = t.ng;
+    for (int b = 0; b < nb; b++)
+        if (!h->bg_set[b]) throw Error("background not set for block " + std::to_string(b));
+    // per-block constants
+    std::vector<BlockMeta> meta(nb);
+    const int NR = 11, NA = 3;
+    std::vector<double> rad((size_t)nb * NR * NI), ang((size_t)nb * NA * NJ);
+    for (int b = 0; b < nb; b++) {
+        const BlockGeom &g = h->lay.geom[b];
+        const double c12 = (double)(1.0f / 12.0f);
+        meta[b] = {c12 / g.dxi, c12 / g.dxj, c12 / g.dxk, g.dxi, g.dxj, g.dxk};
+        const double *B = h->bg.data() + (size_t)b * 8 * NI;
+        for (int I = 0; I < NI; I++) {
+            const double gg = B[I], rho0 = B[I + NI], p0 = B[I + 2 * NI], T0 = B[I + 3 * NI];
+            const double g1 = B[I + 4 * NI], g3 = B[I + 5 * NI], heat = B[I + 6 * NI], xr = B[I + 7 * NI];
+            // 3_6/ModBlock.f90:255-262 products, then the per-term products of ModEquation
+            const double p0r = p0 / rho0, rho0T0 = rho0 * T0, gm1 = g3 - 1.0, x = g.xi[I];
+            double *row = rad.data() + (size_t)b * NR * NI;
+            row[0 * NI + I] = x;
+            row[1 * NI + I] = 1.0 / x;
+            row[2 * NI + I] = 1.0 / (x * x);
+            row[3 * NI + I] = rho0;
+            row[4 * NI + I] = g1 * p0r;
+            row[5 * NI + I] = gm1 * rho0T0;
+            row[6 * NI + I] = gg / rho0;
+            row[7 * NI + I] = -1.0 / (xr * xr);
+            row[8 * NI + I] = (1.0 * (heat + 0.0)) / rho0T0;
+            row[9 * NI + I] = (1.0 / xr) * std::sqrt(g1 * p0r);
+            row[10 * NI + I] = std::sqrt(rho0);
+        }
+        double *arow = ang.data() + (size_t)b * NA * NJ;
+        for (int J = 0; J < NJ; J++) {
+            const double s = std::sin(g.xj[J]);
+            arow[J] = s;
+            arow[NJ + J] = 1.0 / s;
+            arow[2 * NJ + J] = std::cos(g.xj[J]) / s;
+        }
+    }
+    // reorganise radial/angular rows as [array][block][index] for simple pointers
+    std::vector<double> radT(rad.size()), angT(ang.size());
+    for (int b = 0; b < nb; b++) {
+        for (int a = 0; a < NR; a++)
+            std::memcpy(&radT[((size_t)a * nb + b) * NI], &rad[((size_t)b * NR + a) * NI], sizeof(double) * NI);
+        for (int a = 0; a < NA; a++)
+            std::memcpy(&angT[((size_t)a * nb + b) * NJ], &ang[((size_t)b * NA + a) * NJ], sizeof(double) * NJ);
+    }
+    h->meta.alloc(sizeof(BlockMeta) * nb);
+    h->radial.alloc(sizeof(double) * radT.size());
+    h->angular.alloc(sizeof(double) * angT.size());
+    HIPCHECK(hipMemcpy(h->meta.p, meta.data(), sizeof(BlockMeta) * nb, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(h->radial.p, radT.data(), sizeof(double) * radT.size(), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(h->angular.p, angT.data(), sizeof(double) * angT.size(), hipMemcpyHostToDevice));
+    const double *R = h->radial.as<double>(), *A = h->angular.as<double>();
+    const size_t rs = (size_t)nb * NI, as = (size_t)nb * NJ;
+    DevTables &dt = h->dt;
+    dt.meta = h->meta.as<BlockMeta>();
+    dt.xi = R; dt.rinv = R + rs; dt.r2inv = R + 2 * rs; dt.rho0 = R + 3 * rs; dt.gp = R + 4 * rs; dt.gq = R + 5 * rs;
+    dt.g_over_rho0 = R + 6 * rs; dt.xir2 = R + 7 * rs; dt.heat = R + 8 * rs; dt.cs = R + 9 * rs; dt.sqrt_rho0 = R + 10 * rs;
+    dt.sint = A; dt.sinv = A + as; dt.cot = A + 2 * as;
+
+    // ghost entries: plain first, Yin-Yang after
+    const auto &G = h->lay.ghosts;
+    const long long P = (long long)NI * NJ * NK;
+    auto cell = [&](int i, int j, int k) -> long long {
+        return ((long long)(i + ng - 1) * NJ + (j + ng - 1)) * NK + (k + ng - 1);
+    };
+    if ((long long)nb * 8 * P >= (1ll << 31)) throw Error("state too large for 32-bit ghost offsets");
+    std::vector<int32_t> dst, src;
+    std::vector<double> w, rot;
+    dst.reserve(G.size()); src.reserve(G.size()); w.reserve(3 * G.size());
+    int n_plain = 0, n_yy = 0;
+    for (int pass = 0; pass < 2; pass++)
+        for (const GhostEntry &e : G) {
+            if (e.yy != pass) continue;
+            dst.push_back((int32_t)((long long)e.recv_block * 8 * P + cell(e.gi, e.gj, e.gk)));
+            src.push_back((int32_t)((long long)e.donor * 8 * P + cell(e.i0, e.j0, e.k0)));
+            w.push_back(e.w[0]); w.push_back(e.w[1]); w.push_back(e.w[2]);
+            if (pass) { rot.push_back(e.rot[0]); rot.push_back(e.rot[1]); n_yy++; }
+            else n_plain++;
+        }
+    h->ghost_dst.alloc(sizeof(int32_t) * dst.size());
+    h->ghost_src.alloc(sizeof(int32_t) * src.size());
+    h->ghost_w.alloc(sizeof(double) * w.size());
+    h->ghost_rot.alloc(sizeof(double) * std::max<size_t>(rot.size(), 2));
+    if (!dst.empty()) {
+        HIPCHECK(hipMemcpy(h->ghost_dst.p, dst.data(), sizeof(int32_t) * dst.size(), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(h->ghost_src.p, src.data(), sizeof(int32_t) * src.size(), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(h->ghost_w.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+    }
+    if (!rot.empty()) HIPCHECK(hipMemcpy(h->ghost_rot.p, rot.data(), sizeof(double) * rot.size(), hipMemcpyHostToDevice));
+    h->gh = {h->ghost_dst.as<int32_t>(), h->ghost_src.as<int32_t>(), h->ghost_w.as<double>(), h->ghost_rot.as<double>(),
+             n_plain, n_yy};
+
+    std::vector<int32_t> bl;
+    for (int b = 0; b < nb; b++) {
+        if (h->lay.geom[b].bottom) bl.push_back(b * 2 + 0);
+        if (h->lay.geom[b].top) bl.push_back(b * 2 + 1);
+    }
+    h->bnd_list.alloc(sizeof(int32_t) * std::max<size_t>(bl.size(), 1));
+    if (!bl.empty()) HIPCHECK(hipMemcpy(h->bnd_list.p, bl.data(), sizeof(int32_t) * bl.size(), hipMemcpyHostToDevice));
+    h->bnd = {h->bnd_list.as<int32_t>(), (int)bl.size()};
+}
+
+void alloc_state(falcon_handle *h) {
+    const size_t n = (size_t)h->dims.nblocks * 8 * h->dims.P;
+    for (int q = 0; q < 2; q++) {
+        if (!h->U[q].p) {
+            h->U[q].alloc(n * sizeof(double));
+            HIPCHECK(hipMemset(h->U[q].p, 0, n * sizeof(double)));
+        }
+    }
+    if (!h->dtp.p) { h->dtp.alloc(sizeof(double)); HIPCHECK(hipMemset(h->dtp.p, 0, sizeof(double))); }
+    if (!h->partial.p) h->partial.alloc(sizeof(double) * std::max(1, h->dims.nblocks));
+}
+
+void alloc_stage(falcon_handle *h) {
+    const size_t n = (size_t)h->dims.nblocks * 8 * h->dims.P;
+    for (int q = 0; q < 2; q++) {
+        if (h->precision == 0 && !h->S64[q].p) {
+            h->S64[q].alloc(n * sizeof(double));
+            HIPCHECK(hipMemset(h->S64[q].p, 0, n * sizeof(double)));
+        }
+        if (h->precision == 1 && !h->S32[q].p) {
+            h->S32[q].alloc(n * sizeof(float));
+            HIPCHECK(hipMemset(h->S32[q].p, 0, n * sizeof(float)));
+        }
+    }
+}
+
+// One RK4 step: ModAdvance_rk4 (3_6/ModAdvance.f90:18-112) with stage
+// buffers ping-ponged and the diffusion written to the second state buffer.
+// The diffusion reads the state's radial ghosts as stage 1's boundary call left
+// them (the reference applies boundaries only to each stage's input).
+void enqueue_step(falcon_handle *h, double cfl, int cur, hipEvent_t *ev = nullptr) {
+    const Dims &d = h->dims;
+    const DevTables &t = h->dt;
+    hipStream_t s = h->stream;
+    double *U = h->U[cur].as<double>(), *Un = h->U[1 - cur].as<double>();
+    double *dtp = h->dtp.as<double>();
+    auto mark = [&](int i) { if (ev) HIPCHECK(hipEventRecord(ev[i], s)); };
+    mark(0);
+    launch_dt(d, t, U, h->partial.as<double>(), dtp, cfl, s);
+    mark(1);
+    if (h->precision == 0) {
+        double *S0 = h->S64[0].as<double>(), *S1 = h->S64[1].as<double>();
+        const double *in[4] = {U, S0, S1, S0};
+        double *out[4] = {S0, S1, S0, U};
+        for (int k = 1; k <= 4; k++) {
+            launch_boundary<double>(d, h->bnd, const_cast<double *>(in[k - 1]), s);
+            mark(1 + 4 * (k - 1) + 1);
+            launch_stage<double, double, double>(d, t, in[k - 1], U, out[k - 1], dtp, (double)(5.0f - (float)k), s);
+            mark(1 + 4 * (k - 1) + 2);
+            launch_exchange<double>(d, h->gh, out[k - 1], s);
+            mark(1 + 4 * (k - 1) + 3);
+            mark(1 + 4 * (k - 1) + 4);
+        }
+        launch_diffusion<double>(d, t, U, Un, dtp, s);
+    } else {
+        float *S0 = h->S32[0].as<float>(), *S1 = h->S32[1].as<float>();
+        // stage 1: FP64 state in, FP32 arithmetic, FP32 stage buffer out
+        launch_boundary<double>(d, h->bnd, U, s);
+        mark(2);
+        launch_stage<float, double, float>(d, t, U, U, S0, dtp, 4.0, s);
+        mark(3);
+        launch_exchange<float>(d, h->gh, S0, s);
+        mark(4); mark(5);
+        launch_boundary<float>(d, h->bnd, S0, s);
+        mark(6);
+        launch_stage<float, float, float>(d, t, S0, U, S1, dtp, 3.0, s);
+        mark(7);
+        launch_exchange<float>(d, h->gh, S1, s);
+        mark(8); mark(9);
+        launch_boundary<float>(d, h->bnd, S1, s);
+        mark(10);
+        launch_stage<float, float, float>(d, t, S1, U, S0, dtp, 2.0, s);
+        mark(11);
+        launch_exchange<float>(d, h->gh, S0, s);
+        mark(12); mark(13);
+        launch_boundary<float>(d, h->bnd, S0, s);
+        mark(14);
+        launch_stage<float, float, double>(d, t, S0, U, U, dtp, 1.0, s);
+        mark(15);
+        launch_exchange<double>(d, h->gh, U, s);
+        mark(16); mark(17);
+        launch_diffusion<float>(d, t, U, Un, dtp, s);
+    }
+    mark(18);
+    launch_exchange<double>(d, h->gh, Un, s);
+    mark(19);
+}
+
+void check_launch() { HIPCHECK(hipGetLastError()); }
+
+}  // namespace
+
+extern "C" {
+
+int falcon_last_error(char *buf, int len) {
+    if (!buf || len <= 0) return -1;
+    std::snprintf(buf, (size_t)len, "%s", g_err.c_str());
+    return 0;
+}
+
+int falcon_create(falcon_handle_t *out, int ni, int nj, int nk, int ng, double r_lo_cgs, double r_hi_cgs) {
+    return guard([&] {
+        if (!out) throw Error("null output handle");
+        if (ni < 1 || nj < 1 || nk < 1 || ng != 2) throw Error("block sizes must be >= 1 and ng must be 2");
+        if (!(r_hi_cgs > r_lo_cgs) || !(r_lo_cgs > 0)) throw Error("invalid radial range");
+        auto h = std::make_unique<falcon_handle>();
+        h->lay.tree.init(ni, nj, nk, ng, r_lo_cgs, r_hi_cgs);
+        *out = h.release();
+    });
+}
+
+int falcon_destroy(falcon_handle_t h) {
+    return guard([&] { delete h; });
+}
+
+int falcon_tree_divide_all(falcon_handle_t h, int nr, int nt, int np) {
+    return guard([&] {
+        if (h->labelled) throw Error("tree already labelled");
+        if (nr < 1 || nt < 1 || np < 1 || nr > 64 || nt > 64 || np > 64) throw Error("split counts must be in [1,64]");
+        h->lay.tree.divide_all(nr, nt, np);
+    });
+}
+
+int falcon_tree_amr(falcon_handle_t h, const int flags[3], int if_global, const double rng_rsun[6]) {
+    return guard([&] {
+        if (h->labelled) throw Error("tree already labelled");
+        h->lay.tree.amr(flags, if_global, rng_rsun);
+    });
+}
+
+int falcon_tree_set_nodes(falcon_handle_t h, int n_nodes, const int *parent, const int *is_yin, const double *rng6) {
+    return guard([&] {
+        if (h->labelled) throw Error("tree already labelled");
+        Tree &t = h->lay.tree;
+        if (n_nodes < 2) throw Error("need at least the two roots");
+        // map given pre-order nodes to the library's node table; children of a
+        // node keep their given order.
+        std::vector<std::vector<int>> kids(n_nodes);
+        std::vector<int> roots;
+        for (int n = 0; n < n_nodes; n++) {
+            if (parent[n] < 0) roots.push_back(n);
+            else {
+                if (parent[n] >= n) throw Error("nodes must be in pre-order");
+                kids[parent[n]].push_back(n);
+            }
+        }
+        if (roots.size() != 2 || !is_yin[roots[0]] || is_yin[roots[1]]) throw Error("expected Yin root then Yang root");
+        t.nodes.clear();
+        std::vector<int> map(n_nodes, -1);
+        auto mk = [&](int n) {
+            Node nd;
+            nd.yin = is_yin[n] ? 1 : 0;
+            for (int a = 0; a < 3; a++) { nd.rng[a][0] = rng6[6 * n + a]; nd.rng[a][1] = rng6[6 * n + 3 + a]; }
+            t.nodes.push_back(nd);
+            return (int)t.nodes.size() - 1;
+        };
+        map[roots[0]] = mk(roots[0]);
+        map[roots[1]] = mk(roots[1]);
+        // breadth over given nodes: allocate children contiguously
+        std::vector<int> queue = {roots[0], roots[1]};
+        for (size_t qi = 0; qi < queue.size(); qi++) {
+            int n = queue[qi];
+            if (kids[n].empty()) continue;
+            int first = (int)t.nodes.size();
+            for (int c : kids[n]) { map[c] = mk(c); queue.push_back(c); }
+            Node &p = t.nodes[map[n]];
+            p.leaf = 0;
+            p.first_child = first;
+            // children per axis: distinct lower bounds along each axis
+            int N[3];
+            for (int a = 0; a < 3; a++) {
+                std::vector<double> lo;
+                for (int c : kids[n]) lo.push_back(rng6[6 * c + a]);
+                std::sort(lo.begin(), lo.end());
+                N[a] = (int)(std::unique(lo.begin(), lo.end()) - lo.begin());
+            }
+            if (N[0] * N[1] * N[2] != (int)kids[n].size()) throw Error("children do not form a tensor split");
+            p.N[0] = N[0]; p.N[1] = N[1]; p.N[2] = N[2];
+        }
+    });
+}
+
+int falcon_nblocks(falcon_handle_t h, int *nblocks, int *nleaf_yin) {
+    return guard([&] {
+        h->need_label();
+        if (nblocks) *nblocks = h->lay.tree.nleaf;
+        if (nleaf_yin) *nleaf_yin = h->lay.tree.nleaf_yin;
+    });
+}
+
+int falcon_block_info(falcon_handle_t h, int ib, double *rng6, int *flags3, double *xi, double *xj, double *xk) {
+    return guard([&] {
+        h->need_label();
+        h->check_block(ib);
+        const Tree &t = h->lay.tree;
+        const Node &n = t.nodes[t.leaf_node[ib]];
+        if (rng6) for (int a = 0; a < 3; a++) { rng6[a] = n.rng[a][0]; rng6[3 + a] = n.rng[a][1]; }
+        // centres per ModBlock_InitGrid (same code path as the table builder)
+        const int ng = t.ng;
+        const int nn[3] = {t.ni, t.nj, t.nk};
+        double *xs[3] = {xi, xj, xk};
+        for (int a = 0; a < 3; a++) {
+            if (!xs[a]) continue;
+            for (int i = 1 - ng; i <= nn[a] + ng; i++)
+                xs[a][i + ng - 1] = (n.rng[a][0] * (double)((float)(nn[a] - i) + 0.5f) +
+                                     n.rng[a][1] * (double)((float)i - 0.5f)) / (double)nn[a];
+        }
+        if (flags3) {
+            double lo = (n.rng[0][0] * (double)((float)(t.ni + ng - 1) + 0.5f) + n.rng[0][1] * (double)((float)(1 - ng) - 0.5f)) / (double)t.ni;
+            double hi = (n.rng[0][0] * (double)((float)(-ng) + 0.5f) + n.rng[0][1] * (double)((float)(t.ni + ng) - 0.5f)) / (double)t.ni;
+            flags3[0] = n.yin;
+            flags3[1] = hi > t.r_range[1];
+            flags3[2] = lo < t.r_range[0];
+        }
+    });
+}
+
+int falcon_build_tables(falcon_handle_t h) {
+    return guard([&] {
+        h->need_label();
+        if (h->built) return;
+        Layout &L = h->lay;
+        L.keep_gcib = L.tree.nleaf <= 4096;
+        L.build(host_threads());
+        h->built = true;
+    });
+}
+
+int falcon_set_gc_all(falcon_handle_t h) {
+    if (falcon_build_tables(h) != 0) return -1;
+    return guard([&] {
+        const Tree &t = h->lay.tree;
+        Dims &d = h->dims;
+        d.ni = t.ni; d.nj = t.nj; d.nk = t.nk; d.ng = t.ng;
+        d.NI = h->NI(); d.NJ = h->NJ(); d.NK = h->NK();
+        d.nblocks = t.nleaf;
+        d.tiles_j = (t.nj + 15) / 16;
+        d.tiles_k = (t.nk + 15) / 16;
+        d.P = (long long)d.NI * d.NJ * d.NK;
+        if (!h->stream) HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        alloc_state(h);
+        h->tables = true;
+    });
+}
+
+int falcon_gc_iblocks(falcon_handle_t h, int ib, int32_t *out) {
+    return guard([&] {
+        h->need_built();
+        h->check_block(ib);
+        if (h->lay.gcib.empty()) throw Error("ghost donor maps are kept only for layouts of <= 4096 blocks");
+        const auto &g = h->lay.gcib[ib];
+        std::memcpy(out, g.data(), sizeof(int32_t) * g.size());
+    });
+}
+
+int falcon_gc_ntargets(falcon_handle_t h, int ib, int *ntargets) {
+    return guard([&] {
+        h->need_built();
+        h->check_block(ib);
+        *ntargets = (int)h->lay.targets[ib].size();
+    });
+}
+
+int falcon_gc_target(falcon_handle_t h, int ib, int it, int *donor, int *ngc, int32_t *ijk) {
+    return guard([&] {
+        h->need_built();
+        h->check_block(ib);
+        const auto &T = h->lay.targets[ib];
+        if (it < 0 || it >= (int)T.size()) throw Error("target index out of range");
+        const int NI = h->NI(), NJ = h->NJ(), ng = h->lay.tree.ng;
+        const auto &lst = T[it].second;
+        if (donor) *donor = T[it].first;
+        if (ngc) *ngc = (int)lst.size();
+        if (ijk) {
+            const size_t n = lst.size();
+            for (size_t g = 0; g < n; g++) {
+                int32_t q = lst[g];
+                ijk[g] = q % NI - ng + 1;
+                ijk[g + n] = (q / NI) % NJ - ng + 1;
+                ijk[g + 2 * n] = q / (NI * NJ) - ng + 1;
+            }
+        }
+    });
+}
+
+int falcon_gc_entries(falcon_handle_t h, int64_t *n, int32_t *recv, int32_t *donor, int32_t *gijk, int32_t *bijk,
+                      double *w, int32_t *yy, double *rot) {
+    return guard([&] {
+        h->need_built();
+        const auto &G = h->lay.ghosts;
+        *n = (int64_t)G.size();
+        if (!recv) return;
+        for (size_t e = 0; e < G.size(); e++) {
+            const GhostEntry &x = G[e];
+            recv[e] = x.recv_block;
+            donor[e] = x.donor;
+            gijk[3 * e] = x.gi; gijk[3 * e + 1] = x.gj; gijk[3 * e + 2] = x.gk;
+            bijk[3 * e] = x.i0; bijk[3 * e + 1] = x.j0; bijk[3 * e + 2] = x.k0;
+            w[3 * e] = x.w[0]; w[3 * e + 1] = x.w[1]; w[3 * e + 2] = x.w[2];
+            yy[e] = x.yy;
+            rot[2 * e] = x.rot[0]; rot[2 * e + 1] = x.rot[1];
+        }
+    });
+}
+
+int falcon_set_background(falcon_handle_t h, int ib, const double *bg) {
+    return guard([&] {
+        h->need_label();
+        h->check_block(ib);
+        const int NI = h->NI();
+        std::memcpy(h->bg.data() + (size_t)ib * 8 * NI, bg, sizeof(double) * 8 * NI);
+        h->bg_set[ib] = 1;
+    });
+}
+
+int falcon_background_polytrope(falcon_handle_t h, double contrast, double rho_top) {
+    return guard([&] {
+        h->need_label();
+        const Tree &t = h->lay.tree;
+        PolytropeParams p;
+        p.r_lo = t.r_range[0]; p.r_hi = t.r_range[1]; p.contrast = contrast; p.rho_top = rho_top;
+        const int NI = h->NI(), ng = t.ng;
+        std::vector<double> xi(NI);
+        for (int b = 0; b < t.nleaf; b++) {
+            const Node &n = t.nodes[t.leaf_node[b]];
+            for (int i = 1 - ng; i <= t.ni + ng; i++)
+                xi[i + ng - 1] = (n.rng[0][0] * (double)((float)(t.ni - i) + 0.5f) + n.rng[0][1] * (double)((float)i - 0.5f)) / (double)t.ni;
+            double *B = h->bg.data() + (size_t)b * 8 * NI;
+            for (int I = 0; I < NI; I++) {
+                double o[8];
+                polytrope(p, xi[I], o);
+                for (int a = 0; a < 8; a++) B[(size_t)a * NI + I] = o[a];
+            }
+            h->bg_set[b] = 1;
+        }
+    });
+}
+
+static void ensure_device_tables(falcon_handle_t h) {
+    if (!h->dt.meta) upload_tables(h);
+}
+
+int falcon_set_state(falcon_handle_t h, int ib, const double *prim) {
+    return guard([&] {
+        h->need_tables();
+        h->check_block(ib);
+        const size_t P = (size_t)h->dims.P;
+        std::vector<double> tmp(8 * P);
+        fortran_to_device(prim, tmp.data(), h->NI(), h->NJ(), h->NK());
+        HIPCHECK(hipMemcpy(h->U[h->cur].as<double>() + (size_t)ib * 8 * P, tmp.data(), sizeof(double) * 8 * P,
+                           hipMemcpyHostToDevice));
+    });
+}
+
+int falcon_get_state(falcon_handle_t h, int ib, double *prim) {
+    return guard([&] {
+        h->need_tables();
+        h->check_block(ib);
+        const size_t P = (size_t)h->dims.P;
+        std::vector<double> tmp(8 * P);
+        HIPCHECK(hipStreamSynchronize(h->stream));
+        HIPCHECK(hipMemcpy(tmp.data(), h->U[h->cur].as<double>() + (size_t)ib * 8 * P, sizeof(double) * 8 * P,
+                           hipMemcpyDeviceToHost));
+        device_to_fortran(tmp.data(), prim, h->NI(), h->NJ(), h->NK());
+    });
+}
+
+int falcon_init_synthetic(falcon_handle_t h, uint64_t seed, double s_amp, double b_ratio) {
+    return guard([&] {
+        h->need_tables();
+        ensure_device_tables(h);
+        std::vector<double> st;
+        synthetic_state(h->lay, h->bg, seed, s_amp, b_ratio, host_threads(), st);
+        const int nb = h->dims.nblocks;
+        const size_t P = (size_t)h->dims.P;
+        std::vector<double> dev((size_t)nb * 8 * P);
+        std::vector<std::thread> th;
+        int nt = std::min(host_threads(), nb);
+        for (int w = 0; w < nt; w++)
+            th.emplace_back([&, w]() {
+                for (int b = w; b < nb; b += nt)
+                    fortran_to_device(st.data() + (size_t)b * 8 * P, dev.data() + (size_t)b * 8 * P, h->NI(), h->NJ(), h->NK());
+            });
+        for (auto &x : th) x.join();
+        HIPCHECK(hipMemcpy(h->U[h->cur].p, dev.data(), sizeof(double) * dev.size(), hipMemcpyHostToDevice));
+        launch_exchange<double>(h->dims, h->gh, h->U[h->cur].as<double>(), h->stream);
+        launch_exchange<double>(h->dims, h->gh, h->U[h->cur].as<double>(), h->stream);
+        check_launch();
+        HIPCHECK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int falcon_set_precision(falcon_handle_t h, int mode) {
+    return guard([&] {
+        if (mode != 0 && mode != 1) throw Error("precision must be 0 (FP64) or 1 (FP32 deviation)");
+        h->precision = mode;
+    });
+}
+
+int falcon_sendrecv_gc(falcon_handle_t h, int which) {
+    return guard([&] {
+        h->need_tables();
+        ensure_device_tables(h);
+        if (which == 0) launch_exchange<double>(h->dims, h->gh, h->U[h->cur].as<double>(), h->stream);
+        else {
+            alloc_stage(h);
+            if (h->precision == 0) launch_exchange<double>(h->dims, h->gh, h->S64[0].as<double>(), h->stream);
+            else launch_exchange<float>(h->dims, h->gh, h->S32[0].as<float>(), h->stream);
+        }
+        check_launch();
+        HIPCHECK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int falcon_compute_dt(falcon_handle_t h, double cfl, double *dt) {
+    return guard([&] {
+        h->need_tables();
+        ensure_device_tables(h);
+        launch_dt(h->dims, h->dt, h->U[h->cur].as<double>(), h->partial.as<double>(), h->dtp.as<double>(), cfl, h->stream);
+        check_launch();
+        HIPCHECK(hipMemcpyAsync(dt, h->dtp.p, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHECK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int falcon_advance(falcon_handle_t h, int nsteps, double cfl, int use_graph, double *dt_last) {
+    return guard([&] {
+        h->need_tables();
+        ensure_device_tables(h);
+        alloc_stage(h);
+        if (nsteps < 0) throw Error("nsteps < 0");
+        if (use_graph && nsteps >= 2) {
+            if (!h->graph || h->graph_cur != h->cur || h->graph_prec != h->precision || h->graph_cfl != cfl) {
+                if (h->graph) { HIPCHECK(hipGraphExecDestroy(h->graph)); h->graph = nullptr; }
+                hipGraph_t g;
+                HIPCHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+                enqueue_step(h, cfl, h->cur);
+                enqueue_step(h, cfl, 1 - h->cur);
+                HIPCHECK(hipStreamEndCapture(h->stream, &g));
+                HIPCHECK(hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0));
+                HIPCHECK(hipGraphDestroy(g));
+                h->graph_cur = h->cur; h->graph_prec = h->precision; h->graph_cfl = cfl;
+            }
+            int pairs = nsteps / 2;
+            for (int p = 0; p < pairs; p++) HIPCHECK(hipGraphLaunch(h->graph, h->stream));
+            for (int s = 2 * pairs; s < nsteps; s++) { enqueue_step(h, cfl, h->cur); h->cur = 1 - h->cur; }
+        } else {
+            for (int s = 0; s < nsteps; s++) { enqueue_step(h, cfl, h->cur); h->cur = 1 - h->cur; }
+        }
+        check_launch();
+        if (dt_last) {
+            HIPCHECK(hipMemcpyAsync(dt_last, h->dtp.p, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        }
+        HIPCHECK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int falcon_advance_rk4(falcon_handle_t h, double cfl, double *dt_out) {
+    return falcon_advance(h, 1, cfl, 0, dt_out);
+}
+
+int falcon_rhs(falcon_handle_t h, double *r_out) {
+    return guard([&] {
+        h->need_tables();
+        ensure_device_tables(h);
+        const Dims &d = h->dims;
+        const size_t nn = (size_t)d.ni * d.nj * d.nk;
+        h->rbuf.alloc(sizeof(double) * nn * 8 * d.nblocks);
+        double *U = h->U[h->cur].as<double>();
+        launch_boundary<double>(d, h->bnd, U, h->stream);
+        if (h->precision == 0) launch_rhs_only<double, double>(d, h->dt, U, h->rbuf.as<double>(), h->stream);
+        else launch_rhs_only<float, double>(d, h->dt, U, h->rbuf.as<double>(), h->stream);
+        check_launch();
+        std::vector<double> tmp(nn * 8 * d.nblocks);
+        HIPCHECK(hipStreamSynchronize(h->stream));
+        HIPCHECK(hipMemcpy(tmp.data(), h->rbuf.p, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
+        // device [b][v][i][j][k] -> Fortran (ni,nj,nk,8) per block
+        for (int b = 0; b < d.nblocks; b++)
+            for (int v = 0; v < 8; v++)
+                for (int i = 0; i < d.ni; i++)
+                    for (int j = 0; j < d.nj; j++)
+                        for (int k = 0; k < d.nk; k++)
+                            r_out[((size_t)b * 8 + v) * nn + (size_t)i + (size_t)d.ni * (j + (size_t)d.nj * k)] =
+                                tmp[((size_t)b * 8 + v) * nn + ((size_t)i * d.nj + j) * d.nk + k];
+    });
+}
+
+int falcon_diffusion_rhs(falcon_handle_t h, double *r_out) {
+    return guard([&] {
+        h->need_tables();
+        ensure_device_tables(h);
+        const Dims &d = h->dims;
+        const size_t nn = (size_t)d.ni * d.nj * d.nk;
+        h->rbuf.alloc(sizeof(double) * nn * 8 * d.nblocks);
+        double *U = h->U[h->cur].as<double>();
+        if (h->precision == 0) launch_diffusion_only<double>(d, h->dt, U, h->rbuf.as<double>(), h->stream);
+        else launch_diffusion_only<float>(d, h->dt, U, h->rbuf.as<double>(), h->stream);
+        check_launch();
+        std::vector<double> tmp(nn * 8 * d.nblocks);
+        HIPCHECK(hipStreamSynchronize(h->stream));
+        HIPCHECK(hipMemcpy(tmp.data(), h->rbuf.p, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
+        for (int b = 0; b < d.nblocks; b++)
+            for (int v = 0; v < 8; v++)
+                for (int i = 0; i < d.ni; i++)
+                    for (int j = 0; j < d.nj; j++)
+                        for (int k = 0; k < d.nk; k++)
+                            r_out[((size_t)b * 8 + v) * nn + (size_t)i + (size_t)d.ni * (j + (size_t)d.nj * k)] =
+                                tmp[((size_t)b * 8 + v) * nn + ((size_t)i * d.nj + j) * d.nk + k];
+    });
+}
+
+int falcon_profile_step(falcon_handle_t h, int nsteps, double cfl, double *out_ms) {
+    return guard([&] {
+        h->need_tables();
+        ensure_device_tables(h);
+        alloc_stage(h);
+        const int NE = 20;
+        hipEvent_t ev[NE];
+        for (int i = 0; i < NE; i++) HIPCHECK(hipEventCreate(&ev[i]));
+        double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (int s = 0; s < nsteps; s++) {
+            enqueue_step(h, cfl, h->cur, ev);
+            h->cur = 1 - h->cur;
+            HIPCHECK(hipStreamSynchronize(h->stream));
+            float ms;
+            auto el = [&](int a, int b) { HIPCHECK(hipEventElapsedTime(&ms, ev[a], ev[b])); return (double)ms; };
+            acc[0] += el(0, 1);
+            for (int k = 0; k < 4; k++) {
+                const int base = 1 + 4 * k;
+                acc[1] += el(base, base + 1);
+                acc[2] += el(base + 1, base + 2);
+                if (k < 3) acc[3] += el(base + 2, base + 3);
+                else acc[4] += el(base + 2, base + 3);
+            }
+            acc[5] += el(17, 18);
+            acc[4] += el(18, 19);
+            acc[6] += el(0, 19);
+        }
+        for (int i = 0; i < NE; i++) (void)hipEventDestroy(ev[i]);
+        const double n = nsteps > 0 ? nsteps : 1;
+        out_ms[0] = acc[0] / n;          // dt
+        out_ms[1] = acc[1] / (4 * n);    // boundary (per launch)
+        out_ms[2] = acc[2] / (4 * n);    // stage (per launch)
+        out_ms[3] = acc[3] / (3 * n);    // exchange on stage buffers (per launch)
+        out_ms[4] = acc[4] / (2 * n);    // exchange on state (per launch)
+        out_ms[5] = acc[5] / n;          // diffusion
+        out_ms[6] = acc[6] / n;          // whole step
+    });
+}
+
+int falcon_device_bytes(falcon_handle_t h, int64_t *bytes) {
+    return guard([&] {
+        const DevBuf *all[] = {&h->meta, &h->radial, &h->angular, &h->ghost_dst, &h->ghost_src, &h->ghost_w, &h->ghost_rot,
+                               &h->bnd_list, &h->U[0], &h->U[1], &h->S64[0], &h->S64[1], &h->S32[0], &h->S32[1],
+                               &h->dtp, &h->partial, &h->rbuf};
+        int64_t s = 0;
+        for (const DevBuf *b : all) s += (int64_t)b->bytes;
+        *bytes = s;
+    });
+}
+
+int falcon_synchronize(falcon_handle_t h) {
+    return guard([&] {
+        if (h->stream) HIPCHECK(hipStreamSynchronize(h->stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,634 @@
+// Device kernels of the RK4 hot path for gfx950.
+//
+// Layout in HBM: every state buffer is [block][var][I][J][K] with the padded
+// block (NI,NJ,NK) = (ni+2ng, nj+2ng, nk+2ng) contiguous and K (phi) fastest,
+// so a wavefront's loads along phi are contiguous.
+//
+// Stage and diffusion kernels march each block along r.  A workgroup of 256
+// lanes owns a 16x16 (theta, phi) tile of one block; lane (ty, tx) owns one
+// (j, k) column and keeps a 5-plane register window of all 8 variables for the
+// r-stencils, while the current r-plane with its 2-cell theta/phi halo is staged
+// in double-buffered LDS for the theta/phi stencils.  Corner (edge) halo cells
+// are never needed by these stencils, so they are never loaded.
+//
+// Every arithmetic expression keeps the operand order of the reference line it
+// restates; with -ffp-contract=off the FP64 instantiation is bit-identical to
+// the reference.  The FP32 instantiation (T = float) is the deviation path: the
+// stencil arithmetic runs in FP32 while the stage accumulation into the FP64
+// state, the ghost interpolation weights and the time step stay FP64.
+#include "device.hpp"
+
+namespace falcon {
+
+constexpr int TILE = 16;
+constexpr int PL = TILE + 4;   // LDS plane edge (tile + 2 halo each side)
+
+// ModDerivative_1st_O4_3D (3_6/ModDerivative.f90:16-19):
+// (1./12./dx)*( -f(+2) + 8.*f(+1) - 8.*f(-1) + f(-2) )
+template <typename T>
+__device__ __forceinline__ T d4(T fm2, T fm1, T fp1, T fp2, T c) {
+    return c * ((((-fp2) + T(8) * fp1) - T(8) * fm1) + fm2);
+}
+
+// ModLinReconstruct_minmod (3_6/ModLinReconstruct.f90:37-48) for one triple
+template <typename T>
+__device__ __forceinline__ T minmod(T ul, T u, T ur) {
+    T a = (ur - ul) * T(0.5);
+    T b = T(2) * (ur - u);
+    T c = T(2) * (u - ul);
+    T ab = a * b, bc = b * c;
+    T m1 = ab > T(0) ? ab : T(0);
+    T m2 = bc > T(0) ? bc : T(0);
+    T s = (m1 * m2) * T(1.0e10) - T(1.0e-30f);
+    T half = copysign(T(0.5), s);
+    T fa = fabs(a), fb = fabs(b), fc = fabs(c);
+    T mn = fa < fb ? fa : fb;
+    mn = mn < fc ? mn : fc;
+    return (half + T(0.5)) * copysign(mn, a);
+}
+
+// One face of ModDiffusion_Aritificial_1 (3_6/ModDiffusion.f90:50-59) between
+// cell y (value b) and cell x = y+1 (value c); a = y-1, d = x+1.
+// MAX(0.0, NaN) is taken as 0.0 (the comparison below is false for NaN).
+template <typename T>
+__device__ __forceinline__ T face_flux(T a, T b, T c, T d) {
+    T dX = minmod(b, c, d);
+    T dY = minmod(a, b, c);
+    T num = ((c - dX * T(0.5)) - b) - dY * T(0.5);
+    T ph = T(1) + T(2) * (num / (c - b) - T(1));
+    ph = ph > T(0) ? ph : T(0);
+    return ((-T(0.5)) * ph) * num;
+}
+
+// Shared r-marching skeleton: loads the column window and the halo'd plane.
+template <typename T, typename TIN>
+struct Marcher {
+    const TIN *blk;
+    long long P;
+    int SI, SJ;
+    int J, K;           // padded column of this lane
+    bool act;
+    bool hv;
+    long long hoff;     // halo point offset within a plane
+    int hly, hlx;
+
+    __device__ void init(const Dims &d, const TIN *in, int b, int j0, int k0, int TJa, int TKa) {
+        P = d.P;
+        SI = d.NJ * d.NK;
+        SJ = d.NK;
+        blk = in + (long long)b * 8 * P;
+        const int tx = threadIdx.x & (TILE - 1), ty = threadIdx.x / TILE;
+        act = ty < TJa && tx < TKa;
+        J = j0 + ty + d.ng;
+        K = k0 + tx + d.ng;
+        const int h = threadIdx.x;
+        hv = false; hly = 0; hlx = 0;
+        if (h < 4 * TILE) {
+            int r = h / TILE, kk = h % TILE;
+            hly = r < 2 ? r : TJa + r;
+            hlx = kk + 2;
+            hv = kk < TKa;
+        } else if (h < 8 * TILE) {
+            int h2 = h - 4 * TILE;
+            int c = h2 / TILE, jj = h2 % TILE;
+            hlx = c < 2 ? c : TKa + c;
+            hly = jj + 2;
+            hv = jj < TJa;
+        }
+        hoff = (long long)(j0 + hly) * SJ + (k0 + hlx);
+    }
+    __device__ __forceinline__ T col(int v, int I) const {
+        return (T)blk[(long long)v * P + (long long)I * SI + (long long)J * SJ + K];
+    }
+    __device__ __forceinline__ T halo(int v, int I) const {
+        return (T)blk[(long long)v * P + (long long)I * SI + hoff];
+    }
+};
+
+// ---------------------------------------------------------------- stage
+
+// RHS of ModEquation_Dynamo (3_6/ModEquation.f90:24-162) for one cell.
+// win[v][q]: variable v at r-offset q-2 of this lane's column; L(v,dj,dk): LDS plane.
+template <typename T, typename LDSF>
+__device__ __forceinline__ void cell_rhs(const T (&win)[8][5], LDSF L, const T xr[5], const T rho0r[5], const T gpr[5],
+                                         const T gqr[5], const T st[5], T rinv, T r2inv, T xir2, T heat, T gor,
+                                         T sinv, T cot, T cr, T ct, T cp, T R[8]) {
+    const T rho0c = rho0r[2];
+    const T vr = win[1][2], vt = win[2][2], vp = win[3][2];
+    const T br = win[4][2], bt = win[5][2], bp = win[6][2];
+
+    // Mass conservation (:50-61) with ModSpherical_div (:104-159)
+    {
+        T w[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) w[q] = (rho0r[q] * win[1][q]) * (xr[q] * xr[q]);
+        T t1 = d4(w[0], w[1], w[3], w[4], cr) * r2inv;
+        T t2 = d4((rho0c * L(2, -2, 0)) * st[0], (rho0c * L(2, -1, 0)) * st[1], (rho0c * L(2, 1, 0)) * st[3],
+                  (rho0c * L(2, 2, 0)) * st[4], ct);
+        T dd = d4(rho0c * L(3, 0, -2), rho0c * L(3, 0, -1), rho0c * L(3, 0, 1), rho0c * L(3, 0, 2), cp);
+        t2 = t2 + dd;
+        t2 = t2 * sinv;
+        t2 = t2 * rinv;
+        R[0] = xir2 * (t1 + t2);
+    }
+    // A_dot_Grad_f (ModSpherical.f90:241-279) with A = v at the centre
+    auto adg = [&](int f) -> T {
+        T o = vr * d4(win[f][0], win[f][1], win[f][3], win[f][4], cr);
+        T tmp = vt * d4(L(f, -2, 0), L(f, -1, 0), L(f, 1, 0), L(f, 2, 0), ct);
+        o = o + tmp * rinv;
+        tmp = (vp * d4(L(f, 0, -2), L(f, 0, -1), L(f, 0, 1), L(f, 0, 2), cp)) * rinv;
+        o = o + sinv * tmp;
+        return o;
+    };
+    // Inertial force (:66-74): -A_dot_nabla_B(v,v) (ModSpherical.f90:281-343)
+    {
+        T a1 = adg(1), a2 = adg(2), a3 = adg(3);
+        T tr = (-vt) * vt - vp * vp;
+        a1 = a1 + tr * rinv;
+        T tt = (-vp) * vp;
+        tt = tt * cot;
+        tt = tt + vt * vr;
+        a2 = a2 + tt * rinv;
+        T tp = vp * vt;
+        tp = tp * cot;
+        tp = tp + vp * vr;
+        a3 = a3 + tp * rinv;
+        R[1] = -a1; R[2] = -a2; R[3] = -a3;
+    }
+    // Pressure gradient (:79-92): p1 (:40-46), ModSpherical_Grad_f (:201-239)
+    {
+        T pr[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) pr[q] = gpr[q] * win[0][q] + gqr[q] * win[7][q];
+        auto p1L = [&](int dj, int dk) -> T { return gpr[2] * L(0, dj, dk) + gqr[2] * L(7, dj, dk); };
+        T t1 = d4(pr[0], pr[1], pr[3], pr[4], cr);
+        T t2 = d4(p1L(-2, 0), p1L(-1, 0), p1L(1, 0), p1L(2, 0), ct);
+        T t3 = d4(p1L(0, -2), p1L(0, -1), p1L(0, 1), p1L(0, 2), cp);
+        T g2 = t2 * rinv;
+        T g3 = (t3 * rinv) * sinv;
+        R[1] = R[1] - t1 / rho0c;
+        R[2] = R[2] - g2 / rho0c;
+        R[3] = R[3] - g3 / rho0c;
+    }
+    // Lorentz force (:97-116): curl(B) (ModSpherical.f90:38-102) x B / rho0
+    {
+        T o1 = d4(L(6, -2, 0) * st[0], L(6, -1, 0) * st[1], L(6, 1, 0) * st[3], L(6, 2, 0) * st[4], ct) -
+               d4(L(5, 0, -2), L(5, 0, -1), L(5, 0, 1), L(5, 0, 2), cp);
+        T o2 = d4(L(4, 0, -2) / st[2], L(4, 0, -1) / st[2], L(4, 0, 1) / st[2], L(4, 0, 2) / st[2], cp) -
+               d4(win[6][0] * xr[0], win[6][1] * xr[1], win[6][3] * xr[3], win[6][4] * xr[4], cr);
+        T o3 = d4(win[5][0] * xr[0], win[5][1] * xr[1], win[5][3] * xr[3], win[5][4] * xr[4], cr) -
+               d4(L(4, -2, 0), L(4, -1, 0), L(4, 1, 0), L(4, 2, 0), ct);
+        o1 = o1 * rinv; o2 = o2 * rinv; o3 = o3 * rinv;
+        o1 = o1 * sinv;
+        T c1 = o2 * bp - o3 * bt;
+        T c2 = o3 * br - o1 * bp;
+        T c3 = o1 * bt - o2 * br;
+        R[1] = R[1] + c1 / rho0c;
+        R[2] = R[2] + c2 / rho0c;
+        R[3] = R[3] + c3 / rho0c;
+    }
+    // Gravity (:121-127)
+    R[1] = R[1] - gor * win[0][2];
+    // Entropy advection and heating (:132-151)
+    R[7] = -adg(7);
+    R[7] = R[7] + heat;
+    // Induction (:154-162): curl(v x B), cross product per point (ModSpherical.f90:26-36)
+    {
+        auto e1L = [&](int dj, int dk) -> T { return L(2, dj, dk) * L(6, dj, dk) - L(3, dj, dk) * L(5, dj, dk); };
+        auto e2L = [&](int dj, int dk) -> T { return L(3, dj, dk) * L(4, dj, dk) - L(1, dj, dk) * L(6, dj, dk); };
+        auto e3L = [&](int dj, int dk) -> T { return L(1, dj, dk) * L(5, dj, dk) - L(2, dj, dk) * L(4, dj, dk); };
+        T e2r[5], e3r[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            e2r[q] = win[3][q] * win[4][q] - win[1][q] * win[6][q];
+            e3r[q] = win[1][q] * win[5][q] - win[2][q] * win[4][q];
+        }
+        T o1 = d4(e3L(-2, 0) * st[0], e3L(-1, 0) * st[1], e3L(1, 0) * st[3], e3L(2, 0) * st[4], ct) -
+               d4(e2L(0, -2), e2L(0, -1), e2L(0, 1), e2L(0, 2), cp);
+        T o2 = d4(e1L(0, -2) / st[2], e1L(0, -1) / st[2], e1L(0, 1) / st[2], e1L(0, 2) / st[2], cp) -
+               d4(e3r[0] * xr[0], e3r[1] * xr[1], e3r[3] * xr[3], e3r[4] * xr[4], cr);
+        T o3 = d4(e2r[0] * xr[0], e2r[1] * xr[1], e2r[3] * xr[3], e2r[4] * xr[4], cr) -
+               d4(e1L(-2, 0), e1L(-1, 0), e1L(1, 0), e1L(2, 0), ct);
+        o1 = o1 * rinv; o2 = o2 * rinv; o3 = o3 * rinv;
+        o1 = o1 * sinv;
+        R[4] = o1; R[5] = o2; R[6] = o3;
+    }
+}
+
+// MODE 0: out = U + (dt*R)/div on the interior (3_6/ModAdvance.f90:87-93).
+// MODE 1: write R (interior layout [block][var][i][j][k], FP64) for parity tests.
+template <typename T, typename TIN, typename TOUT, int MODE>
+__global__ void __launch_bounds__(256) k_stage(Dims d, DevTables t, const TIN *__restrict__ in,
+                                               const double *__restrict__ U, TOUT *__restrict__ out,
+                                               const double *__restrict__ dtp, double div) {
+    __shared__ T lds[2][8][PL][PL];
+    const int tpb = d.tiles_j * d.tiles_k;
+    const int b = blockIdx.x / tpb;
+    const int trem = blockIdx.x - b * tpb;
+    const int tj = trem / d.tiles_k, tk = trem - tj * d.tiles_k;
+    const int j0 = tj * TILE, k0 = tk * TILE;
+    const int TJa = min(TILE, d.nj - j0), TKa = min(TILE, d.nk - k0);
+    const int tx = threadIdx.x & (TILE - 1), ty = threadIdx.x / TILE;
+
+    Marcher<T, TIN> m;
+    m.init(d, in, b, j0, k0, TJa, TKa);
+
+    const long long rb = (long long)b * d.NI, tb = (long long)b * d.NJ;
+    const BlockMeta mt = t.meta[b];
+    const T cr = (T)mt.cr, ct = (T)mt.ct, cp = (T)mt.cp;
+    // theta rows are read only by active lanes (inactive lanes of a partial
+    // tile have J beyond the padded range)
+    T st[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) st[q] = m.act ? (T)t.sint[tb + m.J - 2 + q] : T(1);
+    const T sinv = m.act ? (T)t.sinv[tb + m.J] : T(1), cot = m.act ? (T)t.cot[tb + m.J] : T(0);
+    const double dt = MODE == 0 ? *dtp : 0.0;
+
+    T win[8][5];
+    T nxt[8], hn[8];
+#pragma unroll
+    for (int v = 0; v < 8; v++) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) win[v][q] = m.act ? m.col(v, q) : T(0);
+        hn[v] = m.hv ? m.halo(v, d.ng) : T(0);
+    }
+
+    for (int i = 0; i < d.ni; i++) {
+        const int I = i + d.ng;
+        const int buf = i & 1;
+        // stage this plane: own column centre plus halo strips
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+            if (m.act) lds[buf][v][ty + 2][tx + 2] = win[v][2];
+            if (m.hv) lds[buf][v][m.hly][m.hlx] = hn[v];
+        }
+        // prefetch the next window plane and the next halo
+        const bool more = i + 1 < d.ni;
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+            nxt[v] = (m.act && more) ? m.col(v, I + 3) : T(0);
+            hn[v] = (m.hv && more) ? m.halo(v, I + 1) : T(0);
+        }
+        __syncthreads();
+        if (m.act) {
+            T xr[5], rho0r[5], gpr[5], gqr[5];
+#pragma unroll
+            for (int q = 0; q < 5; q++) {
+                xr[q] = (T)t.xi[rb + I - 2 + q];
+                rho0r[q] = (T)t.rho0[rb + I - 2 + q];
+                gpr[q] = (T)t.gp[rb + I - 2 + q];
+                gqr[q] = (T)t.gq[rb + I - 2 + q];
+            }
+            auto L = [&](int v, int dj, int dk) -> T { return lds[buf][v][ty + 2 + dj][tx + 2 + dk]; };
+            T R[8];
+            cell_rhs<T>(win, L, xr, rho0r, gpr, gqr, st, (T)t.rinv[rb + I], (T)t.r2inv[rb + I], (T)t.xir2[rb + I],
+                        (T)t.heat[rb + I], (T)t.g_over_rho0[rb + I], sinv, cot, cr, ct, cp, R);
+            if (MODE == 0) {
+                const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + (long long)m.J * m.SJ + m.K;
+#pragma unroll
+                for (int v = 0; v < 8; v++) {
+                    const long long q = base + (long long)v * d.P;
+                    out[q] = (TOUT)(U[q] + (dt * (double)R[v]) / div);
+                }
+            } else {
+                const long long nn = (long long)d.ni * d.nj * d.nk;
+                const long long base = (long long)b * 8 * nn + ((long long)i * d.nj + (j0 + ty)) * d.nk + (k0 + tx);
+#pragma unroll
+                for (int v = 0; v < 8; v++) ((double *)out)[base + (long long)v * nn] = (double)R[v];
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+            win[v][0] = win[v][1]; win[v][1] = win[v][2]; win[v][2] = win[v][3]; win[v][3] = win[v][4];
+            win[v][4] = nxt[v];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- diffusion
+
+// ModDiffusion_Aritificial_1 (3_6/ModDiffusion.f90:9-150), h = 2, all 8 vars.
+// The characteristic speed c of each face is read where the reference reads it:
+// the section c(1:ni+1,:,:) (:63) spans the padded transverse range while flux
+// spans 1:nj,1:nk, so the speed comes from transverse index (index - ng); same
+// for :96 and :130.
+// MODE 0: out = U + (0.5*dt)*R (3_6/ModAdvance.f90:104-107); MODE 1: write R.
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const double *__restrict__ U,
+                                                   double *__restrict__ out, const double *__restrict__ dtp) {
+    __shared__ T lds[2][8][PL][PL];
+    const int tpb = d.tiles_j * d.tiles_k;
+    const int b = blockIdx.x / tpb;
+    const int trem = blockIdx.x - b * tpb;
+    const int tj = trem / d.tiles_k, tk = trem - tj * d.tiles_k;
+    const int j0 = tj * TILE, k0 = tk * TILE;
+    const int TJa = min(TILE, d.nj - j0), TKa = min(TILE, d.nk - k0);
+    const int tx = threadIdx.x & (TILE - 1), ty = threadIdx.x / TILE;
+
+    Marcher<T, double> m;
+    m.init(d, U, b, j0, k0, TJa, TKa);
+    const long long rb = (long long)b * d.NI, tb = (long long)b * d.NJ;
+    const BlockMeta mt = t.meta[b];
+    const T dxi = (T)mt.dxi, dxj = (T)mt.dxj, dxk = (T)mt.dxk;
+    const T sintc = m.act ? (T)t.sint[tb + m.J] : T(1);
+    const double dt = MODE == 0 ? *dtp : 0.0;
+
+    T win[8][5];
+    T nxt[8], hn[8];
+#pragma unroll
+    for (int v = 0; v < 8; v++) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) win[v][q] = m.act ? m.col(v, q) : T(0);
+        hn[v] = m.hv ? m.halo(v, d.ng) : T(0);
+    }
+    const double *Ub = U + (long long)b * 8 * d.P;
+
+    for (int i = 0; i < d.ni; i++) {
+        const int I = i + d.ng;
+        const int buf = i & 1;
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+            if (m.act) lds[buf][v][ty + 2][tx + 2] = win[v][2];
+            if (m.hv) lds[buf][v][m.hly][m.hlx] = hn[v];
+        }
+        const bool more = i + 1 < d.ni;
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+            nxt[v] = (m.act && more) ? m.col(v, I + 3) : T(0);
+            hn[v] = (m.hv && more) ? m.halo(v, I + 1) : T(0);
+        }
+        __syncthreads();
+        if (m.act) {
+            const int J = m.J, K = m.K;
+            auto at = [&](int v, int II, int JJ, int KK) -> T {
+                return (T)Ub[(long long)v * d.P + (long long)II * m.SI + (long long)JJ * m.SJ + KK];
+            };
+            // speeds |v_d| + c_s at the cells the reference reads (see above)
+            const T csI = (T)t.cs[rb + I], csIm = (T)t.cs[rb + I - 1], csIp = (T)t.cs[rb + I + 1], csI2 = (T)t.cs[rb + I - 2];
+            const T c1m = fabs(at(1, I - 1, J - 2, K - 2)) + csIm;
+            const T c10 = fabs(at(1, I, J - 2, K - 2)) + csI;
+            const T c1p = fabs(at(1, I + 1, J - 2, K - 2)) + csIp;
+            const T c2m = fabs(at(2, I - 2, J - 1, K - 2)) + csI2;
+            const T c20 = fabs(at(2, I - 2, J, K - 2)) + csI2;
+            const T c2p = fabs(at(2, I - 2, J + 1, K - 2)) + csI2;
+            const T c3m = fabs(at(3, I - 2, J - 2, K - 1)) + csI2;
+            const T c30 = fabs(at(3, I - 2, J - 2, K)) + csI2;
+            const T c3p = fabs(at(3, I - 2, J - 2, K + 1)) + csI2;
+            const T xic = (T)t.xi[rb + I];
+            const T L2 = dxj * xic;
+            const T L3 = (dxk * xic) * sintc;
+            auto Lv = [&](int v, int dj, int dk) -> T { return lds[buf][v][ty + 2 + dj][tx + 2 + dk]; };
+            T R[8];
+#pragma unroll
+            for (int v = 0; v < 8; v++) {
+                T r = T(0);
+                {
+                    T f0 = (face_flux(win[v][0], win[v][1], win[v][2], win[v][3]) * (c10 + c1m)) * T(0.5);
+                    T f1 = (face_flux(win[v][1], win[v][2], win[v][3], win[v][4]) * (c1p + c10)) * T(0.5);
+                    r = r + (f0 - f1) / dxi;
+                }
+                {
+                    T a = Lv(v, -2, 0), bb = Lv(v, -1, 0), c = Lv(v, 0, 0), dd = Lv(v, 1, 0), e = Lv(v, 2, 0);
+                    T f0 = (face_flux(a, bb, c, dd) * (c20 + c2m)) * T(0.5);
+                    T f1 = (face_flux(bb, c, dd, e) * (c2p + c20)) * T(0.5);
+                    r = r + (f0 - f1) / L2;
+                }
+                {
+                    T a = Lv(v, 0, -2), bb = Lv(v, 0, -1), c = Lv(v, 0, 0), dd = Lv(v, 0, 1), e = Lv(v, 0, 2);
+                    T f0 = (face_flux(a, bb, c, dd) * (c30 + c3m)) * T(0.5);
+                    T f1 = (face_flux(bb, c, dd, e) * (c3p + c30)) * T(0.5);
+                    r = r + (f0 - f1) / L3;
+                }
+                R[v] = r;
+            }
+            if (MODE == 0) {
+                const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + (long long)J * m.SJ + K;
+#pragma unroll
+                for (int v = 0; v < 8; v++) {
+                    const long long q = base + (long long)v * d.P;
+                    out[q] = U[q] + (0.5 * dt) * (double)R[v];
+                }
+            } else {
+                const long long nn = (long long)d.ni * d.nj * d.nk;
+                const long long base = (long long)b * 8 * nn + ((long long)i * d.nj + (j0 + ty)) * d.nk + (k0 + tx);
+#pragma unroll
+                for (int v = 0; v < 8; v++) out[base + (long long)v * nn] = (double)R[v];
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+            win[v][0] = win[v][1]; win[v][1] = win[v][2]; win[v][2] = win[v][3]; win[v][3] = win[v][4];
+            win[v][4] = nxt[v];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- exchange
+
+// ModCommunication_SendRecvGC_new (3_6/ModCommunication.f90:678-721) as one
+// table-driven gather: trilinear in the fixed term order of ModMath.f90:283-291
+// with FP64 weights and FP64 accumulation, then the Yin-Yang rotation of v and B
+// (ModYinYang_VecConv_1D, ModYinYang.f90:59-60).  Each ghost has one donor, and
+// donors are read only at interior cells (clamped bases touch a donor ghost only
+// with an exactly-zero weight), so entries are independent.
+template <typename TS>
+__global__ void __launch_bounds__(256) k_exchange(Dims d, DevGhosts g, TS *__restrict__ st) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.n_plain + g.n_yy) return;
+    const long long P = d.P;
+    const int SI = d.NJ * d.NK, SJ = d.NK;
+    const int dst = g.dst[e], src = g.src[e];
+    const double wi = g.w[3 * (long long)e], wj = g.w[3 * (long long)e + 1], wk = g.w[3 * (long long)e + 2];
+    double val[8];
+#pragma unroll
+    for (int v = 0; v < 8; v++) {
+        const TS *F = st + src + (long long)v * P;
+        double t = (double)F[0] * (1.0 - wi) * (1.0 - wj) * (1.0 - wk);
+        t = t + (double)F[SI] * (wi) * (1.0 - wj) * (1.0 - wk);
+        t = t + (double)F[SJ] * (1.0 - wi) * (wj) * (1.0 - wk);
+        t = t + (double)F[1] * (1.0 - wi) * (1.0 - wj) * (wk);
+        t = t + (double)F[SI + SJ] * (wi) * (wj) * (1.0 - wk);
+        t = t + (double)F[SJ + 1] * (1.0 - wi) * (wj) * (wk);
+        t = t + (double)F[SI + 1] * (wi) * (1.0 - wj) * (wk);
+        t = t + (double)F[SI + SJ + 1] * (wi) * (wj) * (wk);
+        val[v] = t;
+    }
+    if (e >= g.n_plain) {
+        const long long q = 2 * (long long)(e - g.n_plain);
+        const double a = g.rot[q], bb = g.rot[q + 1];
+        double b2 = -(a * val[5]) - bb * val[6];
+        double b3 = -(a * val[6]) + bb * val[5];
+        val[5] = b2; val[6] = b3;
+        double v2 = -(a * val[2]) - bb * val[3];
+        double v3 = -(a * val[3]) + bb * val[2];
+        val[2] = v2; val[3] = v3;
+    }
+#pragma unroll
+    for (int v = 0; v < 8; v++) st[dst + (long long)v * P] = (TS)val[v];
+}
+
+// ---------------------------------------------------------------- boundary
+
+// ModBoundary_Dynamo_primitives (3_6/ModBoundary.f90:10-55): radial mirror of
+// the ng ghost planes of top/bottom blocks over all (j,k), parities per variable.
+template <typename TS>
+__global__ void __launch_bounds__(256) k_boundary(Dims d, DevBoundary bl, TS *__restrict__ st) {
+    const long long plane = (long long)d.NJ * d.NK;
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long per = (long long)d.ng * plane;
+    if (tid >= per * bl.n) return;
+    const int e = (int)(tid / per);
+    const long long rem = tid - (long long)e * per;
+    const int g = (int)(rem / plane);
+    const long long jk = rem - (long long)g * plane;
+    const int b = bl.list[e] >> 1, side = bl.list[e] & 1;
+    int dI, sI;
+    if (side) { dI = d.ni + d.ng + g; sI = d.ni + d.ng - 1 - g; }
+    else { dI = g; sI = 2 * d.ng - 1 - g; }
+    // signs: top (+,-,+,+,+,-,-,+), bottom (+,-,+,+,-,+,+,+)
+    const unsigned neg = side ? 0x62u : 0x12u;   // bit v set -> negate variable v
+    TS *base = st + (long long)b * 8 * d.P;
+#pragma unroll
+    for (int v = 0; v < 8; v++) {
+        TS x = base[(long long)v * d.P + (long long)sI * plane + jk];
+        base[(long long)v * d.P + (long long)dI * plane + jk] = ((neg >> v) & 1u) ? -x : x;
+    }
+}
+
+// ---------------------------------------------------------------- dt
+
+// ModWaveSpeed_Dynamo (3_6/ModWaveSpeed.f90:12-24) and the per-block
+// minval(h/v_wave) of ModTimeStep_Dynamo (3_6/ModTimeStep.f90:27).
+__global__ void __launch_bounds__(256) k_dt_partial(Dims d, DevTables t, const double *__restrict__ U,
+                                                    double *__restrict__ partial) {
+    const int b = blockIdx.x;
+    const long long rb = (long long)b * d.NI, tb = (long long)b * d.NJ;
+    const BlockMeta mt = t.meta[b];
+    const double *Ub = U + (long long)b * 8 * d.P;
+    const int SI = d.NJ * d.NK, SJ = d.NK;
+    const int n = d.ni * d.nj * d.nk;
+    double mn = INFINITY;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        const int k = c % d.nk, j = (c / d.nk) % d.nj, i = c / (d.nk * d.nj);
+        const int I = i + d.ng, J = j + d.ng, K = k + d.ng;
+        const long long q = (long long)I * SI + (long long)J * SJ + K;
+        const double vr = Ub[q + d.P], vt = Ub[q + 2 * d.P], vp = Ub[q + 3 * d.P];
+        const double br = Ub[q + 4 * d.P], bt = Ub[q + 5 * d.P], bp = Ub[q + 6 * d.P];
+        double w = t.cs[rb + I] + sqrt(vr * vr + vt * vt + vp * vp);
+        w = w + sqrt(br * br + bt * bt + bp * bp) / t.sqrt_rho0[rb + I];
+        // h_LLL (3_6/ModBlock.f90:420-422)
+        const double xi = t.xi[rb + I];
+        double h = mt.dxi, h2 = xi * mt.dxj, h3 = (xi * t.sint[tb + J]) * mt.dxk;
+        h = h < h2 ? h : h2;
+        h = h < h3 ? h : h3;
+        const double x = h / w;
+        mn = x < mn ? x : mn;
+    }
+    // wavefront shuffle, then LDS across the 4 wavefronts (min is exact in any order)
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_down(mn, off, 64);
+        mn = o < mn ? o : mn;
+    }
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mn;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = red[w] < r ? red[w] : r;
+        partial[b] = r;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_dt_final(const double *__restrict__ partial, int n, double cfl,
+                                                  double *__restrict__ dtp) {
+    double mn = INFINITY;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) mn = partial[i] < mn ? partial[i] : mn;
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_down(mn, off, 64);
+        mn = o < mn ? o : mn;
+    }
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mn;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = red[w] < r ? red[w] : r;
+        // dt = min(1.e20, CFL*min) (3_6/ModTimeStep.f90:254-262)
+        double dt = cfl * r;
+        const double big = (double)1.0e20f;
+        dtp[0] = dt < big ? dt : big;
+    }
+}
+
+__global__ void k_copy(const double *__restrict__ s, double *__restrict__ d, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) d[i] = s[i];
+}
+
+// ---------------------------------------------------------------- launchers
+
+static inline unsigned tiles(const Dims &d) { return (unsigned)(d.nblocks * d.tiles_j * d.tiles_k); }
+
+template <typename T, typename TIN, typename TOUT>
+void launch_stage(const Dims &d, const DevTables &t, const TIN *in, const double *U, TOUT *out, const double *dtp,
+                  double div, hipStream_t s) {
+    hipLaunchKernelGGL((k_stage<T, TIN, TOUT, 0>), dim3(tiles(d)), dim3(256), 0, s, d, t, in, U, out, dtp, div);
+}
+
+template <typename T, typename TIN>
+void launch_rhs_only(const Dims &d, const DevTables &t, const TIN *in, double *R, hipStream_t s) {
+    hipLaunchKernelGGL((k_stage<T, TIN, double, 1>), dim3(tiles(d)), dim3(256), 0, s, d, t, in, (const double *)nullptr,
+                       R, (const double *)nullptr, 1.0);
+}
+
+template <typename T>
+void launch_diffusion(const Dims &d, const DevTables &t, const double *U, double *out, const double *dtp, hipStream_t s) {
+    hipLaunchKernelGGL((k_diffusion<T, 0>), dim3(tiles(d)), dim3(256), 0, s, d, t, U, out, dtp);
+}
+
+template <typename T>
+void launch_diffusion_only(const Dims &d, const DevTables &t, const double *U, double *R, hipStream_t s) {
+    hipLaunchKernelGGL((k_diffusion<T, 1>), dim3(tiles(d)), dim3(256), 0, s, d, t, U, R, (const double *)nullptr);
+}
+
+template <typename TS>
+void launch_exchange(const Dims &d, const DevGhosts &g, TS *state, hipStream_t s) {
+    const int n = g.n_plain + g.n_yy;
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_exchange<TS>), dim3((n + 255) / 256), dim3(256), 0, s, d, g, state);
+}
+
+template <typename TS>
+void launch_boundary(const Dims &d, const DevBoundary &b, TS *state, hipStream_t s) {
+    if (b.n == 0) return;
+    const long long n = (long long)b.n * d.ng * d.NJ * d.NK;
+    hipLaunchKernelGGL((k_boundary<TS>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, b, state);
+}
+
+void launch_dt(const Dims &d, const DevTables &t, const double *U, double *partial, double *dtp, double cfl, hipStream_t s) {
+    hipLaunchKernelGGL(k_dt_partial, dim3(d.nblocks), dim3(256), 0, s, d, t, U, partial);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, s, (const double *)partial, d.nblocks, cfl, dtp);
+}
+
+void launch_copy(const double *src, double *dst, long long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, s, src, dst, n);
+}
+
+// explicit instantiations: FP64 path, FP32-deviation path
+template void launch_stage<double, double, double>(const Dims &, const DevTables &, const double *, const double *, double *, const double *, double, hipStream_t);
+template void launch_stage<float, double, float>(const Dims &, const DevTables &, const double *, const double *, float *, const double *, double, hipStream_t);
+template void launch_stage<float, float, float>(const Dims &, const DevTables &, const float *, const double *, float *, const double *, double, hipStream_t);
+template void launch_stage<float, float, double>(const Dims &, const DevTables &, const float *, const double *, double *, const double *, double, hipStream_t);
+template void launch_rhs_only<double, double>(const Dims &, const DevTables &, const double *, double *, hipStream_t);
+template void launch_rhs_only<float, double>(const Dims &, const DevTables &, const double *, double *, hipStream_t);
+template void launch_diffusion<double>(const Dims &, const DevTables &, const double *, double *, const double *, hipStream_t);
+template void launch_diffusion<float>(const Dims &, const DevTables &, const double *, double *, const double *, hipStream_t);
+template void launch_diffusion_only<double>(const Dims &, const DevTables &, const double *, double *, hipStream_t);
+template void launch_diffusion_only<float>(const Dims &, const DevTables &, const double *, double *, hipStream_t);
+template void launch_exchange<double>(const Dims &, const DevGhosts &, double *, hipStream_t);
+template void launch_exchange<float>(const Dims &, const DevGhosts &, float *, hipStream_t);
+template void launch_boundary<double>(const Dims &, const DevBoundary &, double *, hipStream_t);
+template void launch_boundary<float>(const Dims &, const DevBoundary &, float *, hipStream_t);
+
+}  // namespace falcon

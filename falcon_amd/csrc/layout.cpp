@@ -1,0 +1,331 @@
+// Host-side layout builder (see layout.hpp).  Compiled with -ffp-contract=off:
+// every floating-point expression below is evaluated in the order the cited
+// reference line evaluates it, so tables are bit-identical to the reference's.
+#include "layout.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <thread>
+
+namespace falcon {
+
+// 3_6/ModConst.f90:2,5 are default-real (float) parameters.
+static const float kDpiF = 3.1415926535f;
+static const float kRsunF = 6.955e10f;
+
+void coordconv(const double in[3], double out[3]) {
+    double st = std::sin(in[1]), sp = std::sin(in[2]), cp = std::cos(in[2]), ct = std::cos(in[1]);
+    out[0] = in[0];
+    out[1] = std::acos(st * sp);
+    double a = std::acos(-((st * cp) / std::sqrt((st * st) * (cp * cp) + ct * ct)));
+    out[2] = std::copysign(std::fabs(a), ct);  // SIGN(a, cos t)
+}
+
+void Tree::init(int ni_, int nj_, int nk_, int ng_, double r_lo_cgs, double r_hi_cgs) {
+    ni = ni_; nj = nj_; nk = nk_; ng = ng_;
+    r_range[0] = r_lo_cgs; r_range[1] = r_hi_cgs;
+    nodes.clear();
+    Node root;
+    root.rng[0][0] = r_lo_cgs; root.rng[0][1] = r_hi_cgs;
+    root.rng[1][0] = (double)(kDpiF * 0.25f); root.rng[1][1] = (double)(kDpiF * 0.75f);
+    root.rng[2][0] = (double)(-kDpiF * 0.75f); root.rng[2][1] = (double)(kDpiF * 0.75f);
+    root.yin = 1;
+    nodes.push_back(root);
+    root.yin = 0;
+    nodes.push_back(root);
+    nleaf = nleaf_yin = 0;
+    leaf_node.clear();
+}
+
+// Split rule of YinYangTree_Divide (3_6/ModYinYangTree.f90:245-255): r by a
+// geometric rule with default-real exponents, theta/phi linearly.  N per axis is
+// general; N in {1,2} is the reference's own case.
+void Tree::divide(int node, const int N[3]) {
+    if (!nodes[node].leaf) return;
+    Node parent = nodes[node];
+    int first = (int)nodes.size();
+    for (int ir = 1; ir <= N[0]; ir++)
+        for (int it = 1; it <= N[1]; it++)
+            for (int ip = 1; ip <= N[2]; ip++) {
+                Node c;
+                c.yin = parent.yin;
+                float ea0 = ((float)(N[0] - ir) + 1.0f) / (float)N[0];
+                float ea1 = ((float)(N[0] - ir) + 0.0f) / (float)N[0];
+                float eb0 = ((float)(-ir) + 1.0f) / (float)N[0];
+                float eb1 = ((float)(-ir) + 0.0f) / (float)N[0];
+                c.rng[0][0] = std::pow(parent.rng[0][0], (double)ea0) / std::pow(parent.rng[0][1], (double)eb0);
+                c.rng[0][1] = std::pow(parent.rng[0][0], (double)ea1) / std::pow(parent.rng[0][1], (double)eb1);
+                const int n2[2] = {N[1], N[2]};
+                const int idx[2] = {it, ip};
+                for (int a = 0; a < 2; a++) {
+                    float fa0 = (float)(n2[a] - idx[a]) + 1.0f, fa1 = (float)(n2[a] - idx[a]) + 0.0f;
+                    float fb0 = (float)(-idx[a]) + 1.0f, fb1 = (float)(-idx[a]) + 0.0f;
+                    double lo = parent.rng[a + 1][0], hi = parent.rng[a + 1][1], n = (double)n2[a];
+                    c.rng[a + 1][0] = (lo * (double)fa0) / n - (hi * (double)fb0) / n;
+                    c.rng[a + 1][1] = (lo * (double)fa1) / n - (hi * (double)fb1) / n;
+                }
+                nodes.push_back(c);
+            }
+    Node &p = nodes[node];
+    p.first_child = first;
+    p.N[0] = N[0]; p.N[1] = N[1]; p.N[2] = N[2];
+    p.leaf = 0;
+}
+
+void Tree::divide_all(int nr, int nt, int np) {
+    const int N[3] = {nr, nt, np};
+    std::vector<int> stack = {1, 0};
+    std::vector<int> leaves;
+    // collect current leaves first (depth-first, children in order), then divide
+    while (!stack.empty()) {
+        int n = stack.back(); stack.pop_back();
+        if (nodes[n].leaf) { leaves.push_back(n); continue; }
+        int nc = nodes[n].N[0] * nodes[n].N[1] * nodes[n].N[2];
+        for (int c = nc - 1; c >= 0; c--) stack.push_back(nodes[n].first_child + c);
+    }
+    for (int n : leaves) divide(n, N);
+}
+
+// ModMath_IfLinesInterSect (3_6/ModMath.f90:5-16)
+static bool lines_intersect(const double a[2], const double b[2]) {
+    return (a[0] - b[0]) * (a[0] - b[1]) < 0.0 || (a[1] - b[0]) * (a[1] - b[1]) < 0.0 ||
+           (b[0] - a[0]) * (b[0] - a[1]) < 0.0 || (b[1] - a[0]) * (b[1] - a[1]) < 0.0;
+}
+
+// ModAMR_Divide_OneBranch (3_6/ModAMR.f90:65-99) over both panels
+void Tree::amr(const int flags[3], int global, const double rng_rsun[6]) {
+    const int N[3] = {flags[0] ? 2 : 1, flags[1] ? 2 : 1, flags[2] ? 2 : 1};
+    double rr[2] = {rng_rsun[0] * (double)kRsunF, rng_rsun[3] * (double)kRsunF};
+    double tr[2] = {rng_rsun[1], rng_rsun[4]};
+    double pr[2] = {rng_rsun[2], rng_rsun[5]};
+    std::vector<int> leaves;
+    std::vector<int> stack = {1, 0};
+    while (!stack.empty()) {
+        int n = stack.back(); stack.pop_back();
+        if (nodes[n].leaf) { leaves.push_back(n); continue; }
+        int nc = nodes[n].N[0] * nodes[n].N[1] * nodes[n].N[2];
+        for (int c = nc - 1; c >= 0; c--) stack.push_back(nodes[n].first_child + c);
+    }
+    for (int n : leaves) {
+        const Node &nd = nodes[n];
+        bool go;
+        if (global) go = lines_intersect(nd.rng[0], rr);
+        else go = nd.yin && lines_intersect(nd.rng[0], rr) && lines_intersect(nd.rng[1], tr) && lines_intersect(nd.rng[2], pr);
+        if (go) divide(n, N);
+    }
+}
+
+// YinYangTree_LabelNodes (3_6/ModYinYangTree.f90:402-420): depth-first, Yin then Yang
+void Tree::label() {
+    nleaf = 0;
+    leaf_node.clear();
+    for (int root = 0; root < 2; root++) {
+        std::vector<int> stack = {root};
+        while (!stack.empty()) {
+            int n = stack.back(); stack.pop_back();
+            if (nodes[n].leaf) {
+                nodes[n].leaf_id = ++nleaf;
+                leaf_node.push_back(n);
+                continue;
+            }
+            int nc = nodes[n].N[0] * nodes[n].N[1] * nodes[n].N[2];
+            for (int c = nc - 1; c >= 0; c--) stack.push_back(nodes[n].first_child + c);
+        }
+        if (root == 0) nleaf_yin = nleaf;
+    }
+}
+
+// YinYangTree_IfInsideNode (3_6/ModYinYangTree.f90:424-445); the point in the
+// other panel's frame is computed once by the caller (CoordConv is deterministic).
+bool Tree::inside(int node, const double rtp[3], const double rtp_other[3], int yin) const {
+    const Node &n = nodes[node];
+    const double *h = (yin == n.yin) ? rtp : rtp_other;
+    return (h[0] - n.rng[0][0]) * (h[0] - n.rng[0][1]) <= 0.0 &&
+           (h[1] - n.rng[1][0]) * (h[1] - n.rng[1][1]) <= 0.0 &&
+           (h[2] - n.rng[2][0]) * (h[2] - n.rng[2][1]) <= 0.0;
+}
+
+// YinYangTree_FindNode_OneBranch (3_6/ModYinYangTree.f90:468-505): the first
+// child, in child order, that contains the point and whose subtree resolves it.
+// For an N-ary node only children whose per-axis intervals contain the point
+// can pass the inside test, so they are enumerated per axis instead of scanning
+// all N0*N1*N2 children; the visiting order is unchanged.
+int Tree::find_branch(int node, const double rtp[3], const double rtp_other[3], int yin) const {
+    if (!inside(node, rtp, rtp_other, yin)) return -777;
+    const Node &n = nodes[node];
+    if (n.leaf) return n.leaf_id;
+    const int N0 = n.N[0], N1 = n.N[1], N2 = n.N[2];
+    const int nc = N0 * N1 * N2;
+    if (nc <= 8) {
+        for (int c = 0; c < nc; c++) {
+            int ch = n.first_child + c;
+            int r = -777;
+            if (inside(ch, rtp, rtp_other, yin)) r = find_branch(ch, rtp, rtp_other, yin);
+            if (r != -777) return r;
+        }
+        return -777;
+    }
+    const double *h = (yin == n.yin) ? rtp : rtp_other;  // children share the parent's panel
+    int cand[3][64];
+    int ncand[3] = {0, 0, 0};
+    const int Ns[3] = {N0, N1, N2};
+    for (int a = 0; a < 3; a++) {
+        for (int q = 0; q < Ns[a] && ncand[a] < 64; q++) {
+            int ci = a == 0 ? q * N1 * N2 : (a == 1 ? q * N2 : q);
+            const Node &c = nodes[n.first_child + ci];
+            if ((h[a] - c.rng[a][0]) * (h[a] - c.rng[a][1]) <= 0.0) cand[a][ncand[a]++] = q;
+        }
+    }
+    for (int x = 0; x < ncand[0]; x++)
+        for (int y = 0; y < ncand[1]; y++)
+            for (int z = 0; z < ncand[2]; z++) {
+                int ch = n.first_child + (cand[0][x] * N1 + cand[1][y]) * N2 + cand[2][z];
+                int r = -777;
+                if (inside(ch, rtp, rtp_other, yin)) r = find_branch(ch, rtp, rtp_other, yin);
+                if (r != -777) return r;
+            }
+    return -777;
+}
+
+// YinYangTree_FindNode (3_6/ModYinYangTree.f90:449-464): Yin root tested first.
+int Tree::find(const double rtp[3], int yin) const {
+    double other[3];
+    coordconv(rtp, other);
+    if (inside(0, rtp, other, yin)) return find_branch(0, rtp, other, yin);
+    return find_branch(1, rtp, other, yin);
+}
+
+static void init_geom(const Tree &t, const Node &n, BlockGeom &g) {
+    const int ni = t.ni, nj = t.nj, nk = t.nk, ng = t.ng;
+    g.yin = n.yin;
+    std::memcpy(g.rng, n.rng, sizeof(g.rng));
+    const int nn[3] = {ni, nj, nk};
+    std::vector<double> *x[3] = {&g.xi, &g.xj, &g.xk};
+    // ModBlock_InitGrid (3_6/ModBlock.f90:310-317)
+    for (int a = 0; a < 3; a++) {
+        x[a]->resize(nn[a] + 2 * ng);
+        for (int i = 1 - ng; i <= nn[a] + ng; i++)
+            (*x[a])[i + ng - 1] = (n.rng[a][0] * (double)((float)(nn[a] - i) + 0.5f) +
+                                   n.rng[a][1] * (double)((float)i - 0.5f)) / (double)nn[a];
+    }
+    g.dxi = (n.rng[0][1] - n.rng[0][0]) / (double)ni;
+    g.dxj = (n.rng[1][1] - n.rng[1][0]) / (double)nj;
+    g.dxk = (n.rng[2][1] - n.rng[2][0]) / (double)nk;
+    // YinYangTree_if_top_bottom (3_6/ModYinYangTree.f90:273-288)
+    g.top = g.xi[ni + 2 * ng - 1] > t.r_range[1];
+    g.bottom = g.xi[0] < t.r_range[0];
+}
+
+void Layout::build(int nthreads) {
+    if (tree.nleaf == 0) tree.label();
+    const int nb = tree.nleaf;
+    const int ni = tree.ni, nj = tree.nj, nk = tree.nk, ng = tree.ng;
+    const int NI = ni + 2 * ng, NJ = nj + 2 * ng, NK = nk + 2 * ng;
+    geom.assign(nb, BlockGeom());
+    for (int b = 0; b < nb; b++) init_geom(tree, tree.nodes[tree.leaf_node[b]], geom[b]);
+    targets.assign(nb, {});
+    if (keep_gcib) gcib.assign(nb, {});
+    std::vector<std::vector<GhostEntry>> per(nb);
+    if (nthreads < 1) nthreads = 1;
+
+    auto work = [&](int b) {
+        const BlockGeom &g = geom[b];
+        std::vector<int32_t> donor((size_t)NI * NJ * NK, -777);
+        // ModGC_FindGC_Single (3_6/ModGC.f90:37-50): i outer, k inner
+        for (int i = 1 - ng; i <= ni + ng; i++)
+            for (int j = 1 - ng; j <= nj + ng; j++)
+                for (int k = 1 - ng; k <= nk + ng; k++) {
+                    if (i < 1 || j < 1 || k < 1 || i > ni || j > nj || k > nk) {
+                        double rtp[3] = {g.xi[i + ng - 1], g.xj[j + ng - 1], g.xk[k + ng - 1]};
+                        donor[(size_t)(i + ng - 1) + (size_t)NI * ((j + ng - 1) + (size_t)NJ * (k + ng - 1))] = tree.find(rtp, g.yin);
+                    }
+                }
+        // ModGC_GetGC_Targets_single (3_6/ModGC.f90:53-147): donors in first-appearance order
+        auto &tg = targets[b];
+        std::vector<int> slot(nb + 1, -1);
+        for (int i = 1 - ng; i <= ni + ng; i++)
+            for (int j = 1 - ng; j <= nj + ng; j++)
+                for (int k = 1 - ng; k <= nk + ng; k++) {
+                    if (!(i < 1 || i > ni || j < 1 || j > nj || k < 1 || k > nk)) continue;
+                    size_t q = (size_t)(i + ng - 1) + (size_t)NI * ((j + ng - 1) + (size_t)NJ * (k + ng - 1));
+                    int d = donor[q];
+                    if (d == -777) continue;
+                    if (slot[d] < 0) { slot[d] = (int)tg.size(); tg.push_back({d, {}}); }
+                    tg[slot[d]].second.push_back((int32_t)q);
+                }
+        // interpolation base and weights (3_6/ModMath.f90:265-280) in the donor frame
+        auto &out = per[b];
+        for (auto &t : tg) {
+            const int d = t.first - 1;
+            const BlockGeom &s = geom[d];
+            const bool yy = s.yin != g.yin;
+            const double dxi = s.xi[ng + 1] - s.xi[ng], dxj = s.xj[ng + 1] - s.xj[ng], dxk = s.xk[ng + 1] - s.xk[ng];
+            const double base = (double)((float)(-ng) + 1.0f);
+            const double lim[3] = {(double)((float)ni + 0.0f), (double)((float)nj + 0.0f), (double)((float)nk + 0.0f)};
+            for (int32_t q : t.second) {
+                int gi = (int)(q % NI) - ng + 1, gj = (int)((q / NI) % NJ) - ng + 1, gk = (int)(q / ((size_t)NI * NJ)) - ng + 1;
+                double x[3] = {g.xi[gi + ng - 1], g.xj[gj + ng - 1], g.xk[gk + ng - 1]};
+                if (yy) { double o[3]; coordconv(x, o); x[0] = o[0]; x[1] = o[1]; x[2] = o[2]; }
+                double p[3] = {base + (x[0] - s.xi[0]) / dxi, base + (x[1] - s.xj[0]) / dxj, base + (x[2] - s.xk[0]) / dxk};
+                GhostEntry e;
+                e.recv_block = b; e.gi = gi; e.gj = gj; e.gk = gk; e.donor = d;
+                int32_t base_idx[3];
+                for (int a = 0; a < 3; a++) {
+                    double v = p[a] < lim[a] ? p[a] : lim[a];
+                    v = v > 1.0 ? v : 1.0;
+                    base_idx[a] = (int32_t)std::floor(v);
+                    e.w[a] = v - (double)base_idx[a];
+                }
+                e.i0 = base_idx[0]; e.j0 = base_idx[1]; e.k0 = base_idx[2];
+                e.yy = yy;
+                e.rot[0] = e.rot[1] = 0.0;
+                if (yy) {
+                    // ModYinYang_VecConv_1D (3_6/ModYinYang.f90:48-61) coefficients
+                    double o[3];
+                    coordconv(x, o);
+                    e.rot[0] = std::sin(x[2]) * std::sin(o[2]);
+                    e.rot[1] = std::cos(x[2]) / std::sin(o[1]);
+                }
+                out.push_back(e);
+            }
+        }
+        if (keep_gcib) {
+            for (auto &v : donor) if (v != -777 && v < 0) v = -777;
+            gcib[b] = std::move(donor);
+        }
+    };
+
+    std::vector<std::thread> th;
+    int nt = std::min(nthreads, nb);
+    for (int t = 0; t < nt; t++)
+        th.emplace_back([&, t]() { for (int b = t; b < nb; b += nt) work(b); });
+    for (auto &x : th) x.join();
+
+    size_t total = 0;
+    for (auto &v : per) total += v.size();
+    ghosts.clear();
+    ghosts.reserve(total);
+    for (auto &v : per) { ghosts.insert(ghosts.end(), v.begin(), v.end()); std::vector<GhostEntry>().swap(v); }
+}
+
+void polytrope(const PolytropeParams &p, double r, double out[8]) {
+    const double n = 1.5, G = 6.674e-8, M = 1.989e33, kB = 1.380649e-16, mp = 1.67262192369e-24;
+    const double zeta_lo = std::pow(p.contrast, 1.0 / n);
+    const double beta = (zeta_lo - 1.0) / (p.r_hi / p.r_lo - 1.0);
+    const double zeta = 1.0 + beta * (p.r_hi / r - 1.0);
+    const double pr = G * M / ((n + 1.0) * beta * p.r_hi);  // p_b / rho_b
+    const double rho = p.rho_top * std::pow(zeta, n);
+    const double pres = p.rho_top * pr * std::pow(zeta, n + 1.0);
+    out[0] = (n + 1.0) * pr * beta * p.r_hi / (r * r);  // = G M / r^2, hydrostatic with the profile
+    out[1] = rho;
+    out[2] = pres;
+    out[3] = p.mu * mp / kB * pr * zeta;
+    out[4] = 5.0 / 3.0;
+    out[5] = 5.0 / 3.0;
+    out[6] = 0.0;
+    out[7] = 1.0;
+}
+
+}  // namespace falcon

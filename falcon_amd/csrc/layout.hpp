@@ -1,0 +1,88 @@
+// Host-side layout: Yin-Yang block tree, per-block geometry, background
+// profiles and the ghost-fill tables that drive the device exchange kernel.
+//
+// Tree semantics follow 3_6/ModYinYangTree.f90 (roots :91-110, split rule
+// :239-255, depth-first leaf labelling Yin then Yang :294-310/:402-420, point
+// location :424-505).  Ghost tables follow 3_6/ModGC.f90:25-147 (donor per
+// ghost, targets in first-appearance order) and the interpolation base/weights
+// of 3_6/ModMath.f90:250-293 evaluated once on the host.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace falcon {
+
+constexpr int kNVar = 8;  // rho1, vr, vt, vp, br, bt, bp, s1 (3_6/ModBlock.f90:153-163 without psi)
+
+struct Node {
+    int yin = 1;
+    int leaf = 1;
+    int leaf_id = 0;       // 1-based once labelled
+    int first_child = -1;  // index into Tree::nodes
+    int N[3] = {1, 1, 1};  // children per axis (children ordered r-major, phi-minor)
+    double rng[3][2] = {};
+};
+
+struct Tree {
+    int ni = 0, nj = 0, nk = 0, ng = 2;
+    double r_range[2] = {0, 0};
+    std::vector<Node> nodes;   // [0] Yin root, [1] Yang root
+    int nleaf = 0, nleaf_yin = 0;
+    std::vector<int> leaf_node;  // leaf id - 1 -> node index
+
+    void init(int ni_, int nj_, int nk_, int ng_, double r_lo_cgs, double r_hi_cgs);
+    void divide(int node, const int N[3]);
+    void divide_all(int nr, int nt, int np);
+    void amr(const int flags[3], int global, const double rng_rsun[6]);
+    void label();
+    // Leaf id (1-based) of the node containing rtp given in panel `yin`, or -777.
+    int find(const double rtp[3], int yin) const;
+
+  private:
+    int find_branch(int node, const double rtp[3], const double rtp_other[3], int yin) const;
+    bool inside(int node, const double rtp[3], const double rtp_other[3], int yin) const;
+};
+
+struct BlockGeom {
+    int yin = 1, top = 0, bottom = 0;
+    double rng[3][2] = {};
+    double dxi = 0, dxj = 0, dxk = 0;
+    std::vector<double> xi, xj, xk;  // padded centres, [0] <-> index -ng+1
+};
+
+// One ghost cell of one receiving block, as it is filled from its donor.
+struct GhostEntry {
+    int32_t recv_block;   // 0-based
+    int32_t gi, gj, gk;   // ghost indices in the receiving block (Fortran, -ng+1..n+ng)
+    int32_t donor;        // 0-based
+    int32_t i0, j0, k0;   // donor base cell (Fortran indices, clamped to [1,n])
+    double w[3];          // trilinear weights
+    int32_t yy;           // 1 if donor and receiver are in different panels
+    double rot[2];        // a = sin(p)sin(p'), b = cos(p)/sin(t') (donor-frame point)
+};
+
+struct Layout {
+    Tree tree;
+    std::vector<BlockGeom> geom;
+    // per block: ordered list of (donor leaf id, [ghost ordinal...]) = GC_targets
+    std::vector<std::vector<std::pair<int32_t, std::vector<int32_t>>>> targets;
+    std::vector<GhostEntry> ghosts;        // grouped by receiving block, then target order
+    std::vector<std::vector<int32_t>> gcib; // optional: GC_iBlocks_III per block
+    bool keep_gcib = false;
+
+    int nblocks() const { return tree.nleaf; }
+    void build(int nthreads);
+};
+
+// Yin<->Yang point map (3_6/ModYinYang.f90:8-17); same glibc calls as the reference.
+void coordconv(const double in[3], double out[3]);
+
+// n=1.5 spherical polytrope background at radius r (cm): writes g, rho0, p0,
+// T0, gamma1, gamma3, heat, xi.  rho(r_lo)/rho(r_hi) = contrast.
+struct PolytropeParams {
+    double r_lo, r_hi, contrast = 20.0, rho_top = 1.0e-2, mu = 0.6;
+};
+void polytrope(const PolytropeParams &p, double r, double out[8]);
+
+}  // namespace falcon

@@ -1,0 +1,113 @@
+/* falcon_hip.h - C-ABI of the MI355X-native Falcon hot path (libfalcon_hip.so).
+ *
+ * Plain C types only.  Every function returns 0 on success and -1 on failure;
+ * falcon_last_error() then holds the message (the Fortran shims map != 0 to
+ * `stop 1`, the reference's only failure mode).  A handle owns one Yin-Yang
+ * block tree with every block resident in HBM; host arrays are stale until
+ * falcon_get_state().  One HIP stream per handle; a handle is not re-entrant.
+ *
+ * Host arrays use the reference's Fortran layout: a padded block array is
+ * (-ng+1:ni+ng, -ng+1:nj+ng, -ng+1:nk+ng, 8) column-major (r fastest), with the
+ * variables rho1, vr, vt, vp, br, bt, bp, s1 (3_6/ModBlock.f90:153-163, psi
+ * dropped: no term of the reference writes or reads it when DivB_option=0).
+ * Block ids are 0-based here; the reference's leaf labels are id+1.
+ */
+#ifndef FALCON_HIP_H
+#define FALCON_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct falcon_handle *falcon_handle_t;
+
+/* ---------------------------------------------------------------- tree
+ * Replaces YinYangTree_InitTree (3_6/ModYinYangTree.f90:91-110) plus
+ * ModAMR_set_grid (3_6/ModAMR.f90:13-30) for drivers that build the tree in
+ * the library.  r range in cm (3_6/ModReadParameters.f90:71). */
+int falcon_create(falcon_handle_t *out, int ni, int nj, int nk, int ng, double r_lo_cgs, double r_hi_cgs);
+int falcon_destroy(falcon_handle_t h);
+/* Divide every leaf into nr x nt x np children with the split rule of
+ * YinYangTree_Divide (3_6/ModYinYangTree.f90:210-269); {1,2} per axis is
+ * YinYangTree_DivideAll (:129-136), larger N generalises the same formulas. */
+int falcon_tree_divide_all(falcon_handle_t h, int nr, int nt, int np);
+/* One AMR pass, ModAMR_Divide_OneBranch (3_6/ModAMR.f90:65-99): flags =
+ * rtp_if_divide, rng_rsun = rtp_range_Rsun(3,2) column-major. */
+int falcon_tree_amr(falcon_handle_t h, const int flags[3], int if_global, const double rng_rsun[6]);
+/* Drop-in path: take the Fortran driver's own tree.  Nodes in depth-first
+ * pre-order (Yin root first, then its subtree, then the Yang root); parent[n] = -1
+ * for roots; rng6 = rtp_range(3,2) column-major per node. */
+int falcon_tree_set_nodes(falcon_handle_t h, int n_nodes, const int *parent, const int *is_yin, const double *rng6);
+/* Leaf count after labelling (YinYangTree_SetAll, 3_6/ModYinYangTree.f90:294-344). */
+int falcon_nblocks(falcon_handle_t h, int *nblocks, int *nleaf_yin);
+/* xijk_range(3,2), flags = (if_yin, if_top, if_bottom), padded cell centres. */
+int falcon_block_info(falcon_handle_t h, int ib, double *rng6, int *flags3, double *xi, double *xj, double *xk);
+
+/* ---------------------------------------------------------------- tables
+ * ModCommunication_SetGCAll (3_6/ModCommunication.f90:22-64): donor search,
+ * GC targets, interpolation bases/weights and Yin-Yang rotations, built on the
+ * host and uploaded once.  Also allocates every block's state in HBM. */
+int falcon_set_gc_all(falcon_handle_t h);
+/* Host half of falcon_set_gc_all (no device calls): tables only. */
+int falcon_build_tables(falcon_handle_t h);
+/* GC_iBlocks_III of block ib (padded, Fortran order; -777 = outside the domain,
+ * interior cells -777 as after ModGC_FindGC_Single, 3_6/ModGC.f90:34). */
+int falcon_gc_iblocks(falcon_handle_t h, int ib, int32_t *out);
+int falcon_gc_ntargets(falcon_handle_t h, int ib, int *ntargets);
+/* GC_targets(it) of block ib (3_6/ModGC.f90:111-143): donor leaf label (1-based),
+ * nGC, ijk_list(nGC,3) column-major.  ijk may be NULL to query ngc. */
+int falcon_gc_target(falcon_handle_t h, int ib, int it, int *donor, int *ngc, int32_t *ijk);
+/* All ghost entries in device order: n on query (arrays NULL), else per entry
+ * recv block, donor block, ghost ijk[3], base ijk[3], weights[3], yy flag, rot[2]. */
+int falcon_gc_entries(falcon_handle_t h, int64_t *n, int32_t *recv, int32_t *donor, int32_t *gijk, int32_t *bijk,
+                      double *w, int32_t *yy, double *rot);
+
+/* ---------------------------------------------------------------- data
+ * Background profiles per block, replacing ModBlock_InitSSM's calls to
+ * ModStratification_new_get_vars (3_6/ModBlock.f90:232-249): bg is 8 x NI
+ * column-major: g, rho0, p0, T0, gamma1, gamma3, heat (diffusion+cooling), xi_rsst. */
+int falcon_set_background(falcon_handle_t h, int ib, const double *bg);
+/* n=1.5 polytrope for every block: rho(r_lo)/rho(r_hi) = contrast. */
+int falcon_background_polytrope(falcon_handle_t h, double contrast, double rho_top);
+int falcon_set_state(falcon_handle_t h, int ib, const double *prim);
+int falcon_get_state(falcon_handle_t h, int ib, double *prim);
+/* Seeded synthetic initial state on the interior: rho1 = v = 0, s1 = s_amp*c_p*U(-1,1),
+ * B = curl of a seeded smooth vector potential scaled to max|B|/sqrt(rho0) = b_ratio*c_s.
+ * Ghosts are then filled by two exchanges (3_6/test1.f90:143-144). */
+int falcon_init_synthetic(falcon_handle_t h, uint64_t seed, double s_amp, double b_ratio);
+
+/* ---------------------------------------------------------------- stepping
+ * precision: 0 = FP64 everywhere; 1 = FP32 stencil arithmetic on the deviation
+ * state with FP64 background, stage accumulation, ghost weights and dt. */
+int falcon_set_precision(falcon_handle_t h, int mode);
+/* ModCommunication_SendRecvGC_new(Tree, if_rk) (3_6/ModCommunication.f90:655-795):
+ * which = 0 fills the state's ghosts, 1 the current stage buffer's. */
+int falcon_sendrecv_gc(falcon_handle_t h, int which);
+/* ModAdvance_rk4(Tree, .true., dt) (3_6/ModAdvance.f90:18-112): one step, dt out. */
+int falcon_advance_rk4(falcon_handle_t h, double cfl, double *dt_out);
+/* nsteps steps without host synchronisation; use_graph captures two steps in a
+ * hipGraph and replays it.  dt_last = dt of the last step. */
+int falcon_advance(falcon_handle_t h, int nsteps, double cfl, int use_graph, double *dt_last);
+/* ModWaveSpeed_Dynamo + ModTimeStep_Dynamo (3_6/ModWaveSpeed.f90, ModTimeStep.f90). */
+int falcon_compute_dt(falcon_handle_t h, double cfl, double *dt);
+/* Single operators for parity: R of ModEquation_Dynamo after ModBoundary on the
+ * state (3_6/ModEquation.f90:24-37), and of ModDiffusion_Aritificial_1 (h=2).
+ * Out: per block (ni,nj,nk,8) Fortran order, blocks consecutive. */
+int falcon_rhs(falcon_handle_t h, double *r_out);
+int falcon_diffusion_rhs(falcon_handle_t h, double *r_out);
+/* Average kernel durations (ms, HIP events on the handle's stream) over nsteps
+ * steps: out[0] dt, [1] boundary, [2] stage, [3] exchange (stage buffers),
+ * [4] exchange (state), [5] diffusion, [6] whole step. */
+int falcon_profile_step(falcon_handle_t h, int nsteps, double cfl, double *out_ms);
+/* Bytes of device memory held by the handle. */
+int falcon_device_bytes(falcon_handle_t h, int64_t *bytes);
+int falcon_synchronize(falcon_handle_t h);
+
+/* ---------------------------------------------------------------- errors */
+int falcon_last_error(char *buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

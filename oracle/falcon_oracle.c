@@ -903,3 +903,14 @@ double orc_step(orc_ctx *c, double cfl) {
     orc_exchange(c, 0);
     return dt;
 }
+
+/* Yin-Yang point map and the VecConv coefficients (sin p sin p', cos p / sin t')
+ * for one donor-frame point, as used by vecconv() above. */
+void orc_coordconv(const double in[3], double out[3]) { coordconv(in, out); }
+void orc_vecconv_coeffs(const double x[3], double out[2]) {
+    double o[3];
+    coordconv(x, o);
+    double sp = sin(x[2]), spo = sin(o[2]), cp = cos(x[2]), sto = sin(o[1]);
+    out[0] = sp * spo;
+    out[1] = cp / sto;
+}

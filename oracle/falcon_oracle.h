@@ -56,6 +56,8 @@ void orc_diffusion(orc_ctx *c, int ib, double *r_out); /* ModDiffusion_Aritifici
 double orc_dt(orc_ctx *c, double cfl);               /* ModWaveSpeed + ModTimeStep */
 double orc_step(orc_ctx *c, double cfl);             /* one ModAdvance_rk4; returns dt */
 void orc_set_threads(int n);
+void orc_coordconv(const double in[3], double out[3]);
+void orc_vecconv_coeffs(const double x[3], double out[2]);
 
 /* ---- heat conduction oracle (heat_oracle.c) */
 #include "heat_oracle.h"
