@@ -1,0 +1,144 @@
+"""Benchmark: RK4 cell-updates/s on one MI355X, FP32-deviation vs FP64.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+A "step" is one full RK4 step (4 RHS stages, artificial diffusion, 5 ghost
+fills) over every block of the workload; all blocks are resident in HBM when
+the timed region starts.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+from falcon_amd import Shell, configs  # noqa: E402
+
+WORKLOADS = {
+    "shell_128x256x768_b16": configs.SHELL_128x256x768_B16,
+    "shell_64x128x384_b16": configs.SHELL_64x128x384_B16,
+    "shell_24x48x144_b8": configs.SHELL_24x48x144_B8,
+}
+
+# algorithmic bytes per interior cell per stage-kernel launch (compulsory
+# interior traffic: read the stage input, read U^n, write the output)
+STAGE_BYTES = {"fp64": [192, 192, 192, 192], "fp32": [128, 128, 128, 160]}
+HBM_PEAK_GBS = 8000.0
+
+
+def run_precision(shell, prec, steps, warmup, cfl):
+    shell.set_precision(prec)
+    shell.init_synthetic(seed=12345, s_amp=1e-4, b_ratio=1e-3)
+    shell.advance(warmup, cfl, graph=True)
+    shell.synchronize()
+    t0 = time.perf_counter()
+    shell.advance(steps, cfl, graph=True)
+    shell.synchronize()
+    el = time.perf_counter() - t0
+    prof = shell.profile_step(3, cfl)
+    cells = shell.interior_cells
+    stage_bytes = float(np.mean(STAGE_BYTES[prec])) * cells
+    achieved = stage_bytes / (prof["stage"] * 1e-3) / 1e9
+    return {
+        "cell_updates_per_s": cells * steps / el,
+        "ms_per_step": 1e3 * el / steps,
+        "kernel_ms": prof,
+        "stage_roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--workload", default="shell_128x256x768_b16", choices=sorted(WORKLOADS))
+    ap.add_argument("--cfl", type=float, default=0.5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
+        sys.exit(2)
+
+    lay = WORKLOADS[args.workload]
+    t0 = time.perf_counter()
+    shell = Shell(lay)
+    shell.background_polytrope(20.0)
+    shell.set_gc_all()
+    setup_s = time.perf_counter() - t0
+    print(f"# setup {args.workload}: {shell.nblocks} blocks, {shell.interior_cells} cells, {setup_s:.1f} s",
+          file=sys.stderr, flush=True)
+
+    res = {}
+    for prec in ("fp64", "fp32"):
+        res[prec] = run_precision(shell, prec, args.steps, args.warmup, args.cfl)
+        print(f"# {prec}: {res[prec]['cell_updates_per_s']:.4e} cell-updates/s, "
+              f"{res[prec]['ms_per_step']:.3f} ms/step, kernels {res[prec]['kernel_ms']}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = run_cpu_baseline()
+
+    head = res["fp32"]
+    out = {
+        "metric": "RK4 cell-updates/s (FP32-deviation path)",
+        "value": head["cell_updates_per_s"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 stencils / f64 state",
+        "data": "synthetic (seeded polytrope shell, entropy noise, curl-A field)",
+        "config": {"workload": args.workload, "blocks": shell.nblocks, "block": [lay["ni"], lay["nj"], lay["nk"]],
+                   "interior_cells": shell.interior_cells, "cfl": args.cfl},
+        "fp64": {"value": res["fp64"]["cell_updates_per_s"], "ms_per_step": res["fp64"]["ms_per_step"],
+                 "kernel_ms": res["fp64"]["kernel_ms"], "stage_roofline": res["fp64"]["stage_roofline"]},
+        "fp32_over_fp64": head["cell_updates_per_s"] / res["fp64"]["cell_updates_per_s"],
+        "kernel_ms": head["kernel_ms"],
+        "roofline": head["stage_roofline"],
+        "setup_s": setup_s,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def run_cpu_baseline():
+    """Oracle FP64 steps of a 192-block 16^3 shell on the host cores (bounded sample)."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import pyoracle
+    lay = configs.cubic((32, 64, 192), 16)
+    threads = min(os.cpu_count() or 1, 16)
+    pyoracle.set_threads(threads)
+    o = pyoracle.Oracle(lay)
+    s = Shell(lay)
+    s.background_polytrope(20.0)
+    s.set_gc_all()
+    s.init_synthetic(seed=12345)
+    for ib in range(o.nblocks):
+        o.set_background(ib, s.get_background(ib))
+        o.set_state(ib, s.get_state(ib))
+    o.step(0.5)  # warm caches
+    n, t0 = 0, time.perf_counter()
+    while True:
+        o.step(0.5)
+        n += 1
+        if time.perf_counter() - t0 > 10.0 or n >= 20:
+            break
+    el = time.perf_counter() - t0
+    cells = o.nblocks * 16 ** 3
+    return {"value": cells * n / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"oracle FP64 RK4, 32x64x192-per-panel shell in 16^3 blocks ({o.nblocks} blocks), {n} steps, {el:.1f} s"}
+
+
+if __name__ == "__main__":
+    main()

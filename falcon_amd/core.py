@@ -47,6 +47,7 @@ def load():
         "falcon_gc_entries": [vp, i64p, ip, ip, ip, ip, dp, ip, dp],
         "falcon_set_background": [vp, C.c_int, dp],
         "falcon_background_polytrope": [vp, C.c_double, C.c_double],
+        "falcon_get_background": [vp, C.c_int, dp],
         "falcon_set_state": [vp, C.c_int, dp],
         "falcon_get_state": [vp, C.c_int, dp],
         "falcon_init_synthetic": [vp, C.c_uint64, C.c_double, C.c_double],
@@ -172,6 +173,11 @@ class Shell:
     def set_background(self, ib, bg):
         bg = np.ascontiguousarray(bg, dtype=np.float64)
         _check(self.L.falcon_set_background(self.h, ib, _p(bg)))
+
+    def get_background(self, ib):
+        out = np.zeros((8, self.pad[0]))
+        _check(self.L.falcon_get_background(self.h, ib, _p(out)))
+        return out
 
     def background_polytrope(self, contrast=20.0, rho_top=1.0e-2):
         _check(self.L.falcon_background_polytrope(self.h, contrast, rho_top))

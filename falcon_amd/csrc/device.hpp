@@ -46,20 +46,24 @@ struct DevBoundary {
 
 // launchers (kernels.hip)
 template <typename T, typename TIN, typename TOUT>
-void launch_stage(const Dims &d, const DevTables &t, const TIN *in, const double *U, TOUT *out,
+void launch_stage(const Dims &d, const DevTables &t, const TIN *in, const double *U, TOUT *out, float *shadow,
                   const double *dtp, double div, hipStream_t s);
-template <typename T>
-void launch_diffusion(const Dims &d, const DevTables &t, const double *U, double *out, const double *dtp, hipStream_t s);
+template <typename T, typename TIN>
+void launch_diffusion(const Dims &d, const DevTables &t, const TIN *in, const double *U, double *out, float *shadow,
+                      double *partial, const double *dtp, hipStream_t s);
 template <typename TS>
 void launch_exchange(const Dims &d, const DevGhosts &g, TS *state, hipStream_t s);
 template <typename TS>
 void launch_boundary(const Dims &d, const DevBoundary &b, TS *state, hipStream_t s);
-void launch_dt(const Dims &d, const DevTables &t, const double *U, double *partial, double *dtp, double cfl, hipStream_t s);
+template <typename TS>
+void launch_copy_radial(const Dims &d, const DevBoundary &b, const TS *src, TS *dst, hipStream_t s);
+void launch_dt_partial(const Dims &d, const DevTables &t, const double *U, double *partial, hipStream_t s);
+void launch_dt_final(const Dims &d, const double *partial, double *dtp, double cfl, hipStream_t s);
+void launch_to_f32(const double *src, float *dst, long long n, hipStream_t s);
 // single-operator entry points used by parity tests: R only
 template <typename T, typename TIN>
 void launch_rhs_only(const Dims &d, const DevTables &t, const TIN *in, double *R, hipStream_t s);
 template <typename T>
 void launch_diffusion_only(const Dims &d, const DevTables &t, const double *U, double *R, hipStream_t s);
-void launch_copy(const double *src, double *dst, long long n, hipStream_t s);
 
 }  // namespace falcon

@@ -86,8 +86,15 @@ struct falcon_handle {
     DevBoundary bnd{};
     // device memory
     DevBuf meta, radial, angular, ghost_dst, ghost_src, ghost_w, ghost_rot, bnd_list;
-    DevBuf U[2], S64[2], S32[2], dtp, partial, rbuf;
+    DevBuf U[2], F32[2], S64[2], S32[2], dtp, partial, rbuf;
     int cur = 0;             // index of the current state buffer in U[]
+    // validity of derived device data
+    bool partials_valid = false;   // partial[] holds min(h/v_wave) of U[cur]
+    bool shadow_valid = false;     // F32[cur] (with ghosts) is the FP32 copy of U[cur]
+    bool u_ghosts_valid = true;    // U[cur]'s ghost cells are filled
+    // profiling: (kind, start event, end event) per launch when enabled
+    std::vector<hipEvent_t> *prof_ev = nullptr;
+    std::vector<int> *prof_kind = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_cur = -1, graph_prec = -1;
     double graph_cfl = -1;
@@ -95,7 +102,7 @@ struct falcon_handle {
     ~falcon_handle() {
         if (graph) (void)hipGraphExecDestroy(graph);
         DevBuf *all[] = {&meta, &radial, &angular, &ghost_dst, &ghost_src, &ghost_w, &ghost_rot, &bnd_list,
-                         &U[0], &U[1], &S64[0], &S64[1], &S32[0], &S32[1], &dtp, &partial, &rbuf};
+                         &U[0], &U[1], &F32[0], &F32[1], &S64[0], &S64[1], &S32[0], &S32[1], &dtp, &partial, &rbuf};
         for (DevBuf *b : all) b->free();
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -257,7 +264,7 @@ void alloc_state(falcon_handle *h) {
         }
     }
     if (!h->dtp.p) { h->dtp.alloc(sizeof(double)); HIPCHECK(hipMemset(h->dtp.p, 0, sizeof(double))); }
-    if (!h->partial.p) h->partial.alloc(sizeof(double) * std::max(1, h->dims.nblocks));
+    if (!h->partial.p) h->partial.alloc(sizeof(double) * std::max(1, h->dims.nblocks * h->dims.tiles_j * h->dims.tiles_k));
 }
 
 void alloc_stage(falcon_handle *h) {
@@ -271,69 +278,116 @@ void alloc_stage(falcon_handle *h) {
             h->S32[q].alloc(n * sizeof(float));
             HIPCHECK(hipMemset(h->S32[q].p, 0, n * sizeof(float)));
         }
+        if (h->precision == 1 && !h->F32[q].p) {
+            h->F32[q].alloc(n * sizeof(float));
+            HIPCHECK(hipMemset(h->F32[q].p, 0, n * sizeof(float)));
+        }
     }
 }
 
-// One RK4 step: ModAdvance_rk4 (3_6/ModAdvance.f90:18-112) with stage
-// buffers ping-ponged and the diffusion written to the second state buffer.
+enum Kind { K_DT, K_BC, K_STAGE, K_XSTAGE, K_XSTATE, K_DIFF, K_OTHER, K_NKIND };
+
+template <typename F>
+void timed(falcon_handle *h, int kind, F &&f) {
+    if (!h->prof_ev) { f(); return; }
+    hipEvent_t a, b;
+    HIPCHECK(hipEventCreate(&a));
+    HIPCHECK(hipEventCreate(&b));
+    HIPCHECK(hipEventRecord(a, h->stream));
+    f();
+    HIPCHECK(hipEventRecord(b, h->stream));
+    h->prof_ev->push_back(a);
+    h->prof_ev->push_back(b);
+    h->prof_kind->push_back(kind);
+}
+
+// One RK4 step: ModAdvance_rk4 (3_6/ModAdvance.f90:18-112).  Stage buffers are
+// ping-ponged; the diffusion writes the other state buffer, which becomes U^{n+1}.
 // The diffusion reads the state's radial ghosts as stage 1's boundary call left
-// them (the reference applies boundaries only to each stage's input).
-void enqueue_step(falcon_handle *h, double cfl, int cur, hipEvent_t *ev = nullptr) {
+// them (the reference applies boundaries only to each stage's input), and those
+// rows are then copied into U^{n+1} so its ghosts hold what primitive_IV holds.
+// dt comes from the partial minima the previous diffusion left (or k_dt_tile).
+// FP32 mode: the stencil input of every stage and of the diffusion is FP32 (the
+// stage buffers and an FP32 shadow of the state); U^n, the stage accumulation
+// and U^{n+1} stay FP64; every ghost fill runs on FP32 data with FP64 weights.
+void enqueue_step(falcon_handle *h, double cfl, int cur) {
     const Dims &d = h->dims;
     const DevTables &t = h->dt;
     hipStream_t s = h->stream;
     double *U = h->U[cur].as<double>(), *Un = h->U[1 - cur].as<double>();
-    double *dtp = h->dtp.as<double>();
-    auto mark = [&](int i) { if (ev) HIPCHECK(hipEventRecord(ev[i], s)); };
-    mark(0);
-    launch_dt(d, t, U, h->partial.as<double>(), dtp, cfl, s);
-    mark(1);
+    double *dtp = h->dtp.as<double>(), *part = h->partial.as<double>();
+    timed(h, K_DT, [&] { launch_dt_final(d, part, dtp, cfl, s); });
     if (h->precision == 0) {
         double *S0 = h->S64[0].as<double>(), *S1 = h->S64[1].as<double>();
         const double *in[4] = {U, S0, S1, S0};
         double *out[4] = {S0, S1, S0, U};
         for (int k = 1; k <= 4; k++) {
-            launch_boundary<double>(d, h->bnd, const_cast<double *>(in[k - 1]), s);
-            mark(1 + 4 * (k - 1) + 1);
-            launch_stage<double, double, double>(d, t, in[k - 1], U, out[k - 1], dtp, (double)(5.0f - (float)k), s);
-            mark(1 + 4 * (k - 1) + 2);
-            launch_exchange<double>(d, h->gh, out[k - 1], s);
-            mark(1 + 4 * (k - 1) + 3);
-            mark(1 + 4 * (k - 1) + 4);
+            timed(h, K_BC, [&] { launch_boundary<double>(d, h->bnd, const_cast<double *>(in[k - 1]), s); });
+            timed(h, K_STAGE, [&] {
+                launch_stage<double, double, double>(d, t, in[k - 1], U, out[k - 1], nullptr, dtp,
+                                                     (double)(5.0f - (float)k), s);
+            });
+            timed(h, k < 4 ? K_XSTAGE : K_XSTATE, [&] { launch_exchange<double>(d, h->gh, out[k - 1], s); });
         }
-        launch_diffusion<double>(d, t, U, Un, dtp, s);
+        timed(h, K_DIFF, [&] { launch_diffusion<double, double>(d, t, U, U, Un, nullptr, part, dtp, s); });
+        timed(h, K_OTHER, [&] { launch_copy_radial<double>(d, h->bnd, U, Un, s); });
+        timed(h, K_XSTATE, [&] { launch_exchange<double>(d, h->gh, Un, s); });
     } else {
         float *S0 = h->S32[0].as<float>(), *S1 = h->S32[1].as<float>();
-        // stage 1: FP64 state in, FP32 arithmetic, FP32 stage buffer out
-        launch_boundary<double>(d, h->bnd, U, s);
-        mark(2);
-        launch_stage<float, double, float>(d, t, U, U, S0, dtp, 4.0, s);
-        mark(3);
-        launch_exchange<float>(d, h->gh, S0, s);
-        mark(4); mark(5);
-        launch_boundary<float>(d, h->bnd, S0, s);
-        mark(6);
-        launch_stage<float, float, float>(d, t, S0, U, S1, dtp, 3.0, s);
-        mark(7);
-        launch_exchange<float>(d, h->gh, S1, s);
-        mark(8); mark(9);
-        launch_boundary<float>(d, h->bnd, S1, s);
-        mark(10);
-        launch_stage<float, float, float>(d, t, S1, U, S0, dtp, 2.0, s);
-        mark(11);
-        launch_exchange<float>(d, h->gh, S0, s);
-        mark(12); mark(13);
-        launch_boundary<float>(d, h->bnd, S0, s);
-        mark(14);
-        launch_stage<float, float, double>(d, t, S0, U, U, dtp, 1.0, s);
-        mark(15);
-        launch_exchange<double>(d, h->gh, U, s);
-        mark(16); mark(17);
-        launch_diffusion<float>(d, t, U, Un, dtp, s);
+        float *F = h->F32[cur].as<float>(), *Fn = h->F32[1 - cur].as<float>();
+        const float *in[4] = {F, S0, S1, S0};
+        float *out[3] = {S0, S1, S0};
+        for (int k = 1; k <= 4; k++) {
+            timed(h, K_BC, [&] { launch_boundary<float>(d, h->bnd, const_cast<float *>(in[k - 1]), s); });
+            const double div = (double)(5.0f - (float)k);
+            if (k < 4) {
+                timed(h, K_STAGE, [&] { launch_stage<float, float, float>(d, t, in[k - 1], U, out[k - 1], nullptr, dtp, div, s); });
+                timed(h, K_XSTAGE, [&] { launch_exchange<float>(d, h->gh, out[k - 1], s); });
+            } else {
+                timed(h, K_STAGE, [&] { launch_stage<float, float, double>(d, t, in[3], U, U, F, dtp, div, s); });
+                timed(h, K_XSTATE, [&] { launch_exchange<float>(d, h->gh, F, s); });
+            }
+        }
+        timed(h, K_DIFF, [&] { launch_diffusion<float, float>(d, t, F, U, Un, Fn, part, dtp, s); });
+        timed(h, K_XSTATE, [&] { launch_exchange<float>(d, h->gh, Fn, s); });
     }
-    mark(18);
-    launch_exchange<double>(d, h->gh, Un, s);
-    mark(19);
+}
+
+// Bring derived device data in line with U[cur] before stepping.
+void prepare_steps(falcon_handle *h) {
+    const Dims &d = h->dims;
+    const long long n = (long long)d.nblocks * 8 * d.P;
+    if (h->precision == 1 && !h->shadow_valid) {
+        if (!h->u_ghosts_valid) {
+            launch_exchange<double>(d, h->gh, h->U[h->cur].as<double>(), h->stream);
+            h->u_ghosts_valid = true;
+        }
+        launch_to_f32(h->U[h->cur].as<double>(), h->F32[h->cur].as<float>(), n, h->stream);
+        h->shadow_valid = true;
+    }
+    if (h->precision == 0 && !h->u_ghosts_valid) {
+        launch_exchange<double>(d, h->gh, h->U[h->cur].as<double>(), h->stream);
+        h->u_ghosts_valid = true;
+    }
+    if (!h->partials_valid) {
+        launch_dt_partial(d, h->dt, h->U[h->cur].as<double>(), h->partial.as<double>(), h->stream);
+        h->partials_valid = true;
+    }
+}
+
+void after_steps(falcon_handle *h) {
+    if (h->precision == 1) { h->u_ghosts_valid = false; h->shadow_valid = true; }
+    else { h->u_ghosts_valid = true; h->shadow_valid = false; }
+    h->partials_valid = true;
+}
+
+// After FP32-mode steps the FP64 state's ghosts are refilled from its interior.
+void refresh_u_ghosts(falcon_handle *h) {
+    if (h->u_ghosts_valid) return;
+    double *U = h->U[h->cur].as<double>();
+    launch_exchange<double>(h->dims, h->gh, U, h->stream);
+    launch_boundary<double>(h->dims, h->bnd, U, h->stream);
+    h->u_ghosts_valid = true;
 }
 
 void check_launch() { HIPCHECK(hipGetLastError()); }
@@ -563,6 +617,16 @@ int falcon_set_background(falcon_handle_t h, int ib, const double *bg) {
     });
 }
 
+int falcon_get_background(falcon_handle_t h, int ib, double *bg) {
+    return guard([&] {
+        h->need_label();
+        h->check_block(ib);
+        if (!h->bg_set[ib]) throw Error("background not set for block " + std::to_string(ib));
+        const int NI = h->NI();
+        std::memcpy(bg, h->bg.data() + (size_t)ib * 8 * NI, sizeof(double) * 8 * NI);
+    });
+}
+
 int falcon_background_polytrope(falcon_handle_t h, double contrast, double rho_top) {
     return guard([&] {
         h->need_label();
@@ -597,8 +661,11 @@ int falcon_set_state(falcon_handle_t h, int ib, const double *prim) {
         const size_t P = (size_t)h->dims.P;
         std::vector<double> tmp(8 * P);
         fortran_to_device(prim, tmp.data(), h->NI(), h->NJ(), h->NK());
+        HIPCHECK(hipStreamSynchronize(h->stream));
         HIPCHECK(hipMemcpy(h->U[h->cur].as<double>() + (size_t)ib * 8 * P, tmp.data(), sizeof(double) * 8 * P,
                            hipMemcpyHostToDevice));
+        h->partials_valid = false;
+        h->shadow_valid = false;
     });
 }
 
@@ -608,6 +675,7 @@ int falcon_get_state(falcon_handle_t h, int ib, double *prim) {
         h->check_block(ib);
         const size_t P = (size_t)h->dims.P;
         std::vector<double> tmp(8 * P);
+        if (h->dt.meta) refresh_u_ghosts(h);
         HIPCHECK(hipStreamSynchronize(h->stream));
         HIPCHECK(hipMemcpy(tmp.data(), h->U[h->cur].as<double>() + (size_t)ib * 8 * P, sizeof(double) * 8 * P,
                            hipMemcpyDeviceToHost));
@@ -635,6 +703,9 @@ int falcon_init_synthetic(falcon_handle_t h, uint64_t seed, double s_amp, double
         HIPCHECK(hipMemcpy(h->U[h->cur].p, dev.data(), sizeof(double) * dev.size(), hipMemcpyHostToDevice));
         launch_exchange<double>(h->dims, h->gh, h->U[h->cur].as<double>(), h->stream);
         launch_exchange<double>(h->dims, h->gh, h->U[h->cur].as<double>(), h->stream);
+        h->partials_valid = false;
+        h->shadow_valid = false;
+        h->u_ghosts_valid = true;
         check_launch();
         HIPCHECK(hipStreamSynchronize(h->stream));
     });
@@ -643,6 +714,10 @@ int falcon_init_synthetic(falcon_handle_t h, uint64_t seed, double s_amp, double
 int falcon_set_precision(falcon_handle_t h, int mode) {
     return guard([&] {
         if (mode != 0 && mode != 1) throw Error("precision must be 0 (FP64) or 1 (FP32 deviation)");
+        if (mode != h->precision && h->dt.meta) {
+            refresh_u_ghosts(h);
+            h->shadow_valid = false;
+        }
         h->precision = mode;
     });
 }
@@ -651,8 +726,10 @@ int falcon_sendrecv_gc(falcon_handle_t h, int which) {
     return guard([&] {
         h->need_tables();
         ensure_device_tables(h);
-        if (which == 0) launch_exchange<double>(h->dims, h->gh, h->U[h->cur].as<double>(), h->stream);
-        else {
+        if (which == 0) {
+            launch_exchange<double>(h->dims, h->gh, h->U[h->cur].as<double>(), h->stream);
+            h->shadow_valid = false;
+        } else {
             alloc_stage(h);
             if (h->precision == 0) launch_exchange<double>(h->dims, h->gh, h->S64[0].as<double>(), h->stream);
             else launch_exchange<float>(h->dims, h->gh, h->S32[0].as<float>(), h->stream);
@@ -666,7 +743,9 @@ int falcon_compute_dt(falcon_handle_t h, double cfl, double *dt) {
     return guard([&] {
         h->need_tables();
         ensure_device_tables(h);
-        launch_dt(h->dims, h->dt, h->U[h->cur].as<double>(), h->partial.as<double>(), h->dtp.as<double>(), cfl, h->stream);
+        launch_dt_partial(h->dims, h->dt, h->U[h->cur].as<double>(), h->partial.as<double>(), h->stream);
+        launch_dt_final(h->dims, h->partial.as<double>(), h->dtp.as<double>(), cfl, h->stream);
+        h->partials_valid = true;
         check_launch();
         HIPCHECK(hipMemcpyAsync(dt, h->dtp.p, sizeof(double), hipMemcpyDeviceToHost, h->stream));
         HIPCHECK(hipStreamSynchronize(h->stream));
@@ -679,6 +758,8 @@ int falcon_advance(falcon_handle_t h, int nsteps, double cfl, int use_graph, dou
         ensure_device_tables(h);
         alloc_stage(h);
         if (nsteps < 0) throw Error("nsteps < 0");
+        if (nsteps == 0) return;
+        prepare_steps(h);
         if (use_graph && nsteps >= 2) {
             if (!h->graph || h->graph_cur != h->cur || h->graph_prec != h->precision || h->graph_cfl != cfl) {
                 if (h->graph) { HIPCHECK(hipGraphExecDestroy(h->graph)); h->graph = nullptr; }
@@ -697,6 +778,7 @@ int falcon_advance(falcon_handle_t h, int nsteps, double cfl, int use_graph, dou
         } else {
             for (int s = 0; s < nsteps; s++) { enqueue_step(h, cfl, h->cur); h->cur = 1 - h->cur; }
         }
+        after_steps(h);
         check_launch();
         if (dt_last) {
             HIPCHECK(hipMemcpyAsync(dt_last, h->dtp.p, sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -716,6 +798,7 @@ int falcon_rhs(falcon_handle_t h, double *r_out) {
         const Dims &d = h->dims;
         const size_t nn = (size_t)d.ni * d.nj * d.nk;
         h->rbuf.alloc(sizeof(double) * nn * 8 * d.nblocks);
+        refresh_u_ghosts(h);
         double *U = h->U[h->cur].as<double>();
         launch_boundary<double>(d, h->bnd, U, h->stream);
         if (h->precision == 0) launch_rhs_only<double, double>(d, h->dt, U, h->rbuf.as<double>(), h->stream);
@@ -764,45 +847,51 @@ int falcon_profile_step(falcon_handle_t h, int nsteps, double cfl, double *out_m
         h->need_tables();
         ensure_device_tables(h);
         alloc_stage(h);
-        const int NE = 20;
-        hipEvent_t ev[NE];
-        for (int i = 0; i < NE; i++) HIPCHECK(hipEventCreate(&ev[i]));
-        double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-        for (int s = 0; s < nsteps; s++) {
-            enqueue_step(h, cfl, h->cur, ev);
+        prepare_steps(h);
+        double acc[K_NKIND] = {0}, cnt[K_NKIND] = {0}, whole = 0;
+        for (int st = 0; st < nsteps; st++) {
+            std::vector<hipEvent_t> ev;
+            std::vector<int> kind;
+            h->prof_ev = &ev;
+            h->prof_kind = &kind;
+            hipEvent_t a, b;
+            HIPCHECK(hipEventCreate(&a));
+            HIPCHECK(hipEventCreate(&b));
+            HIPCHECK(hipEventRecord(a, h->stream));
+            try {
+                enqueue_step(h, cfl, h->cur);
+            } catch (...) {
+                h->prof_ev = nullptr; h->prof_kind = nullptr;
+                throw;
+            }
+            h->prof_ev = nullptr; h->prof_kind = nullptr;
+            HIPCHECK(hipEventRecord(b, h->stream));
             h->cur = 1 - h->cur;
             HIPCHECK(hipStreamSynchronize(h->stream));
             float ms;
-            auto el = [&](int a, int b) { HIPCHECK(hipEventElapsedTime(&ms, ev[a], ev[b])); return (double)ms; };
-            acc[0] += el(0, 1);
-            for (int k = 0; k < 4; k++) {
-                const int base = 1 + 4 * k;
-                acc[1] += el(base, base + 1);
-                acc[2] += el(base + 1, base + 2);
-                if (k < 3) acc[3] += el(base + 2, base + 3);
-                else acc[4] += el(base + 2, base + 3);
+            for (size_t q = 0; q < kind.size(); q++) {
+                HIPCHECK(hipEventElapsedTime(&ms, ev[2 * q], ev[2 * q + 1]));
+                acc[kind[q]] += ms;
+                cnt[kind[q]] += 1;
             }
-            acc[5] += el(17, 18);
-            acc[4] += el(18, 19);
-            acc[6] += el(0, 19);
+            HIPCHECK(hipEventElapsedTime(&ms, a, b));
+            whole += ms;
+            for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
         }
-        for (int i = 0; i < NE; i++) (void)hipEventDestroy(ev[i]);
-        const double n = nsteps > 0 ? nsteps : 1;
-        out_ms[0] = acc[0] / n;          // dt
-        out_ms[1] = acc[1] / (4 * n);    // boundary (per launch)
-        out_ms[2] = acc[2] / (4 * n);    // stage (per launch)
-        out_ms[3] = acc[3] / (3 * n);    // exchange on stage buffers (per launch)
-        out_ms[4] = acc[4] / (2 * n);    // exchange on state (per launch)
-        out_ms[5] = acc[5] / n;          // diffusion
-        out_ms[6] = acc[6] / n;          // whole step
+        after_steps(h);
+        const int order[6] = {K_DT, K_BC, K_STAGE, K_XSTAGE, K_XSTATE, K_DIFF};
+        for (int q = 0; q < 6; q++) out_ms[q] = cnt[order[q]] > 0 ? acc[order[q]] / cnt[order[q]] : 0.0;
+        out_ms[6] = nsteps > 0 ? whole / nsteps : 0.0;
     });
 }
 
 int falcon_device_bytes(falcon_handle_t h, int64_t *bytes) {
     return guard([&] {
         const DevBuf *all[] = {&h->meta, &h->radial, &h->angular, &h->ghost_dst, &h->ghost_src, &h->ghost_w, &h->ghost_rot,
-                               &h->bnd_list, &h->U[0], &h->U[1], &h->S64[0], &h->S64[1], &h->S32[0], &h->S32[1],
-                               &h->dtp, &h->partial, &h->rbuf};
+                               &h->bnd_list, &h->U[0], &h->U[1], &h->F32[0], &h->F32[1], &h->S64[0], &h->S64[1],
+                               &h->S32[0], &h->S32[1], &h->dtp, &h->partial, &h->rbuf};
         int64_t s = 0;
         for (const DevBuf *b : all) s += (int64_t)b->bytes;
         *bytes = s;

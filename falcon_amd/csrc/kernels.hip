@@ -215,12 +215,14 @@ __device__ __forceinline__ void cell_rhs(const T (&win)[8][5], LDSF L, const T x
     }
 }
 
-// MODE 0: out = U + (dt*R)/div on the interior (3_6/ModAdvance.f90:87-93).
+// MODE 0: out = U + (dt*R)/div on the interior (3_6/ModAdvance.f90:87-93);
+//         with SHADOW the same value is also stored rounded to FP32 in `shadow`.
 // MODE 1: write R (interior layout [block][var][i][j][k], FP64) for parity tests.
-template <typename T, typename TIN, typename TOUT, int MODE>
+template <typename T, typename TIN, typename TOUT, int MODE, bool SHADOW>
 __global__ void __launch_bounds__(256) k_stage(Dims d, DevTables t, const TIN *__restrict__ in,
                                                const double *__restrict__ U, TOUT *__restrict__ out,
-                                               const double *__restrict__ dtp, double div) {
+                                               float *__restrict__ shadow, const double *__restrict__ dtp,
+                                               double div) {
     __shared__ T lds[2][8][PL][PL];
     const int tpb = d.tiles_j * d.tiles_k;
     const int b = blockIdx.x / tpb;
@@ -288,7 +290,9 @@ __global__ void __launch_bounds__(256) k_stage(Dims d, DevTables t, const TIN *_
 #pragma unroll
                 for (int v = 0; v < 8; v++) {
                     const long long q = base + (long long)v * d.P;
-                    out[q] = (TOUT)(U[q] + (dt * (double)R[v]) / div);
+                    const double x = U[q] + (dt * (double)R[v]) / div;
+                    out[q] = (TOUT)x;
+                    if (SHADOW) shadow[q] = (float)x;
                 }
             } else {
                 const long long nn = (long long)d.ni * d.nj * d.nk;
@@ -312,11 +316,17 @@ __global__ void __launch_bounds__(256) k_stage(Dims d, DevTables t, const TIN *_
 // the section c(1:ni+1,:,:) (:63) spans the padded transverse range while flux
 // spans 1:nj,1:nk, so the speed comes from transverse index (index - ng); same
 // for :96 and :130.
-// MODE 0: out = U + (0.5*dt)*R (3_6/ModAdvance.f90:104-107); MODE 1: write R.
-template <typename T, int MODE>
-__global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const double *__restrict__ U,
-                                                   double *__restrict__ out, const double *__restrict__ dtp) {
+// MODE 0: out = U + (0.5*dt)*R (3_6/ModAdvance.f90:104-107); `shadow` (if not
+// null) receives the same value in FP32, and the workgroup's min(h/v_wave) of the
+// new state (the next step's ModTimeStep_Dynamo input) goes to partial[tile].
+// MODE 1: write R.  `in` is the stencil input (the state or its FP32 shadow).
+template <typename T, typename TIN, int MODE>
+__global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const TIN *__restrict__ in,
+                                                   const double *__restrict__ U, double *__restrict__ out,
+                                                   float *__restrict__ shadow, double *__restrict__ partial,
+                                                   const double *__restrict__ dtp) {
     __shared__ T lds[2][8][PL][PL];
+    __shared__ double red[4];
     const int tpb = d.tiles_j * d.tiles_k;
     const int b = blockIdx.x / tpb;
     const int trem = blockIdx.x - b * tpb;
@@ -325,11 +335,13 @@ __global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const do
     const int TJa = min(TILE, d.nj - j0), TKa = min(TILE, d.nk - k0);
     const int tx = threadIdx.x & (TILE - 1), ty = threadIdx.x / TILE;
 
-    Marcher<T, double> m;
-    m.init(d, U, b, j0, k0, TJa, TKa);
+    Marcher<T, TIN> m;
+    m.init(d, in, b, j0, k0, TJa, TKa);
     const long long rb = (long long)b * d.NI, tb = (long long)b * d.NJ;
     const BlockMeta mt = t.meta[b];
     const T dxi = (T)mt.dxi, dxj = (T)mt.dxj, dxk = (T)mt.dxk;
+    double hmin = INFINITY;
+    const double sint64 = m.act ? t.sint[tb + m.J] : 1.0;
     const T sintc = m.act ? (T)t.sint[tb + m.J] : T(1);
     const double dt = MODE == 0 ? *dtp : 0.0;
 
@@ -341,7 +353,7 @@ __global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const do
         for (int q = 0; q < 5; q++) win[v][q] = m.act ? m.col(v, q) : T(0);
         hn[v] = m.hv ? m.halo(v, d.ng) : T(0);
     }
-    const double *Ub = U + (long long)b * 8 * d.P;
+    const TIN *Ub = in + (long long)b * 8 * d.P;
 
     for (int i = 0; i < d.ni; i++) {
         const int I = i + d.ng;
@@ -378,7 +390,10 @@ __global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const do
             const T L2 = dxj * xic;
             const T L3 = (dxk * xic) * sintc;
             auto Lv = [&](int v, int dj, int dk) -> T { return lds[buf][v][ty + 2 + dj][tx + 2 + dk]; };
-            T R[8];
+            const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + (long long)J * m.SJ + K;
+            const long long nn = (long long)d.ni * d.nj * d.nk;
+            const long long rbase = (long long)b * 8 * nn + ((long long)i * d.nj + (j0 + ty)) * d.nk + (k0 + tx);
+            double vv = 0.0, bb2 = 0.0;   // |v|^2 and |B|^2 of the new state, summed in component order
 #pragma unroll
             for (int v = 0; v < 8; v++) {
                 T r = T(0);
@@ -399,26 +414,50 @@ __global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const do
                     T f1 = (face_flux(bb, c, dd, e) * (c3p + c30)) * T(0.5);
                     r = r + (f0 - f1) / L3;
                 }
-                R[v] = r;
+                if (MODE == 0) {
+                    const long long q = base + (long long)v * d.P;
+                    const double x = U[q] + (0.5 * dt) * (double)r;
+                    out[q] = x;
+                    if (shadow) shadow[q] = (float)x;
+                    if (v == 1) vv = x * x;
+                    if (v == 2 || v == 3) vv = vv + x * x;
+                    if (v == 4) bb2 = x * x;
+                    if (v == 5 || v == 6) bb2 = bb2 + x * x;
+                } else {
+                    out[rbase + (long long)v * nn] = (double)r;
+                }
             }
             if (MODE == 0) {
-                const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + (long long)J * m.SJ + K;
-#pragma unroll
-                for (int v = 0; v < 8; v++) {
-                    const long long q = base + (long long)v * d.P;
-                    out[q] = U[q] + (0.5 * dt) * (double)R[v];
-                }
-            } else {
-                const long long nn = (long long)d.ni * d.nj * d.nk;
-                const long long base = (long long)b * 8 * nn + ((long long)i * d.nj + (j0 + ty)) * d.nk + (k0 + tx);
-#pragma unroll
-                for (int v = 0; v < 8; v++) out[base + (long long)v * nn] = (double)R[v];
+                // ModWaveSpeed_Dynamo (3_6/ModWaveSpeed.f90:12-24) on the new state and
+                // h_LLL (3_6/ModBlock.f90:420-422), as k_dt_tile computes them
+                double w = t.cs[rb + I] + sqrt(vv);
+                w = w + sqrt(bb2) / t.sqrt_rho0[rb + I];
+                const double xi = t.xi[rb + I];
+                double h = mt.dxi, h2 = xi * mt.dxj, h3 = (xi * sint64) * mt.dxk;
+                h = h < h2 ? h : h2;
+                h = h < h3 ? h : h3;
+                const double x = h / w;
+                hmin = x < hmin ? x : hmin;
             }
         }
 #pragma unroll
         for (int v = 0; v < 8; v++) {
             win[v][0] = win[v][1]; win[v][1] = win[v][2]; win[v][2] = win[v][3]; win[v][3] = win[v][4];
             win[v][4] = nxt[v];
+        }
+    }
+    if (MODE == 0 && partial) {
+        // min is exact in any order: wavefront shuffle, then across the 4 wavefronts
+        for (int off = 32; off > 0; off >>= 1) {
+            double o = __shfl_down(hmin, off, 64);
+            hmin = o < hmin ? o : hmin;
+        }
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = hmin;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double r = red[0];
+            for (int q = 1; q < 4; q++) r = red[q] < r ? red[q] : r;
+            partial[blockIdx.x] = r;
         }
     }
 }
@@ -431,9 +470,14 @@ __global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const do
 // (ModYinYang_VecConv_1D, ModYinYang.f90:59-60).  Each ghost has one donor, and
 // donors are read only at interior cells (clamped bases touch a donor ghost only
 // with an exactly-zero weight), so entries are independent.
+// The grid is a multiple of 8 workgroups; workgroup w runs on XCD w % 8, so
+// each XCD gets one contiguous eighth of the entries and its own L2 serves the
+// donor cells its neighbouring entries share.
 template <typename TS>
 __global__ void __launch_bounds__(256) k_exchange(Dims d, DevGhosts g, TS *__restrict__ st) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned per = gridDim.x / 8;
+    const unsigned wg = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    const int e = (int)(wg * blockDim.x + threadIdx.x);
     if (e >= g.n_plain + g.n_yy) return;
     const long long P = d.P;
     const int SI = d.NJ * d.NK, SJ = d.NK;
@@ -497,45 +541,52 @@ __global__ void __launch_bounds__(256) k_boundary(Dims d, DevBoundary bl, TS *__
 
 // ---------------------------------------------------------------- dt
 
-// ModWaveSpeed_Dynamo (3_6/ModWaveSpeed.f90:12-24) and the per-block
-// minval(h/v_wave) of ModTimeStep_Dynamo (3_6/ModTimeStep.f90:27).
-__global__ void __launch_bounds__(256) k_dt_partial(Dims d, DevTables t, const double *__restrict__ U,
-                                                    double *__restrict__ partial) {
-    const int b = blockIdx.x;
+// ModWaveSpeed_Dynamo (3_6/ModWaveSpeed.f90:12-24) and minval(h/v_wave) of
+// ModTimeStep_Dynamo (3_6/ModTimeStep.f90:27) over one 16x16 (theta,phi) tile
+// column of a block, all r.  Used when the state was set from outside; during
+// stepping the diffusion kernel produces the same partials.
+__global__ void __launch_bounds__(256) k_dt_tile(Dims d, DevTables t, const double *__restrict__ U,
+                                                 double *__restrict__ partial) {
+    __shared__ double red[4];
+    const int tpb = d.tiles_j * d.tiles_k;
+    const int b = blockIdx.x / tpb;
+    const int trem = blockIdx.x - b * tpb;
+    const int tj = trem / d.tiles_k, tk = trem - tj * d.tiles_k;
+    const int tx = threadIdx.x & (TILE - 1), ty = threadIdx.x / TILE;
+    const int j = tj * TILE + ty, k = tk * TILE + tx;
     const long long rb = (long long)b * d.NI, tb = (long long)b * d.NJ;
     const BlockMeta mt = t.meta[b];
     const double *Ub = U + (long long)b * 8 * d.P;
     const int SI = d.NJ * d.NK, SJ = d.NK;
-    const int n = d.ni * d.nj * d.nk;
     double mn = INFINITY;
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
-        const int k = c % d.nk, j = (c / d.nk) % d.nj, i = c / (d.nk * d.nj);
-        const int I = i + d.ng, J = j + d.ng, K = k + d.ng;
-        const long long q = (long long)I * SI + (long long)J * SJ + K;
-        const double vr = Ub[q + d.P], vt = Ub[q + 2 * d.P], vp = Ub[q + 3 * d.P];
-        const double br = Ub[q + 4 * d.P], bt = Ub[q + 5 * d.P], bp = Ub[q + 6 * d.P];
-        double w = t.cs[rb + I] + sqrt(vr * vr + vt * vt + vp * vp);
-        w = w + sqrt(br * br + bt * bt + bp * bp) / t.sqrt_rho0[rb + I];
-        // h_LLL (3_6/ModBlock.f90:420-422)
-        const double xi = t.xi[rb + I];
-        double h = mt.dxi, h2 = xi * mt.dxj, h3 = (xi * t.sint[tb + J]) * mt.dxk;
-        h = h < h2 ? h : h2;
-        h = h < h3 ? h : h3;
-        const double x = h / w;
-        mn = x < mn ? x : mn;
+    if (j < d.nj && k < d.nk) {
+        const int J = j + d.ng, K = k + d.ng;
+        const double st = t.sint[tb + J];
+        for (int i = 0; i < d.ni; i++) {
+            const int I = i + d.ng;
+            const long long q = (long long)I * SI + (long long)J * SJ + K;
+            const double vr = Ub[q + d.P], vt = Ub[q + 2 * d.P], vp = Ub[q + 3 * d.P];
+            const double br = Ub[q + 4 * d.P], bt = Ub[q + 5 * d.P], bp = Ub[q + 6 * d.P];
+            double w = t.cs[rb + I] + sqrt(vr * vr + vt * vt + vp * vp);
+            w = w + sqrt(br * br + bt * bt + bp * bp) / t.sqrt_rho0[rb + I];
+            const double xi = t.xi[rb + I];
+            double h = mt.dxi, h2 = xi * mt.dxj, h3 = (xi * st) * mt.dxk;
+            h = h < h2 ? h : h2;
+            h = h < h3 ? h : h3;
+            const double x = h / w;
+            mn = x < mn ? x : mn;
+        }
     }
-    // wavefront shuffle, then LDS across the 4 wavefronts (min is exact in any order)
     for (int off = 32; off > 0; off >>= 1) {
         double o = __shfl_down(mn, off, 64);
         mn = o < mn ? o : mn;
     }
-    __shared__ double red[4];
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mn;
     __syncthreads();
     if (threadIdx.x == 0) {
         double r = red[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = red[w] < r ? red[w] : r;
-        partial[b] = r;
+        for (int q = 1; q < 4; q++) r = red[q] < r ? red[q] : r;
+        partial[blockIdx.x] = r;
     }
 }
 
@@ -560,10 +611,29 @@ __global__ void __launch_bounds__(256) k_dt_final(const double *__restrict__ par
     }
 }
 
-__global__ void k_copy(const double *__restrict__ s, double *__restrict__ d, long long n) {
+// Radial ghost planes of top/bottom blocks copied from one buffer to another.
+template <typename TS>
+__global__ void __launch_bounds__(256) k_copy_radial(Dims d, DevBoundary bl, const TS *__restrict__ src,
+                                                     TS *__restrict__ dst) {
+    const long long plane = (long long)d.NJ * d.NK;
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long per = (long long)d.ng * plane;
+    if (tid >= per * bl.n) return;
+    const int e = (int)(tid / per);
+    const long long rem = tid - (long long)e * per;
+    const int g = (int)(rem / plane);
+    const long long jk = rem - (long long)g * plane;
+    const int b = bl.list[e] >> 1, side = bl.list[e] & 1;
+    const int I = side ? d.ni + d.ng + g : g;
+    const long long base = (long long)b * 8 * d.P + (long long)I * plane + jk;
+#pragma unroll
+    for (int v = 0; v < 8; v++) dst[base + (long long)v * d.P] = src[base + (long long)v * d.P];
+}
+
+__global__ void k_to_f32(const double *__restrict__ s, float *__restrict__ d, long long n) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long stride = (long long)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) d[i] = s[i];
+    for (; i < n; i += stride) d[i] = (float)s[i];
 }
 
 // ---------------------------------------------------------------- launchers
@@ -571,32 +641,40 @@ __global__ void k_copy(const double *__restrict__ s, double *__restrict__ d, lon
 static inline unsigned tiles(const Dims &d) { return (unsigned)(d.nblocks * d.tiles_j * d.tiles_k); }
 
 template <typename T, typename TIN, typename TOUT>
-void launch_stage(const Dims &d, const DevTables &t, const TIN *in, const double *U, TOUT *out, const double *dtp,
-                  double div, hipStream_t s) {
-    hipLaunchKernelGGL((k_stage<T, TIN, TOUT, 0>), dim3(tiles(d)), dim3(256), 0, s, d, t, in, U, out, dtp, div);
+void launch_stage(const Dims &d, const DevTables &t, const TIN *in, const double *U, TOUT *out, float *shadow,
+                  const double *dtp, double div, hipStream_t s) {
+    if (shadow)
+        hipLaunchKernelGGL((k_stage<T, TIN, TOUT, 0, true>), dim3(tiles(d)), dim3(256), 0, s, d, t, in, U, out, shadow,
+                           dtp, div);
+    else
+        hipLaunchKernelGGL((k_stage<T, TIN, TOUT, 0, false>), dim3(tiles(d)), dim3(256), 0, s, d, t, in, U, out,
+                           (float *)nullptr, dtp, div);
 }
 
 template <typename T, typename TIN>
 void launch_rhs_only(const Dims &d, const DevTables &t, const TIN *in, double *R, hipStream_t s) {
-    hipLaunchKernelGGL((k_stage<T, TIN, double, 1>), dim3(tiles(d)), dim3(256), 0, s, d, t, in, (const double *)nullptr,
-                       R, (const double *)nullptr, 1.0);
+    hipLaunchKernelGGL((k_stage<T, TIN, double, 1, false>), dim3(tiles(d)), dim3(256), 0, s, d, t, in,
+                       (const double *)nullptr, R, (float *)nullptr, (const double *)nullptr, 1.0);
 }
 
-template <typename T>
-void launch_diffusion(const Dims &d, const DevTables &t, const double *U, double *out, const double *dtp, hipStream_t s) {
-    hipLaunchKernelGGL((k_diffusion<T, 0>), dim3(tiles(d)), dim3(256), 0, s, d, t, U, out, dtp);
+template <typename T, typename TIN>
+void launch_diffusion(const Dims &d, const DevTables &t, const TIN *in, const double *U, double *out, float *shadow,
+                      double *partial, const double *dtp, hipStream_t s) {
+    hipLaunchKernelGGL((k_diffusion<T, TIN, 0>), dim3(tiles(d)), dim3(256), 0, s, d, t, in, U, out, shadow, partial, dtp);
 }
 
 template <typename T>
 void launch_diffusion_only(const Dims &d, const DevTables &t, const double *U, double *R, hipStream_t s) {
-    hipLaunchKernelGGL((k_diffusion<T, 1>), dim3(tiles(d)), dim3(256), 0, s, d, t, U, R, (const double *)nullptr);
+    hipLaunchKernelGGL((k_diffusion<T, double, 1>), dim3(tiles(d)), dim3(256), 0, s, d, t, U, U, R, (float *)nullptr,
+                       (double *)nullptr, (const double *)nullptr);
 }
 
 template <typename TS>
 void launch_exchange(const Dims &d, const DevGhosts &g, TS *state, hipStream_t s) {
     const int n = g.n_plain + g.n_yy;
     if (n == 0) return;
-    hipLaunchKernelGGL((k_exchange<TS>), dim3((n + 255) / 256), dim3(256), 0, s, d, g, state);
+    const unsigned wgs = (unsigned)(((n + 255) / 256 + 7) / 8 * 8);
+    hipLaunchKernelGGL((k_exchange<TS>), dim3(wgs), dim3(256), 0, s, d, g, state);
 }
 
 template <typename TS>
@@ -606,29 +684,40 @@ void launch_boundary(const Dims &d, const DevBoundary &b, TS *state, hipStream_t
     hipLaunchKernelGGL((k_boundary<TS>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, b, state);
 }
 
-void launch_dt(const Dims &d, const DevTables &t, const double *U, double *partial, double *dtp, double cfl, hipStream_t s) {
-    hipLaunchKernelGGL(k_dt_partial, dim3(d.nblocks), dim3(256), 0, s, d, t, U, partial);
-    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, s, (const double *)partial, d.nblocks, cfl, dtp);
+template <typename TS>
+void launch_copy_radial(const Dims &d, const DevBoundary &b, const TS *src, TS *dst, hipStream_t s) {
+    if (b.n == 0) return;
+    const long long n = (long long)b.n * d.ng * d.NJ * d.NK;
+    hipLaunchKernelGGL((k_copy_radial<TS>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, b, src, dst);
 }
 
-void launch_copy(const double *src, double *dst, long long n, hipStream_t s) {
-    hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, s, src, dst, n);
+void launch_dt_partial(const Dims &d, const DevTables &t, const double *U, double *partial, hipStream_t s) {
+    hipLaunchKernelGGL(k_dt_tile, dim3(tiles(d)), dim3(256), 0, s, d, t, U, partial);
+}
+
+void launch_dt_final(const Dims &d, const double *partial, double *dtp, double cfl, hipStream_t s) {
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, s, partial, (int)tiles(d), cfl, dtp);
+}
+
+void launch_to_f32(const double *src, float *dst, long long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_to_f32, dim3(4096), dim3(256), 0, s, src, dst, n);
 }
 
 // explicit instantiations: FP64 path, FP32-deviation path
-template void launch_stage<double, double, double>(const Dims &, const DevTables &, const double *, const double *, double *, const double *, double, hipStream_t);
-template void launch_stage<float, double, float>(const Dims &, const DevTables &, const double *, const double *, float *, const double *, double, hipStream_t);
-template void launch_stage<float, float, float>(const Dims &, const DevTables &, const float *, const double *, float *, const double *, double, hipStream_t);
-template void launch_stage<float, float, double>(const Dims &, const DevTables &, const float *, const double *, double *, const double *, double, hipStream_t);
+template void launch_stage<double, double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, const double *, double, hipStream_t);
+template void launch_stage<float, float, float>(const Dims &, const DevTables &, const float *, const double *, float *, float *, const double *, double, hipStream_t);
+template void launch_stage<float, float, double>(const Dims &, const DevTables &, const float *, const double *, double *, float *, const double *, double, hipStream_t);
 template void launch_rhs_only<double, double>(const Dims &, const DevTables &, const double *, double *, hipStream_t);
 template void launch_rhs_only<float, double>(const Dims &, const DevTables &, const double *, double *, hipStream_t);
-template void launch_diffusion<double>(const Dims &, const DevTables &, const double *, double *, const double *, hipStream_t);
-template void launch_diffusion<float>(const Dims &, const DevTables &, const double *, double *, const double *, hipStream_t);
+template void launch_diffusion<double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, double *, const double *, hipStream_t);
+template void launch_diffusion<float, float>(const Dims &, const DevTables &, const float *, const double *, double *, float *, double *, const double *, hipStream_t);
 template void launch_diffusion_only<double>(const Dims &, const DevTables &, const double *, double *, hipStream_t);
 template void launch_diffusion_only<float>(const Dims &, const DevTables &, const double *, double *, hipStream_t);
 template void launch_exchange<double>(const Dims &, const DevGhosts &, double *, hipStream_t);
 template void launch_exchange<float>(const Dims &, const DevGhosts &, float *, hipStream_t);
 template void launch_boundary<double>(const Dims &, const DevBoundary &, double *, hipStream_t);
 template void launch_boundary<float>(const Dims &, const DevBoundary &, float *, hipStream_t);
+template void launch_copy_radial<double>(const Dims &, const DevBoundary &, const double *, double *, hipStream_t);
+template void launch_copy_radial<float>(const Dims &, const DevBoundary &, const float *, float *, hipStream_t);
 
 }  // namespace falcon

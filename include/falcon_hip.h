@@ -68,6 +68,7 @@ int falcon_gc_entries(falcon_handle_t h, int64_t *n, int32_t *recv, int32_t *don
  * ModStratification_new_get_vars (3_6/ModBlock.f90:232-249): bg is 8 x NI
  * column-major: g, rho0, p0, T0, gamma1, gamma3, heat (diffusion+cooling), xi_rsst. */
 int falcon_set_background(falcon_handle_t h, int ib, const double *bg);
+int falcon_get_background(falcon_handle_t h, int ib, double *bg);
 /* n=1.5 polytrope for every block: rho(r_lo)/rho(r_hi) = contrast. */
 int falcon_background_polytrope(falcon_handle_t h, double contrast, double rho_top);
 int falcon_set_state(falcon_handle_t h, int ib, const double *prim);

@@ -18,29 +18,31 @@ R_SUN = float(np.float32(6.955e10))
 
 
 def synthetic_background(xi, seed_shift=0.0):
-    """n=1.5 polytrope profiles at radii xi (cm): columns g, rho0, p0, T0, gamma1, gamma3, heat, xi.
+    """n=1.5 polytrope profiles at radii xi (cm): rows g, rho0, p0, T0, gamma1, gamma3, heat, xi.
 
-    Test input only: any smooth positive profile exercises the same code."""
-    n = 1.5
+    Test input only.  Built from +,-,*,/ and sqrt alone (all correctly rounded),
+    so the same xi gives the same bits on every machine and numpy version."""
     G, M = 6.674e-8, 1.989e33
     x = xi / R_SUN
     beta = 14.86
     zeta = 1.0 + beta * (1.0 / x - 1.0)
+    sz = np.sqrt(zeta)
     rho_b = 1.0e-2
-    pb_over_rhob = G * M / ((n + 1.0) * beta * R_SUN)
-    rho0 = rho_b * zeta ** n
-    p0 = rho_b * pb_over_rhob * zeta ** (n + 1.0)
+    pb_over_rhob = G * M / (2.5 * beta * R_SUN)
+    rho0 = rho_b * (zeta * sz)                 # zeta**1.5
+    p0 = rho_b * pb_over_rhob * (zeta * zeta * sz)   # zeta**2.5
     T0 = 1.0e4 * zeta
-    g = G * M / xi ** 2
+    g = G * M / (xi * xi)
     gam = np.full_like(xi, 5.0 / 3.0)
-    heat = 1.0e-3 * np.sin(3.0 * x + seed_shift)
+    heat = 1.0e-3 * (x - 0.85 + seed_shift)
     xirs = 1.0 + 0.5 * x
     return np.stack([g, rho0, p0, T0, gam, gam * 1.01, heat, xirs])
 
 
 def synthetic_state(rng, pad, amp):
-    """Random padded state (8, NK, NJ, NI), every variable nonzero."""
-    s = rng.standard_normal((8,) + tuple(pad[::-1]))
+    """Random padded state (8, NK, NJ, NI), every variable nonzero; uniform
+    draws scaled per variable (uniform doubles are stable across numpy versions)."""
+    s = 2.0 * rng.random((8,) + tuple(pad[::-1])) - 1.0
     return s * np.asarray(amp, dtype=np.float64)[:, None, None, None]
 
 

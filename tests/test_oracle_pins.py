@@ -1,0 +1,179 @@
+"""Pins of the oracle by closed forms and invariants (CPU).
+
+These fix what the reference fixtures leave open and would catch a dropped
+term, a wrong sign or a transposed index in the restatement: the 4th-order
+stencil with the reference's single-precision 1/12, the spherical metric
+factors, the radial boundary parities, the NaN-pinned limiter, the time step,
+and the Yin-Yang maps.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import pyoracle
+from falcon_amd import configs
+
+F12 = float(np.float32(1.0) / np.float32(12.0)) * 12.0   # 12 * float(1/12) = 1 + 3e-8
+
+LAY = {"ni": 8, "nj": 8, "nk": 8, "ng": 2, "r_rsun": (0.7, 1.0),
+       "ops": [("amr", (0, 1, 1), 1, configs.AMR_ALL_R)]}
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _setup(fields, bg_fn=None):
+    """fields(xi, xj, xk) -> dict var -> array (NK,NJ,NI) on the padded block."""
+    o = pyoracle.Oracle(LAY)
+    for ib in range(o.nblocks):
+        xi, xj, xk = o.centres(ib)
+        NI = xi.size
+        bg = np.zeros((8, NI))
+        bg[0] = 0.0          # g
+        bg[1] = 1.0          # rho0
+        bg[2] = 1.0          # p0
+        bg[3] = 1.0          # T0
+        bg[4] = 5.0 / 3.0    # gamma1
+        bg[5] = 5.0 / 3.0    # gamma3
+        bg[6] = 0.0          # heat
+        bg[7] = 1.0          # xi_rsst
+        if bg_fn:
+            bg = bg_fn(xi, bg)
+        st = np.zeros((8, xk.size, xj.size, NI))
+        K, J, I = np.meshgrid(xk, xj, xi, indexing="ij")
+        for v, f in fields(I, J, K).items():
+            st[v] = f
+        o.set_background(ib, bg)
+        o.set_state(ib, st)
+    return o
+
+
+def _interior(o, a):
+    return a[..., 2:-2, 2:-2, 2:-2]
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_advection_stencil_exact_on_cubics_with_float_twelfth(axis):
+    """R(s1) = -(v . grad) s1 with v a unit component and s1 cubic along it:
+    the 4th-order stencil is exact on cubics, up to the reference's float(1/12)."""
+    def fields(r, t, p):
+        x = [r / 6.955e10, t, p][axis]
+        s1 = 3.0 + 2.0 * x - 1.5 * x * x + 0.7 * x * x * x
+        v = [0.0 * r, 0.0 * r, 0.0 * r]
+        v[axis] = 1.0 + 0.0 * r
+        return {7: s1, 1: v[0], 2: v[1], 3: v[2]}
+    o = _setup(fields)
+    for ib in range(o.nblocks):
+        xi, xj, xk = o.centres(ib)
+        R = o.rhs(ib)[7]
+        K, J, I = np.meshgrid(xk[2:-2], xj[2:-2], xi[2:-2], indexing="ij")
+        x = [I / 6.955e10, J, K][axis]
+        ds = 2.0 - 3.0 * x + 2.1 * x * x
+        if axis == 0:
+            expect = -ds / 6.955e10 * F12
+        elif axis == 1:
+            expect = -(1.0 / I) * ds * F12
+        else:
+            expect = -(1.0 / (I * np.sin(J))) * ds * F12
+        if axis == 0:
+            # every block touches both radial boundaries, whose mirrored ghosts
+            # are not the polynomial: compare two cells away from them
+            R, expect = R[:, :, 2:-2], expect[:, :, 2:-2]
+        err = np.max(np.abs(R - expect)) / np.max(np.abs(expect))
+        assert err < 2e-9, err
+        # and the exact 1/12 would be off by 3e-8: the float literal is reproduced
+        assert np.max(np.abs(R - expect / F12)) / np.max(np.abs(expect)) > 1e-8
+
+
+def test_continuity_metric_terms():
+    """rho0 = 1, xi = 1, v = (a, 0, 0): R(rho1) = -(1/r^2) d(r^2 a)/dr = -2a/r exactly
+    for the stencil (r^2 is a polynomial); v = (0, c, 0): the stencil on sin(t) is
+    exactly cos(t) (8 sin h - sin 2h)/(6h), so R(rho1) = -c/(r sin t) times that."""
+    a = 3.0
+    o = _setup(lambda r, t, p: {1: a + 0.0 * r})
+    c = 2.0
+    o2 = _setup(lambda r, t, p: {2: c + 0.0 * r})
+    for ib in range(o.nblocks):
+        xi, xj, xk = o.centres(ib)
+        K, J, I = np.meshgrid(xk[2:-2], xj[2:-2], xi[2:-2], indexing="ij")
+        R = o.rhs(ib)[0][:, :, 2:-2]   # away from the mirrored radial ghosts
+        ex = ((-2.0 * a / I) * F12)[:, :, 2:-2]
+        assert np.max(np.abs(R - ex)) <= 1e-9 * np.max(np.abs(ex))
+        R2 = o2.rhs(ib)[0]
+        rng, _ = o2.block_info(ib)
+        h = (rng[4] - rng[1]) / 8
+        exp2 = -(c / (I * np.sin(J))) * np.cos(J) * (8.0 * np.sin(h) - np.sin(2.0 * h)) / (6.0 * h) * F12
+        assert np.max(np.abs(R2 - exp2)) <= 1e-12 * np.max(np.abs(exp2))
+
+
+def test_radial_boundary_parities():
+    """ModBoundary_Dynamo_primitives parities: top (+,-,+,+,+,-,-,+), bottom (+,-,+,+,-,+,+,+)."""
+    rng = np.random.default_rng(3)
+    o = _setup(lambda r, t, p: {v: rng.standard_normal(r.shape) for v in range(8)})
+    top = np.array([1, -1, 1, 1, 1, -1, -1, 1.0])
+    bot = np.array([1, -1, 1, 1, -1, 1, 1, 1.0])
+    seen_top = seen_bot = False
+    for ib in range(o.nblocks):
+        _, fl = o.block_info(ib)
+        o.rhs(ib)   # applies the boundary to the state
+        u = o.get_state(ib)
+        if fl[1]:
+            seen_top = True
+            for g in range(2):
+                np.testing.assert_array_equal(u[:, :, :, -2 + g], top[:, None, None] * u[:, :, :, -3 - g])
+        if fl[2]:
+            seen_bot = True
+            for g in range(2):
+                np.testing.assert_array_equal(u[:, :, :, 1 - g], bot[:, None, None] * u[:, :, :, 2 + g])
+    assert seen_top and seen_bot
+
+
+def test_diffusion_of_constant_and_linear_fields_is_zero():
+    """Equal neighbours give 0/0 in the limiter; it must resolve to zero flux
+    (MAX(0,NaN)=0).  Linear fields have equal slopes: phi clamps to 0, no flux."""
+    o = _setup(lambda r, t, p: {v: 1.0 + v + 0.0 * r for v in range(8)})
+    for ib in range(o.nblocks):
+        d = o.diffusion(ib)
+        assert np.all(d == 0.0)
+    o2 = _setup(lambda r, t, p: {0: 2.0 + 1e-10 * r, 7: 1.0 + 0.3 * p, 1: 5.0 - 2.0 * t})
+    for ib in range(o2.nblocks):
+        d = o2.diffusion(ib)
+        assert np.all(np.isfinite(d))
+        assert np.all(d == 0.0)
+
+
+def test_time_step_closed_form():
+    """v = B = 0: dt = CFL * min(h / c_s), h = min(dr, r dt, r sin t dp)."""
+    o = _setup(lambda r, t, p: {7: 0.0 * r},
+               bg_fn=lambda xi, bg: np.vstack([bg[:2], (2.0 + xi / 1e10)[None], bg[3:]]))
+    best = np.inf
+    for ib in range(o.nblocks):
+        rng, _ = o.block_info(ib)
+        xi, xj, xk = o.centres(ib)
+        dr, dth, dph = (rng[3] - rng[0]) / 8, (rng[4] - rng[1]) / 8, (rng[5] - rng[2]) / 8
+        K, J, I = np.meshgrid(xk[2:-2], xj[2:-2], xi[2:-2], indexing="ij")
+        h = np.minimum(np.minimum(dr, I * dth), I * np.sin(J) * dph)
+        cs = np.sqrt((5.0 / 3.0) * (2.0 + I / 1e10))
+        best = min(best, np.min(h / cs))
+    assert abs(o.dt(0.5) - 0.5 * best) <= 1e-14 * 0.5 * best
+
+
+def test_yinyang_map_involution_and_rotation_orthogonal():
+    L = pyoracle.lib()
+    L.orc_coordconv.argtypes = [C.POINTER(C.c_double)] * 2
+    L.orc_vecconv_coeffs.argtypes = [C.POINTER(C.c_double)] * 2
+    rng = np.random.default_rng(7)
+    for _ in range(2000):
+        p = np.array([5e10, rng.uniform(0.7, 2.4), rng.uniform(-2.3, 2.3)])
+        q, back, co = np.zeros(3), np.zeros(3), np.zeros(2)
+        L.orc_coordconv(_p(p), _p(q))
+        L.orc_coordconv(_p(q), _p(back))
+        assert np.allclose(back, p, rtol=0, atol=1e-12)
+        # the map preserves Cartesian position up to the axis permutation
+        xyz = lambda s: np.array([np.sin(s[1]) * np.cos(s[2]), np.sin(s[1]) * np.sin(s[2]), np.cos(s[1])])
+        a, b = xyz(p), xyz(q)
+        assert np.allclose([-a[0], a[2], a[1]], b, atol=1e-12)
+        L.orc_vecconv_coeffs(_p(p), _p(co))
+        assert abs(co[0] ** 2 + co[1] ** 2 - 1.0) < 1e-11
