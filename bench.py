@@ -61,7 +61,12 @@ def main():
     ap.add_argument("--workload", default="shell_128x256x768_b16", choices=sorted(WORKLOADS))
     ap.add_argument("--cfl", type=float, default=0.5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=float, default=0.0,
+                    help="only run a device-to-device copy of this many GiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
         sys.exit(2)
@@ -110,6 +115,19 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def copy_calibration(gib):
+    """A plain copy of known size through torch: its FETCH_SIZE/WRITE_SIZE per
+    dispatch give the scale between the counters and bytes moved."""
+    import torch
+    n = int(gib * (1 << 30)) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": n * 4}))
 
 
 def run_cpu_baseline():

@@ -30,13 +30,15 @@ struct Dims {
     long long P;            // cells per variable per block (NI*NJ*NK)
 };
 
-// Ghost-fill tables (device): plain entries first, Yin-Yang entries after.
+// Ghost-fill tables (device), one row per ghost entry: same-panel entries in
+// [0, yy_begin), Yin-Yang entries in [yy_begin, n_total), each set in the order
+// of the receiving blocks' target lists.
 struct DevGhosts {
     const int32_t *dst;     // block*8*P + cell of the ghost (variable 0)
     const int32_t *src;     // donor*8*P + cell of the interpolation base (variable 0)
     const double *w;        // 3 weights per entry
-    const double *rot;      // 2 rotation coefficients per Yin-Yang entry
-    int n_plain, n_yy;
+    const double *rot;      // 2 rotation coefficients per entry (read for Yin-Yang entries only)
+    int n_total, yy_begin;
 };
 
 struct DevBoundary {

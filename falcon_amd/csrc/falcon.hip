@@ -86,6 +86,7 @@ struct falcon_handle {
     DevBoundary bnd{};
     // device memory
     DevBuf meta, radial, angular, ghost_dst, ghost_src, ghost_w, ghost_rot, bnd_list;
+
     DevBuf U[2], F32[2], S64[2], S32[2], dtp, partial, rbuf;
     int cur = 0;             // index of the current state buffer in U[]
     // validity of derived device data
@@ -102,6 +103,7 @@ struct falcon_handle {
     ~falcon_handle() {
         if (graph) (void)hipGraphExecDestroy(graph);
         DevBuf *all[] = {&meta, &radial, &angular, &ghost_dst, &ghost_src, &ghost_w, &ghost_rot, &bnd_list,
+
                          &U[0], &U[1], &F32[0], &F32[1], &S64[0], &S64[1], &S32[0], &S32[1], &dtp, &partial, &rbuf};
         for (DevBuf *b : all) b->free();
         if (stream) (void)hipStreamDestroy(stream);
@@ -212,38 +214,40 @@ void upload_tables(falcon_handle *h) {
     dt.g_over_rho0 = R + 6 * rs; dt.xir2 = R + 7 * rs; dt.heat = R + 8 * rs; dt.cs = R + 9 * rs; dt.sqrt_rho0 = R + 10 * rs;
     dt.sint = A; dt.sinv = A + as; dt.cot = A + 2 * as;
 
-    // ghost entries: plain first, Yin-Yang after
+    // ghost entries: same-panel first, then Yin-Yang, each in target-list order
     const auto &G = h->lay.ghosts;
     const long long P = (long long)NI * NJ * NK;
     auto cell = [&](int i, int j, int k) -> long long {
         return ((long long)(i + ng - 1) * NJ + (j + ng - 1)) * NK + (k + ng - 1);
     };
     if ((long long)nb * 8 * P >= (1ll << 31)) throw Error("state too large for 32-bit ghost offsets");
-    std::vector<int32_t> dst, src;
-    std::vector<double> w, rot;
-    dst.reserve(G.size()); src.reserve(G.size()); w.reserve(3 * G.size());
-    int n_plain = 0, n_yy = 0;
+    const size_t NG = G.size();
+    std::vector<size_t> order;
+    order.reserve(NG);
     for (int pass = 0; pass < 2; pass++)
-        for (const GhostEntry &e : G) {
-            if (e.yy != pass) continue;
-            dst.push_back((int32_t)((long long)e.recv_block * 8 * P + cell(e.gi, e.gj, e.gk)));
-            src.push_back((int32_t)((long long)e.donor * 8 * P + cell(e.i0, e.j0, e.k0)));
-            w.push_back(e.w[0]); w.push_back(e.w[1]); w.push_back(e.w[2]);
-            if (pass) { rot.push_back(e.rot[0]); rot.push_back(e.rot[1]); n_yy++; }
-            else n_plain++;
-        }
-    h->ghost_dst.alloc(sizeof(int32_t) * dst.size());
-    h->ghost_src.alloc(sizeof(int32_t) * src.size());
-    h->ghost_w.alloc(sizeof(double) * w.size());
-    h->ghost_rot.alloc(sizeof(double) * std::max<size_t>(rot.size(), 2));
-    if (!dst.empty()) {
-        HIPCHECK(hipMemcpy(h->ghost_dst.p, dst.data(), sizeof(int32_t) * dst.size(), hipMemcpyHostToDevice));
-        HIPCHECK(hipMemcpy(h->ghost_src.p, src.data(), sizeof(int32_t) * src.size(), hipMemcpyHostToDevice));
-        HIPCHECK(hipMemcpy(h->ghost_w.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+        for (size_t e = 0; e < NG; e++)
+            if (G[e].yy == pass) order.push_back(e);
+    int yy_begin = 0;
+    while (yy_begin < (int)NG && !G[order[yy_begin]].yy) yy_begin++;
+    std::vector<int32_t> dst(NG), src(NG);
+    std::vector<double> w(3 * NG), rot(2 * NG);
+    for (size_t q = 0; q < NG; q++) {
+        const GhostEntry &e = G[order[q]];
+        dst[q] = (int32_t)((long long)e.recv_block * 8 * P + cell(e.gi, e.gj, e.gk));
+        src[q] = (int32_t)((long long)e.donor * 8 * P + cell(e.i0, e.j0, e.k0));
+        w[3 * q] = e.w[0]; w[3 * q + 1] = e.w[1]; w[3 * q + 2] = e.w[2];
+        rot[2 * q] = e.rot[0]; rot[2 * q + 1] = e.rot[1];
     }
-    if (!rot.empty()) HIPCHECK(hipMemcpy(h->ghost_rot.p, rot.data(), sizeof(double) * rot.size(), hipMemcpyHostToDevice));
+    auto up = [&](DevBuf &buf, const void *data, size_t bytes) {
+        buf.alloc(std::max<size_t>(bytes, 16));
+        if (bytes) HIPCHECK(hipMemcpy(buf.p, data, bytes, hipMemcpyHostToDevice));
+    };
+    up(h->ghost_dst, dst.data(), sizeof(int32_t) * NG);
+    up(h->ghost_src, src.data(), sizeof(int32_t) * NG);
+    up(h->ghost_w, w.data(), sizeof(double) * 3 * NG);
+    up(h->ghost_rot, rot.data(), sizeof(double) * 2 * NG);
     h->gh = {h->ghost_dst.as<int32_t>(), h->ghost_src.as<int32_t>(), h->ghost_w.as<double>(), h->ghost_rot.as<double>(),
-             n_plain, n_yy};
+             (int)NG, yy_begin};
 
     std::vector<int32_t> bl;
     for (int b = 0; b < nb; b++) {
@@ -890,6 +894,7 @@ int falcon_profile_step(falcon_handle_t h, int nsteps, double cfl, double *out_m
 int falcon_device_bytes(falcon_handle_t h, int64_t *bytes) {
     return guard([&] {
         const DevBuf *all[] = {&h->meta, &h->radial, &h->angular, &h->ghost_dst, &h->ghost_src, &h->ghost_w, &h->ghost_rot,
+
                                &h->bnd_list, &h->U[0], &h->U[1], &h->F32[0], &h->F32[1], &h->S64[0], &h->S64[1],
                                &h->S32[0], &h->S32[1], &h->dtp, &h->partial, &h->rbuf};
         int64_t s = 0;
