@@ -38,3 +38,11 @@ def small_mixed():
             "ops": [("amr", (0, 1, 1), 1, AMR_ALL_R),
                     ("amr", (1, 0, 0), 1, AMR_ALL_R),
                     ("amr", (0, 1, 1), 0, (0.8, 1.2, -0.5, 0.95, 1.6, 0.3))]}
+
+
+def cubic_refined(cells_per_panel, block, region=(0.8, 1.2, -0.5, 0.95, 1.6, 0.3)):
+    """`cubic` plus one local 2x2x2 refinement of the Yin leaves in `region`
+    (rng6 in R_sun / radians): cubic blocks at two levels, coarse/fine ghosts."""
+    lay = cubic(cells_per_panel, block)
+    lay["ops"] = lay["ops"] + [("amr", (1, 1, 1), 0, region)]
+    return lay

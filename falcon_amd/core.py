@@ -61,6 +61,15 @@ def load():
         "falcon_profile_step": [vp, C.c_int, C.c_double, dp],
         "falcon_device_bytes": [vp, i64p],
         "falcon_synchronize": [vp],
+        "falcon_heat_setup": [vp, C.c_double, C.c_double, C.c_int],
+        "falcon_heat_info": [vp, ip, i64p, dp],
+        "falcon_heat_set_dt": [vp, dp],
+        "falcon_heat_set_T": [vp, dp],
+        "falcon_heat_seed_T": [vp, C.c_uint64, C.c_double],
+        "falcon_heat_solve": [vp, C.c_double, C.c_int, C.c_int, ip, dp, dp],
+        "falcon_heat_get_T": [vp, C.c_int, dp],
+        "falcon_heat_apply": [vp, dp, dp],
+        "falcon_heat_vcycle": [vp, dp, dp],
         "falcon_last_error": [C.c_char_p, C.c_int],
     }
     for name, args in sig.items():
@@ -243,3 +252,57 @@ class Shell:
     @property
     def interior_cells(self):
         return self.nblocks * self.ni * self.nj * self.nk
+
+
+class HeatSolver:
+    """Implicit heat conduction on a Shell's blocks (cubic, power-of-two edge):
+    backward Euler, MG-preconditioned flexible CG, `nsys` systems solved in the
+    same launches.  Interior fields are numpy arrays of shape
+    (nsys, nblocks, n, n, n) in the C view of Fortran order per block (i fastest)."""
+
+    def __init__(self, shell, kappa0=1.0, r_ref_rsun=0.7, nsys=1):
+        self.s = shell
+        self.L = shell.L
+        self.nsys = nsys
+        _check(self.L.falcon_heat_setup(shell.h, kappa0, r_ref_rsun, nsys))
+        lv, cells, dte = C.c_int32(), C.c_int64(), C.c_double()
+        _check(self.L.falcon_heat_info(shell.h, C.byref(lv), C.byref(cells), C.byref(dte)))
+        self.levels, self.cells, self.dt_explicit = lv.value, cells.value, dte.value
+        n = shell.ni
+        self.shape = (nsys, shell.nblocks, n, n, n)
+
+    def set_dt(self, dt):
+        dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (self.nsys,)))
+        _check(self.L.falcon_heat_set_dt(self.s.h, _p(dt)))
+
+    def set_T(self, T):
+        T = np.ascontiguousarray(T, dtype=np.float64).reshape(self.shape)
+        _check(self.L.falcon_heat_set_T(self.s.h, _p(T)))
+
+    def seed_T(self, seed, sigma):
+        _check(self.L.falcon_heat_seed_T(self.s.h, seed, sigma))
+
+    def solve(self, tol=1e-10, max_iter=500, use_corrector=False):
+        it = np.zeros(self.nsys, np.int32)
+        rel = np.zeros(self.nsys)
+        sec = C.c_double()
+        _check(self.L.falcon_heat_solve(self.s.h, tol, max_iter, 1 if use_corrector else 0, _p(it, C.c_int32), _p(rel),
+                                        C.byref(sec)))
+        return it, rel, sec.value
+
+    def get_T(self, sys=0):
+        out = np.zeros(self.shape[1:])
+        _check(self.L.falcon_heat_get_T(self.s.h, sys, _p(out)))
+        return out
+
+    def apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(self.shape)
+        y = np.zeros(self.shape)
+        _check(self.L.falcon_heat_apply(self.s.h, _p(x), _p(y)))
+        return y
+
+    def vcycle(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64).reshape(self.shape)
+        z = np.zeros(self.shape)
+        _check(self.L.falcon_heat_vcycle(self.s.h, _p(r), _p(z)))
+        return z

@@ -16,6 +16,7 @@
 #include "../../include/falcon_hip.h"
 #include "device.hpp"
 #include "layout.hpp"
+#include "heat.hpp"
 
 namespace falcon {
 void synthetic_state(const Layout &lay, const std::vector<double> &bg, uint64_t seed, double s_amp, double b_ratio,
@@ -98,6 +99,8 @@ struct falcon_handle {
     std::vector<int> *prof_kind = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_cur = -1, graph_prec = -1;
+    std::unique_ptr<HeatSolver> heat;
+    HeatParams heat_params;
     double graph_cfl = -1;
 
     ~falcon_handle() {
@@ -397,6 +400,24 @@ void refresh_u_ghosts(falcon_handle *h) {
 void check_launch() { HIPCHECK(hipGetLastError()); }
 
 }  // namespace
+
+// host Fortran order (i fastest) <-> device [i][j][k] (k fastest) for m^3 blocks
+void heat_to_device(const double *src, double *dst, int m, long long nblk) {
+    const long long M3 = (long long)m * m * m;
+    for (long long b = 0; b < nblk; b++)
+        for (int k = 0; k < m; k++)
+            for (int j = 0; j < m; j++)
+                for (int i = 0; i < m; i++)
+                    dst[b * M3 + ((long long)i * m + j) * m + k] = src[b * M3 + i + (long long)m * (j + (long long)m * k)];
+}
+void heat_to_host(const double *src, double *dst, int m, long long nblk) {
+    const long long M3 = (long long)m * m * m;
+    for (long long b = 0; b < nblk; b++)
+        for (int k = 0; k < m; k++)
+            for (int j = 0; j < m; j++)
+                for (int i = 0; i < m; i++)
+                    dst[b * M3 + i + (long long)m * (j + (long long)m * k)] = src[b * M3 + ((long long)i * m + j) * m + k];
+}
 
 extern "C" {
 
@@ -900,6 +921,119 @@ int falcon_device_bytes(falcon_handle_t h, int64_t *bytes) {
         int64_t s = 0;
         for (const DevBuf *b : all) s += (int64_t)b->bytes;
         *bytes = s;
+    });
+}
+
+// ---------------------------------------------------------------- heat conduction
+
+int falcon_heat_setup(falcon_handle_t h, double kappa0, double r_ref_rsun, int nsys) {
+    if (falcon_build_tables(h) != 0) return -1;
+    return guard([&] {
+        if (nsys < 1) throw Error("nsys must be >= 1");
+        if (!(kappa0 > 0)) throw Error("kappa0 must be positive");
+        h->heat_params.kappa0 = kappa0;
+        h->heat_params.r_ref = r_ref_rsun * (double)6.955e10f;
+        h->heat.reset(new HeatSolver());
+        h->heat->setup(h->lay, h->heat_params, nsys, host_threads());
+    });
+}
+
+static HeatSolver &need_heat(falcon_handle_t h) {
+    if (!h->heat) throw Error("falcon_heat_setup has not been called");
+    return *h->heat;
+}
+
+int falcon_heat_info(falcon_handle_t h, int *levels, int64_t *cells, double *dt_explicit) {
+    return guard([&] {
+        HeatSolver &H = need_heat(h);
+        if (levels) *levels = H.levels();
+        if (cells) *cells = H.cells();
+        if (dt_explicit) *dt_explicit = H.explicit_dt();
+    });
+}
+
+int falcon_heat_set_dt(falcon_handle_t h, const double *dt) {
+    return guard([&] {
+        HeatSolver &H = need_heat(h);
+        H.set_dt(std::vector<double>(dt, dt + H.nsys()));
+    });
+}
+
+int falcon_heat_set_T(falcon_handle_t h, const double *T) {
+    return guard([&] {
+        HeatSolver &H = need_heat(h);
+        const long long n = H.cells() * H.nsys();
+        std::vector<double> dev(n);
+        heat_to_device(T, dev.data(), H.m(), n / ((long long)H.m() * H.m() * H.m()));
+        H.set_rhs_from_T(dev);
+    });
+}
+
+int falcon_heat_seed_T(falcon_handle_t h, uint64_t seed, double sigma) {
+    return guard([&] {
+        HeatSolver &H = need_heat(h);
+        const long long n = H.cells() * H.nsys();
+        std::vector<double> T(n);
+        // Gaussian T' (Box-Muller on splitmix64 draws), one stream per cell
+        for (long long c = 0; c < n; c++) {
+            uint64_t z = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(c + 1);
+            auto mix = [](uint64_t x) {
+                x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+                x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+                return x ^ (x >> 31);
+            };
+            const double u1 = ((mix(z) >> 11) + 1.0) * (1.0 / 9007199254740993.0);
+            const double u2 = (mix(z ^ 0xD1B54A32D192ED03ull) >> 11) * (1.0 / 9007199254740992.0);
+            T[c] = sigma * std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+        }
+        H.set_rhs_from_T(T);
+    });
+}
+
+int falcon_heat_solve(falcon_handle_t h, double tol, int max_iter, int use_corrector, int32_t *iters, double *relres,
+                      double *seconds) {
+    return guard([&] {
+        HeatSolver &H = need_heat(h);
+        HeatResult r = H.solve(tol, max_iter, use_corrector != 0);
+        for (int s = 0; s < H.nsys(); s++) {
+            if (iters) iters[s] = r.iters[s];
+            if (relres) relres[s] = r.relres[s];
+        }
+        if (seconds) *seconds = r.seconds;
+    });
+}
+
+int falcon_heat_get_T(falcon_handle_t h, int sys, double *T) {
+    return guard([&] {
+        HeatSolver &H = need_heat(h);
+        if (sys < 0 || sys >= H.nsys()) throw Error("system index out of range");
+        std::vector<double> dev;
+        H.get_solution(sys, dev);
+        heat_to_host(dev.data(), T, H.m(), (long long)dev.size() / ((long long)H.m() * H.m() * H.m()));
+    });
+}
+
+int falcon_heat_apply(falcon_handle_t h, const double *x, double *y) {
+    return guard([&] {
+        HeatSolver &H = need_heat(h);
+        const long long n = H.cells() * H.nsys();
+        const long long nblk = n / ((long long)H.m() * H.m() * H.m());
+        std::vector<double> xd(n), yd;
+        heat_to_device(x, xd.data(), H.m(), nblk);
+        H.apply(xd, yd);
+        heat_to_host(yd.data(), y, H.m(), nblk);
+    });
+}
+
+int falcon_heat_vcycle(falcon_handle_t h, const double *r, double *z) {
+    return guard([&] {
+        HeatSolver &H = need_heat(h);
+        const long long n = H.cells() * H.nsys();
+        const long long nblk = n / ((long long)H.m() * H.m() * H.m());
+        std::vector<double> rd(n), zd;
+        heat_to_device(r, rd.data(), H.m(), nblk);
+        H.vcycle_host(rd, zd);
+        heat_to_host(zd.data(), z, H.m(), nblk);
     });
 }
 

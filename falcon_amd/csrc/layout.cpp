@@ -329,3 +329,148 @@ void polytrope(const PolytropeParams &p, double r, double out[8]) {
 }
 
 }  // namespace falcon
+
+// ---------------------------------------------------------------------------
+// Heat-conduction levels.  Face areas, volumes and centre distances follow
+// ModMultigrid_InitGrid / ModBlock_InitGrid (3_5/ModMultigrid.f90:150-233,
+// 3_6/ModBlock.f90:367-415); the operator follows the finite-volume assembly of
+// ModPFSS_setup (3_5/ModPFSS.f90:68-84) with kappa(r) at the faces.
+namespace falcon {
+
+namespace {
+struct LevelGeom {
+    double rng[3][2];
+    int yin;
+    std::vector<double> xc[3];   // padded centres, [0] <-> index 0 (one ghost layer)
+    std::vector<double> xf[3];   // faces 1..m+1 -> [0..m]
+    double dx[3];
+};
+
+LevelGeom level_geom(const Node &n, int m) {
+    LevelGeom g;
+    std::memcpy(g.rng, n.rng, sizeof(g.rng));
+    g.yin = n.yin;
+    for (int a = 0; a < 3; a++) {
+        g.xc[a].resize(m + 2);
+        g.xf[a].resize(m + 1);
+        for (int i = 0; i <= m + 1; i++)
+            g.xc[a][i] = (n.rng[a][0] * (double)((float)(m - i) + 0.5f) + n.rng[a][1] * (double)((float)i - 0.5f)) / (double)m;
+        for (int i = 1; i <= m + 1; i++)
+            g.xf[a][i - 1] = (n.rng[a][0] * (double)(m + 1 - i) + n.rng[a][1] * (double)(i - 1)) / (double)m;
+        g.dx[a] = (n.rng[a][1] - n.rng[a][0]) / (double)m;
+    }
+    return g;
+}
+}  // namespace
+
+void build_heat_levels(const Layout &lay, const HeatParams &hp, int nthreads, std::vector<HeatLevel> &levels) {
+    const Tree &t = lay.tree;
+    if (t.ni != t.nj || t.ni != t.nk) throw std::runtime_error("heat levels need cubic blocks");
+    int n = t.ni;
+    if (n & (n - 1)) throw std::runtime_error("heat levels need a power-of-two block edge");
+    const int nb = t.nleaf;
+    levels.clear();
+    for (int m = n; m >= 1; m /= 2) {
+        HeatLevel L;
+        L.m = m;
+        const int M3 = m * m * m, P = (m + 2) * (m + 2) * (m + 2);
+        L.V.assign((size_t)nb * M3, 0.0);
+        L.offsum.assign((size_t)nb * M3, 0.0);
+        L.Ai.assign((size_t)nb * (m + 1) * m * m, 0.0);
+        L.Aj.assign((size_t)nb * m * (m + 1) * m, 0.0);
+        L.Ak.assign((size_t)nb * m * m * (m + 1), 0.0);
+        std::vector<std::vector<int32_t>> gd(nb), gs(nb);
+        std::vector<std::vector<double>> gw(nb);
+        auto kap = [&](double r) { double x = r / hp.r_ref; return hp.kappa0 / (x * x * x); };
+        auto work = [&](int b) {
+            const Node &nd = t.nodes[t.leaf_node[b]];
+            LevelGeom g = level_geom(nd, m);
+            const bool top = lay.geom[b].top, bottom = lay.geom[b].bottom;
+            const double *rf = g.xf[0].data(), *tf = g.xf[1].data(), *pf = g.xf[2].data();
+            const double *rc = g.xc[0].data(), *tc = g.xc[1].data();
+            double *V = &L.V[(size_t)b * M3], *off = &L.offsum[(size_t)b * M3];
+            double *Ai = &L.Ai[(size_t)b * (m + 1) * m * m], *Aj = &L.Aj[(size_t)b * m * (m + 1) * m],
+                   *Ak = &L.Ak[(size_t)b * m * m * (m + 1)];
+            // face coefficients, indexed [i][j][k] with the face index on its own axis
+            for (int i = 0; i <= m; i++)
+                for (int j = 0; j < m; j++)
+                    for (int k = 0; k < m; k++) {
+                        double S = rf[i] * rf[i] * (std::cos(tf[j]) - std::cos(tf[j + 1])) * (pf[k + 1] - pf[k]);
+                        double c = kap(rf[i]) * S / g.dx[0];
+                        if ((i == 0 && bottom) || (i == m && top)) c = 0.0;   // insulated radial boundary
+                        Ai[((size_t)i * m + j) * m + k] = c;
+                    }
+            for (int i = 0; i < m; i++)
+                for (int j = 0; j <= m; j++)
+                    for (int k = 0; k < m; k++) {
+                        double S = 0.5 * (rf[i + 1] * rf[i + 1] - rf[i] * rf[i]) * (pf[k + 1] - pf[k]) * std::sin(tf[j]);
+                        double D = rc[i + 1] * g.dx[1];
+                        Aj[((size_t)i * (m + 1) + j) * m + k] = kap(rc[i + 1]) * S / D;
+                    }
+            for (int i = 0; i < m; i++)
+                for (int j = 0; j < m; j++)
+                    for (int k = 0; k <= m; k++) {
+                        double S = 0.5 * (rf[i + 1] * rf[i + 1] - rf[i] * rf[i]) * (tf[j + 1] - tf[j]);
+                        double D = rc[i + 1] * g.dx[2] * std::sin(tc[j + 1]);
+                        Ak[((size_t)i * m + j) * (m + 1) + k] = kap(rc[i + 1]) * S / D;
+                    }
+            for (int i = 0; i < m; i++)
+                for (int j = 0; j < m; j++)
+                    for (int k = 0; k < m; k++) {
+                        const size_t c = ((size_t)i * m + j) * m + k;
+                        V[c] = 1.0 / 3.0 * (rf[i + 1] * rf[i + 1] * rf[i + 1] - rf[i] * rf[i] * rf[i]) *
+                               (std::cos(tf[j]) - std::cos(tf[j + 1])) * (pf[k + 1] - pf[k]);
+                        off[c] = Ai[((size_t)i * m + j) * m + k] + Ai[((size_t)(i + 1) * m + j) * m + k] +
+                                 Aj[((size_t)i * (m + 1) + j) * m + k] + Aj[((size_t)i * (m + 1) + j + 1) * m + k] +
+                                 Ak[((size_t)i * m + j) * (m + 1) + k] + Ak[((size_t)i * m + j) * (m + 1) + k + 1];
+                    }
+            // face ghosts (6 faces, no edges/corners); radial domain boundary faces have
+            // zero coefficients, so their ghosts are left unfilled
+            for (int I = 0; I <= m + 1; I++)
+                for (int J = 0; J <= m + 1; J++)
+                    for (int K = 0; K <= m + 1; K++) {
+                        const int out = (I == 0 || I == m + 1) + (J == 0 || J == m + 1) + (K == 0 || K == m + 1);
+                        if (out != 1) continue;
+                        double x[3] = {g.xc[0][I], g.xc[1][J], g.xc[2][K]};
+                        int d = t.find(x, g.yin);
+                        if (d == -777) continue;
+                        d -= 1;
+                        const LevelGeom dg = level_geom(t.nodes[t.leaf_node[d]], m);
+                        if (dg.yin != g.yin) { double o[3]; coordconv(x, o); x[0] = o[0]; x[1] = o[1]; x[2] = o[2]; }
+                        // linear inter/extrapolation inside the donor block: the point's padded
+                        // index p lies in [0.5, m+0.5]; the stencil base is kept in [1, m-1] so
+                        // only interior donor cells are read and fields linear in (r, theta, phi)
+                        // are reproduced exactly across blocks of different spacing.  m = 1
+                        // copies the donor's single cell.
+                        int base[3];
+                        double w[3];
+                        for (int a = 0; a < 3; a++) {
+                            if (m == 1) { base[a] = 1; w[a] = 0.0; continue; }
+                            const double dxa = dg.xc[a][2] - dg.xc[a][1];
+                            double p = (x[a] - dg.xc[a][0]) / dxa;    // padded index of the point
+                            p = p < (double)m + 0.5 ? p : (double)m + 0.5;
+                            p = p > 0.5 ? p : 0.5;
+                            int bs = (int)std::floor(p);
+                            bs = bs < 1 ? 1 : (bs > m - 1 ? m - 1 : bs);
+                            base[a] = bs;
+                            w[a] = p - (double)bs;
+                        }
+                        gd[b].push_back((int32_t)((size_t)b * P + ((size_t)I * (m + 2) + J) * (m + 2) + K));
+                        gs[b].push_back((int32_t)((size_t)d * P + ((size_t)base[0] * (m + 2) + base[1]) * (m + 2) + base[2]));
+                        gw[b].push_back(w[0]); gw[b].push_back(w[1]); gw[b].push_back(w[2]);
+                    }
+        };
+        std::vector<std::thread> th;
+        int nt = std::max(1, std::min(nthreads, nb));
+        for (int w = 0; w < nt; w++) th.emplace_back([&, w]() { for (int b = w; b < nb; b += nt) work(b); });
+        for (auto &x : th) x.join();
+        for (int b = 0; b < nb; b++) {
+            L.g_dst.insert(L.g_dst.end(), gd[b].begin(), gd[b].end());
+            L.g_src.insert(L.g_src.end(), gs[b].begin(), gs[b].end());
+            L.g_w.insert(L.g_w.end(), gw[b].begin(), gw[b].end());
+        }
+        levels.push_back(std::move(L));
+    }
+}
+
+}  // namespace falcon

@@ -75,6 +75,29 @@ struct Layout {
     void build(int nthreads);
 };
 
+// ---------------------------------------------------------------- heat levels
+// One multigrid level of the heat operator: every block holds m^3 cells with one
+// ghost layer; ghost cells on the 6 faces are filled from donors by trilinear
+// inter/extrapolation inside the donor block (no edges or corners: the operator
+// is 7-point).
+struct HeatLevel {
+    int m = 0;                           // cells per block edge
+    std::vector<double> V;               // [block][m^3] cell volumes
+    std::vector<double> Ai, Aj, Ak;      // [block][(m+1) m m] etc.: kappa S / D per face (>= 0)
+    std::vector<double> offsum;          // [block][m^3] sum of the cell's face coefficients
+    // face ghosts: receiving padded cell, donor padded base cell, weights
+    std::vector<int32_t> g_dst, g_src;   // block*(m+2)^3 + padded cell
+    std::vector<double> g_w;             // 3 per entry
+};
+
+struct HeatParams {
+    double kappa0 = 1.0;     // kappa(r) = kappa0 * (r / r_ref)^-3
+    double r_ref = 0.0;      // cm
+};
+
+// Builds levels m = n, n/2, ..., 1 for cubic blocks (ni = nj = nk = n, power of 2).
+void build_heat_levels(const Layout &lay, const HeatParams &hp, int nthreads, std::vector<HeatLevel> &levels);
+
 // Yin<->Yang point map (3_6/ModYinYang.f90:8-17); same glibc calls as the reference.
 void coordconv(const double in[3], double out[3]);
 

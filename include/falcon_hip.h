@@ -105,6 +105,34 @@ int falcon_profile_step(falcon_handle_t h, int nsteps, double cfl, double *out_m
 int falcon_device_bytes(falcon_handle_t h, int64_t *bytes);
 int falcon_synchronize(falcon_handle_t h);
 
+/* ---------------------------------------------------------------- heat conduction
+ * ModHeatConduction (3_6/ModHeatConduction.f90:1-16) is an empty module in the
+ * reference; these entry points define its solve in the shape of
+ * ModPFSS_setup/ModPFSS_solve (3_5/ModPFSS.f90:21,102):
+ *   (V/dt) T + sum_f kappa_f S_f/D_f (T - T_nb) = (V/dt) T^n,
+ *   kappa(r) = kappa0 (r / r_ref)^-3, insulated radial boundaries,
+ * solved by flexible CG preconditioned with a per-block multigrid V-cycle
+ * (3_5/ModLinearSolver.f90:201-427 as a method), FP64, for nsys systems at once.
+ * T arrays: nsys x nblocks x (n,n,n) Fortran order per block, cubic blocks. */
+int falcon_heat_setup(falcon_handle_t h, double kappa0, double r_ref_rsun, int nsys);
+int falcon_heat_info(falcon_handle_t h, int *levels, int64_t *cells, double *dt_explicit);
+int falcon_heat_set_dt(falcon_handle_t h, const double *dt);
+/* right-hand side b = (V/dt) T^n from T^n (set dt first) */
+int falcon_heat_set_T(falcon_handle_t h, const double *T);
+/* seeded Gaussian T^n with standard deviation sigma */
+int falcon_heat_seed_T(falcon_handle_t h, uint64_t seed, double sigma);
+/* ModHeatConduction_solve: to ||b - A T||/||b|| <= tol; use_corrector adds the
+ * learned correction to the V-cycle (falcon_cnn_*).  Per system iterations and
+ * final relative residual; wall seconds of the solve. */
+int falcon_heat_solve(falcon_handle_t h, double tol, int max_iter, int use_corrector, int32_t *iters, double *relres,
+                      double *seconds);
+int falcon_heat_get_T(falcon_handle_t h, int sys, double *T);
+/* y = A x for all systems (parity tests) */
+int falcon_heat_apply(falcon_handle_t h, const double *x, double *y);
+/* z = M r: one V-cycle of the preconditioner for all systems, zero initial
+ * guess (ModMultigrid_Vcycle, 3_5/ModMultigrid.f90:338-376, as a method) */
+int falcon_heat_vcycle(falcon_handle_t h, const double *r, double *z);
+
 /* ---------------------------------------------------------------- errors */
 int falcon_last_error(char *buf, int len);
 

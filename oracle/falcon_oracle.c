@@ -914,3 +914,358 @@ void orc_vecconv_coeffs(const double x[3], double out[2]) {
     out[0] = sp * spo;
     out[1] = cp / sto;
 }
+
+/* ======================================================================
+ * Heat conduction oracle.  The reference has no heat operator
+ * (3_6/ModHeatConduction.f90:15-16); the method restated here is the one the
+ * product defines: backward Euler (V/dt)T + sum_f kappa_f S_f/D_f (T - T_nb)
+ * = (V/dt) T^n with face areas/volumes/distances of ModMultigrid_InitGrid
+ * (3_5/ModMultigrid.f90:150-233), the finite-volume assembly of ModPFSS_setup
+ * (3_5/ModPFSS.f90:68-84), insulated radial boundaries, kappa(r) =
+ * kappa0 (r/r_ref)^-3; per-block multigrid levels with face ghosts filled by
+ * trilinear inter/extrapolation inside the donor found by FindNode (exact for
+ * fields linear in r, theta, phi, whatever the two blocks' spacings); V-cycle
+ * with damped Jacobi (omega 0.8, 2+2 sweeps, 8 on the 1-cell level), summed
+ * restriction, injection prolongation; flexible CG (FCG(1)).  Serial FP64.
+ * Arrays: Fortran order (i fastest), padded (m+2)^3 with one ghost layer.
+ * ====================================================================== */
+
+typedef struct {
+    int m;
+    double *V, *Ai, *Aj, *Ak, *off;    /* per block: m^3, (m+1)m m (i fastest, face index first) ... */
+    int ng;                             /* ghost entries */
+    long *gdst, *gsrc;                  /* global padded indices (block*(m+2)^3 + cell) */
+    double *gw;
+} hlev;
+
+struct orc_heat {
+    orc_ctx *c;
+    int nlev, nb;
+    hlev L[8];
+    double dt_expl;
+};
+
+#define HP(m, i, j, k) ((long)(i) + (long)((m) + 2) * ((long)(j) + (long)((m) + 2) * (long)(k)))  /* padded, 0..m+1 */
+
+static void lev_centres(const oblock *b, int m, double *xc[3], double *xf[3]) {
+    for (int a = 0; a < 3; a++)
+        for (int i = 0; i <= m + 1; i++) {
+            xc[a][i] = (b->rng[a][0] * (double)((float)(m - i) + 0.5f) + b->rng[a][1] * (double)((float)i - 0.5f)) / (double)m;
+            if (i >= 1) xf[a][i - 1] = (b->rng[a][0] * (double)(m + 1 - i) + b->rng[a][1] * (double)(i - 1)) / (double)m;
+        }
+}
+
+orc_heat *orc_heat_create(orc_ctx *c, double kappa0, double r_ref) {
+    orc_heat *H = calloc(1, sizeof(orc_heat));
+    H->c = c; H->nb = c->nleaf;
+    int n = c->ni;
+    int l = 0;
+    double worst = 0.0;
+    for (int m = n; m >= 1; m /= 2, l++) {
+        hlev *L = &H->L[l];
+        L->m = m;
+        long M3 = (long)m * m * m, P = (long)(m + 2) * (m + 2) * (m + 2);
+        L->V = calloc((size_t)H->nb * M3, sizeof(double));
+        L->off = calloc((size_t)H->nb * M3, sizeof(double));
+        L->Ai = calloc((size_t)H->nb * (m + 1) * m * m, sizeof(double));
+        L->Aj = calloc((size_t)H->nb * (m + 1) * m * m, sizeof(double));
+        L->Ak = calloc((size_t)H->nb * (m + 1) * m * m, sizeof(double));
+        L->gdst = malloc(sizeof(long) * (size_t)H->nb * 6 * m * m);
+        L->gsrc = malloc(sizeof(long) * (size_t)H->nb * 6 * m * m);
+        L->gw = malloc(sizeof(double) * 3 * (size_t)H->nb * 6 * m * m);
+        L->ng = 0;
+        double *xc[3], *xf[3], *dc[3], *df[3];
+        for (int a = 0; a < 3; a++) {
+            xc[a] = malloc(sizeof(double) * (m + 2)); xf[a] = malloc(sizeof(double) * (m + 1));
+            dc[a] = malloc(sizeof(double) * (m + 2)); df[a] = malloc(sizeof(double) * (m + 1));
+        }
+        for (int ib = 0; ib < H->nb; ib++) {
+            oblock *b = &c->b[ib];
+            lev_centres(b, m, xc, xf);
+            double dx[3];
+            for (int a = 0; a < 3; a++) dx[a] = (b->rng[a][1] - b->rng[a][0]) / (double)m;
+            double *V = L->V + (size_t)ib * M3, *off = L->off + (size_t)ib * M3;
+            double *Ai = L->Ai + (size_t)ib * (m + 1) * m * m, *Aj = L->Aj + (size_t)ib * (m + 1) * m * m,
+                   *Ak = L->Ak + (size_t)ib * (m + 1) * m * m;
+            /* face index along its own axis first: Ai[(fi*m + j)*m + k], Aj[(i*(m+1)+fj)*m + k], Ak[(i*m+j)*(m+1)+fk] */
+            for (int i = 0; i <= m; i++) for (int j = 0; j < m; j++) for (int k = 0; k < m; k++) {
+                double x = xf[0][i] / r_ref, kap = kappa0 / (x * x * x);
+                double S = xf[0][i] * xf[0][i] * (cos(xf[1][j]) - cos(xf[1][j + 1])) * (xf[2][k + 1] - xf[2][k]);
+                double cf = kap * S / dx[0];
+                if ((i == 0 && b->bottom) || (i == m && b->top)) cf = 0.0;
+                Ai[((long)i * m + j) * m + k] = cf;
+            }
+            for (int i = 0; i < m; i++) for (int j = 0; j <= m; j++) for (int k = 0; k < m; k++) {
+                double x = xc[0][i + 1] / r_ref, kap = kappa0 / (x * x * x);
+                double S = 0.5 * (xf[0][i + 1] * xf[0][i + 1] - xf[0][i] * xf[0][i]) * (xf[2][k + 1] - xf[2][k]) * sin(xf[1][j]);
+                Aj[((long)i * (m + 1) + j) * m + k] = kap * S / (xc[0][i + 1] * dx[1]);
+            }
+            for (int i = 0; i < m; i++) for (int j = 0; j < m; j++) for (int k = 0; k <= m; k++) {
+                double x = xc[0][i + 1] / r_ref, kap = kappa0 / (x * x * x);
+                double S = 0.5 * (xf[0][i + 1] * xf[0][i + 1] - xf[0][i] * xf[0][i]) * (xf[1][j + 1] - xf[1][j]);
+                Ak[((long)i * m + j) * (m + 1) + k] = kap * S / (xc[0][i + 1] * dx[2] * sin(xc[1][j + 1]));
+            }
+            for (int i = 0; i < m; i++) for (int j = 0; j < m; j++) for (int k = 0; k < m; k++) {
+                long q = (long)i + (long)m * ((long)j + (long)m * k);
+                V[q] = 1.0 / 3.0 * (xf[0][i + 1] * xf[0][i + 1] * xf[0][i + 1] - xf[0][i] * xf[0][i] * xf[0][i]) *
+                       (cos(xf[1][j]) - cos(xf[1][j + 1])) * (xf[2][k + 1] - xf[2][k]);
+                off[q] = Ai[((long)i * m + j) * m + k] + Ai[((long)(i + 1) * m + j) * m + k] +
+                         Aj[((long)i * (m + 1) + j) * m + k] + Aj[((long)i * (m + 1) + j + 1) * m + k] +
+                         Ak[((long)i * m + j) * (m + 1) + k] + Ak[((long)i * m + j) * (m + 1) + k + 1];
+                if (l == 0) {
+                    double r = xc[0][i + 1], th = xc[1][j + 1];
+                    double xx = r / r_ref, kap = kappa0 / (xx * xx * xx);
+                    double h2 = r * dx[1], h3 = r * sin(th) * dx[2];
+                    double v = 2.0 * kap * (1.0 / (dx[0] * dx[0]) + 1.0 / (h2 * h2) + 1.0 / (h3 * h3));
+                    if (v > worst) worst = v;
+                }
+            }
+            /* face ghosts */
+            for (int I = 0; I <= m + 1; I++) for (int J = 0; J <= m + 1; J++) for (int K = 0; K <= m + 1; K++) {
+                int out = (I == 0 || I == m + 1) + (J == 0 || J == m + 1) + (K == 0 || K == m + 1);
+                if (out != 1) continue;
+                double x[3] = {xc[0][I], xc[1][J], xc[2][K]};
+                const onode *nd = find_node(c, x, b->yin);
+                if (!nd) continue;
+                int d = nd->ileaf - 1;
+                oblock *db = &c->b[d];
+                lev_centres(db, m, dc, df);
+                if (db->yin != b->yin) { double o[3]; coordconv(x, o); x[0] = o[0]; x[1] = o[1]; x[2] = o[2]; }
+                /* linear inter/extrapolation inside the donor block: p in [0.5, m+0.5],
+                 * stencil base in [1, m-1] (interior cells only); m = 1 copies the cell */
+                int base[3]; double w[3];
+                for (int a = 0; a < 3; a++) {
+                    if (m == 1) { base[a] = 1; w[a] = 0.0; continue; }
+                    double dxa = dc[a][2] - dc[a][1];
+                    double p = (x[a] - dc[a][0]) / dxa;
+                    p = p < (double)m + 0.5 ? p : (double)m + 0.5;
+                    p = p > 0.5 ? p : 0.5;
+                    int bs = (int)floor(p);
+                    bs = bs < 1 ? 1 : (bs > m - 1 ? m - 1 : bs);
+                    base[a] = bs;
+                    w[a] = p - (double)bs;
+                }
+                L->gdst[L->ng] = (long)ib * P + HP(m, I, J, K);
+                L->gsrc[L->ng] = (long)d * P + HP(m, base[0], base[1], base[2]);
+                L->gw[3 * L->ng] = w[0]; L->gw[3 * L->ng + 1] = w[1]; L->gw[3 * L->ng + 2] = w[2];
+                L->ng++;
+            }
+        }
+        for (int a = 0; a < 3; a++) { free(xc[a]); free(xf[a]); free(dc[a]); free(df[a]); }
+    }
+    H->nlev = l;
+    H->dt_expl = 1.0 / worst;
+    return H;
+}
+
+void orc_heat_destroy(orc_heat *H) {
+    if (!H) return;
+    for (int l = 0; l < H->nlev; l++) {
+        hlev *L = &H->L[l];
+        free(L->V); free(L->off); free(L->Ai); free(L->Aj); free(L->Ak); free(L->gdst); free(L->gsrc); free(L->gw);
+    }
+    free(H);
+}
+
+double orc_heat_dt_explicit(const orc_heat *H) { return H->dt_expl; }
+int orc_heat_levels(const orc_heat *H) { return H->nlev; }
+
+static void h_exchange(const hlev *L, double *x) {
+    const int m = L->m;
+    const long SJ = m + 2, SK = (long)(m + 2) * (m + 2);
+    for (int e = 0; e < L->ng; e++) {
+        const double *F = x + L->gsrc[e];
+        double wi = L->gw[3 * e], wj = L->gw[3 * e + 1], wk = L->gw[3 * e + 2];
+        double t = F[0] * (1.0 - wi) * (1.0 - wj) * (1.0 - wk);
+        t = t + F[1] * (wi) * (1.0 - wj) * (1.0 - wk);
+        t = t + F[SJ] * (1.0 - wi) * (wj) * (1.0 - wk);
+        t = t + F[SK] * (1.0 - wi) * (1.0 - wj) * (wk);
+        t = t + F[1 + SJ] * (wi) * (wj) * (1.0 - wk);
+        t = t + F[SJ + SK] * (1.0 - wi) * (wj) * (wk);
+        t = t + F[1 + SK] * (wi) * (1.0 - wj) * (wk);
+        t = t + F[1 + SJ + SK] * (wi) * (wj) * (wk);
+        x[L->gdst[e]] = t;
+    }
+}
+
+/* mode 0: y = A x; 1: y = r - A x; 2: y(padded) = x + omega (r - A x)/diag */
+static void h_apply(const hlev *L, int nb, double inv_dt, const double *x, const double *r, double *y, int mode, double omega) {
+    const int m = L->m;
+    const long M3 = (long)m * m * m, P = (long)(m + 2) * (m + 2) * (m + 2);
+    for (int b = 0; b < nb; b++) {
+        const double *X = x + (long)b * P;
+        const double *Ai = L->Ai + (size_t)b * (m + 1) * m * m, *Aj = L->Aj + (size_t)b * (m + 1) * m * m,
+                     *Ak = L->Ak + (size_t)b * (m + 1) * m * m;
+        for (int k = 0; k < m; k++) for (int j = 0; j < m; j++) for (int i = 0; i < m; i++) {
+            long q = (long)i + (long)m * ((long)j + (long)m * k);
+            long pc = HP(m, i + 1, j + 1, k + 1);
+            double cim = Ai[((long)i * m + j) * m + k], cip = Ai[((long)(i + 1) * m + j) * m + k];
+            double cjm = Aj[((long)i * (m + 1) + j) * m + k], cjp = Aj[((long)i * (m + 1) + j + 1) * m + k];
+            double ckm = Ak[((long)i * m + j) * (m + 1) + k], ckp = Ak[((long)i * m + j) * (m + 1) + k + 1];
+            double diag = L->V[(long)b * M3 + q] * inv_dt + L->off[(long)b * M3 + q];
+            double nbv = cim * X[pc - 1] + cip * X[pc + 1];
+            nbv = nbv + cjm * X[pc - (m + 2)] + cjp * X[pc + (m + 2)];
+            nbv = nbv + ckm * X[pc - (long)(m + 2) * (m + 2)] + ckp * X[pc + (long)(m + 2) * (m + 2)];
+            double ax = diag * X[pc] - nbv;
+            long io = (long)b * M3 + q;
+            if (mode == 0) y[io] = ax;
+            else if (mode == 1) y[io] = r[io] - ax;
+            else y[(long)b * P + pc] = X[pc] + omega * (r[io] - ax) / diag;
+        }
+    }
+}
+
+typedef struct { double *Z, *Z2, *R, *RES; } hwork;
+
+static void h_vcycle(orc_heat *H, hwork *W, double inv_dt, const double *r0, double *z0_out) {
+    const int nb = H->nb;
+    W[0].R = (double *)r0;
+    for (int l = 0; l < H->nlev; l++) {
+        const hlev *L = &H->L[l];
+        const int m = L->m;
+        const long P = (long)(m + 2) * (m + 2) * (m + 2), M3 = (long)m * m * m;
+        memset(W[l].Z, 0, sizeof(double) * (size_t)nb * P);
+        int sweeps = (l == H->nlev - 1) ? 8 : 2;
+        for (int s = 0; s < sweeps; s++) {
+            h_exchange(L, W[l].Z);
+            memcpy(W[l].Z2, W[l].Z, sizeof(double) * (size_t)nb * P);
+            h_apply(L, nb, inv_dt, W[l].Z, W[l].R, W[l].Z2, 2, 0.8);
+            double *t = W[l].Z; W[l].Z = W[l].Z2; W[l].Z2 = t;
+        }
+        if (l == H->nlev - 1) break;
+        h_exchange(L, W[l].Z);
+        h_apply(L, nb, inv_dt, W[l].Z, W[l].R, W[l].RES, 1, 0.0);
+        const int mc = m / 2;
+        const long Mc = (long)mc * mc * mc;
+        for (int b = 0; b < nb; b++)
+            for (int k = 0; k < mc; k++) for (int j = 0; j < mc; j++) for (int i = 0; i < mc; i++) {
+                double t = 0.0;
+                for (int di = 0; di < 2; di++) for (int dj = 0; dj < 2; dj++) for (int dk = 0; dk < 2; dk++)
+                    t += W[l].RES[(long)b * M3 + (2 * i + di) + (long)m * ((2 * j + dj) + (long)m * (2 * k + dk))];
+                W[l + 1].R[(long)b * Mc + i + (long)mc * (j + (long)mc * k)] = t;
+            }
+    }
+    for (int l = H->nlev - 2; l >= 0; l--) {
+        const hlev *L = &H->L[l];
+        const int m = L->m, mc = m / 2;
+        const long P = (long)(m + 2) * (m + 2) * (m + 2), Pc = (long)(mc + 2) * (mc + 2) * (mc + 2);
+        for (int b = 0; b < nb; b++)
+            for (int k = 0; k < m; k++) for (int j = 0; j < m; j++) for (int i = 0; i < m; i++)
+                W[l].Z[(long)b * P + HP(m, i + 1, j + 1, k + 1)] += W[l + 1].Z[(long)b * Pc + HP(mc, i / 2 + 1, j / 2 + 1, k / 2 + 1)];
+        for (int s = 0; s < 2; s++) {
+            h_exchange(L, W[l].Z);
+            memcpy(W[l].Z2, W[l].Z, sizeof(double) * (size_t)nb * P);
+            h_apply(L, nb, inv_dt, W[l].Z, W[l].R, W[l].Z2, 2, 0.8);
+            double *t = W[l].Z; W[l].Z = W[l].Z2; W[l].Z2 = t;
+        }
+    }
+    const int m = H->L[0].m;
+    memcpy(z0_out, W[0].Z, sizeof(double) * (size_t)nb * (m + 2) * (m + 2) * (m + 2));
+}
+
+/* pad/unpad helpers (interior Fortran order per block) */
+static void h_pad(int m, int nb, const double *in, double *out) {
+    const long M3 = (long)m * m * m, P = (long)(m + 2) * (m + 2) * (m + 2);
+    memset(out, 0, sizeof(double) * (size_t)nb * P);
+    for (int b = 0; b < nb; b++)
+        for (int k = 0; k < m; k++) for (int j = 0; j < m; j++) for (int i = 0; i < m; i++)
+            out[(long)b * P + HP(m, i + 1, j + 1, k + 1)] = in[(long)b * M3 + i + (long)m * (j + (long)m * k)];
+}
+
+void orc_heat_apply(orc_heat *H, double dt, const double *x, double *y) {
+    const hlev *L = &H->L[0];
+    const int m = L->m;
+    double *xp = malloc(sizeof(double) * (size_t)H->nb * (m + 2) * (m + 2) * (m + 2));
+    h_pad(m, H->nb, x, xp);
+    h_exchange(L, xp);
+    h_apply(L, H->nb, 1.0 / dt, xp, NULL, y, 0, 0.0);
+    free(xp);
+}
+
+static void h_work_alloc(orc_heat *H, hwork *W) {
+    for (int l = 0; l < H->nlev; l++) {
+        int ml = H->L[l].m;
+        long Pl = (long)(ml + 2) * (ml + 2) * (ml + 2), Ml = (long)ml * ml * ml;
+        W[l].Z = calloc((size_t)H->nb * Pl, sizeof(double)); W[l].Z2 = calloc((size_t)H->nb * Pl, sizeof(double));
+        W[l].R = l ? calloc((size_t)H->nb * Ml, sizeof(double)) : NULL; W[l].RES = calloc((size_t)H->nb * Ml, sizeof(double));
+    }
+}
+
+static void h_work_free(orc_heat *H, hwork *W) {
+    W[0].R = NULL;  /* level 0 aliases the caller's residual during the V-cycle */
+    for (int l = 0; l < H->nlev; l++) { free(W[l].Z); free(W[l].Z2); free(W[l].R); free(W[l].RES); }
+}
+
+void orc_heat_vcycle(orc_heat *H, double dt, const double *r, double *z) {
+    const int nb = H->nb, m = H->L[0].m;
+    const long M3 = (long)m * m * m, P = (long)(m + 2) * (m + 2) * (m + 2);
+    hwork W[8];
+    h_work_alloc(H, W);
+    double *zp = malloc(sizeof(double) * (size_t)nb * P);
+    h_vcycle(H, W, 1.0 / dt, r, zp);
+    for (int b = 0; b < nb; b++)
+        for (int k = 0; k < m; k++) for (int j = 0; j < m; j++) for (int i = 0; i < m; i++)
+            z[(long)b * M3 + i + (long)m * (j + (long)m * k)] = zp[(long)b * P + HP(m, i + 1, j + 1, k + 1)];
+    free(zp);
+    h_work_free(H, W);
+}
+
+static double h_dot(int m, int nb, const double *a, int apad, const double *b, int bpad) {
+    const long M3 = (long)m * m * m, P = (long)(m + 2) * (m + 2) * (m + 2);
+    double s = 0.0;
+    for (int blk = 0; blk < nb; blk++)
+        for (int k = 0; k < m; k++) for (int j = 0; j < m; j++) for (int i = 0; i < m; i++) {
+            long ii = (long)blk * M3 + i + (long)m * (j + (long)m * k);
+            long pp = (long)blk * P + HP(m, i + 1, j + 1, k + 1);
+            s += (apad ? a[pp] : a[ii]) * (bpad ? b[pp] : b[ii]);
+        }
+    return s;
+}
+
+/* Solve A T = (V/dt) T^n to ||b - A T||/||b|| <= tol by FCG(1) with the V-cycle.
+ * Returns iterations; x_out interior (Fortran order per block). */
+int orc_heat_solve(orc_heat *H, double dt, const double *Tn, double tol, int max_iter, double *x_out, double *relres) {
+    const int nb = H->nb, m = H->L[0].m;
+    const long M3 = (long)m * m * m, P = (long)(m + 2) * (m + 2) * (m + 2), N = (long)nb * M3;
+    const double inv_dt = 1.0 / dt;
+    hwork W[8];
+    h_work_alloc(H, W);
+    double *b = malloc(sizeof(double) * N), *r = malloc(sizeof(double) * N), *x = calloc(N, sizeof(double));
+    double *q = malloc(sizeof(double) * N), *p = calloc((size_t)nb * P, sizeof(double)), *z = malloc(sizeof(double) * nb * P);
+    for (long c = 0; c < N; c++) { b[c] = H->L[0].V[c] * inv_dt * Tn[c]; r[c] = b[c]; }
+    double bn = h_dot(m, nb, b, 0, b, 0);
+    if (bn == 0.0) {  /* b = 0: the solution is 0 */
+        memset(x_out, 0, sizeof(double) * N);
+        if (relres) *relres = 0.0;
+        h_work_free(H, W);
+        free(b); free(r); free(x); free(q); free(p); free(z);
+        return 0;
+    }
+    h_vcycle(H, W, inv_dt, r, z);
+    for (long c = 0; c < (long)nb * P; c++) p[c] = z[c];
+    int it = 0;
+    double rel = 1.0, gamma = 1.0;
+    while (1) {
+        h_exchange(&H->L[0], p);
+        h_apply(&H->L[0], nb, inv_dt, p, NULL, q, 0, 0.0);
+        gamma = h_dot(m, nb, p, 1, q, 0);
+        double alpha = h_dot(m, nb, p, 1, r, 0) / gamma;
+        for (int blk = 0; blk < nb; blk++)
+            for (long c = 0; c < M3; c++) {
+                int i = (int)(c % m), j = (int)((c / m) % m), k = (int)(c / ((long)m * m));
+                x[blk * M3 + c] += alpha * p[(long)blk * P + HP(m, i + 1, j + 1, k + 1)];
+                r[blk * M3 + c] -= alpha * q[blk * M3 + c];
+            }
+        it++;
+        rel = sqrt(h_dot(m, nb, r, 0, r, 0) / bn);
+        if (rel <= tol || it >= max_iter) break;
+        h_vcycle(H, W, inv_dt, r, z);
+        double beta = h_dot(m, nb, z, 1, q, 0) / gamma;
+        for (long c = 0; c < (long)nb * P; c++) p[c] = z[c] - beta * p[c];
+    }
+    memcpy(x_out, x, sizeof(double) * N);
+    if (relres) *relres = rel;
+    h_work_free(H, W);
+    free(b); free(r); free(x); free(q); free(p); free(z);
+    return it;
+}

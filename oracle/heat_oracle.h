@@ -1,4 +1,16 @@
-/* Heat-conduction oracle: declarations (heat_oracle.c). TEST INFRASTRUCTURE ONLY. */
+/* Heat-conduction oracle declarations (implemented in falcon_oracle.c).
+ * TEST INFRASTRUCTURE ONLY. */
 #ifndef FALCON_HEAT_ORACLE_H
 #define FALCON_HEAT_ORACLE_H
+typedef struct orc_heat orc_heat;
+struct orc_ctx;
+orc_heat *orc_heat_create(struct orc_ctx *c, double kappa0, double r_ref_cgs);
+void orc_heat_destroy(orc_heat *H);
+double orc_heat_dt_explicit(const orc_heat *H);
+int orc_heat_levels(const orc_heat *H);
+/* y = A x, interior arrays nblocks x (m,m,m) Fortran order */
+void orc_heat_apply(orc_heat *H, double dt, const double *x, double *y);
+/* z = M r: one V-cycle, zero initial guess; interior arrays, Fortran order per block */
+void orc_heat_vcycle(orc_heat *H, double dt, const double *r, double *z);
+int orc_heat_solve(orc_heat *H, double dt, const double *Tn, double tol, int max_iter, double *x_out, double *relres);
 #endif

@@ -55,6 +55,17 @@ def lib():
         L.orc_step.argtypes = [vp, C.c_double]
         L.orc_step.restype = C.c_double
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_heat_create.restype = vp
+        L.orc_heat_create.argtypes = [vp, C.c_double, C.c_double]
+        L.orc_heat_destroy.argtypes = [vp]
+        L.orc_heat_dt_explicit.restype = C.c_double
+        L.orc_heat_dt_explicit.argtypes = [vp]
+        L.orc_heat_levels.argtypes = [vp]
+        L.orc_heat_apply.argtypes = [vp, C.c_double, dp, dp]
+        L.orc_heat_vcycle.argtypes = [vp, C.c_double, dp, dp]
+        L.orc_heat_solve.argtypes = [vp, C.c_double, dp, C.c_double, C.c_int, dp, dp]
+        L.orc_coordconv.argtypes = [dp, dp]
+        L.orc_vecconv_coeffs.argtypes = [dp, dp]
         _lib = L
     return _lib
 
@@ -165,6 +176,52 @@ class Oracle:
 
     def step(self, cfl):
         return self.L.orc_step(self.h, cfl)
+
+
+class HeatOracle:
+    """Heat operator and MG-preconditioned FCG of the oracle on an Oracle layout
+    (cubic blocks).  Interior arrays are (nblocks, n, n, n) in C view of Fortran
+    (i fastest), i.e. numpy shape (nblocks, nk, nj, ni)."""
+
+    def __init__(self, orc, kappa0, r_ref_rsun):
+        self.L = lib()
+        self.o = orc
+        self.h = self.L.orc_heat_create(orc.h, kappa0, r_ref_rsun * float(np.float32(6.955e10)))
+        self.n = orc.ni
+        self.shape = (orc.nblocks, orc.nk, orc.nj, orc.ni)
+
+    def __del__(self):
+        try:
+            self.L.orc_heat_destroy(self.h)
+        except Exception:
+            pass
+
+    @property
+    def dt_explicit(self):
+        return self.L.orc_heat_dt_explicit(self.h)
+
+    @property
+    def levels(self):
+        return self.L.orc_heat_levels(self.h)
+
+    def apply(self, dt, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.shape)
+        self.L.orc_heat_apply(self.h, dt, _p(x, C.c_double), _p(y, C.c_double))
+        return y
+
+    def vcycle(self, dt, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros(self.shape)
+        self.L.orc_heat_vcycle(self.h, dt, _p(r, C.c_double), _p(z, C.c_double))
+        return z
+
+    def solve(self, dt, Tn, tol=1e-10, max_iter=500):
+        Tn = np.ascontiguousarray(Tn, dtype=np.float64)
+        x = np.zeros(self.shape)
+        rel = np.zeros(1)
+        it = self.L.orc_heat_solve(self.h, dt, _p(Tn, C.c_double), tol, max_iter, _p(x, C.c_double), _p(rel, C.c_double))
+        return x, it, float(rel[0])
 
 
 def set_threads(n):
