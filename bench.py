@@ -1,10 +1,14 @@
-"""Benchmark: RK4 cell-updates/s on one MI355X, FP32-deviation vs FP64.
+"""Benchmark on one MI355X: RK4 cell-updates/s (FP32-deviation vs FP64) and
+heat-solve seconds to 1e-10.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+                    [--no-heat] [--no-cpu-baseline]
 
 A "step" is one full RK4 step (4 RHS stages, artificial diffusion, 5 ghost
 fills) over every block of the workload; all blocks are resident in HBM when
-the timed region starts.  Prints one JSON line.
+the timed region starts.  `value` is the FP32-deviation path's cell-updates/s
+on the 128x256x768-per-panel shell; the FP64 path, the stage kernel's
+roofline, the heat leg and the CPU baselines ride in the same JSON line.
 """
 import argparse
 import json
@@ -17,7 +21,7 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
-from falcon_amd import Shell, configs  # noqa: E402
+from falcon_amd import HeatSolver, Shell, configs  # noqa: E402
 
 WORKLOADS = {
     "shell_128x256x768_b16": configs.SHELL_128x256x768_B16,
@@ -61,6 +65,7 @@ def main():
     ap.add_argument("--workload", default="shell_128x256x768_b16", choices=sorted(WORKLOADS))
     ap.add_argument("--cfl", type=float, default=0.5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-heat", action="store_true", help="skip the heat-solve leg")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="only run a device-to-device copy of this many GiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -86,13 +91,23 @@ def main():
         print(f"# {prec}: {res[prec]['cell_updates_per_s']:.4e} cell-updates/s, "
               f"{res[prec]['ms_per_step']:.3f} ms/step, kernels {res[prec]['kernel_ms']}", file=sys.stderr, flush=True)
 
+    shell.close()
+    heat = None
+    if not args.no_heat:
+        heat = run_heat(cpu_baseline=not args.no_cpu_baseline)
+        print(f"# heat: {heat['mg_pcg']}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = run_cpu_baseline()
 
     head = res["fp32"]
+    traffic = load_traffic(args.workload)
+    for prec in ("fp32", "fp64"):
+        t = traffic.get(prec, {}).get("stage_bytes_per_launch")
+        res[prec]["stage_roofline"]["traffic"] = t
     out = {
-        "metric": "RK4 cell-updates/s (FP32-deviation path)",
+        "metric": "RK4 cell-updates/s FP32-dev vs FP64; heat-solve s @1e-10 learned FCG vs MG-PCG",
         "value": head["cell_updates_per_s"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -113,6 +128,7 @@ def main():
         "roofline": head["stage_roofline"],
         "setup_s": setup_s,
         "cpu_baseline": cpu,
+        "heat": heat,
     }
     print(json.dumps(out))
 
@@ -131,9 +147,46 @@ def copy_calibration(gib):
 
 
 def run_cpu_baseline():
-    """Oracle FP64 steps of a 192-block 16^3 shell on the host cores (bounded sample)."""
+    """Bounded CPU sample of the same RK4 method on the host cores.
+
+    Preferred: the reference's own Fortran numerics (oracle/_ref/falcon_ref_harness,
+    compiled from the reference sources by oracle/Makefile; one process, one
+    core) stepping the 24x48x144-per-panel shell in the reference's own block
+    layout.  Fallback when that binary is absent: the FP64 C oracle with OpenMP."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
+    sys.path.insert(0, os.path.join(REPO, "tests", "golden"))
     import pyoracle
+    if os.path.exists(pyoracle.HARNESS):
+        return _cpu_baseline_reference(pyoracle)
+    return _cpu_baseline_port(pyoracle)
+
+
+def _cpu_baseline_reference(pyoracle):
+    import subprocess
+    import tempfile
+    import ref_io
+    lay = configs.SHELL_REF_24x48x144
+    s = Shell(lay)
+    s.background_polytrope(20.0)
+    s.set_gc_all()
+    s.init_synthetic(seed=12345, s_amp=1e-4, b_ratio=1e-3)
+    bgs = [s.get_background(ib) for ib in range(s.nblocks)]
+    sts = [s.get_state(ib) for ib in range(s.nblocks)]
+    s.close()
+    nsteps = 12
+    with tempfile.TemporaryDirectory() as td:
+        fi, fo = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        ref_io.write_input(fi, lay, 0.5, nsteps, [], 2, bgs, sts)
+        out = subprocess.run([pyoracle.HARNESS, fi, fo], check=True, capture_output=True, text=True).stdout
+    line = [ln for ln in out.splitlines() if ln.startswith("harness_steps")][-1].split()
+    el = float(line[3])
+    cells = len(bgs) * lay["ni"] * lay["nj"] * lay["nk"]
+    return {"value": cells * nsteps / el, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+            "sample": f"reference Fortran RK4 (flang -O2, one process), 24x48x144-per-panel shell in "
+                      f"{len(bgs)} blocks of 24x6x18, {nsteps} steps, {el:.1f} s"}
+
+
+def _cpu_baseline_port(pyoracle):
     lay = configs.cubic((32, 64, 192), 16)
     threads = min(os.cpu_count() or 1, 16)
     pyoracle.set_threads(threads)
@@ -156,6 +209,53 @@ def run_cpu_baseline():
     cells = o.nblocks * 16 ** 3
     return {"value": cells * n / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": f"oracle FP64 RK4, 32x64x192-per-panel shell in 16^3 blocks ({o.nblocks} blocks), {n} steps, {el:.1f} s"}
+
+
+HEAT_WORKLOAD = "heat_shell_64x128x384_b16"
+
+
+def run_heat(cpu_baseline=True, repeats=3):
+    """Backward-Euler heat solve to 1e-10: 64x128x384 cells per panel in 16^3
+    blocks, kappa = kappa0 (r/0.7 R_sun)^-3, dt = 100 x explicit limit, seeded
+    Gaussian T' (sigma 1e-3); MG-preconditioned FCG in FP64."""
+    lay = configs.SHELL_64x128x384_B16
+    s = Shell(lay)
+    hs = HeatSolver(s, 1.0, 0.7, 1)
+    dt = 100.0 * hs.dt_explicit
+    hs.set_dt(dt)
+    T = 1e-3 * np.random.default_rng(12345).standard_normal(hs.shape)
+    hs.set_T(T)
+    hs.solve(1e-10, 300)                    # warm-up (first launches)
+    runs = [hs.solve(1e-10, 300) for _ in range(repeats)]
+    secs = sorted(r[2] for r in runs)
+    it, rel = int(runs[0][0][0]), float(runs[0][1][0])
+    out = {"workload": HEAT_WORKLOAD, "cells": hs.cells, "levels": hs.levels, "dt_over_explicit": 100.0,
+           "tol": 1e-10,
+           "mg_pcg": {"seconds": secs[len(secs) // 2], "iterations": it, "relres": rel},
+           "learned_fcg": None}
+    if cpu_baseline:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import pyoracle
+        o = pyoracle.Oracle(lay)
+        H = pyoracle.HeatOracle(o, 1.0, 0.7)
+        t0 = time.perf_counter()
+        _, ito, relo = H.solve(dt, T[0], 1e-10, 300)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": el, "unit": "s", "cores": 1, "kind": "port", "iterations": ito,
+                               "sample": "oracle FP64 MG-preconditioned FCG, same system, one solve to 1e-10"}
+    return out
+
+
+def load_traffic(workload):
+    """HBM bytes per launch of the stage kernels from rocprofv3 --pmc runs
+    (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known size), stored
+    as data in profiles/pmc_traffic.json; None when not measured."""
+    path = os.path.join(REPO, "profiles", "pmc_traffic.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        data = json.load(f)
+    return data.get(workload, {})
 
 
 if __name__ == "__main__":

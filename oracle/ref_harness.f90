@@ -41,6 +41,7 @@ program falcon_ref_harness
     real(8)                     ::  r_range_cgs(2),rng(3,2),dt
     real(8),allocatable         ::  rtp_ranges(:,:,:),bg(:,:),st(:,:,:,:)
     integer                     ::  nstate,npad_i,npad_j,npad_k,v,ig,it
+    integer(8)                  ::  clk0,clk1,clk_rate
     type(BlockType),pointer     ::  B
 
     call get_command_argument(1,fin)
@@ -193,6 +194,7 @@ program falcon_ref_harness
         end do
 
         idump=1
+        call system_clock(clk0,clk_rate)
         do istep=1,nsteps
             call harness_rk4(Tree,dt)
             if (idump<=ndump) then
@@ -202,6 +204,9 @@ program falcon_ref_harness
                 end if
             end if
         end do
+        call system_clock(clk1)
+        ! wall time of the stepping loop (the CPU baseline reads this line)
+        write(*,'(a,i0,a,es16.9)') 'harness_steps ',nsteps,' seconds ',real(clk1-clk0,8)/real(clk_rate,8)
     end if
     close(ui); close(uo)
 
