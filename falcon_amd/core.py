@@ -31,6 +31,7 @@ def load():
         raise FalconError(f"{LIB_PATH} is missing: build it with `python -m falcon_amd.build`")
     L = C.CDLL(LIB_PATH)
     vp, ip, dp, i64p = C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_int64)
+    fp = C.POINTER(C.c_float)
     sig = {
         "falcon_create": [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double],
         "falcon_destroy": [vp],
@@ -70,6 +71,15 @@ def load():
         "falcon_heat_get_T": [vp, C.c_int, dp],
         "falcon_heat_apply": [vp, dp, dp],
         "falcon_heat_vcycle": [vp, dp, dp],
+        "falcon_cnn_init": [vp, C.c_uint64],
+        "falcon_cnn_set_params": [vp, fp],
+        "falcon_cnn_get_params": [vp, fp],
+        "falcon_cnn_forward": [vp, C.c_int64, fp, fp],
+        "falcon_cnn_gradient": [vp, C.c_int, fp, fp, dp],
+        "falcon_cnn_train": [vp, C.c_int64, fp, fp, C.c_int, C.c_int, C.c_double, C.c_uint64, dp],
+        "falcon_heat_train_corrector": [vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_double,
+                                        C.c_uint64, dp, i64p, dp],
+        "falcon_heat_attach_corrector": [vp],
         "falcon_last_error": [C.c_char_p, C.c_int],
     }
     for name, args in sig.items():
@@ -306,3 +316,59 @@ class HeatSolver:
         z = np.zeros(self.shape)
         _check(self.L.falcon_heat_vcycle(self.s.h, _p(r), _p(z)))
         return z
+
+    def train_corrector(self, npairs=2000, dt_range=(10.0, 300.0), max_solves=50, epochs=10, batch=128, lr=2e-3,
+                        seed=1):
+        """Train the learned V-cycle correction on pairs generated by solves of
+        this solver's systems and attach it; returns (loss, pairs, seconds)."""
+        loss, n, sec = C.c_double(), C.c_int64(), C.c_double()
+        _check(self.L.falcon_heat_train_corrector(self.s.h, npairs, dt_range[0], dt_range[1], max_solves, epochs, batch,
+                                                  lr, seed, C.byref(loss), C.byref(n), C.byref(sec)))
+        return loss.value, n.value, sec.value
+
+    def attach_corrector(self):
+        _check(self.L.falcon_heat_attach_corrector(self.s.h))
+
+
+class Cnn:
+    """The learned correction's network of a Shell's handle (forward, gradient,
+    training on given images).  Images are float32 arrays (n, 16, 16, 16)."""
+    NPARAM = 7809
+
+    def __init__(self, shell, seed=None):
+        self.s, self.L = shell, shell.L
+        if seed is not None:
+            _check(self.L.falcon_cnn_init(shell.h, seed))
+
+    @property
+    def params(self):
+        p = np.zeros(self.NPARAM, np.float32)
+        _check(self.L.falcon_cnn_get_params(self.s.h, _p(p, C.c_float)))
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.NPARAM
+        _check(self.L.falcon_cnn_set_params(self.s.h, _p(p, C.c_float)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        out = np.zeros_like(x)
+        _check(self.L.falcon_cnn_forward(self.s.h, x.shape[0], _p(x, C.c_float), _p(out, C.c_float)))
+        return out
+
+    def gradient(self, x, gout):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        gout = np.ascontiguousarray(gout, dtype=np.float32)
+        g = np.zeros(self.NPARAM)
+        _check(self.L.falcon_cnn_gradient(self.s.h, x.shape[0], _p(x, C.c_float), _p(gout, C.c_float), _p(g)))
+        return g
+
+    def train(self, x, y, epochs=1, batch=64, lr=1e-3, seed=0):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        loss = C.c_double()
+        _check(self.L.falcon_cnn_train(self.s.h, x.shape[0], _p(x, C.c_float), _p(y, C.c_float), epochs, batch, lr,
+                                       seed, C.byref(loss)))
+        return loss.value

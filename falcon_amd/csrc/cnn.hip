@@ -1,0 +1,644 @@
+// Learned V-cycle correction network on MFMA (gfx950): forward (fused, all
+// three layers in LDS), data and weight gradients (implicit GEMM on
+// v_mfma_f32_16x16x32_bf16), mean-squared-error loss and Adam.
+//
+// MFMA operand images (16x16x32, 64 lanes, lane l = 16 g + n):
+//   A (16 x 32): lane l holds A[n][8g .. 8g+7]
+//   B (32 x 16): lane l holds B[8g .. 8g+7][n]
+//   D (16 x 16): lane l holds D[4g .. 4g+3][n]
+// A convolution is computed as D = W^T X: rows = output channels, columns =
+// 16 voxels of one k-line, K = (tap, input channel) pairs, so each lane ends
+// with 4 consecutive output channels of one voxel (one 8-byte store of
+// channel-fastest activations).
+//
+// Layouts: images x/out [img][4096] float; activations and their gradients
+// [img][4096][16] bfloat16 (channel fastest); voxel v = (i*16 + j)*16 + k.
+// LDS planes carry a zero halo: [18][18] (x) or [18][18][16] (activations).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cnn.hpp"
+
+namespace falcon {
+
+#define CCHECK(x)                                                                                     \
+    do {                                                                                              \
+        hipError_t e_ = (x);                                                                          \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+typedef unsigned short us8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int PL = 18 * 18;                 // plane with halo
+constexpr int OFF_W1 = 0, OFF_B1 = 432, OFF_W2 = 448, OFF_B2 = 7360, OFF_W3 = 7376, OFF_B3 = 7808;
+// packed operand images, 64 lanes x 8 bf16 per MFMA step:
+// P1 layer-1 forward (1 step), P2 layer-2 forward (14), P3 layer-3 forward (14),
+// Q3 layer-3 data gradient (1), Q2 layer-2 data gradient (14)
+constexpr int PK_P1 = 0, PK_P2 = 64, PK_P3 = PK_P2 + 14 * 64, PK_Q3 = PK_P3 + 14 * 64, PK_Q2 = PK_Q3 + 64,
+              PK_TOTAL = PK_Q2 + 14 * 64;
+
+// fused forward LDS carve (uint16 elements)
+constexpr int F_H1 = 0, F_H2 = 3 * PL * 16, F_Z = 6 * PL * 16, F_X = 7 * PL * 16, F_TOTAL = F_X + 3 * PL;
+// single-layer LDS carve: ring of 3 input planes + zero plane
+constexpr int S_IN = 0, S_Z = 3 * PL * 16, S_TOTAL = 4 * PL * 16;
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u = u + 0x7fffu + ((u >> 16) & 1u);   // round to nearest, ties to even
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ f4 mfma(us8 a, us8 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0,
+                                                   0);
+}
+__device__ __forceinline__ void tap_off(int t, int &di, int &dj, int &dk) {
+    di = t / 9 - 1;
+    dj = (t / 3) % 3 - 1;
+    dk = t % 3 - 1;
+}
+__device__ __forceinline__ uint2 pack4(float a, float b, float c, float d) {
+    return make_uint2((uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16), (uint32_t)f2bf(c) | ((uint32_t)f2bf(d) << 16));
+}
+__device__ __forceinline__ float relu(float a) { return a > 0.0f ? a : 0.0f; }
+
+// ------------------------------------------------------------------ packing
+__global__ void __launch_bounds__(256) k_cnn_pack(const float *__restrict__ p, uint16_t *__restrict__ pk) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= PK_TOTAL * 8) return;
+    const int e = q & 7, entry = q >> 3, lane = entry & 63, row = lane & 15, g = lane >> 4;
+    float w = 0.0f;
+    if (entry < PK_P2) {                                   // P1: k = tap, row = co
+        const int t = 8 * g + e;
+        if (t < 27) w = p[OFF_W1 + t * 16 + row];
+    } else if (entry < PK_P3) {                            // P2: k = (t, ci), row = co
+        const int k = 32 * ((entry - PK_P2) >> 6) + 8 * g + e, t = k >> 4, c = k & 15;
+        if (t < 27) w = p[OFF_W2 + (t * 16 + c) * 16 + row];
+    } else if (entry < PK_Q3) {                            // P3: k = (t, ci), row 0 = the output
+        const int k = 32 * ((entry - PK_P3) >> 6) + 8 * g + e, t = k >> 4, c = k & 15;
+        if (t < 27 && row == 0) w = p[OFF_W3 + t * 16 + c];
+    } else if (entry < PK_Q2) {                            // Q3: k = tap (flipped), row = ci of layer 3
+        const int t = 8 * g + e;
+        if (t < 27) w = p[OFF_W3 + (26 - t) * 16 + row];
+    } else {                                               // Q2: k = (t flipped, co), row = ci of layer 2
+        const int k = 32 * ((entry - PK_Q2) >> 6) + 8 * g + e, t = k >> 4, co = k & 15;
+        if (t < 27) w = p[OFF_W2 + ((26 - t) * 16 + row) * 16 + co];
+    }
+    pk[q] = f2bf(w);
+}
+
+// ------------------------------------------------------------------ fused forward
+// One workgroup per image; march over the 16 i-planes keeping rings of 3 planes
+// of x, h1 and h2 in LDS: step n loads x plane n+1, computes h1 plane n, h2
+// plane n-1 and the output plane n-2.  STORE also writes h1/h2 (training).
+template <bool STORE>
+__global__ void __launch_bounds__(256) k_cnn_fwd(const float *__restrict__ x, float *__restrict__ out,
+                                                 const us8 *__restrict__ pk, const float *__restrict__ prm,
+                                                 uint16_t *__restrict__ h1g, uint16_t *__restrict__ h2g) {
+    __shared__ __align__(16) uint16_t sm[F_TOTAL];
+    const long long img = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, kk = lane & 15;
+    for (int q = tid; q < F_TOTAL / 2; q += 256) reinterpret_cast<uint32_t *>(sm)[q] = 0u;
+
+    const us8 w1 = pk[PK_P1 + lane];
+    us8 w2[14], w3[14];
+#pragma unroll
+    for (int s = 0; s < 14; s++) {
+        w2[s] = pk[PK_P2 + s * 64 + lane];
+        w3[s] = pk[PK_P3 + s * 64 + lane];
+    }
+    float bb1[4], bb2[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        bb1[r] = prm[OFF_B1 + 4 * g + r];
+        bb2[r] = prm[OFF_B2 + 4 * g + r];
+    }
+    const float bb3 = prm[OFF_B3];
+    const float *X = x + img * 4096;
+    __syncthreads();
+
+    auto load_x = [&](int q) {
+        const int j = tid >> 4, k = tid & 15;
+        sm[F_X + (q % 3) * PL + (j + 1) * 18 + k + 1] = f2bf(X[(q * 16 + j) * 16 + k]);
+    };
+    auto xplane = [&](int q) -> const uint16_t * { return (q < 0 || q > 15) ? sm + F_Z : sm + F_X + (q % 3) * PL; };
+    auto hplane = [&](int base, int q) -> const uint16_t * {
+        return (q < 0 || q > 15) ? sm + F_Z : sm + base + (q % 3) * PL * 16;
+    };
+
+    load_x(0);
+    for (int n = 0; n <= 17; n++) {
+        if (n + 1 <= 15) load_x(n + 1);
+        __syncthreads();
+        if (n <= 15) {                          // h1 plane n
+            const int p = n;
+#pragma unroll
+            for (int jl = 0; jl < 4; jl++) {
+                const int j = wave + 4 * jl;
+                us8 b;
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const int t = 8 * g + e;
+                    uint16_t val = 0;
+                    if (t < 27) {
+                        int di, dj, dk;
+                        tap_off(t, di, dj, dk);
+                        val = xplane(p + di)[(j + dj + 1) * 18 + kk + dk + 1];
+                    }
+                    b[e] = val;
+                }
+                const f4 acc = mfma(w1, b, f4{0.0f, 0.0f, 0.0f, 0.0f});
+                const uint2 o = pack4(relu(acc[0] + bb1[0]), relu(acc[1] + bb1[1]), relu(acc[2] + bb1[2]),
+                                      relu(acc[3] + bb1[3]));
+                *reinterpret_cast<uint2 *>(sm + F_H1 + (p % 3) * PL * 16 + ((j + 1) * 18 + kk + 1) * 16 + 4 * g) = o;
+                if (STORE) *reinterpret_cast<uint2 *>(h1g + ((img * 4096 + (p * 16 + j) * 16 + kk) * 16 + 4 * g)) = o;
+            }
+        }
+        __syncthreads();
+        if (n >= 1 && n <= 16) {                // h2 plane n-1
+            const int p = n - 1;
+#pragma unroll
+            for (int jl = 0; jl < 4; jl++) {
+                const int j = wave + 4 * jl;
+                f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int s = 0; s < 14; s++) {
+                    const int t = 2 * s + (g >> 1);
+                    const uint16_t *src = sm + F_Z;
+                    if (t < 27) {
+                        int di, dj, dk;
+                        tap_off(t, di, dj, dk);
+                        src = hplane(F_H1, p + di) + ((j + dj + 1) * 18 + kk + dk + 1) * 16 + 8 * (g & 1);
+                    }
+                    acc = mfma(w2[s], *reinterpret_cast<const us8 *>(src), acc);
+                }
+                const uint2 o = pack4(relu(acc[0] + bb2[0]), relu(acc[1] + bb2[1]), relu(acc[2] + bb2[2]),
+                                      relu(acc[3] + bb2[3]));
+                *reinterpret_cast<uint2 *>(sm + F_H2 + (p % 3) * PL * 16 + ((j + 1) * 18 + kk + 1) * 16 + 4 * g) = o;
+                if (STORE) *reinterpret_cast<uint2 *>(h2g + ((img * 4096 + (p * 16 + j) * 16 + kk) * 16 + 4 * g)) = o;
+            }
+        }
+        __syncthreads();
+        if (n >= 2) {                           // output plane n-2
+            const int p = n - 2;
+#pragma unroll
+            for (int jl = 0; jl < 4; jl++) {
+                const int j = wave + 4 * jl;
+                f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int s = 0; s < 14; s++) {
+                    const int t = 2 * s + (g >> 1);
+                    const uint16_t *src = sm + F_Z;
+                    if (t < 27) {
+                        int di, dj, dk;
+                        tap_off(t, di, dj, dk);
+                        src = hplane(F_H2, p + di) + ((j + dj + 1) * 18 + kk + dk + 1) * 16 + 8 * (g & 1);
+                    }
+                    acc = mfma(w3[s], *reinterpret_cast<const us8 *>(src), acc);
+                }
+                if (g == 0) out[img * 4096 + (p * 16 + j) * 16 + kk] = acc[0] + bb3;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ data gradients
+// gout[v][c] = (mask[v][c] > 0) ? sum_{t,ci} Wq[t][ci][c] gin[v + off_t][ci] : 0, stored as bfloat16.
+// CIN = 1: gin is the float output gradient (layer 3, weights Q3); CIN = 16: gin is
+// the bfloat16 layer-2 output gradient (weights Q2).
+template <int CIN>
+__global__ void __launch_bounds__(256) k_cnn_dgrad(const void *__restrict__ gin_, const uint16_t *__restrict__ mask,
+                                                   uint16_t *__restrict__ gout, const us8 *__restrict__ pk) {
+    __shared__ __align__(16) uint16_t sm[S_TOTAL];
+    const long long img = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, kk = lane & 15;
+    constexpr int S = CIN == 1 ? 1 : 14;
+    constexpr int STRIDE = CIN == 1 ? 1 : 16;   // elements per voxel in an LDS plane
+    for (int q = tid; q < S_TOTAL / 2; q += 256) reinterpret_cast<uint32_t *>(sm)[q] = 0u;
+    us8 w[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) w[s] = pk[(CIN == 1 ? PK_Q3 : PK_Q2) + s * 64 + lane];
+    __syncthreads();
+
+    auto load = [&](int q) {
+        const int j = tid >> 4, k = tid & 15;
+        const long long v = img * 4096 + (q * 16 + j) * 16 + k;
+        uint16_t *dst = sm + S_IN + (q % 3) * PL * STRIDE + ((j + 1) * 18 + k + 1) * STRIDE;
+        if (CIN == 1) {
+            dst[0] = f2bf(reinterpret_cast<const float *>(gin_)[v]);
+        } else {
+            const uint4 *src = reinterpret_cast<const uint4 *>(reinterpret_cast<const uint16_t *>(gin_) + v * 16);
+            reinterpret_cast<uint4 *>(dst)[0] = src[0];
+            reinterpret_cast<uint4 *>(dst)[1] = src[1];
+        }
+    };
+    auto plane = [&](int q) -> const uint16_t * {
+        return (q < 0 || q > 15) ? sm + S_Z : sm + S_IN + (q % 3) * PL * STRIDE;
+    };
+
+    load(0);
+    for (int p = 0; p <= 15; p++) {
+        if (p + 1 <= 15) load(p + 1);
+        __syncthreads();
+#pragma unroll
+        for (int jl = 0; jl < 4; jl++) {
+            const int j = wave + 4 * jl;
+            f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (CIN == 1) {
+                us8 b;
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const int t = 8 * g + e;
+                    uint16_t val = 0;
+                    if (t < 27) {
+                        int di, dj, dk;
+                        tap_off(t, di, dj, dk);
+                        val = plane(p + di)[(j + dj + 1) * 18 + kk + dk + 1];
+                    }
+                    b[e] = val;
+                }
+                acc = mfma(w[0], b, acc);
+            } else {
+#pragma unroll
+                for (int s = 0; s < S; s++) {
+                    const int t = 2 * s + (g >> 1);
+                    const uint16_t *src = sm + S_Z;
+                    if (t < 27) {
+                        int di, dj, dk;
+                        tap_off(t, di, dj, dk);
+                        src = plane(p + di) + ((j + dj + 1) * 18 + kk + dk + 1) * 16 + 8 * (g & 1);
+                    }
+                    acc = mfma(w[s], *reinterpret_cast<const us8 *>(src), acc);
+                }
+            }
+            const long long o = (img * 4096 + (p * 16 + j) * 16 + kk) * 16 + 4 * g;
+            const uint2 mk = *reinterpret_cast<const uint2 *>(mask + o);
+            const float m0 = bf2f((uint16_t)(mk.x & 0xffffu)), m1 = bf2f((uint16_t)(mk.x >> 16)),
+                        m2 = bf2f((uint16_t)(mk.y & 0xffffu)), m3 = bf2f((uint16_t)(mk.y >> 16));
+            *reinterpret_cast<uint2 *>(gout + o) = pack4(m0 > 0.0f ? acc[0] : 0.0f, m1 > 0.0f ? acc[1] : 0.0f,
+                                                         m2 > 0.0f ? acc[2] : 0.0f, m3 > 0.0f ? acc[3] : 0.0f);
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ weight gradients
+// Per image, into part[img][kCnnParams] (the parameter layout):
+//   L = 3: dW3[t][ci] = sum_v h2[v+t][ci] g3[v]      (in = h2, g = g3 float)
+//   L = 2: dW2[t][ci][co] = sum_v h1[v+t][ci] g2[v][co]
+//   L = 1: dW1[t][co] = sum_v x[v+t] g1[v][co]        (in = x float)
+// and the bias gradient of the layer, sum_v g[v][co].
+// D[co][n] = sum over 32-voxel chunks of A[co][voxel] B[voxel][n]; n = ci
+// (L = 2, 3; taps spread over the 4 waves) or n = tap (L = 1; wave w < 2 holds
+// taps 16w .. 16w+15).
+template <int L>
+__global__ void __launch_bounds__(256) k_cnn_wgrad(const void *__restrict__ in_, const void *__restrict__ g_,
+                                                   float *__restrict__ part) {
+    constexpr int ISTR = L == 1 ? 1 : 16;       // input elements per voxel
+    constexpr int GCH = L == 3 ? 1 : 16;        // gradient channels
+    __shared__ __align__(16) uint16_t sIn[4 * PL * ISTR];   // ring of 3 + zero plane
+    __shared__ __align__(16) uint16_t sG[256 * GCH];        // gradient plane, [j][k][c]
+    __shared__ float sBias[16][16];
+    const long long img = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, n = lane & 15;
+    for (int q = tid; q < 4 * PL * ISTR / 2; q += 256) reinterpret_cast<uint32_t *>(sIn)[q] = 0u;
+    const uint16_t *zp = sIn + 3 * PL * ISTR;
+
+    auto load_in = [&](int q) {
+        const int j = tid >> 4, k = tid & 15;
+        const long long v = img * 4096 + (q * 16 + j) * 16 + k;
+        uint16_t *dst = sIn + (q % 3) * PL * ISTR + ((j + 1) * 18 + k + 1) * ISTR;
+        if (L == 1) {
+            dst[0] = f2bf(reinterpret_cast<const float *>(in_)[v]);
+        } else {
+            const uint4 *src = reinterpret_cast<const uint4 *>(reinterpret_cast<const uint16_t *>(in_) + v * 16);
+            reinterpret_cast<uint4 *>(dst)[0] = src[0];
+            reinterpret_cast<uint4 *>(dst)[1] = src[1];
+        }
+    };
+    auto plane = [&](int q) -> const uint16_t * { return (q < 0 || q > 15) ? zp : sIn + (q % 3) * PL * ISTR; };
+
+    f4 acc[7];
+#pragma unroll
+    for (int a = 0; a < 7; a++) acc[a] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    float bias_acc = 0.0f;    // thread tid: channel tid & 15, voxels of its plane rows (tid >> 4)
+    __syncthreads();
+    load_in(0);
+    for (int p = 0; p <= 15; p++) {
+        if (p + 1 <= 15) load_in(p + 1);
+        {   // gradient plane p
+            const int j = tid >> 4, k = tid & 15;
+            const long long v = img * 4096 + (p * 16 + j) * 16 + k;
+            if (L == 3) {
+                sG[tid] = f2bf(reinterpret_cast<const float *>(g_)[v]);
+            } else {
+                const uint4 *src = reinterpret_cast<const uint4 *>(reinterpret_cast<const uint16_t *>(g_) + v * 16);
+                reinterpret_cast<uint4 *>(sG + tid * 16)[0] = src[0];
+                reinterpret_cast<uint4 *>(sG + tid * 16)[1] = src[1];
+            }
+        }
+        __syncthreads();
+        {   // bias: thread (c = tid & 15, row = tid >> 4) sums its row of 16 voxels
+            const int c = tid & 15, jr = tid >> 4;
+            if (L == 3) {
+                if (c == 0)
+                    for (int k = 0; k < 16; k++) bias_acc += reinterpret_cast<const float *>(g_)[img * 4096 + (p * 16 + jr) * 16 + k];
+            } else {
+                for (int k = 0; k < 16; k++) bias_acc += bf2f(sG[(jr * 16 + k) * 16 + c]);
+            }
+        }
+#pragma unroll 1
+        for (int ch = 0; ch < 8; ch++) {          // 32-voxel chunk: lines 2ch, 2ch+1
+            const int jj = 2 * ch + (g >> 1), k0 = 8 * (g & 1);
+            us8 a;                                 // A[co = n][voxels 8g..8g+7]
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int vv = jj * 16 + k0 + e;
+                a[e] = (L == 3) ? (n == 0 ? sG[vv] : (uint16_t)0) : sG[vv * 16 + n];
+            }
+            if (L == 1) {
+                if (wave < 2) {
+                    const int t = 16 * wave + n;
+                    us8 b;
+#pragma unroll
+                    for (int e = 0; e < 8; e++) {
+                        uint16_t val = 0;
+                        if (t < 27) {
+                            int di, dj, dk;
+                            tap_off(t, di, dj, dk);
+                            val = plane(p + di)[(jj + dj + 1) * 18 + k0 + e + dk + 1];
+                        }
+                        b[e] = val;
+                    }
+                    acc[0] = mfma(a, b, acc[0]);
+                }
+            } else {
+#pragma unroll
+                for (int a7 = 0; a7 < 7; a7++) {
+                    const int t = wave + 4 * a7;
+                    if (t < 27) {
+                        int di, dj, dk;
+                        tap_off(t, di, dj, dk);
+                        const uint16_t *P = plane(p + di);
+                        us8 b;
+#pragma unroll
+                        for (int e = 0; e < 8; e++) b[e] = P[((jj + dj + 1) * 18 + k0 + e + dk + 1) * 16 + n];
+                        acc[a7] = mfma(a, b, acc[a7]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    float *P = part + img * (long long)kCnnParams;
+    // D[co = 4g + r][n]
+    if (L == 1) {
+        if (wave < 2) {
+            const int t = 16 * wave + n;
+            if (t < 27)
+                for (int r = 0; r < 4; r++) P[OFF_W1 + t * 16 + 4 * g + r] = acc[0][r];
+        }
+    } else {
+        for (int a7 = 0; a7 < 7; a7++) {
+            const int t = wave + 4 * a7;
+            if (t >= 27) continue;
+            if (L == 2) {
+                for (int r = 0; r < 4; r++) P[OFF_W2 + (t * 16 + n) * 16 + 4 * g + r] = acc[a7][r];
+            } else if (g == 0) {
+                P[OFF_W3 + t * 16 + n] = acc[a7][0];
+            }
+        }
+    }
+    sBias[tid >> 4][tid & 15] = bias_acc;
+    __syncthreads();
+    if (tid < 16) {
+        float s = 0.0f;
+        for (int r = 0; r < 16; r++) s += sBias[r][tid];
+        if (L == 1) P[OFF_B1 + tid] = s;
+        else if (L == 2) P[OFF_B2 + tid] = s;
+        else if (tid == 0) P[OFF_B3] = s;
+    }
+}
+
+// grad[q] = sum over images of part[img][q], fixed order, FP64
+__global__ void __launch_bounds__(256) k_cnn_reduce(const float *__restrict__ part, int nimg, double *__restrict__ grad) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= kCnnParams) return;
+    double s = 0.0;
+    for (int i = 0; i < nimg; i++) s += (double)part[(long long)i * kCnnParams + q];
+    grad[q] = s;
+}
+
+// g3 = 2 (out - y) / N ; loss partials sum (out - y)^2
+__global__ void __launch_bounds__(256) k_cnn_loss(const float *__restrict__ out, const float *__restrict__ y, long long n,
+                                                  float *__restrict__ g3, double *__restrict__ part) {
+    __shared__ double red[4];
+    double acc = 0.0;
+    const float sc = (float)(2.0 / (double)n);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const float d = out[i] - y[i];
+        g3[i] = sc * d;
+        acc += (double)d * (double)d;
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+__global__ void k_cnn_sum(const double *__restrict__ part, int n, double scale, double *__restrict__ out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += part[i];
+    *out = s * scale;
+}
+
+__global__ void __launch_bounds__(256) k_cnn_adam(float *__restrict__ p, float *__restrict__ m, float *__restrict__ v,
+                                                  const double *__restrict__ grad, float lr, float c1, float c2) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= kCnnParams) return;
+    const float g = (float)grad[q];
+    const float mm = 0.9f * m[q] + 0.1f * g, vv = 0.999f * v[q] + 0.001f * g * g;
+    m[q] = mm;
+    v[q] = vv;
+    p[q] = p[q] - lr * (mm * c1) / (sqrtf(vv * c2) + 1e-8f);
+}
+
+// batch gather: dst[b][4096] = src[idx[b]][4096]
+__global__ void __launch_bounds__(256) k_cnn_gather(const float *__restrict__ src, const long long *__restrict__ idx,
+                                                    float *__restrict__ dst) {
+    const long long b = blockIdx.x;
+    const long long s = idx[b];
+    for (int v = threadIdx.x; v < 4096; v += 256) dst[b * 4096 + v] = src[s * 4096 + v];
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host class
+
+CnnModel::CnnModel() {
+    CCHECK(hipMalloc(&params_, sizeof(float) * kCnnParams));
+    CCHECK(hipMalloc(&adam_m_, sizeof(float) * kCnnParams));
+    CCHECK(hipMalloc(&adam_v_, sizeof(float) * kCnnParams));
+    CCHECK(hipMalloc(&grad_, sizeof(double) * kCnnParams));
+    CCHECK(hipMalloc(&packed_, sizeof(uint16_t) * PK_TOTAL * 8));
+    CCHECK(hipMalloc(&loss_part_, sizeof(double) * 1025));
+    init(12345);
+}
+
+CnnModel::~CnnModel() {
+    for (void *p : {(void *)params_, (void *)adam_m_, (void *)adam_v_, (void *)grad_, packed_, (void *)loss_part_,
+                    (void *)h1_, (void *)h2_, (void *)g2_, (void *)g1_, (void *)out_, (void *)g3_, (void *)part_,
+                    (void *)xb_, (void *)yb_})
+        if (p) (void)hipFree(p);
+}
+
+void CnnModel::init(uint64_t seed) {
+    std::mt19937_64 rng(seed);
+    std::vector<float> p(kCnnParams, 0.0f);
+    auto fill = [&](int off, int n, double std) {
+        std::normal_distribution<double> nd(0.0, std);
+        for (int i = 0; i < n; i++) p[off + i] = (float)nd(rng);
+    };
+    fill(OFF_W1, 432, std::sqrt(2.0 / 27.0));
+    fill(OFF_W2, 6912, std::sqrt(2.0 / 432.0));
+    fill(OFF_W3, 432, std::sqrt(1.0 / 432.0));
+    set_params(p.data());
+}
+
+void CnnModel::set_params(const float *p) {
+    CCHECK(hipMemcpy(params_, p, sizeof(float) * kCnnParams, hipMemcpyHostToDevice));
+    std::vector<float> z(kCnnParams, 0.0f);
+    CCHECK(hipMemcpy(adam_m_, z.data(), sizeof(float) * kCnnParams, hipMemcpyHostToDevice));
+    CCHECK(hipMemcpy(adam_v_, z.data(), sizeof(float) * kCnnParams, hipMemcpyHostToDevice));
+    adam_t_ = 0;
+    packed_valid_ = false;
+}
+
+void CnnModel::get_params(float *p) const {
+    CCHECK(hipDeviceSynchronize());
+    CCHECK(hipMemcpy(p, params_, sizeof(float) * kCnnParams, hipMemcpyDeviceToHost));
+}
+
+void CnnModel::pack(hipStream_t stream) {
+    if (packed_valid_) return;
+    hipLaunchKernelGGL(k_cnn_pack, dim3((PK_TOTAL * 8 + 255) / 256), dim3(256), 0, stream, (const float *)params_,
+                       (uint16_t *)packed_);
+    packed_valid_ = true;
+}
+
+void CnnModel::ensure_batch(int nimg) {
+    if (nimg <= max_batch_) return;
+    for (void *p : {(void *)h1_, (void *)h2_, (void *)g2_, (void *)g1_, (void *)out_, (void *)g3_, (void *)part_,
+                    (void *)xb_, (void *)yb_})
+        if (p) CCHECK(hipFree(p));
+    const size_t n = (size_t)nimg;
+    CCHECK(hipMalloc(&h1_, n * 4096 * 16 * sizeof(uint16_t)));
+    CCHECK(hipMalloc(&h2_, n * 4096 * 16 * sizeof(uint16_t)));
+    CCHECK(hipMalloc(&g2_, n * 4096 * 16 * sizeof(uint16_t)));
+    CCHECK(hipMalloc(&g1_, n * 4096 * 16 * sizeof(uint16_t)));
+    CCHECK(hipMalloc(&out_, n * 4096 * sizeof(float)));
+    CCHECK(hipMalloc(&g3_, n * 4096 * sizeof(float)));
+    CCHECK(hipMalloc(&xb_, n * 4096 * sizeof(float)));
+    CCHECK(hipMalloc(&yb_, n * 4096 * sizeof(float)));
+    CCHECK(hipMalloc(&part_, n * kCnnParams * sizeof(float)));
+    max_batch_ = nimg;
+}
+
+void CnnModel::infer(const float *x, float *out, long long nimg, hipStream_t stream) {
+    if (nimg <= 0) return;
+    pack(stream);
+    hipLaunchKernelGGL((k_cnn_fwd<false>), dim3((unsigned)nimg), dim3(256), 0, stream, x, out, (const us8 *)packed_,
+                       (const float *)params_, (uint16_t *)nullptr, (uint16_t *)nullptr);
+}
+
+void CnnModel::backward(const float *x, int nimg, hipStream_t stream) {
+    hipLaunchKernelGGL((k_cnn_dgrad<1>), dim3(nimg), dim3(256), 0, stream, (const void *)g3_, (const uint16_t *)h2_, g2_,
+                       (const us8 *)packed_);
+    hipLaunchKernelGGL((k_cnn_dgrad<16>), dim3(nimg), dim3(256), 0, stream, (const void *)g2_, (const uint16_t *)h1_, g1_,
+                       (const us8 *)packed_);
+    hipLaunchKernelGGL((k_cnn_wgrad<3>), dim3(nimg), dim3(256), 0, stream, (const void *)h2_, (const void *)g3_, part_);
+    hipLaunchKernelGGL((k_cnn_wgrad<2>), dim3(nimg), dim3(256), 0, stream, (const void *)h1_, (const void *)g2_, part_);
+    hipLaunchKernelGGL((k_cnn_wgrad<1>), dim3(nimg), dim3(256), 0, stream, (const void *)x, (const void *)g1_, part_);
+    hipLaunchKernelGGL(k_cnn_reduce, dim3((kCnnParams + 255) / 256), dim3(256), 0, stream, (const float *)part_, nimg,
+                       grad_);
+}
+
+void CnnModel::gradient(const float *x, const float *gout, int nimg, double *grad_host, hipStream_t stream) {
+    ensure_batch(nimg);
+    pack(stream);
+    hipLaunchKernelGGL((k_cnn_fwd<true>), dim3(nimg), dim3(256), 0, stream, x, out_, (const us8 *)packed_,
+                       (const float *)params_, h1_, h2_);
+    CCHECK(hipMemcpyAsync(g3_, gout, sizeof(float) * 4096 * (size_t)nimg, hipMemcpyDeviceToDevice, stream));
+    backward(x, nimg, stream);
+    CCHECK(hipStreamSynchronize(stream));
+    CCHECK(hipMemcpy(grad_host, grad_, sizeof(double) * kCnnParams, hipMemcpyDeviceToHost));
+}
+
+double CnnModel::train_step(const float *x, const float *y, int nimg, double lr, hipStream_t stream) {
+    ensure_batch(nimg);
+    pack(stream);
+    hipLaunchKernelGGL((k_cnn_fwd<true>), dim3(nimg), dim3(256), 0, stream, x, out_, (const us8 *)packed_,
+                       (const float *)params_, h1_, h2_);
+    const long long n = 4096LL * nimg;
+    hipLaunchKernelGGL(k_cnn_loss, dim3(1024), dim3(256), 0, stream, (const float *)out_, y, n, g3_, loss_part_);
+    hipLaunchKernelGGL(k_cnn_sum, dim3(1), dim3(64), 0, stream, (const double *)loss_part_, 1024, 1.0 / (double)n,
+                       loss_part_ + 1024);
+    backward(x, nimg, stream);
+    adam_t_++;
+    const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, (double)adam_t_)));
+    const float c2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)adam_t_)));
+    hipLaunchKernelGGL(k_cnn_adam, dim3((kCnnParams + 255) / 256), dim3(256), 0, stream, params_, adam_m_, adam_v_,
+                       (const double *)grad_, (float)lr, c1, c2);
+    packed_valid_ = false;
+    double loss = 0.0;
+    CCHECK(hipMemcpyAsync(&loss, loss_part_ + 1024, sizeof(double), hipMemcpyDeviceToHost, stream));
+    CCHECK(hipStreamSynchronize(stream));
+    return loss;
+}
+
+double CnnModel::train(const float *x, const float *y, long long nimg, int epochs, int batch, double lr, uint64_t seed,
+                       hipStream_t stream) {
+    if (nimg <= 0 || epochs <= 0) return 0.0;
+    batch = (int)std::min<long long>(batch, nimg);
+    ensure_batch(batch);
+    std::mt19937_64 rng(seed);
+    std::vector<long long> idx(nimg);
+    long long *didx = nullptr;
+    CCHECK(hipMalloc(&didx, sizeof(long long) * nimg));
+    double last = 0.0;
+    try {
+        for (int ep = 0; ep < epochs; ep++) {
+            for (long long i = 0; i < nimg; i++) idx[i] = i;
+            std::shuffle(idx.begin(), idx.end(), rng);
+            CCHECK(hipMemcpy(didx, idx.data(), sizeof(long long) * nimg, hipMemcpyHostToDevice));
+            double tot = 0.0;
+            long long seen = 0;
+            for (long long b0 = 0; b0 + batch <= nimg; b0 += batch) {
+                hipLaunchKernelGGL(k_cnn_gather, dim3(batch), dim3(256), 0, stream, x, (const long long *)(didx + b0), xb_);
+                hipLaunchKernelGGL(k_cnn_gather, dim3(batch), dim3(256), 0, stream, y, (const long long *)(didx + b0), yb_);
+                tot += train_step(xb_, yb_, batch, lr, stream) * batch;
+                seen += batch;
+            }
+            last = seen ? tot / (double)seen : 0.0;
+        }
+    } catch (...) {
+        (void)hipFree(didx);
+        throw;
+    }
+    CCHECK(hipFree(didx));
+    return last;
+}
+
+}  // namespace falcon

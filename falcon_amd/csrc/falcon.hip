@@ -2,12 +2,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -15,8 +17,9 @@
 
 #include "../../include/falcon_hip.h"
 #include "device.hpp"
-#include "layout.hpp"
+#include "cnn.hpp"
 #include "heat.hpp"
+#include "layout.hpp"
 
 namespace falcon {
 void synthetic_state(const Layout &lay, const std::vector<double> &bg, uint64_t seed, double s_amp, double b_ratio,
@@ -101,6 +104,7 @@ struct falcon_handle {
     int graph_cur = -1, graph_prec = -1;
     std::unique_ptr<HeatSolver> heat;
     HeatParams heat_params;
+    std::unique_ptr<CnnModel> cnn;
     double graph_cfl = -1;
 
     ~falcon_handle() {
@@ -1035,6 +1039,112 @@ int falcon_heat_vcycle(falcon_handle_t h, const double *r, double *z) {
         H.vcycle_host(rd, zd);
         heat_to_host(zd.data(), z, H.m(), nblk);
     });
+}
+
+// ---------------------------------------------------------------- learned correction
+
+static CnnModel &need_cnn(falcon_handle_t h) {
+    if (!h->cnn) h->cnn.reset(new CnnModel());
+    return *h->cnn;
+}
+
+// device copy of a host array, freed on scope exit
+struct DevTmp {
+    void *p = nullptr;
+    explicit DevTmp(size_t bytes) { HIPCHECK(hipMalloc(&p, std::max<size_t>(bytes, 1))); }
+    ~DevTmp() { if (p) (void)hipFree(p); }
+};
+
+int falcon_cnn_init(falcon_handle_t h, uint64_t seed) {
+    return guard([&] { need_cnn(h).init(seed); });
+}
+
+int falcon_cnn_set_params(falcon_handle_t h, const float *params) {
+    return guard([&] { need_cnn(h).set_params(params); });
+}
+
+int falcon_cnn_get_params(falcon_handle_t h, float *params) {
+    return guard([&] { need_cnn(h).get_params(params); });
+}
+
+int falcon_cnn_forward(falcon_handle_t h, int64_t nimg, const float *x, float *out) {
+    return guard([&] {
+        if (nimg < 0) throw Error("nimg must be >= 0");
+        CnnModel &c = need_cnn(h);
+        const size_t bytes = sizeof(float) * 4096 * (size_t)nimg;
+        DevTmp dx(bytes), dy(bytes);
+        HIPCHECK(hipMemcpy(dx.p, x, bytes, hipMemcpyHostToDevice));
+        c.infer((const float *)dx.p, (float *)dy.p, nimg, h->stream);
+        HIPCHECK(hipStreamSynchronize(h->stream));
+        HIPCHECK(hipMemcpy(out, dy.p, bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+int falcon_cnn_gradient(falcon_handle_t h, int nimg, const float *x, const float *gout, double *grad) {
+    return guard([&] {
+        if (nimg < 1) throw Error("nimg must be >= 1");
+        CnnModel &c = need_cnn(h);
+        const size_t bytes = sizeof(float) * 4096 * (size_t)nimg;
+        DevTmp dx(bytes), dg(bytes);
+        HIPCHECK(hipMemcpy(dx.p, x, bytes, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(dg.p, gout, bytes, hipMemcpyHostToDevice));
+        c.gradient((const float *)dx.p, (const float *)dg.p, nimg, grad, h->stream);
+    });
+}
+
+int falcon_cnn_train(falcon_handle_t h, int64_t nimg, const float *x, const float *y, int epochs, int batch, double lr,
+                     uint64_t seed, double *loss) {
+    return guard([&] {
+        if (nimg < 1 || epochs < 0 || batch < 1) throw Error("bad training sizes");
+        CnnModel &c = need_cnn(h);
+        const size_t bytes = sizeof(float) * 4096 * (size_t)nimg;
+        DevTmp dx(bytes), dy(bytes);
+        HIPCHECK(hipMemcpy(dx.p, x, bytes, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(dy.p, y, bytes, hipMemcpyHostToDevice));
+        const double l = c.train((const float *)dx.p, (const float *)dy.p, nimg, epochs, batch, lr, seed, h->stream);
+        if (loss) *loss = l;
+    });
+}
+
+int falcon_heat_train_corrector(falcon_handle_t h, int npairs, double dt_lo, double dt_hi, int max_solves, int epochs,
+                                int batch, double lr, uint64_t seed, double *loss, int64_t *pairs_made,
+                                double *seconds) {
+    return guard([&] {
+        HeatSolver &H = need_heat(h);
+        if (npairs < 1 || !(dt_lo > 0) || !(dt_hi >= dt_lo) || max_solves < 1) throw Error("bad training settings");
+        CnnModel &c = need_cnn(h);
+        const auto t0 = std::chrono::steady_clock::now();
+        const size_t cap = (size_t)npairs;
+        DevTmp dX(sizeof(float) * 4096 * cap), dY(sizeof(float) * 4096 * cap), dW(sizeof(double) * 4096 * cap),
+            dS(sizeof(double) * cap), dRs(sizeof(int) * cap), dRb(sizeof(int) * cap);
+        HeatRecord rec;
+        rec.capacity = (long long)cap;
+        rec.per_snapshot = std::max(1, std::min(32, npairs / 64 + 1));
+        rec.rng = seed ^ 0x5851F42D4C957F2DULL;
+        rec.X = (float *)dX.p; rec.Y = (float *)dY.p; rec.W = (double *)dW.p; rec.scale = (double *)dS.p;
+        rec.rs = (int *)dRs.p; rec.rb = (int *)dRb.p;
+        std::mt19937_64 rng(seed);
+        std::uniform_real_distribution<double> U(std::log(dt_lo), std::log(dt_hi));
+        std::normal_distribution<double> N01(0.0, 1.0);
+        const int ns = H.nsys();
+        std::vector<double> dts(ns), T(H.cells() * ns);
+        for (int k = 0; k < max_solves && rec.count < rec.capacity; k++) {
+            for (int s = 0; s < ns; s++) dts[s] = H.explicit_dt() * std::exp(U(rng));
+            for (auto &v : T) v = 1e-3 * N01(rng);
+            H.set_dt(dts);
+            H.set_rhs_from_T(T);
+            H.solve(1e-10, 300, kSolveRecord, &rec);
+        }
+        const double l = c.train(rec.X, rec.Y, rec.count, epochs, batch, lr, seed + 1, h->stream);
+        H.set_corrector(&c);
+        if (loss) *loss = l;
+        if (pairs_made) *pairs_made = rec.count;
+        if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    });
+}
+
+int falcon_heat_attach_corrector(falcon_handle_t h) {
+    return guard([&] { need_heat(h).set_corrector(&need_cnn(h)); });
 }
 
 int falcon_synchronize(falcon_handle_t h) {

@@ -199,19 +199,30 @@ __global__ void __launch_bounds__(64) k_hdot_final(const double *__restrict__ pa
     }
 }
 
-// FCG scalars on the device (one thread per system)
-// what 0: alpha = <p,r>/<p,q>  (dots: [0]=<p,q>, [1]=<p,r>), store gamma
-// what 1: beta  = <z,q>/gamma  (dots: [0]=<z,q>)
+// Per-system scalars on the device (one thread per system), kSc slots each:
+// [0] alpha = <p,r>/<p,q>, [1] beta = <z,q>/gamma, [2] gamma = <p,q>,
+// [3] feature scale of the learned correction, [4] its step along c.
+constexpr int kSc = 8;
+// what 0: alpha and gamma (dots: [0]=<p,q>, [1]=<p,r>)
+// what 1: beta (dots: [0]=<z,q>)
+// what 2: correction step <c,r'>/<c,Ac> (dots: [0]=<c,r'>, [1]=<c,Ac>), 0 unless <c,Ac> > 0
+// what 3: feature scale sqrt(sum / ncell) (dots: [0]=sum (r'/diag)^2)
 __global__ void k_hscalars(int what, int nsys, const double *__restrict__ dots, double *__restrict__ sc,
-                           const int *__restrict__ active) {
+                           const int *__restrict__ active, double ncell) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsys) return;
-    double *S = sc + 4 * s;   // alpha, beta, gamma, spare
+    double *S = sc + kSc * s;
     if (what == 0) {
         S[2] = dots[3 * s];
         S[0] = active[s] ? dots[3 * s + 1] / dots[3 * s] : 0.0;
-    } else {
+    } else if (what == 1) {
         S[1] = active[s] ? dots[3 * s] / S[2] : 0.0;
+    } else if (what == 2) {
+        const double num = dots[3 * s], den = dots[3 * s + 1];
+        const double a = num / den;
+        S[4] = (active[s] && den > 0.0 && a == a && a < 1e300 && a > -1e300) ? a : 0.0;
+    } else {
+        S[3] = sqrt(dots[3 * s] / ncell);
     }
 }
 
@@ -222,7 +233,7 @@ __global__ void __launch_bounds__(256) k_hupdate_xr(HLev L, double *__restrict__
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= (long long)L.nb * L.M3) return;
     const int s = blockIdx.y;
-    const double a = sc[4 * s];
+    const double a = sc[kSc * s];
     const int m = L.m;
     const int b = (int)(c / L.M3);
     const int qq = (int)(c - (long long)b * L.M3);
@@ -238,7 +249,7 @@ __global__ void __launch_bounds__(256) k_hupdate_p(HLev L, double *__restrict__ 
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= (long long)L.nb * L.M3) return;
     const int s = blockIdx.y;
-    const double be = use_beta ? sc[4 * s + 1] : 0.0;
+    const double be = use_beta ? sc[kSc * s + 1] : 0.0;
     const int m = L.m;
     const int b = (int)(c / L.M3);
     const int qq = (int)(c - (long long)b * L.M3);
@@ -260,6 +271,136 @@ __global__ void __launch_bounds__(256) k_hcopy(const double *__restrict__ a, dou
     if (i < n) b[i] = a[i];
 }
 
+// ---- learned correction of the V-cycle
+// Az = A z and r' = r - A z (z padded with fresh ghosts; Az, r, r' interior)
+__global__ void __launch_bounds__(256) k_hresid2(HLev L, const double *__restrict__ z, const double *__restrict__ r,
+                                                 double *__restrict__ Az, double *__restrict__ rp,
+                                                 const double *__restrict__ inv_dt, long long pstride, long long istride) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (long long)L.nb * L.M3) return;
+    const int s = blockIdx.y;
+    const int m = L.m;
+    const int b = (int)(c / L.M3);
+    const int q = (int)(c - (long long)b * L.M3);
+    const int i = q / (m * m), j = (q / m) % m, k = q % m;
+    const double *X = z + s * pstride + (long long)b * L.P;
+    const long long S1 = m + 2, SI = S1 * S1;
+    const long long pc = pad_index(m, i, j, k);
+    const double *ai = L.Ai + (long long)b * (m + 1) * m * m;
+    const double *aj = L.Aj + (long long)b * m * (m + 1) * m;
+    const double *ak = L.Ak + (long long)b * m * m * (m + 1);
+    const double cim = ai[((long long)i * m + j) * m + k], cip = ai[((long long)(i + 1) * m + j) * m + k];
+    const double cjm = aj[((long long)i * (m + 1) + j) * m + k], cjp = aj[((long long)i * (m + 1) + j + 1) * m + k];
+    const double ckm = ak[((long long)i * m + j) * (m + 1) + k], ckp = ak[((long long)i * m + j) * (m + 1) + k + 1];
+    const double diag = L.V[(long long)b * L.M3 + q] * inv_dt[s] + L.off[(long long)b * L.M3 + q];
+    double nb = cim * X[pc - SI] + cip * X[pc + SI];
+    nb = nb + cjm * X[pc - S1] + cjp * X[pc + S1];
+    nb = nb + ckm * X[pc - 1] + ckp * X[pc + 1];
+    const double ax = diag * X[pc] - nb;
+    const long long io = s * istride + c;
+    Az[io] = ax;
+    rp[io] = r[io] - ax;
+}
+
+// partial sums of (r'/diag)^2 per system (partial[s][0][wg], as k_hdot)
+__global__ void __launch_bounds__(256) k_hfeatnorm(HLev L, const double *__restrict__ rp, const double *__restrict__ inv_dt,
+                                                   double *__restrict__ partial, long long istride) {
+    __shared__ double red[4];
+    const int s = blockIdx.y;
+    const long long N = (long long)L.nb * L.M3;
+    double acc = 0.0;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < N; c += (long long)gridDim.x * blockDim.x) {
+        const double u = rp[s * istride + c] / (L.V[c] * inv_dt[s] + L.off[c]);
+        acc += u * u;
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[((long long)s * 3) * gridDim.x + blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+// network input: x[s][b][v] = (r'/diag) / scale_s as float (0 when the scale is 0)
+__global__ void __launch_bounds__(256) k_hfeat(HLev L, const double *__restrict__ rp, const double *__restrict__ inv_dt,
+                                               const double *__restrict__ sc, float *__restrict__ xin, long long istride) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (long long)L.nb * L.M3) return;
+    const int s = blockIdx.y;
+    const double scale = sc[kSc * s + 3];
+    const double u = rp[s * istride + c] / (L.V[c] * inv_dt[s] + L.off[c]);
+    xin[s * istride + c] = scale > 0.0 ? (float)(u / scale) : 0.0f;
+}
+
+// correction c (padded) = scale_s * network output
+__global__ void __launch_bounds__(256) k_hcorr(HLev L, const float *__restrict__ yout, const double *__restrict__ sc,
+                                               double *__restrict__ cpad, long long pstride, long long istride) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (long long)L.nb * L.M3) return;
+    const int s = blockIdx.y;
+    const int m = L.m;
+    const int b = (int)(c / L.M3);
+    const int q = (int)(c - (long long)b * L.M3);
+    cpad[s * pstride + (long long)b * L.P + pad_index(m, q / (m * m), (q / m) % m, q % m)] =
+        sc[kSc * s + 3] * (double)yout[s * istride + c];
+}
+
+// z += a c (padded interiors), Az += a Ac (interior), a = correction step of the system
+__global__ void __launch_bounds__(256) k_hzcorr(HLev L, double *__restrict__ z, const double *__restrict__ cpad,
+                                                double *__restrict__ Az, const double *__restrict__ Ac,
+                                                const double *__restrict__ sc, long long pstride, long long istride) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (long long)L.nb * L.M3) return;
+    const int s = blockIdx.y;
+    const double a = sc[kSc * s + 4];
+    const int m = L.m;
+    const int b = (int)(c / L.M3);
+    const int q = (int)(c - (long long)b * L.M3);
+    const long long o = s * pstride + (long long)b * L.P + pad_index(m, q / (m * m), (q / m) % m, q % m);
+    z[o] = z[o] + a * cpad[o];
+    const long long io = s * istride + c;
+    Az[io] = Az[io] + a * Ac[io];
+}
+
+// q = Az - beta q (interior), the FCG direction's image without an operator apply
+__global__ void __launch_bounds__(256) k_hupdate_q(double *__restrict__ q, const double *__restrict__ Az,
+                                                   const double *__restrict__ sc, long long istride, int use_beta) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= istride) return;
+    const int s = blockIdx.y;
+    const double be = use_beta ? sc[kSc * s + 1] : 0.0;
+    const long long io = s * istride + c;
+    q[io] = Az[io] - be * q[io];
+}
+
+// training records: copy the network input image and w = x + z of record slot
+// r (system rs[r], block rb[r]) into X[r] (float) and W[r] (double)
+__global__ void __launch_bounds__(256) k_hrecord(HLev L, const float *__restrict__ xin, const double *__restrict__ x,
+                                                 const double *__restrict__ z, const int *__restrict__ rs,
+                                                 const int *__restrict__ rb, float *__restrict__ X, double *__restrict__ W,
+                                                 const double *__restrict__ sc, double *__restrict__ rscale,
+                                                 long long pstride, long long istride) {
+    const long long r = blockIdx.x;
+    const int s = rs[r], b = rb[r], m = L.m;
+    if (threadIdx.x == 0) rscale[r] = sc[kSc * s + 3];
+    for (int q = threadIdx.x; q < (int)L.M3; q += blockDim.x) {
+        const long long io = s * istride + (long long)b * L.M3 + q;
+        X[r * L.M3 + q] = xin[io];
+        W[r * L.M3 + q] = x[io] + z[s * pstride + (long long)b * L.P + pad_index(m, q / (m * m), (q / m) % m, q % m)];
+    }
+}
+
+// targets: Y[r] = (x*[system][block] - W[r]) / scale[r]
+__global__ void __launch_bounds__(256) k_htarget(HLev L, const double *__restrict__ x, const int *__restrict__ rs,
+                                                 const int *__restrict__ rb, const double *__restrict__ rscale,
+                                                 const double *__restrict__ W, float *__restrict__ Y, long long istride) {
+    const long long r = blockIdx.x;
+    const int s = rs[r], b = rb[r];
+    const double sc = rscale[r];
+    for (int q = threadIdx.x; q < (int)L.M3; q += blockDim.x) {
+        const double e = x[s * istride + (long long)b * L.M3 + q] - W[r * L.M3 + q];
+        Y[r * L.M3 + q] = sc > 0.0 ? (float)(e / sc) : 0.0f;
+    }
+}
+
 // ------------------------------------------------------------------ solver
 
 struct HeatSolver::Impl {
@@ -274,6 +415,12 @@ struct HeatSolver::Impl {
     double *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *b = nullptr, *tmp = nullptr;
     double *inv_dt = nullptr, *partial = nullptr, *dots = nullptr, *sc = nullptr;
     int *active = nullptr;
+    // learned correction work arrays (allocated on first use)
+    double *Az = nullptr, *rp = nullptr, *Ac = nullptr, *cpad = nullptr;
+    float *xin = nullptr, *yout = nullptr;
+    void ensure_learned();
+    void features(double *z);
+    void learned_correction(double *z);
     int dot_wg = 1024;
     int pre = 2, post = 2, coarse_sweeps = 8;
     double omega = 0.8;
@@ -417,7 +564,7 @@ void HeatSolver::setup(const Layout &lay, const HeatParams &hp, int nsys, int nt
     I.inv_dt = I.dalloc(nsys);
     I.partial = I.dalloc((size_t)nsys * 3 * I.dot_wg);
     I.dots = I.dalloc((size_t)nsys * 3);
-    I.sc = I.dalloc((size_t)nsys * 4);
+    I.sc = I.dalloc((size_t)nsys * kSc);
     void *a = nullptr;
     HCHECK(hipMalloc(&a, sizeof(int) * nsys));
     I.allocs.push_back(a);
@@ -497,9 +644,62 @@ void HeatSolver::vcycle_host(const std::vector<double> &r, std::vector<double> &
 }
 
 HeatResult HeatSolver::solve(double tol, int max_iter, bool use_corrector) {
+    return solve(tol, max_iter, use_corrector ? kSolveLearned : kSolvePlain, nullptr);
+}
+
+void HeatSolver::Impl::ensure_learned() {
+    if (Az) return;
+    Az = dalloc(ist(0) * nsys);
+    rp = dalloc(ist(0) * nsys);
+    Ac = dalloc(ist(0) * nsys);
+    cpad = dalloc(pst(0) * nsys);
+    void *a = nullptr;
+    HCHECK(hipMalloc(&a, sizeof(float) * ist(0) * nsys));
+    allocs.push_back(a);
+    xin = (float *)a;
+    HCHECK(hipMalloc(&a, sizeof(float) * ist(0) * nsys));
+    allocs.push_back(a);
+    yout = (float *)a;
+}
+
+// r' = r - A z, Az, and the network input images (features scaled per system)
+void HeatSolver::Impl::features(double *z) {
+    exchange(0, z);
+    hipLaunchKernelGGL(k_hresid2, grid_cells(0), dim3(256), 0, stream, L[0], (const double *)z, (const double *)r, Az, rp,
+                       (const double *)inv_dt, pst(0), ist(0));
+    hipLaunchKernelGGL(k_hfeatnorm, dim3((unsigned)dot_wg, (unsigned)nsys), dim3(256), 0, stream, L[0],
+                       (const double *)rp, (const double *)inv_dt, partial, ist(0));
+    hipLaunchKernelGGL(k_hdot_final, dim3((unsigned)nsys), dim3(64), 0, stream, (const double *)partial, dot_wg, 1, dots);
+    hipLaunchKernelGGL(k_hscalars, dim3((nsys + 63) / 64), dim3(64), 0, stream, 3, nsys, (const double *)dots, sc,
+                       (const int *)active, (double)ist(0));
+    hipLaunchKernelGGL(k_hfeat, grid_cells(0), dim3(256), 0, stream, L[0], (const double *)rp, (const double *)inv_dt,
+                       (const double *)sc, xin, ist(0));
+}
+
+// z += a c with c = scale * net(features) and a = <c,r'>/<c,Ac>; Az += a Ac
+void HeatSolver::Impl::learned_correction(double *z) {
+    corr->infer(xin, yout, (long long)nsys * nb, stream);
+    hipLaunchKernelGGL(k_hcorr, grid_cells(0), dim3(256), 0, stream, L[0], (const float *)yout, (const double *)sc, cpad,
+                       pst(0), ist(0));
+    exchange(0, cpad);
+    hipLaunchKernelGGL((k_happly<0>), grid_cells(0), dim3(256), 0, stream, L[0], (const double *)cpad,
+                       (const double *)nullptr, Ac, (const double *)inv_dt, pst(0), ist(0), 0.0);
+    dot(0, 2, cpad, 1, rp, 0, cpad, 1, Ac, 0);
+    hipLaunchKernelGGL(k_hscalars, dim3((nsys + 63) / 64), dim3(64), 0, stream, 2, nsys, (const double *)dots, sc,
+                       (const int *)active, 0.0);
+    hipLaunchKernelGGL(k_hzcorr, grid_cells(0), dim3(256), 0, stream, L[0], z, (const double *)cpad, Az,
+                       (const double *)Ac, (const double *)sc, pst(0), ist(0));
+}
+
+HeatResult HeatSolver::solve(double tol, int max_iter, int mode, HeatRecord *rec) {
     Impl &I = *d;
     const int ns = I.nsys;
     const long long ni = I.ist(0);
+    const bool learned = mode == kSolveLearned;
+    if (learned && !I.corr) throw std::runtime_error("no learned correction is attached to the heat solver");
+    if ((learned || mode == kSolveRecord) && I.L[0].m != 16) throw std::runtime_error("the learned correction needs 16^3 blocks");
+    if (mode == kSolveRecord && !rec) throw std::runtime_error("record mode needs a record buffer");
+    if (learned || mode == kSolveRecord) I.ensure_learned();
     HeatResult res;
     res.iters.assign(ns, 0);
     res.relres.assign(ns, 0.0);
@@ -508,6 +708,7 @@ HeatResult HeatSolver::solve(double tol, int max_iter, bool use_corrector) {
     HCHECK(hipStreamSynchronize(I.stream));
     const auto t0 = std::chrono::steady_clock::now();
     const dim3 g0 = I.grid_cells(0);
+    const long long rec0 = rec ? rec->count : 0;
     // x = 0, r = b
     I.zero(I.x, ni * ns);
     hipLaunchKernelGGL(k_hcopy, dim3((unsigned)((ni * ns + 255) / 256)), dim3(256), 0, I.stream, (const double *)I.b, I.r,
@@ -521,31 +722,60 @@ HeatResult HeatSolver::solve(double tol, int max_iter, bool use_corrector) {
     for (int s = 0; s < ns; s++)
         if (bn[3 * s] == 0.0) { done[s] = 1; act[s] = 0; }
     HCHECK(hipMemcpy(I.active, act.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
+
+    auto record = [&](const double *z) {
+        // R (system, block) slots drawn among the systems still iterating
+        std::vector<int> live;
+        for (int s = 0; s < ns; s++)
+            if (!done[s]) live.push_back(s);
+        const long long room = rec->capacity - rec->count;
+        const int R = (int)std::min<long long>(rec->per_snapshot, room);
+        if (R <= 0 || live.empty()) return;
+        std::vector<int> rs(R), rb(R);
+        for (int k = 0; k < R; k++) {
+            rec->rng = rec->rng * 6364136223846793005ULL + 1442695040888963407ULL;
+            const uint64_t u = rec->rng >> 11;
+            rs[k] = live[u % live.size()];
+            rb[k] = (int)((u / live.size()) % (uint64_t)I.nb);
+        }
+        HCHECK(hipMemcpyAsync(rec->rs + rec->count, rs.data(), sizeof(int) * R, hipMemcpyHostToDevice, I.stream));
+        HCHECK(hipMemcpyAsync(rec->rb + rec->count, rb.data(), sizeof(int) * R, hipMemcpyHostToDevice, I.stream));
+        hipLaunchKernelGGL(k_hrecord, dim3(R), dim3(256), 0, I.stream, I.L[0], (const float *)I.xin, (const double *)I.x, z,
+                           (const int *)(rec->rs + rec->count), (const int *)(rec->rb + rec->count), rec->X + rec->count * 4096,
+                           rec->W + rec->count * 4096, (const double *)I.sc, rec->scale + rec->count, I.pst(0), ni);
+        HCHECK(hipStreamSynchronize(I.stream));   // rs/rb host vectors leave scope
+        rec->count += R;
+    };
     auto precond = [&]() -> double * {
         double *z = I.Z[0];
         I.vcycle(I.r, z);
-        if (use_corrector && I.corr) {
-            // residual the V-cycle leaves, then the corrector's estimate of its error
-            I.exchange(0, z);
-            hipLaunchKernelGGL((k_happly<1>), I.grid_cells(0), dim3(256), 0, I.stream, I.L[0], (const double *)z,
-                               (const double *)I.r, I.tmp, (const double *)I.inv_dt, I.pst(0), I.ist(0), 0.0);
-            I.corr->correct(I.tmp, z, I.stream);
+        if (learned) {
+            I.features(z);
+            I.learned_correction(z);
+        } else if (mode == kSolveRecord) {
+            I.features(z);
+            record(z);
         }
         return z;
     };
     double *z = precond();
     hipLaunchKernelGGL(k_hupdate_p, g0, dim3(256), 0, I.stream, I.L[0], I.p, (const double *)z, (const double *)I.sc,
                        I.pst(0), 0);
+    if (learned)
+        hipLaunchKernelGGL(k_hupdate_q, dim3((unsigned)((ni + 255) / 256), (unsigned)ns), dim3(256), 0, I.stream, I.q,
+                           (const double *)I.Az, (const double *)I.sc, ni, 0);
     int it = 0;
     bool all0 = true;
     for (int s = 0; s < ns; s++) all0 = all0 && done[s];
     while (!all0) {
-        I.exchange(0, I.p);
-        hipLaunchKernelGGL((k_happly<0>), g0, dim3(256), 0, I.stream, I.L[0], (const double *)I.p, (const double *)nullptr,
-                           I.q, (const double *)I.inv_dt, I.pst(0), ni, 0.0);
+        if (!learned) {
+            I.exchange(0, I.p);
+            hipLaunchKernelGGL((k_happly<0>), g0, dim3(256), 0, I.stream, I.L[0], (const double *)I.p,
+                               (const double *)nullptr, I.q, (const double *)I.inv_dt, I.pst(0), ni, 0.0);
+        }
         I.dot(0, 2, I.p, 1, I.q, 0, I.p, 1, I.r, 0);
         hipLaunchKernelGGL(k_hscalars, dim3((ns + 63) / 64), dim3(64), 0, I.stream, 0, ns, (const double *)I.dots, I.sc,
-                           (const int *)I.active);
+                           (const int *)I.active, 0.0);
         hipLaunchKernelGGL(k_hupdate_xr, g0, dim3(256), 0, I.stream, I.L[0], I.x, I.r, (const double *)I.p,
                            (const double *)I.q, (const double *)I.sc, I.pst(0), ni);
         it++;
@@ -566,12 +796,23 @@ HeatResult HeatSolver::solve(double tol, int max_iter, bool use_corrector) {
         z = precond();
         I.dot(0, 1, z, 1, I.q, 0);
         hipLaunchKernelGGL(k_hscalars, dim3((ns + 63) / 64), dim3(64), 0, I.stream, 1, ns, (const double *)I.dots, I.sc,
-                           (const int *)I.active);
+                           (const int *)I.active, 0.0);
         hipLaunchKernelGGL(k_hupdate_p, g0, dim3(256), 0, I.stream, I.L[0], I.p, (const double *)z, (const double *)I.sc,
                            I.pst(0), 1);
+        if (learned)
+            hipLaunchKernelGGL(k_hupdate_q, dim3((unsigned)((ni + 255) / 256), (unsigned)ns), dim3(256), 0, I.stream, I.q,
+                               (const double *)I.Az, (const double *)I.sc, ni, 1);
     }
     HCHECK(hipStreamSynchronize(I.stream));
     res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (rec && rec->count > rec0) {
+        // targets from the converged solution: the error each recorded V-cycle left
+        const long long nrec = rec->count - rec0;
+        hipLaunchKernelGGL(k_htarget, dim3((unsigned)nrec), dim3(256), 0, I.stream, I.L[0], (const double *)I.x,
+                           (const int *)(rec->rs + rec0), (const int *)(rec->rb + rec0), (const double *)(rec->scale + rec0),
+                           (const double *)(rec->W + rec0 * 4096), rec->Y + rec0 * 4096, ni);
+        HCHECK(hipStreamSynchronize(I.stream));
+    }
     HCHECK(hipGetLastError());
     return res;
 }

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <vector>
 
 #include "layout.hpp"
@@ -20,14 +21,29 @@ struct HeatResult {
     double seconds = 0.0;
 };
 
-// Optional correction applied after the V-cycle inside the preconditioner:
-// given the residual left by the V-cycle (interior, system-major) it adds its
-// estimate of the remaining error to z (padded, system-major), on `stream`.
+// Learned correction of the V-cycle: maps block images of the normalised
+// residual left by the V-cycle to images of the normalised error estimate,
+// x[img][4096] -> out[img][4096] with img = system * nblocks + block and the
+// block's cells in [i][j][k] order (device pointers, on `stream`).
 class HeatCorrector {
   public:
     virtual ~HeatCorrector() {}
-    virtual void correct(const double *resid, double *z, hipStream_t stream) = 0;
+    virtual void infer(const float *x, float *out, long long nimg, hipStream_t stream) = 0;
 };
+
+// Training records collected during a solve (device arrays, capacity images):
+// X network input images, W = x + z of the recorded V-cycle, Y targets (error
+// left by the V-cycle over the feature scale), per record system, block, scale.
+struct HeatRecord {
+    long long capacity = 0, count = 0;
+    int per_snapshot = 32;           // records drawn per V-cycle application
+    uint64_t rng = 0;
+    float *X = nullptr, *Y = nullptr;
+    double *W = nullptr, *scale = nullptr;
+    int *rs = nullptr, *rb = nullptr;
+};
+
+enum { kSolvePlain = 0, kSolveLearned = 1, kSolveRecord = 2 };
 
 class HeatSolver {
   public:
@@ -46,6 +62,9 @@ class HeatSolver {
     // z = M r, one V-cycle, system-major interior vectors (parity tests)
     void vcycle_host(const std::vector<double> &r, std::vector<double> &z);
     HeatResult solve(double tol, int max_iter, bool use_corrector);
+    // mode kSolvePlain / kSolveLearned (needs a corrector) / kSolveRecord (plain
+    // FCG that appends training records to rec)
+    HeatResult solve(double tol, int max_iter, int mode, HeatRecord *rec);
     void get_solution(int s, std::vector<double> &T) const;
     void get_rhs(int s, std::vector<double> &b) const;
     void set_corrector(HeatCorrector *c);

@@ -133,6 +133,34 @@ int falcon_heat_apply(falcon_handle_t h, const double *x, double *y);
  * guess (ModMultigrid_Vcycle, 3_5/ModMultigrid.f90:338-376, as a method) */
 int falcon_heat_vcycle(falcon_handle_t h, const double *r, double *z);
 
+/* ---------------------------------------------------------------- learned correction
+ * No counterpart in the reference.  A 1 -> 16 -> 16 -> 1 network of 3x3x3
+ * convolutions over 16^3 block images (zero padding, ReLU after layers 1-2),
+ * BF16 on MFMA with FP32 accumulation, FP32 master weights.  Parameters (7809
+ * floats): W1[27][16], b1[16], W2[27][16][16] (tap, in, out), b2[16],
+ * W3[27][16], b3; tap t = (di+1)*9 + (dj+1)*3 + (dk+1); images [4096] with
+ * voxel (i*16 + j)*16 + k. */
+int falcon_cnn_init(falcon_handle_t h, uint64_t seed);
+int falcon_cnn_set_params(falcon_handle_t h, const float *params);
+int falcon_cnn_get_params(falcon_handle_t h, float *params);
+int falcon_cnn_forward(falcon_handle_t h, int64_t nimg, const float *x, float *out);
+/* sum over the images of d(sum gout*out)/d(params) */
+int falcon_cnn_gradient(falcon_handle_t h, int nimg, const float *x, const float *gout, double *grad);
+/* Adam on the mean squared error, shuffled minibatches; last epoch's mean loss */
+int falcon_cnn_train(falcon_handle_t h, int64_t nimg, const float *x, const float *y, int epochs, int batch, double lr,
+                     uint64_t seed, double *loss);
+/* Solver-generated training: batch solves of the handle's nsys systems with dt
+ * drawn log-uniformly in [dt_lo, dt_hi] x the explicit limit and seeded Gaussian
+ * T' (sigma 1e-3) record npairs (normalised residual left by the V-cycle,
+ * normalised error it left) block-image pairs, the network trains on them and
+ * is attached to the solver (falcon_heat_solve with use_corrector = 1).  Leaves
+ * the last training system's dt and right-hand side set. */
+int falcon_heat_train_corrector(falcon_handle_t h, int npairs, double dt_lo, double dt_hi, int max_solves, int epochs,
+                                int batch, double lr, uint64_t seed, double *loss, int64_t *pairs_made,
+                                double *seconds);
+/* attach the handle's network (e.g. after falcon_cnn_set_params) */
+int falcon_heat_attach_corrector(falcon_handle_t h);
+
 /* ---------------------------------------------------------------- errors */
 int falcon_last_error(char *buf, int len);
 

@@ -1210,6 +1210,13 @@ void orc_heat_vcycle(orc_heat *H, double dt, const double *r, double *z) {
     h_work_free(H, W);
 }
 
+void orc_heat_diag(orc_heat *H, double dt, double *d) {
+    const hlev *L = &H->L[0];
+    const long N = (long)H->nb * L->m * L->m * L->m;
+    const double inv_dt = 1.0 / dt;
+    for (long c = 0; c < N; c++) d[c] = L->V[c] * inv_dt + L->off[c];
+}
+
 static double h_dot(int m, int nb, const double *a, int apad, const double *b, int bpad) {
     const long M3 = (long)m * m * m, P = (long)(m + 2) * (m + 2) * (m + 2);
     double s = 0.0;

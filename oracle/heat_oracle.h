@@ -12,5 +12,7 @@ int orc_heat_levels(const orc_heat *H);
 void orc_heat_apply(orc_heat *H, double dt, const double *x, double *y);
 /* z = M r: one V-cycle, zero initial guess; interior arrays, Fortran order per block */
 void orc_heat_vcycle(orc_heat *H, double dt, const double *r, double *z);
+/* Jacobi diagonal V/dt + sum of face coefficients, interior Fortran order */
+void orc_heat_diag(orc_heat *H, double dt, double *d);
 int orc_heat_solve(orc_heat *H, double dt, const double *Tn, double tol, int max_iter, double *x_out, double *relres);
 #endif

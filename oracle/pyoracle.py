@@ -63,7 +63,13 @@ def lib():
         L.orc_heat_levels.argtypes = [vp]
         L.orc_heat_apply.argtypes = [vp, C.c_double, dp, dp]
         L.orc_heat_vcycle.argtypes = [vp, C.c_double, dp, dp]
+        L.orc_heat_diag.argtypes = [vp, C.c_double, dp]
         L.orc_heat_solve.argtypes = [vp, C.c_double, dp, C.c_double, C.c_int, dp, dp]
+        fp = C.POINTER(C.c_float)
+        L.cnn_round_bf16.restype = C.c_float
+        L.cnn_round_bf16.argtypes = [C.c_float]
+        L.cnn_forward.argtypes = [fp, C.c_int, fp, dp, dp, dp, C.c_int]
+        L.cnn_backward.argtypes = [fp, C.c_int, fp, dp, dp, dp, dp, C.c_int]
         L.orc_coordconv.argtypes = [dp, dp]
         L.orc_vecconv_coeffs.argtypes = [dp, dp]
         _lib = L
@@ -216,12 +222,50 @@ class HeatOracle:
         self.L.orc_heat_vcycle(self.h, dt, _p(r, C.c_double), _p(z, C.c_double))
         return z
 
+    def diag(self, dt):
+        d = np.zeros(self.shape)
+        self.L.orc_heat_diag(self.h, dt, _p(d, C.c_double))
+        return d
+
     def solve(self, dt, Tn, tol=1e-10, max_iter=500):
         Tn = np.ascontiguousarray(Tn, dtype=np.float64)
         x = np.zeros(self.shape)
         rel = np.zeros(1)
         it = self.L.orc_heat_solve(self.h, dt, _p(Tn, C.c_double), tol, max_iter, _p(x, C.c_double), _p(rel, C.c_double))
         return x, it, float(rel[0])
+
+
+CNN_NPARAM = 7809
+
+
+class CnnOracle:
+    """The 1->16->16->1 3^3 network of the learned V-cycle correction (FP64).
+    Images are float arrays (nimg, 16, 16, 16) indexed [i][j][k]."""
+
+    def __init__(self):
+        self.L = lib()
+
+    def forward(self, params, x, bf16=True):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        n = x.shape[0]
+        h1 = np.zeros((n, 4096, 16)); h2 = np.zeros((n, 4096, 16)); out = np.zeros((n, 16, 16, 16))
+        self.L.cnn_forward(_p(params, C.c_float), n, _p(x, C.c_float), _p(h1, C.c_double), _p(h2, C.c_double),
+                           _p(out, C.c_double), int(bf16))
+        return out, h1, h2
+
+    def backward(self, params, x, h1, h2, gout, bf16=True):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        gout = np.ascontiguousarray(gout, dtype=np.float64)
+        g = np.zeros(CNN_NPARAM)
+        self.L.cnn_backward(_p(params, C.c_float), x.shape[0], _p(x, C.c_float), _p(h1, C.c_double), _p(h2, C.c_double),
+                            _p(gout, C.c_double), _p(g, C.c_double), int(bf16))
+        return g
+
+    def round_bf16(self, a):
+        f = np.vectorize(lambda v: self.L.cnn_round_bf16(float(v)))
+        return f(np.asarray(a, dtype=np.float32)).astype(np.float32)
 
 
 def set_threads(n):
