@@ -65,7 +65,10 @@ def main():
     ap.add_argument("--workload", default="shell_128x256x768_b16", choices=sorted(WORKLOADS))
     ap.add_argument("--cfl", type=float, default=0.5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-heat", action="store_true", help="skip the heat-solve leg")
+    ap.add_argument("--learned-sweeps", default="2,2",
+                    help="finest-level pre,post smoothing sweeps of the V-cycle under the learned correction")
+    ap.add_argument("--legs", default="rk,heat,learned",
+                    help="comma list of legs: rk (RK4 FP64/FP32), heat (MG-PCG), learned (learned FCG vs MG-PCG)")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="only run a device-to-device copy of this many GiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -76,56 +79,64 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
         sys.exit(2)
 
+    legs = set(args.legs.split(","))
     lay = WORKLOADS[args.workload]
-    t0 = time.perf_counter()
-    shell = Shell(lay)
-    shell.background_polytrope(20.0)
-    shell.set_gc_all()
-    setup_s = time.perf_counter() - t0
-    print(f"# setup {args.workload}: {shell.nblocks} blocks, {shell.interior_cells} cells, {setup_s:.1f} s",
-          file=sys.stderr, flush=True)
+    res, setup_s, nblocks, ncells = None, None, None, None
+    if "rk" in legs:
+        t0 = time.perf_counter()
+        shell = Shell(lay)
+        shell.background_polytrope(20.0)
+        shell.set_gc_all()
+        setup_s = time.perf_counter() - t0
+        nblocks, ncells = shell.nblocks, shell.interior_cells
+        print(f"# setup {args.workload}: {nblocks} blocks, {ncells} cells, {setup_s:.1f} s", file=sys.stderr, flush=True)
+        res = {}
+        for prec in ("fp64", "fp32"):
+            res[prec] = run_precision(shell, prec, args.steps, args.warmup, args.cfl)
+            print(f"# {prec}: {res[prec]['cell_updates_per_s']:.4e} cell-updates/s, "
+                  f"{res[prec]['ms_per_step']:.3f} ms/step, kernels {res[prec]['kernel_ms']}", file=sys.stderr,
+                  flush=True)
+        shell.close()
+        traffic = load_traffic(args.workload)
+        for prec in ("fp32", "fp64"):
+            res[prec]["stage_roofline"]["traffic"] = traffic.get(prec, {}).get("stage_bytes_per_launch")
 
-    res = {}
-    for prec in ("fp64", "fp32"):
-        res[prec] = run_precision(shell, prec, args.steps, args.warmup, args.cfl)
-        print(f"# {prec}: {res[prec]['cell_updates_per_s']:.4e} cell-updates/s, "
-              f"{res[prec]['ms_per_step']:.3f} ms/step, kernels {res[prec]['kernel_ms']}", file=sys.stderr, flush=True)
-
-    shell.close()
     heat = None
-    if not args.no_heat:
+    if "heat" in legs:
         heat = run_heat(cpu_baseline=not args.no_cpu_baseline)
         print(f"# heat: {heat['mg_pcg']}", file=sys.stderr, flush=True)
+    if "learned" in legs:
+        learned = run_learned(sweeps=tuple(int(v) for v in args.learned_sweeps.split(",")))
+        print(f"# learned: {json.dumps(learned)}", file=sys.stderr, flush=True)
+        heat = heat or {}
+        heat["learned_fcg"] = learned
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if "rk" in legs and not args.no_cpu_baseline:
         cpu = run_cpu_baseline()
 
-    head = res["fp32"]
-    traffic = load_traffic(args.workload)
-    for prec in ("fp32", "fp64"):
-        t = traffic.get(prec, {}).get("stage_bytes_per_launch")
-        res[prec]["stage_roofline"]["traffic"] = t
+    head = res["fp32"] if res else None
     out = {
         "metric": "RK4 cell-updates/s FP32-dev vs FP64; heat-solve s @1e-10 learned FCG vs MG-PCG",
-        "value": head["cell_updates_per_s"],
+        "value": head["cell_updates_per_s"] if head else None,
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": head["ms_per_step"],
+        "ms_per_step": head["ms_per_step"] if head else None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 stencils / f64 state",
-        "data": "synthetic (seeded polytrope shell, entropy noise, curl-A field)",
-        "config": {"workload": args.workload, "blocks": shell.nblocks, "block": [lay["ni"], lay["nj"], lay["nk"]],
-                   "interior_cells": shell.interior_cells, "cfl": args.cfl},
-        "fp64": {"value": res["fp64"]["cell_updates_per_s"], "ms_per_step": res["fp64"]["ms_per_step"],
-                 "kernel_ms": res["fp64"]["kernel_ms"], "stage_roofline": res["fp64"]["stage_roofline"]},
-        "fp32_over_fp64": head["cell_updates_per_s"] / res["fp64"]["cell_updates_per_s"],
-        "kernel_ms": head["kernel_ms"],
-        "roofline": head["stage_roofline"],
+        "data": "synthetic (seeded polytrope shell, entropy noise, curl-A field; seeded Gaussian T')",
+        "config": {"workload": args.workload, "blocks": nblocks, "block": [lay["ni"], lay["nj"], lay["nk"]],
+                   "interior_cells": ncells, "cfl": args.cfl},
+        "fp64": ({"value": res["fp64"]["cell_updates_per_s"], "ms_per_step": res["fp64"]["ms_per_step"],
+                  "kernel_ms": res["fp64"]["kernel_ms"], "stage_roofline": res["fp64"]["stage_roofline"]}
+                 if res else None),
+        "fp32_over_fp64": head["cell_updates_per_s"] / res["fp64"]["cell_updates_per_s"] if head else None,
+        "kernel_ms": head["kernel_ms"] if head else None,
+        "roofline": head["stage_roofline"] if head else None,
         "setup_s": setup_s,
         "cpu_baseline": cpu,
         "heat": heat,
@@ -243,6 +254,44 @@ def run_heat(cpu_baseline=True, repeats=3):
         el = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": el, "unit": "s", "cores": 1, "kind": "port", "iterations": ito,
                                "sample": "oracle FP64 MG-preconditioned FCG, same system, one solve to 1e-10"}
+    return out
+
+
+LEARNED_WORKLOAD = "heat_amr_32x64x192_b16_x32"
+
+
+def run_learned(npairs=2000, epochs=20, batch=64, lr=2e-3, repeats=3, sweeps=(2, 2)):
+    """32 concurrent backward-Euler solves on the locally refined shell
+    (level-0 32x64x192 per panel in 16^3 blocks, seeded 20% of blocks to level
+    1 and 5% to level 2), dt spread geometrically over 10-300 x the explicit
+    limit, seeded Gaussian T' (sigma 1e-3), to 1e-10: plain MG-FCG against
+    FCG with the learned correction, trained first on `npairs` block-image
+    pairs generated by the solver's own systems (training time reported
+    separately, not in the solve time)."""
+    lay = configs.SHELL_AMR_32x64x192_B16
+    nsys = 32
+    s = Shell(lay)
+    hs = HeatSolver(s, 1.0, 0.7, nsys)
+    hs.set_learned_sweeps(*sweeps)
+    loss, pairs, train_s = hs.train_corrector(npairs=npairs, dt_range=(10.0, 300.0), max_solves=20, epochs=epochs,
+                                              batch=batch, lr=lr, seed=12345)
+    dts = hs.dt_explicit * 10.0 * 30.0 ** (np.arange(nsys) / (nsys - 1))
+    T = 1e-3 * np.random.default_rng(54321).standard_normal(hs.shape)
+    hs.set_dt(dts)
+    hs.set_T(T)
+    out = {"workload": LEARNED_WORKLOAD, "blocks": s.nblocks, "systems": nsys, "cells_per_system": hs.cells,
+           "tol": 1e-10, "dt_over_explicit": [10.0, 300.0],
+           "learned_finest_sweeps": list(sweeps),
+           "training": {"pairs": pairs, "epochs": epochs, "batch": batch, "lr": lr, "final_mse": loss,
+                        "seconds": train_s}}
+    for name, learned in (("mg_pcg", False), ("learned_fcg", True)):
+        hs.solve(1e-10, 300, use_corrector=learned)        # warm-up
+        runs = [hs.solve(1e-10, 300, use_corrector=learned) for _ in range(repeats)]
+        secs = sorted(r[2] for r in runs)
+        it, rel = runs[0][0], runs[0][1]
+        out[name] = {"seconds": secs[len(secs) // 2], "iterations_max": int(it.max()),
+                     "iterations_mean": float(it.mean()), "relres_max": float(rel.max())}
+    out["speedup"] = out["mg_pcg"]["seconds"] / out["learned_fcg"]["seconds"]
     return out
 
 

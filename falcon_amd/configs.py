@@ -3,8 +3,10 @@
 Each layout is a dict understood by falcon_amd.Shell and the oracle:
 ("split", (nr, nt, np)) divides every leaf into nr x nt x np children with the
 reference's split rule; ("amr", flags, if_global, rng6_rsun) is one
-ModAMR_Divide_OneBranch pass (3_6/ModAMR.f90:65-99).
+ModAMR_Divide_OneBranch pass (3_6/ModAMR.f90:65-99); ("refine", ordinals)
+splits the listed leaves (0-based, depth-first order) 2x2x2.
 """
+import numpy as np
 
 R_RANGE = (0.7, 1.0)
 AMR_ALL_R = (0.6, 0.0, 0.0, 1.1, 0.0, 0.0)   # strictly contains both radial ends
@@ -46,3 +48,27 @@ def cubic_refined(cells_per_panel, block, region=(0.8, 1.2, -0.5, 0.95, 1.6, 0.3
     lay = cubic(cells_per_panel, block)
     lay["ops"] = lay["ops"] + [("amr", (1, 1, 1), 0, region)]
     return lay
+
+
+def cubic_random_amr(cells_per_panel, block, frac1=0.20, frac2=0.05, seed=12345):
+    """`cubic` plus seeded local refinement: round(frac1 * N0) of the N0
+    level-0 blocks split to level 1, then round(frac2 * N0) of the level-1
+    blocks split to level 2 (leaves chosen without replacement)."""
+    lay = cubic(cells_per_panel, block)
+    nr, nt, npp = cells_per_panel
+    n0 = 2 * (nr // block) * (nt // block) * (npp // block)
+    rng = np.random.default_rng(seed)
+    levels = [0] * n0
+    first = sorted(rng.choice(n0, size=int(round(frac1 * n0)), replace=False).tolist())
+    new = []
+    for i, lv in enumerate(levels):
+        new += [1] * 8 if i in set(first) else [lv]
+    levels = new
+    lvl1 = [i for i, lv in enumerate(levels) if lv == 1]
+    second = sorted(rng.choice(lvl1, size=int(round(frac2 * n0)), replace=False).tolist())
+    lay["ops"] = lay["ops"] + [("refine", first), ("refine", second)]
+    return lay
+
+
+# learned-FCG workload: level-0 32x64x192 per panel in 16^3 blocks, 20% / 5% to levels 1 / 2
+SHELL_AMR_32x64x192_B16 = cubic_random_amr((32, 64, 192), 16)

@@ -38,6 +38,7 @@ def load():
         "falcon_tree_divide_all": [vp, C.c_int, C.c_int, C.c_int],
         "falcon_tree_amr": [vp, ip, C.c_int, dp],
         "falcon_tree_set_nodes": [vp, C.c_int, ip, ip, dp],
+        "falcon_tree_refine": [vp, C.c_int, ip],
         "falcon_nblocks": [vp, ip, ip],
         "falcon_block_info": [vp, C.c_int, dp, ip, dp, dp, dp],
         "falcon_set_gc_all": [vp],
@@ -80,6 +81,7 @@ def load():
         "falcon_heat_train_corrector": [vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_double,
                                         C.c_uint64, dp, i64p, dp],
         "falcon_heat_attach_corrector": [vp],
+        "falcon_heat_set_learned_sweeps": [vp, C.c_int, C.c_int],
         "falcon_last_error": [C.c_char_p, C.c_int],
     }
     for name, args in sig.items():
@@ -122,6 +124,9 @@ class Shell:
         for op in layout.get("ops", []):
             if op[0] == "split":
                 _check(L.falcon_tree_divide_all(h, *[int(x) for x in op[1]]))
+            elif op[0] == "refine":
+                ids = np.asarray(op[1], dtype=np.int32)
+                _check(L.falcon_tree_refine(h, len(ids), _p(ids, C.c_int32)))
             elif op[0] == "amr":
                 f = np.asarray(op[1], dtype=np.int32)
                 r = np.asarray(op[3], dtype=np.float64)
@@ -328,6 +333,10 @@ class HeatSolver:
 
     def attach_corrector(self):
         _check(self.L.falcon_heat_attach_corrector(self.s.h))
+
+    def set_learned_sweeps(self, pre0, post0):
+        """Finest-level smoothing sweeps of the V-cycle used with the learned correction."""
+        _check(self.L.falcon_heat_set_learned_sweeps(self.s.h, pre0, post0))
 
 
 class Cnn:

@@ -1,10 +1,10 @@
 // Learned V-cycle correction network on MFMA (gfx950): forward (fused, all
 // three layers in LDS), data and weight gradients (implicit GEMM on
-// v_mfma_f32_16x16x32_bf16), mean-squared-error loss and Adam.
+// v_mfma_f32_16x16x32_bf16 / 16x16x16_bf16), mean-squared-error loss and Adam.
 //
-// MFMA operand images (16x16x32, 64 lanes, lane l = 16 g + n):
-//   A (16 x 32): lane l holds A[n][8g .. 8g+7]
-//   B (32 x 16): lane l holds B[8g .. 8g+7][n]
+// MFMA operand images (64 lanes, lane l = 16 g + n):
+//   16x16x32: A lane l holds A[n][8g .. 8g+7], B lane l holds B[8g .. 8g+7][n]
+//   16x16x16: A lane l holds A[n][4g .. 4g+3], B lane l holds B[4g .. 4g+3][n]
 //   D (16 x 16): lane l holds D[4g .. 4g+3][n]
 // A convolution is computed as D = W^T X: rows = output channels, columns =
 // 16 voxels of one k-line, K = (tap, input channel) pairs, so each lane ends
@@ -13,7 +13,9 @@
 //
 // Layouts: images x/out [img][4096] float; activations and their gradients
 // [img][4096][16] bfloat16 (channel fastest); voxel v = (i*16 + j)*16 + k.
-// LDS planes carry a zero halo: [18][18] (x) or [18][18][16] (activations).
+// LDS planes: x [18][18] with a zero halo; forward activations [16][16][16]
+// (k-shifts by DPP, out-of-block lines skipped); gradient kernels [18][18][16]
+// with a zero halo.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,20 +38,24 @@ namespace falcon {
 
 typedef unsigned short us8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
 constexpr int PL = 18 * 18;                 // plane with halo
 constexpr int OFF_W1 = 0, OFF_B1 = 432, OFF_W2 = 448, OFF_B2 = 7360, OFF_W3 = 7376, OFF_B3 = 7808;
-// packed operand images, 64 lanes x 8 bf16 per MFMA step:
-// P1 layer-1 forward (1 step), P2 layer-2 forward (14), P3 layer-3 forward (14),
-// Q3 layer-3 data gradient (1), Q2 layer-2 data gradient (14)
-constexpr int PK_P1 = 0, PK_P2 = 64, PK_P3 = PK_P2 + 14 * 64, PK_Q3 = PK_P3 + 14 * 64, PK_Q2 = PK_Q3 + 64,
-              PK_TOTAL = PK_Q2 + 14 * 64;
+// packed operand images of the 16x16x32 form, 64 lanes x 8 bf16 per MFMA step:
+// P1 layer-1 forward (1 step), Q3 layer-3 data gradient (1), Q2 layer-2 data gradient (14)
+constexpr int PK_P1 = 0, PK_Q3 = 64, PK_Q2 = 128, PK_TOTAL = PK_Q2 + 14 * 64;
 
-// fused forward LDS carve (uint16 elements)
-constexpr int F_H1 = 0, F_H2 = 3 * PL * 16, F_Z = 6 * PL * 16, F_X = 7 * PL * 16, F_TOTAL = F_X + 3 * PL;
+// forward operand images of the 16x16x16 form (4 bf16 per lane): W2 per tap
+// [27][64], W3 per (di,dj) pair [9][64] with rows n = dk+1 (n < 3)
+constexpr int PF_W2 = 0, PF_W3 = 27 * 64, PF_TOTAL = 36 * 64;
+// fused forward LDS carve (uint16 elements): h1 and h2 rings of 3 planes
+// [16 j][16 k][16 c], x ring of 3 planes [18][18] with halo plus a zero plane
+constexpr int PLANE = 16 * 16 * 16;
+constexpr int F_H1 = 0, F_H2 = 3 * PLANE, F_X = 6 * PLANE, F_XZ = F_X + 3 * PL, F_TOTAL = F_XZ + PL;
 // single-layer LDS carve: ring of 3 input planes + zero plane
 constexpr int S_IN = 0, S_Z = 3 * PL * 16, S_TOTAL = 4 * PL * 16;
 
@@ -63,6 +69,17 @@ __device__ __forceinline__ f4 mfma(us8 a, us8 b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0,
                                                    0);
 }
+// 16x16x16 form (K = 16 input channels of one tap): lane l = 16g + n holds
+// A[n][4g .. 4g+3] and B[4g .. 4g+3][n]; D as above.
+__device__ __forceinline__ f4 mfma16(uint2 a, uint2 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s4, a), __builtin_bit_cast(s4, b), c, 0, 0, 0);
+}
+// lane k of each 16-lane row takes lane k-1 (shr) or k+1 (shl); 0 at the row ends:
+// a shift along the k-line with the block's zero padding
+__device__ __forceinline__ uint32_t shr1(uint32_t v) { return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false); }
+__device__ __forceinline__ uint32_t shl1(uint32_t v) { return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x101, 0xf, 0xf, false); }
+__device__ __forceinline__ float shr1f(float v) { return __uint_as_float(shr1(__float_as_uint(v))); }
+__device__ __forceinline__ float shl1f(float v) { return __uint_as_float(shl1(__float_as_uint(v))); }
 __device__ __forceinline__ void tap_off(int t, int &di, int &dj, int &dk) {
     di = t / 9 - 1;
     dj = (t / 3) % 3 - 1;
@@ -79,15 +96,9 @@ __global__ void __launch_bounds__(256) k_cnn_pack(const float *__restrict__ p, u
     if (q >= PK_TOTAL * 8) return;
     const int e = q & 7, entry = q >> 3, lane = entry & 63, row = lane & 15, g = lane >> 4;
     float w = 0.0f;
-    if (entry < PK_P2) {                                   // P1: k = tap, row = co
+    if (entry < PK_Q3) {                                   // P1: k = tap, row = co
         const int t = 8 * g + e;
         if (t < 27) w = p[OFF_W1 + t * 16 + row];
-    } else if (entry < PK_P3) {                            // P2: k = (t, ci), row = co
-        const int k = 32 * ((entry - PK_P2) >> 6) + 8 * g + e, t = k >> 4, c = k & 15;
-        if (t < 27) w = p[OFF_W2 + (t * 16 + c) * 16 + row];
-    } else if (entry < PK_Q3) {                            // P3: k = (t, ci), row 0 = the output
-        const int k = 32 * ((entry - PK_P3) >> 6) + 8 * g + e, t = k >> 4, c = k & 15;
-        if (t < 27 && row == 0) w = p[OFF_W3 + t * 16 + c];
     } else if (entry < PK_Q2) {                            // Q3: k = tap (flipped), row = ci of layer 3
         const int t = 8 * g + e;
         if (t < 27) w = p[OFF_W3 + (26 - t) * 16 + row];
@@ -98,26 +109,60 @@ __global__ void __launch_bounds__(256) k_cnn_pack(const float *__restrict__ p, u
     pk[q] = f2bf(w);
 }
 
+__global__ void __launch_bounds__(256) k_cnn_pack16(const float *__restrict__ p, uint16_t *__restrict__ pk) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= PF_TOTAL * 4) return;
+    const int e = q & 3, entry = q >> 2, lane = entry & 63, n = lane & 15, g = lane >> 4;
+    float w = 0.0f;
+    if (entry < PF_W3) {                       // W2[t][ci = 4g+e][co = n]
+        const int t = entry >> 6;
+        w = p[OFF_W2 + (t * 16 + 4 * g + e) * 16 + n];
+    } else {                                   // W3[pair*3 + n][ci = 4g+e], rows n < 3
+        const int pr = (entry - PF_W3) >> 6;
+        if (n < 3) w = p[OFF_W3 + (pr * 3 + n) * 16 + 4 * g + e];
+    }
+    pk[q] = f2bf(w);
+}
+
 // ------------------------------------------------------------------ fused forward
 // One workgroup per image; march over the 16 i-planes keeping rings of 3 planes
 // of x, h1 and h2 in LDS: step n loads x plane n+1, computes h1 plane n, h2
-// plane n-1 and the output plane n-2.  STORE also writes h1/h2 (training).
-template <bool STORE>
+// plane n-1 and the output plane n-2.  Wave w owns the k-lines j = 4w .. 4w+3
+// of every plane.  Layers 2 and 3 run on 16x16x16 MFMAs with K = the 16
+// channels of one tap: each input line is read from LDS once per (plane
+// offset) and serves the 3 output lines it touches; its dk = -1 / +1 images are
+// DPP row shifts of the loaded operand (layer 2) or of the accumulated tap
+// rows (layer 3, whose A rows are the 3 dk taps).  STORE also writes h1/h2.
+struct HeatIO {                 // solver form of the forward (HEAT = true)
+    const double *scale;       // per system, stride scale_stride
+    int scale_stride, nb;
+    long long pstride;
+    double *cpad;
+};
+
+template <bool STORE, bool HEAT>
 __global__ void __launch_bounds__(256) k_cnn_fwd(const float *__restrict__ x, float *__restrict__ out,
-                                                 const us8 *__restrict__ pk, const float *__restrict__ prm,
-                                                 uint16_t *__restrict__ h1g, uint16_t *__restrict__ h2g) {
+                                                 const us8 *__restrict__ pk, const uint2 *__restrict__ pf,
+                                                 const float *__restrict__ prm, uint16_t *__restrict__ h1g,
+                                                 uint16_t *__restrict__ h2g, HeatIO hio) {
     __shared__ __align__(16) uint16_t sm[F_TOTAL];
     const long long img = blockIdx.x;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, kk = lane & 15;
+    double sc = 1.0;
+    float inv_sc = 1.0f;
+    if (HEAT) {
+        sc = hio.scale[(img / hio.nb) * hio.scale_stride];
+        inv_sc = sc > 0.0 ? (float)(1.0 / sc) : 0.0f;
+    }
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, n = lane & 15;
+    const int j0 = 4 * wave;
     for (int q = tid; q < F_TOTAL / 2; q += 256) reinterpret_cast<uint32_t *>(sm)[q] = 0u;
 
     const us8 w1 = pk[PK_P1 + lane];
-    us8 w2[14], w3[14];
+    uint2 w2[27], w3[9];
 #pragma unroll
-    for (int s = 0; s < 14; s++) {
-        w2[s] = pk[PK_P2 + s * 64 + lane];
-        w3[s] = pk[PK_P3 + s * 64 + lane];
-    }
+    for (int t = 0; t < 27; t++) w2[t] = pf[PF_W2 + t * 64 + lane];
+#pragma unroll
+    for (int pr = 0; pr < 9; pr++) w3[pr] = pf[PF_W3 + pr * 64 + lane];
     float bb1[4], bb2[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
@@ -130,22 +175,20 @@ __global__ void __launch_bounds__(256) k_cnn_fwd(const float *__restrict__ x, fl
 
     auto load_x = [&](int q) {
         const int j = tid >> 4, k = tid & 15;
-        sm[F_X + (q % 3) * PL + (j + 1) * 18 + k + 1] = f2bf(X[(q * 16 + j) * 16 + k]);
+        const float v = X[(q * 16 + j) * 16 + k];
+        sm[F_X + (q % 3) * PL + (j + 1) * 18 + k + 1] = f2bf(HEAT ? v * inv_sc : v);
     };
-    auto xplane = [&](int q) -> const uint16_t * { return (q < 0 || q > 15) ? sm + F_Z : sm + F_X + (q % 3) * PL; };
-    auto hplane = [&](int base, int q) -> const uint16_t * {
-        return (q < 0 || q > 15) ? sm + F_Z : sm + base + (q % 3) * PL * 16;
-    };
+    auto xplane = [&](int q) -> const uint16_t * { return (q < 0 || q > 15) ? sm + F_XZ : sm + F_X + (q % 3) * PL; };
 
     load_x(0);
-    for (int n = 0; n <= 17; n++) {
-        if (n + 1 <= 15) load_x(n + 1);
+    for (int step = 0; step <= 17; step++) {
+        if (step + 1 <= 15) load_x(step + 1);
         __syncthreads();
-        if (n <= 15) {                          // h1 plane n
-            const int p = n;
+        if (step <= 15) {                       // h1 plane step
+            const int p = step;
 #pragma unroll
-            for (int jl = 0; jl < 4; jl++) {
-                const int j = wave + 4 * jl;
+            for (int jj = 0; jj < 4; jj++) {
+                const int j = j0 + jj;
                 us8 b;
 #pragma unroll
                 for (int e = 0; e < 8; e++) {
@@ -154,60 +197,92 @@ __global__ void __launch_bounds__(256) k_cnn_fwd(const float *__restrict__ x, fl
                     if (t < 27) {
                         int di, dj, dk;
                         tap_off(t, di, dj, dk);
-                        val = xplane(p + di)[(j + dj + 1) * 18 + kk + dk + 1];
+                        val = xplane(p + di)[(j + dj + 1) * 18 + n + dk + 1];
                     }
                     b[e] = val;
                 }
                 const f4 acc = mfma(w1, b, f4{0.0f, 0.0f, 0.0f, 0.0f});
                 const uint2 o = pack4(relu(acc[0] + bb1[0]), relu(acc[1] + bb1[1]), relu(acc[2] + bb1[2]),
                                       relu(acc[3] + bb1[3]));
-                *reinterpret_cast<uint2 *>(sm + F_H1 + (p % 3) * PL * 16 + ((j + 1) * 18 + kk + 1) * 16 + 4 * g) = o;
-                if (STORE) *reinterpret_cast<uint2 *>(h1g + ((img * 4096 + (p * 16 + j) * 16 + kk) * 16 + 4 * g)) = o;
+                *reinterpret_cast<uint2 *>(sm + F_H1 + (p % 3) * PLANE + (j * 16 + n) * 16 + 4 * g) = o;
+                if (STORE) *reinterpret_cast<uint2 *>(h1g + ((img * 4096 + (p * 16 + j) * 16 + n) * 16 + 4 * g)) = o;
             }
         }
         __syncthreads();
-        if (n >= 1 && n <= 16) {                // h2 plane n-1
-            const int p = n - 1;
+        if (step >= 1 && step <= 16) {          // h2 plane step-1
+            const int p = step - 1;
+            f4 acc[4];
 #pragma unroll
-            for (int jl = 0; jl < 4; jl++) {
-                const int j = wave + 4 * jl;
-                f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int jj = 0; jj < 4; jj++) acc[jj] = f4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-                for (int s = 0; s < 14; s++) {
-                    const int t = 2 * s + (g >> 1);
-                    const uint16_t *src = sm + F_Z;
-                    if (t < 27) {
-                        int di, dj, dk;
-                        tap_off(t, di, dj, dk);
-                        src = hplane(F_H1, p + di) + ((j + dj + 1) * 18 + kk + dk + 1) * 16 + 8 * (g & 1);
+            for (int di = -1; di <= 1; di++) {
+                const int q = p + di;
+                if (q < 0 || q > 15) continue;
+                const uint16_t *P = sm + F_H1 + (q % 3) * PLANE;
+#pragma unroll
+                for (int jo = -1; jo <= 4; jo++) {
+                    const int J = j0 + jo;
+                    if (J < 0 || J > 15) continue;
+                    const uint2 b0 = *reinterpret_cast<const uint2 *>(P + (J * 16 + n) * 16 + 4 * g);
+                    const uint2 bm = make_uint2(shr1(b0.x), shr1(b0.y)), bp = make_uint2(shl1(b0.x), shl1(b0.y));
+#pragma unroll
+                    for (int jj = 0; jj < 4; jj++) {
+                        const int dj = jo - jj;
+                        if (dj < -1 || dj > 1) continue;
+                        const int tb = (di + 1) * 9 + (dj + 1) * 3;
+                        acc[jj] = mfma16(w2[tb + 0], bm, acc[jj]);
+                        acc[jj] = mfma16(w2[tb + 1], b0, acc[jj]);
+                        acc[jj] = mfma16(w2[tb + 2], bp, acc[jj]);
                     }
-                    acc = mfma(w2[s], *reinterpret_cast<const us8 *>(src), acc);
                 }
-                const uint2 o = pack4(relu(acc[0] + bb2[0]), relu(acc[1] + bb2[1]), relu(acc[2] + bb2[2]),
-                                      relu(acc[3] + bb2[3]));
-                *reinterpret_cast<uint2 *>(sm + F_H2 + (p % 3) * PL * 16 + ((j + 1) * 18 + kk + 1) * 16 + 4 * g) = o;
-                if (STORE) *reinterpret_cast<uint2 *>(h2g + ((img * 4096 + (p * 16 + j) * 16 + kk) * 16 + 4 * g)) = o;
+            }
+#pragma unroll
+            for (int jj = 0; jj < 4; jj++) {
+                const int j = j0 + jj;
+                const uint2 o = pack4(relu(acc[jj][0] + bb2[0]), relu(acc[jj][1] + bb2[1]), relu(acc[jj][2] + bb2[2]),
+                                      relu(acc[jj][3] + bb2[3]));
+                *reinterpret_cast<uint2 *>(sm + F_H2 + (p % 3) * PLANE + (j * 16 + n) * 16 + 4 * g) = o;
+                if (STORE) *reinterpret_cast<uint2 *>(h2g + ((img * 4096 + (p * 16 + j) * 16 + n) * 16 + 4 * g)) = o;
             }
         }
         __syncthreads();
-        if (n >= 2) {                           // output plane n-2
-            const int p = n - 2;
+        if (step >= 2) {                        // output plane step-2
+            const int p = step - 2;
+            f4 acc[4];
 #pragma unroll
-            for (int jl = 0; jl < 4; jl++) {
-                const int j = wave + 4 * jl;
-                f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int jj = 0; jj < 4; jj++) acc[jj] = f4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-                for (int s = 0; s < 14; s++) {
-                    const int t = 2 * s + (g >> 1);
-                    const uint16_t *src = sm + F_Z;
-                    if (t < 27) {
-                        int di, dj, dk;
-                        tap_off(t, di, dj, dk);
-                        src = hplane(F_H2, p + di) + ((j + dj + 1) * 18 + kk + dk + 1) * 16 + 8 * (g & 1);
+            for (int di = -1; di <= 1; di++) {
+                const int q = p + di;
+                if (q < 0 || q > 15) continue;
+                const uint16_t *P = sm + F_H2 + (q % 3) * PLANE;
+#pragma unroll
+                for (int jo = -1; jo <= 4; jo++) {
+                    const int J = j0 + jo;
+                    if (J < 0 || J > 15) continue;
+                    const uint2 b0 = *reinterpret_cast<const uint2 *>(P + (J * 16 + n) * 16 + 4 * g);
+#pragma unroll
+                    for (int jj = 0; jj < 4; jj++) {
+                        const int dj = jo - jj;
+                        if (dj < -1 || dj > 1) continue;
+                        acc[jj] = mfma16(w3[(di + 1) * 3 + (dj + 1)], b0, acc[jj]);
                     }
-                    acc = mfma(w3[s], *reinterpret_cast<const us8 *>(src), acc);
                 }
-                if (g == 0) out[img * 4096 + (p * 16 + j) * 16 + kk] = acc[0] + bb3;
+            }
+            // rows 0, 1, 2 hold the dk = -1, 0, +1 tap sums at the input voxel:
+            // out[k] = R0[k-1] + R1[k] + R2[k+1]
+#pragma unroll
+            for (int jj = 0; jj < 4; jj++) {
+                const float o = shr1f(acc[jj][0]) + acc[jj][1] + shl1f(acc[jj][2]);
+                if (g == 0) {
+                    if (HEAT) {
+                        const long long sys = img / hio.nb, b = img - sys * hio.nb;
+                        hio.cpad[sys * hio.pstride + b * 5832 + ((p + 1) * 18 + j0 + jj + 1) * 18 + n + 1] =
+                            sc * (double)(o + bb3);
+                    } else {
+                        out[img * 4096 + (p * 16 + j0 + jj) * 16 + n] = o + bb3;
+                    }
+                }
             }
         }
     }
@@ -492,7 +567,7 @@ CnnModel::CnnModel() {
     CCHECK(hipMalloc(&adam_m_, sizeof(float) * kCnnParams));
     CCHECK(hipMalloc(&adam_v_, sizeof(float) * kCnnParams));
     CCHECK(hipMalloc(&grad_, sizeof(double) * kCnnParams));
-    CCHECK(hipMalloc(&packed_, sizeof(uint16_t) * PK_TOTAL * 8));
+    CCHECK(hipMalloc(&packed_, sizeof(uint16_t) * (PK_TOTAL * 8 + PF_TOTAL * 4)));
     CCHECK(hipMalloc(&loss_part_, sizeof(double) * 1025));
     init(12345);
 }
@@ -535,6 +610,8 @@ void CnnModel::pack(hipStream_t stream) {
     if (packed_valid_) return;
     hipLaunchKernelGGL(k_cnn_pack, dim3((PK_TOTAL * 8 + 255) / 256), dim3(256), 0, stream, (const float *)params_,
                        (uint16_t *)packed_);
+    hipLaunchKernelGGL(k_cnn_pack16, dim3((PF_TOTAL * 4 + 255) / 256), dim3(256), 0, stream, (const float *)params_,
+                       (uint16_t *)packed_ + PK_TOTAL * 8);
     packed_valid_ = true;
 }
 
@@ -559,8 +636,19 @@ void CnnModel::ensure_batch(int nimg) {
 void CnnModel::infer(const float *x, float *out, long long nimg, hipStream_t stream) {
     if (nimg <= 0) return;
     pack(stream);
-    hipLaunchKernelGGL((k_cnn_fwd<false>), dim3((unsigned)nimg), dim3(256), 0, stream, x, out, (const us8 *)packed_,
-                       (const float *)params_, (uint16_t *)nullptr, (uint16_t *)nullptr);
+    hipLaunchKernelGGL((k_cnn_fwd<false, false>), dim3((unsigned)nimg), dim3(256), 0, stream, x, out,
+                       (const us8 *)packed_, (const uint2 *)((uint16_t *)packed_ + PK_TOTAL * 8), (const float *)params_,
+                       (uint16_t *)nullptr, (uint16_t *)nullptr, HeatIO{});
+}
+
+void CnnModel::infer_heat(const float *x, const double *scale, int scale_stride, int nb, long long pstride, double *cpad,
+                          long long nimg, hipStream_t stream) {
+    if (nimg <= 0) return;
+    pack(stream);
+    HeatIO hio{scale, scale_stride, nb, pstride, cpad};
+    hipLaunchKernelGGL((k_cnn_fwd<false, true>), dim3((unsigned)nimg), dim3(256), 0, stream, x, (float *)nullptr,
+                       (const us8 *)packed_, (const uint2 *)((uint16_t *)packed_ + PK_TOTAL * 8), (const float *)params_,
+                       (uint16_t *)nullptr, (uint16_t *)nullptr, hio);
 }
 
 void CnnModel::backward(const float *x, int nimg, hipStream_t stream) {
@@ -578,8 +666,8 @@ void CnnModel::backward(const float *x, int nimg, hipStream_t stream) {
 void CnnModel::gradient(const float *x, const float *gout, int nimg, double *grad_host, hipStream_t stream) {
     ensure_batch(nimg);
     pack(stream);
-    hipLaunchKernelGGL((k_cnn_fwd<true>), dim3(nimg), dim3(256), 0, stream, x, out_, (const us8 *)packed_,
-                       (const float *)params_, h1_, h2_);
+    hipLaunchKernelGGL((k_cnn_fwd<true, false>), dim3(nimg), dim3(256), 0, stream, x, out_, (const us8 *)packed_,
+                       (const uint2 *)((uint16_t *)packed_ + PK_TOTAL * 8), (const float *)params_, h1_, h2_, HeatIO{});
     CCHECK(hipMemcpyAsync(g3_, gout, sizeof(float) * 4096 * (size_t)nimg, hipMemcpyDeviceToDevice, stream));
     backward(x, nimg, stream);
     CCHECK(hipStreamSynchronize(stream));
@@ -589,8 +677,8 @@ void CnnModel::gradient(const float *x, const float *gout, int nimg, double *gra
 double CnnModel::train_step(const float *x, const float *y, int nimg, double lr, hipStream_t stream) {
     ensure_batch(nimg);
     pack(stream);
-    hipLaunchKernelGGL((k_cnn_fwd<true>), dim3(nimg), dim3(256), 0, stream, x, out_, (const us8 *)packed_,
-                       (const float *)params_, h1_, h2_);
+    hipLaunchKernelGGL((k_cnn_fwd<true, false>), dim3(nimg), dim3(256), 0, stream, x, out_, (const us8 *)packed_,
+                       (const uint2 *)((uint16_t *)packed_ + PK_TOTAL * 8), (const float *)params_, h1_, h2_, HeatIO{});
     const long long n = 4096LL * nimg;
     hipLaunchKernelGGL(k_cnn_loss, dim3(1024), dim3(256), 0, stream, (const float *)out_, y, n, g3_, loss_part_);
     hipLaunchKernelGGL(k_cnn_sum, dim3(1), dim3(64), 0, stream, (const double *)loss_part_, 1024, 1.0 / (double)n,

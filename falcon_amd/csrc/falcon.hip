@@ -461,6 +461,14 @@ int falcon_tree_amr(falcon_handle_t h, const int flags[3], int if_global, const 
     });
 }
 
+int falcon_tree_refine(falcon_handle_t h, int n, const int *leaf_ordinals) {
+    return guard([&] {
+        if (h->labelled) throw Error("tree already labelled");
+        if (n < 0) throw Error("n must be >= 0");
+        h->lay.tree.refine(std::vector<int>(leaf_ordinals, leaf_ordinals + n));
+    });
+}
+
 int falcon_tree_set_nodes(falcon_handle_t h, int n_nodes, const int *parent, const int *is_yin, const double *rng6) {
     return guard([&] {
         if (h->labelled) throw Error("tree already labelled");
@@ -1141,6 +1149,10 @@ int falcon_heat_train_corrector(falcon_handle_t h, int npairs, double dt_lo, dou
         if (pairs_made) *pairs_made = rec.count;
         if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     });
+}
+
+int falcon_heat_set_learned_sweeps(falcon_handle_t h, int pre0, int post0) {
+    return guard([&] { need_heat(h).set_learned_sweeps(pre0, post0); });
 }
 
 int falcon_heat_attach_corrector(falcon_handle_t h) {

@@ -272,18 +272,27 @@ __global__ void __launch_bounds__(256) k_hcopy(const double *__restrict__ a, dou
 }
 
 // ---- learned correction of the V-cycle
-// Az = A z and r' = r - A z (z padded with fresh ghosts; Az, r, r' interior)
-__global__ void __launch_bounds__(256) k_hresid2(HLev L, const double *__restrict__ z, const double *__restrict__ r,
-                                                 double *__restrict__ Az, double *__restrict__ rp,
-                                                 const double *__restrict__ inv_dt, long long pstride, long long istride) {
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= (long long)L.nb * L.M3) return;
-    const int s = blockIdx.y;
+// Block partial sums of up to 3 per-thread values into partial[s][p][blockIdx.x].
+template <int NP>
+__device__ __forceinline__ void block_partials(const double (&v)[NP], double *__restrict__ partial, int s) {
+    __shared__ double red[NP][4];
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+        double a = v[p];
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+        if ((threadIdx.x & 63) == 0) red[p][threadIdx.x >> 6] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < NP) {
+        const int p = threadIdx.x;
+        partial[((long long)s * 3 + p) * gridDim.x + blockIdx.x] = ((red[p][0] + red[p][1]) + red[p][2]) + red[p][3];
+    }
+}
+
+__device__ __forceinline__ double apply_cell(const HLev &L, const double *__restrict__ X, int b, int q, double inv_dt,
+                                             double &diag) {
     const int m = L.m;
-    const int b = (int)(c / L.M3);
-    const int q = (int)(c - (long long)b * L.M3);
     const int i = q / (m * m), j = (q / m) % m, k = q % m;
-    const double *X = z + s * pstride + (long long)b * L.P;
     const long long S1 = m + 2, SI = S1 * S1;
     const long long pc = pad_index(m, i, j, k);
     const double *ai = L.Ai + (long long)b * (m + 1) * m * m;
@@ -292,81 +301,100 @@ __global__ void __launch_bounds__(256) k_hresid2(HLev L, const double *__restric
     const double cim = ai[((long long)i * m + j) * m + k], cip = ai[((long long)(i + 1) * m + j) * m + k];
     const double cjm = aj[((long long)i * (m + 1) + j) * m + k], cjp = aj[((long long)i * (m + 1) + j + 1) * m + k];
     const double ckm = ak[((long long)i * m + j) * (m + 1) + k], ckp = ak[((long long)i * m + j) * (m + 1) + k + 1];
-    const double diag = L.V[(long long)b * L.M3 + q] * inv_dt[s] + L.off[(long long)b * L.M3 + q];
+    diag = L.V[(long long)b * L.M3 + q] * inv_dt + L.off[(long long)b * L.M3 + q];
     double nb = cim * X[pc - SI] + cip * X[pc + SI];
     nb = nb + cjm * X[pc - S1] + cjp * X[pc + S1];
     nb = nb + ckm * X[pc - 1] + ckp * X[pc + 1];
-    const double ax = diag * X[pc] - nb;
-    const long long io = s * istride + c;
-    Az[io] = ax;
-    rp[io] = r[io] - ax;
+    return diag * X[pc] - nb;
 }
 
-// partial sums of (r'/diag)^2 per system (partial[s][0][wg], as k_hdot)
-__global__ void __launch_bounds__(256) k_hfeatnorm(HLev L, const double *__restrict__ rp, const double *__restrict__ inv_dt,
-                                                   double *__restrict__ partial, long long istride) {
-    __shared__ double red[4];
+// Az = A z, r' = r - A z, network input u = r'/diag (float, unscaled) and
+// partial sums of u^2 (z padded with fresh ghosts)
+__global__ void __launch_bounds__(256) k_hfeat(HLev L, const double *__restrict__ z, const double *__restrict__ r,
+                                               double *__restrict__ Az, double *__restrict__ rp, float *__restrict__ u,
+                                               const double *__restrict__ inv_dt, double *__restrict__ partial,
+                                               long long pstride, long long istride) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int s = blockIdx.y;
-    const long long N = (long long)L.nb * L.M3;
-    double acc = 0.0;
-    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < N; c += (long long)gridDim.x * blockDim.x) {
-        const double u = rp[s * istride + c] / (L.V[c] * inv_dt[s] + L.off[c]);
-        acc += u * u;
+    double v[1] = {0.0};
+    if (c < (long long)L.nb * L.M3) {
+        const int b = (int)(c / L.M3);
+        const int q = (int)(c - (long long)b * L.M3);
+        double diag;
+        const double ax = apply_cell(L, z + s * pstride + (long long)b * L.P, b, q, inv_dt[s], diag);
+        const long long io = s * istride + c;
+        const double res = r[io] - ax;
+        Az[io] = ax;
+        rp[io] = res;
+        const double w = res / diag;
+        u[io] = (float)w;
+        v[0] = w * w;
     }
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) partial[((long long)s * 3) * gridDim.x + blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+    block_partials<1>(v, partial, s);
 }
 
-// network input: x[s][b][v] = (r'/diag) / scale_s as float (0 when the scale is 0)
-__global__ void __launch_bounds__(256) k_hfeat(HLev L, const double *__restrict__ rp, const double *__restrict__ inv_dt,
-                                               const double *__restrict__ sc, float *__restrict__ xin, long long istride) {
+// Ac = A c and partial sums of c.r' and c.Ac (c padded with fresh ghosts)
+__global__ void __launch_bounds__(256) k_happly_dot(HLev L, const double *__restrict__ cpad, const double *__restrict__ rp,
+                                                    double *__restrict__ Ac, const double *__restrict__ inv_dt,
+                                                    double *__restrict__ partial, long long pstride, long long istride) {
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= (long long)L.nb * L.M3) return;
     const int s = blockIdx.y;
-    const double scale = sc[kSc * s + 3];
-    const double u = rp[s * istride + c] / (L.V[c] * inv_dt[s] + L.off[c]);
-    xin[s * istride + c] = scale > 0.0 ? (float)(u / scale) : 0.0f;
+    double v[2] = {0.0, 0.0};
+    if (c < (long long)L.nb * L.M3) {
+        const int b = (int)(c / L.M3);
+        const int q = (int)(c - (long long)b * L.M3);
+        const double *C = cpad + s * pstride + (long long)b * L.P;
+        double diag;
+        const double ac = apply_cell(L, C, b, q, inv_dt[s], diag);
+        const int m = L.m;
+        const double cv = C[pad_index(m, q / (m * m), (q / m) % m, q % m)];
+        const long long io = s * istride + c;
+        Ac[io] = ac;
+        v[0] = cv * rp[io];
+        v[1] = cv * ac;
+    }
+    block_partials<2>(v, partial, s);
 }
 
-// correction c (padded) = scale_s * network output
-__global__ void __launch_bounds__(256) k_hcorr(HLev L, const float *__restrict__ yout, const double *__restrict__ sc,
-                                               double *__restrict__ cpad, long long pstride, long long istride) {
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= (long long)L.nb * L.M3) return;
-    const int s = blockIdx.y;
-    const int m = L.m;
-    const int b = (int)(c / L.M3);
-    const int q = (int)(c - (long long)b * L.M3);
-    cpad[s * pstride + (long long)b * L.P + pad_index(m, q / (m * m), (q / m) % m, q % m)] =
-        sc[kSc * s + 3] * (double)yout[s * istride + c];
-}
-
-// z += a c (padded interiors), Az += a Ac (interior), a = correction step of the system
+// z += a c (padded interiors), Az += a Ac, partial sums of z.q (q = A p of the
+// previous direction), a = correction step of the system
 __global__ void __launch_bounds__(256) k_hzcorr(HLev L, double *__restrict__ z, const double *__restrict__ cpad,
                                                 double *__restrict__ Az, const double *__restrict__ Ac,
-                                                const double *__restrict__ sc, long long pstride, long long istride) {
+                                                const double *__restrict__ q, const double *__restrict__ sc,
+                                                double *__restrict__ partial, long long pstride, long long istride) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    double v[1] = {0.0};
+    if (c < (long long)L.nb * L.M3) {
+        const double a = sc[kSc * s + 4];
+        const int m = L.m;
+        const int b = (int)(c / L.M3);
+        const int qq = (int)(c - (long long)b * L.M3);
+        const long long o = s * pstride + (long long)b * L.P + pad_index(m, qq / (m * m), (qq / m) % m, qq % m);
+        const double zn = z[o] + a * cpad[o];
+        z[o] = zn;
+        const long long io = s * istride + c;
+        Az[io] = Az[io] + a * Ac[io];
+        v[0] = zn * q[io];
+    }
+    block_partials<1>(v, partial, s);
+}
+
+// p = z - beta p (padded interiors) and q = Az - beta q: the new direction and
+// its image without an operator apply
+__global__ void __launch_bounds__(256) k_hupdate_pq(HLev L, double *__restrict__ p, const double *__restrict__ z,
+                                                    double *__restrict__ q, const double *__restrict__ Az,
+                                                    const double *__restrict__ sc, long long pstride, long long istride,
+                                                    int use_beta) {
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= (long long)L.nb * L.M3) return;
     const int s = blockIdx.y;
-    const double a = sc[kSc * s + 4];
+    const double be = use_beta ? sc[kSc * s + 1] : 0.0;
     const int m = L.m;
     const int b = (int)(c / L.M3);
-    const int q = (int)(c - (long long)b * L.M3);
-    const long long o = s * pstride + (long long)b * L.P + pad_index(m, q / (m * m), (q / m) % m, q % m);
-    z[o] = z[o] + a * cpad[o];
-    const long long io = s * istride + c;
-    Az[io] = Az[io] + a * Ac[io];
-}
-
-// q = Az - beta q (interior), the FCG direction's image without an operator apply
-__global__ void __launch_bounds__(256) k_hupdate_q(double *__restrict__ q, const double *__restrict__ Az,
-                                                   const double *__restrict__ sc, long long istride, int use_beta) {
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= istride) return;
-    const int s = blockIdx.y;
-    const double be = use_beta ? sc[kSc * s + 1] : 0.0;
+    const int qq = (int)(c - (long long)b * L.M3);
+    const long long o = s * pstride + (long long)b * L.P + pad_index(m, qq / (m * m), (qq / m) % m, qq % m);
+    p[o] = z[o] - be * p[o];
     const long long io = s * istride + c;
     q[io] = Az[io] - be * q[io];
 }
@@ -383,7 +411,8 @@ __global__ void __launch_bounds__(256) k_hrecord(HLev L, const float *__restrict
     if (threadIdx.x == 0) rscale[r] = sc[kSc * s + 3];
     for (int q = threadIdx.x; q < (int)L.M3; q += blockDim.x) {
         const long long io = s * istride + (long long)b * L.M3 + q;
-        X[r * L.M3 + q] = xin[io];
+        const double scale = sc[kSc * s + 3];
+        X[r * L.M3 + q] = scale > 0.0 ? (float)((double)xin[io] / scale) : 0.0f;
         W[r * L.M3 + q] = x[io] + z[s * pstride + (long long)b * L.P + pad_index(m, q / (m * m), (q / m) % m, q % m)];
     }
 }
@@ -416,11 +445,17 @@ struct HeatSolver::Impl {
     double *inv_dt = nullptr, *partial = nullptr, *dots = nullptr, *sc = nullptr;
     int *active = nullptr;
     // learned correction work arrays (allocated on first use)
-    double *Az = nullptr, *rp = nullptr, *Ac = nullptr, *cpad = nullptr;
-    float *xin = nullptr, *yout = nullptr;
+    double *Az = nullptr, *rp = nullptr, *Ac = nullptr, *cpad = nullptr, *partial_cells = nullptr;
+    float *xin = nullptr;
+    int learned_pre0 = 2, learned_post0 = 2;
+    unsigned ncell_blocks() const { return (unsigned)((ist(0) + 255) / 256); }
     void ensure_learned();
     void features(double *z);
     void learned_correction(double *z);
+    void final_dots(int npairs, unsigned nwg) {
+        hipLaunchKernelGGL(k_hdot_final, dim3((unsigned)nsys), dim3(64), 0, stream, (const double *)partial_cells, (int)nwg,
+                           npairs, dots);
+    }
     int dot_wg = 1024;
     int pre = 2, post = 2, coarse_sweeps = 8;
     double omega = 0.8;
@@ -469,12 +504,15 @@ struct HeatSolver::Impl {
     const double *inv_dt_l(int) const { return inv_dt; }
 
     // V-cycle: z0 (padded, level 0) = M r0 (interior, level 0)
-    void vcycle(const double *r0, double *&z0) {
+    // pre0 / post0: smoothing sweeps on the finest level (others use pre / post)
+    void vcycle(const double *r0, double *&z0, int pre0 = -1, int post0 = -1) {
+        if (pre0 < 0) pre0 = pre;
+        if (post0 < 0) post0 = post;
         R[0] = const_cast<double *>(r0);
         Z[0] = z0;
         for (int l = 0; l < nlev; l++) {
             zero(Z[l], pst(l) * nsys);
-            const int sweeps = (l == nlev - 1) ? coarse_sweeps : pre;
+            const int sweeps = (l == nlev - 1) ? coarse_sweeps : (l == 0 ? pre0 : pre);
             for (int s = 0; s < sweeps; s++) jacobi(l, Z[l], Z2[l], R[l]);
             if (l == nlev - 1) break;
             exchange(l, Z[l]);
@@ -486,7 +524,8 @@ struct HeatSolver::Impl {
         for (int l = nlev - 2; l >= 0; l--) {
             hipLaunchKernelGGL(k_hprolong, grid_cells(l), dim3(256), 0, stream, L[l].m, nb, Z[l + 1], Z[l], pst(l),
                                pst(l + 1));
-            for (int s = 0; s < post; s++) jacobi(l, Z[l], Z2[l], R[l]);
+            const int sweeps = l == 0 ? post0 : post;
+            for (int s = 0; s < sweeps; s++) jacobi(l, Z[l], Z2[l], R[l]);
         }
         z0 = Z[0];
     }
@@ -578,6 +617,11 @@ long long HeatSolver::cells() const { return (long long)d->nb * d->L[0].M3; }
 int HeatSolver::m() const { return d->L[0].m; }
 hipStream_t HeatSolver::stream() const { return d->stream; }
 void HeatSolver::set_corrector(HeatCorrector *c) { d->corr = c; }
+void HeatSolver::set_learned_sweeps(int pre0, int post0) {
+    if (pre0 < 0 || post0 < 0 || pre0 > 8 || post0 > 8) throw std::runtime_error("sweeps must be in [0, 8]");
+    d->learned_pre0 = pre0;
+    d->learned_post0 = post0;
+}
 
 double HeatSolver::explicit_dt() const { return d->dt_expl; }
 
@@ -653,42 +697,39 @@ void HeatSolver::Impl::ensure_learned() {
     rp = dalloc(ist(0) * nsys);
     Ac = dalloc(ist(0) * nsys);
     cpad = dalloc(pst(0) * nsys);
+    partial_cells = dalloc((size_t)nsys * 3 * ncell_blocks());
     void *a = nullptr;
     HCHECK(hipMalloc(&a, sizeof(float) * ist(0) * nsys));
     allocs.push_back(a);
     xin = (float *)a;
-    HCHECK(hipMalloc(&a, sizeof(float) * ist(0) * nsys));
-    allocs.push_back(a);
-    yout = (float *)a;
 }
 
-// r' = r - A z, Az, and the network input images (features scaled per system)
+// Az = A z, r' = r - A z, network input r'/diag and its per-system scale
+// (root mean square) in sc[3]
 void HeatSolver::Impl::features(double *z) {
     exchange(0, z);
-    hipLaunchKernelGGL(k_hresid2, grid_cells(0), dim3(256), 0, stream, L[0], (const double *)z, (const double *)r, Az, rp,
-                       (const double *)inv_dt, pst(0), ist(0));
-    hipLaunchKernelGGL(k_hfeatnorm, dim3((unsigned)dot_wg, (unsigned)nsys), dim3(256), 0, stream, L[0],
-                       (const double *)rp, (const double *)inv_dt, partial, ist(0));
-    hipLaunchKernelGGL(k_hdot_final, dim3((unsigned)nsys), dim3(64), 0, stream, (const double *)partial, dot_wg, 1, dots);
+    hipLaunchKernelGGL(k_hfeat, grid_cells(0), dim3(256), 0, stream, L[0], (const double *)z, (const double *)r, Az, rp, xin,
+                       (const double *)inv_dt, partial_cells, pst(0), ist(0));
+    final_dots(1, ncell_blocks());
     hipLaunchKernelGGL(k_hscalars, dim3((nsys + 63) / 64), dim3(64), 0, stream, 3, nsys, (const double *)dots, sc,
                        (const int *)active, (double)ist(0));
-    hipLaunchKernelGGL(k_hfeat, grid_cells(0), dim3(256), 0, stream, L[0], (const double *)rp, (const double *)inv_dt,
-                       (const double *)sc, xin, ist(0));
 }
 
-// z += a c with c = scale * net(features) and a = <c,r'>/<c,Ac>; Az += a Ac
+// c = scale * net(features / scale), a = <c,r'>/<c,Ac>, z += a c, Az += a Ac,
+// and beta = <z, q>/gamma of the FCG direction update (sc[1])
 void HeatSolver::Impl::learned_correction(double *z) {
-    corr->infer(xin, yout, (long long)nsys * nb, stream);
-    hipLaunchKernelGGL(k_hcorr, grid_cells(0), dim3(256), 0, stream, L[0], (const float *)yout, (const double *)sc, cpad,
-                       pst(0), ist(0));
+    corr->infer_heat(xin, sc + 3, kSc, nb, pst(0), cpad, (long long)nsys * nb, stream);
     exchange(0, cpad);
-    hipLaunchKernelGGL((k_happly<0>), grid_cells(0), dim3(256), 0, stream, L[0], (const double *)cpad,
-                       (const double *)nullptr, Ac, (const double *)inv_dt, pst(0), ist(0), 0.0);
-    dot(0, 2, cpad, 1, rp, 0, cpad, 1, Ac, 0);
+    hipLaunchKernelGGL(k_happly_dot, grid_cells(0), dim3(256), 0, stream, L[0], (const double *)cpad, (const double *)rp,
+                       Ac, (const double *)inv_dt, partial_cells, pst(0), ist(0));
+    final_dots(2, ncell_blocks());
     hipLaunchKernelGGL(k_hscalars, dim3((nsys + 63) / 64), dim3(64), 0, stream, 2, nsys, (const double *)dots, sc,
                        (const int *)active, 0.0);
     hipLaunchKernelGGL(k_hzcorr, grid_cells(0), dim3(256), 0, stream, L[0], z, (const double *)cpad, Az,
-                       (const double *)Ac, (const double *)sc, pst(0), ist(0));
+                       (const double *)Ac, (const double *)q, (const double *)sc, partial_cells, pst(0), ist(0));
+    final_dots(1, ncell_blocks());
+    hipLaunchKernelGGL(k_hscalars, dim3((nsys + 63) / 64), dim3(64), 0, stream, 1, nsys, (const double *)dots, sc,
+                       (const int *)active, 0.0);
 }
 
 HeatResult HeatSolver::solve(double tol, int max_iter, int mode, HeatRecord *rec) {
@@ -748,22 +789,26 @@ HeatResult HeatSolver::solve(double tol, int max_iter, int mode, HeatRecord *rec
     };
     auto precond = [&]() -> double * {
         double *z = I.Z[0];
-        I.vcycle(I.r, z);
         if (learned) {
+            I.vcycle(I.r, z, I.learned_pre0, I.learned_post0);
             I.features(z);
-            I.learned_correction(z);
+            I.learned_correction(z);        // also beta for the direction update
         } else if (mode == kSolveRecord) {
+            I.vcycle(I.r, z, I.learned_pre0, I.learned_post0);
             I.features(z);
             record(z);
+        } else {
+            I.vcycle(I.r, z);
         }
         return z;
     };
     double *z = precond();
-    hipLaunchKernelGGL(k_hupdate_p, g0, dim3(256), 0, I.stream, I.L[0], I.p, (const double *)z, (const double *)I.sc,
-                       I.pst(0), 0);
     if (learned)
-        hipLaunchKernelGGL(k_hupdate_q, dim3((unsigned)((ni + 255) / 256), (unsigned)ns), dim3(256), 0, I.stream, I.q,
-                           (const double *)I.Az, (const double *)I.sc, ni, 0);
+        hipLaunchKernelGGL(k_hupdate_pq, g0, dim3(256), 0, I.stream, I.L[0], I.p, (const double *)z, I.q,
+                           (const double *)I.Az, (const double *)I.sc, I.pst(0), ni, 0);
+    else
+        hipLaunchKernelGGL(k_hupdate_p, g0, dim3(256), 0, I.stream, I.L[0], I.p, (const double *)z, (const double *)I.sc,
+                           I.pst(0), 0);
     int it = 0;
     bool all0 = true;
     for (int s = 0; s < ns; s++) all0 = all0 && done[s];
@@ -794,14 +839,16 @@ HeatResult HeatSolver::solve(double tol, int max_iter, int mode, HeatRecord *rec
         if (changed) HCHECK(hipMemcpyAsync(I.active, act.data(), sizeof(int) * ns, hipMemcpyHostToDevice, I.stream));
         if (all || it >= max_iter) break;
         z = precond();
-        I.dot(0, 1, z, 1, I.q, 0);
-        hipLaunchKernelGGL(k_hscalars, dim3((ns + 63) / 64), dim3(64), 0, I.stream, 1, ns, (const double *)I.dots, I.sc,
-                           (const int *)I.active, 0.0);
-        hipLaunchKernelGGL(k_hupdate_p, g0, dim3(256), 0, I.stream, I.L[0], I.p, (const double *)z, (const double *)I.sc,
-                           I.pst(0), 1);
-        if (learned)
-            hipLaunchKernelGGL(k_hupdate_q, dim3((unsigned)((ni + 255) / 256), (unsigned)ns), dim3(256), 0, I.stream, I.q,
-                               (const double *)I.Az, (const double *)I.sc, ni, 1);
+        if (learned) {
+            hipLaunchKernelGGL(k_hupdate_pq, g0, dim3(256), 0, I.stream, I.L[0], I.p, (const double *)z, I.q,
+                               (const double *)I.Az, (const double *)I.sc, I.pst(0), ni, 1);
+        } else {
+            I.dot(0, 1, z, 1, I.q, 0);
+            hipLaunchKernelGGL(k_hscalars, dim3((ns + 63) / 64), dim3(64), 0, I.stream, 1, ns, (const double *)I.dots,
+                               I.sc, (const int *)I.active, 0.0);
+            hipLaunchKernelGGL(k_hupdate_p, g0, dim3(256), 0, I.stream, I.L[0], I.p, (const double *)z,
+                               (const double *)I.sc, I.pst(0), 1);
+        }
     }
     HCHECK(hipStreamSynchronize(I.stream));
     res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

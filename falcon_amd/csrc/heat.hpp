@@ -29,6 +29,12 @@ class HeatCorrector {
   public:
     virtual ~HeatCorrector() {}
     virtual void infer(const float *x, float *out, long long nimg, hipStream_t stream) = 0;
+    // Solver form: image img = system * nb + block; the network sees
+    // x / scale[system * scale_stride] and the result, times the same scale, is
+    // written as FP64 into the padded (18^3 per block) array cpad at
+    // system * pstride + block * 5832.  A zero scale gives a zero image.
+    virtual void infer_heat(const float *x, const double *scale, int scale_stride, int nb, long long pstride,
+                            double *cpad, long long nimg, hipStream_t stream) = 0;
 };
 
 // Training records collected during a solve (device arrays, capacity images):
@@ -68,6 +74,9 @@ class HeatSolver {
     void get_solution(int s, std::vector<double> &T) const;
     void get_rhs(int s, std::vector<double> &b) const;
     void set_corrector(HeatCorrector *c);
+    // finest-level smoothing sweeps of the V-cycle used with the learned
+    // correction (recording and learned solves); the plain method keeps 2 / 2
+    void set_learned_sweeps(int pre0, int post0);
     // residual left by the V-cycle for a given right-hand side (training pairs)
     void vcycle_residual(const double *r_dev, double *z_dev_padded, double *resid_dev);
     double *device_solution();

@@ -117,6 +117,24 @@ void Tree::amr(const int flags[3], int global, const double rng_rsun[6]) {
     }
 }
 
+// Local refinement: split the listed leaves (0-based ordinals in the current
+// depth-first order, Yin then Yang) into 2 x 2 x 2 children.
+void Tree::refine(const std::vector<int> &ordinals) {
+    std::vector<int> leaves;
+    std::vector<int> stack = {1, 0};
+    while (!stack.empty()) {
+        int n = stack.back(); stack.pop_back();
+        if (nodes[n].leaf) { leaves.push_back(n); continue; }
+        int nc = nodes[n].N[0] * nodes[n].N[1] * nodes[n].N[2];
+        for (int c = nc - 1; c >= 0; c--) stack.push_back(nodes[n].first_child + c);
+    }
+    const int N[3] = {2, 2, 2};
+    for (int o : ordinals) {
+        if (o < 0 || o >= (int)leaves.size()) throw std::runtime_error("refine: leaf ordinal out of range");
+        divide(leaves[o], N);
+    }
+}
+
 // YinYangTree_LabelNodes (3_6/ModYinYangTree.f90:402-420): depth-first, Yin then Yang
 void Tree::label() {
     nleaf = 0;

@@ -35,6 +35,7 @@ struct Tree {
     void divide(int node, const int N[3]);
     void divide_all(int nr, int nt, int np);
     void amr(const int flags[3], int global, const double rng_rsun[6]);
+    void refine(const std::vector<int> &ordinals);
     void label();
     // Leaf id (1-based) of the node containing rtp given in panel `yin`, or -777.
     int find(const double rtp[3], int yin) const;

@@ -35,6 +35,11 @@ int falcon_tree_divide_all(falcon_handle_t h, int nr, int nt, int np);
 /* One AMR pass, ModAMR_Divide_OneBranch (3_6/ModAMR.f90:65-99): flags =
  * rtp_if_divide, rng_rsun = rtp_range_Rsun(3,2) column-major. */
 int falcon_tree_amr(falcon_handle_t h, const int flags[3], int if_global, const double rng_rsun[6]);
+/* Local refinement (no counterpart in the reference's control files, which
+ * refine by rtp_range only): split the listed leaves, given as 0-based
+ * ordinals in the current depth-first order (Yin then Yang), into 2x2x2
+ * children with YinYangTree_Divide's rule. */
+int falcon_tree_refine(falcon_handle_t h, int n, const int *leaf_ordinals);
 /* Drop-in path: take the Fortran driver's own tree.  Nodes in depth-first
  * pre-order (Yin root first, then its subtree, then the Yang root); parent[n] = -1
  * for roots; rng6 = rtp_range(3,2) column-major per node. */
@@ -158,6 +163,10 @@ int falcon_cnn_train(falcon_handle_t h, int64_t nimg, const float *x, const floa
 int falcon_heat_train_corrector(falcon_handle_t h, int npairs, double dt_lo, double dt_hi, int max_solves, int epochs,
                                 int batch, double lr, uint64_t seed, double *loss, int64_t *pairs_made,
                                 double *seconds);
+/* Finest-level pre/post smoothing sweeps of the V-cycle that the learned
+ * correction follows (training records and learned solves; default 2 / 2, the
+ * plain method's).  The network can take the place of finest-level sweeps. */
+int falcon_heat_set_learned_sweeps(falcon_handle_t h, int pre0, int post0);
 /* attach the handle's network (e.g. after falcon_cnn_set_params) */
 int falcon_heat_attach_corrector(falcon_handle_t h);
 

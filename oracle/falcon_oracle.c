@@ -173,6 +173,32 @@ void orc_amr(orc_ctx *c, const int flags[3], int if_global, const double rng_rsu
     amr_branch(&c->yang, N, if_global, rr, tr, pr);
 }
 
+/* local refinement: split the listed leaves (0-based depth-first ordinals, Yin then Yang) 2x2x2 */
+static void leaves_dfs(onode *n, onode **out, int *k) {
+    if (n->leaf) out[(*k)++] = n;
+    else for (int i = 0; i < n->nch; i++) leaves_dfs(&n->ch[i], out, k);
+}
+static int count_leaves(const onode *n) {
+    if (n->leaf) return 1;
+    int s = 0;
+    for (int i = 0; i < n->nch; i++) s += count_leaves(&n->ch[i]);
+    return s;
+}
+
+int orc_refine(orc_ctx *c, int n, const int *ordinals) {
+    int total = count_leaves(&c->yin) + count_leaves(&c->yang), k = 0;
+    onode **lv = malloc(sizeof(onode *) * (size_t)total);
+    leaves_dfs(&c->yin, lv, &k);
+    leaves_dfs(&c->yang, lv, &k);
+    const int N[3] = {2, 2, 2};
+    for (int i = 0; i < n; i++) {
+        if (ordinals[i] < 0 || ordinals[i] >= total) { free(lv); return -1; }
+        node_divide(lv[ordinals[i]], N);
+    }
+    free(lv);
+    return 0;
+}
+
 static void label(orc_ctx *c, onode *n) {
     if (n->leaf) n->ileaf = ++c->nleaf;
     else for (int i = 0; i < n->nch; i++) label(c, &n->ch[i]);

@@ -26,6 +26,8 @@ void orc_destroy(orc_ctx *c);
 void orc_divide_all(orc_ctx *c, int nr, int nt, int np);
 /* One ModAMR_Divide_OneBranch pass (3_6/ModAMR.f90:65-99) on both panels. */
 void orc_amr(orc_ctx *c, const int flags[3], int if_global, const double rng_rsun[6]);
+/* split the listed leaves (0-based depth-first ordinals, Yin then Yang) 2x2x2; -1 on a bad ordinal */
+int orc_refine(orc_ctx *c, int n, const int *ordinals);
 /* Label leaves, build blocks, geometry and ghost tables. Returns leaf count. */
 int orc_finalize(orc_ctx *c);
 

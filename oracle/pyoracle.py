@@ -33,6 +33,7 @@ def lib():
         L.orc_destroy.argtypes = [vp]
         L.orc_divide_all.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.orc_amr.argtypes = [vp, ip, C.c_int, dp]
+        L.orc_refine.argtypes = [vp, C.c_int, ip]
         L.orc_finalize.argtypes = [vp]
         L.orc_nblocks.argtypes = [vp]
         L.orc_nleaf_yin.argtypes = [vp]
@@ -94,6 +95,10 @@ class Oracle:
         for op in layout.get("ops", []):
             if op[0] == "split":
                 L.orc_divide_all(self.h, *op[1])
+            elif op[0] == "refine":
+                ids = np.asarray(op[1], dtype=np.int32)
+                if L.orc_refine(self.h, len(ids), _p(ids, C.c_int32)) != 0:
+                    raise ValueError("refine: leaf ordinal out of range")
             else:
                 _, flags, glob, rng = op
                 f = np.asarray(flags, dtype=np.int32)

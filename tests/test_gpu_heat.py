@@ -18,6 +18,7 @@ KAPPA0, RREF = 1.0, 0.7
 LAYOUTS = {
     "cubic8": configs.cubic((16, 32, 96), 8),
     "refined8": configs.cubic_refined((16, 32, 96), 8),
+    "random_amr8": configs.cubic_random_amr((16, 32, 96), 8, seed=3),
 }
 _cache = {}
 

@@ -19,6 +19,7 @@ LAYOUTS = {
     "ref128": configs.SHELL_REF_24x48x144,
     "cubic_2x4x12": configs.cubic((16, 32, 96), 8),
     "cubic_3x6x18": configs.SHELL_24x48x144_B8,
+    "random_amr": configs.cubic_random_amr((16, 32, 96), 8, seed=3),
 }
 
 
