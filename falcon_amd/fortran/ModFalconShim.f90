@@ -1,0 +1,133 @@
+! The library handle of the drop-in path.  The Fortran driver keeps its YYTree
+! (3_6/ModYinYangTree.f90:55-81) and BlockType data; libfalcon_hip.so keeps every
+! block resident in HBM.  Host primitive arrays are stale between
+! falcon_sync_to_host calls, which the driver makes before ModCheck/ModSave.
+module ModFalconShim
+    use iso_c_binding
+    use falcon_c
+    use ModParameters,  only: ni, nj, nk, ng
+    use ModYinYangTree, only: YYTree, TreeNode
+    use ModBlock,       only: BlockType
+    implicit none
+    private
+    public :: falcon_h, falcon_check, falcon_attach, falcon_push_state, falcon_sync_to_host
+
+    type(c_ptr), save   :: falcon_h = c_null_ptr
+
+contains
+
+    ! The reference's only failure mode is `stop 1` (3_6/ModCheck.f90:38).
+    subroutine falcon_check(rc)
+        integer(c_int), intent(in)      :: rc
+        character(kind=c_char)          :: buf(1024)
+        character(len=1024)             :: msg
+        integer                         :: i, ierr
+        if (rc == 0) return
+        ierr = falcon_last_error(buf, 1024)
+        msg = ''
+        do i = 1, 1024
+            if (buf(i) == c_null_char) exit
+            msg(i:i) = buf(i)
+        end do
+        write(*, '(a,a)') 'falcon: ', trim(msg)
+        stop 1
+    end subroutine falcon_check
+
+    recursive subroutine count_nodes(node, n)
+        type(TreeNode), intent(in)      :: node
+        integer, intent(inout)          :: n
+        integer                         :: c
+        n = n + 1
+        if (.not. node%if_leaf) then
+            do c = 1, size(node%children)
+                call count_nodes(node%children(c), n)
+            end do
+        end if
+    end subroutine count_nodes
+
+    ! depth-first pre-order, children in their own order: the library's node list
+    recursive subroutine flatten(node, par, k, parent, is_yin, rng6)
+        type(TreeNode), intent(in)      :: node
+        integer, intent(in)             :: par
+        integer, intent(inout)          :: k
+        integer(c_int), intent(inout)   :: parent(:), is_yin(:)
+        real(c_double), intent(inout)   :: rng6(:,:)
+        integer                         :: me, c
+        k = k + 1
+        me = k
+        parent(me) = par - 1                    ! 0-based, -1 for a root
+        is_yin(me) = merge(1, 0, node%if_yin)
+        rng6(1:3, me) = node%rtp_range(:, 1)
+        rng6(4:6, me) = node%rtp_range(:, 2)
+        if (.not. node%if_leaf) then
+            do c = 1, size(node%children)
+                call flatten(node%children(c), me, k, parent, is_yin, rng6)
+            end do
+        end if
+    end subroutine flatten
+
+    ! Register the driver's tree, 1-D background and state with the library, and
+    ! build every table (in place of ModCommunication_SetGCAll, 3_6/ModCommunication.f90:22-64).
+    subroutine falcon_attach(Tree)
+        type(YYTree), intent(inout), target :: Tree
+        integer(c_int), allocatable     :: parent(:), is_yin(:)
+        real(c_double), allocatable     :: rng6(:,:), bg(:,:)
+        integer(c_int)                  :: nb, ny
+        integer                         :: n, k, ib
+        type(BlockType), pointer        :: B
+
+        if (c_associated(falcon_h)) call falcon_check(falcon_destroy(falcon_h))
+        call falcon_check(falcon_create(falcon_h, ni, nj, nk, ng, Tree%r_range_CGS(1), Tree%r_range_CGS(2)))
+        n = 0
+        call count_nodes(Tree%Yin, n)
+        call count_nodes(Tree%Yang, n)
+        allocate(parent(n), is_yin(n), rng6(6, n))
+        k = 0
+        call flatten(Tree%Yin, 0, k, parent, is_yin, rng6)
+        call flatten(Tree%Yang, 0, k, parent, is_yin, rng6)
+        call falcon_check(falcon_tree_set_nodes(falcon_h, int(n, c_int), parent, is_yin, rng6))
+        call falcon_check(falcon_nblocks(falcon_h, nb, ny))
+        if (nb /= Tree%nLocalBlocks) then
+            write(*, '(a)') 'falcon: one process owns every block on its GPU'
+            stop 1
+        end if
+        ! 1-D profiles as ModBlock_InitSSM reads them (3_6/ModBlock.f90:232-249)
+        allocate(bg(-ng+1:ni+ng, 8))
+        do ib = 1, Tree%nLocalBlocks
+            B => Tree%LocalBlocks(ib)
+            bg(:, 1) = B%g_I
+            bg(:, 2) = B%rho0_I
+            bg(:, 3) = B%p0_I
+            bg(:, 4) = B%te0_I
+            bg(:, 5) = B%gamma1_I
+            bg(:, 6) = B%gamma3_I
+            bg(:, 7) = B%diffusion_I + B%cooling_I
+            bg(:, 8) = B%Xi_rsst_I
+            call falcon_check(falcon_set_background(falcon_h, int(B%iBlock - 1, c_int), bg))
+        end do
+        call falcon_check(falcon_set_gc_all(falcon_h))
+        call falcon_push_state(Tree)
+    end subroutine falcon_attach
+
+    ! host -> HBM: rho1, vr, vt, vp, br, bt, bp, s1 of every block (psi stays on the host)
+    subroutine falcon_push_state(Tree)
+        type(YYTree), intent(inout), target :: Tree
+        integer                         :: ib
+        do ib = 1, Tree%nLocalBlocks
+            call falcon_check(falcon_set_state(falcon_h, int(Tree%LocalBlocks(ib)%iBlock - 1, c_int), &
+                                               Tree%LocalBlocks(ib)%primitive_IV(:, :, :, 1:8)))
+        end do
+    end subroutine falcon_push_state
+
+    ! HBM -> host, before ModCheck_primitive / ModSave_DoAll (3_6/test1.f90:61,69)
+    subroutine falcon_sync_to_host(Tree)
+        type(YYTree), intent(inout), target :: Tree
+        real(c_double), allocatable     :: prim(:, :, :, :)
+        integer                         :: ib
+        allocate(prim(-ng+1:ni+ng, -ng+1:nj+ng, -ng+1:nk+ng, 8))
+        do ib = 1, Tree%nLocalBlocks
+            call falcon_check(falcon_get_state(falcon_h, int(Tree%LocalBlocks(ib)%iBlock - 1, c_int), prim))
+            Tree%LocalBlocks(ib)%primitive_IV(:, :, :, 1:8) = prim
+        end do
+    end subroutine falcon_sync_to_host
+end module ModFalconShim

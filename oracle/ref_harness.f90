@@ -14,6 +14,10 @@
 ! the 1-D profiles come from the input file because the FreeEOS table that
 ! ModStratification_new reads is not in the reference tree.
 !
+! Built with -DFALCON_DROPIN (oracle/Makefile target `dropin`), the same
+! driver calls the Fortran shims of falcon_amd/fortran (ModCommunication_SetGCAll,
+! ModCommunication_SendRecvGC_new, ModAdvance_rk4) instead, stepping on the GPU.
+!
 ! Usage: falcon_ref_harness <input.bin> <output.bin>
 program falcon_ref_harness
     use ModParameters,  only:   ni,nj,nk,ng,nvar,iGeometry,iEquation,MpiSize,MpiRank,&
@@ -32,6 +36,13 @@ program falcon_ref_harness
     use ModBoundary,    only:   ModBoundary_Dynamo_primitives
     use ModWaveSpeed,   only:   ModWaveSpeed_Dynamo
     use ModTimeStep,    only:   ModTimeStep_Dynamo
+#ifdef FALCON_DROPIN
+    ! drop-in build: the shims of falcon_amd/fortran in place of the reference's
+    ! ModAdvance / ModCommunication, stepping on the GPU
+    use ModCommunication, only: ModCommunication_SetGCAll,ModCommunication_SendRecvGC_new
+    use ModAdvance,       only: ModAdvance_rk4
+    use ModFalconShim,    only: falcon_sync_to_host
+#endif
     implicit none
 
     type(YYTree),target         ::  Tree
@@ -167,9 +178,16 @@ program falcon_ref_harness
             B%primitive=>B%primitive_IV
         end do
 
-        ! initial ghost fill as 3_6/test1.f90:143-144
+        ! initial ghost fill as 3_6/test1.f90:133,143-144
+#ifdef FALCON_DROPIN
+        call ModCommunication_SetGCAll(Tree)
+        call ModCommunication_SendRecvGC_new(Tree,.false.)
+        call ModCommunication_SendRecvGC_new(Tree,.false.)
+        call falcon_sync_to_host(Tree)
+#else
         call harness_exchange(Tree,.false.)
         call harness_exchange(Tree,.false.)
+#endif
         call harness_dump(Tree,uo,0,0.0d0)
 
         ! single-operator dumps on the exchanged initial state
@@ -196,9 +214,16 @@ program falcon_ref_harness
         idump=1
         call system_clock(clk0,clk_rate)
         do istep=1,nsteps
+#ifdef FALCON_DROPIN
+            call ModAdvance_rk4(Tree,.true.,dt)
+#else
             call harness_rk4(Tree,dt)
+#endif
             if (idump<=ndump) then
                 if (dump_steps(idump)==istep) then
+#ifdef FALCON_DROPIN
+                    call falcon_sync_to_host(Tree)
+#endif
                     call harness_dump(Tree,uo,istep,dt)
                     idump=idump+1
                 end if
@@ -228,6 +253,13 @@ contains
         allocate(B%p1_III            (-ng+1:ni+ng,-ng+1:nj+ng,-ng+1:nk+ng))
         allocate(B%Xi_rsst_III       (-ng+1:ni+ng,-ng+1:nj+ng,-ng+1:nk+ng))
         allocate(B%v_wave_III        (-ng+1:ni+ng,-ng+1:nj+ng,-ng+1:nk+ng))
+        ! 1-D profiles as ModBlock_InitSSM keeps them (3_6/ModBlock.f90:238-248)
+        allocate(B%g_I(-ng+1:ni+ng),B%p0_I(-ng+1:ni+ng),B%rho0_I(-ng+1:ni+ng),B%te0_I(-ng+1:ni+ng))
+        allocate(B%gamma1_I(-ng+1:ni+ng),B%gamma3_I(-ng+1:ni+ng),B%diffusion_I(-ng+1:ni+ng))
+        allocate(B%cooling_I(-ng+1:ni+ng),B%Xi_rsst_I(-ng+1:ni+ng))
+        B%g_I=bg(:,1); B%rho0_I=bg(:,2); B%p0_I=bg(:,3); B%te0_I=bg(:,4)
+        B%gamma1_I=bg(:,5); B%gamma3_I=bg(:,6); B%diffusion_I=bg(:,7); B%cooling_I=0.0d0
+        B%Xi_rsst_I=bg(:,8)
         B%if_SSM=.true.
         do j=-ng+1,nj+ng
             do k=-ng+1,nk+ng
