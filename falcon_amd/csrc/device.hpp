@@ -38,7 +38,12 @@ struct DevGhosts {
     const int32_t *src;     // donor*8*P + cell of the interpolation base (variable 0)
     const double *w;        // 3 weights per entry
     const double *rot;      // 2 rotation coefficients per entry (read for Yin-Yang entries only)
-    int n_total, yy_begin;
+    int first, n_total, yy_begin;  // weighted entries [first, n_total): same-panel, then Yin-Yang from yy_begin
+    const int2 *copy;       // (dst, src) of the entries filled by a plain copy, sorted by dst
+    int n_copy;
+    const int2 *lerp;       // (dst, src) of the entries interpolated along r only, sorted by dst
+    const double *lerp_w;   // their r weight
+    int n_lerp;
 };
 
 struct DevBoundary {
@@ -47,8 +52,8 @@ struct DevBoundary {
 };
 
 // launchers (kernels.hip)
-template <typename T, typename TIN, typename TOUT>
-void launch_stage(const Dims &d, const DevTables &t, const TIN *in, const double *U, TOUT *out, float *shadow,
+template <typename T, typename TIN, typename TOUT, typename TB>
+void launch_stage(const Dims &d, const DevTables &t, const TIN *in, const TB *U, TOUT *out, float *shadow,
                   const double *dtp, double div, hipStream_t s);
 template <typename T, typename TIN>
 void launch_diffusion(const Dims &d, const DevTables &t, const TIN *in, const double *U, double *out, float *shadow,

@@ -86,10 +86,10 @@ struct falcon_handle {
     hipStream_t stream = nullptr;
     Dims dims{};
     DevTables dt{};
-    DevGhosts gh{};
+    DevGhosts gh64{}, gh32{};   // FP64 buffers (exact) and the FP32 deviation path
     DevBoundary bnd{};
     // device memory
-    DevBuf meta, radial, angular, ghost_dst, ghost_src, ghost_w, ghost_rot, bnd_list;
+    DevBuf meta, radial, angular, ghost64[4], ghost32[4], ghost_copy32, ghost_lerp32[2], bnd_list;   // ghost tables
 
     DevBuf U[2], F32[2], S64[2], S32[2], dtp, partial, rbuf;
     int cur = 0;             // index of the current state buffer in U[]
@@ -109,7 +109,9 @@ struct falcon_handle {
 
     ~falcon_handle() {
         if (graph) (void)hipGraphExecDestroy(graph);
-        DevBuf *all[] = {&meta, &radial, &angular, &ghost_dst, &ghost_src, &ghost_w, &ghost_rot, &bnd_list,
+        DevBuf *all[] = {&meta, &radial, &angular, &ghost64[0], &ghost64[1], &ghost64[2], &ghost64[3], &ghost32[0],
+                         &ghost32[1], &ghost32[2], &ghost32[3], &ghost_copy32, &ghost_lerp32[0], &ghost_lerp32[1],
+                         &bnd_list,
 
                          &U[0], &U[1], &F32[0], &F32[1], &S64[0], &S64[1], &S32[0], &S32[1], &dtp, &partial, &rbuf};
         for (DevBuf *b : all) b->free();
@@ -228,33 +230,76 @@ void upload_tables(falcon_handle *h) {
         return ((long long)(i + ng - 1) * NJ + (j + ng - 1)) * NK + (k + ng - 1);
     };
     if ((long long)nb * 8 * P >= (1ll << 31)) throw Error("state too large for 32-bit ghost offsets");
-    const size_t NG = G.size();
-    std::vector<size_t> order;
-    order.reserve(NG);
-    for (int pass = 0; pass < 2; pass++)
-        for (size_t e = 0; e < NG; e++)
-            if (G[e].yy == pass) order.push_back(e);
-    int yy_begin = 0;
-    while (yy_begin < (int)NG && !G[order[yy_begin]].yy) yy_begin++;
-    std::vector<int32_t> dst(NG), src(NG);
-    std::vector<double> w(3 * NG), rot(2 * NG);
-    for (size_t q = 0; q < NG; q++) {
-        const GhostEntry &e = G[order[q]];
-        dst[q] = (int32_t)((long long)e.recv_block * 8 * P + cell(e.gi, e.gj, e.gk));
-        src[q] = (int32_t)((long long)e.donor * 8 * P + cell(e.i0, e.j0, e.k0));
-        w[3 * q] = e.w[0]; w[3 * q + 1] = e.w[1]; w[3 * q + 2] = e.w[2];
-        rot[2 * q] = e.rot[0]; rot[2 * q + 1] = e.rot[1];
+    // FP64 buffers: every entry weighted, same-panel entries first, then
+    // Yin-Yang, each in target-list order.  FP32 buffers of the deviation path:
+    // same-panel entries whose weights lie within 1e-12 of 0 or 1 (same-level
+    // neighbours whose cell centres coincide up to rounding) become plain copies
+    // from the nearest donor cell, sorted by destination; the rest keep the
+    // weighted path in the same order.
+    const double eps = 1e-12;
+    auto near = [&](const GhostEntry &g) {
+        if (g.yy) return false;
+        for (int a = 0; a < 3; a++)
+            if (!(g.w[a] < eps || g.w[a] > 1.0 - eps)) return false;
+        return true;
+    };
+    auto snapped = [&](double w) { return w < eps || w > 1.0 - eps; };
+    auto lerp_r = [&](const GhostEntry &g) { return !g.yy && !snapped(g.w[0]) && snapped(g.w[1]) && snapped(g.w[2]); };
+    const long long SI = (long long)NJ * NK, SJ = NK;
+    std::vector<int2> copy32;
+    std::vector<std::pair<int2, double>> lerp32;
+    for (size_t e = 0; e < G.size(); e++) {
+        const GhostEntry &g = G[e];
+        const int32_t d0 = (int32_t)((long long)g.recv_block * 8 * P + cell(g.gi, g.gj, g.gk));
+        const long long s0 = (long long)g.donor * 8 * P + cell(g.i0, g.j0, g.k0) + (g.w[1] > 0.5 ? SJ : 0) +
+                             (g.w[2] > 0.5 ? 1 : 0);
+        if (near(g)) copy32.push_back(make_int2(d0, (int32_t)(s0 + (g.w[0] > 0.5 ? SI : 0))));
+        else if (lerp_r(g)) lerp32.push_back({make_int2(d0, (int32_t)s0), g.w[0]});
     }
+    std::sort(copy32.begin(), copy32.end(), [](const int2 &a, const int2 &b) { return a.x < b.x; });
+    std::sort(lerp32.begin(), lerp32.end(), [](const auto &a, const auto &b) { return a.first.x < b.first.x; });
+    std::vector<int2> lerp_idx(lerp32.size());
+    std::vector<double> lerp_w(lerp32.size());
+    for (size_t q = 0; q < lerp32.size(); q++) { lerp_idx[q] = lerp32[q].first; lerp_w[q] = lerp32[q].second; }
     auto up = [&](DevBuf &buf, const void *data, size_t bytes) {
         buf.alloc(std::max<size_t>(bytes, 16));
         if (bytes) HIPCHECK(hipMemcpy(buf.p, data, bytes, hipMemcpyHostToDevice));
     };
-    up(h->ghost_dst, dst.data(), sizeof(int32_t) * NG);
-    up(h->ghost_src, src.data(), sizeof(int32_t) * NG);
-    up(h->ghost_w, w.data(), sizeof(double) * 3 * NG);
-    up(h->ghost_rot, rot.data(), sizeof(double) * 2 * NG);
-    h->gh = {h->ghost_dst.as<int32_t>(), h->ghost_src.as<int32_t>(), h->ghost_w.as<double>(), h->ghost_rot.as<double>(),
-             (int)NG, yy_begin};
+    // weighted table of the entries passing `keep`, plain then Yin-Yang
+    auto weighted = [&](auto keep, DevBuf *bufs) -> DevGhosts {
+        std::vector<size_t> order;
+        for (int pass = 0; pass < 2; pass++)
+            for (size_t e = 0; e < G.size(); e++)
+                if (G[e].yy == pass && keep(G[e])) order.push_back(e);
+        const size_t NG = order.size();
+        int yy_begin = 0;
+        while (yy_begin < (int)NG && !G[order[yy_begin]].yy) yy_begin++;
+        std::vector<int32_t> dst(NG), src(NG);
+        std::vector<double> w(3 * NG), rot(2 * NG);
+        for (size_t q = 0; q < NG; q++) {
+            const GhostEntry &e = G[order[q]];
+            dst[q] = (int32_t)((long long)e.recv_block * 8 * P + cell(e.gi, e.gj, e.gk));
+            src[q] = (int32_t)((long long)e.donor * 8 * P + cell(e.i0, e.j0, e.k0));
+            w[3 * q] = e.w[0]; w[3 * q + 1] = e.w[1]; w[3 * q + 2] = e.w[2];
+            rot[2 * q] = e.rot[0]; rot[2 * q + 1] = e.rot[1];
+        }
+        up(bufs[0], dst.data(), sizeof(int32_t) * NG);
+        up(bufs[1], src.data(), sizeof(int32_t) * NG);
+        up(bufs[2], w.data(), sizeof(double) * 3 * NG);
+        up(bufs[3], rot.data(), sizeof(double) * 2 * NG);
+        return DevGhosts{bufs[0].as<int32_t>(), bufs[1].as<int32_t>(), bufs[2].as<double>(), bufs[3].as<double>(),
+                         0, (int)NG, yy_begin, nullptr, 0, nullptr, nullptr, 0};
+    };
+    h->gh64 = weighted([](const GhostEntry &) { return true; }, h->ghost64);
+    h->gh32 = weighted([&](const GhostEntry &g) { return !near(g) && !lerp_r(g); }, h->ghost32);
+    up(h->ghost_copy32, copy32.data(), sizeof(int2) * copy32.size());
+    up(h->ghost_lerp32[0], lerp_idx.data(), sizeof(int2) * lerp_idx.size());
+    up(h->ghost_lerp32[1], lerp_w.data(), sizeof(double) * lerp_w.size());
+    h->gh32.copy = h->ghost_copy32.as<int2>();
+    h->gh32.n_copy = (int)copy32.size();
+    h->gh32.lerp = h->ghost_lerp32[0].as<int2>();
+    h->gh32.lerp_w = h->ghost_lerp32[1].as<double>();
+    h->gh32.n_lerp = (int)lerp_idx.size();
 
     std::vector<int32_t> bl;
     for (int b = 0; b < nb; b++) {
@@ -335,14 +380,14 @@ void enqueue_step(falcon_handle *h, double cfl, int cur) {
         for (int k = 1; k <= 4; k++) {
             timed(h, K_BC, [&] { launch_boundary<double>(d, h->bnd, const_cast<double *>(in[k - 1]), s); });
             timed(h, K_STAGE, [&] {
-                launch_stage<double, double, double>(d, t, in[k - 1], U, out[k - 1], nullptr, dtp,
-                                                     (double)(5.0f - (float)k), s);
+                launch_stage<double, double, double, double>(d, t, in[k - 1], U, out[k - 1], nullptr, dtp,
+                                                             (double)(5.0f - (float)k), s);
             });
-            timed(h, k < 4 ? K_XSTAGE : K_XSTATE, [&] { launch_exchange<double>(d, h->gh, out[k - 1], s); });
+            timed(h, k < 4 ? K_XSTAGE : K_XSTATE, [&] { launch_exchange<double>(d, h->gh64, out[k - 1], s); });
         }
         timed(h, K_DIFF, [&] { launch_diffusion<double, double>(d, t, U, U, Un, nullptr, part, dtp, s); });
         timed(h, K_OTHER, [&] { launch_copy_radial<double>(d, h->bnd, U, Un, s); });
-        timed(h, K_XSTATE, [&] { launch_exchange<double>(d, h->gh, Un, s); });
+        timed(h, K_XSTATE, [&] { launch_exchange<double>(d, h->gh64, Un, s); });
     } else {
         float *S0 = h->S32[0].as<float>(), *S1 = h->S32[1].as<float>();
         float *F = h->F32[cur].as<float>(), *Fn = h->F32[1 - cur].as<float>();
@@ -352,15 +397,18 @@ void enqueue_step(falcon_handle *h, double cfl, int cur) {
             timed(h, K_BC, [&] { launch_boundary<float>(d, h->bnd, const_cast<float *>(in[k - 1]), s); });
             const double div = (double)(5.0f - (float)k);
             if (k < 4) {
-                timed(h, K_STAGE, [&] { launch_stage<float, float, float>(d, t, in[k - 1], U, out[k - 1], nullptr, dtp, div, s); });
-                timed(h, K_XSTAGE, [&] { launch_exchange<float>(d, h->gh, out[k - 1], s); });
+                // intermediate stage values are FP32 temporaries: base = FP32 copy of U^n
+                timed(h, K_STAGE, [&] {
+                    launch_stage<float, float, float, float>(d, t, in[k - 1], F, out[k - 1], nullptr, dtp, div, s);
+                });
+                timed(h, K_XSTAGE, [&] { launch_exchange<float>(d, h->gh32, out[k - 1], s); });
             } else {
-                timed(h, K_STAGE, [&] { launch_stage<float, float, double>(d, t, in[3], U, U, F, dtp, div, s); });
-                timed(h, K_XSTATE, [&] { launch_exchange<float>(d, h->gh, F, s); });
+                timed(h, K_STAGE, [&] { launch_stage<float, float, double, double>(d, t, in[3], U, U, F, dtp, div, s); });
+                timed(h, K_XSTATE, [&] { launch_exchange<float>(d, h->gh32, F, s); });
             }
         }
         timed(h, K_DIFF, [&] { launch_diffusion<float, float>(d, t, F, U, Un, Fn, part, dtp, s); });
-        timed(h, K_XSTATE, [&] { launch_exchange<float>(d, h->gh, Fn, s); });
+        timed(h, K_XSTATE, [&] { launch_exchange<float>(d, h->gh32, Fn, s); });
     }
 }
 
@@ -370,14 +418,14 @@ void prepare_steps(falcon_handle *h) {
     const long long n = (long long)d.nblocks * 8 * d.P;
     if (h->precision == 1 && !h->shadow_valid) {
         if (!h->u_ghosts_valid) {
-            launch_exchange<double>(d, h->gh, h->U[h->cur].as<double>(), h->stream);
+            launch_exchange<double>(d, h->gh64, h->U[h->cur].as<double>(), h->stream);
             h->u_ghosts_valid = true;
         }
         launch_to_f32(h->U[h->cur].as<double>(), h->F32[h->cur].as<float>(), n, h->stream);
         h->shadow_valid = true;
     }
     if (h->precision == 0 && !h->u_ghosts_valid) {
-        launch_exchange<double>(d, h->gh, h->U[h->cur].as<double>(), h->stream);
+        launch_exchange<double>(d, h->gh64, h->U[h->cur].as<double>(), h->stream);
         h->u_ghosts_valid = true;
     }
     if (!h->partials_valid) {
@@ -396,7 +444,7 @@ void after_steps(falcon_handle *h) {
 void refresh_u_ghosts(falcon_handle *h) {
     if (h->u_ghosts_valid) return;
     double *U = h->U[h->cur].as<double>();
-    launch_exchange<double>(h->dims, h->gh, U, h->stream);
+    launch_exchange<double>(h->dims, h->gh64, U, h->stream);
     launch_boundary<double>(h->dims, h->bnd, U, h->stream);
     h->u_ghosts_valid = true;
 }
@@ -738,8 +786,8 @@ int falcon_init_synthetic(falcon_handle_t h, uint64_t seed, double s_amp, double
             });
         for (auto &x : th) x.join();
         HIPCHECK(hipMemcpy(h->U[h->cur].p, dev.data(), sizeof(double) * dev.size(), hipMemcpyHostToDevice));
-        launch_exchange<double>(h->dims, h->gh, h->U[h->cur].as<double>(), h->stream);
-        launch_exchange<double>(h->dims, h->gh, h->U[h->cur].as<double>(), h->stream);
+        launch_exchange<double>(h->dims, h->gh64, h->U[h->cur].as<double>(), h->stream);
+        launch_exchange<double>(h->dims, h->gh64, h->U[h->cur].as<double>(), h->stream);
         h->partials_valid = false;
         h->shadow_valid = false;
         h->u_ghosts_valid = true;
@@ -764,12 +812,12 @@ int falcon_sendrecv_gc(falcon_handle_t h, int which) {
         h->need_tables();
         ensure_device_tables(h);
         if (which == 0) {
-            launch_exchange<double>(h->dims, h->gh, h->U[h->cur].as<double>(), h->stream);
+            launch_exchange<double>(h->dims, h->gh64, h->U[h->cur].as<double>(), h->stream);
             h->shadow_valid = false;
         } else {
             alloc_stage(h);
-            if (h->precision == 0) launch_exchange<double>(h->dims, h->gh, h->S64[0].as<double>(), h->stream);
-            else launch_exchange<float>(h->dims, h->gh, h->S32[0].as<float>(), h->stream);
+            if (h->precision == 0) launch_exchange<double>(h->dims, h->gh64, h->S64[0].as<double>(), h->stream);
+            else launch_exchange<float>(h->dims, h->gh32, h->S32[0].as<float>(), h->stream);
         }
         check_launch();
         HIPCHECK(hipStreamSynchronize(h->stream));
@@ -926,7 +974,9 @@ int falcon_profile_step(falcon_handle_t h, int nsteps, double cfl, double *out_m
 
 int falcon_device_bytes(falcon_handle_t h, int64_t *bytes) {
     return guard([&] {
-        const DevBuf *all[] = {&h->meta, &h->radial, &h->angular, &h->ghost_dst, &h->ghost_src, &h->ghost_w, &h->ghost_rot,
+        const DevBuf *all[] = {&h->meta, &h->radial, &h->angular, &h->ghost64[0], &h->ghost64[1], &h->ghost64[2],
+                               &h->ghost64[3], &h->ghost32[0], &h->ghost32[1], &h->ghost32[2], &h->ghost32[3],
+                               &h->ghost_copy32, &h->ghost_lerp32[0], &h->ghost_lerp32[1],
 
                                &h->bnd_list, &h->U[0], &h->U[1], &h->F32[0], &h->F32[1], &h->S64[0], &h->S64[1],
                                &h->S32[0], &h->S32[1], &h->dtp, &h->partial, &h->rbuf};

@@ -215,12 +215,15 @@ __device__ __forceinline__ void cell_rhs(const T (&win)[8][5], LDSF L, const T x
     }
 }
 
-// MODE 0: out = U + (dt*R)/div on the interior (3_6/ModAdvance.f90:87-93);
-//         with SHADOW the same value is also stored rounded to FP32 in `shadow`.
+// MODE 0: out = U + (dt*R)/div on the interior (3_6/ModAdvance.f90:87-93), the
+//         sum formed in FP64 from the base U of type TB (the FP64 state, or for
+//         the FP32 path's intermediate stages its FP32 copy: those stage values
+//         are stored in FP32 anyway); with SHADOW the same value is also stored
+//         rounded to FP32 in `shadow`.
 // MODE 1: write R (interior layout [block][var][i][j][k], FP64) for parity tests.
-template <typename T, typename TIN, typename TOUT, int MODE, bool SHADOW>
+template <typename T, typename TIN, typename TOUT, typename TB, int MODE, bool SHADOW>
 __global__ void __launch_bounds__(256) k_stage(Dims d, DevTables t, const TIN *__restrict__ in,
-                                               const double *__restrict__ U, TOUT *__restrict__ out,
+                                               const TB *__restrict__ U, TOUT *__restrict__ out,
                                                float *__restrict__ shadow, const double *__restrict__ dtp,
                                                double div) {
     __shared__ T lds[2][8][PL][PL];
@@ -290,7 +293,7 @@ __global__ void __launch_bounds__(256) k_stage(Dims d, DevTables t, const TIN *_
 #pragma unroll
                 for (int v = 0; v < 8; v++) {
                     const long long q = base + (long long)v * d.P;
-                    const double x = U[q] + (dt * (double)R[v]) / div;
+                    const double x = (double)U[q] + (dt * (double)R[v]) / div;
                     out[q] = (TOUT)x;
                     if (SHADOW) shadow[q] = (float)x;
                 }
@@ -471,15 +474,15 @@ __global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const TI
 // donors are read only at interior cells (clamped bases touch a donor ghost only
 // with an exactly-zero weight), so entries are independent.
 //
-// SKIP (used for FP64 buffers): a corner whose weight factor is exactly zero
-// contributes f*0 = +-0 to the sum in the reference; its load is skipped and
-// the term taken as +0, which leaves every nonzero result bit-identical.  On the
-// production layouts this removes 60% of the corner loads (theta weights are
-// always exactly 0 or 1) and shortens the FP64 fill by 15%; for FP32 buffers
-// the unpredicated gather measures faster.
+// SKIP: a corner whose weight factor is exactly zero contributes f*0 = +-0 to
+// the sum in the reference; its load is skipped and the term taken as +0, which
+// leaves every nonzero result bit-identical.  On the production layouts this
+// removes 60% of the FP64 corner loads and shortens the FP64 fill by 15%.  For
+// the FP32 buffers (whose near-copies go to the copy table) the unpredicated
+// gather measures faster.
 template <typename TS, bool SKIP>
 __global__ void __launch_bounds__(256) k_exchange(Dims d, DevGhosts g, TS *__restrict__ st) {
-    const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int e = g.first + (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (e >= g.n_total) return;
     const long long P = d.P;
     const int SI = d.NJ * d.NK, SJ = d.NK;
@@ -521,6 +524,48 @@ __global__ void __launch_bounds__(256) k_exchange(Dims d, DevGhosts g, TS *__res
     }
 #pragma unroll
     for (int v = 0; v < 8; v++) st[dst + (long long)v * P] = (TS)val[v];
+}
+
+// Ghost entries filled by a plain copy, sorted by destination so consecutive
+// lanes copy consecutive phi cells.  For FP64 buffers these are the same-panel
+// entries whose three weights are exactly 0: the weighted sum reduces to the
+// base cell (an interior donor cell, the base being clamped to [1, n]) plus +0
+// from the vanishing terms.  For the FP32 buffers of the deviation path they
+// also include entries whose weights lie within 1e-12 of 0 or 1 (same-level
+// neighbours whose cell centres coincide up to rounding), copied from the
+// nearest donor cell.
+template <typename TS>
+__global__ void __launch_bounds__(256) k_exchange_copy(Dims d, DevGhosts g, TS *__restrict__ st) {
+    const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= g.n_copy) return;
+    const int2 c = g.copy[e];
+    const long long P = d.P;
+    TS val[8];
+#pragma unroll
+    for (int v = 0; v < 8; v++) val[v] = st[c.y + (long long)v * P];
+#pragma unroll
+    for (int v = 0; v < 8; v++) st[c.x + (long long)v * P] = val[v] + TS(0);
+}
+
+// FP32 buffers: same-panel entries whose theta/phi weights lie within 1e-12 of
+// 0 or 1 and whose r weight does not (neighbours across a radial block face,
+// whose radial spacings differ): the two r corners of the weighted sum.
+template <typename TS>
+__global__ void __launch_bounds__(256) k_exchange_lerp(Dims d, DevGhosts g, TS *__restrict__ st) {
+    const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= g.n_lerp) return;
+    const int2 c = g.lerp[e];
+    const double wi = g.lerp_w[e];
+    const long long P = d.P;
+    const int SI = d.NJ * d.NK;
+    double val[8];
+#pragma unroll
+    for (int v = 0; v < 8; v++) {
+        const TS *F = st + c.y + (long long)v * P;
+        val[v] = (double)F[0] * (1.0 - wi) + (double)F[SI] * wi;
+    }
+#pragma unroll
+    for (int v = 0; v < 8; v++) st[c.x + (long long)v * P] = (TS)val[v];
 }
 
 // ---------------------------------------------------------------- boundary
@@ -652,20 +697,20 @@ __global__ void k_to_f32(const double *__restrict__ s, float *__restrict__ d, lo
 
 static inline unsigned tiles(const Dims &d) { return (unsigned)(d.nblocks * d.tiles_j * d.tiles_k); }
 
-template <typename T, typename TIN, typename TOUT>
-void launch_stage(const Dims &d, const DevTables &t, const TIN *in, const double *U, TOUT *out, float *shadow,
+template <typename T, typename TIN, typename TOUT, typename TB>
+void launch_stage(const Dims &d, const DevTables &t, const TIN *in, const TB *U, TOUT *out, float *shadow,
                   const double *dtp, double div, hipStream_t s) {
     if (shadow)
-        hipLaunchKernelGGL((k_stage<T, TIN, TOUT, 0, true>), dim3(tiles(d)), dim3(256), 0, s, d, t, in, U, out, shadow,
-                           dtp, div);
+        hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, true>), dim3(tiles(d)), dim3(256), 0, s, d, t, in, U, out,
+                           shadow, dtp, div);
     else
-        hipLaunchKernelGGL((k_stage<T, TIN, TOUT, 0, false>), dim3(tiles(d)), dim3(256), 0, s, d, t, in, U, out,
+        hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, false>), dim3(tiles(d)), dim3(256), 0, s, d, t, in, U, out,
                            (float *)nullptr, dtp, div);
 }
 
 template <typename T, typename TIN>
 void launch_rhs_only(const Dims &d, const DevTables &t, const TIN *in, double *R, hipStream_t s) {
-    hipLaunchKernelGGL((k_stage<T, TIN, double, 1, false>), dim3(tiles(d)), dim3(256), 0, s, d, t, in,
+    hipLaunchKernelGGL((k_stage<T, TIN, double, double, 1, false>), dim3(tiles(d)), dim3(256), 0, s, d, t, in,
                        (const double *)nullptr, R, (float *)nullptr, (const double *)nullptr, 1.0);
 }
 
@@ -683,9 +728,13 @@ void launch_diffusion_only(const Dims &d, const DevTables &t, const double *U, d
 
 template <typename TS>
 void launch_exchange(const Dims &d, const DevGhosts &g, TS *state, hipStream_t s) {
-    if (g.n_total == 0) return;
-    hipLaunchKernelGGL((k_exchange<TS, sizeof(TS) == 8>), dim3((unsigned)((g.n_total + 255) / 256)), dim3(256), 0, s, d,
-                       g, state);
+    if (g.n_copy > 0)
+        hipLaunchKernelGGL((k_exchange_copy<TS>), dim3((unsigned)((g.n_copy + 255) / 256)), dim3(256), 0, s, d, g, state);
+    if (g.n_lerp > 0)
+        hipLaunchKernelGGL((k_exchange_lerp<TS>), dim3((unsigned)((g.n_lerp + 255) / 256)), dim3(256), 0, s, d, g, state);
+    if (g.n_total > g.first)
+        hipLaunchKernelGGL((k_exchange<TS, sizeof(TS) == 8>), dim3((unsigned)((g.n_total - g.first + 255) / 256)),
+                           dim3(256), 0, s, d, g, state);
 }
 
 template <typename TS>
@@ -715,9 +764,9 @@ void launch_to_f32(const double *src, float *dst, long long n, hipStream_t s) {
 }
 
 // explicit instantiations: FP64 path, FP32-deviation path
-template void launch_stage<double, double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, const double *, double, hipStream_t);
-template void launch_stage<float, float, float>(const Dims &, const DevTables &, const float *, const double *, float *, float *, const double *, double, hipStream_t);
-template void launch_stage<float, float, double>(const Dims &, const DevTables &, const float *, const double *, double *, float *, const double *, double, hipStream_t);
+template void launch_stage<double, double, double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, const double *, double, hipStream_t);
+template void launch_stage<float, float, float, float>(const Dims &, const DevTables &, const float *, const float *, float *, float *, const double *, double, hipStream_t);
+template void launch_stage<float, float, double, double>(const Dims &, const DevTables &, const float *, const double *, double *, float *, const double *, double, hipStream_t);
 template void launch_rhs_only<double, double>(const Dims &, const DevTables &, const double *, double *, hipStream_t);
 template void launch_rhs_only<float, double>(const Dims &, const DevTables &, const double *, double *, hipStream_t);
 template void launch_diffusion<double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, double *, const double *, hipStream_t);
