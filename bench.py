@@ -30,8 +30,11 @@ WORKLOADS = {
 }
 
 # algorithmic bytes per interior cell per stage-kernel launch (compulsory
-# interior traffic: read the stage input, read U^n, write the output)
-STAGE_BYTES = {"fp64": [192, 192, 192, 192], "fp32": [128, 128, 128, 160]}
+# interior traffic: read the stage input, read the accumulation base, write the
+# output).  FP32 path: stages 1-3 read the FP32 stage input and the FP32 copy of
+# U^n and write FP32 (32+32+32); stage 4 reads the FP32 input and FP64 U^n and
+# writes FP64 U and its FP32 copy (32+64+64+32... with the in-place U: 32+64+64)
+STAGE_BYTES = {"fp64": [192, 192, 192, 192], "fp32": [96, 96, 96, 160]}
 HBM_PEAK_GBS = 8000.0
 
 

@@ -350,11 +350,17 @@ __global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const TI
 
     T win[8][5];
     T nxt[8], hn[8];
+    // the upper r-face flux of plane i is the lower r-face flux of plane i+1:
+    // same four values, same speed sum (speed at the upper cell first), so it
+    // is carried instead of recomputed
+    T fr[8];
+    bool have_fr = false;
 #pragma unroll
     for (int v = 0; v < 8; v++) {
 #pragma unroll
         for (int q = 0; q < 5; q++) win[v][q] = m.act ? m.col(v, q) : T(0);
         hn[v] = m.hv ? m.halo(v, d.ng) : T(0);
+        fr[v] = T(0);
     }
     const TIN *Ub = in + (long long)b * 8 * d.P;
 
@@ -401,8 +407,10 @@ __global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const TI
             for (int v = 0; v < 8; v++) {
                 T r = T(0);
                 {
-                    T f0 = (face_flux(win[v][0], win[v][1], win[v][2], win[v][3]) * (c10 + c1m)) * T(0.5);
-                    T f1 = (face_flux(win[v][1], win[v][2], win[v][3], win[v][4]) * (c1p + c10)) * T(0.5);
+                    const T f0 = have_fr ? fr[v]
+                                         : (face_flux(win[v][0], win[v][1], win[v][2], win[v][3]) * (c10 + c1m)) * T(0.5);
+                    const T f1 = (face_flux(win[v][1], win[v][2], win[v][3], win[v][4]) * (c1p + c10)) * T(0.5);
+                    fr[v] = f1;
                     r = r + (f0 - f1) / dxi;
                 }
                 {
@@ -443,6 +451,7 @@ __global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const TI
                 hmin = x < hmin ? x : hmin;
             }
         }
+        have_fr = true;
 #pragma unroll
         for (int v = 0; v < 8; v++) {
             win[v][0] = win[v][1]; win[v][1] = win[v][2]; win[v][2] = win[v][3]; win[v][3] = win[v][4];
