@@ -70,6 +70,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--learned-sweeps", default="2,2",
                     help="finest-level pre,post smoothing sweeps of the V-cycle under the learned correction")
+    ap.add_argument("--learned-pairs", type=int, default=2000, help="solver-generated residual/error snapshots")
+    ap.add_argument("--learned-blocks", type=int, default=16, help="block images recorded per snapshot")
+    ap.add_argument("--learned-epochs", type=int, default=20)
     ap.add_argument("--legs", default="rk,heat,learned",
                     help="comma list of legs: rk (RK4 FP64/FP32), heat (MG-PCG), learned (learned FCG vs MG-PCG)")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
@@ -109,7 +112,8 @@ def main():
         heat = run_heat(cpu_baseline=not args.no_cpu_baseline)
         print(f"# heat: {heat['mg_pcg']}", file=sys.stderr, flush=True)
     if "learned" in legs:
-        learned = run_learned(sweeps=tuple(int(v) for v in args.learned_sweeps.split(",")))
+        learned = run_learned(npairs=args.learned_pairs, blocks_per_pair=args.learned_blocks,
+                              epochs=args.learned_epochs, sweeps=tuple(int(v) for v in args.learned_sweeps.split(",")))
         print(f"# learned: {json.dumps(learned)}", file=sys.stderr, flush=True)
         heat = heat or {}
         heat["learned_fcg"] = learned
@@ -263,7 +267,7 @@ def run_heat(cpu_baseline=True, repeats=3):
 LEARNED_WORKLOAD = "heat_amr_32x64x192_b16_x32"
 
 
-def run_learned(npairs=2000, epochs=20, batch=64, lr=2e-3, repeats=3, sweeps=(2, 2)):
+def run_learned(npairs=2000, blocks_per_pair=16, epochs=20, batch=128, lr=2e-3, repeats=3, sweeps=(2, 2)):
     """32 concurrent backward-Euler solves on the locally refined shell
     (level-0 32x64x192 per panel in 16^3 blocks, seeded 20% of blocks to level
     1 and 5% to level 2), dt spread geometrically over 10-300 x the explicit
@@ -276,8 +280,8 @@ def run_learned(npairs=2000, epochs=20, batch=64, lr=2e-3, repeats=3, sweeps=(2,
     s = Shell(lay)
     hs = HeatSolver(s, 1.0, 0.7, nsys)
     hs.set_learned_sweeps(*sweeps)
-    loss, pairs, train_s = hs.train_corrector(npairs=npairs, dt_range=(10.0, 300.0), max_solves=20, epochs=epochs,
-                                              batch=batch, lr=lr, seed=12345)
+    loss, pairs, train_s = hs.train_corrector(npairs=npairs, blocks_per_pair=blocks_per_pair, dt_range=(10.0, 300.0),
+                                              max_solves=20, epochs=epochs, batch=batch, lr=lr, seed=12345)
     dts = hs.dt_explicit * 10.0 * 30.0 ** (np.arange(nsys) / (nsys - 1))
     T = 1e-3 * np.random.default_rng(54321).standard_normal(hs.shape)
     hs.set_dt(dts)
@@ -285,8 +289,8 @@ def run_learned(npairs=2000, epochs=20, batch=64, lr=2e-3, repeats=3, sweeps=(2,
     out = {"workload": LEARNED_WORKLOAD, "blocks": s.nblocks, "systems": nsys, "cells_per_system": hs.cells,
            "tol": 1e-10, "dt_over_explicit": [10.0, 300.0],
            "learned_finest_sweeps": list(sweeps),
-           "training": {"pairs": pairs, "epochs": epochs, "batch": batch, "lr": lr, "final_mse": loss,
-                        "seconds": train_s}}
+           "training": {"pairs": pairs, "blocks_per_pair": blocks_per_pair, "epochs": epochs, "batch": batch, "lr": lr,
+                        "final_mse": loss, "seconds": train_s}}
     for name, learned in (("mg_pcg", False), ("learned_fcg", True)):
         hs.solve(1e-10, 300, use_corrector=learned)        # warm-up
         runs = [hs.solve(1e-10, 300, use_corrector=learned) for _ in range(repeats)]

@@ -78,10 +78,11 @@ def load():
         "falcon_cnn_forward": [vp, C.c_int64, fp, fp],
         "falcon_cnn_gradient": [vp, C.c_int, fp, fp, dp],
         "falcon_cnn_train": [vp, C.c_int64, fp, fp, C.c_int, C.c_int, C.c_double, C.c_uint64, dp],
-        "falcon_heat_train_corrector": [vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_double,
-                                        C.c_uint64, dp, i64p, dp],
+        "falcon_heat_train_corrector": [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                        C.c_double, C.c_uint64, dp, i64p, dp],
         "falcon_heat_attach_corrector": [vp],
         "falcon_heat_set_learned_sweeps": [vp, C.c_int, C.c_int],
+        "falcon_heat_set_graph": [vp, C.c_int],
         "falcon_last_error": [C.c_char_p, C.c_int],
     }
     for name, args in sig.items():
@@ -322,17 +323,22 @@ class HeatSolver:
         _check(self.L.falcon_heat_vcycle(self.s.h, _p(r), _p(z)))
         return z
 
-    def train_corrector(self, npairs=2000, dt_range=(10.0, 300.0), max_solves=50, epochs=10, batch=128, lr=2e-3,
-                        seed=1):
-        """Train the learned V-cycle correction on pairs generated by solves of
-        this solver's systems and attach it; returns (loss, pairs, seconds)."""
+    def train_corrector(self, npairs=2000, blocks_per_pair=16, dt_range=(10.0, 300.0), max_solves=50, epochs=10,
+                        batch=128, lr=2e-3, seed=1):
+        """Train the learned V-cycle correction on residual/error snapshots
+        generated by solves of this solver's systems (blocks_per_pair block
+        images each) and attach it; returns (loss, pairs, seconds)."""
         loss, n, sec = C.c_double(), C.c_int64(), C.c_double()
-        _check(self.L.falcon_heat_train_corrector(self.s.h, npairs, dt_range[0], dt_range[1], max_solves, epochs, batch,
-                                                  lr, seed, C.byref(loss), C.byref(n), C.byref(sec)))
+        _check(self.L.falcon_heat_train_corrector(self.s.h, npairs, blocks_per_pair, dt_range[0], dt_range[1], max_solves,
+                                                  epochs, batch, lr, seed, C.byref(loss), C.byref(n), C.byref(sec)))
         return loss.value, n.value, sec.value
 
     def attach_corrector(self):
         _check(self.L.falcon_heat_attach_corrector(self.s.h))
+
+    def set_graph(self, on):
+        """Replay each FCG iteration as a captured hipGraph (default) or launch eagerly."""
+        _check(self.L.falcon_heat_set_graph(self.s.h, 1 if on else 0))
 
     def set_learned_sweeps(self, pre0, post0):
         """Finest-level smoothing sweeps of the V-cycle used with the learned correction."""

@@ -708,6 +708,8 @@ double CnnModel::train(const float *x, const float *y, long long nimg, int epoch
     double last = 0.0;
     try {
         for (int ep = 0; ep < epochs; ep++) {
+            // learning rate decays linearly from lr to lr/10 over the epochs
+            const double lr_ep = epochs > 1 ? lr * (1.0 - 0.9 * (double)ep / (double)(epochs - 1)) : lr;
             for (long long i = 0; i < nimg; i++) idx[i] = i;
             std::shuffle(idx.begin(), idx.end(), rng);
             CCHECK(hipMemcpy(didx, idx.data(), sizeof(long long) * nimg, hipMemcpyHostToDevice));
@@ -716,7 +718,7 @@ double CnnModel::train(const float *x, const float *y, long long nimg, int epoch
             for (long long b0 = 0; b0 + batch <= nimg; b0 += batch) {
                 hipLaunchKernelGGL(k_cnn_gather, dim3(batch), dim3(256), 0, stream, x, (const long long *)(didx + b0), xb_);
                 hipLaunchKernelGGL(k_cnn_gather, dim3(batch), dim3(256), 0, stream, y, (const long long *)(didx + b0), yb_);
-                tot += train_step(xb_, yb_, batch, lr, stream) * batch;
+                tot += train_step(xb_, yb_, batch, lr_ep, stream) * batch;
                 seen += batch;
             }
             last = seen ? tot / (double)seen : 0.0;

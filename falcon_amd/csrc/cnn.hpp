@@ -30,11 +30,13 @@ class CnnModel : public HeatCorrector {
     void infer(const float *x, float *out, long long nimg, hipStream_t stream) override;
     void infer_heat(const float *x, const double *scale, int scale_stride, int nb, long long pstride, double *cpad,
                     long long nimg, hipStream_t stream) override;
+    void prepare(hipStream_t stream) override { pack(stream); }
     // sum over the batch of the gradient of sum(gout * out) (device x/gout, host grad)
     void gradient(const float *x, const float *gout, int nimg, double *grad_host, hipStream_t stream);
     // one Adam step on the mean squared error over a batch (device x, y); returns the loss
     double train_step(const float *x, const float *y, int nimg, double lr, hipStream_t stream);
-    // epochs over nimg device images in shuffled minibatches; returns the last epoch's mean loss
+    // epochs over nimg device images in shuffled minibatches, learning rate decaying
+    // linearly to lr/10; returns the last epoch's mean loss
     double train(const float *x, const float *y, long long nimg, int epochs, int batch, double lr, uint64_t seed,
                  hipStream_t stream);
     int max_batch() const { return max_batch_; }

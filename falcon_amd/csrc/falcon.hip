@@ -1164,20 +1164,21 @@ int falcon_cnn_train(falcon_handle_t h, int64_t nimg, const float *x, const floa
     });
 }
 
-int falcon_heat_train_corrector(falcon_handle_t h, int npairs, double dt_lo, double dt_hi, int max_solves, int epochs,
-                                int batch, double lr, uint64_t seed, double *loss, int64_t *pairs_made,
-                                double *seconds) {
+int falcon_heat_train_corrector(falcon_handle_t h, int npairs, int blocks_per_pair, double dt_lo, double dt_hi,
+                                int max_solves, int epochs, int batch, double lr, uint64_t seed, double *loss,
+                                int64_t *pairs_made, double *seconds) {
     return guard([&] {
         HeatSolver &H = need_heat(h);
-        if (npairs < 1 || !(dt_lo > 0) || !(dt_hi >= dt_lo) || max_solves < 1) throw Error("bad training settings");
+        if (npairs < 1 || blocks_per_pair < 1 || !(dt_lo > 0) || !(dt_hi >= dt_lo) || max_solves < 1)
+            throw Error("bad training settings");
         CnnModel &c = need_cnn(h);
         const auto t0 = std::chrono::steady_clock::now();
-        const size_t cap = (size_t)npairs;
+        const size_t cap = (size_t)npairs * (size_t)blocks_per_pair;
         DevTmp dX(sizeof(float) * 4096 * cap), dY(sizeof(float) * 4096 * cap), dW(sizeof(double) * 4096 * cap),
             dS(sizeof(double) * cap), dRs(sizeof(int) * cap), dRb(sizeof(int) * cap);
         HeatRecord rec;
         rec.capacity = (long long)cap;
-        rec.per_snapshot = std::max(1, std::min(32, npairs / 64 + 1));
+        rec.per_snapshot = blocks_per_pair;
         rec.rng = seed ^ 0x5851F42D4C957F2DULL;
         rec.X = (float *)dX.p; rec.Y = (float *)dY.p; rec.W = (double *)dW.p; rec.scale = (double *)dS.p;
         rec.rs = (int *)dRs.p; rec.rb = (int *)dRb.p;
@@ -1196,9 +1197,13 @@ int falcon_heat_train_corrector(falcon_handle_t h, int npairs, double dt_lo, dou
         const double l = c.train(rec.X, rec.Y, rec.count, epochs, batch, lr, seed + 1, h->stream);
         H.set_corrector(&c);
         if (loss) *loss = l;
-        if (pairs_made) *pairs_made = rec.count;
+        if (pairs_made) *pairs_made = rec.pairs;
         if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     });
+}
+
+int falcon_heat_set_graph(falcon_handle_t h, int on) {
+    return guard([&] { need_heat(h).set_graph(on != 0); });
 }
 
 int falcon_heat_set_learned_sweeps(falcon_handle_t h, int pre0, int post0) {

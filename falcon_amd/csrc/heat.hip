@@ -221,9 +221,42 @@ __global__ void k_hscalars(int what, int nsys, const double *__restrict__ dots, 
         const double num = dots[3 * s], den = dots[3 * s + 1];
         const double a = num / den;
         S[4] = (active[s] && den > 0.0 && a == a && a < 1e300 && a > -1e300) ? a : 0.0;
-    } else {
+    } else if (what == 3) {
         S[3] = sqrt(dots[3 * s] / ncell);
+    } else {
+        S[5] = dots[3 * s];                 // <b, b>
     }
+}
+
+// Convergence on the device (one workgroup): for every system still active,
+// count the iteration, store ||r||/||b|| and deactivate it at tol (or on a
+// non-finite residual); out[0] = number of systems left active.
+__global__ void __launch_bounds__(256) k_hconverge(int nsys, const double *__restrict__ dots,
+                                                   const double *__restrict__ sc, double tol, int *__restrict__ active,
+                                                   int *__restrict__ iters, double *__restrict__ relres,
+                                                   int *__restrict__ out) {
+    __shared__ int cnt[256];
+    int c = 0;
+    for (int s = threadIdx.x; s < nsys; s += blockDim.x) {
+        if (!active[s]) continue;
+        const double rel = sqrt(dots[3 * s] / sc[kSc * s + 5]);
+        iters[s] = iters[s] + 1;
+        relres[s] = rel;
+        if (rel <= tol || !(rel == rel) || rel > 1e300) active[s] = 0;
+        else c++;
+    }
+    cnt[threadIdx.x] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int k = 0; k < (int)blockDim.x; k++) t += cnt[k];
+        out[0] = t;
+    }
+}
+
+__global__ void k_hizero(int *__restrict__ a, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
 }
 
 // x += alpha p ; r -= alpha q (p padded, x, r, q interior)
@@ -448,6 +481,15 @@ struct HeatSolver::Impl {
     double *Az = nullptr, *rp = nullptr, *Ac = nullptr, *cpad = nullptr, *partial_cells = nullptr;
     float *xin = nullptr;
     int learned_pre0 = 2, learned_post0 = 2;
+    // captured FCG iteration per mode (plain, learned) and device-side convergence state
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    bool use_graph = true;
+    int *iters_dev = nullptr, *nactive_dev = nullptr;
+    double *relres_dev = nullptr;
+    void drop_graphs() {
+        for (auto &g : graph)
+            if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    }
     unsigned ncell_blocks() const { return (unsigned)((ist(0) + 255) / 256); }
     void ensure_learned();
     void features(double *z);
@@ -479,6 +521,7 @@ struct HeatSolver::Impl {
         return (T *)p_;
     }
     ~Impl() {
+        drop_graphs();
         for (void *a : allocs) (void)hipFree(a);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -608,6 +651,12 @@ void HeatSolver::setup(const Layout &lay, const HeatParams &hp, int nsys, int nt
     HCHECK(hipMalloc(&a, sizeof(int) * nsys));
     I.allocs.push_back(a);
     I.active = (int *)a;
+    HCHECK(hipMalloc(&a, sizeof(int) * (nsys + 1)));
+    I.allocs.push_back(a);
+    I.iters_dev = (int *)a;
+    I.nactive_dev = I.iters_dev + nsys;
+    I.relres_dev = I.dalloc(nsys);
+    I.drop_graphs();
     I.dt.assign(nsys, 1.0);
 }
 
@@ -616,11 +665,16 @@ int HeatSolver::nsys() const { return d->nsys; }
 long long HeatSolver::cells() const { return (long long)d->nb * d->L[0].M3; }
 int HeatSolver::m() const { return d->L[0].m; }
 hipStream_t HeatSolver::stream() const { return d->stream; }
-void HeatSolver::set_corrector(HeatCorrector *c) { d->corr = c; }
+void HeatSolver::set_corrector(HeatCorrector *c) {
+    d->corr = c;
+    d->drop_graphs();
+}
+void HeatSolver::set_graph(bool on) { d->use_graph = on; }
 void HeatSolver::set_learned_sweeps(int pre0, int post0) {
     if (pre0 < 0 || post0 < 0 || pre0 > 8 || post0 > 8) throw std::runtime_error("sweeps must be in [0, 8]");
     d->learned_pre0 = pre0;
     d->learned_post0 = post0;
+    d->drop_graphs();
 }
 
 double HeatSolver::explicit_dt() const { return d->dt_expl; }
@@ -755,6 +809,8 @@ HeatResult HeatSolver::solve(double tol, int max_iter, int mode, HeatRecord *rec
     hipLaunchKernelGGL(k_hcopy, dim3((unsigned)((ni * ns + 255) / 256)), dim3(256), 0, I.stream, (const double *)I.b, I.r,
                        ni * ns);
     I.dot(0, 1, I.b, 0, I.b, 0);
+    hipLaunchKernelGGL(k_hscalars, dim3((ns + 63) / 64), dim3(64), 0, I.stream, 4, ns, (const double *)I.dots, I.sc,
+                       (const int *)I.active, 0.0);
     std::vector<double> bn(3 * ns), rn(3 * ns);
     HCHECK(hipMemcpyAsync(bn.data(), I.dots, sizeof(double) * 3 * ns, hipMemcpyDeviceToHost, I.stream));
     HCHECK(hipStreamSynchronize(I.stream));
@@ -765,20 +821,21 @@ HeatResult HeatSolver::solve(double tol, int max_iter, int mode, HeatRecord *rec
     HCHECK(hipMemcpy(I.active, act.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
 
     auto record = [&](const double *z) {
-        // R (system, block) slots drawn among the systems still iterating
-        std::vector<int> live;
-        for (int s = 0; s < ns; s++)
-            if (!done[s]) live.push_back(s);
-        const long long room = rec->capacity - rec->count;
-        const int R = (int)std::min<long long>(rec->per_snapshot, room);
-        if (R <= 0 || live.empty()) return;
-        std::vector<int> rs(R), rb(R);
-        for (int k = 0; k < R; k++) {
-            rec->rng = rec->rng * 6364136223846793005ULL + 1442695040888963407ULL;
-            const uint64_t u = rec->rng >> 11;
-            rs[k] = live[u % live.size()];
-            rb[k] = (int)((u / live.size()) % (uint64_t)I.nb);
+        // one snapshot (residual/error pair) per system still iterating, each
+        // contributing per_snapshot random blocks, while room is left
+        std::vector<int> rs, rb;
+        for (int s = 0; s < ns; s++) {
+            if (done[s]) continue;
+            if (rec->count + (long long)rs.size() + rec->per_snapshot > rec->capacity) break;
+            for (int k = 0; k < rec->per_snapshot; k++) {
+                rec->rng = rec->rng * 6364136223846793005ULL + 1442695040888963407ULL;
+                rs.push_back(s);
+                rb.push_back((int)((rec->rng >> 11) % (uint64_t)I.nb));
+            }
+            rec->pairs++;
         }
+        const int R = (int)rs.size();
+        if (R == 0) return;
         HCHECK(hipMemcpyAsync(rec->rs + rec->count, rs.data(), sizeof(int) * R, hipMemcpyHostToDevice, I.stream));
         HCHECK(hipMemcpyAsync(rec->rb + rec->count, rb.data(), sizeof(int) * R, hipMemcpyHostToDevice, I.stream));
         hipLaunchKernelGGL(k_hrecord, dim3(R), dim3(256), 0, I.stream, I.L[0], (const float *)I.xin, (const double *)I.x, z,
@@ -812,6 +869,70 @@ HeatResult HeatSolver::solve(double tol, int max_iter, int mode, HeatRecord *rec
     int it = 0;
     bool all0 = true;
     for (int s = 0; s < ns; s++) all0 = all0 && done[s];
+
+    if (mode != kSolveRecord && I.use_graph) {
+        // One FCG iteration captured as a hipGraph and replayed; the convergence
+        // test runs on the device (k_hconverge), the host reads one counter per
+        // iteration.  Arithmetic and iteration counts equal the eager loop's.
+        auto body = [&]() {
+            if (!learned) {
+                I.exchange(0, I.p);
+                hipLaunchKernelGGL((k_happly<0>), g0, dim3(256), 0, I.stream, I.L[0], (const double *)I.p,
+                                   (const double *)nullptr, I.q, (const double *)I.inv_dt, I.pst(0), ni, 0.0);
+            }
+            I.dot(0, 2, I.p, 1, I.q, 0, I.p, 1, I.r, 0);
+            hipLaunchKernelGGL(k_hscalars, dim3((ns + 63) / 64), dim3(64), 0, I.stream, 0, ns, (const double *)I.dots,
+                               I.sc, (const int *)I.active, 0.0);
+            hipLaunchKernelGGL(k_hupdate_xr, g0, dim3(256), 0, I.stream, I.L[0], I.x, I.r, (const double *)I.p,
+                               (const double *)I.q, (const double *)I.sc, I.pst(0), ni);
+            I.dot(0, 1, I.r, 0, I.r, 0);
+            hipLaunchKernelGGL(k_hconverge, dim3(1), dim3(256), 0, I.stream, ns, (const double *)I.dots,
+                               (const double *)I.sc, tol, I.active, I.iters_dev, I.relres_dev, I.nactive_dev);
+            double *zz = precond();
+            if (learned) {
+                hipLaunchKernelGGL(k_hupdate_pq, g0, dim3(256), 0, I.stream, I.L[0], I.p, (const double *)zz, I.q,
+                                   (const double *)I.Az, (const double *)I.sc, I.pst(0), ni, 1);
+            } else {
+                I.dot(0, 1, zz, 1, I.q, 0);
+                hipLaunchKernelGGL(k_hscalars, dim3((ns + 63) / 64), dim3(64), 0, I.stream, 1, ns,
+                                   (const double *)I.dots, I.sc, (const int *)I.active, 0.0);
+                hipLaunchKernelGGL(k_hupdate_p, g0, dim3(256), 0, I.stream, I.L[0], I.p, (const double *)zz,
+                                   (const double *)I.sc, I.pst(0), 1);
+            }
+        };
+        hipLaunchKernelGGL(k_hizero, dim3((ns + 255) / 256), dim3(256), 0, I.stream, I.iters_dev, ns);
+        if (learned) I.corr->prepare(I.stream);
+        hipGraphExec_t &G = I.graph[learned ? 1 : 0];
+        if (!G) {
+            hipGraph_t g;
+            HCHECK(hipStreamBeginCapture(I.stream, hipStreamCaptureModeThreadLocal));
+            body();
+            HCHECK(hipStreamEndCapture(I.stream, &g));
+            HCHECK(hipGraphInstantiate(&G, g, nullptr, nullptr, 0));
+            HCHECK(hipGraphDestroy(g));
+        }
+        int nact = 0;
+        for (int s = 0; s < ns; s++) nact += done[s] ? 0 : 1;
+        int launched = 0;
+        while (nact > 0 && launched < max_iter) {
+            HCHECK(hipGraphLaunch(G, I.stream));
+            launched++;
+            HCHECK(hipMemcpyAsync(&nact, I.nactive_dev, sizeof(int), hipMemcpyDeviceToHost, I.stream));
+            HCHECK(hipStreamSynchronize(I.stream));
+        }
+        std::vector<int> itd(ns);
+        std::vector<double> rel(ns);
+        HCHECK(hipMemcpy(itd.data(), I.iters_dev, sizeof(int) * ns, hipMemcpyDeviceToHost));
+        HCHECK(hipMemcpy(rel.data(), I.relres_dev, sizeof(double) * ns, hipMemcpyDeviceToHost));
+        res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        for (int s = 0; s < ns; s++) {
+            res.iters[s] = done[s] ? 0 : itd[s];
+            res.relres[s] = done[s] ? 0.0 : rel[s];
+        }
+        HCHECK(hipGetLastError());
+        return res;
+    }
+
     while (!all0) {
         if (!learned) {
             I.exchange(0, I.p);

@@ -35,14 +35,17 @@ class HeatCorrector {
     // system * pstride + block * 5832.  A zero scale gives a zero image.
     virtual void infer_heat(const float *x, const double *scale, int scale_stride, int nb, long long pstride,
                             double *cpad, long long nimg, hipStream_t stream) = 0;
+    // bring derived device data (packed weights) up to date before a captured solve
+    virtual void prepare(hipStream_t stream) = 0;
 };
 
 // Training records collected during a solve (device arrays, capacity images):
 // X network input images, W = x + z of the recorded V-cycle, Y targets (error
 // left by the V-cycle over the feature scale), per record system, block, scale.
 struct HeatRecord {
-    long long capacity = 0, count = 0;
-    int per_snapshot = 32;           // records drawn per V-cycle application
+    long long capacity = 0, count = 0;   // block images
+    long long pairs = 0;                 // snapshots: one system at one V-cycle application
+    int per_snapshot = 16;               // blocks recorded per snapshot
     uint64_t rng = 0;
     float *X = nullptr, *Y = nullptr;
     double *W = nullptr, *scale = nullptr;
@@ -77,6 +80,9 @@ class HeatSolver {
     // finest-level smoothing sweeps of the V-cycle used with the learned
     // correction (recording and learned solves); the plain method keeps 2 / 2
     void set_learned_sweeps(int pre0, int post0);
+    // plain and learned solves replay one captured FCG iteration (default) or
+    // launch the same kernels eagerly
+    void set_graph(bool on);
     // residual left by the V-cycle for a given right-hand side (training pairs)
     void vcycle_residual(const double *r_dev, double *z_dev_padded, double *resid_dev);
     double *device_solution();

@@ -132,6 +132,9 @@ int falcon_heat_seed_T(falcon_handle_t h, uint64_t seed, double sigma);
 int falcon_heat_solve(falcon_handle_t h, double tol, int max_iter, int use_corrector, int32_t *iters, double *relres,
                       double *seconds);
 int falcon_heat_get_T(falcon_handle_t h, int sys, double *T);
+/* on (default): each FCG iteration is one replay of a captured hipGraph with the
+ * convergence test on the device; off: the same kernels launched one by one */
+int falcon_heat_set_graph(falcon_handle_t h, int on);
 /* y = A x for all systems (parity tests) */
 int falcon_heat_apply(falcon_handle_t h, const double *x, double *y);
 /* z = M r: one V-cycle of the preconditioner for all systems, zero initial
@@ -156,13 +159,15 @@ int falcon_cnn_train(falcon_handle_t h, int64_t nimg, const float *x, const floa
                      uint64_t seed, double *loss);
 /* Solver-generated training: batch solves of the handle's nsys systems with dt
  * drawn log-uniformly in [dt_lo, dt_hi] x the explicit limit and seeded Gaussian
- * T' (sigma 1e-3) record npairs (normalised residual left by the V-cycle,
- * normalised error it left) block-image pairs, the network trains on them and
- * is attached to the solver (falcon_heat_solve with use_corrector = 1).  Leaves
- * the last training system's dt and right-hand side set. */
-int falcon_heat_train_corrector(falcon_handle_t h, int npairs, double dt_lo, double dt_hi, int max_solves, int epochs,
-                                int batch, double lr, uint64_t seed, double *loss, int64_t *pairs_made,
-                                double *seconds);
+ * T' (sigma 1e-3) record npairs snapshots (one system at one V-cycle
+ * application: normalised residual left by the V-cycle, normalised error it
+ * left), each as blocks_per_pair random block images; the network trains on
+ * them (Adam, learning rate decaying linearly to lr/10) and is attached to the
+ * solver (falcon_heat_solve with use_corrector = 1).  Leaves the last training
+ * system's dt and right-hand side set. */
+int falcon_heat_train_corrector(falcon_handle_t h, int npairs, int blocks_per_pair, double dt_lo, double dt_hi,
+                                int max_solves, int epochs, int batch, double lr, uint64_t seed, double *loss,
+                                int64_t *pairs_made, double *seconds);
 /* Finest-level pre/post smoothing sweeps of the V-cycle that the learned
  * correction follows (training records and learned solves; default 2 / 2, the
  * plain method's).  The network can take the place of finest-level sweeps. */

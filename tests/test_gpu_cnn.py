@@ -79,7 +79,7 @@ def test_training_fits_a_teacher():
     y = pyoracle.CnnOracle().forward(_params(6), x, bf16=False)[0].astype(np.float32)
     power = float(np.mean(y.astype(np.float64) ** 2))
     print("[cnn train] 64 images, label power", power, flush=True)
-    loss = c.train(x, y, epochs=30, batch=16, lr=3e-3, seed=1)
+    loss = c.train(x, y, epochs=60, batch=16, lr=3e-3, seed=1)
     print("final loss", loss, flush=True)
     assert loss < 0.2 * power
 
@@ -148,10 +148,10 @@ def test_trained_correction_beats_plain_mg_fcg():
     lay = configs.cubic((32, 64, 192), 16)
     s = Shell(lay)
     hs = HeatSolver(s, 1.0, 0.7, 4)
-    loss, npairs, sec = hs.train_corrector(npairs=1024, dt_range=(10.0, 300.0), max_solves=20, epochs=8, batch=64,
-                                           lr=2e-3, seed=3)
+    loss, npairs, sec = hs.train_corrector(npairs=256, blocks_per_pair=4, dt_range=(10.0, 300.0), max_solves=20,
+                                           epochs=8, batch=64, lr=2e-3, seed=3)
     print("[train corrector] pairs", npairs, "loss", loss, "seconds", sec, flush=True)
-    assert npairs == 1024 and np.isfinite(loss)
+    assert npairs == 256 and np.isfinite(loss)
     dts = np.array([10.0, 30.0, 100.0, 300.0]) * hs.dt_explicit
     T = 1e-3 * np.random.default_rng(10).standard_normal(hs.shape)
     hs.set_dt(dts)

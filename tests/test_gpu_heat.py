@@ -130,3 +130,27 @@ def test_full_size_shell_converges():
     x = hs.get_T(0)[None]
     b = hs.apply(np.ones_like(T)) * T
     assert np.linalg.norm(b - hs.apply(x)) / np.linalg.norm(b) < 2e-10
+
+
+def test_graph_replays_equal_eager_solves():
+    """Each FCG iteration is a replayed hipGraph: repeated solves on one handle,
+    with dt and T' changed between them, equal the eagerly launched solves bit
+    for bit (iterations, residuals, solutions)."""
+    name = "refined8"
+    H = _oracle(name)
+    s, hs = _gpu(name, 2)
+    rng = np.random.default_rng(16)
+    for rep, fac in enumerate((30.0, 300.0, 100.0)):
+        dt = np.array([fac, 2.0 * fac]) * H.dt_explicit
+        T = 1e-3 * rng.standard_normal(hs.shape)
+        hs.set_dt(dt)
+        hs.set_T(T)
+        hs.set_graph(False)
+        it_e, rel_e, _ = hs.solve(1e-10, 200)
+        x_e = [hs.get_T(k) for k in range(2)]
+        hs.set_graph(True)
+        it_g, rel_g, _ = hs.solve(1e-10, 200)
+        print("replay", rep, it_e.tolist(), it_g.tolist(), flush=True)
+        assert np.array_equal(it_e, it_g) and np.array_equal(rel_e, rel_g)
+        for k in range(2):
+            assert np.array_equal(hs.get_T(k), x_e[k])
