@@ -136,8 +136,9 @@ __global__ void __launch_bounds__(256) k_cnn_pack16(const float *__restrict__ p,
 struct HeatIO {                 // solver form of the forward (HEAT = true)
     const double *scale;       // per system, stride scale_stride
     int scale_stride, nb;
-    long long pstride;
-    double *cpad;
+    long long istride;         // cells per system of c
+    double *c;                 // FP64 output, [system][block][i][j][k]
+    const int *active;         // systems still iterating (others are skipped)
 };
 
 template <bool STORE, bool HEAT>
@@ -150,6 +151,7 @@ __global__ void __launch_bounds__(256) k_cnn_fwd(const float *__restrict__ x, fl
     double sc = 1.0;
     float inv_sc = 1.0f;
     if (HEAT) {
+        if (!hio.active[img / hio.nb]) return;
         sc = hio.scale[(img / hio.nb) * hio.scale_stride];
         inv_sc = sc > 0.0 ? (float)(1.0 / sc) : 0.0f;
     }
@@ -277,8 +279,7 @@ __global__ void __launch_bounds__(256) k_cnn_fwd(const float *__restrict__ x, fl
                 if (g == 0) {
                     if (HEAT) {
                         const long long sys = img / hio.nb, b = img - sys * hio.nb;
-                        hio.cpad[sys * hio.pstride + b * 5832 + ((p + 1) * 18 + j0 + jj + 1) * 18 + n + 1] =
-                            sc * (double)(o + bb3);
+                        hio.c[sys * hio.istride + b * 4096 + (p * 16 + j0 + jj) * 16 + n] = sc * (double)(o + bb3);
                     } else {
                         out[img * 4096 + (p * 16 + j0 + jj) * 16 + n] = o + bb3;
                     }
@@ -641,11 +642,11 @@ void CnnModel::infer(const float *x, float *out, long long nimg, hipStream_t str
                        (uint16_t *)nullptr, (uint16_t *)nullptr, HeatIO{});
 }
 
-void CnnModel::infer_heat(const float *x, const double *scale, int scale_stride, int nb, long long pstride, double *cpad,
-                          long long nimg, hipStream_t stream) {
+void CnnModel::infer_heat(const float *x, const double *scale, int scale_stride, int nb, long long istride, double *c,
+                          long long nimg, const int *active, hipStream_t stream) {
     if (nimg <= 0) return;
     pack(stream);
-    HeatIO hio{scale, scale_stride, nb, pstride, cpad};
+    HeatIO hio{scale, scale_stride, nb, istride, c, active};
     hipLaunchKernelGGL((k_cnn_fwd<false, true>), dim3((unsigned)nimg), dim3(256), 0, stream, x, (float *)nullptr,
                        (const us8 *)packed_, (const uint2 *)((uint16_t *)packed_ + PK_TOTAL * 8), (const float *)params_,
                        (uint16_t *)nullptr, (uint16_t *)nullptr, hio);

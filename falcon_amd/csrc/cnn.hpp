@@ -28,8 +28,8 @@ class CnnModel : public HeatCorrector {
     void get_params(float *p) const;
     // out[img][4096] = net(x[img][4096]) (device pointers, stream-ordered)
     void infer(const float *x, float *out, long long nimg, hipStream_t stream) override;
-    void infer_heat(const float *x, const double *scale, int scale_stride, int nb, long long pstride, double *cpad,
-                    long long nimg, hipStream_t stream) override;
+    void infer_heat(const float *x, const double *scale, int scale_stride, int nb, long long istride, double *c,
+                    long long nimg, const int *active, hipStream_t stream) override;
     void prepare(hipStream_t stream) override { pack(stream); }
     // sum over the batch of the gradient of sum(gout * out) (device x/gout, host grad)
     void gradient(const float *x, const float *gout, int nimg, double *grad_host, hipStream_t stream);

@@ -13,8 +13,18 @@
 // with damping).  Krylov: flexible CG (FCG(1)), everything FP64; dot products
 // are deterministic two-pass reductions (wavefront shuffle, LDS, second pass).
 //
+// Layout: every vector holds interior cells only, [system][block][i][j][k].
+// Ghost values are never stored: a stencil kernel evaluates the face ghost of
+// a boundary cell from the donor table (copy or trilinear) when it needs it,
+// so a Jacobi sweep is one launch and reads each vector once.  Coefficients
+// are [block][i][j] tables (independent of phi).  Fusions: the first sweep
+// from zero is omega r/diag; the residual is restricted in the kernel that
+// forms it; the first post-smoothing sweep reads x + P(coarse) on the fly;
+// dot products are accumulated by the kernels producing their operands.
+//
 // All kernels carry a system index in blockIdx.y: nsys independent systems
-// (same grid, own dt and right-hand side) are solved in the same launches.
+// (same grid, own dt and right-hand side) are solved in the same launches, and
+// a system that has converged (active[s] = 0) exits every kernel at once.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,171 +52,460 @@ namespace falcon {
 // ------------------------------------------------------------------ kernels
 
 struct HLev {
-    int m, nb;
-    long long P;            // padded cells per block
-    long long M3;           // interior cells per block
-    const double *V, *Ai, *Aj, *Ak, *off;
-    const int32_t *gdst, *gsrc;
-    const double *gw;
-    int ng;                 // ghost entries
+    int m, lm, nb;          // cells per edge, log2(m), blocks
+    long long M3;           // cells per block
+    const double *V, *Ai, *Aj, *Ak, *off;   // [block][i][j] coefficient tables
+    const int2 *gsw;        // per face ghost: (donor base cell or -1, weights index or -1 = copy)
+    const double *gw;       // 3 weights per weighted ghost
 };
 
-__device__ __forceinline__ long long pad_index(int m, int i, int j, int k) {
-    return ((long long)(i + 1) * (m + 2) + (j + 1)) * (m + 2) + (k + 1);
-}
-
-// face-ghost fill of one level: trilinear inter/extrapolation inside the donor
-// block, term order of ModMath.f90:283-291 (weights may lie in [-0.5, 1.5])
-__global__ void __launch_bounds__(256) k_hx(HLev L, double *__restrict__ x, long long sys_stride) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= L.ng) return;
-    double *X = x + (long long)blockIdx.y * sys_stride;
-    const int S1 = L.m + 2, SI = S1 * S1;
-    const int src = L.gsrc[e];
-    const double wi = L.gw[3 * e], wj = L.gw[3 * e + 1], wk = L.gw[3 * e + 2];
-    const double *F = X + src;
-    double t = F[0] * (1.0 - wi) * (1.0 - wj) * (1.0 - wk);
-    t = t + F[SI] * (wi) * (1.0 - wj) * (1.0 - wk);
-    t = t + F[S1] * (1.0 - wi) * (wj) * (1.0 - wk);
-    t = t + F[1] * (1.0 - wi) * (1.0 - wj) * (wk);
-    t = t + F[SI + S1] * (wi) * (wj) * (1.0 - wk);
-    t = t + F[S1 + 1] * (1.0 - wi) * (wj) * (wk);
-    t = t + F[SI + 1] * (wi) * (1.0 - wj) * (wk);
-    t = t + F[SI + S1 + 1] * (wi) * (wj) * (wk);
-    X[L.gdst[e]] = t;
-}
-
-// A x on the interior.  MODE 0: y = A x;  MODE 1: y = r - A x;
-// MODE 2: damped Jacobi, y = x + omega (r - A x) / diag (padded output).
-template <int MODE>
-__global__ void __launch_bounds__(256) k_happly(HLev L, const double *__restrict__ x, const double *__restrict__ r,
-                                                double *__restrict__ y, const double *__restrict__ inv_dt,
-                                                long long pstride, long long istride, double omega) {
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= (long long)L.nb * L.M3) return;
-    const int s = blockIdx.y;
-    const int m = L.m;
-    const int b = (int)(c / L.M3);
-    const int q = (int)(c - (long long)b * L.M3);
-    const int i = q / (m * m), j = (q / m) % m, k = q % m;
-    const double *X = x + s * pstride + (long long)b * L.P;
-    const long long S1 = m + 2, SI = S1 * S1;
-    const long long pc = pad_index(m, i, j, k);
-    const double *ai = L.Ai + (long long)b * (m + 1) * m * m;
-    const double *aj = L.Aj + (long long)b * m * (m + 1) * m;
-    const double *ak = L.Ak + (long long)b * m * m * (m + 1);
-    const double cim = ai[((long long)i * m + j) * m + k], cip = ai[((long long)(i + 1) * m + j) * m + k];
-    const double cjm = aj[((long long)i * (m + 1) + j) * m + k], cjp = aj[((long long)i * (m + 1) + j + 1) * m + k];
-    const double ckm = ak[((long long)i * m + j) * (m + 1) + k], ckp = ak[((long long)i * m + j) * (m + 1) + k + 1];
-    const double diag = L.V[(long long)b * L.M3 + q] * inv_dt[s] + L.off[(long long)b * L.M3 + q];
-    const double xc = X[pc];
-    double nb = cim * X[pc - SI] + cip * X[pc + SI];
-    nb = nb + cjm * X[pc - S1] + cjp * X[pc + S1];
-    nb = nb + ckm * X[pc - 1] + ckp * X[pc + 1];
-    const double ax = diag * xc - nb;
-    const long long io = s * istride + c;
-    if (MODE == 0) y[io] = ax;
-    else if (MODE == 1) y[io] = r[io] - ax;
-    else y[s * pstride + (long long)b * L.P + pc] = xc + omega * (r[io] - ax) / diag;
-}
-
-// residual restriction: coarse cell = sum of its 8 children (integrated residual)
-__global__ void __launch_bounds__(256) k_hrestrict(int mf, int nb, const double *__restrict__ rf, double *__restrict__ rc,
-                                                   long long fstride, long long cstride) {
-    const int mc = mf / 2;
-    const long long Mc = (long long)mc * mc * mc, Mf = (long long)mf * mf * mf;
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nb * Mc) return;
-    const int s = blockIdx.y;
-    const int b = (int)(c / Mc);
-    const int q = (int)(c - b * Mc);
-    const int i = q / (mc * mc), j = (q / mc) % mc, k = q % mc;
-    const double *R = rf + s * fstride + b * Mf;
-    double t = 0.0;
-    for (int di = 0; di < 2; di++)
-        for (int dj = 0; dj < 2; dj++)
-            for (int dk = 0; dk < 2; dk++) t += R[((long long)(2 * i + di) * mf + (2 * j + dj)) * mf + (2 * k + dk)];
-    rc[s * cstride + c] = t;
-}
-
-// prolongation by injection: fine z += coarse z of the parent cell (padded arrays)
-__global__ void __launch_bounds__(256) k_hprolong(int mf, int nb, const double *__restrict__ zc, double *__restrict__ zf,
-                                                  long long fstride, long long cstride) {
-    const int mc = mf / 2;
-    const long long Mf = (long long)mf * mf * mf;
-    const long long Pf = (long long)(mf + 2) * (mf + 2) * (mf + 2), Pc = (long long)(mc + 2) * (mc + 2) * (mc + 2);
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nb * Mf) return;
-    const int s = blockIdx.y;
-    const int b = (int)(c / Mf);
-    const int q = (int)(c - b * Mf);
-    const int i = q / (mf * mf), j = (q / mf) % mf, k = q % mf;
-    zf[s * fstride + b * Pf + pad_index(mf, i, j, k)] += zc[s * cstride + b * Pc + pad_index(mc, i / 2, j / 2, k / 2)];
-}
-
-// interior <-> padded copies and fills
-__global__ void __launch_bounds__(256) k_hzero(double *__restrict__ a, long long n) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = 0.0;
-}
-
-// up to 3 dot products per system over the interior: a_p . b_p, with a or b padded
-// (flags) or interior.  partial[s][p][wg]
-struct DotArgs {
-    const double *a[3], *b[3];
-    int apad[3], bpad[3];
-    int npairs;
+struct Cell {
+    int b, i, j, k;
 };
 
-__global__ void __launch_bounds__(256) k_hdot(HLev L, DotArgs D, double *__restrict__ partial, long long pstride,
-                                              long long istride) {
-    __shared__ double red[3][4];
-    const int s = blockIdx.y;
-    const int m = L.m;
-    const long long N = (long long)L.nb * L.M3;
-    double acc[3] = {0.0, 0.0, 0.0};
-    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < N; c += (long long)gridDim.x * blockDim.x) {
-        const int b = (int)(c / L.M3);
-        const int q = (int)(c - (long long)b * L.M3);
-        const long long pp = s * pstride + (long long)b * L.P + pad_index(m, q / (m * m), (q / m) % m, q % m);
-        const long long ii = s * istride + c;
-        for (int p = 0; p < D.npairs; p++) {
-            const double x = D.apad[p] ? D.a[p][pp] : D.a[p][ii];
-            const double y = D.bpad[p] ? D.b[p][pp] : D.b[p][ii];
-            acc[p] += x * y;
-        }
+__device__ __forceinline__ Cell cell_of(const HLev &L, long long c) {
+    const int lm = L.lm;
+    Cell r;
+    r.b = (int)(c >> (3 * lm));
+    const int q = (int)(c & (L.M3 - 1));
+    r.i = q >> (2 * lm);
+    r.j = (q >> lm) & (L.m - 1);
+    r.k = q & (L.m - 1);
+    return r;
+}
+
+// interior index of the coarse parent of fine cell c (edge 2^lm -> 2^(lm-1))
+__device__ __forceinline__ long long parent_of(int lm, long long c) {
+    const int m = 1 << lm, lc = lm - 1;
+    const int b = (int)(c >> (3 * lm));
+    const int q = (int)(c & ((1LL << (3 * lm)) - 1));
+    const int i = q >> (2 * lm), j = (q >> lm) & (m - 1), k = q & (m - 1);
+    return ((long long)b << (3 * lc)) + ((i >> 1) << (2 * lc)) + ((j >> 1) << lc) + (k >> 1);
+}
+
+// the field a stencil reads: x, or x + P xc (injection of the coarse level)
+template <bool PROL>
+struct Field {
+    const double *x, *xc;
+    int lm;
+    __device__ __forceinline__ double operator()(long long c) const {
+        if (PROL) return x[c] + xc[parent_of(lm, c)];
+        return x[c];
     }
-    for (int p = 0; p < D.npairs; p++) {
-        double v = acc[p];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-        if ((threadIdx.x & 63) == 0) red[p][threadIdx.x >> 6] = v;
+};
+
+__device__ __forceinline__ double diag_at(const HLev &L, const Cell &q, double inv_dt) {
+    const long long t2 = ((long long)q.b << (2 * L.lm)) + (q.i << L.lm) + q.j;
+    return L.V[t2] * inv_dt + L.off[t2];
+}
+
+// Block partial sums of NP per-thread values into partial[s][p][blockIdx.x].
+template <int NP>
+__device__ __forceinline__ void block_partials(const double (&v)[NP], double *__restrict__ partial, int s) {
+    __shared__ double red[NP][4];
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+        double a = v[p];
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+        if ((threadIdx.x & 63) == 0) red[p][threadIdx.x >> 6] = a;
     }
     __syncthreads();
-    if (threadIdx.x < D.npairs) {
+    if (threadIdx.x < NP) {
         const int p = threadIdx.x;
         partial[((long long)s * 3 + p) * gridDim.x + blockIdx.x] = ((red[p][0] + red[p][1]) + red[p][2]) + red[p][3];
     }
 }
 
-__global__ void __launch_bounds__(64) k_hdot_final(const double *__restrict__ partial, int nwg, int npairs,
+// Workgroup -> block-group order: the dispatcher deals workgroups round-robin
+// to the 8 XCDs (each with its own L2); remapping gives every XCD a contiguous
+// run of block groups, so face neighbours in phi and theta (adjacent block
+// ids) are gathered from the same L2 while it still holds them.
+__device__ __forceinline__ int xcd_group(int w, int nwg) {
+    const int q = nwg >> 3, r = nwg & 7, x = w & 7;
+    return x * q + (x < r ? x : r) + (w >> 3);
+}
+
+// ---- block-tile stencil kernels
+// A workgroup of 256 threads owns 256/m^2 whole blocks (one for m = 16);
+// thread (block, j, k) owns the column i = 0..m-1 (lanes of one i-plane are
+// consecutive, so every load instruction is coalesced).  Phase 1 issues every
+// load at once: the 6 ghost-table entries per thread, the x column, the op's
+// input columns and the workgroup's coefficient tables.  Phase 2 gathers the
+// face ghosts (1536 per workgroup for any m: the first donor of every entry,
+// then the rest of the weighted ones) and stores the x tile (plane-major:
+// tile[i][thread]), the ghosts and the coefficients in LDS.  Phase 3 applies
+// the operator from registers (i +- 1) and LDS (j, k +- 1, ghosts, coefficients).
+
+// trilinear inter/extrapolation inside the donor block from base cell c, in
+// the term order of ModMath.f90:283-291 (weights may lie in [-0.5, 1.5]); f0 = X(c)
+template <bool PROL>
+__device__ __forceinline__ double trilinear(const HLev &L, const Field<PROL> &X, long long c, int wid, double f0) {
+    const double wi = L.gw[3 * wid], wj = L.gw[3 * wid + 1], wk = L.gw[3 * wid + 2];
+    const long long S1 = L.m, SI = (long long)L.m * L.m;
+    const double f1 = X(c + SI), f2 = X(c + S1), f3 = X(c + 1), f4 = X(c + SI + S1), f5 = X(c + S1 + 1),
+                 f6 = X(c + SI + 1), f7 = X(c + SI + S1 + 1);
+    double t = f0 * (1.0 - wi) * (1.0 - wj) * (1.0 - wk);
+    t = t + f1 * (wi) * (1.0 - wj) * (1.0 - wk);
+    t = t + f2 * (1.0 - wi) * (wj) * (1.0 - wk);
+    t = t + f3 * (1.0 - wi) * (1.0 - wj) * (wk);
+    t = t + f4 * (wi) * (wj) * (1.0 - wk);
+    t = t + f5 * (1.0 - wi) * (wj) * (wk);
+    t = t + f6 * (wi) * (1.0 - wj) * (wk);
+    t = t + f7 * (wi) * (wj) * (wk);
+    return t;
+}
+
+// Per-cell epilogues: (io, x, A x, diag, in0, in1) -> stores and NP dot terms.
+// IN0 / IN1: whether the op reads its input vectors in0 / in1 at the cell.
+template <bool DOT>
+struct OpApply {                 // y = A x; DOT: x.y, x.r
+    double *y;
+    const double *in0, *in1;     // r (DOT); unused
+    static constexpr int NP = 2;
+    static constexpr bool IN0 = DOT, IN1 = false;
+    __device__ __forceinline__ void cell(long long io, double x, double ax, double, double r, double,
+                                         double (&v)[NP]) const {
+        y[io] = ax;
+        if (DOT) { v[0] += x * ax; v[1] += x * r; }
+    }
+};
+template <bool DOT>
+struct OpJacobi {                // y = x + omega (r - A x)/diag; DOT: y.dq
+    double *y;
+    const double *in0, *in1;     // r; dq (DOT)
+    double omega;
+    static constexpr int NP = 1;
+    static constexpr bool IN0 = true, IN1 = DOT;
+    __device__ __forceinline__ void cell(long long io, double x, double ax, double diag, double r, double dq,
+                                         double (&v)[NP]) const {
+        const double yn = x + omega * (r - ax) / diag;
+        y[io] = yn;
+        if (DOT) v[0] += yn * dq;
+    }
+};
+struct OpFeat {                  // Az, r' = r - Az, u = r'/diag (float), u.u
+    double *Az, *rp;
+    float *u;
+    const double *in0, *in1;     // r; unused
+    static constexpr int NP = 1;
+    static constexpr bool IN0 = true, IN1 = false;
+    __device__ __forceinline__ void cell(long long io, double, double ax, double diag, double r, double,
+                                         double (&v)[NP]) const {
+        const double res = r - ax;
+        Az[io] = ax;
+        rp[io] = res;
+        const double w = res / diag;
+        u[io] = (float)w;
+        v[0] += w * w;
+    }
+};
+struct OpApplyDot {              // Ac = A c; c.r', c.Ac
+    double *Ac;
+    const double *in0, *in1;     // r'; unused
+    static constexpr int NP = 2;
+    static constexpr bool IN0 = true, IN1 = false;
+    __device__ __forceinline__ void cell(long long io, double x, double ax, double, double rp, double,
+                                         double (&v)[NP]) const {
+        Ac[io] = ax;
+        v[0] += x * rp;
+        v[1] += x * ax;
+    }
+};
+
+// LDS layout (doubles): edge rows of the x tile (m = 16 only: the first and
+// last j-row of every wavefront, [wave][2][i][k]), face ghosts [6 * 256],
+// coefficient tables Ai [bpw][m+1][m], Aj [bpw][m][m+1], Ak, V, off [bpw][m][m]
+template <int LM>
+struct TileLds {
+    static constexpr int m = 1 << LM, mm = m * m, bpw = 256 >> (2 * LM);
+    static constexpr int NE = LM == 4 ? 4 * 2 * 16 * 16 : 0;
+    static constexpr int G0 = NE, CI = G0 + 6 * 256, NI = bpw * (m + 1) * m, CJ = CI + NI, NJ = bpw * m * (m + 1),
+                         CK = CJ + NJ, CV = CK + bpw * mm, CO = CV + bpw * mm, SIZE = CO + bpw * mm;
+    static constexpr int NC = SIZE - CI;              // coefficient doubles
+    static constexpr int NCT = (NC + 255) / 256;      // per thread
+};
+
+// x of lane l - 1 / l + 1 within rows of 16 lanes (DPP row shifts)
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u & 0xffffffffull), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// phases 1-2: col = the thread's x column, a0 / a1 its columns of in0 / in1,
+// sm = tile + ghosts + coefficients.  Ends with a barrier.
+template <int LM, bool PROL, bool IN0, bool IN1>
+__device__ __forceinline__ void load_tile(const HLev &L, const Field<PROL> &F, int b0, bool own, long long cb,
+                                          const double *__restrict__ in0, const double *__restrict__ in1, long long so,
+                                          double (&col)[1 << LM], double (&a0)[1 << LM], double (&a1)[1 << LM],
+                                          double *__restrict__ sm) {
+    using T = TileLds<LM>;
+    constexpr int m = T::m, mm = T::mm, bpw = T::bpw;
+    const int tid = threadIdx.x;
+    const int nbw = L.nb - b0 < bpw ? L.nb - b0 : bpw;
+    const int ng = nbw * 6 * mm;
+    const int2 *E = L.gsw + (long long)b0 * 6 * mm;
+    int2 e[6];
+#pragma unroll
+    for (int u = 0; u < 6; u++) {
+        const int g = tid + 256 * u;
+        e[u] = g < ng ? E[g] : make_int2(-1, -1);
+    }
+#pragma unroll
+    for (int i = 0; i < m; i++) {
+        const long long c = cb + (long long)i * mm;
+        col[i] = own ? F(c) : 0.0;
+        if (IN0) a0[i] = own ? in0[so + c] : 0.0;
+        if (IN1) a1[i] = own ? in1[so + c] : 0.0;
+    }
+    // coefficient tables of the workgroup's blocks: 5 contiguous ranges
+    double cf[T::NCT];
+#pragma unroll
+    for (int u = 0; u < T::NCT; u++) {
+        const int q = tid + 256 * u;
+        double v = 0.0;
+        if (q < T::NI) {
+            if (q < nbw * (m + 1) * m) v = L.Ai[(long long)b0 * (m + 1) * m + q];
+        } else if (q < T::NI + T::NJ) {
+            const int r = q - T::NI;
+            if (r < nbw * m * (m + 1)) v = L.Aj[(long long)b0 * m * (m + 1) + r];
+        } else if (q < T::NC) {
+            const int r = q - T::NI - T::NJ, which = r / (bpw * mm), w = r - which * (bpw * mm);
+            const double *src = which == 0 ? L.Ak : (which == 1 ? L.V : L.off);
+            if (w < nbw * mm) v = src[((long long)b0 << (2 * LM)) + w];
+        }
+        cf[u] = v;
+    }
+    // ghosts: the first donor value of every entry, then the weighted ones
+    double f0[6];
+#pragma unroll
+    for (int u = 0; u < 6; u++) f0[u] = e[u].x >= 0 ? F(e[u].x) : 0.0;
+#pragma unroll
+    for (int u = 0; u < 6; u++) {
+        double g = f0[u];
+        if (e[u].y >= 0) g = trilinear(L, F, e[u].x, e[u].y, f0[u]);
+        sm[T::G0 + tid + 256 * u] = g;
+    }
+#pragma unroll
+    for (int u = 0; u < T::NCT; u++)
+        if (tid + 256 * u < T::NC) sm[T::CI + tid + 256 * u] = cf[u];
+    if (LM == 4) {
+        // wavefront edge rows: lanes 0-15 (j = 4w) and 48-63 (j = 4w + 3)
+        const int w = tid >> 6, l = tid & 63;
+        if (l < 16 || l >= 48) {
+            const int side = l < 16 ? 0 : 1;
+#pragma unroll
+            for (int i = 0; i < m; i++) sm[((w * 2 + side) * 16 + i) * 16 + (l & 15)] = col[i];
+        }
+    }
+    __syncthreads();
+}
+
+// A x at cell i of the thread's column: i +- 1 from col, k +- 1 from the
+// neighbouring lanes (DPP), j +- 1 from the lanes m apart (within the
+// wavefront; for m = 16 the wavefront's first / last row reads the LDS edge
+// rows of the neighbouring wavefront), face ghosts and coefficients from LDS.
+// Every lane of the workgroup calls it (cross-lane operations).
+template <int LM>
+__device__ __forceinline__ double tile_apply(const double *__restrict__ sm, const double (&col)[1 << LM], int i, int lb,
+                                             int j, int k, double inv_dt, double &diag) {
+    using T = TileLds<LM>;
+    constexpr int m = T::m, mm = T::mm;
+    const int t = threadIdx.x, lane = t & 63;
+    const int g = T::G0 + lb * 6 * mm;
+    const double x0 = col[i];
+    const double dkm = dpp_d<0x111>(x0), dkp = dpp_d<0x101>(x0);
+    double djm = __shfl(x0, (lane - m) & 63), djp = __shfl(x0, (lane + m) & 63);
+    if (LM == 4) {
+        const int w = t >> 6;
+        if (lane < 16 && w > 0) djm = sm[(((w - 1) * 2 + 1) * 16 + i) * 16 + k];
+        if (lane >= 48 && w < 3) djp = sm[(((w + 1) * 2 + 0) * 16 + i) * 16 + k];
+    }
+    const double xim = i > 0 ? col[i > 0 ? i - 1 : 0] : sm[g + j * m + k];
+    const double xip = i < m - 1 ? col[i < m - 1 ? i + 1 : 0] : sm[g + mm + j * m + k];
+    const double xjm = j > 0 ? djm : sm[g + 2 * mm + i * m + k];
+    const double xjp = j < m - 1 ? djp : sm[g + 3 * mm + i * m + k];
+    const double xkm = k > 0 ? dkm : sm[g + 4 * mm + i * m + j];
+    const double xkp = k < m - 1 ? dkp : sm[g + 5 * mm + i * m + j];
+    const double *ai = sm + T::CI + lb * (m + 1) * m;
+    const double *aj = sm + T::CJ + lb * m * (m + 1);
+    const int t2 = lb * mm + i * m + j;
+    const double cim = ai[i * m + j], cip = ai[(i + 1) * m + j];
+    const double cjm = aj[i * (m + 1) + j], cjp = aj[i * (m + 1) + j + 1];
+    const double ck = sm[T::CK + t2];
+    diag = sm[T::CV + t2] * inv_dt + sm[T::CO + t2];
+    double nb = cim * xim + cip * xip;
+    nb = nb + cjm * xjm + cjp * xjp;
+    nb = nb + ck * xkm + ck * xkp;
+    return diag * x0 - nb;
+}
+
+// One operator pass over every cell of a level: op.cell at each cell, dot
+// products per workgroup into partial (when given).  PROL: the field is x + P xc.
+template <int LM, bool PROL, class OP>
+__global__ void __launch_bounds__(256) k_hmarch(HLev L, const double *__restrict__ x, const double *__restrict__ xc,
+                                                const double *__restrict__ inv_dt, const int *__restrict__ active,
+                                                long long istride, long long cstride, OP op, double *__restrict__ partial) {
+    constexpr int m = 1 << LM, mm = m * m, bpw = 256 >> (2 * LM);
+    __shared__ double sm[TileLds<LM>::SIZE];
+    const int s = blockIdx.y;
+    if (!active[s]) return;
+    const int tid = threadIdx.x;
+    const int lb = tid >> (2 * LM), j = (tid >> LM) & (m - 1), k = tid & (m - 1);
+    const int b0 = xcd_group(blockIdx.x, gridDim.x) * bpw, b = b0 + lb;
+    const bool own = b < L.nb;
+    const Field<PROL> F{x + s * istride, PROL ? xc + s * cstride : nullptr, LM};
+    const long long so = s * istride;
+    const long long cb = ((long long)b << (3 * LM)) + (j << LM) + k;
+    double col[m], a0[m], a1[m];
+    load_tile<LM, PROL, OP::IN0, OP::IN1>(L, F, b0, own, cb, op.in0, op.in1, so, col, a0, a1, sm);
+    double v[OP::NP] = {};
+    const double idt = inv_dt[s];
+#pragma unroll
+    for (int i = 0; i < m; i++) {
+        double diag;
+        const double ax = tile_apply<LM>(sm, col, i, lb, j, k, idt, diag);
+        if (own) op.cell(so + cb + (long long)i * mm, col[i], ax, diag, OP::IN0 ? a0[i] : 0.0, OP::IN1 ? a1[i] : 0.0, v);
+    }
+    if (partial) block_partials<OP::NP>(v, partial, s);
+}
+
+// residual restriction fused with the residual: coarse cell = sum over its 8
+// children (di, dj, dk order) of r - A x (integrated residual)
+template <int LM>
+__global__ void __launch_bounds__(256) k_hresrestrict(HLev L, const double *__restrict__ x, const double *__restrict__ r,
+                                                      double *__restrict__ rc, const double *__restrict__ inv_dt,
+                                                      const int *__restrict__ active, long long istride,
+                                                      long long cstride) {
+    constexpr int m = 1 << LM, mm = m * m, bpw = 256 >> (2 * LM), LC = LM - 1, mc = m / 2;
+    __shared__ double sm[TileLds<LM>::SIZE];
+    const int s = blockIdx.y;
+    if (!active[s]) return;
+    const int tid = threadIdx.x;
+    const int lb = tid >> (2 * LM), j = (tid >> LM) & (m - 1), k = tid & (m - 1);
+    const int b0 = xcd_group(blockIdx.x, gridDim.x) * bpw, b = b0 + lb;
+    const bool own = b < L.nb;
+    const Field<false> F{x + s * istride, nullptr, LM};
+    const long long so = s * istride;
+    const long long cb = ((long long)b << (3 * LM)) + (j << LM) + k;
+    double col[m], a0[m], a1[m];
+    load_tile<LM, false, true, false>(L, F, b0, own, cb, r, nullptr, so, col, a0, a1, sm);
+    double res[m];
+    const double idt = inv_dt[s];
+#pragma unroll
+    for (int i = 0; i < m; i++) {
+        double diag;
+        const double ax = tile_apply<LM>(sm, col, i, lb, j, k, idt, diag);
+        res[i] = own ? a0[i] - ax : 0.0;
+    }
+    // children sums through LDS (the ghost area, free once every lane has
+    // applied the operator), two fine planes at a time (plane pair ic in
+    // rs[2][256]); the coarse cells of the pair are (lbc, ic, jc, kc)
+    double *rs = sm + TileLds<LM>::G0;
+    constexpr int ncp = bpw * mc * mc;       // coarse cells per plane pair
+    __syncthreads();
+#pragma unroll
+    for (int ic = 0; ic < mc; ic++) {
+        rs[tid] = res[2 * ic];
+        rs[256 + tid] = res[2 * ic + 1];
+        __syncthreads();
+        if (tid < ncp) {
+            const int lbc = tid / (mc * mc), jc = (tid / mc) % mc, kc = tid % mc;
+            if (b0 + lbc < L.nb) {
+                const double *c0 = rs + lbc * mm + (2 * jc) * m + 2 * kc, *c1 = c0 + 256;
+                double t = 0.0;
+                t += c0[0]; t += c0[1]; t += c0[m]; t += c0[m + 1];
+                t += c1[0]; t += c1[1]; t += c1[m]; t += c1[m + 1];
+                rc[s * cstride + ((long long)(b0 + lbc) << (3 * LC)) + (ic << (2 * LC)) + (jc << LC) + kc] = t;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Jacobi from x = 0: y = (omega r)/diag (the damped Jacobi value at x = 0)
+__global__ void __launch_bounds__(256) k_hjacobi0(HLev L, const double *__restrict__ r, double *__restrict__ y,
+                                                  const double *__restrict__ inv_dt, const int *__restrict__ active,
+                                                  long long istride, double omega) {
+    const int s = blockIdx.y;
+    if (!active[s]) return;
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (long long)L.nb * L.M3) return;
+    const long long io = s * istride + c;
+    y[io] = (omega * r[io]) / diag_at(L, cell_of(L, c), inv_dt[s]);
+}
+
+// prolongation by injection, stored: fine x += coarse x of the parent cell
+__global__ void __launch_bounds__(256) k_hprolong(HLev L, double *__restrict__ zf, const double *__restrict__ zc,
+                                                  const int *__restrict__ active, long long fstride, long long cstride) {
+    const int s = blockIdx.y;
+    if (!active[s]) return;
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (long long)L.nb * L.M3) return;
+    zf[s * fstride + c] += zc[s * cstride + parent_of(L.lm, c)];
+}
+
+__global__ void __launch_bounds__(256) k_hzero(double *__restrict__ a, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0.0;
+}
+
+// ---- elementwise level-0 kernels: a workgroup owns kElem consecutive cells,
+// thread t the cells t, t + 256, ...; dot products per workgroup
+constexpr int kElem = 4096;
+
+// up to 2 dot products per system: partial[s][p][wg]
+__global__ void __launch_bounds__(256) k_hdot(long long n, const double *__restrict__ a0, const double *__restrict__ b0,
+                                              const double *__restrict__ a1, const double *__restrict__ b1, int npairs,
+                                              const int *__restrict__ active, double *__restrict__ partial,
+                                              long long istride) {
+    const int s = blockIdx.y;
+    if (active && !active[s]) return;
+    double v[2] = {0.0, 0.0};
+    const long long c0 = (long long)blockIdx.x * kElem + threadIdx.x;
+#pragma unroll 4
+    for (int u = 0; u < kElem / 256; u++) {
+        const long long c = c0 + 256 * u;
+        if (c < n) {
+            const long long io = s * istride + c;
+            v[0] += a0[io] * b0[io];
+            if (npairs > 1) v[1] += a1[io] * b1[io];
+        }
+    }
+    block_partials<2>(v, partial, s);
+}
+
+// sum of nwg partials per (system, pair), fixed order
+__global__ void __launch_bounds__(256) k_hdot_final(const double *__restrict__ partial, int nwg, int npairs,
                                                    double *__restrict__ out) {
+    __shared__ double red[4];
     const int s = blockIdx.x;
     for (int p = 0; p < npairs; p++) {
-        double v = 0.0;
-        for (int w = threadIdx.x; w < nwg; w += 64) v += partial[((long long)s * 3 + p) * nwg + w];
+        const double *P = partial + ((long long)s * 3 + p) * nwg;
+        double v0 = 0.0, v1 = 0.0;
+        int w = threadIdx.x;
+        for (; w + 256 < nwg; w += 512) { v0 += P[w]; v1 += P[w + 256]; }
+        if (w < nwg) v0 += P[w];
+        double v = v0 + v1;
         for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-        if (threadIdx.x == 0) out[s * 3 + p] = v;
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) out[s * 3 + p] = ((red[0] + red[1]) + red[2]) + red[3];
+        __syncthreads();
     }
 }
 
 // Per-system scalars on the device (one thread per system), kSc slots each:
 // [0] alpha = <p,r>/<p,q>, [1] beta = <z,q>/gamma, [2] gamma = <p,q>,
-// [3] feature scale of the learned correction, [4] its step along c.
+// [3] feature scale of the learned correction, [4] its step along c, [5] <b,b>.
 constexpr int kSc = 8;
 // what 0: alpha and gamma (dots: [0]=<p,q>, [1]=<p,r>)
 // what 1: beta (dots: [0]=<z,q>)
 // what 2: correction step <c,r'>/<c,Ac> (dots: [0]=<c,r'>, [1]=<c,Ac>), 0 unless <c,Ac> > 0
 // what 3: feature scale sqrt(sum / ncell) (dots: [0]=sum (r'/diag)^2)
+// what 4: <b,b> (dots: [0])
 __global__ void k_hscalars(int what, int nsys, const double *__restrict__ dots, double *__restrict__ sc,
                            const int *__restrict__ active, double ncell) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -222,9 +521,9 @@ __global__ void k_hscalars(int what, int nsys, const double *__restrict__ dots, 
         const double a = num / den;
         S[4] = (active[s] && den > 0.0 && a == a && a < 1e300 && a > -1e300) ? a : 0.0;
     } else if (what == 3) {
-        S[3] = sqrt(dots[3 * s] / ncell);
+        S[3] = active[s] ? sqrt(dots[3 * s] / ncell) : 0.0;
     } else {
-        S[5] = dots[3 * s];                 // <b, b>
+        S[5] = dots[3 * s];
     }
 }
 
@@ -259,44 +558,52 @@ __global__ void k_hizero(int *__restrict__ a, int n) {
     if (i < n) a[i] = 0;
 }
 
-// x += alpha p ; r -= alpha q (p padded, x, r, q interior)
-__global__ void __launch_bounds__(256) k_hupdate_xr(HLev L, double *__restrict__ x, double *__restrict__ r,
+// x += alpha p ; r -= alpha q, and partial sums of r.r
+__global__ void __launch_bounds__(256) k_hupdate_xr(long long n, double *__restrict__ x, double *__restrict__ r,
                                                     const double *__restrict__ p, const double *__restrict__ q,
-                                                    const double *__restrict__ sc, long long pstride, long long istride) {
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= (long long)L.nb * L.M3) return;
+                                                    const double *__restrict__ sc, const int *__restrict__ active,
+                                                    double *__restrict__ partial, long long istride) {
     const int s = blockIdx.y;
+    if (!active[s]) return;
     const double a = sc[kSc * s];
-    const int m = L.m;
-    const int b = (int)(c / L.M3);
-    const int qq = (int)(c - (long long)b * L.M3);
-    const double pv = p[s * pstride + (long long)b * L.P + pad_index(m, qq / (m * m), (qq / m) % m, qq % m)];
-    const long long io = s * istride + c;
-    x[io] = x[io] + a * pv;
-    r[io] = r[io] - a * q[io];
+    double v[1] = {0.0};
+    const long long c0 = (long long)blockIdx.x * kElem + threadIdx.x;
+#pragma unroll 4
+    for (int u = 0; u < kElem / 256; u++) {
+        const long long c = c0 + 256 * u;
+        if (c < n) {
+            const long long io = s * istride + c;
+            x[io] = x[io] + a * p[io];
+            const double rn = r[io] - a * q[io];
+            r[io] = rn;
+            v[0] += rn * rn;
+        }
+    }
+    block_partials<1>(v, partial, s);
 }
 
-// p = z - beta p (padded interiors); beta = 0 starts a fresh direction
-__global__ void __launch_bounds__(256) k_hupdate_p(HLev L, double *__restrict__ p, const double *__restrict__ z,
-                                                   const double *__restrict__ sc, long long pstride, int use_beta) {
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= (long long)L.nb * L.M3) return;
+// p = z - beta p; beta = 0 starts a fresh direction
+__global__ void __launch_bounds__(256) k_hupdate_p(long long n, double *__restrict__ p, const double *__restrict__ z,
+                                                   const double *__restrict__ sc, const int *__restrict__ active,
+                                                   long long istride, int use_beta) {
     const int s = blockIdx.y;
+    if (!active[s]) return;
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
     const double be = use_beta ? sc[kSc * s + 1] : 0.0;
-    const int m = L.m;
-    const int b = (int)(c / L.M3);
-    const int qq = (int)(c - (long long)b * L.M3);
-    const long long o = s * pstride + (long long)b * L.P + pad_index(m, qq / (m * m), (qq / m) % m, qq % m);
-    p[o] = z[o] - be * p[o];
+    const long long io = s * istride + c;
+    p[io] = z[io] - be * p[io];
 }
 
-// b = (V/dt) T for the interior (T interior layout)
+// b = (V/dt) T
 __global__ void __launch_bounds__(256) k_hrhs(HLev L, const double *__restrict__ T, double *__restrict__ bvec,
                                               const double *__restrict__ inv_dt, long long istride) {
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= (long long)L.nb * L.M3) return;
     const int s = blockIdx.y;
-    bvec[s * istride + c] = L.V[c] * inv_dt[s] * T[s * istride + c];
+    const Cell q = cell_of(L, c);
+    const long long t2 = ((long long)q.b << (2 * L.lm)) + (q.i << L.lm) + q.j;
+    bvec[s * istride + c] = L.V[t2] * inv_dt[s] * T[s * istride + c];
 }
 
 __global__ void __launch_bounds__(256) k_hcopy(const double *__restrict__ a, double *__restrict__ b, long long n) {
@@ -304,132 +611,60 @@ __global__ void __launch_bounds__(256) k_hcopy(const double *__restrict__ a, dou
     if (i < n) b[i] = a[i];
 }
 
-// ---- learned correction of the V-cycle
-// Block partial sums of up to 3 per-thread values into partial[s][p][blockIdx.x].
-template <int NP>
-__device__ __forceinline__ void block_partials(const double (&v)[NP], double *__restrict__ partial, int s) {
-    __shared__ double red[NP][4];
-#pragma unroll
-    for (int p = 0; p < NP; p++) {
-        double a = v[p];
-        for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
-        if ((threadIdx.x & 63) == 0) red[p][threadIdx.x >> 6] = a;
-    }
-    __syncthreads();
-    if (threadIdx.x < NP) {
-        const int p = threadIdx.x;
-        partial[((long long)s * 3 + p) * gridDim.x + blockIdx.x] = ((red[p][0] + red[p][1]) + red[p][2]) + red[p][3];
-    }
-}
+// ---- learned correction of the V-cycle (features: k_hmarch with OpFeat;
+// A c: k_hmarch with OpApplyDot)
 
-__device__ __forceinline__ double apply_cell(const HLev &L, const double *__restrict__ X, int b, int q, double inv_dt,
-                                             double &diag) {
-    const int m = L.m;
-    const int i = q / (m * m), j = (q / m) % m, k = q % m;
-    const long long S1 = m + 2, SI = S1 * S1;
-    const long long pc = pad_index(m, i, j, k);
-    const double *ai = L.Ai + (long long)b * (m + 1) * m * m;
-    const double *aj = L.Aj + (long long)b * m * (m + 1) * m;
-    const double *ak = L.Ak + (long long)b * m * m * (m + 1);
-    const double cim = ai[((long long)i * m + j) * m + k], cip = ai[((long long)(i + 1) * m + j) * m + k];
-    const double cjm = aj[((long long)i * (m + 1) + j) * m + k], cjp = aj[((long long)i * (m + 1) + j + 1) * m + k];
-    const double ckm = ak[((long long)i * m + j) * (m + 1) + k], ckp = ak[((long long)i * m + j) * (m + 1) + k + 1];
-    diag = L.V[(long long)b * L.M3 + q] * inv_dt + L.off[(long long)b * L.M3 + q];
-    double nb = cim * X[pc - SI] + cip * X[pc + SI];
-    nb = nb + cjm * X[pc - S1] + cjp * X[pc + S1];
-    nb = nb + ckm * X[pc - 1] + ckp * X[pc + 1];
-    return diag * X[pc] - nb;
-}
-
-// Az = A z, r' = r - A z, network input u = r'/diag (float, unscaled) and
-// partial sums of u^2 (z padded with fresh ghosts)
-__global__ void __launch_bounds__(256) k_hfeat(HLev L, const double *__restrict__ z, const double *__restrict__ r,
-                                               double *__restrict__ Az, double *__restrict__ rp, float *__restrict__ u,
-                                               const double *__restrict__ inv_dt, double *__restrict__ partial,
-                                               long long pstride, long long istride) {
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = blockIdx.y;
-    double v[1] = {0.0};
-    if (c < (long long)L.nb * L.M3) {
-        const int b = (int)(c / L.M3);
-        const int q = (int)(c - (long long)b * L.M3);
-        double diag;
-        const double ax = apply_cell(L, z + s * pstride + (long long)b * L.P, b, q, inv_dt[s], diag);
-        const long long io = s * istride + c;
-        const double res = r[io] - ax;
-        Az[io] = ax;
-        rp[io] = res;
-        const double w = res / diag;
-        u[io] = (float)w;
-        v[0] = w * w;
-    }
-    block_partials<1>(v, partial, s);
-}
-
-// Ac = A c and partial sums of c.r' and c.Ac (c padded with fresh ghosts)
-__global__ void __launch_bounds__(256) k_happly_dot(HLev L, const double *__restrict__ cpad, const double *__restrict__ rp,
-                                                    double *__restrict__ Ac, const double *__restrict__ inv_dt,
-                                                    double *__restrict__ partial, long long pstride, long long istride) {
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = blockIdx.y;
-    double v[2] = {0.0, 0.0};
-    if (c < (long long)L.nb * L.M3) {
-        const int b = (int)(c / L.M3);
-        const int q = (int)(c - (long long)b * L.M3);
-        const double *C = cpad + s * pstride + (long long)b * L.P;
-        double diag;
-        const double ac = apply_cell(L, C, b, q, inv_dt[s], diag);
-        const int m = L.m;
-        const double cv = C[pad_index(m, q / (m * m), (q / m) % m, q % m)];
-        const long long io = s * istride + c;
-        Ac[io] = ac;
-        v[0] = cv * rp[io];
-        v[1] = cv * ac;
-    }
-    block_partials<2>(v, partial, s);
-}
-
-// z += a c (padded interiors), Az += a Ac, partial sums of z.q (q = A p of the
-// previous direction), a = correction step of the system
-__global__ void __launch_bounds__(256) k_hzcorr(HLev L, double *__restrict__ z, const double *__restrict__ cpad,
+// z += a c, Az += a Ac, partial sums of z.q (q = A p of the previous direction)
+__global__ void __launch_bounds__(256) k_hzcorr(long long n, double *__restrict__ z, const double *__restrict__ cc,
                                                 double *__restrict__ Az, const double *__restrict__ Ac,
                                                 const double *__restrict__ q, const double *__restrict__ sc,
-                                                double *__restrict__ partial, long long pstride, long long istride) {
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+                                                const int *__restrict__ active, double *__restrict__ partial,
+                                                long long istride) {
     const int s = blockIdx.y;
+    if (!active[s]) return;
+    const double a = sc[kSc * s + 4];
     double v[1] = {0.0};
-    if (c < (long long)L.nb * L.M3) {
-        const double a = sc[kSc * s + 4];
-        const int m = L.m;
-        const int b = (int)(c / L.M3);
-        const int qq = (int)(c - (long long)b * L.M3);
-        const long long o = s * pstride + (long long)b * L.P + pad_index(m, qq / (m * m), (qq / m) % m, qq % m);
-        const double zn = z[o] + a * cpad[o];
-        z[o] = zn;
-        const long long io = s * istride + c;
-        Az[io] = Az[io] + a * Ac[io];
-        v[0] = zn * q[io];
+    const long long c0 = (long long)blockIdx.x * kElem + threadIdx.x;
+#pragma unroll 4
+    for (int u = 0; u < kElem / 256; u++) {
+        const long long c = c0 + 256 * u;
+        if (c < n) {
+            const long long io = s * istride + c;
+            const double zn = z[io] + a * cc[io];
+            z[io] = zn;
+            Az[io] = Az[io] + a * Ac[io];
+            v[0] += zn * q[io];
+        }
     }
     block_partials<1>(v, partial, s);
 }
 
-// p = z - beta p (padded interiors) and q = Az - beta q: the new direction and
-// its image without an operator apply
-__global__ void __launch_bounds__(256) k_hupdate_pq(HLev L, double *__restrict__ p, const double *__restrict__ z,
+// p = z - beta p and q = Az - beta q (the new direction and its image without
+// an operator apply), with partial sums of p.q and p.r for the next step
+__global__ void __launch_bounds__(256) k_hupdate_pq(long long n, double *__restrict__ p, const double *__restrict__ z,
                                                     double *__restrict__ q, const double *__restrict__ Az,
-                                                    const double *__restrict__ sc, long long pstride, long long istride,
-                                                    int use_beta) {
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= (long long)L.nb * L.M3) return;
+                                                    const double *__restrict__ r, const double *__restrict__ sc,
+                                                    const int *__restrict__ active, double *__restrict__ partial,
+                                                    long long istride, int use_beta) {
     const int s = blockIdx.y;
+    if (!active[s]) return;
     const double be = use_beta ? sc[kSc * s + 1] : 0.0;
-    const int m = L.m;
-    const int b = (int)(c / L.M3);
-    const int qq = (int)(c - (long long)b * L.M3);
-    const long long o = s * pstride + (long long)b * L.P + pad_index(m, qq / (m * m), (qq / m) % m, qq % m);
-    p[o] = z[o] - be * p[o];
-    const long long io = s * istride + c;
-    q[io] = Az[io] - be * q[io];
+    double v[2] = {0.0, 0.0};
+    const long long c0 = (long long)blockIdx.x * kElem + threadIdx.x;
+#pragma unroll 4
+    for (int u = 0; u < kElem / 256; u++) {
+        const long long c = c0 + 256 * u;
+        if (c < n) {
+            const long long io = s * istride + c;
+            const double pn = z[io] - be * p[io];
+            const double qn = Az[io] - be * q[io];
+            p[io] = pn;
+            q[io] = qn;
+            v[0] += pn * qn;
+            v[1] += pn * r[io];
+        }
+    }
+    block_partials<2>(v, partial, s);
 }
 
 // training records: copy the network input image and w = x + z of record slot
@@ -438,15 +673,15 @@ __global__ void __launch_bounds__(256) k_hrecord(HLev L, const float *__restrict
                                                  const double *__restrict__ z, const int *__restrict__ rs,
                                                  const int *__restrict__ rb, float *__restrict__ X, double *__restrict__ W,
                                                  const double *__restrict__ sc, double *__restrict__ rscale,
-                                                 long long pstride, long long istride) {
+                                                 long long istride) {
     const long long r = blockIdx.x;
-    const int s = rs[r], b = rb[r], m = L.m;
+    const int s = rs[r], b = rb[r];
     if (threadIdx.x == 0) rscale[r] = sc[kSc * s + 3];
     for (int q = threadIdx.x; q < (int)L.M3; q += blockDim.x) {
         const long long io = s * istride + (long long)b * L.M3 + q;
         const double scale = sc[kSc * s + 3];
         X[r * L.M3 + q] = scale > 0.0 ? (float)((double)xin[io] / scale) : 0.0f;
-        W[r * L.M3 + q] = x[io] + z[s * pstride + (long long)b * L.P + pad_index(m, q / (m * m), (q / m) % m, q % m)];
+        W[r * L.M3 + q] = x[io] + z[io];
     }
 }
 
@@ -471,14 +706,14 @@ struct HeatSolver::Impl {
     std::vector<HeatLevel> host;
     std::vector<HLev> L;
     std::vector<void *> allocs;
-    // per level work arrays (device)
-    std::vector<double *> Z, Z2, R, RES;
+    // per level work arrays (device, interior cells): smoother iterate pair, right-hand side
+    std::vector<double *> Z, Z2, R;
     // level-0 FCG arrays
     double *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *b = nullptr, *tmp = nullptr;
     double *inv_dt = nullptr, *partial = nullptr, *dots = nullptr, *sc = nullptr;
     int *active = nullptr;
     // learned correction work arrays (allocated on first use)
-    double *Az = nullptr, *rp = nullptr, *Ac = nullptr, *cpad = nullptr, *partial_cells = nullptr;
+    double *Az = nullptr, *rp = nullptr, *Ac = nullptr, *cc = nullptr;
     float *xin = nullptr;
     int learned_pre0 = 2, learned_post0 = 2;
     // captured FCG iteration per mode (plain, learned) and device-side convergence state
@@ -490,15 +725,25 @@ struct HeatSolver::Impl {
         for (auto &g : graph)
             if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
     }
-    unsigned ncell_blocks() const { return (unsigned)((ist(0) + 255) / 256); }
+    // workgroups of the level-0 kernels: block-marching (256/m^2 blocks each)
+    // and elementwise (kElem cells each); their dot partials are per workgroup
+    unsigned march_wgs(int l) const {
+        const int bpw = 256 >> (2 * L[l].lm);
+        return (unsigned)((nb + bpw - 1) / bpw);
+    }
+    unsigned elem_wgs() const { return (unsigned)((ist(0) + kElem - 1) / kElem); }
+    dim3 grid_elem() const { return dim3(elem_wgs(), (unsigned)nsys); }
     void ensure_learned();
     void features(double *z);
     void learned_correction(double *z);
     void final_dots(int npairs, unsigned nwg) {
-        hipLaunchKernelGGL(k_hdot_final, dim3((unsigned)nsys), dim3(64), 0, stream, (const double *)partial_cells, (int)nwg,
+        hipLaunchKernelGGL(k_hdot_final, dim3((unsigned)nsys), dim3(256), 0, stream, (const double *)partial, (int)nwg,
                            npairs, dots);
     }
-    int dot_wg = 1024;
+    void scalars(int what, double ncell = 0.0) {
+        hipLaunchKernelGGL(k_hscalars, dim3((nsys + 63) / 64), dim3(64), 0, stream, what, nsys, (const double *)dots, sc,
+                           (const int *)active, ncell);
+    }
     int pre = 2, post = 2, coarse_sweeps = 8;
     double omega = 0.8;
     std::vector<double> dt;
@@ -525,64 +770,109 @@ struct HeatSolver::Impl {
         for (void *a : allocs) (void)hipFree(a);
         if (stream) (void)hipStreamDestroy(stream);
     }
-    long long pst(int l) const { return (long long)nb * L[l].P; }
     long long ist(int l) const { return (long long)nb * L[l].M3; }
-    dim3 grid_cells(int l) const { return dim3((unsigned)((nb * L[l].M3 + 255) / 256), (unsigned)nsys); }
-
-    void exchange(int l, double *x_) {
-        if (L[l].ng == 0) return;
-        hipLaunchKernelGGL(k_hx, dim3((unsigned)((L[l].ng + 255) / 256), (unsigned)nsys), dim3(256), 0, stream, L[l], x_,
-                           pst(l));
-    }
+    dim3 grid_cells(int l) const { return dim3((unsigned)((ist(l) + 255) / 256), (unsigned)nsys); }
     void zero(double *a, long long n) {
         hipLaunchKernelGGL(k_hzero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, a, n);
     }
-    void jacobi(int l, double *&z, double *&z2, const double *rr) {
-        exchange(l, z);
-        hipLaunchKernelGGL((k_happly<2>), grid_cells(l), dim3(256), 0, stream, L[l], z, rr, z2, inv_dt_l(l), pst(l),
-                           ist(l), omega);
-        std::swap(z, z2);
+    void set_all_active() {
+        std::vector<int> one(nsys, 1);
+        HCHECK(hipMemcpy(active, one.data(), sizeof(int) * nsys, hipMemcpyHostToDevice));
     }
-    // coarse levels see the same dt (the operator is rediscretised per level)
-    const double *inv_dt_l(int) const { return inv_dt; }
-
-    // V-cycle: z0 (padded, level 0) = M r0 (interior, level 0)
-    // pre0 / post0: smoothing sweeps on the finest level (others use pre / post)
-    void vcycle(const double *r0, double *&z0, int pre0 = -1, int post0 = -1) {
-        if (pre0 < 0) pre0 = pre;
-        if (post0 < 0) post0 = post;
-        R[0] = const_cast<double *>(r0);
-        Z[0] = z0;
-        for (int l = 0; l < nlev; l++) {
-            zero(Z[l], pst(l) * nsys);
-            const int sweeps = (l == nlev - 1) ? coarse_sweeps : (l == 0 ? pre0 : pre);
-            for (int s = 0; s < sweeps; s++) jacobi(l, Z[l], Z2[l], R[l]);
-            if (l == nlev - 1) break;
-            exchange(l, Z[l]);
-            hipLaunchKernelGGL((k_happly<1>), grid_cells(l), dim3(256), 0, stream, L[l], Z[l], R[l], RES[l], inv_dt,
-                               pst(l), ist(l), 0.0);
-            hipLaunchKernelGGL(k_hrestrict, grid_cells(l + 1), dim3(256), 0, stream, L[l].m, nb, RES[l], R[l + 1], ist(l),
-                               ist(l + 1));
+    template <bool PROL, class OP>
+    void launch_march(int l, const double *x_, const double *xc, long long cs, const OP &op, double *part) {
+        const dim3 g(march_wgs(l), (unsigned)nsys);
+        const HLev &D = L[l];
+        const double *idt = inv_dt;
+        const int *ac = active;
+        const long long is = ist(l);
+        switch (D.lm) {
+            case 0: hipLaunchKernelGGL((k_hmarch<0, PROL, OP>), g, dim3(256), 0, stream, D, x_, xc, idt, ac, is, cs, op, part); break;
+            case 1: hipLaunchKernelGGL((k_hmarch<1, PROL, OP>), g, dim3(256), 0, stream, D, x_, xc, idt, ac, is, cs, op, part); break;
+            case 2: hipLaunchKernelGGL((k_hmarch<2, PROL, OP>), g, dim3(256), 0, stream, D, x_, xc, idt, ac, is, cs, op, part); break;
+            case 3: hipLaunchKernelGGL((k_hmarch<3, PROL, OP>), g, dim3(256), 0, stream, D, x_, xc, idt, ac, is, cs, op, part); break;
+            case 4: hipLaunchKernelGGL((k_hmarch<4, PROL, OP>), g, dim3(256), 0, stream, D, x_, xc, idt, ac, is, cs, op, part); break;
+            default: throw std::runtime_error("heat blocks are at most 16^3");
         }
+    }
+    // R[l+1] = restriction of R[l] - A Z[l]
+    void restrict_residual(int l) {
+        const dim3 g(march_wgs(l), (unsigned)nsys);
+        const HLev &D = L[l];
+        const double *z = Z[l], *rr = R[l], *idt = inv_dt;
+        const int *ac = active;
+        switch (D.lm) {
+            case 1: hipLaunchKernelGGL(k_hresrestrict<1>, g, dim3(256), 0, stream, D, z, rr, R[l + 1], idt, ac, ist(l), ist(l + 1)); break;
+            case 2: hipLaunchKernelGGL(k_hresrestrict<2>, g, dim3(256), 0, stream, D, z, rr, R[l + 1], idt, ac, ist(l), ist(l + 1)); break;
+            case 3: hipLaunchKernelGGL(k_hresrestrict<3>, g, dim3(256), 0, stream, D, z, rr, R[l + 1], idt, ac, ist(l), ist(l + 1)); break;
+            case 4: hipLaunchKernelGGL(k_hresrestrict<4>, g, dim3(256), 0, stream, D, z, rr, R[l + 1], idt, ac, ist(l), ist(l + 1)); break;
+            default: throw std::runtime_error("no restriction below 2^3 cells per block");
+        }
+    }
+    // y = A x (and per-workgroup partial sums of x.y, x.r when r is given)
+    void apply0(const double *x_, double *y, const double *r_dot) {
+        if (r_dot) launch_march<false>(0, x_, nullptr, 0, OpApply<true>{y, r_dot, nullptr}, partial);
+        else launch_march<false>(0, x_, nullptr, 0, OpApply<false>{y, nullptr, nullptr}, nullptr);
+    }
+    // one damped Jacobi sweep Z[l] -> Z2[l] (swapped); prol: the field is
+    // Z[l] + P Z[l+1]; dq: accumulate <new z, dq> (level 0)
+    void sweep(int l, bool prol, const double *dq) {
+        const double *zc = Z[l + 1 < nlev ? l + 1 : l];
+        const long long cs = ist(l + 1 < nlev ? l + 1 : l);
+        if (dq) {
+            const OpJacobi<true> op{Z2[l], R[l], dq, omega};
+            if (prol) launch_march<true>(l, Z[l], zc, cs, op, partial);
+            else launch_march<false>(l, Z[l], nullptr, 0, op, partial);
+        } else {
+            const OpJacobi<false> op{Z2[l], R[l], nullptr, omega};
+            if (prol) launch_march<true>(l, Z[l], zc, cs, op, nullptr);
+            else launch_march<false>(l, Z[l], nullptr, 0, op, nullptr);
+        }
+        std::swap(Z[l], Z2[l]);
+    }
+    // first sweep from zero: Z[l] = omega R[l] / diag
+    void sweep0(int l) {
+        hipLaunchKernelGGL(k_hjacobi0, grid_cells(l), dim3(256), 0, stream, L[l], (const double *)R[l], Z[l],
+                           (const double *)inv_dt, (const int *)active, ist(l), omega);
+    }
+
+    // V-cycle: z0 (level 0) = M r0.  pre0 / post0: smoothing sweeps on the
+    // finest level (others use pre / post).  dq: accumulate <z0, dq> into the
+    // partial sums; returns the number of workgroup partials to sum (0 without dq).
+    unsigned vcycle(const double *r0, double *&z0, int pre0, int post0, const double *dq) {
+        R[0] = const_cast<double *>(r0);
+        for (int l = 0; l < nlev; l++) {
+            const int sweeps = (l == nlev - 1) ? coarse_sweeps : (l == 0 ? pre0 : pre);
+            if (sweeps == 0) {
+                zero(Z[l], ist(l) * nsys);
+            } else {
+                sweep0(l);
+                for (int s = 1; s < sweeps; s++) sweep(l, false, nullptr);
+            }
+            if (l == nlev - 1) break;
+            restrict_residual(l);
+        }
+        unsigned nwg = 0;
         for (int l = nlev - 2; l >= 0; l--) {
-            hipLaunchKernelGGL(k_hprolong, grid_cells(l), dim3(256), 0, stream, L[l].m, nb, Z[l + 1], Z[l], pst(l),
-                               pst(l + 1));
             const int sweeps = l == 0 ? post0 : post;
-            for (int s = 0; s < sweeps; s++) jacobi(l, Z[l], Z2[l], R[l]);
+            if (sweeps == 0) {
+                hipLaunchKernelGGL(k_hprolong, grid_cells(l), dim3(256), 0, stream, L[l], Z[l], (const double *)Z[l + 1],
+                                   (const int *)active, ist(l), ist(l + 1));
+                continue;
+            }
+            for (int s = 0; s < sweeps; s++) {
+                const bool last0 = l == 0 && s == sweeps - 1 && dq;
+                sweep(l, s == 0, last0 ? dq : nullptr);
+                if (last0) nwg = march_wgs(0);
+            }
+        }
+        if (dq && !nwg) {
+            hipLaunchKernelGGL(k_hdot, grid_elem(), dim3(256), 0, stream, ist(0), (const double *)Z[0], dq,
+                               (const double *)nullptr, (const double *)nullptr, 1, (const int *)active, partial, ist(0));
+            nwg = elem_wgs();
         }
         z0 = Z[0];
-    }
-
-    void dot(int l, int npairs, const double *a0, int ap0, const double *b0, int bp0, const double *a1 = nullptr,
-             int ap1 = 0, const double *b1 = nullptr, int bp1 = 0) {
-        DotArgs D{};
-        D.a[0] = a0; D.apad[0] = ap0; D.b[0] = b0; D.bpad[0] = bp0;
-        D.a[1] = a1; D.apad[1] = ap1; D.b[1] = b1; D.bpad[1] = bp1;
-        D.npairs = npairs;
-        hipLaunchKernelGGL(k_hdot, dim3((unsigned)dot_wg, (unsigned)nsys), dim3(256), 0, stream, L[l], D, partial, pst(l),
-                           ist(l));
-        hipLaunchKernelGGL(k_hdot_final, dim3((unsigned)nsys), dim3(64), 0, stream, (const double *)partial, dot_wg, npairs,
-                           dots);
+        return nwg;
     }
 };
 
@@ -623,34 +913,39 @@ void HeatSolver::setup(const Layout &lay, const HeatParams &hp, int nsys, int nt
         const HeatLevel &H = I.host[l];
         HLev &D = I.L[l];
         D.m = H.m; D.nb = I.nb;
-        D.P = (long long)(H.m + 2) * (H.m + 2) * (H.m + 2);
+        D.lm = 0;
+        while ((1 << D.lm) < H.m) D.lm++;
+        if (D.lm > 4) throw std::runtime_error("heat blocks are at most 16^3");
         D.M3 = (long long)H.m * H.m * H.m;
+        if ((long long)I.nb * D.M3 > 0x7fffffffLL) throw std::runtime_error("heat level exceeds 2^31 cells");
         D.V = I.upload(H.V); D.Ai = I.upload(H.Ai); D.Aj = I.upload(H.Aj); D.Ak = I.upload(H.Ak);
         D.off = I.upload(H.offsum);
-        D.gdst = I.upload(H.g_dst); D.gsrc = I.upload(H.g_src); D.gw = I.upload(H.g_w);
-        D.ng = (int)H.g_dst.size();
+        std::vector<int2> gsw(H.g_src.size());
+        for (size_t g = 0; g < gsw.size(); g++) gsw[g] = make_int2(H.g_src[g], H.g_wid[g]);
+        D.gsw = I.upload(gsw);
+        D.gw = I.upload(H.g_w);
     }
-    I.Z.resize(I.nlev); I.Z2.resize(I.nlev); I.R.resize(I.nlev); I.RES.resize(I.nlev);
+    I.Z.resize(I.nlev); I.Z2.resize(I.nlev); I.R.resize(I.nlev);
     for (int l = 0; l < I.nlev; l++) {
-        I.Z[l] = I.dalloc(I.pst(l) * nsys);
-        I.Z2[l] = I.dalloc(I.pst(l) * nsys);
+        I.Z[l] = I.dalloc(I.ist(l) * nsys);
+        I.Z2[l] = I.dalloc(I.ist(l) * nsys);
         I.R[l] = l ? I.dalloc(I.ist(l) * nsys) : nullptr;
-        I.RES[l] = I.dalloc(I.ist(l) * nsys);
     }
     I.x = I.dalloc(I.ist(0) * nsys);
     I.r = I.dalloc(I.ist(0) * nsys);
     I.q = I.dalloc(I.ist(0) * nsys);
     I.b = I.dalloc(I.ist(0) * nsys);
     I.tmp = I.dalloc(I.ist(0) * nsys);
-    I.p = I.dalloc(I.pst(0) * nsys);
+    I.p = I.dalloc(I.ist(0) * nsys);
     I.inv_dt = I.dalloc(nsys);
-    I.partial = I.dalloc((size_t)nsys * 3 * I.dot_wg);
+    I.partial = I.dalloc((size_t)nsys * 3 * std::max(I.march_wgs(0), I.elem_wgs()));
     I.dots = I.dalloc((size_t)nsys * 3);
     I.sc = I.dalloc((size_t)nsys * kSc);
     void *a = nullptr;
     HCHECK(hipMalloc(&a, sizeof(int) * nsys));
     I.allocs.push_back(a);
     I.active = (int *)a;
+    I.set_all_active();
     HCHECK(hipMalloc(&a, sizeof(int) * (nsys + 1)));
     I.allocs.push_back(a);
     I.iters_dev = (int *)a;
@@ -702,19 +997,9 @@ void HeatSolver::apply(const std::vector<double> &x, std::vector<double> &y) {
     Impl &I = *d;
     const long long n = I.ist(0) * I.nsys;
     if ((long long)x.size() != n) throw std::runtime_error("x has the wrong size");
-    std::vector<double> pad(I.pst(0) * I.nsys, 0.0);
-    const int m = I.L[0].m;
-    for (int s = 0; s < I.nsys; s++)
-        for (int bb = 0; bb < I.nb; bb++)
-            for (int c = 0; c < m * m * m; c++) {
-                const int i = c / (m * m), j = (c / m) % m, k = c % m;
-                pad[s * I.pst(0) + (long long)bb * I.L[0].P + ((long long)(i + 1) * (m + 2) + (j + 1)) * (m + 2) + k + 1] =
-                    x[s * I.ist(0) + (long long)bb * m * m * m + c];
-            }
-    HCHECK(hipMemcpy(I.p, pad.data(), sizeof(double) * pad.size(), hipMemcpyHostToDevice));
-    I.exchange(0, I.p);
-    hipLaunchKernelGGL((k_happly<0>), I.grid_cells(0), dim3(256), 0, I.stream, I.L[0], (const double *)I.p,
-                       (const double *)nullptr, I.q, (const double *)I.inv_dt, I.pst(0), I.ist(0), 0.0);
+    I.set_all_active();
+    HCHECK(hipMemcpy(I.p, x.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    I.apply0(I.p, I.q, nullptr);
     y.resize(n);
     HCHECK(hipStreamSynchronize(I.stream));
     HCHECK(hipMemcpy(y.data(), I.q, sizeof(double) * n, hipMemcpyDeviceToHost));
@@ -724,21 +1009,13 @@ void HeatSolver::vcycle_host(const std::vector<double> &r, std::vector<double> &
     Impl &I = *d;
     const long long n = I.ist(0) * I.nsys;
     if ((long long)r.size() != n) throw std::runtime_error("r has the wrong size");
+    I.set_all_active();
     HCHECK(hipMemcpy(I.q, r.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    double *zp = I.Z[0];
-    I.vcycle(I.q, zp);
-    std::vector<double> pad(I.pst(0) * I.nsys);
-    HCHECK(hipStreamSynchronize(I.stream));
-    HCHECK(hipMemcpy(pad.data(), zp, sizeof(double) * pad.size(), hipMemcpyDeviceToHost));
-    const int m = I.L[0].m;
+    double *zp = nullptr;
+    I.vcycle(I.q, zp, I.pre, I.post, nullptr);
     z.resize(n);
-    for (int s = 0; s < I.nsys; s++)
-        for (int bb = 0; bb < I.nb; bb++)
-            for (int c = 0; c < m * m * m; c++) {
-                const int i = c / (m * m), j = (c / m) % m, k = c % m;
-                z[s * I.ist(0) + (long long)bb * m * m * m + c] =
-                    pad[s * I.pst(0) + (long long)bb * I.L[0].P + ((long long)(i + 1) * (m + 2) + (j + 1)) * (m + 2) + k + 1];
-            }
+    HCHECK(hipStreamSynchronize(I.stream));
+    HCHECK(hipMemcpy(z.data(), zp, sizeof(double) * n, hipMemcpyDeviceToHost));
 }
 
 HeatResult HeatSolver::solve(double tol, int max_iter, bool use_corrector) {
@@ -750,8 +1027,7 @@ void HeatSolver::Impl::ensure_learned() {
     Az = dalloc(ist(0) * nsys);
     rp = dalloc(ist(0) * nsys);
     Ac = dalloc(ist(0) * nsys);
-    cpad = dalloc(pst(0) * nsys);
-    partial_cells = dalloc((size_t)nsys * 3 * ncell_blocks());
+    cc = dalloc(ist(0) * nsys);
     void *a = nullptr;
     HCHECK(hipMalloc(&a, sizeof(float) * ist(0) * nsys));
     allocs.push_back(a);
@@ -761,71 +1037,70 @@ void HeatSolver::Impl::ensure_learned() {
 // Az = A z, r' = r - A z, network input r'/diag and its per-system scale
 // (root mean square) in sc[3]
 void HeatSolver::Impl::features(double *z) {
-    exchange(0, z);
-    hipLaunchKernelGGL(k_hfeat, grid_cells(0), dim3(256), 0, stream, L[0], (const double *)z, (const double *)r, Az, rp, xin,
-                       (const double *)inv_dt, partial_cells, pst(0), ist(0));
-    final_dots(1, ncell_blocks());
-    hipLaunchKernelGGL(k_hscalars, dim3((nsys + 63) / 64), dim3(64), 0, stream, 3, nsys, (const double *)dots, sc,
-                       (const int *)active, (double)ist(0));
+    launch_march<false>(0, z, nullptr, 0, OpFeat{Az, rp, xin, r, nullptr}, partial);
+    final_dots(1, march_wgs(0));
+    scalars(3, (double)ist(0));
 }
 
 // c = scale * net(features / scale), a = <c,r'>/<c,Ac>, z += a c, Az += a Ac,
 // and beta = <z, q>/gamma of the FCG direction update (sc[1])
 void HeatSolver::Impl::learned_correction(double *z) {
-    corr->infer_heat(xin, sc + 3, kSc, nb, pst(0), cpad, (long long)nsys * nb, stream);
-    exchange(0, cpad);
-    hipLaunchKernelGGL(k_happly_dot, grid_cells(0), dim3(256), 0, stream, L[0], (const double *)cpad, (const double *)rp,
-                       Ac, (const double *)inv_dt, partial_cells, pst(0), ist(0));
-    final_dots(2, ncell_blocks());
-    hipLaunchKernelGGL(k_hscalars, dim3((nsys + 63) / 64), dim3(64), 0, stream, 2, nsys, (const double *)dots, sc,
-                       (const int *)active, 0.0);
-    hipLaunchKernelGGL(k_hzcorr, grid_cells(0), dim3(256), 0, stream, L[0], z, (const double *)cpad, Az,
-                       (const double *)Ac, (const double *)q, (const double *)sc, partial_cells, pst(0), ist(0));
-    final_dots(1, ncell_blocks());
-    hipLaunchKernelGGL(k_hscalars, dim3((nsys + 63) / 64), dim3(64), 0, stream, 1, nsys, (const double *)dots, sc,
-                       (const int *)active, 0.0);
+    corr->infer_heat(xin, sc + 3, kSc, nb, ist(0), cc, (long long)nsys * nb, active, stream);
+    launch_march<false>(0, cc, nullptr, 0, OpApplyDot{Ac, rp, nullptr}, partial);
+    final_dots(2, march_wgs(0));
+    scalars(2);
+    hipLaunchKernelGGL(k_hzcorr, grid_elem(), dim3(256), 0, stream, ist(0), z, (const double *)cc, Az, (const double *)Ac,
+                       (const double *)q, (const double *)sc, (const int *)active, partial, ist(0));
+    final_dots(1, elem_wgs());
+    scalars(1);
 }
 
 HeatResult HeatSolver::solve(double tol, int max_iter, int mode, HeatRecord *rec) {
     Impl &I = *d;
     const int ns = I.nsys;
     const long long ni = I.ist(0);
-    const bool learned = mode == kSolveLearned;
+    const bool learned = mode == kSolveLearned, recording = mode == kSolveRecord;
     if (learned && !I.corr) throw std::runtime_error("no learned correction is attached to the heat solver");
-    if ((learned || mode == kSolveRecord) && I.L[0].m != 16) throw std::runtime_error("the learned correction needs 16^3 blocks");
-    if (mode == kSolveRecord && !rec) throw std::runtime_error("record mode needs a record buffer");
-    if (learned || mode == kSolveRecord) I.ensure_learned();
+    if ((learned || recording) && I.L[0].m != 16) throw std::runtime_error("the learned correction needs 16^3 blocks");
+    if (recording && !rec) throw std::runtime_error("record mode needs a record buffer");
+    if (learned || recording) I.ensure_learned();
     HeatResult res;
     res.iters.assign(ns, 0);
     res.relres.assign(ns, 0.0);
-    std::vector<int> act(ns, 1);
-    HCHECK(hipMemcpy(I.active, act.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
+    I.set_all_active();
     HCHECK(hipStreamSynchronize(I.stream));
     const auto t0 = std::chrono::steady_clock::now();
     const dim3 g0 = I.grid_cells(0);
     const long long rec0 = rec ? rec->count : 0;
-    // x = 0, r = b
+    // x = 0, r = b, <b, b>
     I.zero(I.x, ni * ns);
     hipLaunchKernelGGL(k_hcopy, dim3((unsigned)((ni * ns + 255) / 256)), dim3(256), 0, I.stream, (const double *)I.b, I.r,
                        ni * ns);
-    I.dot(0, 1, I.b, 0, I.b, 0);
-    hipLaunchKernelGGL(k_hscalars, dim3((ns + 63) / 64), dim3(64), 0, I.stream, 4, ns, (const double *)I.dots, I.sc,
-                       (const int *)I.active, 0.0);
-    std::vector<double> bn(3 * ns), rn(3 * ns);
+    hipLaunchKernelGGL(k_hdot, I.grid_elem(), dim3(256), 0, I.stream, ni, (const double *)I.b, (const double *)I.b,
+                       (const double *)nullptr, (const double *)nullptr, 1, (const int *)nullptr, I.partial, ni);
+    I.final_dots(1, I.elem_wgs());
+    I.scalars(4);
+    std::vector<double> bn(3 * ns);
     HCHECK(hipMemcpyAsync(bn.data(), I.dots, sizeof(double) * 3 * ns, hipMemcpyDeviceToHost, I.stream));
     HCHECK(hipStreamSynchronize(I.stream));
     // a system with b = 0 has the solution 0: it never becomes active
-    std::vector<int> done(ns, 0);
-    for (int s = 0; s < ns; s++)
-        if (bn[3 * s] == 0.0) { done[s] = 1; act[s] = 0; }
+    std::vector<int> act(ns, 1);
+    int nact = 0;
+    for (int s = 0; s < ns; s++) {
+        if (bn[3 * s] == 0.0) act[s] = 0;
+        nact += act[s];
+    }
     HCHECK(hipMemcpy(I.active, act.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_hizero, dim3((ns + 255) / 256), dim3(256), 0, I.stream, I.iters_dev, ns);
 
     auto record = [&](const double *z) {
         // one snapshot (residual/error pair) per system still iterating, each
         // contributing per_snapshot random blocks, while room is left
+        HCHECK(hipMemcpyAsync(act.data(), I.active, sizeof(int) * ns, hipMemcpyDeviceToHost, I.stream));
+        HCHECK(hipStreamSynchronize(I.stream));
         std::vector<int> rs, rb;
         for (int s = 0; s < ns; s++) {
-            if (done[s]) continue;
+            if (!act[s]) continue;
             if (rec->count + (long long)rs.size() + rec->per_snapshot > rec->capacity) break;
             for (int k = 0; k < rec->per_snapshot; k++) {
                 rec->rng = rec->rng * 6364136223846793005ULL + 1442695040888963407ULL;
@@ -840,139 +1115,82 @@ HeatResult HeatSolver::solve(double tol, int max_iter, int mode, HeatRecord *rec
         HCHECK(hipMemcpyAsync(rec->rb + rec->count, rb.data(), sizeof(int) * R, hipMemcpyHostToDevice, I.stream));
         hipLaunchKernelGGL(k_hrecord, dim3(R), dim3(256), 0, I.stream, I.L[0], (const float *)I.xin, (const double *)I.x, z,
                            (const int *)(rec->rs + rec->count), (const int *)(rec->rb + rec->count), rec->X + rec->count * 4096,
-                           rec->W + rec->count * 4096, (const double *)I.sc, rec->scale + rec->count, I.pst(0), ni);
+                           rec->W + rec->count * 4096, (const double *)I.sc, rec->scale + rec->count, ni);
         HCHECK(hipStreamSynchronize(I.stream));   // rs/rb host vectors leave scope
         rec->count += R;
     };
-    auto precond = [&]() -> double * {
-        double *z = I.Z[0];
+    // preconditioner and direction update; first: the initial direction p = z
+    auto precond_direction = [&](bool first) {
+        double *z = nullptr;
         if (learned) {
-            I.vcycle(I.r, z, I.learned_pre0, I.learned_post0);
+            I.vcycle(I.r, z, I.learned_pre0, I.learned_post0, nullptr);
             I.features(z);
             I.learned_correction(z);        // also beta for the direction update
-        } else if (mode == kSolveRecord) {
-            I.vcycle(I.r, z, I.learned_pre0, I.learned_post0);
+            hipLaunchKernelGGL(k_hupdate_pq, I.grid_elem(), dim3(256), 0, I.stream, ni, I.p, (const double *)z, I.q,
+                               (const double *)I.Az, (const double *)I.r, (const double *)I.sc, (const int *)I.active,
+                               I.partial, ni, first ? 0 : 1);
+            return;
+        }
+        const int pre0 = recording ? I.learned_pre0 : I.pre, post0 = recording ? I.learned_post0 : I.post;
+        const unsigned nwg = I.vcycle(I.r, z, pre0, post0, first ? nullptr : I.q);
+        if (!first) {
+            I.final_dots(1, nwg);
+            I.scalars(1);
+        }
+        if (recording) {
             I.features(z);
             record(z);
-        } else {
-            I.vcycle(I.r, z);
         }
-        return z;
+        hipLaunchKernelGGL(k_hupdate_p, g0, dim3(256), 0, I.stream, ni, I.p, (const double *)z, (const double *)I.sc,
+                           (const int *)I.active, ni, first ? 0 : 1);
     };
-    double *z = precond();
-    if (learned)
-        hipLaunchKernelGGL(k_hupdate_pq, g0, dim3(256), 0, I.stream, I.L[0], I.p, (const double *)z, I.q,
-                           (const double *)I.Az, (const double *)I.sc, I.pst(0), ni, 0);
-    else
-        hipLaunchKernelGGL(k_hupdate_p, g0, dim3(256), 0, I.stream, I.L[0], I.p, (const double *)z, (const double *)I.sc,
-                           I.pst(0), 0);
-    int it = 0;
-    bool all0 = true;
-    for (int s = 0; s < ns; s++) all0 = all0 && done[s];
+    // one FCG iteration; the convergence test runs on the device (k_hconverge)
+    auto body = [&]() {
+        if (!learned) I.apply0(I.p, I.q, I.r);           // q = A p with <p,q>, <p,r>
+        I.final_dots(2, learned ? I.elem_wgs() : I.march_wgs(0));   // learned: from the direction update
+        I.scalars(0);
+        hipLaunchKernelGGL(k_hupdate_xr, I.grid_elem(), dim3(256), 0, I.stream, ni, I.x, I.r, (const double *)I.p,
+                           (const double *)I.q, (const double *)I.sc, (const int *)I.active, I.partial, ni);
+        I.final_dots(1, I.elem_wgs());
+        hipLaunchKernelGGL(k_hconverge, dim3(1), dim3(256), 0, I.stream, ns, (const double *)I.dots, (const double *)I.sc,
+                           tol, I.active, I.iters_dev, I.relres_dev, I.nactive_dev);
+        precond_direction(false);
+    };
 
-    if (mode != kSolveRecord && I.use_graph) {
-        // One FCG iteration captured as a hipGraph and replayed; the convergence
-        // test runs on the device (k_hconverge), the host reads one counter per
-        // iteration.  Arithmetic and iteration counts equal the eager loop's.
-        auto body = [&]() {
-            if (!learned) {
-                I.exchange(0, I.p);
-                hipLaunchKernelGGL((k_happly<0>), g0, dim3(256), 0, I.stream, I.L[0], (const double *)I.p,
-                                   (const double *)nullptr, I.q, (const double *)I.inv_dt, I.pst(0), ni, 0.0);
-            }
-            I.dot(0, 2, I.p, 1, I.q, 0, I.p, 1, I.r, 0);
-            hipLaunchKernelGGL(k_hscalars, dim3((ns + 63) / 64), dim3(64), 0, I.stream, 0, ns, (const double *)I.dots,
-                               I.sc, (const int *)I.active, 0.0);
-            hipLaunchKernelGGL(k_hupdate_xr, g0, dim3(256), 0, I.stream, I.L[0], I.x, I.r, (const double *)I.p,
-                               (const double *)I.q, (const double *)I.sc, I.pst(0), ni);
-            I.dot(0, 1, I.r, 0, I.r, 0);
-            hipLaunchKernelGGL(k_hconverge, dim3(1), dim3(256), 0, I.stream, ns, (const double *)I.dots,
-                               (const double *)I.sc, tol, I.active, I.iters_dev, I.relres_dev, I.nactive_dev);
-            double *zz = precond();
-            if (learned) {
-                hipLaunchKernelGGL(k_hupdate_pq, g0, dim3(256), 0, I.stream, I.L[0], I.p, (const double *)zz, I.q,
-                                   (const double *)I.Az, (const double *)I.sc, I.pst(0), ni, 1);
-            } else {
-                I.dot(0, 1, zz, 1, I.q, 0);
-                hipLaunchKernelGGL(k_hscalars, dim3((ns + 63) / 64), dim3(64), 0, I.stream, 1, ns,
-                                   (const double *)I.dots, I.sc, (const int *)I.active, 0.0);
-                hipLaunchKernelGGL(k_hupdate_p, g0, dim3(256), 0, I.stream, I.L[0], I.p, (const double *)zz,
-                                   (const double *)I.sc, I.pst(0), 1);
-            }
-        };
-        hipLaunchKernelGGL(k_hizero, dim3((ns + 255) / 256), dim3(256), 0, I.stream, I.iters_dev, ns);
+    if (nact > 0) {
         if (learned) I.corr->prepare(I.stream);
-        hipGraphExec_t &G = I.graph[learned ? 1 : 0];
-        if (!G) {
-            hipGraph_t g;
-            HCHECK(hipStreamBeginCapture(I.stream, hipStreamCaptureModeThreadLocal));
-            body();
-            HCHECK(hipStreamEndCapture(I.stream, &g));
-            HCHECK(hipGraphInstantiate(&G, g, nullptr, nullptr, 0));
-            HCHECK(hipGraphDestroy(g));
-        }
-        int nact = 0;
-        for (int s = 0; s < ns; s++) nact += done[s] ? 0 : 1;
-        int launched = 0;
-        while (nact > 0 && launched < max_iter) {
-            HCHECK(hipGraphLaunch(G, I.stream));
-            launched++;
-            HCHECK(hipMemcpyAsync(&nact, I.nactive_dev, sizeof(int), hipMemcpyDeviceToHost, I.stream));
-            HCHECK(hipStreamSynchronize(I.stream));
-        }
-        std::vector<int> itd(ns);
-        std::vector<double> rel(ns);
-        HCHECK(hipMemcpy(itd.data(), I.iters_dev, sizeof(int) * ns, hipMemcpyDeviceToHost));
-        HCHECK(hipMemcpy(rel.data(), I.relres_dev, sizeof(double) * ns, hipMemcpyDeviceToHost));
-        res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        for (int s = 0; s < ns; s++) {
-            res.iters[s] = done[s] ? 0 : itd[s];
-            res.relres[s] = done[s] ? 0.0 : rel[s];
-        }
-        HCHECK(hipGetLastError());
-        return res;
+        precond_direction(true);
     }
-
-    while (!all0) {
-        if (!learned) {
-            I.exchange(0, I.p);
-            hipLaunchKernelGGL((k_happly<0>), g0, dim3(256), 0, I.stream, I.L[0], (const double *)I.p,
-                               (const double *)nullptr, I.q, (const double *)I.inv_dt, I.pst(0), ni, 0.0);
-        }
-        I.dot(0, 2, I.p, 1, I.q, 0, I.p, 1, I.r, 0);
-        hipLaunchKernelGGL(k_hscalars, dim3((ns + 63) / 64), dim3(64), 0, I.stream, 0, ns, (const double *)I.dots, I.sc,
-                           (const int *)I.active, 0.0);
-        hipLaunchKernelGGL(k_hupdate_xr, g0, dim3(256), 0, I.stream, I.L[0], I.x, I.r, (const double *)I.p,
-                           (const double *)I.q, (const double *)I.sc, I.pst(0), ni);
-        it++;
-        I.dot(0, 1, I.r, 0, I.r, 0);
-        HCHECK(hipMemcpyAsync(rn.data(), I.dots, sizeof(double) * 3 * ns, hipMemcpyDeviceToHost, I.stream));
+    const bool graph = I.use_graph && !recording;
+    hipGraphExec_t &G = I.graph[learned ? 1 : 0];
+    if (graph && nact > 0 && !G) {
+        // One FCG iteration captured as a hipGraph and replayed: the host reads
+        // one counter per iteration.  Arithmetic equals the eager launches'.
+        hipGraph_t g;
+        HCHECK(hipStreamBeginCapture(I.stream, hipStreamCaptureModeThreadLocal));
+        body();
+        HCHECK(hipStreamEndCapture(I.stream, &g));
+        HCHECK(hipGraphInstantiate(&G, g, nullptr, nullptr, 0));
+        HCHECK(hipGraphDestroy(g));
+    }
+    int launched = 0;
+    while (nact > 0 && launched < max_iter) {
+        if (graph) HCHECK(hipGraphLaunch(G, I.stream));
+        else body();
+        launched++;
+        HCHECK(hipMemcpyAsync(&nact, I.nactive_dev, sizeof(int), hipMemcpyDeviceToHost, I.stream));
         HCHECK(hipStreamSynchronize(I.stream));
-        bool all = true, changed = false;
-        for (int s = 0; s < ns; s++) {
-            if (done[s]) continue;
-            const double rel = std::sqrt(rn[3 * s] / bn[3 * s]);
-            res.relres[s] = rel;
-            res.iters[s] = it;
-            if (rel <= tol || !std::isfinite(rel)) { done[s] = 1; act[s] = 0; changed = true; }
-            else all = false;
-        }
-        if (changed) HCHECK(hipMemcpyAsync(I.active, act.data(), sizeof(int) * ns, hipMemcpyHostToDevice, I.stream));
-        if (all || it >= max_iter) break;
-        z = precond();
-        if (learned) {
-            hipLaunchKernelGGL(k_hupdate_pq, g0, dim3(256), 0, I.stream, I.L[0], I.p, (const double *)z, I.q,
-                               (const double *)I.Az, (const double *)I.sc, I.pst(0), ni, 1);
-        } else {
-            I.dot(0, 1, z, 1, I.q, 0);
-            hipLaunchKernelGGL(k_hscalars, dim3((ns + 63) / 64), dim3(64), 0, I.stream, 1, ns, (const double *)I.dots,
-                               I.sc, (const int *)I.active, 0.0);
-            hipLaunchKernelGGL(k_hupdate_p, g0, dim3(256), 0, I.stream, I.L[0], I.p, (const double *)z,
-                               (const double *)I.sc, I.pst(0), 1);
-        }
     }
-    HCHECK(hipStreamSynchronize(I.stream));
+    std::vector<int> itd(ns);
+    std::vector<double> rel(ns);
+    HCHECK(hipMemcpy(itd.data(), I.iters_dev, sizeof(int) * ns, hipMemcpyDeviceToHost));
+    HCHECK(hipMemcpy(rel.data(), I.relres_dev, sizeof(double) * ns, hipMemcpyDeviceToHost));
     res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (int s = 0; s < ns; s++) {
+        const bool zero_rhs = bn[3 * s] == 0.0;
+        res.iters[s] = zero_rhs ? 0 : itd[s];
+        res.relres[s] = zero_rhs ? 0.0 : rel[s];
+    }
     if (rec && rec->count > rec0) {
         // targets from the converged solution: the error each recorded V-cycle left
         const long long nrec = rec->count - rec0;
@@ -996,26 +1214,6 @@ void HeatSolver::get_rhs(int s, std::vector<double> &b) const {
     const Impl &I = *d;
     b.resize(I.ist(0));
     HCHECK(hipMemcpy(b.data(), I.b + s * I.ist(0), sizeof(double) * I.ist(0), hipMemcpyDeviceToHost));
-}
-
-void HeatSolver::vcycle_residual(const double *r_dev, double *z_dev_padded, double *resid_dev) {
-    Impl &I = *d;
-    double *z = I.Z[0];
-    I.vcycle(r_dev, z);
-    I.exchange(0, z);
-    hipLaunchKernelGGL((k_happly<1>), I.grid_cells(0), dim3(256), 0, I.stream, I.L[0], (const double *)z, r_dev, resid_dev,
-                       (const double *)I.inv_dt, I.pst(0), I.ist(0), 0.0);
-    if (z_dev_padded)
-        hipLaunchKernelGGL(k_hcopy, dim3((unsigned)((I.pst(0) * I.nsys + 255) / 256)), dim3(256), 0, I.stream,
-                           (const double *)z, z_dev_padded, I.pst(0) * I.nsys);
-}
-
-double *HeatSolver::device_solution() { return d->x; }
-double *HeatSolver::device_rhs() { return d->b; }
-const HLevView HeatSolver::level0_view() const {
-    HLevView v;
-    v.m = d->L[0].m; v.nb = d->nb; v.P = d->L[0].P; v.M3 = d->L[0].M3; v.nsys = d->nsys;
-    return v;
 }
 
 }  // namespace falcon

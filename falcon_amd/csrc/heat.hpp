@@ -9,12 +9,6 @@
 
 namespace falcon {
 
-struct HLevView {
-    int m, nb;
-    long long P, M3;
-    int nsys;
-};
-
 struct HeatResult {
     std::vector<int> iters;
     std::vector<double> relres;
@@ -31,10 +25,10 @@ class HeatCorrector {
     virtual void infer(const float *x, float *out, long long nimg, hipStream_t stream) = 0;
     // Solver form: image img = system * nb + block; the network sees
     // x / scale[system * scale_stride] and the result, times the same scale, is
-    // written as FP64 into the padded (18^3 per block) array cpad at
-    // system * pstride + block * 5832.  A zero scale gives a zero image.
-    virtual void infer_heat(const float *x, const double *scale, int scale_stride, int nb, long long pstride,
-                            double *cpad, long long nimg, hipStream_t stream) = 0;
+    // written as FP64 to c[system * istride + block * 4096 + cell].  A zero
+    // scale gives a zero image; systems with active[system] = 0 are skipped.
+    virtual void infer_heat(const float *x, const double *scale, int scale_stride, int nb, long long istride, double *c,
+                            long long nimg, const int *active, hipStream_t stream) = 0;
     // bring derived device data (packed weights) up to date before a captured solve
     virtual void prepare(hipStream_t stream) = 0;
 };
@@ -83,11 +77,6 @@ class HeatSolver {
     // plain and learned solves replay one captured FCG iteration (default) or
     // launch the same kernels eagerly
     void set_graph(bool on);
-    // residual left by the V-cycle for a given right-hand side (training pairs)
-    void vcycle_residual(const double *r_dev, double *z_dev_padded, double *resid_dev);
-    double *device_solution();
-    double *device_rhs();
-    const HLevView level0_view() const;
 
   private:
     struct Impl;

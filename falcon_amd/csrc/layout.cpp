@@ -391,65 +391,62 @@ void build_heat_levels(const Layout &lay, const HeatParams &hp, int nthreads, st
     for (int m = n; m >= 1; m /= 2) {
         HeatLevel L;
         L.m = m;
-        const int M3 = m * m * m, P = (m + 2) * (m + 2) * (m + 2);
-        L.V.assign((size_t)nb * M3, 0.0);
-        L.offsum.assign((size_t)nb * M3, 0.0);
-        L.Ai.assign((size_t)nb * (m + 1) * m * m, 0.0);
-        L.Aj.assign((size_t)nb * m * (m + 1) * m, 0.0);
-        L.Ak.assign((size_t)nb * m * m * (m + 1), 0.0);
-        std::vector<std::vector<int32_t>> gd(nb), gs(nb);
+        const int M2 = m * m, M3 = m * m * m;
+        L.V.assign((size_t)nb * M2, 0.0);
+        L.offsum.assign((size_t)nb * M2, 0.0);
+        L.Ai.assign((size_t)nb * (m + 1) * m, 0.0);
+        L.Aj.assign((size_t)nb * m * (m + 1), 0.0);
+        L.Ak.assign((size_t)nb * M2, 0.0);
+        L.g_src.assign((size_t)nb * 6 * M2, -1);
+        L.g_wid.assign((size_t)nb * 6 * M2, -1);
         std::vector<std::vector<double>> gw(nb);
+        std::vector<std::vector<int32_t>> gwslot(nb);     // ghost ordinal of each weighted entry
         auto kap = [&](double r) { double x = r / hp.r_ref; return hp.kappa0 / (x * x * x); };
         auto work = [&](int b) {
             const Node &nd = t.nodes[t.leaf_node[b]];
             LevelGeom g = level_geom(nd, m);
             const bool top = lay.geom[b].top, bottom = lay.geom[b].bottom;
-            const double *rf = g.xf[0].data(), *tf = g.xf[1].data(), *pf = g.xf[2].data();
+            const double *rf = g.xf[0].data(), *tf = g.xf[1].data();
             const double *rc = g.xc[0].data(), *tc = g.xc[1].data();
-            double *V = &L.V[(size_t)b * M3], *off = &L.offsum[(size_t)b * M3];
-            double *Ai = &L.Ai[(size_t)b * (m + 1) * m * m], *Aj = &L.Aj[(size_t)b * m * (m + 1) * m],
-                   *Ak = &L.Ak[(size_t)b * m * m * (m + 1)];
-            // face coefficients, indexed [i][j][k] with the face index on its own axis
+            const double dph = g.dx[2];
+            double *V = &L.V[(size_t)b * M2], *off = &L.offsum[(size_t)b * M2];
+            double *Ai = &L.Ai[(size_t)b * (m + 1) * m], *Aj = &L.Aj[(size_t)b * m * (m + 1)], *Ak = &L.Ak[(size_t)b * M2];
             for (int i = 0; i <= m; i++)
-                for (int j = 0; j < m; j++)
-                    for (int k = 0; k < m; k++) {
-                        double S = rf[i] * rf[i] * (std::cos(tf[j]) - std::cos(tf[j + 1])) * (pf[k + 1] - pf[k]);
-                        double c = kap(rf[i]) * S / g.dx[0];
-                        if ((i == 0 && bottom) || (i == m && top)) c = 0.0;   // insulated radial boundary
-                        Ai[((size_t)i * m + j) * m + k] = c;
-                    }
+                for (int j = 0; j < m; j++) {
+                    double S = rf[i] * rf[i] * (std::cos(tf[j]) - std::cos(tf[j + 1])) * dph;
+                    double c = kap(rf[i]) * S / g.dx[0];
+                    if ((i == 0 && bottom) || (i == m && top)) c = 0.0;   // insulated radial boundary
+                    Ai[i * m + j] = c;
+                }
             for (int i = 0; i < m; i++)
-                for (int j = 0; j <= m; j++)
-                    for (int k = 0; k < m; k++) {
-                        double S = 0.5 * (rf[i + 1] * rf[i + 1] - rf[i] * rf[i]) * (pf[k + 1] - pf[k]) * std::sin(tf[j]);
-                        double D = rc[i + 1] * g.dx[1];
-                        Aj[((size_t)i * (m + 1) + j) * m + k] = kap(rc[i + 1]) * S / D;
-                    }
+                for (int j = 0; j <= m; j++) {
+                    double S = 0.5 * (rf[i + 1] * rf[i + 1] - rf[i] * rf[i]) * dph * std::sin(tf[j]);
+                    double D = rc[i + 1] * g.dx[1];
+                    Aj[i * (m + 1) + j] = kap(rc[i + 1]) * S / D;
+                }
             for (int i = 0; i < m; i++)
-                for (int j = 0; j < m; j++)
-                    for (int k = 0; k <= m; k++) {
-                        double S = 0.5 * (rf[i + 1] * rf[i + 1] - rf[i] * rf[i]) * (tf[j + 1] - tf[j]);
-                        double D = rc[i + 1] * g.dx[2] * std::sin(tc[j + 1]);
-                        Ak[((size_t)i * m + j) * (m + 1) + k] = kap(rc[i + 1]) * S / D;
-                    }
+                for (int j = 0; j < m; j++) {
+                    double S = 0.5 * (rf[i + 1] * rf[i + 1] - rf[i] * rf[i]) * (tf[j + 1] - tf[j]);
+                    double D = rc[i + 1] * g.dx[2] * std::sin(tc[j + 1]);
+                    Ak[i * m + j] = kap(rc[i + 1]) * S / D;
+                }
             for (int i = 0; i < m; i++)
-                for (int j = 0; j < m; j++)
-                    for (int k = 0; k < m; k++) {
-                        const size_t c = ((size_t)i * m + j) * m + k;
-                        V[c] = 1.0 / 3.0 * (rf[i + 1] * rf[i + 1] * rf[i + 1] - rf[i] * rf[i] * rf[i]) *
-                               (std::cos(tf[j]) - std::cos(tf[j + 1])) * (pf[k + 1] - pf[k]);
-                        off[c] = Ai[((size_t)i * m + j) * m + k] + Ai[((size_t)(i + 1) * m + j) * m + k] +
-                                 Aj[((size_t)i * (m + 1) + j) * m + k] + Aj[((size_t)i * (m + 1) + j + 1) * m + k] +
-                                 Ak[((size_t)i * m + j) * (m + 1) + k] + Ak[((size_t)i * m + j) * (m + 1) + k + 1];
-                    }
-            // face ghosts (6 faces, no edges/corners); radial domain boundary faces have
-            // zero coefficients, so their ghosts are left unfilled
-            for (int I = 0; I <= m + 1; I++)
-                for (int J = 0; J <= m + 1; J++)
-                    for (int K = 0; K <= m + 1; K++) {
-                        const int out = (I == 0 || I == m + 1) + (J == 0 || J == m + 1) + (K == 0 || K == m + 1);
-                        if (out != 1) continue;
-                        double x[3] = {g.xc[0][I], g.xc[1][J], g.xc[2][K]};
+                for (int j = 0; j < m; j++) {
+                    V[i * m + j] = 1.0 / 3.0 * (rf[i + 1] * rf[i + 1] * rf[i + 1] - rf[i] * rf[i] * rf[i]) *
+                                   (std::cos(tf[j]) - std::cos(tf[j + 1])) * dph;
+                    off[i * m + j] = Ai[i * m + j] + Ai[(i + 1) * m + j] + Aj[i * (m + 1) + j] + Aj[i * (m + 1) + j + 1] +
+                                     Ak[i * m + j] + Ak[i * m + j];
+                }
+            // face ghosts in face order i-, i+, j-, j+, k-, k+; radial domain boundary
+            // faces (no donor) keep src = -1 and a zero coefficient
+            for (int f = 0; f < 6; f++)
+                for (int t1 = 0; t1 < m; t1++)
+                    for (int t2 = 0; t2 < m; t2++) {
+                        const int ax = f / 2, side = f % 2;
+                        int P[3];
+                        const int tt[2] = {t1 + 1, t2 + 1};
+                        for (int a = 0, u = 0; a < 3; a++) P[a] = a == ax ? (side ? m + 1 : 0) : tt[u++];
+                        double x[3] = {g.xc[0][P[0]], g.xc[1][P[1]], g.xc[2][P[2]]};
                         int d = t.find(x, g.yin);
                         if (d == -777) continue;
                         d -= 1;
@@ -459,9 +456,11 @@ void build_heat_levels(const Layout &lay, const HeatParams &hp, int nthreads, st
                         // index p lies in [0.5, m+0.5]; the stencil base is kept in [1, m-1] so
                         // only interior donor cells are read and fields linear in (r, theta, phi)
                         // are reproduced exactly across blocks of different spacing.  m = 1
-                        // copies the donor's single cell.
+                        // copies the donor's single cell.  Weights within 1e-12 of 0 or 1 are
+                        // snapped (the rounding residue of coincident cell centres).
                         int base[3];
                         double w[3];
+                        bool copy = true;
                         for (int a = 0; a < 3; a++) {
                             if (m == 1) { base[a] = 1; w[a] = 0.0; continue; }
                             const double dxa = dg.xc[a][2] - dg.xc[a][1];
@@ -471,11 +470,21 @@ void build_heat_levels(const Layout &lay, const HeatParams &hp, int nthreads, st
                             int bs = (int)std::floor(p);
                             bs = bs < 1 ? 1 : (bs > m - 1 ? m - 1 : bs);
                             base[a] = bs;
-                            w[a] = p - (double)bs;
+                            double wa = p - (double)bs;
+                            if (std::fabs(wa) < 1e-12) wa = 0.0;
+                            else if (std::fabs(wa - 1.0) < 1e-12) wa = 1.0;
+                            w[a] = wa;
+                            copy = copy && (wa == 0.0 || wa == 1.0);
                         }
-                        gd[b].push_back((int32_t)((size_t)b * P + ((size_t)I * (m + 2) + J) * (m + 2) + K));
-                        gs[b].push_back((int32_t)((size_t)d * P + ((size_t)base[0] * (m + 2) + base[1]) * (m + 2) + base[2]));
-                        gw[b].push_back(w[0]); gw[b].push_back(w[1]); gw[b].push_back(w[2]);
+                        const size_t gi = ((size_t)b * 6 + f) * M2 + (size_t)t1 * m + t2;
+                        long long src = (long long)d * M3 + ((long long)(base[0] - 1) * m + (base[1] - 1)) * m + (base[2] - 1);
+                        if (copy) {
+                            src += (long long)w[0] * M2 + (long long)w[1] * m + (long long)w[2];
+                        } else {
+                            gwslot[b].push_back((int32_t)gi);
+                            gw[b].push_back(w[0]); gw[b].push_back(w[1]); gw[b].push_back(w[2]);
+                        }
+                        L.g_src[gi] = (int32_t)src;
                     }
         };
         std::vector<std::thread> th;
@@ -483,9 +492,13 @@ void build_heat_levels(const Layout &lay, const HeatParams &hp, int nthreads, st
         for (int w = 0; w < nt; w++) th.emplace_back([&, w]() { for (int b = w; b < nb; b += nt) work(b); });
         for (auto &x : th) x.join();
         for (int b = 0; b < nb; b++) {
-            L.g_dst.insert(L.g_dst.end(), gd[b].begin(), gd[b].end());
-            L.g_src.insert(L.g_src.end(), gs[b].begin(), gs[b].end());
+            for (size_t e = 0; e < gwslot[b].size(); e++) L.g_wid[gwslot[b][e]] = (int32_t)(L.g_w.size() / 3 + e);
             L.g_w.insert(L.g_w.end(), gw[b].begin(), gw[b].end());
+        }
+        for (size_t gi = 0; gi < L.g_src.size(); gi++) {
+            if (L.g_src[gi] < 0) L.n_none++;
+            else if (L.g_wid[gi] < 0) L.n_copy++;
+            else L.n_weighted++;
         }
         levels.push_back(std::move(L));
     }

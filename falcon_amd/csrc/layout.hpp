@@ -77,18 +77,28 @@ struct Layout {
 };
 
 // ---------------------------------------------------------------- heat levels
-// One multigrid level of the heat operator: every block holds m^3 cells with one
-// ghost layer; ghost cells on the 6 faces are filled from donors by trilinear
-// inter/extrapolation inside the donor block (no edges or corners: the operator
-// is 7-point).
+// One multigrid level of the heat operator: every block holds m^3 cells.  The
+// 7-point operator reads one face ghost per face-adjacent cell; ghost values
+// are trilinear inter/extrapolations inside the donor block (weights within
+// 1e-12 of 0 or 1 snapped, so same-level neighbours are plain copies),
+// evaluated by the consumer from the tables below, never stored.
+// Every coefficient depends on (i, j) only: the phi spacing of a block is
+// uniform (dphi = (phi_hi - phi_lo) / m), so tables are [block][i][j].
 struct HeatLevel {
     int m = 0;                           // cells per block edge
-    std::vector<double> V;               // [block][m^3] cell volumes
-    std::vector<double> Ai, Aj, Ak;      // [block][(m+1) m m] etc.: kappa S / D per face (>= 0)
-    std::vector<double> offsum;          // [block][m^3] sum of the cell's face coefficients
-    // face ghosts: receiving padded cell, donor padded base cell, weights
-    std::vector<int32_t> g_dst, g_src;   // block*(m+2)^3 + padded cell
-    std::vector<double> g_w;             // 3 per entry
+    std::vector<double> V;               // [block][m][m] cell volumes
+    std::vector<double> Ai;              // [block][m+1][m] radial faces (0 on insulated boundaries)
+    std::vector<double> Aj;              // [block][m][m+1] theta faces
+    std::vector<double> Ak;              // [block][m][m] phi faces (both faces of a cell)
+    std::vector<double> offsum;          // [block][m][m] sum of the cell's six face coefficients
+    // face ghosts: 6 m^2 per block in face order i-, i+, j-, j+, k-, k+, each
+    // face [t1][t2] over its tangential axes in (i, j, k) order.  g_src: donor
+    // interior base cell (block * m^3 + (i m + j) m + k), -1 without a donor
+    // (insulated radial face); g_wid: -1 for a copy, else the index of the 3
+    // weights in g_w
+    std::vector<int32_t> g_src, g_wid;
+    std::vector<double> g_w;
+    long long n_copy = 0, n_weighted = 0, n_none = 0;
 };
 
 struct HeatParams {

@@ -946,11 +946,14 @@ void orc_vecconv_coeffs(const double x[3], double out[2]) {
  * (3_6/ModHeatConduction.f90:15-16); the method restated here is the one the
  * product defines: backward Euler (V/dt)T + sum_f kappa_f S_f/D_f (T - T_nb)
  * = (V/dt) T^n with face areas/volumes/distances of ModMultigrid_InitGrid
- * (3_5/ModMultigrid.f90:150-233), the finite-volume assembly of ModPFSS_setup
+ * (3_5/ModMultigrid.f90:150-233) on the block's uniform phi spacing
+ * dphi = (phi_hi - phi_lo)/m (so every coefficient depends on r and theta
+ * only), the finite-volume assembly of ModPFSS_setup
  * (3_5/ModPFSS.f90:68-84), insulated radial boundaries, kappa(r) =
  * kappa0 (r/r_ref)^-3; per-block multigrid levels with face ghosts filled by
  * trilinear inter/extrapolation inside the donor found by FindNode (exact for
- * fields linear in r, theta, phi, whatever the two blocks' spacings); V-cycle
+ * fields linear in r, theta, phi, whatever the two blocks' spacings; weights
+ * within 1e-12 of 0 or 1 snapped to 0 or 1); V-cycle
  * with damped Jacobi (omega 0.8, 2+2 sweeps, 8 on the 1-cell level), summed
  * restriction, injection prolongation; flexible CG (FCG(1)).  Serial FP64.
  * Arrays: Fortran order (i fastest), padded (m+2)^3 with one ghost layer.
@@ -1016,14 +1019,14 @@ orc_heat *orc_heat_create(orc_ctx *c, double kappa0, double r_ref) {
             /* face index along its own axis first: Ai[(fi*m + j)*m + k], Aj[(i*(m+1)+fj)*m + k], Ak[(i*m+j)*(m+1)+fk] */
             for (int i = 0; i <= m; i++) for (int j = 0; j < m; j++) for (int k = 0; k < m; k++) {
                 double x = xf[0][i] / r_ref, kap = kappa0 / (x * x * x);
-                double S = xf[0][i] * xf[0][i] * (cos(xf[1][j]) - cos(xf[1][j + 1])) * (xf[2][k + 1] - xf[2][k]);
+                double S = xf[0][i] * xf[0][i] * (cos(xf[1][j]) - cos(xf[1][j + 1])) * dx[2];
                 double cf = kap * S / dx[0];
                 if ((i == 0 && b->bottom) || (i == m && b->top)) cf = 0.0;
                 Ai[((long)i * m + j) * m + k] = cf;
             }
             for (int i = 0; i < m; i++) for (int j = 0; j <= m; j++) for (int k = 0; k < m; k++) {
                 double x = xc[0][i + 1] / r_ref, kap = kappa0 / (x * x * x);
-                double S = 0.5 * (xf[0][i + 1] * xf[0][i + 1] - xf[0][i] * xf[0][i]) * (xf[2][k + 1] - xf[2][k]) * sin(xf[1][j]);
+                double S = 0.5 * (xf[0][i + 1] * xf[0][i + 1] - xf[0][i] * xf[0][i]) * dx[2] * sin(xf[1][j]);
                 Aj[((long)i * (m + 1) + j) * m + k] = kap * S / (xc[0][i + 1] * dx[1]);
             }
             for (int i = 0; i < m; i++) for (int j = 0; j < m; j++) for (int k = 0; k <= m; k++) {
@@ -1034,7 +1037,7 @@ orc_heat *orc_heat_create(orc_ctx *c, double kappa0, double r_ref) {
             for (int i = 0; i < m; i++) for (int j = 0; j < m; j++) for (int k = 0; k < m; k++) {
                 long q = (long)i + (long)m * ((long)j + (long)m * k);
                 V[q] = 1.0 / 3.0 * (xf[0][i + 1] * xf[0][i + 1] * xf[0][i + 1] - xf[0][i] * xf[0][i] * xf[0][i]) *
-                       (cos(xf[1][j]) - cos(xf[1][j + 1])) * (xf[2][k + 1] - xf[2][k]);
+                       (cos(xf[1][j]) - cos(xf[1][j + 1])) * dx[2];
                 off[q] = Ai[((long)i * m + j) * m + k] + Ai[((long)(i + 1) * m + j) * m + k] +
                          Aj[((long)i * (m + 1) + j) * m + k] + Aj[((long)i * (m + 1) + j + 1) * m + k] +
                          Ak[((long)i * m + j) * (m + 1) + k] + Ak[((long)i * m + j) * (m + 1) + k + 1];
@@ -1070,6 +1073,8 @@ orc_heat *orc_heat_create(orc_ctx *c, double kappa0, double r_ref) {
                     bs = bs < 1 ? 1 : (bs > m - 1 ? m - 1 : bs);
                     base[a] = bs;
                     w[a] = p - (double)bs;
+                    if (fabs(w[a]) < 1e-12) w[a] = 0.0;
+                    else if (fabs(w[a] - 1.0) < 1e-12) w[a] = 1.0;
                 }
                 L->gdst[L->ng] = (long)ib * P + HP(m, I, J, K);
                 L->gsrc[L->ng] = (long)d * P + HP(m, base[0], base[1], base[2]);
