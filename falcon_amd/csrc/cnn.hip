@@ -49,13 +49,18 @@ constexpr int OFF_W1 = 0, OFF_B1 = 432, OFF_W2 = 448, OFF_B2 = 7360, OFF_W3 = 73
 // P1 layer-1 forward (1 step), Q3 layer-3 data gradient (1), Q2 layer-2 data gradient (14)
 constexpr int PK_P1 = 0, PK_Q3 = 64, PK_Q2 = 128, PK_TOTAL = PK_Q2 + 14 * 64;
 
-// forward operand images of the 16x16x16 form (4 bf16 per lane): W2 per tap
-// [27][64], W3 per (di,dj) pair [9][64] with rows n = dk+1 (n < 3)
-constexpr int PF_W2 = 0, PF_W3 = 27 * 64, PF_TOTAL = 36 * 64;
-// fused forward LDS carve (uint16 elements): h1 and h2 rings of 3 planes
-// [16 j][16 k][16 c], x ring of 3 planes [18][18] with halo plus a zero plane
-constexpr int PLANE = 16 * 16 * 16;
-constexpr int F_H1 = 0, F_H2 = 3 * PLANE, F_X = 6 * PLANE, F_XZ = F_X + 3 * PL, F_TOTAL = F_XZ + PL;
+// forward operand images (bf16 elements, lane-major):
+//   L1  [3 di][64] x 4: 16x16x16 A, rows co, K = the 9 (dj,dk) taps of plane offset di
+//   L2A [9 (di,dj)][64] x 8: 16x16x32 A, rows co, K = (dk = -1: ci 0..15, dk = +1: ci 0..15)
+//   L2B [9 (di,dj)][64] x 4: 16x16x16 A, rows co, K = ci of tap dk = 0
+//   L3  [3 di][64] x 4: 16x16x16 A, rows r = 4 (dj+1) + (dk+1) (r % 4 < 3, r < 12), K = ci
+constexpr int PF_L1 = 0, PF_L2A = PF_L1 + 3 * 64 * 4, PF_L2B = PF_L2A + 9 * 64 * 8, PF_L3 = PF_L2B + 9 * 64 * 4,
+              PF_TOTAL = PF_L3 + 3 * 64 * 4;
+// fused forward LDS carve (uint16 elements): h1 and h2 rings of 3 planes, each
+// [18 j][2 channel halves][18 k][8 c] with zero halo rows and voxels; x ring
+// of 3 planes [18][18] with a zero halo
+constexpr int HROW = 2 * 18 * 8, HPL = 18 * HROW;
+constexpr int F_H1 = 0, F_H2 = 3 * HPL, F_X = 6 * HPL, F_TOTAL = F_X + 3 * PL;
 // single-layer LDS carve: ring of 3 input planes + zero plane
 constexpr int S_IN = 0, S_Z = 3 * PL * 16, S_TOTAL = 4 * PL * 16;
 
@@ -109,17 +114,28 @@ __global__ void __launch_bounds__(256) k_cnn_pack(const float *__restrict__ p, u
     pk[q] = f2bf(w);
 }
 
-__global__ void __launch_bounds__(256) k_cnn_pack16(const float *__restrict__ p, uint16_t *__restrict__ pk) {
+__global__ void __launch_bounds__(256) k_cnn_packf(const float *__restrict__ p, uint16_t *__restrict__ pk) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= PF_TOTAL * 4) return;
-    const int e = q & 3, entry = q >> 2, lane = entry & 63, n = lane & 15, g = lane >> 4;
+    if (q >= PF_TOTAL) return;
     float w = 0.0f;
-    if (entry < PF_W3) {                       // W2[t][ci = 4g+e][co = n]
-        const int t = entry >> 6;
-        w = p[OFF_W2 + (t * 16 + 4 * g + e) * 16 + n];
-    } else {                                   // W3[pair*3 + n][ci = 4g+e], rows n < 3
-        const int pr = (entry - PF_W3) >> 6;
-        if (n < 3) w = p[OFF_W3 + (pr * 3 + n) * 16 + 4 * g + e];
+    if (q < PF_L2A) {                                      // L1: W1[t = 9 di + s][co]
+        const int e = q & 3, entry = q >> 2, di = entry >> 6, lane = entry & 63, co = lane & 15, gg = lane >> 4;
+        const int s9 = 4 * gg + e;
+        if (s9 < 9) w = p[OFF_W1 + (9 * di + s9) * 16 + co];
+    } else if (q < PF_L2B) {                               // L2A: W2[t][ci][co], dk = -1 / +1
+        const int r = q - PF_L2A, e = r & 7, entry = r >> 3, pr = entry >> 6, lane = entry & 63, co = lane & 15,
+                  gg = lane >> 4;
+        const int dk = gg < 2 ? 0 : 2, ci = 8 * (gg & 1) + e, t = 3 * pr + dk;
+        w = p[OFF_W2 + (t * 16 + ci) * 16 + co];
+    } else if (q < PF_L3) {                                // L2B: dk = 0
+        const int r = q - PF_L2B, e = r & 3, entry = r >> 2, pr = entry >> 6, lane = entry & 63, co = lane & 15,
+                  gg = lane >> 4;
+        w = p[OFF_W2 + ((3 * pr + 1) * 16 + 4 * gg + e) * 16 + co];
+    } else {                                               // L3: W3[t][ci]
+        const int r = q - PF_L3, e = r & 3, entry = r >> 2, di = entry >> 6, lane = entry & 63, row = lane & 15,
+                  gg = lane >> 4;
+        const int dj = row >> 2, dk = row & 3;
+        if (dj < 3 && dk < 3) w = p[OFF_W3 + (9 * di + 3 * dj + dk) * 16 + 4 * gg + e];
     }
     pk[q] = f2bf(w);
 }
@@ -128,11 +144,16 @@ __global__ void __launch_bounds__(256) k_cnn_pack16(const float *__restrict__ p,
 // One workgroup per image; march over the 16 i-planes keeping rings of 3 planes
 // of x, h1 and h2 in LDS: step n loads x plane n+1, computes h1 plane n, h2
 // plane n-1 and the output plane n-2.  Wave w owns the k-lines j = 4w .. 4w+3
-// of every plane.  Layers 2 and 3 run on 16x16x16 MFMAs with K = the 16
-// channels of one tap: each input line is read from LDS once per (plane
-// offset) and serves the 3 output lines it touches; its dk = -1 / +1 images are
-// DPP row shifts of the loaded operand (layer 2) or of the accumulated tap
-// rows (layer 3, whose A rows are the 3 dk taps).  STORE also writes h1/h2.
+// of every plane; an MFMA column is one voxel of a k-line.
+//   layer 1: per plane offset di one 16x16x16 MFMA, K = the 9 (dj, dk) taps;
+//   layer 2: per input line (plane, row) one 16x16x32 MFMA over the dk = -1 and
+//     dk = +1 taps and one 16x16x16 over dk = 0, each operand read once from LDS
+//     (the halo in k makes the shifted reads plain loads) and used for the up
+//     to 3 output lines the row touches;
+//   layer 3: per input line one 16x16x16 MFMA whose rows are the 9 (dj, dk)
+//     taps (accumulated over di), combined by DPP shifts (dk) and lane shuffles
+//     (dj) into the output line.
+// STORE also writes h1/h2 (channel-fastest) for the backward pass.
 struct HeatIO {                 // solver form of the forward (HEAT = true)
     const double *scale;       // per system, stride scale_stride
     int scale_stride, nb;
@@ -141,11 +162,23 @@ struct HeatIO {                 // solver form of the forward (HEAT = true)
     const int *active;         // systems still iterating (others are skipped)
 };
 
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+// 4 floats -> 4 bfloat16 (round to nearest even, v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint2 cvt4(float a, float b, float c, float d) {
+    bf16x4 v;
+    v[0] = (__bf16)a; v[1] = (__bf16)b; v[2] = (__bf16)c; v[3] = (__bf16)d;
+    return __builtin_bit_cast(uint2, v);
+}
+__device__ __forceinline__ uint16_t cvt1(float a) { return __builtin_bit_cast(uint16_t, (__bf16)a); }
+__device__ __forceinline__ f4 mfma32(uint4 a, uint4 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0,
+                                                   0);
+}
+
 template <bool STORE, bool HEAT>
 __global__ void __launch_bounds__(256) k_cnn_fwd(const float *__restrict__ x, float *__restrict__ out,
-                                                 const us8 *__restrict__ pk, const uint2 *__restrict__ pf,
-                                                 const float *__restrict__ prm, uint16_t *__restrict__ h1g,
-                                                 uint16_t *__restrict__ h2g, HeatIO hio) {
+                                                 const uint16_t *__restrict__ pf, const float *__restrict__ prm,
+                                                 uint16_t *__restrict__ h1g, uint16_t *__restrict__ h2g, HeatIO hio) {
     __shared__ __align__(16) uint16_t sm[F_TOTAL];
     const long long img = blockIdx.x;
     double sc = 1.0;
@@ -155,16 +188,25 @@ __global__ void __launch_bounds__(256) k_cnn_fwd(const float *__restrict__ x, fl
         sc = hio.scale[(img / hio.nb) * hio.scale_stride];
         inv_sc = sc > 0.0 ? (float)(1.0 / sc) : 0.0f;
     }
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, n = lane & 15;
+    const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, n = lane & 15;
+    // wavefront index as a scalar: the row-range tests below are uniform
+    // branches, so the MFMA accumulators stay in place across them
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int j0 = 4 * wave;
     for (int q = tid; q < F_TOTAL / 2; q += 256) reinterpret_cast<uint32_t *>(sm)[q] = 0u;
 
-    const us8 w1 = pk[PK_P1 + lane];
-    uint2 w2[27], w3[9];
+    uint2 a1[3], a2b[9], a3[3];
+    uint4 a2a[9];
 #pragma unroll
-    for (int t = 0; t < 27; t++) w2[t] = pf[PF_W2 + t * 64 + lane];
+    for (int d = 0; d < 3; d++) {
+        a1[d] = reinterpret_cast<const uint2 *>(pf + PF_L1)[d * 64 + lane];
+        a3[d] = reinterpret_cast<const uint2 *>(pf + PF_L3)[d * 64 + lane];
+    }
 #pragma unroll
-    for (int pr = 0; pr < 9; pr++) w3[pr] = pf[PF_W3 + pr * 64 + lane];
+    for (int pr = 0; pr < 9; pr++) {
+        a2a[pr] = reinterpret_cast<const uint4 *>(pf + PF_L2A)[pr * 64 + lane];
+        a2b[pr] = reinterpret_cast<const uint2 *>(pf + PF_L2B)[pr * 64 + lane];
+    }
     float bb1[4], bb2[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
@@ -172,15 +214,28 @@ __global__ void __launch_bounds__(256) k_cnn_fwd(const float *__restrict__ x, fl
         bb2[r] = prm[OFF_B2 + 4 * g + r];
     }
     const float bb3 = prm[OFF_B3];
+    // layer-1 gather: K slot s = 4g + e is tap (dj, dk) = (s / 3, s % 3) - 1 at
+    // offset (s/3) * 18 + s % 3 + n from the output row in a halo'd x plane
+    int xo[4];
+    bool xv[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const int s9 = 4 * g + e;
+        xv[e] = s9 < 9;
+        xo[e] = xv[e] ? (s9 / 3) * 18 + s9 % 3 + n : n;
+    }
+    // per-lane LDS offsets in an h plane row: the k-shifted 8-channel chunk
+    // (dk = -1 for g < 2, +1 for g >= 2) and the 4-channel chunk at k
+    const int oA = (g & 1) * 144 + (g < 2 ? n : n + 2) * 8;
+    const int oB = (g >> 1) * 144 + (n + 1) * 8 + (g & 1) * 4;
     const float *X = x + img * 4096;
     __syncthreads();
 
     auto load_x = [&](int q) {
         const int j = tid >> 4, k = tid & 15;
         const float v = X[(q * 16 + j) * 16 + k];
-        sm[F_X + (q % 3) * PL + (j + 1) * 18 + k + 1] = f2bf(HEAT ? v * inv_sc : v);
+        sm[F_X + (q % 3) * PL + (j + 1) * 18 + k + 1] = cvt1(HEAT ? v * inv_sc : v);
     };
-    auto xplane = [&](int q) -> const uint16_t * { return (q < 0 || q > 15) ? sm + F_XZ : sm + F_X + (q % 3) * PL; };
 
     load_x(0);
     for (int step = 0; step <= 17; step++) {
@@ -188,94 +243,102 @@ __global__ void __launch_bounds__(256) k_cnn_fwd(const float *__restrict__ x, fl
         __syncthreads();
         if (step <= 15) {                       // h1 plane step
             const int p = step;
+            f4 acc[4];
+#pragma unroll
+            for (int jj = 0; jj < 4; jj++) acc[jj] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int di = 0; di < 3; di++) {
+                const int q = p + di - 1;
+                if (q < 0 || q > 15) continue;
+                const uint16_t *XP_ = sm + F_X + (q % 3) * PL;
+#pragma unroll
+                for (int jj = 0; jj < 4; jj++) {
+                    const uint16_t *row = XP_ + (j0 + jj) * 18;
+                    uint32_t v[4];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        const uint32_t t = row[xo[e]];
+                        v[e] = xv[e] ? t : 0u;
+                    }
+                    acc[jj] = mfma16(a1[di], make_uint2(v[0] | (v[1] << 16), v[2] | (v[3] << 16)), acc[jj]);
+                }
+            }
 #pragma unroll
             for (int jj = 0; jj < 4; jj++) {
                 const int j = j0 + jj;
-                us8 b;
-#pragma unroll
-                for (int e = 0; e < 8; e++) {
-                    const int t = 8 * g + e;
-                    uint16_t val = 0;
-                    if (t < 27) {
-                        int di, dj, dk;
-                        tap_off(t, di, dj, dk);
-                        val = xplane(p + di)[(j + dj + 1) * 18 + n + dk + 1];
-                    }
-                    b[e] = val;
-                }
-                const f4 acc = mfma(w1, b, f4{0.0f, 0.0f, 0.0f, 0.0f});
-                const uint2 o = pack4(relu(acc[0] + bb1[0]), relu(acc[1] + bb1[1]), relu(acc[2] + bb1[2]),
-                                      relu(acc[3] + bb1[3]));
-                *reinterpret_cast<uint2 *>(sm + F_H1 + (p % 3) * PLANE + (j * 16 + n) * 16 + 4 * g) = o;
+                const uint2 o = cvt4(relu(acc[jj][0] + bb1[0]), relu(acc[jj][1] + bb1[1]), relu(acc[jj][2] + bb1[2]),
+                                     relu(acc[jj][3] + bb1[3]));
+                *reinterpret_cast<uint2 *>(sm + F_H1 + (p % 3) * HPL + (j + 1) * HROW + oB) = o;
                 if (STORE) *reinterpret_cast<uint2 *>(h1g + ((img * 4096 + (p * 16 + j) * 16 + n) * 16 + 4 * g)) = o;
             }
         }
         __syncthreads();
         if (step >= 1 && step <= 16) {          // h2 plane step-1
             const int p = step - 1;
-            f4 acc[4];
+            // the 16x16x32 and 16x16x16 products accumulate in separate chains
+            // (an accumulator is never shared by MFMAs of different shapes)
+            f4 acc[4], acc16[4];
 #pragma unroll
-            for (int jj = 0; jj < 4; jj++) acc[jj] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int jj = 0; jj < 4; jj++) {
+                acc[jj] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+                acc16[jj] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+            }
 #pragma unroll
-            for (int di = -1; di <= 1; di++) {
-                const int q = p + di;
+            for (int di = 0; di < 3; di++) {
+                const int q = p + di - 1;
                 if (q < 0 || q > 15) continue;
-                const uint16_t *P = sm + F_H1 + (q % 3) * PLANE;
+                const uint16_t *P = sm + F_H1 + (q % 3) * HPL;
+                // input rows j0 - 1 .. j0 + 4 (halo rows are zero); each row's
+                // operands feed the output lines within one row of it
 #pragma unroll
-                for (int jo = -1; jo <= 4; jo++) {
-                    const int J = j0 + jo;
-                    if (J < 0 || J > 15) continue;
-                    const uint2 b0 = *reinterpret_cast<const uint2 *>(P + (J * 16 + n) * 16 + 4 * g);
-                    const uint2 bm = make_uint2(shr1(b0.x), shr1(b0.y)), bp = make_uint2(shl1(b0.x), shl1(b0.y));
+                for (int ro = 0; ro < 6; ro++) {
+                    const uint16_t *row = P + (j0 + ro) * HROW;
+                    const uint4 bA = *reinterpret_cast<const uint4 *>(row + oA);
+                    const uint2 bB = *reinterpret_cast<const uint2 *>(row + oB);
 #pragma unroll
                     for (int jj = 0; jj < 4; jj++) {
-                        const int dj = jo - jj;
+                        const int dj = ro - 1 - jj;
                         if (dj < -1 || dj > 1) continue;
-                        const int tb = (di + 1) * 9 + (dj + 1) * 3;
-                        acc[jj] = mfma16(w2[tb + 0], bm, acc[jj]);
-                        acc[jj] = mfma16(w2[tb + 1], b0, acc[jj]);
-                        acc[jj] = mfma16(w2[tb + 2], bp, acc[jj]);
+                        const int pr = di * 3 + dj + 1;
+                        acc[jj] = mfma32(a2a[pr], bA, acc[jj]);
+                        acc16[jj] = mfma16(a2b[pr], bB, acc16[jj]);
                     }
                 }
             }
 #pragma unroll
             for (int jj = 0; jj < 4; jj++) {
                 const int j = j0 + jj;
-                const uint2 o = pack4(relu(acc[jj][0] + bb2[0]), relu(acc[jj][1] + bb2[1]), relu(acc[jj][2] + bb2[2]),
-                                      relu(acc[jj][3] + bb2[3]));
-                *reinterpret_cast<uint2 *>(sm + F_H2 + (p % 3) * PLANE + (j * 16 + n) * 16 + 4 * g) = o;
+                const f4 a = acc[jj] + acc16[jj];
+                const uint2 o = cvt4(relu(a[0] + bb2[0]), relu(a[1] + bb2[1]), relu(a[2] + bb2[2]), relu(a[3] + bb2[3]));
+                *reinterpret_cast<uint2 *>(sm + F_H2 + (p % 3) * HPL + (j + 1) * HROW + oB) = o;
                 if (STORE) *reinterpret_cast<uint2 *>(h2g + ((img * 4096 + (p * 16 + j) * 16 + n) * 16 + 4 * g)) = o;
             }
         }
         __syncthreads();
         if (step >= 2) {                        // output plane step-2
             const int p = step - 2;
-            f4 acc[4];
+            f4 D[6];
 #pragma unroll
-            for (int jj = 0; jj < 4; jj++) acc[jj] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int ro = 0; ro < 6; ro++) D[ro] = f4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int di = -1; di <= 1; di++) {
-                const int q = p + di;
+            for (int di = 0; di < 3; di++) {
+                const int q = p + di - 1;
                 if (q < 0 || q > 15) continue;
-                const uint16_t *P = sm + F_H2 + (q % 3) * PLANE;
+                const uint16_t *P = sm + F_H2 + (q % 3) * HPL;
 #pragma unroll
-                for (int jo = -1; jo <= 4; jo++) {
-                    const int J = j0 + jo;
-                    if (J < 0 || J > 15) continue;
-                    const uint2 b0 = *reinterpret_cast<const uint2 *>(P + (J * 16 + n) * 16 + 4 * g);
-#pragma unroll
-                    for (int jj = 0; jj < 4; jj++) {
-                        const int dj = jo - jj;
-                        if (dj < -1 || dj > 1) continue;
-                        acc[jj] = mfma16(w3[(di + 1) * 3 + (dj + 1)], b0, acc[jj]);
-                    }
-                }
+                for (int ro = 0; ro < 6; ro++)
+                    D[ro] = mfma16(a3[di], *reinterpret_cast<const uint2 *>(P + (j0 + ro) * HROW + oB), D[ro]);
             }
-            // rows 0, 1, 2 hold the dk = -1, 0, +1 tap sums at the input voxel:
-            // out[k] = R0[k-1] + R1[k] + R2[k+1]
+            // lane group g of D[ro] holds the taps dj = g - 1 at input voxel n,
+            // elements dk + 1: t = their sum at output voxel n (DPP for dk)
+            float t[6];
+#pragma unroll
+            for (int ro = 0; ro < 6; ro++) t[ro] = shr1f(D[ro][0]) + D[ro][1] + shl1f(D[ro][2]);
 #pragma unroll
             for (int jj = 0; jj < 4; jj++) {
-                const float o = shr1f(acc[jj][0]) + acc[jj][1] + shl1f(acc[jj][2]);
+                // output row j0 + jj: dj = -1 from row jj (group 0), dj = 0 from
+                // row jj + 1 (group 1), dj = +1 from row jj + 2 (group 2)
+                const float o = (t[jj] + __shfl(t[jj + 1], 16 + n, 64)) + __shfl(t[jj + 2], 32 + n, 64);
                 if (g == 0) {
                     if (HEAT) {
                         const long long sys = img / hio.nb, b = img - sys * hio.nb;
@@ -568,7 +631,7 @@ CnnModel::CnnModel() {
     CCHECK(hipMalloc(&adam_m_, sizeof(float) * kCnnParams));
     CCHECK(hipMalloc(&adam_v_, sizeof(float) * kCnnParams));
     CCHECK(hipMalloc(&grad_, sizeof(double) * kCnnParams));
-    CCHECK(hipMalloc(&packed_, sizeof(uint16_t) * (PK_TOTAL * 8 + PF_TOTAL * 4)));
+    CCHECK(hipMalloc(&packed_, sizeof(uint16_t) * (PK_TOTAL * 8 + PF_TOTAL)));
     CCHECK(hipMalloc(&loss_part_, sizeof(double) * 1025));
     init(12345);
 }
@@ -611,7 +674,7 @@ void CnnModel::pack(hipStream_t stream) {
     if (packed_valid_) return;
     hipLaunchKernelGGL(k_cnn_pack, dim3((PK_TOTAL * 8 + 255) / 256), dim3(256), 0, stream, (const float *)params_,
                        (uint16_t *)packed_);
-    hipLaunchKernelGGL(k_cnn_pack16, dim3((PF_TOTAL * 4 + 255) / 256), dim3(256), 0, stream, (const float *)params_,
+    hipLaunchKernelGGL(k_cnn_packf, dim3((PF_TOTAL + 255) / 256), dim3(256), 0, stream, (const float *)params_,
                        (uint16_t *)packed_ + PK_TOTAL * 8);
     packed_valid_ = true;
 }
@@ -638,7 +701,7 @@ void CnnModel::infer(const float *x, float *out, long long nimg, hipStream_t str
     if (nimg <= 0) return;
     pack(stream);
     hipLaunchKernelGGL((k_cnn_fwd<false, false>), dim3((unsigned)nimg), dim3(256), 0, stream, x, out,
-                       (const us8 *)packed_, (const uint2 *)((uint16_t *)packed_ + PK_TOTAL * 8), (const float *)params_,
+                       (const uint16_t *)packed_ + PK_TOTAL * 8, (const float *)params_,
                        (uint16_t *)nullptr, (uint16_t *)nullptr, HeatIO{});
 }
 
@@ -648,7 +711,7 @@ void CnnModel::infer_heat(const float *x, const double *scale, int scale_stride,
     pack(stream);
     HeatIO hio{scale, scale_stride, nb, istride, c, active};
     hipLaunchKernelGGL((k_cnn_fwd<false, true>), dim3((unsigned)nimg), dim3(256), 0, stream, x, (float *)nullptr,
-                       (const us8 *)packed_, (const uint2 *)((uint16_t *)packed_ + PK_TOTAL * 8), (const float *)params_,
+                       (const uint16_t *)packed_ + PK_TOTAL * 8, (const float *)params_,
                        (uint16_t *)nullptr, (uint16_t *)nullptr, hio);
 }
 
@@ -667,8 +730,8 @@ void CnnModel::backward(const float *x, int nimg, hipStream_t stream) {
 void CnnModel::gradient(const float *x, const float *gout, int nimg, double *grad_host, hipStream_t stream) {
     ensure_batch(nimg);
     pack(stream);
-    hipLaunchKernelGGL((k_cnn_fwd<true, false>), dim3(nimg), dim3(256), 0, stream, x, out_, (const us8 *)packed_,
-                       (const uint2 *)((uint16_t *)packed_ + PK_TOTAL * 8), (const float *)params_, h1_, h2_, HeatIO{});
+    hipLaunchKernelGGL((k_cnn_fwd<true, false>), dim3(nimg), dim3(256), 0, stream, x, out_,
+                       (const uint16_t *)packed_ + PK_TOTAL * 8, (const float *)params_, h1_, h2_, HeatIO{});
     CCHECK(hipMemcpyAsync(g3_, gout, sizeof(float) * 4096 * (size_t)nimg, hipMemcpyDeviceToDevice, stream));
     backward(x, nimg, stream);
     CCHECK(hipStreamSynchronize(stream));
@@ -678,8 +741,8 @@ void CnnModel::gradient(const float *x, const float *gout, int nimg, double *gra
 double CnnModel::train_step(const float *x, const float *y, int nimg, double lr, hipStream_t stream) {
     ensure_batch(nimg);
     pack(stream);
-    hipLaunchKernelGGL((k_cnn_fwd<true, false>), dim3(nimg), dim3(256), 0, stream, x, out_, (const us8 *)packed_,
-                       (const uint2 *)((uint16_t *)packed_ + PK_TOTAL * 8), (const float *)params_, h1_, h2_, HeatIO{});
+    hipLaunchKernelGGL((k_cnn_fwd<true, false>), dim3(nimg), dim3(256), 0, stream, x, out_,
+                       (const uint16_t *)packed_ + PK_TOTAL * 8, (const float *)params_, h1_, h2_, HeatIO{});
     const long long n = 4096LL * nimg;
     hipLaunchKernelGGL(k_cnn_loss, dim3(1024), dim3(256), 0, stream, (const float *)out_, y, n, g3_, loss_part_);
     hipLaunchKernelGGL(k_cnn_sum, dim3(1), dim3(64), 0, stream, (const double *)loss_part_, 1024, 1.0 / (double)n,
