@@ -57,9 +57,11 @@ constexpr int PK_P1 = 0, PK_Q3 = 64, PK_Q2 = 128, PK_TOTAL = PK_Q2 + 14 * 64;
 constexpr int PF_L1 = 0, PF_L2A = PF_L1 + 3 * 64 * 4, PF_L2B = PF_L2A + 9 * 64 * 8, PF_L3 = PF_L2B + 9 * 64 * 4,
               PF_TOTAL = PF_L3 + 3 * 64 * 4;
 // fused forward LDS carve (uint16 elements): h1 and h2 rings of 3 planes, each
-// [18 j][2 channel halves][18 k][8 c] with zero halo rows and voxels; x ring
+// [18 j][4 channel groups][18 k][4 c] with zero halo rows and voxels (a lane's
+// 8-byte access is consecutive to its neighbour's: no bank conflicts); x ring
 // of 3 planes [18][18] with a zero halo
-constexpr int HROW = 2 * 18 * 8, HPL = 18 * HROW;
+constexpr int HGRP = 18 * 4, HROW = 4 * HGRP, HPL = 18 * HROW;
+constexpr int kFwdWaves = 8;                // wavefronts per image (16 / kFwdWaves k-lines each)
 constexpr int F_H1 = 0, F_H2 = 3 * HPL, F_X = 6 * HPL, F_TOTAL = F_X + 3 * PL;
 // single-layer LDS carve: ring of 3 input planes + zero plane
 constexpr int S_IN = 0, S_Z = 3 * PL * 16, S_TOTAL = 4 * PL * 16;
@@ -175,8 +177,8 @@ __device__ __forceinline__ f4 mfma32(uint4 a, uint4 b, f4 c) {
                                                    0);
 }
 
-template <bool STORE, bool HEAT>
-__global__ void __launch_bounds__(256) k_cnn_fwd(const float *__restrict__ x, float *__restrict__ out,
+template <bool STORE, bool HEAT, int NW>
+__global__ void __launch_bounds__(64 * NW) k_cnn_fwd(const float *__restrict__ x, float *__restrict__ out,
                                                  const uint16_t *__restrict__ pf, const float *__restrict__ prm,
                                                  uint16_t *__restrict__ h1g, uint16_t *__restrict__ h2g, HeatIO hio) {
     __shared__ __align__(16) uint16_t sm[F_TOTAL];
@@ -191,9 +193,10 @@ __global__ void __launch_bounds__(256) k_cnn_fwd(const float *__restrict__ x, fl
     const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, n = lane & 15;
     // wavefront index as a scalar: the row-range tests below are uniform
     // branches, so the MFMA accumulators stay in place across them
+    constexpr int LPW = 16 / NW;            // k-lines per wavefront
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int j0 = 4 * wave;
-    for (int q = tid; q < F_TOTAL / 2; q += 256) reinterpret_cast<uint32_t *>(sm)[q] = 0u;
+    const int j0 = LPW * wave;
+    for (int q = tid; q < F_TOTAL / 2; q += 64 * NW) reinterpret_cast<uint32_t *>(sm)[q] = 0u;
 
     uint2 a1[3], a2b[9], a3[3];
     uint4 a2a[9];
@@ -224,35 +227,51 @@ __global__ void __launch_bounds__(256) k_cnn_fwd(const float *__restrict__ x, fl
         xv[e] = s9 < 9;
         xo[e] = xv[e] ? (s9 / 3) * 18 + s9 % 3 + n : n;
     }
-    // per-lane LDS offsets in an h plane row: the k-shifted 8-channel chunk
-    // (dk = -1 for g < 2, +1 for g >= 2) and the 4-channel chunk at k
-    const int oA = (g & 1) * 144 + (g < 2 ? n : n + 2) * 8;
-    const int oB = (g >> 1) * 144 + (n + 1) * 8 + (g & 1) * 4;
+    // per-lane LDS offsets in an h plane row: the k-shifted 8 channels
+    // 8(g&1) .. +7 (dk = -1 for g < 2, +1 for g >= 2) as two 4-channel groups,
+    // and the 4 channels 4g .. 4g+3 at k
+    const int oA = (2 * (g & 1)) * HGRP + (g < 2 ? n : n + 2) * 4;
+    const int oB = g * HGRP + (n + 1) * 4;
     const float *X = x + img * 4096;
     __syncthreads();
 
-    auto load_x = [&](int q) {
-        const int j = tid >> 4, k = tid & 15;
-        const float v = X[(q * 16 + j) * 16 + k];
-        sm[F_X + (q % 3) * PL + (j + 1) * 18 + k + 1] = cvt1(HEAT ? v * inv_sc : v);
+    // Planes -1 and 16 (the block's zero padding in i) are ring slots holding
+    // zeros: slot 2 is not written before plane -1 is read, and plane 16 of x,
+    // h1 and h2 is written as zeros one step after plane 15.  So every layer
+    // runs its MFMAs over 3 planes without branches.
+    // x planes are fetched from HBM one step before they are stored to LDS,
+    // so the load latency overlaps a step of MFMA work
+    const int xj = (tid >> 4) & 15, xk = tid & 15;
+    const bool xl = tid < 256;              // the first 256 lanes move x
+    auto fetch_x = [&](int q) -> float { return q < 16 ? X[(q * 16 + xj) * 16 + xk] : 0.0f; };
+    auto store_x = [&](int q, float v) {
+        sm[F_X + (q % 3) * PL + (xj + 1) * 18 + xk + 1] = cvt1(HEAT ? v * inv_sc : v);
     };
 
-    load_x(0);
+    if (xl) store_x(0, fetch_x(0));
+    float xnext = xl ? fetch_x(1) : 0.0f;
     for (int step = 0; step <= 17; step++) {
-        if (step + 1 <= 15) load_x(step + 1);
+        if (xl && step + 1 <= 16) {
+            store_x(step + 1, xnext);
+            if (step + 2 <= 16) xnext = fetch_x(step + 2);
+        }
         __syncthreads();
+        if (step == 16) {                       // h1 plane 16 = 0
+#pragma unroll
+            for (int jj = 0; jj < LPW; jj++)
+                *reinterpret_cast<uint2 *>(sm + F_H1 + (16 % 3) * HPL + (j0 + jj + 1) * HROW + oB) = make_uint2(0u, 0u);
+        }
         if (step <= 15) {                       // h1 plane step
             const int p = step;
-            f4 acc[4];
+            f4 acc[LPW];
 #pragma unroll
-            for (int jj = 0; jj < 4; jj++) acc[jj] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int jj = 0; jj < LPW; jj++) acc[jj] = f4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int di = 0; di < 3; di++) {
                 const int q = p + di - 1;
-                if (q < 0 || q > 15) continue;
-                const uint16_t *XP_ = sm + F_X + (q % 3) * PL;
+                const uint16_t *XP_ = sm + F_X + ((q + 3) % 3) * PL;
 #pragma unroll
-                for (int jj = 0; jj < 4; jj++) {
+                for (int jj = 0; jj < LPW; jj++) {
                     const uint16_t *row = XP_ + (j0 + jj) * 18;
                     uint32_t v[4];
 #pragma unroll
@@ -264,7 +283,7 @@ __global__ void __launch_bounds__(256) k_cnn_fwd(const float *__restrict__ x, fl
                 }
             }
 #pragma unroll
-            for (int jj = 0; jj < 4; jj++) {
+            for (int jj = 0; jj < LPW; jj++) {
                 const int j = j0 + jj;
                 const uint2 o = cvt4(relu(acc[jj][0] + bb1[0]), relu(acc[jj][1] + bb1[1]), relu(acc[jj][2] + bb1[2]),
                                      relu(acc[jj][3] + bb1[3]));
@@ -273,30 +292,36 @@ __global__ void __launch_bounds__(256) k_cnn_fwd(const float *__restrict__ x, fl
             }
         }
         __syncthreads();
+        if (step == 17) {                       // h2 plane 16 = 0
+#pragma unroll
+            for (int jj = 0; jj < LPW; jj++)
+                *reinterpret_cast<uint2 *>(sm + F_H2 + (16 % 3) * HPL + (j0 + jj + 1) * HROW + oB) = make_uint2(0u, 0u);
+        }
         if (step >= 1 && step <= 16) {          // h2 plane step-1
             const int p = step - 1;
             // the 16x16x32 and 16x16x16 products accumulate in separate chains
             // (an accumulator is never shared by MFMAs of different shapes)
-            f4 acc[4], acc16[4];
+            f4 acc[LPW], acc16[LPW];
 #pragma unroll
-            for (int jj = 0; jj < 4; jj++) {
+            for (int jj = 0; jj < LPW; jj++) {
                 acc[jj] = f4{0.0f, 0.0f, 0.0f, 0.0f};
                 acc16[jj] = f4{0.0f, 0.0f, 0.0f, 0.0f};
             }
 #pragma unroll
             for (int di = 0; di < 3; di++) {
                 const int q = p + di - 1;
-                if (q < 0 || q > 15) continue;
-                const uint16_t *P = sm + F_H1 + (q % 3) * HPL;
-                // input rows j0 - 1 .. j0 + 4 (halo rows are zero); each row's
+                const uint16_t *P = sm + F_H1 + ((q + 3) % 3) * HPL;
+                // input rows j0 - 1 .. j0 + LPW (halo rows are zero); each row's
                 // operands feed the output lines within one row of it
 #pragma unroll
-                for (int ro = 0; ro < 6; ro++) {
+                for (int ro = 0; ro < LPW + 2; ro++) {
                     const uint16_t *row = P + (j0 + ro) * HROW;
-                    const uint4 bA = *reinterpret_cast<const uint4 *>(row + oA);
+                    const uint2 bA0 = *reinterpret_cast<const uint2 *>(row + oA);
+                    const uint2 bA1 = *reinterpret_cast<const uint2 *>(row + oA + HGRP);
+                    const uint4 bA = make_uint4(bA0.x, bA0.y, bA1.x, bA1.y);
                     const uint2 bB = *reinterpret_cast<const uint2 *>(row + oB);
 #pragma unroll
-                    for (int jj = 0; jj < 4; jj++) {
+                    for (int jj = 0; jj < LPW; jj++) {
                         const int dj = ro - 1 - jj;
                         if (dj < -1 || dj > 1) continue;
                         const int pr = di * 3 + dj + 1;
@@ -306,7 +331,7 @@ __global__ void __launch_bounds__(256) k_cnn_fwd(const float *__restrict__ x, fl
                 }
             }
 #pragma unroll
-            for (int jj = 0; jj < 4; jj++) {
+            for (int jj = 0; jj < LPW; jj++) {
                 const int j = j0 + jj;
                 const f4 a = acc[jj] + acc16[jj];
                 const uint2 o = cvt4(relu(a[0] + bb2[0]), relu(a[1] + bb2[1]), relu(a[2] + bb2[2]), relu(a[3] + bb2[3]));
@@ -317,25 +342,24 @@ __global__ void __launch_bounds__(256) k_cnn_fwd(const float *__restrict__ x, fl
         __syncthreads();
         if (step >= 2) {                        // output plane step-2
             const int p = step - 2;
-            f4 D[6];
+            f4 D[LPW + 2];
 #pragma unroll
-            for (int ro = 0; ro < 6; ro++) D[ro] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int ro = 0; ro < LPW + 2; ro++) D[ro] = f4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int di = 0; di < 3; di++) {
                 const int q = p + di - 1;
-                if (q < 0 || q > 15) continue;
-                const uint16_t *P = sm + F_H2 + (q % 3) * HPL;
+                const uint16_t *P = sm + F_H2 + ((q + 3) % 3) * HPL;
 #pragma unroll
-                for (int ro = 0; ro < 6; ro++)
+                for (int ro = 0; ro < LPW + 2; ro++)
                     D[ro] = mfma16(a3[di], *reinterpret_cast<const uint2 *>(P + (j0 + ro) * HROW + oB), D[ro]);
             }
             // lane group g of D[ro] holds the taps dj = g - 1 at input voxel n,
             // elements dk + 1: t = their sum at output voxel n (DPP for dk)
-            float t[6];
+            float t[LPW + 2];
 #pragma unroll
-            for (int ro = 0; ro < 6; ro++) t[ro] = shr1f(D[ro][0]) + D[ro][1] + shl1f(D[ro][2]);
+            for (int ro = 0; ro < LPW + 2; ro++) t[ro] = shr1f(D[ro][0]) + D[ro][1] + shl1f(D[ro][2]);
 #pragma unroll
-            for (int jj = 0; jj < 4; jj++) {
+            for (int jj = 0; jj < LPW; jj++) {
                 // output row j0 + jj: dj = -1 from row jj (group 0), dj = 0 from
                 // row jj + 1 (group 1), dj = +1 from row jj + 2 (group 2)
                 const float o = (t[jj] + __shfl(t[jj + 1], 16 + n, 64)) + __shfl(t[jj + 2], 32 + n, 64);
@@ -700,7 +724,7 @@ void CnnModel::ensure_batch(int nimg) {
 void CnnModel::infer(const float *x, float *out, long long nimg, hipStream_t stream) {
     if (nimg <= 0) return;
     pack(stream);
-    hipLaunchKernelGGL((k_cnn_fwd<false, false>), dim3((unsigned)nimg), dim3(256), 0, stream, x, out,
+    hipLaunchKernelGGL((k_cnn_fwd<false, false, kFwdWaves>), dim3((unsigned)nimg), dim3(64 * kFwdWaves), 0, stream, x, out,
                        (const uint16_t *)packed_ + PK_TOTAL * 8, (const float *)params_,
                        (uint16_t *)nullptr, (uint16_t *)nullptr, HeatIO{});
 }
@@ -710,7 +734,7 @@ void CnnModel::infer_heat(const float *x, const double *scale, int scale_stride,
     if (nimg <= 0) return;
     pack(stream);
     HeatIO hio{scale, scale_stride, nb, istride, c, active};
-    hipLaunchKernelGGL((k_cnn_fwd<false, true>), dim3((unsigned)nimg), dim3(256), 0, stream, x, (float *)nullptr,
+    hipLaunchKernelGGL((k_cnn_fwd<false, true, kFwdWaves>), dim3((unsigned)nimg), dim3(64 * kFwdWaves), 0, stream, x, (float *)nullptr,
                        (const uint16_t *)packed_ + PK_TOTAL * 8, (const float *)params_,
                        (uint16_t *)nullptr, (uint16_t *)nullptr, hio);
 }
@@ -730,7 +754,7 @@ void CnnModel::backward(const float *x, int nimg, hipStream_t stream) {
 void CnnModel::gradient(const float *x, const float *gout, int nimg, double *grad_host, hipStream_t stream) {
     ensure_batch(nimg);
     pack(stream);
-    hipLaunchKernelGGL((k_cnn_fwd<true, false>), dim3(nimg), dim3(256), 0, stream, x, out_,
+    hipLaunchKernelGGL((k_cnn_fwd<true, false, kFwdWaves>), dim3(nimg), dim3(64 * kFwdWaves), 0, stream, x, out_,
                        (const uint16_t *)packed_ + PK_TOTAL * 8, (const float *)params_, h1_, h2_, HeatIO{});
     CCHECK(hipMemcpyAsync(g3_, gout, sizeof(float) * 4096 * (size_t)nimg, hipMemcpyDeviceToDevice, stream));
     backward(x, nimg, stream);
@@ -741,7 +765,7 @@ void CnnModel::gradient(const float *x, const float *gout, int nimg, double *gra
 double CnnModel::train_step(const float *x, const float *y, int nimg, double lr, hipStream_t stream) {
     ensure_batch(nimg);
     pack(stream);
-    hipLaunchKernelGGL((k_cnn_fwd<true, false>), dim3(nimg), dim3(256), 0, stream, x, out_,
+    hipLaunchKernelGGL((k_cnn_fwd<true, false, kFwdWaves>), dim3(nimg), dim3(64 * kFwdWaves), 0, stream, x, out_,
                        (const uint16_t *)packed_ + PK_TOTAL * 8, (const float *)params_, h1_, h2_, HeatIO{});
     const long long n = 4096LL * nimg;
     hipLaunchKernelGGL(k_cnn_loss, dim3(1024), dim3(256), 0, stream, (const float *)out_, y, n, g3_, loss_part_);
