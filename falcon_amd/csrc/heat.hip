@@ -183,8 +183,8 @@ struct OpJacobi {                // y = x + omega (r - A x)/diag; DOT: y.dq
         if (DOT) v[0] += yn * dq;
     }
 };
-struct OpFeat {                  // Az, r' = r - Az, u = r'/diag (float), u.u
-    double *Az, *rp;
+struct OpFeat {                  // r' = r - A z, u = r'/diag (float), u.u
+    double *rp;
     float *u;
     const double *in0, *in1;     // r; unused
     static constexpr int NP = 1;
@@ -192,23 +192,21 @@ struct OpFeat {                  // Az, r' = r - Az, u = r'/diag (float), u.u
     __device__ __forceinline__ void cell(long long io, double, double ax, double diag, double r, double,
                                          double (&v)[NP]) const {
         const double res = r - ax;
-        Az[io] = ax;
         rp[io] = res;
         const double w = res / diag;
         u[io] = (float)w;
         v[0] += w * w;
     }
 };
-struct OpApplyDot {              // Ac = A c; c.r', c.Ac
-    double *Ac;
-    const double *in0, *in1;     // r'; unused
-    static constexpr int NP = 2;
-    static constexpr bool IN0 = true, IN1 = false;
-    __device__ __forceinline__ void cell(long long io, double x, double ax, double, double rp, double,
+struct OpApplyDot {              // c.r', c.Ac, c.q (Ac itself is not stored)
+    const double *in0, *in1;     // r'; q
+    static constexpr int NP = 3;
+    static constexpr bool IN0 = true, IN1 = true;
+    __device__ __forceinline__ void cell(long long, double x, double ax, double, double rp, double q,
                                          double (&v)[NP]) const {
-        Ac[io] = ax;
         v[0] += x * rp;
         v[1] += x * ax;
+        v[2] += x * q;
     }
 };
 
@@ -373,17 +371,20 @@ __global__ void __launch_bounds__(256) k_hmarch(HLev L, const double *__restrict
 }
 
 // residual restriction fused with the residual: coarse cell = sum over its 8
-// children (di, dj, dk order) of r - A x (integrated residual)
+// children (di, dj, dk order) of r - A x (integrated residual).  The children
+// columns (j, k), (j, k+1), (j+1, k), (j+1, k+1) of an even (j, k) lie in one
+// wavefront (lanes t, t + 1, t + m, t + m + 1): their residuals are gathered
+// by lane shuffles, the i pair from the thread's own column.
 template <int LM>
 __global__ void __launch_bounds__(256) k_hresrestrict(HLev L, const double *__restrict__ x, const double *__restrict__ r,
                                                       double *__restrict__ rc, const double *__restrict__ inv_dt,
                                                       const int *__restrict__ active, long long istride,
                                                       long long cstride) {
-    constexpr int m = 1 << LM, mm = m * m, bpw = 256 >> (2 * LM), LC = LM - 1, mc = m / 2;
+    constexpr int m = 1 << LM, mm = m * m, bpw = 256 >> (2 * LM), LC = LM - 1;
     __shared__ double sm[TileLds<LM>::SIZE];
     const int s = blockIdx.y;
     if (!active[s]) return;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63;
     const int lb = tid >> (2 * LM), j = (tid >> LM) & (m - 1), k = tid & (m - 1);
     const int b0 = xcd_group(blockIdx.x, gridDim.x) * bpw, b = b0 + lb;
     const bool own = b < L.nb;
@@ -392,36 +393,22 @@ __global__ void __launch_bounds__(256) k_hresrestrict(HLev L, const double *__re
     const long long cb = ((long long)b << (3 * LM)) + (j << LM) + k;
     double col[m], a0[m], a1[m];
     load_tile<LM, false, true, false>(L, F, b0, own, cb, r, nullptr, so, col, a0, a1, sm);
-    double res[m];
     const double idt = inv_dt[s];
+    const bool out = own && !(j & 1) && !(k & 1);
+    double *R = rc + s * cstride + ((long long)b << (3 * LC)) + ((j >> 1) << LC) + (k >> 1);
 #pragma unroll
-    for (int i = 0; i < m; i++) {
+    for (int ic = 0; ic < m / 2; ic++) {
         double diag;
-        const double ax = tile_apply<LM>(sm, col, i, lb, j, k, idt, diag);
-        res[i] = own ? a0[i] - ax : 0.0;
-    }
-    // children sums through LDS (the ghost area, free once every lane has
-    // applied the operator), two fine planes at a time (plane pair ic in
-    // rs[2][256]); the coarse cells of the pair are (lbc, ic, jc, kc)
-    double *rs = sm + TileLds<LM>::G0;
-    constexpr int ncp = bpw * mc * mc;       // coarse cells per plane pair
-    __syncthreads();
-#pragma unroll
-    for (int ic = 0; ic < mc; ic++) {
-        rs[tid] = res[2 * ic];
-        rs[256 + tid] = res[2 * ic + 1];
-        __syncthreads();
-        if (tid < ncp) {
-            const int lbc = tid / (mc * mc), jc = (tid / mc) % mc, kc = tid % mc;
-            if (b0 + lbc < L.nb) {
-                const double *c0 = rs + lbc * mm + (2 * jc) * m + 2 * kc, *c1 = c0 + 256;
-                double t = 0.0;
-                t += c0[0]; t += c0[1]; t += c0[m]; t += c0[m + 1];
-                t += c1[0]; t += c1[1]; t += c1[m]; t += c1[m + 1];
-                rc[s * cstride + ((long long)(b0 + lbc) << (3 * LC)) + (ic << (2 * LC)) + (jc << LC) + kc] = t;
-            }
-        }
-        __syncthreads();
+        const double ax0 = tile_apply<LM>(sm, col, 2 * ic, lb, j, k, idt, diag);
+        const double ax1 = tile_apply<LM>(sm, col, 2 * ic + 1, lb, j, k, idt, diag);
+        const double o0 = own ? a0[2 * ic] - ax0 : 0.0, o1 = own ? a0[2 * ic + 1] - ax1 : 0.0;
+        const double k0 = __shfl(o0, (lane + 1) & 63, 64), k1 = __shfl(o1, (lane + 1) & 63, 64);
+        const double j0 = __shfl(o0, (lane + m) & 63, 64), j1 = __shfl(o1, (lane + m) & 63, 64);
+        const double q0 = __shfl(o0, (lane + m + 1) & 63, 64), q1 = __shfl(o1, (lane + m + 1) & 63, 64);
+        double t = 0.0;
+        t += o0; t += k0; t += j0; t += q0;
+        t += o1; t += k1; t += j1; t += q1;
+        if (out) R[(long long)ic << (2 * LC)] = t;
     }
 }
 
@@ -499,13 +486,16 @@ __global__ void __launch_bounds__(256) k_hdot_final(const double *__restrict__ p
 
 // Per-system scalars on the device (one thread per system), kSc slots each:
 // [0] alpha = <p,r>/<p,q>, [1] beta = <z,q>/gamma, [2] gamma = <p,q>,
-// [3] feature scale of the learned correction, [4] its step along c, [5] <b,b>.
+// [3] feature scale of the learned correction, [4] its step along c, [5] <b,b>,
+// [6] <z,q> of the V-cycle output.
 constexpr int kSc = 8;
 // what 0: alpha and gamma (dots: [0]=<p,q>, [1]=<p,r>)
 // what 1: beta (dots: [0]=<z,q>)
 // what 2: correction step <c,r'>/<c,Ac> (dots: [0]=<c,r'>, [1]=<c,Ac>), 0 unless <c,Ac> > 0
 // what 3: feature scale sqrt(sum / ncell) (dots: [0]=sum (r'/diag)^2)
 // what 4: <b,b> (dots: [0])
+// what 5: beta of the learned direction from [6], [4] and dots[2] = <c,q>
+// what 6: [6] = <z,q> (dots: [0])
 __global__ void k_hscalars(int what, int nsys, const double *__restrict__ dots, double *__restrict__ sc,
                            const int *__restrict__ active, double ncell) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -522,8 +512,13 @@ __global__ void k_hscalars(int what, int nsys, const double *__restrict__ dots, 
         S[4] = (active[s] && den > 0.0 && a == a && a < 1e300 && a > -1e300) ? a : 0.0;
     } else if (what == 3) {
         S[3] = active[s] ? sqrt(dots[3 * s] / ncell) : 0.0;
-    } else {
+    } else if (what == 4) {
         S[5] = dots[3 * s];
+    } else if (what == 5) {
+        // beta of the learned direction: <z + a c, q>/gamma = (<z,q> + a <c,q>)/gamma
+        S[1] = active[s] ? (S[6] + S[4] * dots[3 * s + 2]) / S[2] : 0.0;
+    } else {
+        S[6] = dots[3 * s];                 // <z, q> of the V-cycle output
     }
 }
 
@@ -612,59 +607,19 @@ __global__ void __launch_bounds__(256) k_hcopy(const double *__restrict__ a, dou
 }
 
 // ---- learned correction of the V-cycle (features: k_hmarch with OpFeat;
-// A c: k_hmarch with OpApplyDot)
-
-// z += a c, Az += a Ac, partial sums of z.q (q = A p of the previous direction)
-__global__ void __launch_bounds__(256) k_hzcorr(long long n, double *__restrict__ z, const double *__restrict__ cc,
-                                                double *__restrict__ Az, const double *__restrict__ Ac,
-                                                const double *__restrict__ q, const double *__restrict__ sc,
-                                                const int *__restrict__ active, double *__restrict__ partial,
-                                                long long istride) {
+// the step's dots: k_hmarch with OpApplyDot)
+// p = z + a c - beta p: the learned direction (a = the correction step);
+// beta = 0 starts a fresh direction
+__global__ void __launch_bounds__(256) k_hupdate_p2(long long n, double *__restrict__ p, const double *__restrict__ z,
+                                                    const double *__restrict__ cc, const double *__restrict__ sc,
+                                                    const int *__restrict__ active, long long istride, int use_beta) {
     const int s = blockIdx.y;
     if (!active[s]) return;
-    const double a = sc[kSc * s + 4];
-    double v[1] = {0.0};
-    const long long c0 = (long long)blockIdx.x * kElem + threadIdx.x;
-#pragma unroll 4
-    for (int u = 0; u < kElem / 256; u++) {
-        const long long c = c0 + 256 * u;
-        if (c < n) {
-            const long long io = s * istride + c;
-            const double zn = z[io] + a * cc[io];
-            z[io] = zn;
-            Az[io] = Az[io] + a * Ac[io];
-            v[0] += zn * q[io];
-        }
-    }
-    block_partials<1>(v, partial, s);
-}
-
-// p = z - beta p and q = Az - beta q (the new direction and its image without
-// an operator apply), with partial sums of p.q and p.r for the next step
-__global__ void __launch_bounds__(256) k_hupdate_pq(long long n, double *__restrict__ p, const double *__restrict__ z,
-                                                    double *__restrict__ q, const double *__restrict__ Az,
-                                                    const double *__restrict__ r, const double *__restrict__ sc,
-                                                    const int *__restrict__ active, double *__restrict__ partial,
-                                                    long long istride, int use_beta) {
-    const int s = blockIdx.y;
-    if (!active[s]) return;
-    const double be = use_beta ? sc[kSc * s + 1] : 0.0;
-    double v[2] = {0.0, 0.0};
-    const long long c0 = (long long)blockIdx.x * kElem + threadIdx.x;
-#pragma unroll 4
-    for (int u = 0; u < kElem / 256; u++) {
-        const long long c = c0 + 256 * u;
-        if (c < n) {
-            const long long io = s * istride + c;
-            const double pn = z[io] - be * p[io];
-            const double qn = Az[io] - be * q[io];
-            p[io] = pn;
-            q[io] = qn;
-            v[0] += pn * qn;
-            v[1] += pn * r[io];
-        }
-    }
-    block_partials<2>(v, partial, s);
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const double a = sc[kSc * s + 4], be = use_beta ? sc[kSc * s + 1] : 0.0;
+    const long long io = s * istride + c;
+    p[io] = (z[io] + a * cc[io]) - be * p[io];
 }
 
 // training records: copy the network input image and w = x + z of record slot
@@ -713,7 +668,7 @@ struct HeatSolver::Impl {
     double *inv_dt = nullptr, *partial = nullptr, *dots = nullptr, *sc = nullptr;
     int *active = nullptr;
     // learned correction work arrays (allocated on first use)
-    double *Az = nullptr, *rp = nullptr, *Ac = nullptr, *cc = nullptr;
+    double *rp = nullptr, *cc = nullptr;
     float *xin = nullptr;
     int learned_pre0 = 2, learned_post0 = 2;
     // captured FCG iteration per mode (plain, learned) and device-side convergence state
@@ -735,7 +690,7 @@ struct HeatSolver::Impl {
     dim3 grid_elem() const { return dim3(elem_wgs(), (unsigned)nsys); }
     void ensure_learned();
     void features(double *z);
-    void learned_correction(double *z);
+    void learned_correction(bool first);
     void final_dots(int npairs, unsigned nwg) {
         hipLaunchKernelGGL(k_hdot_final, dim3((unsigned)nsys), dim3(256), 0, stream, (const double *)partial, (int)nwg,
                            npairs, dots);
@@ -1023,10 +978,8 @@ HeatResult HeatSolver::solve(double tol, int max_iter, bool use_corrector) {
 }
 
 void HeatSolver::Impl::ensure_learned() {
-    if (Az) return;
-    Az = dalloc(ist(0) * nsys);
+    if (rp) return;
     rp = dalloc(ist(0) * nsys);
-    Ac = dalloc(ist(0) * nsys);
     cc = dalloc(ist(0) * nsys);
     void *a = nullptr;
     HCHECK(hipMalloc(&a, sizeof(float) * ist(0) * nsys));
@@ -1034,25 +987,22 @@ void HeatSolver::Impl::ensure_learned() {
     xin = (float *)a;
 }
 
-// Az = A z, r' = r - A z, network input r'/diag and its per-system scale
-// (root mean square) in sc[3]
+// r' = r - A z, network input r'/diag and its per-system scale (root mean
+// square) in sc[3]
 void HeatSolver::Impl::features(double *z) {
-    launch_march<false>(0, z, nullptr, 0, OpFeat{Az, rp, xin, r, nullptr}, partial);
+    launch_march<false>(0, z, nullptr, 0, OpFeat{rp, xin, r, nullptr}, partial);
     final_dots(1, march_wgs(0));
     scalars(3, (double)ist(0));
 }
 
-// c = scale * net(features / scale), a = <c,r'>/<c,Ac>, z += a c, Az += a Ac,
-// and beta = <z, q>/gamma of the FCG direction update (sc[1])
-void HeatSolver::Impl::learned_correction(double *z) {
+// c = scale * net(features / scale) and its step a = <c,r'>/<c,Ac> (sc[4]);
+// unless first, beta = <z + a c, q>/gamma of the direction update (sc[1])
+void HeatSolver::Impl::learned_correction(bool first) {
     corr->infer_heat(xin, sc + 3, kSc, nb, ist(0), cc, (long long)nsys * nb, active, stream);
-    launch_march<false>(0, cc, nullptr, 0, OpApplyDot{Ac, rp, nullptr}, partial);
-    final_dots(2, march_wgs(0));
+    launch_march<false>(0, cc, nullptr, 0, OpApplyDot{rp, q}, partial);
+    final_dots(3, march_wgs(0));
     scalars(2);
-    hipLaunchKernelGGL(k_hzcorr, grid_elem(), dim3(256), 0, stream, ist(0), z, (const double *)cc, Az, (const double *)Ac,
-                       (const double *)q, (const double *)sc, (const int *)active, partial, ist(0));
-    final_dots(1, elem_wgs());
-    scalars(1);
+    if (!first) scalars(5);
 }
 
 HeatResult HeatSolver::solve(double tol, int max_iter, int mode, HeatRecord *rec) {
@@ -1123,12 +1073,15 @@ HeatResult HeatSolver::solve(double tol, int max_iter, int mode, HeatRecord *rec
     auto precond_direction = [&](bool first) {
         double *z = nullptr;
         if (learned) {
-            I.vcycle(I.r, z, I.learned_pre0, I.learned_post0, nullptr);
+            const unsigned nwg = I.vcycle(I.r, z, I.learned_pre0, I.learned_post0, first ? nullptr : I.q);
+            if (!first) {
+                I.final_dots(1, nwg);
+                I.scalars(6);                // <z, q>
+            }
             I.features(z);
-            I.learned_correction(z);        // also beta for the direction update
-            hipLaunchKernelGGL(k_hupdate_pq, I.grid_elem(), dim3(256), 0, I.stream, ni, I.p, (const double *)z, I.q,
-                               (const double *)I.Az, (const double *)I.r, (const double *)I.sc, (const int *)I.active,
-                               I.partial, ni, first ? 0 : 1);
+            I.learned_correction(first);     // step a and (unless first) beta
+            hipLaunchKernelGGL(k_hupdate_p2, g0, dim3(256), 0, I.stream, ni, I.p, (const double *)z, (const double *)I.cc,
+                               (const double *)I.sc, (const int *)I.active, ni, first ? 0 : 1);
             return;
         }
         const int pre0 = recording ? I.learned_pre0 : I.pre, post0 = recording ? I.learned_post0 : I.post;
@@ -1146,8 +1099,8 @@ HeatResult HeatSolver::solve(double tol, int max_iter, int mode, HeatRecord *rec
     };
     // one FCG iteration; the convergence test runs on the device (k_hconverge)
     auto body = [&]() {
-        if (!learned) I.apply0(I.p, I.q, I.r);           // q = A p with <p,q>, <p,r>
-        I.final_dots(2, learned ? I.elem_wgs() : I.march_wgs(0));   // learned: from the direction update
+        I.apply0(I.p, I.q, I.r);                         // q = A p with <p,q>, <p,r>
+        I.final_dots(2, I.march_wgs(0));
         I.scalars(0);
         hipLaunchKernelGGL(k_hupdate_xr, I.grid_elem(), dim3(256), 0, I.stream, ni, I.x, I.r, (const double *)I.p,
                            (const double *)I.q, (const double *)I.sc, (const int *)I.active, I.partial, ni);
