@@ -31,10 +31,10 @@ WORKLOADS = {
 
 # algorithmic bytes per interior cell per stage-kernel launch (compulsory
 # interior traffic: read the stage input, read the accumulation base, write the
-# output).  FP32 path: stages 1-3 read the FP32 stage input and the FP32 copy of
-# U^n and write FP32 (32+32+32); stage 4 reads the FP32 input and FP64 U^n and
-# writes FP64 U and its FP32 copy (32+64+64+32... with the in-place U: 32+64+64)
-STAGE_BYTES = {"fp64": [192, 192, 192, 192], "fp32": [96, 96, 96, 160]}
+# output).  FP64: 64 + 64 + 64.  FP32 path: stages 1-3 read the FP32 stage input
+# and the FP32 copy of U^n and write FP32 (32 + 32 + 32); stage 4 reads the FP32
+# input and FP64 U^n and writes FP64 U^n+1 and its FP32 copy (32 + 64 + 64 + 32)
+STAGE_BYTES = {"fp64": [192, 192, 192, 192], "fp32": [96, 96, 96, 192]}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -70,9 +70,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--learned-sweeps", default="2,2",
                     help="finest-level pre,post smoothing sweeps of the V-cycle under the learned correction")
-    ap.add_argument("--learned-pairs", type=int, default=2000, help="solver-generated residual/error snapshots")
+    ap.add_argument("--learned-pairs", type=int, default=4000, help="solver-generated residual/error snapshots")
     ap.add_argument("--learned-blocks", type=int, default=16, help="block images recorded per snapshot")
-    ap.add_argument("--learned-epochs", type=int, default=20)
+    ap.add_argument("--learned-epochs", type=int, default=60)
     ap.add_argument("--legs", default="rk,heat,learned",
                     help="comma list of legs: rk (RK4 FP64/FP32), heat (MG-PCG), learned (learned FCG vs MG-PCG)")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
@@ -267,7 +267,7 @@ def run_heat(cpu_baseline=True, repeats=3):
 LEARNED_WORKLOAD = "heat_amr_32x64x192_b16_x32"
 
 
-def run_learned(npairs=2000, blocks_per_pair=16, epochs=20, batch=128, lr=2e-3, repeats=3, sweeps=(2, 2)):
+def run_learned(npairs=4000, blocks_per_pair=16, epochs=60, batch=128, lr=2e-3, repeats=3, sweeps=(2, 2)):
     """32 concurrent backward-Euler solves on the locally refined shell
     (level-0 32x64x192 per panel in 16^3 blocks, seeded 20% of blocks to level
     1 and 5% to level 2), dt spread geometrically over 10-300 x the explicit
