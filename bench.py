@@ -73,6 +73,7 @@ def main():
     ap.add_argument("--learned-pairs", type=int, default=4000, help="solver-generated residual/error snapshots")
     ap.add_argument("--learned-blocks", type=int, default=16, help="block images recorded per snapshot")
     ap.add_argument("--learned-epochs", type=int, default=60)
+    ap.add_argument("--learned-blockstep", type=int, default=1, help="1: correction step per 16^3 block, 0: per system")
     ap.add_argument("--legs", default="rk,heat,learned",
                     help="comma list of legs: rk (RK4 FP64/FP32), heat (MG-PCG), learned (learned FCG vs MG-PCG)")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
@@ -113,7 +114,8 @@ def main():
         print(f"# heat: {heat['mg_pcg']}", file=sys.stderr, flush=True)
     if "learned" in legs:
         learned = run_learned(npairs=args.learned_pairs, blocks_per_pair=args.learned_blocks,
-                              epochs=args.learned_epochs, sweeps=tuple(int(v) for v in args.learned_sweeps.split(",")))
+                              epochs=args.learned_epochs, sweeps=tuple(int(v) for v in args.learned_sweeps.split(",")),
+                              blockstep=bool(args.learned_blockstep))
         print(f"# learned: {json.dumps(learned)}", file=sys.stderr, flush=True)
         heat = heat or {}
         heat["learned_fcg"] = learned
@@ -267,7 +269,7 @@ def run_heat(cpu_baseline=True, repeats=3):
 LEARNED_WORKLOAD = "heat_amr_32x64x192_b16_x32"
 
 
-def run_learned(npairs=4000, blocks_per_pair=16, epochs=60, batch=128, lr=2e-3, repeats=3, sweeps=(2, 2)):
+def run_learned(npairs=4000, blocks_per_pair=16, epochs=60, batch=128, lr=2e-3, repeats=3, sweeps=(2, 2), blockstep=True):
     """32 concurrent backward-Euler solves on the locally refined shell
     (level-0 32x64x192 per panel in 16^3 blocks, seeded 20% of blocks to level
     1 and 5% to level 2), dt spread geometrically over 10-300 x the explicit
@@ -280,6 +282,7 @@ def run_learned(npairs=4000, blocks_per_pair=16, epochs=60, batch=128, lr=2e-3, 
     s = Shell(lay)
     hs = HeatSolver(s, 1.0, 0.7, nsys)
     hs.set_learned_sweeps(*sweeps)
+    hs.set_learned_blockstep(blockstep)
     loss, pairs, train_s = hs.train_corrector(npairs=npairs, blocks_per_pair=blocks_per_pair, dt_range=(10.0, 300.0),
                                               max_solves=20, epochs=epochs, batch=batch, lr=lr, seed=12345)
     dts = hs.dt_explicit * 10.0 * 30.0 ** (np.arange(nsys) / (nsys - 1))
@@ -288,7 +291,7 @@ def run_learned(npairs=4000, blocks_per_pair=16, epochs=60, batch=128, lr=2e-3, 
     hs.set_T(T)
     out = {"workload": LEARNED_WORKLOAD, "blocks": s.nblocks, "systems": nsys, "cells_per_system": hs.cells,
            "tol": 1e-10, "dt_over_explicit": [10.0, 300.0],
-           "learned_finest_sweeps": list(sweeps),
+           "learned_finest_sweeps": list(sweeps), "learned_blockstep": bool(blockstep),
            "training": {"pairs": pairs, "blocks_per_pair": blocks_per_pair, "epochs": epochs, "batch": batch, "lr": lr,
                         "final_mse": loss, "seconds": train_s}}
     for name, learned in (("mg_pcg", False), ("learned_fcg", True)):

@@ -83,6 +83,7 @@ def load():
         "falcon_heat_attach_corrector": [vp],
         "falcon_heat_set_learned_sweeps": [vp, C.c_int, C.c_int],
         "falcon_heat_set_graph": [vp, C.c_int],
+        "falcon_heat_set_learned_blockstep": [vp, C.c_int],
         "falcon_last_error": [C.c_char_p, C.c_int],
     }
     for name, args in sig.items():
@@ -339,6 +340,10 @@ class HeatSolver:
     def set_graph(self, on):
         """Replay each FCG iteration as a captured hipGraph (default) or launch eagerly."""
         _check(self.L.falcon_heat_set_graph(self.s.h, 1 if on else 0))
+
+    def set_learned_blockstep(self, on):
+        """Learned correction step per 16^3 block (on, default) or per system (off)."""
+        _check(self.L.falcon_heat_set_learned_blockstep(self.s.h, 1 if on else 0))
 
     def set_learned_sweeps(self, pre0, post0):
         """Finest-level smoothing sweeps of the V-cycle used with the learned correction."""

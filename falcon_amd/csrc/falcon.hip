@@ -1206,6 +1206,10 @@ int falcon_heat_set_graph(falcon_handle_t h, int on) {
     return guard([&] { need_heat(h).set_graph(on != 0); });
 }
 
+int falcon_heat_set_learned_blockstep(falcon_handle_t h, int on) {
+    return guard([&] { need_heat(h).set_learned_blockstep(on != 0); });
+}
+
 int falcon_heat_set_learned_sweeps(falcon_handle_t h, int pre0, int post0) {
     return guard([&] { need_heat(h).set_learned_sweeps(pre0, post0); });
 }

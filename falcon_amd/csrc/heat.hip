@@ -496,6 +496,7 @@ constexpr int kSc = 8;
 // what 4: <b,b> (dots: [0])
 // what 5: beta of the learned direction from [6], [4] and dots[2] = <c,q>
 // what 6: [6] = <z,q> (dots: [0])
+// what 7: beta of the learned direction with per-block steps (dots: [0] = sum_b a_b <c_b,q_b>)
 __global__ void k_hscalars(int what, int nsys, const double *__restrict__ dots, double *__restrict__ sc,
                            const int *__restrict__ active, double ncell) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -514,6 +515,9 @@ __global__ void k_hscalars(int what, int nsys, const double *__restrict__ dots, 
         S[3] = active[s] ? sqrt(dots[3 * s] / ncell) : 0.0;
     } else if (what == 4) {
         S[5] = dots[3 * s];
+    } else if (what == 7) {
+        // beta of the learned direction with per-block steps: (<z,q> + sum_b a_b <c_b,q_b>)/gamma
+        S[1] = active[s] ? (S[6] + dots[3 * s]) / S[2] : 0.0;
     } else if (what == 5) {
         // beta of the learned direction: <z + a c, q>/gamma = (<z,q> + a <c,q>)/gamma
         S[1] = active[s] ? (S[6] + S[4] * dots[3 * s + 2]) / S[2] : 0.0;
@@ -612,14 +616,34 @@ __global__ void __launch_bounds__(256) k_hcopy(const double *__restrict__ a, dou
 // beta = 0 starts a fresh direction
 __global__ void __launch_bounds__(256) k_hupdate_p2(long long n, double *__restrict__ p, const double *__restrict__ z,
                                                     const double *__restrict__ cc, const double *__restrict__ sc,
-                                                    const int *__restrict__ active, long long istride, int use_beta) {
+                                                    const double *__restrict__ ablk, int nb, const int *__restrict__ active,
+                                                    long long istride, int use_beta) {
     const int s = blockIdx.y;
     if (!active[s]) return;
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
-    const double a = sc[kSc * s + 4], be = use_beta ? sc[kSc * s + 1] : 0.0;
+    const double a = ablk ? ablk[(long long)s * nb + (c >> 12)] : sc[kSc * s + 4];
+    const double be = use_beta ? sc[kSc * s + 1] : 0.0;
     const long long io = s * istride + c;
     p[io] = (z[io] + a * cc[io]) - be * p[io];
+}
+
+// Per-block correction steps (16^3 blocks, one block per workgroup of the
+// correction's dot pass): a_b = <c_b,r'_b>/<c_b,(Ac)_b> (0 unless the
+// denominator is positive), and in partial slot 0 the block's a_b <c_b,q_b>
+__global__ void __launch_bounds__(256) k_hblockstep(int nwg, int nb, double *__restrict__ partial,
+                                                    double *__restrict__ ablk, const int *__restrict__ active) {
+    const int s = blockIdx.y;
+    if (!active[s]) return;
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwg) return;
+    const int b = xcd_group(w, nwg);
+    double *P = partial + (long long)s * 3 * nwg;
+    const double num = P[w], den = P[nwg + w], cq = P[2 * nwg + w];
+    const double a = num / den;
+    const double ab = (den > 0.0 && a == a && a < 1e300 && a > -1e300) ? a : 0.0;
+    ablk[(long long)s * nb + b] = ab;
+    P[w] = ab * cq;
 }
 
 // training records: copy the network input image and w = x + z of record slot
@@ -671,6 +695,8 @@ struct HeatSolver::Impl {
     double *rp = nullptr, *cc = nullptr;
     float *xin = nullptr;
     int learned_pre0 = 2, learned_post0 = 2;
+    bool blockstep = true;           // per-block correction steps
+    double *ablk = nullptr;
     // captured FCG iteration per mode (plain, learned) and device-side convergence state
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     bool use_graph = true;
@@ -920,6 +946,10 @@ void HeatSolver::set_corrector(HeatCorrector *c) {
     d->drop_graphs();
 }
 void HeatSolver::set_graph(bool on) { d->use_graph = on; }
+void HeatSolver::set_learned_blockstep(bool on) {
+    d->blockstep = on;
+    d->drop_graphs();
+}
 void HeatSolver::set_learned_sweeps(int pre0, int post0) {
     if (pre0 < 0 || post0 < 0 || pre0 > 8 || post0 > 8) throw std::runtime_error("sweeps must be in [0, 8]");
     d->learned_pre0 = pre0;
@@ -981,6 +1011,7 @@ void HeatSolver::Impl::ensure_learned() {
     if (rp) return;
     rp = dalloc(ist(0) * nsys);
     cc = dalloc(ist(0) * nsys);
+    ablk = dalloc((size_t)nsys * nb);
     void *a = nullptr;
     HCHECK(hipMalloc(&a, sizeof(float) * ist(0) * nsys));
     allocs.push_back(a);
@@ -1000,6 +1031,16 @@ void HeatSolver::Impl::features(double *z) {
 void HeatSolver::Impl::learned_correction(bool first) {
     corr->infer_heat(xin, sc + 3, kSc, nb, ist(0), cc, (long long)nsys * nb, active, stream);
     launch_march<false>(0, cc, nullptr, 0, OpApplyDot{rp, q}, partial);
+    if (blockstep) {
+        const unsigned nwg = march_wgs(0);
+        hipLaunchKernelGGL(k_hblockstep, dim3((nwg + 255) / 256, (unsigned)nsys), dim3(256), 0, stream, (int)nwg, nb,
+                           partial, ablk, (const int *)active);
+        if (!first) {
+            final_dots(1, nwg);
+            scalars(7);
+        }
+        return;
+    }
     final_dots(3, march_wgs(0));
     scalars(2);
     if (!first) scalars(5);
@@ -1081,7 +1122,8 @@ HeatResult HeatSolver::solve(double tol, int max_iter, int mode, HeatRecord *rec
             I.features(z);
             I.learned_correction(first);     // step a and (unless first) beta
             hipLaunchKernelGGL(k_hupdate_p2, g0, dim3(256), 0, I.stream, ni, I.p, (const double *)z, (const double *)I.cc,
-                               (const double *)I.sc, (const int *)I.active, ni, first ? 0 : 1);
+                               (const double *)I.sc, (const double *)(I.blockstep ? I.ablk : nullptr), I.nb,
+                               (const int *)I.active, ni, first ? 0 : 1);
             return;
         }
         const int pre0 = recording ? I.learned_pre0 : I.pre, post0 = recording ? I.learned_post0 : I.post;

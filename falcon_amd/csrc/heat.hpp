@@ -77,6 +77,8 @@ class HeatSolver {
     // plain and learned solves replay one captured FCG iteration (default) or
     // launch the same kernels eagerly
     void set_graph(bool on);
+    // learned correction step per 16^3 block (a_b, default) or one per system
+    void set_learned_blockstep(bool on);
 
   private:
     struct Impl;

@@ -135,6 +135,9 @@ int falcon_heat_get_T(falcon_handle_t h, int sys, double *T);
 /* on (default): each FCG iteration is one replay of a captured hipGraph with the
  * convergence test on the device; off: the same kernels launched one by one */
 int falcon_heat_set_graph(falcon_handle_t h, int on);
+/* learned correction step: one per 16^3 block (on, default) or one per system
+ * (off).  No counterpart in the reference (learned V-cycle correction). */
+int falcon_heat_set_learned_blockstep(falcon_handle_t h, int on);
 /* y = A x for all systems (parity tests) */
 int falcon_heat_apply(falcon_handle_t h, const double *x, double *y);
 /* z = M r: one V-cycle of the preconditioner for all systems, zero initial

@@ -86,10 +86,11 @@ def test_training_fits_a_teacher():
 
 # ---------------------------------------------------------------- learned FCG
 
-def _fcg_learned_restated(H, cnn_oracle, params, dt, Tn, tol, maxit):
+def _fcg_learned_restated(H, cnn_oracle, params, dt, Tn, tol, maxit, blockstep=True):
     """FCG(1) with the learned correction, restated from the oracle's operator
     (apply), V-cycle and network: z = M r, r' = r - A z, u = r'/diag, s = rms(u),
-    c = s net(u/s), z += a c with a = <c,r'>/<c,Ac>."""
+    c = s net(u/s), z += a c with a = <c,r'>/<c,Ac> per block (blockstep) or
+    per system, 0 unless <c,Ac> > 0."""
     b = (H.apply(1.0, np.ones_like(Tn)) / dt) * Tn
     diag = H.diag(dt)
     x = np.zeros_like(b); r = b.copy(); bn = np.linalg.norm(b)
@@ -103,8 +104,13 @@ def _fcg_learned_restated(H, cnn_oracle, params, dt, Tn, tol, maxit):
         y = cnn_oracle.forward(params, img, bf16=True)[0].transpose(0, 3, 2, 1)
         cc = sc * y
         Ac = H.apply(dt, cc)
-        a = np.vdot(cc, rp) / np.vdot(cc, Ac)
-        return z + a * cc
+        nb = cc.shape[0]
+        num = (cc * rp).reshape(nb, -1).sum(axis=1)
+        den = (cc * Ac).reshape(nb, -1).sum(axis=1)
+        if not blockstep:
+            num, den = np.full(nb, num.sum()), np.full(nb, den.sum())
+        a = np.where(den > 0, num / np.where(den > 0, den, 1.0), 0.0)
+        return z + a[:, None, None, None] * cc
 
     z = prec(r); p = z.copy(); it = 0
     while True:
@@ -116,9 +122,11 @@ def _fcg_learned_restated(H, cnn_oracle, params, dt, Tn, tol, maxit):
         z = prec(r); be = np.vdot(z, q) / g; p = z - be * p
 
 
-def test_learned_fcg_matches_oracle_restatement():
-    """Any network gives a convergent FCG through the step a; the GPU path and
-    the restatement agree in iterations (within one) and solution."""
+@pytest.mark.parametrize("blockstep", [True, False])
+def test_learned_fcg_matches_oracle_restatement(blockstep):
+    """Any network gives a convergent FCG through the step a (per block or per
+    system); the GPU path and the restatement agree in iterations (within one)
+    and solution."""
     lay = configs.cubic((16, 32, 96), 16)
     o = pyoracle.Oracle(lay)
     H = pyoracle.HeatOracle(o, 1.0, 0.7)
@@ -128,6 +136,7 @@ def test_learned_fcg_matches_oracle_restatement():
     p = _params(8, scale=0.5)
     c.params = p
     hs.attach_corrector()
+    hs.set_learned_blockstep(blockstep)
     dt = 100.0 * hs.dt_explicit
     T = 1e-3 * np.random.default_rng(9).standard_normal(hs.shape)
     hs.set_dt(dt)
@@ -135,7 +144,7 @@ def test_learned_fcg_matches_oracle_restatement():
     print("[learned fcg vs restatement] 24 blocks", flush=True)
     it, rel, _ = hs.solve(1e-10, 100, use_corrector=True)
     x = hs.get_T(0)
-    xo, ito, relo = _fcg_learned_restated(H, pyoracle.CnnOracle(), p, dt, T[0], 1e-10, 100)
+    xo, ito, relo = _fcg_learned_restated(H, pyoracle.CnnOracle(), p, dt, T[0], 1e-10, 100, blockstep)
     print("gpu", int(it[0]), float(rel[0]), "restated", ito, relo, flush=True)
     assert rel[0] <= 1e-10 and abs(int(it[0]) - ito) <= 1
     assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-8
