@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--learned-blocks", type=int, default=16, help="block images recorded per snapshot")
     ap.add_argument("--learned-epochs", type=int, default=60)
     ap.add_argument("--learned-blockstep", type=int, default=1, help="1: correction step per 16^3 block, 0: per system")
+    ap.add_argument("--learned-iters", type=int, default=5,
+                    help="directions built with the learned correction before the plain V-cycle takes over (0: all)")
     ap.add_argument("--legs", default="rk,heat,learned",
                     help="comma list of legs: rk (RK4 FP64/FP32), heat (MG-PCG), learned (learned FCG vs MG-PCG)")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
@@ -115,7 +117,7 @@ def main():
     if "learned" in legs:
         learned = run_learned(npairs=args.learned_pairs, blocks_per_pair=args.learned_blocks,
                               epochs=args.learned_epochs, sweeps=tuple(int(v) for v in args.learned_sweeps.split(",")),
-                              blockstep=bool(args.learned_blockstep))
+                              blockstep=bool(args.learned_blockstep), learned_iters=args.learned_iters)
         print(f"# learned: {json.dumps(learned)}", file=sys.stderr, flush=True)
         heat = heat or {}
         heat["learned_fcg"] = learned
@@ -269,20 +271,24 @@ def run_heat(cpu_baseline=True, repeats=3):
 LEARNED_WORKLOAD = "heat_amr_32x64x192_b16_x32"
 
 
-def run_learned(npairs=4000, blocks_per_pair=16, epochs=60, batch=128, lr=2e-3, repeats=3, sweeps=(2, 2), blockstep=True):
+def run_learned(npairs=4000, blocks_per_pair=16, epochs=60, batch=128, lr=2e-3, repeats=3, sweeps=(2, 2), blockstep=True,
+                learned_iters=5):
     """32 concurrent backward-Euler solves on the locally refined shell
     (level-0 32x64x192 per panel in 16^3 blocks, seeded 20% of blocks to level
     1 and 5% to level 2), dt spread geometrically over 10-300 x the explicit
     limit, seeded Gaussian T' (sigma 1e-3), to 1e-10: plain MG-FCG against
     FCG with the learned correction, trained first on `npairs` block-image
     pairs generated by the solver's own systems (training time reported
-    separately, not in the solve time)."""
+    separately, not in the solve time).  The learned correction builds the
+    first `learned_iters` search directions, the plain V-cycle the rest
+    (0: all)."""
     lay = configs.SHELL_AMR_32x64x192_B16
     nsys = 32
     s = Shell(lay)
     hs = HeatSolver(s, 1.0, 0.7, nsys)
     hs.set_learned_sweeps(*sweeps)
     hs.set_learned_blockstep(blockstep)
+    hs.set_learned_iters(learned_iters)
     loss, pairs, train_s = hs.train_corrector(npairs=npairs, blocks_per_pair=blocks_per_pair, dt_range=(10.0, 300.0),
                                               max_solves=20, epochs=epochs, batch=batch, lr=lr, seed=12345)
     dts = hs.dt_explicit * 10.0 * 30.0 ** (np.arange(nsys) / (nsys - 1))
@@ -292,6 +298,7 @@ def run_learned(npairs=4000, blocks_per_pair=16, epochs=60, batch=128, lr=2e-3, 
     out = {"workload": LEARNED_WORKLOAD, "blocks": s.nblocks, "systems": nsys, "cells_per_system": hs.cells,
            "tol": 1e-10, "dt_over_explicit": [10.0, 300.0],
            "learned_finest_sweeps": list(sweeps), "learned_blockstep": bool(blockstep),
+           "learned_iters": int(learned_iters),
            "training": {"pairs": pairs, "blocks_per_pair": blocks_per_pair, "epochs": epochs, "batch": batch, "lr": lr,
                         "final_mse": loss, "seconds": train_s}}
     for name, learned in (("mg_pcg", False), ("learned_fcg", True)):

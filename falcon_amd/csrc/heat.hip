@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -699,7 +700,9 @@ struct HeatSolver::Impl {
     double *ablk = nullptr;
     // captured FCG iteration per mode (plain, learned) and device-side convergence state
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    double graph_tol[2] = {0.0, 0.0};   // tolerance baked into each captured k_hconverge
     bool use_graph = true;
+    int learned_iters = 0;              // directions built with the learned correction (0: all)
     int *iters_dev = nullptr, *nactive_dev = nullptr;
     double *relres_dev = nullptr;
     void drop_graphs() {
@@ -950,6 +953,10 @@ void HeatSolver::set_learned_blockstep(bool on) {
     d->blockstep = on;
     d->drop_graphs();
 }
+void HeatSolver::set_learned_iters(int n) {
+    if (n < 0) throw std::runtime_error("the number of learned directions must be >= 0");
+    d->learned_iters = n;
+}
 void HeatSolver::set_learned_sweeps(int pre0, int post0) {
     if (pre0 < 0 || post0 < 0 || pre0 > 8 || post0 > 8) throw std::runtime_error("sweeps must be in [0, 8]");
     d->learned_pre0 = pre0;
@@ -1110,10 +1117,13 @@ HeatResult HeatSolver::solve(double tol, int max_iter, int mode, HeatRecord *rec
         HCHECK(hipStreamSynchronize(I.stream));   // rs/rb host vectors leave scope
         rec->count += R;
     };
-    // preconditioner and direction update; first: the initial direction p = z
-    auto precond_direction = [&](bool first) {
+    // preconditioner and direction update; first: the initial direction p = z;
+    // lrn: learned correction on the V-cycle (else the plain V-cycle).  Both
+    // orthogonalise against the previous direction with <z', q>/<p, q>, so a
+    // solve can switch from one to the other between iterations.
+    auto precond_direction = [&](bool first, bool lrn) {
         double *z = nullptr;
-        if (learned) {
+        if (lrn) {
             const unsigned nwg = I.vcycle(I.r, z, I.learned_pre0, I.learned_post0, first ? nullptr : I.q);
             if (!first) {
                 I.final_dots(1, nwg);
@@ -1140,7 +1150,7 @@ HeatResult HeatSolver::solve(double tol, int max_iter, int mode, HeatRecord *rec
                            (const int *)I.active, ni, first ? 0 : 1);
     };
     // one FCG iteration; the convergence test runs on the device (k_hconverge)
-    auto body = [&]() {
+    auto body = [&](bool lrn) {
         I.apply0(I.p, I.q, I.r);                         // q = A p with <p,q>, <p,r>
         I.final_dots(2, I.march_wgs(0));
         I.scalars(0);
@@ -1149,29 +1159,42 @@ HeatResult HeatSolver::solve(double tol, int max_iter, int mode, HeatRecord *rec
         I.final_dots(1, I.elem_wgs());
         hipLaunchKernelGGL(k_hconverge, dim3(1), dim3(256), 0, I.stream, ns, (const double *)I.dots, (const double *)I.sc,
                            tol, I.active, I.iters_dev, I.relres_dev, I.nactive_dev);
-        precond_direction(false);
+        precond_direction(false, lrn);
     };
-
+    // directions 0 .. nlearn-1 come from the learned correction, later ones
+    // from the plain V-cycle
+    const int nlearn = learned ? (I.learned_iters > 0 ? I.learned_iters : INT_MAX) : 0;
     if (nact > 0) {
         if (learned) I.corr->prepare(I.stream);
-        precond_direction(true);
+        precond_direction(true, nlearn > 0);
     }
     const bool graph = I.use_graph && !recording;
-    hipGraphExec_t &G = I.graph[learned ? 1 : 0];
-    if (graph && nact > 0 && !G) {
-        // One FCG iteration captured as a hipGraph and replayed: the host reads
-        // one counter per iteration.  Arithmetic equals the eager launches'.
-        hipGraph_t g;
-        HCHECK(hipStreamBeginCapture(I.stream, hipStreamCaptureModeThreadLocal));
-        body();
-        HCHECK(hipStreamEndCapture(I.stream, &g));
-        HCHECK(hipGraphInstantiate(&G, g, nullptr, nullptr, 0));
-        HCHECK(hipGraphDestroy(g));
-    }
+    // one FCG iteration per kind of direction captured as a hipGraph and
+    // replayed: the host reads one counter per iteration.  Arithmetic equals
+    // the eager launches'.
+    auto graph_of = [&](bool lrn) -> hipGraphExec_t {
+        const int gi = lrn ? 1 : 0;
+        hipGraphExec_t &G = I.graph[gi];
+        if (G && I.graph_tol[gi] != tol) {
+            HCHECK(hipGraphExecDestroy(G));
+            G = nullptr;
+        }
+        if (!G) {
+            hipGraph_t g;
+            HCHECK(hipStreamBeginCapture(I.stream, hipStreamCaptureModeThreadLocal));
+            body(lrn);
+            HCHECK(hipStreamEndCapture(I.stream, &g));
+            HCHECK(hipGraphInstantiate(&G, g, nullptr, nullptr, 0));
+            HCHECK(hipGraphDestroy(g));
+            I.graph_tol[gi] = tol;
+        }
+        return G;
+    };
     int launched = 0;
     while (nact > 0 && launched < max_iter) {
-        if (graph) HCHECK(hipGraphLaunch(G, I.stream));
-        else body();
+        const bool lrn = launched + 1 < nlearn;          // kind of the direction this iteration builds
+        if (graph) HCHECK(hipGraphLaunch(graph_of(lrn), I.stream));
+        else body(lrn);
         launched++;
         HCHECK(hipMemcpyAsync(&nact, I.nactive_dev, sizeof(int), hipMemcpyDeviceToHost, I.stream));
         HCHECK(hipStreamSynchronize(I.stream));

@@ -79,6 +79,9 @@ class HeatSolver {
     void set_graph(bool on);
     // learned correction step per 16^3 block (a_b, default) or one per system
     void set_learned_blockstep(bool on);
+    // the learned correction builds the first n directions of a solve, the
+    // plain V-cycle the rest (0: every direction)
+    void set_learned_iters(int n);
 
   private:
     struct Impl;

@@ -86,17 +86,20 @@ def test_training_fits_a_teacher():
 
 # ---------------------------------------------------------------- learned FCG
 
-def _fcg_learned_restated(H, cnn_oracle, params, dt, Tn, tol, maxit, blockstep=True):
+def _fcg_learned_restated(H, cnn_oracle, params, dt, Tn, tol, maxit, blockstep=True, nlearn=0):
     """FCG(1) with the learned correction, restated from the oracle's operator
     (apply), V-cycle and network: z = M r, r' = r - A z, u = r'/diag, s = rms(u),
     c = s net(u/s), z += a c with a = <c,r'>/<c,Ac> per block (blockstep) or
-    per system, 0 unless <c,Ac> > 0."""
+    per system, 0 unless <c,Ac> > 0.  With nlearn > 0 only directions
+    0 .. nlearn-1 take the correction, later ones are the plain z = M r."""
     b = (H.apply(1.0, np.ones_like(Tn)) / dt) * Tn
     diag = H.diag(dt)
     x = np.zeros_like(b); r = b.copy(); bn = np.linalg.norm(b)
 
-    def prec(r):
+    def prec(r, k):
         z = H.vcycle(dt, r)
+        if nlearn and k >= nlearn:
+            return z
         rp = r - H.apply(dt, z)
         u = rp / diag
         sc = np.sqrt(np.mean(u * u))
@@ -112,21 +115,22 @@ def _fcg_learned_restated(H, cnn_oracle, params, dt, Tn, tol, maxit, blockstep=T
         a = np.where(den > 0, num / np.where(den > 0, den, 1.0), 0.0)
         return z + a[:, None, None, None] * cc
 
-    z = prec(r); p = z.copy(); it = 0
+    z = prec(r, 0); p = z.copy(); it = 0
     while True:
         q = H.apply(dt, p); g = np.vdot(p, q); al = np.vdot(p, r) / g
         x += al * p; r -= al * q; it += 1
         rel = np.linalg.norm(r) / bn
         if rel <= tol or it >= maxit:
             return x, it, rel
-        z = prec(r); be = np.vdot(z, q) / g; p = z - be * p
+        z = prec(r, it); be = np.vdot(z, q) / g; p = z - be * p
 
 
-@pytest.mark.parametrize("blockstep", [True, False])
-def test_learned_fcg_matches_oracle_restatement(blockstep):
+@pytest.mark.parametrize("blockstep,nlearn", [(True, 0), (False, 0), (True, 3)])
+def test_learned_fcg_matches_oracle_restatement(blockstep, nlearn):
     """Any network gives a convergent FCG through the step a (per block or per
     system); the GPU path and the restatement agree in iterations (within one)
-    and solution."""
+    and solution, also when the plain V-cycle takes over after nlearn
+    directions."""
     lay = configs.cubic((16, 32, 96), 16)
     o = pyoracle.Oracle(lay)
     H = pyoracle.HeatOracle(o, 1.0, 0.7)
@@ -137,14 +141,15 @@ def test_learned_fcg_matches_oracle_restatement(blockstep):
     c.params = p
     hs.attach_corrector()
     hs.set_learned_blockstep(blockstep)
+    hs.set_learned_iters(nlearn)
     dt = 100.0 * hs.dt_explicit
     T = 1e-3 * np.random.default_rng(9).standard_normal(hs.shape)
     hs.set_dt(dt)
     hs.set_T(T)
-    print("[learned fcg vs restatement] 24 blocks", flush=True)
+    print(f"[learned fcg vs restatement] 24 blocks blockstep={blockstep} nlearn={nlearn}", flush=True)
     it, rel, _ = hs.solve(1e-10, 100, use_corrector=True)
     x = hs.get_T(0)
-    xo, ito, relo = _fcg_learned_restated(H, pyoracle.CnnOracle(), p, dt, T[0], 1e-10, 100, blockstep)
+    xo, ito, relo = _fcg_learned_restated(H, pyoracle.CnnOracle(), p, dt, T[0], 1e-10, 100, blockstep, nlearn)
     print("gpu", int(it[0]), float(rel[0]), "restated", ito, relo, flush=True)
     assert rel[0] <= 1e-10 and abs(int(it[0]) - ito) <= 1
     assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-8

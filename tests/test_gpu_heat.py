@@ -154,3 +154,24 @@ def test_graph_replays_equal_eager_solves():
         assert np.array_equal(it_e, it_g) and np.array_equal(rel_e, rel_g)
         for k in range(2):
             assert np.array_equal(hs.get_T(k), x_e[k])
+
+
+def test_graph_follows_a_tolerance_change():
+    """The convergence test inside the captured iteration carries the solve's
+    tolerance: a solve at 1e-6 after one at 1e-10 on the same handle stops at
+    1e-6 and equals the eagerly launched solve at 1e-6."""
+    name = "refined8"
+    H = _oracle(name)
+    s, hs = _gpu(name, 2)
+    dt = np.array([30.0, 300.0]) * H.dt_explicit
+    T = 1e-3 * np.random.default_rng(17).standard_normal(hs.shape)
+    hs.set_dt(dt)
+    hs.set_T(T)
+    hs.set_graph(True)
+    it10, _, _ = hs.solve(1e-10, 200)
+    it6_g, rel6_g, _ = hs.solve(1e-6, 200)
+    hs.set_graph(False)
+    it6_e, rel6_e, _ = hs.solve(1e-6, 200)
+    print("tol change", it10.tolist(), it6_g.tolist(), it6_e.tolist(), flush=True)
+    assert np.array_equal(it6_g, it6_e) and np.array_equal(rel6_g, rel6_e)
+    assert (rel6_g <= 1e-6).all() and (it6_g < it10).all()
