@@ -54,6 +54,7 @@ def load():
         "falcon_get_state": [vp, C.c_int, dp],
         "falcon_init_synthetic": [vp, C.c_uint64, C.c_double, C.c_double],
         "falcon_set_precision": [vp, C.c_int],
+        "falcon_set_ghost_pull": [vp, C.c_int],
         "falcon_sendrecv_gc": [vp, C.c_int],
         "falcon_advance_rk4": [vp, C.c_double, dp],
         "falcon_advance": [vp, C.c_int, C.c_double, C.c_int, dp],
@@ -84,6 +85,7 @@ def load():
         "falcon_heat_set_learned_sweeps": [vp, C.c_int, C.c_int],
         "falcon_heat_set_graph": [vp, C.c_int],
         "falcon_heat_set_learned_blockstep": [vp, C.c_int],
+        "falcon_heat_set_learned_iters": [vp, C.c_int],
         "falcon_last_error": [C.c_char_p, C.c_int],
     }
     for name, args in sig.items():
@@ -224,6 +226,12 @@ class Shell:
     def set_precision(self, mode):
         _check(self.L.falcon_set_precision(self.h, {"fp64": 0, "fp32": 1}.get(mode, mode)))
 
+    def set_ghost_pull(self, on):
+        """FP32 step: stage/diffusion kernels read same-level ghost copies from
+        the neighbours' interiors (on, default) or the ghost fill writes every
+        ghost (off)."""
+        _check(self.L.falcon_set_ghost_pull(self.h, 1 if on else 0))
+
     def sendrecv_gc(self, which=0):
         _check(self.L.falcon_sendrecv_gc(self.h, which))
 
@@ -344,6 +352,11 @@ class HeatSolver:
     def set_learned_blockstep(self, on):
         """Learned correction step per 16^3 block (on, default) or per system (off)."""
         _check(self.L.falcon_heat_set_learned_blockstep(self.s.h, 1 if on else 0))
+
+    def set_learned_iters(self, n):
+        """Build the first n search directions of a learned solve with the learned
+        correction and the rest with the plain V-cycle (0: all of them)."""
+        _check(self.L.falcon_heat_set_learned_iters(self.s.h, int(n)))
 
     def set_learned_sweeps(self, pre0, post0):
         """Finest-level smoothing sweeps of the V-cycle used with the learned correction."""

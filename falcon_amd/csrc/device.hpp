@@ -44,6 +44,15 @@ struct DevGhosts {
     const int2 *lerp;       // (dst, src) of the entries interpolated along r only, sorted by dst
     const double *lerp_w;   // their r weight
     int n_lerp;
+    // copies whose ghosts the consuming kernels read in place (see pull below),
+    // sorted by dst
+    const int2 *copy_rest;
+    int n_copy_rest;
+    // pull[R*27 + (di+1)*9 + (dj+1)*3 + (dk+1)]: the same-level block D whose
+    // interior holds all ghosts of block R in direction (di,dj,dk) as plain
+    // copies, ghost index g <- D's cell g - (di*ni, dj*nj, dk*nk); -1 where any
+    // ghost of that region is filled otherwise.  null: no pull.
+    const int32_t *pull;
 };
 
 struct DevBoundary {
@@ -52,16 +61,22 @@ struct DevBoundary {
 };
 
 // launchers (kernels.hip)
+// pull (null: none): the pull table of DevGhosts; ghosts of `in` in the
+// directions it lists are read from the neighbour's interior instead
 template <typename T, typename TIN, typename TOUT, typename TB>
 void launch_stage(const Dims &d, const DevTables &t, const TIN *in, const TB *U, TOUT *out, float *shadow,
-                  const double *dtp, double div, hipStream_t s);
+                  const double *dtp, double div, hipStream_t s, const int32_t *pull = nullptr);
 template <typename T, typename TIN>
 void launch_diffusion(const Dims &d, const DevTables &t, const TIN *in, const double *U, double *out, float *shadow,
-                      double *partial, const double *dtp, hipStream_t s);
+                      double *partial, const double *dtp, hipStream_t s, const int32_t *pull = nullptr);
+// every ghost entry of g
 template <typename TS>
 void launch_exchange(const Dims &d, const DevGhosts &g, TS *state, hipStream_t s);
+// the entries that kernels reading with pull = g.pull still need
 template <typename TS>
-void launch_boundary(const Dims &d, const DevBoundary &b, TS *state, hipStream_t s);
+void launch_exchange_rest(const Dims &d, const DevGhosts &g, TS *state, hipStream_t s);
+template <typename TS>
+void launch_boundary(const Dims &d, const DevBoundary &b, TS *state, hipStream_t s, const int32_t *pull = nullptr);
 template <typename TS>
 void launch_copy_radial(const Dims &d, const DevBoundary &b, const TS *src, TS *dst, hipStream_t s);
 void launch_dt_partial(const Dims &d, const DevTables &t, const double *U, double *partial, hipStream_t s);

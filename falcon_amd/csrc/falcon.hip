@@ -90,6 +90,8 @@ struct falcon_handle {
     DevBoundary bnd{};
     // device memory
     DevBuf meta, radial, angular, ghost64[4], ghost32[4], ghost_copy32, ghost_lerp32[2], bnd_list;   // ghost tables
+    DevBuf ghost_rest32, ghost_pull32;   // copies not read in place, pull table (FP32 path)
+    bool ghost_pull = true;              // FP32 step: kernels read same-level ghost copies in place
 
     DevBuf U[2], F32[2], S64[2], S32[2], dtp, partial, rbuf;
     int cur = 0;             // index of the current state buffer in U[]
@@ -111,7 +113,7 @@ struct falcon_handle {
         if (graph) (void)hipGraphExecDestroy(graph);
         DevBuf *all[] = {&meta, &radial, &angular, &ghost64[0], &ghost64[1], &ghost64[2], &ghost64[3], &ghost32[0],
                          &ghost32[1], &ghost32[2], &ghost32[3], &ghost_copy32, &ghost_lerp32[0], &ghost_lerp32[1],
-                         &bnd_list,
+                         &bnd_list, &ghost_rest32, &ghost_pull32,
 
                          &U[0], &U[1], &F32[0], &F32[1], &S64[0], &S64[1], &S32[0], &S32[1], &dtp, &partial, &rbuf};
         for (DevBuf *b : all) b->free();
@@ -301,6 +303,76 @@ void upload_tables(falcon_handle *h) {
     h->gh32.lerp_w = h->ghost_lerp32[1].as<double>();
     h->gh32.n_lerp = (int)lerp_idx.size();
 
+    // Pull table of the FP32 step.  A copy entry whose ghost (padded index g)
+    // and donor cell (padded index p, interior) differ by 0 or +-n per axis is
+    // a same-level translation; seen from the donor D the receiver lies in
+    // direction dir = -(g - p)/n.  (D, dir) qualifies when all ghosts of the
+    // image of D's band for dir (ng wide along every axis with dir != 0, n along
+    // the others) are such entries with one receiving block R: then R reads
+    // its ghosts in direction -dir from D (pull[R][-dir] = D) and those entries
+    // leave the copy table the step runs.
+    {
+        const int n3[3] = {h->lay.tree.ni, h->lay.tree.nj, h->lay.tree.nk};
+        const bool fits = n3[0] >= 2 * ng && n3[1] >= 2 * ng && n3[2] >= 2 * ng;
+        std::vector<int32_t> recv((size_t)nb * 27, -1), cnt((size_t)nb * 27, 0);
+        std::vector<char> bad((size_t)nb * 27, 0);
+        auto decode = [&](long long off, int &blk, int p[3]) {
+            blk = (int)(off / (8 * P));
+            const long long c = off - (long long)blk * 8 * P;
+            p[0] = (int)(c / SI);
+            p[1] = (int)((c / SJ) % NJ);
+            p[2] = (int)(c % SJ);
+        };
+        // donor-side direction key of a translation entry, or -1
+        auto key_of = [&](const int2 &e, int &R) -> long long {
+            int D, pg[3], pd[3], dir[3];
+            decode(e.x, R, pg);
+            decode(e.y, D, pd);
+            for (int a = 0; a < 3; a++) {
+                if (pd[a] < ng || pd[a] >= ng + n3[a]) return -1;
+                const int o = pg[a] - pd[a];
+                if (o == 0) dir[a] = 0;
+                else if (o == n3[a]) dir[a] = -1;
+                else if (o == -n3[a]) dir[a] = 1;
+                else return -1;
+            }
+            if (!dir[0] && !dir[1] && !dir[2]) return -1;
+            return (long long)D * 27 + (dir[0] + 1) * 9 + (dir[1] + 1) * 3 + (dir[2] + 1);
+        };
+        if (fits) {
+            for (const int2 &e : copy32) {
+                int R;
+                const long long k = key_of(e, R);
+                if (k < 0) continue;
+                if (recv[k] == -1) recv[k] = R;
+                else if (recv[k] != R) bad[k] = 1;
+                cnt[k]++;
+            }
+        }
+        std::vector<int32_t> by_donor((size_t)nb * 27, -1), pull((size_t)nb * 27, -1);
+        for (long long k = 0; k < (long long)nb * 27; k++) {
+            const int code = (int)(k % 27);
+            const int dir[3] = {code / 9 - 1, (code / 3) % 3 - 1, code % 3 - 1};
+            long long band = 1;
+            for (int a = 0; a < 3; a++) band *= dir[a] ? ng : n3[a];
+            if (fits && recv[k] >= 0 && !bad[k] && cnt[k] == band) {
+                by_donor[k] = recv[k];
+                pull[(size_t)recv[k] * 27 + (26 - code)] = (int32_t)(k / 27);
+            }
+        }
+        std::vector<int2> rest;
+        for (const int2 &e : copy32) {
+            int R;
+            const long long k = fits ? key_of(e, R) : -1;
+            if (k < 0 || by_donor[k] != R) rest.push_back(e);
+        }
+        up(h->ghost_rest32, rest.data(), sizeof(int2) * rest.size());
+        up(h->ghost_pull32, pull.data(), sizeof(int32_t) * pull.size());
+        h->gh32.copy_rest = h->ghost_rest32.as<int2>();
+        h->gh32.n_copy_rest = (int)rest.size();
+        h->gh32.pull = h->ghost_pull32.as<int32_t>();
+    }
+
     std::vector<int32_t> bl;
     for (int b = 0; b < nb; b++) {
         if (h->lay.geom[b].bottom) bl.push_back(b * 2 + 0);
@@ -393,22 +465,31 @@ void enqueue_step(falcon_handle *h, double cfl, int cur) {
         float *F = h->F32[cur].as<float>(), *Fn = h->F32[1 - cur].as<float>();
         const float *in[4] = {F, S0, S1, S0};
         float *out[3] = {S0, S1, S0};
+        // with the pull table, the stage and diffusion kernels read same-level
+        // ghost copies from the neighbours' interiors and the exchange fills
+        // only the other ghosts
+        const int32_t *pull = h->ghost_pull ? h->gh32.pull : nullptr;
+        auto exchange = [&](float *st) {
+            if (pull) launch_exchange_rest<float>(d, h->gh32, st, s);
+            else launch_exchange<float>(d, h->gh32, st, s);
+        };
         for (int k = 1; k <= 4; k++) {
-            timed(h, K_BC, [&] { launch_boundary<float>(d, h->bnd, const_cast<float *>(in[k - 1]), s); });
+            timed(h, K_BC, [&] { launch_boundary<float>(d, h->bnd, const_cast<float *>(in[k - 1]), s, pull); });
             const double div = (double)(5.0f - (float)k);
             if (k < 4) {
                 // intermediate stage values are FP32 temporaries: base = FP32 copy of U^n
                 timed(h, K_STAGE, [&] {
-                    launch_stage<float, float, float, float>(d, t, in[k - 1], F, out[k - 1], nullptr, dtp, div, s);
+                    launch_stage<float, float, float, float>(d, t, in[k - 1], F, out[k - 1], nullptr, dtp, div, s, pull);
                 });
-                timed(h, K_XSTAGE, [&] { launch_exchange<float>(d, h->gh32, out[k - 1], s); });
+                timed(h, K_XSTAGE, [&] { exchange(out[k - 1]); });
             } else {
-                timed(h, K_STAGE, [&] { launch_stage<float, float, double, double>(d, t, in[3], U, U, F, dtp, div, s); });
-                timed(h, K_XSTATE, [&] { launch_exchange<float>(d, h->gh32, F, s); });
+                timed(h, K_STAGE,
+                      [&] { launch_stage<float, float, double, double>(d, t, in[3], U, U, F, dtp, div, s, pull); });
+                timed(h, K_XSTATE, [&] { exchange(F); });
             }
         }
-        timed(h, K_DIFF, [&] { launch_diffusion<float, float>(d, t, F, U, Un, Fn, part, dtp, s); });
-        timed(h, K_XSTATE, [&] { launch_exchange<float>(d, h->gh32, Fn, s); });
+        timed(h, K_DIFF, [&] { launch_diffusion<float, float>(d, t, F, U, Un, Fn, part, dtp, s, pull); });
+        timed(h, K_XSTATE, [&] { exchange(Fn); });
     }
 }
 
@@ -807,6 +888,16 @@ int falcon_set_precision(falcon_handle_t h, int mode) {
     });
 }
 
+int falcon_set_ghost_pull(falcon_handle_t h, int on) {
+    return guard([&] {
+        if ((on != 0) != h->ghost_pull && h->graph) {
+            HIPCHECK(hipGraphExecDestroy(h->graph));
+            h->graph = nullptr;
+        }
+        h->ghost_pull = on != 0;
+    });
+}
+
 int falcon_sendrecv_gc(falcon_handle_t h, int which) {
     return guard([&] {
         h->need_tables();
@@ -977,7 +1068,7 @@ int falcon_device_bytes(falcon_handle_t h, int64_t *bytes) {
         const DevBuf *all[] = {&h->meta, &h->radial, &h->angular, &h->ghost64[0], &h->ghost64[1], &h->ghost64[2],
                                &h->ghost64[3], &h->ghost32[0], &h->ghost32[1], &h->ghost32[2], &h->ghost32[3],
                                &h->ghost_copy32, &h->ghost_lerp32[0], &h->ghost_lerp32[1],
-
+                               &h->ghost_rest32, &h->ghost_pull32,
                                &h->bnd_list, &h->U[0], &h->U[1], &h->F32[0], &h->F32[1], &h->S64[0], &h->S64[1],
                                &h->S32[0], &h->S32[1], &h->dtp, &h->partial, &h->rbuf};
         int64_t s = 0;
@@ -1208,6 +1299,10 @@ int falcon_heat_set_graph(falcon_handle_t h, int on) {
 
 int falcon_heat_set_learned_blockstep(falcon_handle_t h, int on) {
     return guard([&] { need_heat(h).set_learned_blockstep(on != 0); });
+}
+
+int falcon_heat_set_learned_iters(falcon_handle_t h, int n) {
+    return guard([&] { need_heat(h).set_learned_iters(n); });
 }
 
 int falcon_heat_set_learned_sweeps(falcon_handle_t h, int pre0, int post0) {

@@ -16,6 +16,8 @@
 // the reference.  The FP32 instantiation (T = float) is the deviation path: the
 // stencil arithmetic runs in FP32 while the stage accumulation into the FP64
 // state, the ghost interpolation weights and the time step stay FP64.
+#include <stdexcept>
+
 #include "device.hpp"
 
 namespace falcon {
@@ -61,22 +63,42 @@ __device__ __forceinline__ T face_flux(T a, T b, T c, T d) {
 }
 
 // Shared r-marching skeleton: loads the column window and the halo'd plane.
-template <typename T, typename TIN>
+// With a pull table (the row of block b: entry (di+1)*9+(dj+1)*3+(dk+1) is the
+// same-level neighbour whose interior holds this block's ghosts in direction
+// (di,dj,dk), or -1), those ghosts are read from the neighbour at the ghost
+// index minus (di*ni, dj*nj, dk*nk); the radial ghost planes of the column are
+// direction (-1,0,0) / (+1,0,0), the theta/phi halo strips (0,dj,0) / (0,0,dk).
+// PULL = false compiles the plain reads (the pull table is ignored).
+template <typename T, typename TIN, bool PULL = false>
 struct Marcher {
     const TIN *blk;
+    long long dlo, dhi;     // offset from blk of the column source of the radial ghost
+                            // planes below / above the block, plane shift included
+    const TIN *hblk;        // halo strip source
     long long P;
     int SI, SJ;
+    int ilo, ihi;           // first interior plane, first plane above the interior
     int J, K;           // padded column of this lane
     bool act;
     bool hv;
     long long hoff;     // halo point offset within a plane
     int hly, hlx;
 
-    __device__ void init(const Dims &d, const TIN *in, int b, int j0, int k0, int TJa, int TKa) {
+    __device__ __forceinline__ void init(const Dims &d, const TIN *in, int b, int j0, int k0, int TJa, int TKa,
+                         const int32_t *pull = nullptr) {
         P = d.P;
         SI = d.NJ * d.NK;
         SJ = d.NK;
         blk = in + (long long)b * 8 * P;
+        ilo = d.ng;
+        ihi = d.ng + d.ni;
+        hblk = blk;
+        dlo = dhi = 0;
+        if (PULL && pull) {
+            const int Dlo = pull[b * 27 + 4], Dhi = pull[b * 27 + 22];
+            if (Dlo >= 0) dlo = (long long)(Dlo - b) * 8 * P + (long long)d.ni * SI;
+            if (Dhi >= 0) dhi = (long long)(Dhi - b) * 8 * P - (long long)d.ni * SI;
+        }
         const int tx = threadIdx.x & (TILE - 1), ty = threadIdx.x / TILE;
         act = ty < TJa && tx < TKa;
         J = j0 + ty + d.ng;
@@ -96,12 +118,28 @@ struct Marcher {
             hv = jj < TJa;
         }
         hoff = (long long)(j0 + hly) * SJ + (k0 + hlx);
+        if (PULL && pull && hv) {
+            const int HJ = j0 + hly, HK = k0 + hlx;   // padded halo cell
+            const int dj = HJ < d.ng ? -1 : (HJ >= d.ng + d.nj ? 1 : 0);
+            const int dk = HK < d.ng ? -1 : (HK >= d.ng + d.nk ? 1 : 0);
+            if (dj | dk) {
+                const int D = pull[b * 27 + 9 + (dj + 1) * 3 + (dk + 1)];
+                if (D >= 0) {
+                    hblk = in + (long long)D * 8 * P;
+                    hoff = (long long)(HJ - dj * d.nj) * SJ + (HK - dk * d.nk);
+                }
+            }
+        }
     }
+    // I is the same for every lane of the workgroup
     __device__ __forceinline__ T col(int v, int I) const {
-        return (T)blk[(long long)v * P + (long long)I * SI + (long long)J * SJ + K];
+        long long o = (long long)v * P + (long long)I * SI + (long long)J * SJ + K;
+        if (PULL) o += (I < ilo ? dlo : 0ll) + (I >= ihi ? dhi : 0ll);
+        return (T)blk[o];
     }
+    // interior planes only
     __device__ __forceinline__ T halo(int v, int I) const {
-        return (T)blk[(long long)v * P + (long long)I * SI + hoff];
+        return (T)(PULL ? hblk : blk)[(long long)v * P + (long long)I * SI + hoff];
     }
 };
 
@@ -221,11 +259,13 @@ __device__ __forceinline__ void cell_rhs(const T (&win)[8][5], LDSF L, const T x
 //         are stored in FP32 anyway); with SHADOW the same value is also stored
 //         rounded to FP32 in `shadow`.
 // MODE 1: write R (interior layout [block][var][i][j][k], FP64) for parity tests.
+// pull (null: none): ghost cells of `in` in the directions it lists are read
+// from the same-level neighbour's interior (Marcher).
 template <typename T, typename TIN, typename TOUT, typename TB, int MODE, bool SHADOW>
 __global__ void __launch_bounds__(256) k_stage(Dims d, DevTables t, const TIN *__restrict__ in,
                                                const TB *__restrict__ U, TOUT *__restrict__ out,
                                                float *__restrict__ shadow, const double *__restrict__ dtp,
-                                               double div) {
+                                               double div, const int32_t *__restrict__ pull) {
     __shared__ T lds[2][8][PL][PL];
     const int tpb = d.tiles_j * d.tiles_k;
     const int b = blockIdx.x / tpb;
@@ -235,8 +275,9 @@ __global__ void __launch_bounds__(256) k_stage(Dims d, DevTables t, const TIN *_
     const int TJa = min(TILE, d.nj - j0), TKa = min(TILE, d.nk - k0);
     const int tx = threadIdx.x & (TILE - 1), ty = threadIdx.x / TILE;
 
-    Marcher<T, TIN> m;
-    m.init(d, in, b, j0, k0, TJa, TKa);
+    constexpr bool PULL = sizeof(T) == 4 && MODE == 0;
+    Marcher<T, TIN, PULL> m;
+    m.init(d, in, b, j0, k0, TJa, TKa, pull);
 
     const long long rb = (long long)b * d.NI, tb = (long long)b * d.NJ;
     const BlockMeta mt = t.meta[b];
@@ -327,19 +368,32 @@ template <typename T, typename TIN, int MODE>
 __global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const TIN *__restrict__ in,
                                                    const double *__restrict__ U, double *__restrict__ out,
                                                    float *__restrict__ shadow, double *__restrict__ partial,
-                                                   const double *__restrict__ dtp) {
+                                                   const double *__restrict__ dtp, const int32_t *__restrict__ pull) {
     __shared__ T lds[2][8][PL][PL];
     __shared__ double red[4];
+    // offset of the cell read in place of a ghost in direction code c, relative
+    // to the ghost's own offset (0 where the ghost itself is read); set before
+    // the first barrier of the march
+    __shared__ long long pdel[27];
     const int tpb = d.tiles_j * d.tiles_k;
     const int b = blockIdx.x / tpb;
+    if (sizeof(T) == 4 && MODE == 0 && threadIdx.x < 27) {
+        const int c = threadIdx.x;
+        const int D = pull ? pull[b * 27 + c] : -1;
+        const int di = c / 9 - 1, dj = (c / 3) % 3 - 1, dk = c % 3 - 1;
+        pdel[c] = D < 0 ? 0ll
+                        : (long long)(D - b) * 8 * d.P - (long long)di * d.ni * (d.NJ * d.NK) -
+                              (long long)dj * d.nj * d.NK - dk * d.nk;
+    }
     const int trem = blockIdx.x - b * tpb;
     const int tj = trem / d.tiles_k, tk = trem - tj * d.tiles_k;
     const int j0 = tj * TILE, k0 = tk * TILE;
     const int TJa = min(TILE, d.nj - j0), TKa = min(TILE, d.nk - k0);
     const int tx = threadIdx.x & (TILE - 1), ty = threadIdx.x / TILE;
 
-    Marcher<T, TIN> m;
-    m.init(d, in, b, j0, k0, TJa, TKa);
+    constexpr bool PULL = sizeof(T) == 4 && MODE == 0;
+    Marcher<T, TIN, PULL> m;
+    m.init(d, in, b, j0, k0, TJa, TKa, pull);
     const long long rb = (long long)b * d.NI, tb = (long long)b * d.NJ;
     const BlockMeta mt = t.meta[b];
     const T dxi = (T)mt.dxi, dxj = (T)mt.dxj, dxk = (T)mt.dxk;
@@ -381,8 +435,17 @@ __global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const TI
         __syncthreads();
         if (m.act) {
             const int J = m.J, K = m.K;
+            // a ghost in a direction listed in the pull table is the
+            // same-level neighbour's interior cell, translated by n
             auto at = [&](int v, int II, int JJ, int KK) -> T {
-                return (T)Ub[(long long)v * d.P + (long long)II * m.SI + (long long)JJ * m.SJ + KK];
+                long long o = (long long)v * d.P + (long long)II * m.SI + (long long)JJ * m.SJ + KK;
+                if (PULL) {
+                    const int c = (II < d.ng ? 0 : (II >= d.ng + d.ni ? 18 : 9)) +
+                                  (JJ < d.ng ? 0 : (JJ >= d.ng + d.nj ? 6 : 3)) +
+                                  (KK < d.ng ? 0 : (KK >= d.ng + d.nk ? 2 : 1));
+                    o += pdel[c];
+                }
+                return (T)Ub[o];
             };
             // speeds |v_d| + c_s at the cells the reference reads (see above)
             const T csI = (T)t.cs[rb + I], csIm = (T)t.cs[rb + I - 1], csIp = (T)t.cs[rb + I + 1], csI2 = (T)t.cs[rb + I - 2];
@@ -544,10 +607,11 @@ __global__ void __launch_bounds__(256) k_exchange(Dims d, DevGhosts g, TS *__res
 // neighbours whose cell centres coincide up to rounding), copied from the
 // nearest donor cell.
 template <typename TS>
-__global__ void __launch_bounds__(256) k_exchange_copy(Dims d, DevGhosts g, TS *__restrict__ st) {
+__global__ void __launch_bounds__(256) k_exchange_copy(Dims d, const int2 *__restrict__ copy, int n,
+                                                       TS *__restrict__ st) {
     const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (e >= g.n_copy) return;
-    const int2 c = g.copy[e];
+    if (e >= n) return;
+    const int2 c = copy[e];
     const long long P = d.P;
     TS val[8];
 #pragma unroll
@@ -581,8 +645,12 @@ __global__ void __launch_bounds__(256) k_exchange_lerp(Dims d, DevGhosts g, TS *
 
 // ModBoundary_Dynamo_primitives (3_6/ModBoundary.f90:10-55): radial mirror of
 // the ng ghost planes of top/bottom blocks over all (j,k), parities per variable.
+// The mirror source of the theta/phi ghost columns is a ghost cell of the
+// block's own interior plane; with a pull table (see Marcher) those in a listed
+// direction are read from the same-level neighbour's interior.
 template <typename TS>
-__global__ void __launch_bounds__(256) k_boundary(Dims d, DevBoundary bl, TS *__restrict__ st) {
+__global__ void __launch_bounds__(256) k_boundary(Dims d, DevBoundary bl, TS *__restrict__ st,
+                                                  const int32_t *__restrict__ pull) {
     const long long plane = (long long)d.NJ * d.NK;
     const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long per = (long long)d.ng * plane;
@@ -598,9 +666,23 @@ __global__ void __launch_bounds__(256) k_boundary(Dims d, DevBoundary bl, TS *__
     // signs: top (+,-,+,+,+,-,-,+), bottom (+,-,+,+,-,+,+,+)
     const unsigned neg = side ? 0x62u : 0x12u;   // bit v set -> negate variable v
     TS *base = st + (long long)b * 8 * d.P;
+    const TS *src = base;
+    long long sjk = jk;
+    if (pull) {
+        const int J = (int)(jk / d.NK), K = (int)(jk - (long long)J * d.NK);
+        const int dj = J < d.ng ? -1 : (J >= d.ng + d.nj ? 1 : 0);
+        const int dk = K < d.ng ? -1 : (K >= d.ng + d.nk ? 1 : 0);
+        if (dj | dk) {
+            const int D = pull[b * 27 + 9 + (dj + 1) * 3 + (dk + 1)];
+            if (D >= 0) {
+                src = st + (long long)D * 8 * d.P;
+                sjk = (long long)(J - dj * d.nj) * d.NK + (K - dk * d.nk);
+            }
+        }
+    }
 #pragma unroll
     for (int v = 0; v < 8; v++) {
-        TS x = base[(long long)v * d.P + (long long)sI * plane + jk];
+        TS x = src[(long long)v * d.P + (long long)sI * plane + sjk];
         base[(long long)v * d.P + (long long)dI * plane + jk] = ((neg >> v) & 1u) ? -x : x;
     }
 }
@@ -708,37 +790,40 @@ static inline unsigned tiles(const Dims &d) { return (unsigned)(d.nblocks * d.ti
 
 template <typename T, typename TIN, typename TOUT, typename TB>
 void launch_stage(const Dims &d, const DevTables &t, const TIN *in, const TB *U, TOUT *out, float *shadow,
-                  const double *dtp, double div, hipStream_t s) {
+                  const double *dtp, double div, hipStream_t s, const int32_t *pull) {
     if (shadow)
         hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, true>), dim3(tiles(d)), dim3(256), 0, s, d, t, in, U, out,
-                           shadow, dtp, div);
+                           shadow, dtp, div, pull);
     else
         hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, false>), dim3(tiles(d)), dim3(256), 0, s, d, t, in, U, out,
-                           (float *)nullptr, dtp, div);
+                           (float *)nullptr, dtp, div, pull);
 }
 
 template <typename T, typename TIN>
 void launch_rhs_only(const Dims &d, const DevTables &t, const TIN *in, double *R, hipStream_t s) {
     hipLaunchKernelGGL((k_stage<T, TIN, double, double, 1, false>), dim3(tiles(d)), dim3(256), 0, s, d, t, in,
-                       (const double *)nullptr, R, (float *)nullptr, (const double *)nullptr, 1.0);
+                       (const double *)nullptr, R, (float *)nullptr, (const double *)nullptr, 1.0,
+                       (const int32_t *)nullptr);
 }
 
 template <typename T, typename TIN>
 void launch_diffusion(const Dims &d, const DevTables &t, const TIN *in, const double *U, double *out, float *shadow,
-                      double *partial, const double *dtp, hipStream_t s) {
-    hipLaunchKernelGGL((k_diffusion<T, TIN, 0>), dim3(tiles(d)), dim3(256), 0, s, d, t, in, U, out, shadow, partial, dtp);
+                      double *partial, const double *dtp, hipStream_t s, const int32_t *pull) {
+    hipLaunchKernelGGL((k_diffusion<T, TIN, 0>), dim3(tiles(d)), dim3(256), 0, s, d, t, in, U, out, shadow, partial, dtp,
+                       pull);
 }
 
 template <typename T>
 void launch_diffusion_only(const Dims &d, const DevTables &t, const double *U, double *R, hipStream_t s) {
     hipLaunchKernelGGL((k_diffusion<T, double, 1>), dim3(tiles(d)), dim3(256), 0, s, d, t, U, U, R, (float *)nullptr,
-                       (double *)nullptr, (const double *)nullptr);
+                       (double *)nullptr, (const double *)nullptr, (const int32_t *)nullptr);
 }
 
 template <typename TS>
-void launch_exchange(const Dims &d, const DevGhosts &g, TS *state, hipStream_t s) {
-    if (g.n_copy > 0)
-        hipLaunchKernelGGL((k_exchange_copy<TS>), dim3((unsigned)((g.n_copy + 255) / 256)), dim3(256), 0, s, d, g, state);
+static void exchange_with(const Dims &d, const DevGhosts &g, const int2 *copy, int n_copy, TS *state, hipStream_t s) {
+    if (n_copy > 0)
+        hipLaunchKernelGGL((k_exchange_copy<TS>), dim3((unsigned)((n_copy + 255) / 256)), dim3(256), 0, s, d, copy,
+                           n_copy, state);
     if (g.n_lerp > 0)
         hipLaunchKernelGGL((k_exchange_lerp<TS>), dim3((unsigned)((g.n_lerp + 255) / 256)), dim3(256), 0, s, d, g, state);
     if (g.n_total > g.first)
@@ -747,10 +832,21 @@ void launch_exchange(const Dims &d, const DevGhosts &g, TS *state, hipStream_t s
 }
 
 template <typename TS>
-void launch_boundary(const Dims &d, const DevBoundary &b, TS *state, hipStream_t s) {
+void launch_exchange(const Dims &d, const DevGhosts &g, TS *state, hipStream_t s) {
+    exchange_with(d, g, g.copy, g.n_copy, state, s);
+}
+
+template <typename TS>
+void launch_exchange_rest(const Dims &d, const DevGhosts &g, TS *state, hipStream_t s) {
+    if (!g.pull) throw std::runtime_error("no pull table: use the full exchange");
+    exchange_with(d, g, g.copy_rest, g.n_copy_rest, state, s);
+}
+
+template <typename TS>
+void launch_boundary(const Dims &d, const DevBoundary &b, TS *state, hipStream_t s, const int32_t *pull) {
     if (b.n == 0) return;
     const long long n = (long long)b.n * d.ng * d.NJ * d.NK;
-    hipLaunchKernelGGL((k_boundary<TS>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, b, state);
+    hipLaunchKernelGGL((k_boundary<TS>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, b, state, pull);
 }
 
 template <typename TS>
@@ -773,19 +869,20 @@ void launch_to_f32(const double *src, float *dst, long long n, hipStream_t s) {
 }
 
 // explicit instantiations: FP64 path, FP32-deviation path
-template void launch_stage<double, double, double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, const double *, double, hipStream_t);
-template void launch_stage<float, float, float, float>(const Dims &, const DevTables &, const float *, const float *, float *, float *, const double *, double, hipStream_t);
-template void launch_stage<float, float, double, double>(const Dims &, const DevTables &, const float *, const double *, double *, float *, const double *, double, hipStream_t);
+template void launch_stage<double, double, double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, const double *, double, hipStream_t, const int32_t *);
+template void launch_stage<float, float, float, float>(const Dims &, const DevTables &, const float *, const float *, float *, float *, const double *, double, hipStream_t, const int32_t *);
+template void launch_stage<float, float, double, double>(const Dims &, const DevTables &, const float *, const double *, double *, float *, const double *, double, hipStream_t, const int32_t *);
 template void launch_rhs_only<double, double>(const Dims &, const DevTables &, const double *, double *, hipStream_t);
 template void launch_rhs_only<float, double>(const Dims &, const DevTables &, const double *, double *, hipStream_t);
-template void launch_diffusion<double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, double *, const double *, hipStream_t);
-template void launch_diffusion<float, float>(const Dims &, const DevTables &, const float *, const double *, double *, float *, double *, const double *, hipStream_t);
+template void launch_diffusion<double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, double *, const double *, hipStream_t, const int32_t *);
+template void launch_diffusion<float, float>(const Dims &, const DevTables &, const float *, const double *, double *, float *, double *, const double *, hipStream_t, const int32_t *);
 template void launch_diffusion_only<double>(const Dims &, const DevTables &, const double *, double *, hipStream_t);
 template void launch_diffusion_only<float>(const Dims &, const DevTables &, const double *, double *, hipStream_t);
 template void launch_exchange<double>(const Dims &, const DevGhosts &, double *, hipStream_t);
 template void launch_exchange<float>(const Dims &, const DevGhosts &, float *, hipStream_t);
-template void launch_boundary<double>(const Dims &, const DevBoundary &, double *, hipStream_t);
-template void launch_boundary<float>(const Dims &, const DevBoundary &, float *, hipStream_t);
+template void launch_exchange_rest<float>(const Dims &, const DevGhosts &, float *, hipStream_t);
+template void launch_boundary<double>(const Dims &, const DevBoundary &, double *, hipStream_t, const int32_t *);
+template void launch_boundary<float>(const Dims &, const DevBoundary &, float *, hipStream_t, const int32_t *);
 template void launch_copy_radial<double>(const Dims &, const DevBoundary &, const double *, double *, hipStream_t);
 template void launch_copy_radial<float>(const Dims &, const DevBoundary &, const float *, float *, hipStream_t);
 

@@ -87,6 +87,13 @@ int falcon_init_synthetic(falcon_handle_t h, uint64_t seed, double s_amp, double
  * precision: 0 = FP64 everywhere; 1 = FP32 stencil arithmetic on the deviation
  * state with FP64 background, stage accumulation, ghost weights and dt. */
 int falcon_set_precision(falcon_handle_t h, int mode);
+/* FP32 step: on (default), the stage and diffusion kernels read the ghosts
+ * that are plain copies of a same-level neighbour's cells from that
+ * neighbour's interior, and the ghost fill between them runs only the other
+ * entries; off, the ghost fill runs every entry and the kernels read their own
+ * ghost cells.  Both read the same values.  No counterpart in the reference
+ * (its ghost fill is ModCommunication_SendRecvGC_new). */
+int falcon_set_ghost_pull(falcon_handle_t h, int on);
 /* ModCommunication_SendRecvGC_new(Tree, if_rk) (3_6/ModCommunication.f90:655-795):
  * which = 0 fills the state's ghosts, 1 the current stage buffer's. */
 int falcon_sendrecv_gc(falcon_handle_t h, int which);
@@ -138,6 +145,10 @@ int falcon_heat_set_graph(falcon_handle_t h, int on);
 /* learned correction step: one per 16^3 block (on, default) or one per system
  * (off).  No counterpart in the reference (learned V-cycle correction). */
 int falcon_heat_set_learned_blockstep(falcon_handle_t h, int on);
+/* the learned correction builds the first n search directions of a learned
+ * solve and the plain V-cycle the rest; 0 (default): every direction.  No
+ * counterpart in the reference (learned V-cycle correction). */
+int falcon_heat_set_learned_iters(falcon_handle_t h, int n);
 /* y = A x for all systems (parity tests) */
 int falcon_heat_apply(falcon_handle_t h, const double *x, double *y);
 /* z = M r: one V-cycle of the preconditioner for all systems, zero initial
