@@ -145,12 +145,25 @@ struct Marcher {
 
 // ---------------------------------------------------------------- stage
 
+// FP64 inputs of the FP32 path's pressure and gravity terms: gp and gq at the
+// five r-offsets of the column, the radial stencil coefficient, 1/rho0 and
+// g/rho0 of the cell.
+struct Balance64 {
+    const double *gp, *gq;
+    double cr, rho0inv, gor;
+};
+
 // RHS of ModEquation_Dynamo (3_6/ModEquation.f90:24-162) for one cell.
 // win[v][q]: variable v at r-offset q-2 of this lane's column; L(v,dj,dk): LDS plane.
+// With T = float the pressure perturbation p1 = gp rho' + gq s' and the sum of
+// the radial pressure gradient and gravity are formed in FP64 from FP64
+// coefficients (bal): once the pressure perturbation has relaxed both are
+// small differences of large terms, whose FP32 rounding would dominate the
+// velocity error.  T = double keeps the reference's operation order.
 template <typename T, typename LDSF>
 __device__ __forceinline__ void cell_rhs(const T (&win)[8][5], LDSF L, const T xr[5], const T rho0r[5], const T gpr[5],
                                          const T gqr[5], const T st[5], T rinv, T r2inv, T xir2, T heat, T gor,
-                                         T sinv, T cot, T cr, T ct, T cp, T R[8]) {
+                                         T sinv, T cot, T cr, T ct, T cp, T R[8], const Balance64 &bal) {
     const T rho0c = rho0r[2];
     const T vr = win[1][2], vt = win[2][2], vp = win[3][2];
     const T br = win[4][2], bt = win[5][2], bp = win[6][2];
@@ -194,7 +207,21 @@ __device__ __forceinline__ void cell_rhs(const T (&win)[8][5], LDSF L, const T x
         R[1] = -a1; R[2] = -a2; R[3] = -a3;
     }
     // Pressure gradient (:79-92): p1 (:40-46), ModSpherical_Grad_f (:201-239)
-    {
+    double prad = 0.0;   // FP32 path: radial pressure gradient plus gravity, FP64
+    if constexpr (sizeof(T) == 4) {
+        double pr[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) pr[q] = bal.gp[q] * (double)win[0][q] + bal.gq[q] * (double)win[7][q];
+        auto p1L = [&](int dj, int dk) -> double {
+            return bal.gp[2] * (double)L(0, dj, dk) + bal.gq[2] * (double)L(7, dj, dk);
+        };
+        const double t1 = d4(pr[0], pr[1], pr[3], pr[4], bal.cr);
+        const double t2 = d4(p1L(-2, 0), p1L(-1, 0), p1L(1, 0), p1L(2, 0), (double)ct);
+        const double t3 = d4(p1L(0, -2), p1L(0, -1), p1L(0, 1), p1L(0, 2), (double)cp);
+        prad = (-t1) * bal.rho0inv - bal.gor * (double)win[0][2];
+        R[2] = R[2] - (T)((t2 * (double)rinv) * bal.rho0inv);
+        R[3] = R[3] - (T)(((t3 * (double)rinv) * (double)sinv) * bal.rho0inv);
+    } else {
         T pr[5];
 #pragma unroll
         for (int q = 0; q < 5; q++) pr[q] = gpr[q] * win[0][q] + gqr[q] * win[7][q];
@@ -208,6 +235,7 @@ __device__ __forceinline__ void cell_rhs(const T (&win)[8][5], LDSF L, const T x
         R[2] = R[2] - g2 / rho0c;
         R[3] = R[3] - g3 / rho0c;
     }
+    (void)prad;
     // Lorentz force (:97-116): curl(B) (ModSpherical.f90:38-102) x B / rho0
     {
         T o1 = d4(L(6, -2, 0) * st[0], L(6, -1, 0) * st[1], L(6, 1, 0) * st[3], L(6, 2, 0) * st[4], ct) -
@@ -226,7 +254,8 @@ __device__ __forceinline__ void cell_rhs(const T (&win)[8][5], LDSF L, const T x
         R[3] = R[3] + c3 / rho0c;
     }
     // Gravity (:121-127)
-    R[1] = R[1] - gor * win[0][2];
+    if constexpr (sizeof(T) == 4) R[1] = (T)((double)R[1] + prad);
+    else R[1] = R[1] - gor * win[0][2];
     // Entropy advection and heating (:132-151)
     R[7] = -adg(7);
     R[7] = R[7] + heat;
@@ -327,8 +356,16 @@ __global__ void __launch_bounds__(256) k_stage(Dims d, DevTables t, const TIN *_
             }
             auto L = [&](int v, int dj, int dk) -> T { return lds[buf][v][ty + 2 + dj][tx + 2 + dk]; };
             T R[8];
+            Balance64 bal;
+            if (sizeof(T) == 4) {
+                bal.gp = t.gp + rb + I - 2;
+                bal.gq = t.gq + rb + I - 2;
+                bal.cr = mt.cr;
+                bal.rho0inv = 1.0 / t.rho0[rb + I];
+                bal.gor = t.g_over_rho0[rb + I];
+            }
             cell_rhs<T>(win, L, xr, rho0r, gpr, gqr, st, (T)t.rinv[rb + I], (T)t.r2inv[rb + I], (T)t.xir2[rb + I],
-                        (T)t.heat[rb + I], (T)t.g_over_rho0[rb + I], sinv, cot, cr, ct, cp, R);
+                        (T)t.heat[rb + I], (T)t.g_over_rho0[rb + I], sinv, cot, cr, ct, cp, R, bal);
             if (MODE == 0) {
                 const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + (long long)m.J * m.SJ + m.K;
 #pragma unroll
