@@ -78,6 +78,9 @@ def main():
                     help="directions built with the learned correction before the plain V-cycle takes over (0: all)")
     ap.add_argument("--legs", default="rk,heat,learned",
                     help="comma list of legs: rk (RK4 FP64/FP32), heat (MG-PCG), learned (learned FCG vs MG-PCG)")
+    ap.add_argument("--error-steps", type=int, default=0,
+                    help="also step the FP64 and FP32 paths this many steps from the same state and report the "
+                         "FP32 path's per-variable difference (0: skip)")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="only run a device-to-device copy of this many GiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -106,6 +109,10 @@ def main():
                   f"{res[prec]['ms_per_step']:.3f} ms/step, kernels {res[prec]['kernel_ms']}", file=sys.stderr,
                   flush=True)
         shell.close()
+        if args.error_steps > 0:
+            err = fp32_error(lay, args.error_steps, args.cfl)
+            res["fp32_error"] = err
+            print(f"# fp32 vs fp64 after {args.error_steps} steps: {json.dumps(err)}", file=sys.stderr, flush=True)
         traffic = load_traffic(args.workload)
         for prec in ("fp32", "fp64"):
             res[prec]["stage_roofline"]["traffic"] = traffic.get(prec, {}).get("stage_bytes_per_launch")
@@ -146,6 +153,7 @@ def main():
                   "kernel_ms": res["fp64"]["kernel_ms"], "stage_roofline": res["fp64"]["stage_roofline"]}
                  if res else None),
         "fp32_over_fp64": head["cell_updates_per_s"] / res["fp64"]["cell_updates_per_s"] if head else None,
+        "fp32_error": res.get("fp32_error") if res else None,
         "kernel_ms": head["kernel_ms"] if head else None,
         "roofline": head["stage_roofline"] if head else None,
         "setup_s": setup_s,
@@ -153,6 +161,36 @@ def main():
         "heat": heat,
     }
     print(json.dumps(out))
+
+
+def fp32_error(lay, steps, cfl):
+    """FP64 and FP32-deviation paths stepped `steps` times from the same seeded
+    state (each with its own CFL dt): per variable, max|a-b|/max|b| over the
+    interior, its 99.99th percentile, and the number of cells above 1e-6."""
+    sh = {}
+    for prec in ("fp64", "fp32"):
+        s = Shell(lay)
+        s.background_polytrope(20.0)
+        s.set_gc_all()
+        s.set_precision(prec)
+        s.init_synthetic(seed=12345, s_amp=1e-4, b_ratio=1e-3)
+        s.advance(steps, cfl, graph=True)
+        s.synchronize()
+        sh[prec] = s
+    ng = lay.get("ng", 2)
+    nb = sh["fp64"].nblocks
+    diffs, bmax = [], np.zeros(8)
+    for ib in range(nb):
+        a = sh["fp32"].get_state(ib)[:, ng:-ng, ng:-ng, ng:-ng]
+        b = sh["fp64"].get_state(ib)[:, ng:-ng, ng:-ng, ng:-ng]
+        diffs.append(np.abs(a - b).reshape(8, -1).astype(np.float32))
+        bmax = np.maximum(bmax, np.abs(b).reshape(8, -1).max(axis=1))
+    for s in sh.values():
+        s.close()
+    d = np.concatenate(diffs, axis=1) / np.where(bmax > 0, bmax, 1.0)[:, None].astype(np.float32)
+    return {"steps": steps, "max_rel": [float(x) for x in d.max(axis=1)],
+            "q9999_rel": [float(x) for x in np.quantile(d, 0.9999, axis=1)],
+            "cells_above_1e-6": [int(x) for x in (d > 1e-6).sum(axis=1)], "cells": int(d.shape[1])}
 
 
 def copy_calibration(gib):
