@@ -588,7 +588,10 @@ __global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const TI
 // leaves every nonzero result bit-identical.  On the production layouts this
 // removes 60% of the FP64 corner loads and shortens the FP64 fill by 15%.  For
 // the FP32 buffers (whose near-copies go to the copy table) the unpredicated
-// gather measures faster.
+// gather measures twice as fast.  A zero-weight corner may then lie on a donor
+// ghost cell that the ghost pull leaves unwritten; the FP32 buffers are zeroed
+// when allocated and written with finite values only, so that term is f*0 = +-0
+// as well.
 template <typename TS, bool SKIP>
 __global__ void __launch_bounds__(256) k_exchange(Dims d, DevGhosts g, TS *__restrict__ st) {
     const int e = g.first + (int)(blockIdx.x * blockDim.x + threadIdx.x);
