@@ -108,14 +108,17 @@ def test_fp32_deviation_error_vs_fp64():
     assert np.all(err[1:] <= 1e-6), err
 
 
-@pytest.mark.parametrize("name", ["mixed", "cubic8"])
+PULL_LAYOUTS = dict(LAYOUTS, amr16=configs.cubic_random_amr((32, 64, 192), 16))
+
+
+@pytest.mark.parametrize("name", ["mixed", "cubic8", "amr16"])
 def test_fp32_ghost_pull_equals_full_exchange(name):
     """In the FP32 step the stage and diffusion kernels read same-level ghost
     copies from the neighbours' interiors and the ghost fill runs only the
     other entries; the states after several steps (eager and graph replay) are
     bit-identical to those of the full ghost fill."""
     print(f"[fp32 pull] layout={name}", flush=True)
-    lay = LAYOUTS[name]
+    lay = PULL_LAYOUTS[name]
     runs = []
     for pull, graph in ((False, False), (True, False), (True, True)):
         s = Shell(lay)
