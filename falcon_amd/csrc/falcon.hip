@@ -503,6 +503,11 @@ void prepare_steps(falcon_handle *h) {
             h->u_ghosts_valid = true;
         }
         launch_to_f32(h->U[h->cur].as<double>(), h->F32[h->cur].as<float>(), n, h->stream);
+        // ghosts as every later FP32 stage input has them: formed by the FP32
+        // ghost fill from the rounded interiors, which is also what the
+        // kernels' in-place reads give (rounding the FP64 ghosts instead can
+        // differ in the last bit where a weight carries rounding residue)
+        launch_exchange<float>(d, h->gh32, h->F32[h->cur].as<float>(), h->stream);
         h->shadow_valid = true;
     }
     if (h->precision == 0 && !h->u_ghosts_valid) {
