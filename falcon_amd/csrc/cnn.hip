@@ -620,6 +620,46 @@ __global__ void __launch_bounds__(256) k_cnn_loss(const float *__restrict__ out,
     if (threadIdx.x == 0) part[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
+// Gradient-weighted error per 16^3 image, d = out - y:
+//   L = alpha sum_cells d^2 + sum over the faces inside the image (d_a - d_b)^2,
+// an energy-like norm of the diffusion operator (mass term alpha, the face
+// couplings unit) that weighs the error's variation instead of its offset.
+// g3 = (dL/d out) * sc; loss partials per workgroup.  One workgroup walks
+// images blockIdx.x, blockIdx.x + gridDim.x, ...
+__global__ void __launch_bounds__(256) k_cnn_loss_h1(const float *__restrict__ out, const float *__restrict__ y,
+                                                     int nimg, float alpha, float sc, float *__restrict__ g3,
+                                                     double *__restrict__ part) {
+    __shared__ float dl[4096];
+    __shared__ double red[4];
+    double acc = 0.0;
+    for (int img = blockIdx.x; img < nimg; img += gridDim.x) {
+        const long long o = (long long)img * 4096;
+        for (int c = threadIdx.x; c < 4096; c += 256) dl[c] = out[o + c] - y[o + c];
+        __syncthreads();
+        for (int c = threadIdx.x; c < 4096; c += 256) {
+            const int i = c >> 8, j = (c >> 4) & 15, k = c & 15;
+            const float d = dl[c];
+            float g = alpha * d;
+            double l = (double)alpha * (double)d * (double)d;
+            // each face enters the loss once (from its lower cell) and the
+            // gradient of both cells
+            if (i > 0) g += d - dl[c - 256];
+            if (i < 15) { const float e = d - dl[c + 256]; g += e; l += (double)e * (double)e; }
+            if (j > 0) g += d - dl[c - 16];
+            if (j < 15) { const float e = d - dl[c + 16]; g += e; l += (double)e * (double)e; }
+            if (k > 0) g += d - dl[c - 1];
+            if (k < 15) { const float e = d - dl[c + 1]; g += e; l += (double)e * (double)e; }
+            g3[o + c] = sc * (2.0f * g);
+            acc += l;
+        }
+        __syncthreads();
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
 __global__ void k_cnn_sum(const double *__restrict__ part, int n, double scale, double *__restrict__ out) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     double s = 0.0;
@@ -768,9 +808,17 @@ double CnnModel::train_step(const float *x, const float *y, int nimg, double lr,
     hipLaunchKernelGGL((k_cnn_fwd<true, false, kFwdWaves>), dim3(nimg), dim3(64 * kFwdWaves), 0, stream, x, out_,
                        (const uint16_t *)packed_ + PK_TOTAL * 8, (const float *)params_, h1_, h2_, HeatIO{});
     const long long n = 4096LL * nimg;
-    hipLaunchKernelGGL(k_cnn_loss, dim3(1024), dim3(256), 0, stream, (const float *)out_, y, n, g3_, loss_part_);
-    hipLaunchKernelGGL(k_cnn_sum, dim3(1), dim3(64), 0, stream, (const double *)loss_part_, 1024, 1.0 / (double)n,
-                       loss_part_ + 1024);
+    if (h1_alpha_ >= 0.0) {
+        const int nwg = nimg < 1024 ? nimg : 1024;
+        hipLaunchKernelGGL(k_cnn_loss_h1, dim3(nwg), dim3(256), 0, stream, (const float *)out_, y, nimg,
+                           (float)h1_alpha_, (float)(1.0 / (double)n), g3_, loss_part_);
+        hipLaunchKernelGGL(k_cnn_sum, dim3(1), dim3(64), 0, stream, (const double *)loss_part_, nwg, 1.0 / (double)n,
+                           loss_part_ + 1024);
+    } else {
+        hipLaunchKernelGGL(k_cnn_loss, dim3(1024), dim3(256), 0, stream, (const float *)out_, y, n, g3_, loss_part_);
+        hipLaunchKernelGGL(k_cnn_sum, dim3(1), dim3(64), 0, stream, (const double *)loss_part_, 1024, 1.0 / (double)n,
+                           loss_part_ + 1024);
+    }
     backward(x, nimg, stream);
     adam_t_++;
     const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, (double)adam_t_)));

@@ -40,6 +40,9 @@ class CnnModel : public HeatCorrector {
     double train(const float *x, const float *y, long long nimg, int epochs, int batch, double lr, uint64_t seed,
                  hipStream_t stream);
     int max_batch() const { return max_batch_; }
+    // training loss: alpha < 0 the mean squared error (default); alpha >= 0 the
+    // gradient-weighted error alpha |d|^2 + |grad d|^2 per image (k_cnn_loss_h1)
+    void set_loss_h1(double alpha) { h1_alpha_ = alpha; }
 
   private:
     void ensure_batch(int nimg);
@@ -54,6 +57,7 @@ class CnnModel : public HeatCorrector {
     float *out_ = nullptr, *g3_ = nullptr, *part_ = nullptr, *xb_ = nullptr, *yb_ = nullptr;
     double *loss_part_ = nullptr;
     long long adam_t_ = 0;
+    double h1_alpha_ = -1.0;
 };
 
 }  // namespace falcon

@@ -1306,6 +1306,10 @@ int falcon_heat_set_learned_blockstep(falcon_handle_t h, int on) {
     return guard([&] { need_heat(h).set_learned_blockstep(on != 0); });
 }
 
+int falcon_cnn_set_loss(falcon_handle_t h, double h1_alpha) {
+    return guard([&] { need_cnn(h).set_loss_h1(h1_alpha); });
+}
+
 int falcon_heat_set_learned_iters(falcon_handle_t h, int n) {
     return guard([&] { need_heat(h).set_learned_iters(n); });
 }

@@ -84,6 +84,37 @@ def test_training_fits_a_teacher():
     assert loss < 0.2 * power
 
 
+def test_gradient_weighted_loss_step():
+    """One Adam step from fresh optimiser state moves every parameter by
+    -lr sign(g).  Under the gradient-weighted loss the GPU step's signs equal
+    those of the oracle's backward of the loss gradient, with the loss written
+    as axis differences and differentiated by torch autograd."""
+    import torch
+    s = _shell()
+    c = Cnn(s, seed=3)
+    p = _params(4, scale=0.5)
+    c.params = p
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal((4, 16, 16, 16)).astype(np.float32)
+    y = (0.3 * rng.standard_normal((4, 16, 16, 16))).astype(np.float32)
+    alpha, lr = 0.05, 1e-4
+    c.set_loss(alpha)
+    print("[cnn h1 loss step] 4 images", flush=True)
+    c.train(x, y, epochs=1, batch=4, lr=lr, seed=0)
+    step = p.astype(np.float64) - c.params.astype(np.float64)
+    o = pyoracle.CnnOracle()
+    out, h1, h2 = o.forward(p, x, bf16=True)
+    d = torch.tensor(out.astype(np.float64) - y.astype(np.float64), requires_grad=True)
+    loss = alpha * (d ** 2).sum() + sum((torch.diff(d, dim=a) ** 2).sum() for a in (1, 2, 3))
+    (loss / d.numel()).backward()
+    g = o.backward(p, x, h1, h2, d.grad.numpy(), bf16=True)
+    big = np.abs(g) > 0.05 * np.abs(g).max()
+    agree = float(np.mean(np.sign(step[big]) == np.sign(g[big])))
+    print("params compared", int(big.sum()), "sign agreement", agree, flush=True)
+    assert big.sum() > 100 and agree > 0.98
+    assert np.allclose(np.abs(step[big]), lr, rtol=2e-2)
+
+
 # ---------------------------------------------------------------- learned FCG
 
 def _fcg_learned_restated(H, cnn_oracle, params, dt, Tn, tol, maxit, blockstep=True, nlearn=0):
