@@ -621,14 +621,15 @@ __global__ void __launch_bounds__(256) k_cnn_loss(const float *__restrict__ out,
 }
 
 // Gradient-weighted error per 16^3 image, d = out - y:
-//   L = alpha sum_cells d^2 + sum over the faces inside the image (d_a - d_b)^2,
-// an energy-like norm of the diffusion operator (mass term alpha, the face
-// couplings unit) that weighs the error's variation instead of its offset.
+//   L = alpha sum_cells d^2 + sum over the faces inside the image w_a (d_a - d_b)^2,
+// w_a the weight of the face's axis (image axes i, j, k = r, theta, phi): an
+// energy-like norm of the diffusion operator (mass term alpha, face couplings
+// w) that weighs the error's variation instead of its offset.
 // g3 = (dL/d out) * sc; loss partials per workgroup.  One workgroup walks
 // images blockIdx.x, blockIdx.x + gridDim.x, ...
 __global__ void __launch_bounds__(256) k_cnn_loss_h1(const float *__restrict__ out, const float *__restrict__ y,
-                                                     int nimg, float alpha, float sc, float *__restrict__ g3,
-                                                     double *__restrict__ part) {
+                                                     int nimg, float alpha, float wi, float wj, float wk, float sc,
+                                                     float *__restrict__ g3, double *__restrict__ part) {
     __shared__ float dl[4096];
     __shared__ double red[4];
     double acc = 0.0;
@@ -643,12 +644,12 @@ __global__ void __launch_bounds__(256) k_cnn_loss_h1(const float *__restrict__ o
             double l = (double)alpha * (double)d * (double)d;
             // each face enters the loss once (from its lower cell) and the
             // gradient of both cells
-            if (i > 0) g += d - dl[c - 256];
-            if (i < 15) { const float e = d - dl[c + 256]; g += e; l += (double)e * (double)e; }
-            if (j > 0) g += d - dl[c - 16];
-            if (j < 15) { const float e = d - dl[c + 16]; g += e; l += (double)e * (double)e; }
-            if (k > 0) g += d - dl[c - 1];
-            if (k < 15) { const float e = d - dl[c + 1]; g += e; l += (double)e * (double)e; }
+            if (i > 0) g += wi * (d - dl[c - 256]);
+            if (i < 15) { const float e = d - dl[c + 256]; g += wi * e; l += (double)wi * ((double)e * (double)e); }
+            if (j > 0) g += wj * (d - dl[c - 16]);
+            if (j < 15) { const float e = d - dl[c + 16]; g += wj * e; l += (double)wj * ((double)e * (double)e); }
+            if (k > 0) g += wk * (d - dl[c - 1]);
+            if (k < 15) { const float e = d - dl[c + 1]; g += wk * e; l += (double)wk * ((double)e * (double)e); }
             g3[o + c] = sc * (2.0f * g);
             acc += l;
         }
@@ -811,7 +812,8 @@ double CnnModel::train_step(const float *x, const float *y, int nimg, double lr,
     if (h1_alpha_ >= 0.0) {
         const int nwg = nimg < 1024 ? nimg : 1024;
         hipLaunchKernelGGL(k_cnn_loss_h1, dim3(nwg), dim3(256), 0, stream, (const float *)out_, y, nimg,
-                           (float)h1_alpha_, (float)(1.0 / (double)n), g3_, loss_part_);
+                           (float)h1_alpha_, (float)h1_w_[0], (float)h1_w_[1], (float)h1_w_[2],
+                           (float)(1.0 / (double)n), g3_, loss_part_);
         hipLaunchKernelGGL(k_cnn_sum, dim3(1), dim3(64), 0, stream, (const double *)loss_part_, nwg, 1.0 / (double)n,
                            loss_part_ + 1024);
     } else {

@@ -41,8 +41,12 @@ class CnnModel : public HeatCorrector {
                  hipStream_t stream);
     int max_batch() const { return max_batch_; }
     // training loss: alpha < 0 the mean squared error (default); alpha >= 0 the
-    // gradient-weighted error alpha |d|^2 + |grad d|^2 per image (k_cnn_loss_h1)
-    void set_loss_h1(double alpha) { h1_alpha_ = alpha; }
+    // gradient-weighted error alpha |d|^2 + sum_a w_a |d_a d|^2 per image, the
+    // face weights w per image axis (r, theta, phi) (k_cnn_loss_h1)
+    void set_loss_h1(double alpha, double wr, double wt, double wp) {
+        h1_alpha_ = alpha;
+        h1_w_[0] = wr; h1_w_[1] = wt; h1_w_[2] = wp;
+    }
 
   private:
     void ensure_batch(int nimg);
@@ -57,7 +61,7 @@ class CnnModel : public HeatCorrector {
     float *out_ = nullptr, *g3_ = nullptr, *part_ = nullptr, *xb_ = nullptr, *yb_ = nullptr;
     double *loss_part_ = nullptr;
     long long adam_t_ = 0;
-    double h1_alpha_ = -1.0;
+    double h1_alpha_ = -1.0, h1_w_[3] = {1.0, 1.0, 1.0};
 };
 
 }  // namespace falcon

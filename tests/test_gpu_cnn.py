@@ -97,15 +97,15 @@ def test_gradient_weighted_loss_step():
     rng = np.random.default_rng(11)
     x = rng.standard_normal((4, 16, 16, 16)).astype(np.float32)
     y = (0.3 * rng.standard_normal((4, 16, 16, 16))).astype(np.float32)
-    alpha, lr = 0.05, 1e-4
-    c.set_loss(alpha)
+    alpha, lr, w = 0.05, 1e-4, (3.0, 1.0, 0.5)
+    c.set_loss(alpha, w)
     print("[cnn h1 loss step] 4 images", flush=True)
     c.train(x, y, epochs=1, batch=4, lr=lr, seed=0)
     step = p.astype(np.float64) - c.params.astype(np.float64)
     o = pyoracle.CnnOracle()
     out, h1, h2 = o.forward(p, x, bf16=True)
     d = torch.tensor(out.astype(np.float64) - y.astype(np.float64), requires_grad=True)
-    loss = alpha * (d ** 2).sum() + sum((torch.diff(d, dim=a) ** 2).sum() for a in (1, 2, 3))
+    loss = alpha * (d ** 2).sum() + sum(w[a - 1] * (torch.diff(d, dim=a) ** 2).sum() for a in (1, 2, 3))
     (loss / d.numel()).backward()
     g = o.backward(p, x, h1, h2, d.grad.numpy(), bf16=True)
     big = np.abs(g) > 0.05 * np.abs(g).max()
