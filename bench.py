@@ -78,6 +78,8 @@ def main():
                     help="training loss: < 0 mean squared error; >= 0 gradient-weighted error with this mass weight")
     ap.add_argument("--learned-loss-w", default="1,1,1",
                     help="face weights (r, theta, phi) of the gradient-weighted training loss")
+    ap.add_argument("--learned-loss-scale-free", type=int, default=0,
+                    help="1: train on the scale-free form of the gradient-weighted loss")
     ap.add_argument("--learned-iters", type=int, default=0,
                     help="directions built with the learned correction before the plain V-cycle takes over (0: all)")
     ap.add_argument("--legs", default="rk,heat,learned",
@@ -130,7 +132,8 @@ def main():
                               epochs=args.learned_epochs, sweeps=tuple(int(v) for v in args.learned_sweeps.split(",")),
                               blockstep=bool(args.learned_blockstep), learned_iters=args.learned_iters,
                               loss_h1=args.learned_loss_h1,
-                              loss_w=tuple(float(v) for v in args.learned_loss_w.split(",")))
+                              loss_w=tuple(float(v) for v in args.learned_loss_w.split(",")),
+                              loss_scale_free=bool(args.learned_loss_scale_free))
         print(f"# learned: {json.dumps(learned)}", file=sys.stderr, flush=True)
         heat = heat or {}
         heat["learned_fcg"] = learned
@@ -316,7 +319,7 @@ LEARNED_WORKLOAD = "heat_amr_32x64x192_b16_x32"
 
 
 def run_learned(npairs=4000, blocks_per_pair=16, epochs=60, batch=128, lr=2e-3, repeats=3, sweeps=(2, 2), blockstep=True,
-                learned_iters=0, loss_h1=0.01, loss_w=(1.0, 1.0, 1.0)):
+                learned_iters=0, loss_h1=0.01, loss_w=(1.0, 1.0, 1.0), loss_scale_free=False):
     """32 concurrent backward-Euler solves on the locally refined shell
     (level-0 32x64x192 per panel in 16^3 blocks, seeded 20% of blocks to level
     1 and 5% to level 2), dt spread geometrically over 10-300 x the explicit
@@ -334,7 +337,7 @@ def run_learned(npairs=4000, blocks_per_pair=16, epochs=60, batch=128, lr=2e-3, 
     hs.set_learned_sweeps(*sweeps)
     hs.set_learned_blockstep(blockstep)
     hs.set_learned_iters(learned_iters)
-    hs.set_training_loss(loss_h1, loss_w)
+    hs.set_training_loss(loss_h1, loss_w, loss_scale_free)
     loss, pairs, train_s = hs.train_corrector(npairs=npairs, blocks_per_pair=blocks_per_pair, dt_range=(10.0, 300.0),
                                               max_solves=20, epochs=epochs, batch=batch, lr=lr, seed=12345)
     dts = hs.dt_explicit * 10.0 * 30.0 ** (np.arange(nsys) / (nsys - 1))
@@ -346,7 +349,8 @@ def run_learned(npairs=4000, blocks_per_pair=16, epochs=60, batch=128, lr=2e-3, 
            "learned_finest_sweeps": list(sweeps), "learned_blockstep": bool(blockstep),
            "learned_iters": int(learned_iters),
            "training_loss": "mse" if loss_h1 < 0 else {"gradient_weighted_mass": float(loss_h1),
-                                                        "face_weights_r_theta_phi": list(loss_w)},
+                                                        "face_weights_r_theta_phi": list(loss_w),
+                                                        "scale_free": bool(loss_scale_free)},
            "training": {"pairs": pairs, "blocks_per_pair": blocks_per_pair, "epochs": epochs, "batch": batch, "lr": lr,
                         "final_mse": loss, "seconds": train_s}}
     for name, learned in (("mg_pcg", False), ("learned_fcg", True)):

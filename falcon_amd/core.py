@@ -86,7 +86,7 @@ def load():
         "falcon_heat_set_graph": [vp, C.c_int],
         "falcon_heat_set_learned_blockstep": [vp, C.c_int],
         "falcon_heat_set_learned_iters": [vp, C.c_int],
-        "falcon_cnn_set_loss": [vp, C.c_double, C.c_double, C.c_double, C.c_double],
+        "falcon_cnn_set_loss": [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int],
         "falcon_last_error": [C.c_char_p, C.c_int],
     }
     for name, args in sig.items():
@@ -354,11 +354,13 @@ class HeatSolver:
         """Learned correction step per 16^3 block (on, default) or per system (off)."""
         _check(self.L.falcon_heat_set_learned_blockstep(self.s.h, 1 if on else 0))
 
-    def set_training_loss(self, h1_alpha=-1.0, weights=(1.0, 1.0, 1.0)):
+    def set_training_loss(self, h1_alpha=-1.0, weights=(1.0, 1.0, 1.0), scale_free=False):
         """Network training loss: h1_alpha < 0 mean squared error (default);
         h1_alpha >= 0 the gradient-weighted error h1_alpha |d|^2 + sum_a w_a
-        |d_a d|^2, weights = (w_r, w_theta, w_phi)."""
-        _check(self.L.falcon_cnn_set_loss(self.s.h, float(h1_alpha), *[float(w) for w in weights]))
+        |d_a d|^2, weights = (w_r, w_theta, w_phi); scale_free: the fraction
+        of the target's norm that the best multiple of the output leaves."""
+        _check(self.L.falcon_cnn_set_loss(self.s.h, float(h1_alpha), *[float(w) for w in weights],
+                                          1 if scale_free else 0))
 
     def set_learned_iters(self, n):
         """Build the first n search directions of a learned solve with the learned
@@ -392,11 +394,13 @@ class Cnn:
         assert p.size == self.NPARAM
         _check(self.L.falcon_cnn_set_params(self.s.h, _p(p, C.c_float)))
 
-    def set_loss(self, h1_alpha=-1.0, weights=(1.0, 1.0, 1.0)):
+    def set_loss(self, h1_alpha=-1.0, weights=(1.0, 1.0, 1.0), scale_free=False):
         """Training loss: h1_alpha < 0 mean squared error (default); h1_alpha >= 0
         the gradient-weighted error h1_alpha |d|^2 + sum_a w_a |d_a d|^2 per
-        image, weights = (w_r, w_theta, w_phi)."""
-        _check(self.L.falcon_cnn_set_loss(self.s.h, float(h1_alpha), *[float(w) for w in weights]))
+        image, weights = (w_r, w_theta, w_phi); scale_free: the fraction of the
+        target's norm that the best multiple of the output leaves."""
+        _check(self.L.falcon_cnn_set_loss(self.s.h, float(h1_alpha), *[float(w) for w in weights],
+                                          1 if scale_free else 0))
 
     def forward(self, x):
         x = np.ascontiguousarray(x, dtype=np.float32)

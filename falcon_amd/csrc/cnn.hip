@@ -661,6 +661,69 @@ __global__ void __launch_bounds__(256) k_cnn_loss_h1(const float *__restrict__ o
     if (threadIdx.x == 0) part[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
+// Scale-free form of the same norm, per image: with A = alpha I + sum_a w_a
+// (the face-difference operator along axis a), the fraction of e's A-norm that
+// the best multiple of c leaves,
+//   L = 1 - <c,Ae>^2 / (<c,Ac> <e,Ae>),
+// which is what the FCG step with its A-optimal scale a rewards.  dL/dc =
+// -2 <c,Ae> / (<c,Ac><e,Ae>) (Ae - (<c,Ae>/<c,Ac>) Ac); g3 = dL/dc * sc.
+// Images with <c,Ac> or <e,Ae> zero contribute nothing.
+__global__ void __launch_bounds__(256) k_cnn_loss_cos(const float *__restrict__ out, const float *__restrict__ y,
+                                                      int nimg, float alpha, float wi, float wj, float wk, float sc,
+                                                      float *__restrict__ g3, double *__restrict__ part) {
+    __shared__ float cl[4096], el[4096];
+    __shared__ double red[3][4];
+    double lacc = 0.0;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int img = blockIdx.x; img < nimg; img += gridDim.x) {
+        const long long o = (long long)img * 4096;
+        for (int c = threadIdx.x; c < 4096; c += 256) {
+            cl[c] = out[o + c];
+            el[c] = y[o + c];
+        }
+        __syncthreads();
+        // A v at cell c, v in LDS
+        auto apply = [&](const float *v, int c) -> double {
+            const int i = c >> 8, j = (c >> 4) & 15, k = c & 15;
+            const double x = v[c];
+            double a = (double)alpha * x;
+            if (i > 0) a += (double)wi * (x - v[c - 256]);
+            if (i < 15) a += (double)wi * (x - v[c + 256]);
+            if (j > 0) a += (double)wj * (x - v[c - 16]);
+            if (j < 15) a += (double)wj * (x - v[c + 16]);
+            if (k > 0) a += (double)wk * (x - v[c - 1]);
+            if (k < 15) a += (double)wk * (x - v[c + 1]);
+            return a;
+        };
+        double s_ce = 0.0, s_cc = 0.0, s_ee = 0.0;
+        for (int c = threadIdx.x; c < 4096; c += 256) {
+            const double ae = apply(el, c), ac = apply(cl, c);
+            s_ce += (double)cl[c] * ae;
+            s_cc += (double)cl[c] * ac;
+            s_ee += (double)el[c] * ae;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            s_ce += __shfl_down(s_ce, off, 64);
+            s_cc += __shfl_down(s_cc, off, 64);
+            s_ee += __shfl_down(s_ee, off, 64);
+        }
+        if (lane == 0) { red[0][wv] = s_ce; red[1][wv] = s_cc; red[2][wv] = s_ee; }
+        __syncthreads();
+        const double ce = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
+        const double cc = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
+        const double ee = ((red[2][0] + red[2][1]) + red[2][2]) + red[2][3];
+        const bool ok = cc > 0.0 && ee > 0.0;
+        const double f = ok ? -2.0 * ce / (cc * ee) : 0.0, r = ok ? ce / cc : 0.0;
+        for (int c = threadIdx.x; c < 4096; c += 256) {
+            const double g = ok ? f * (apply(el, c) - r * apply(cl, c)) : 0.0;
+            g3[o + c] = (float)(g * (double)sc);
+        }
+        if (threadIdx.x == 0 && ok) lacc += 1.0 - ce * ce / (cc * ee);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = lacc;
+}
+
 __global__ void k_cnn_sum(const double *__restrict__ part, int n, double scale, double *__restrict__ out) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     double s = 0.0;
@@ -809,7 +872,15 @@ double CnnModel::train_step(const float *x, const float *y, int nimg, double lr,
     hipLaunchKernelGGL((k_cnn_fwd<true, false, kFwdWaves>), dim3(nimg), dim3(64 * kFwdWaves), 0, stream, x, out_,
                        (const uint16_t *)packed_ + PK_TOTAL * 8, (const float *)params_, h1_, h2_, HeatIO{});
     const long long n = 4096LL * nimg;
-    if (h1_alpha_ >= 0.0) {
+    if (h1_alpha_ >= 0.0 && loss_cos_) {
+        // mean over the images of the scale-free loss
+        const int nwg = nimg < 1024 ? nimg : 1024;
+        hipLaunchKernelGGL(k_cnn_loss_cos, dim3(nwg), dim3(256), 0, stream, (const float *)out_, y, nimg,
+                           (float)h1_alpha_, (float)h1_w_[0], (float)h1_w_[1], (float)h1_w_[2],
+                           (float)(1.0 / (double)nimg), g3_, loss_part_);
+        hipLaunchKernelGGL(k_cnn_sum, dim3(1), dim3(64), 0, stream, (const double *)loss_part_, nwg,
+                           1.0 / (double)nimg, loss_part_ + 1024);
+    } else if (h1_alpha_ >= 0.0) {
         const int nwg = nimg < 1024 ? nimg : 1024;
         hipLaunchKernelGGL(k_cnn_loss_h1, dim3(nwg), dim3(256), 0, stream, (const float *)out_, y, nimg,
                            (float)h1_alpha_, (float)h1_w_[0], (float)h1_w_[1], (float)h1_w_[2],

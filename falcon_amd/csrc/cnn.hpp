@@ -43,9 +43,12 @@ class CnnModel : public HeatCorrector {
     // training loss: alpha < 0 the mean squared error (default); alpha >= 0 the
     // gradient-weighted error alpha |d|^2 + sum_a w_a |d_a d|^2 per image, the
     // face weights w per image axis (r, theta, phi) (k_cnn_loss_h1)
-    void set_loss_h1(double alpha, double wr, double wt, double wp) {
+    // scale_free: the per-image fraction of e's norm that the best multiple of
+    // the output leaves (k_cnn_loss_cos) instead of the norm of the difference
+    void set_loss_h1(double alpha, double wr, double wt, double wp, bool scale_free) {
         h1_alpha_ = alpha;
         h1_w_[0] = wr; h1_w_[1] = wt; h1_w_[2] = wp;
+        loss_cos_ = scale_free;
     }
 
   private:
@@ -62,6 +65,7 @@ class CnnModel : public HeatCorrector {
     double *loss_part_ = nullptr;
     long long adam_t_ = 0;
     double h1_alpha_ = -1.0, h1_w_[3] = {1.0, 1.0, 1.0};
+    bool loss_cos_ = false;
 };
 
 }  // namespace falcon

@@ -1306,11 +1306,13 @@ int falcon_heat_set_learned_blockstep(falcon_handle_t h, int on) {
     return guard([&] { need_heat(h).set_learned_blockstep(on != 0); });
 }
 
-int falcon_cnn_set_loss(falcon_handle_t h, double h1_alpha, double w_r, double w_t, double w_p) {
+int falcon_cnn_set_loss(falcon_handle_t h, double h1_alpha, double w_r, double w_t, double w_p, int scale_free) {
     return guard([&] {
         if (h1_alpha >= 0.0 && !(w_r >= 0.0 && w_t >= 0.0 && w_p >= 0.0))
             throw Error("loss face weights must be >= 0");
-        need_cnn(h).set_loss_h1(h1_alpha, w_r, w_t, w_p);
+        if (h1_alpha >= 0.0 && scale_free && !(h1_alpha > 0.0))
+            throw Error("the scale-free loss needs a positive mass weight");
+        need_cnn(h).set_loss_h1(h1_alpha, w_r, w_t, w_p, scale_free != 0);
     });
 }
 

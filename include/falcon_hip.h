@@ -150,11 +150,14 @@ int falcon_heat_set_learned_blockstep(falcon_handle_t h, int on);
  * counterpart in the reference (learned V-cycle correction). */
 int falcon_heat_set_learned_iters(falcon_handle_t h, int n);
 /* training loss of the network: h1_alpha < 0 the mean squared error
- * (default); h1_alpha >= 0 the gradient-weighted error
- * h1_alpha |d|^2 + sum_a w_a |differences of d along axis a|^2 over each 16^3
- * image (d = output - target; axes r, theta, phi with weights w_r, w_t, w_p).
- * No counterpart in the reference (learned V-cycle correction). */
-int falcon_cnn_set_loss(falcon_handle_t h, double h1_alpha, double w_r, double w_t, double w_p);
+ * (default); h1_alpha >= 0 the gradient-weighted norm
+ * |v|_A^2 = h1_alpha |v|^2 + sum_a w_a |differences of v along axis a|^2 over
+ * each 16^3 image (axes r, theta, phi with weights w_r, w_t, w_p) of
+ * d = output - target, or with scale_free (h1_alpha > 0) the fraction
+ * 1 - <c,Ae>^2/(|c|_A^2 |e|_A^2) of the target's norm that the best multiple
+ * of the output c leaves.  No counterpart in the reference (learned V-cycle
+ * correction). */
+int falcon_cnn_set_loss(falcon_handle_t h, double h1_alpha, double w_r, double w_t, double w_p, int scale_free);
 /* y = A x for all systems (parity tests) */
 int falcon_heat_apply(falcon_handle_t h, const double *x, double *y);
 /* z = M r: one V-cycle of the preconditioner for all systems, zero initial

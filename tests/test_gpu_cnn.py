@@ -84,11 +84,12 @@ def test_training_fits_a_teacher():
     assert loss < 0.2 * power
 
 
-def test_gradient_weighted_loss_step():
+@pytest.mark.parametrize("scale_free", [False, True])
+def test_gradient_weighted_loss_step(scale_free):
     """One Adam step from fresh optimiser state moves every parameter by
-    -lr sign(g).  Under the gradient-weighted loss the GPU step's signs equal
+    -lr sign(g).  Under the gradient-weighted losses the GPU step's signs equal
     those of the oracle's backward of the loss gradient, with the loss written
-    as axis differences and differentiated by torch autograd."""
+    with axis differences and differentiated by torch autograd."""
     import torch
     s = _shell()
     c = Cnn(s, seed=3)
@@ -98,21 +99,33 @@ def test_gradient_weighted_loss_step():
     x = rng.standard_normal((4, 16, 16, 16)).astype(np.float32)
     y = (0.3 * rng.standard_normal((4, 16, 16, 16))).astype(np.float32)
     alpha, lr, w = 0.05, 1e-4, (3.0, 1.0, 0.5)
-    c.set_loss(alpha, w)
-    print("[cnn h1 loss step] 4 images", flush=True)
+    c.set_loss(alpha, w, scale_free)
+    print(f"[cnn h1 loss step] 4 images scale_free={scale_free}", flush=True)
     c.train(x, y, epochs=1, batch=4, lr=lr, seed=0)
     step = p.astype(np.float64) - c.params.astype(np.float64)
     o = pyoracle.CnnOracle()
     out, h1, h2 = o.forward(p, x, bf16=True)
-    d = torch.tensor(out.astype(np.float64) - y.astype(np.float64), requires_grad=True)
-    loss = alpha * (d ** 2).sum() + sum(w[a - 1] * (torch.diff(d, dim=a) ** 2).sum() for a in (1, 2, 3))
-    (loss / d.numel()).backward()
-    g = o.backward(p, x, h1, h2, d.grad.numpy(), bf16=True)
+    cv = torch.tensor(out.astype(np.float64), requires_grad=True)
+    e = torch.tensor(y.astype(np.float64))
+
+    def inner(u, v):         # per image: alpha <u,v> + sum_a w_a <diff_a u, diff_a v>
+        s = alpha * (u * v).sum(dim=(1, 2, 3))
+        for a in (1, 2, 3):
+            s = s + w[a - 1] * (torch.diff(u, dim=a) * torch.diff(v, dim=a)).sum(dim=(1, 2, 3))
+        return s
+
+    if scale_free:
+        loss = (1.0 - inner(cv, e) ** 2 / (inner(cv, cv) * inner(e, e))).mean()
+    else:
+        loss = inner(cv - e, cv - e).sum() / cv.numel()
+    loss.backward()
+    g = o.backward(p, x, h1, h2, cv.grad.numpy(), bf16=True)
     big = np.abs(g) > 0.05 * np.abs(g).max()
     agree = float(np.mean(np.sign(step[big]) == np.sign(g[big])))
     print("params compared", int(big.sum()), "sign agreement", agree, flush=True)
     assert big.sum() > 100 and agree > 0.98
-    assert np.allclose(np.abs(step[big]), lr, rtol=2e-2)
+    # Adam's first step is lr |g|/(|g| + 1e-8): lr up to the epsilon
+    assert np.allclose(np.abs(step[big]), lr, rtol=0.1)
 
 
 # ---------------------------------------------------------------- learned FCG
