@@ -47,6 +47,7 @@ def load():
         "falcon_gc_ntargets": [vp, C.c_int, ip],
         "falcon_gc_target": [vp, C.c_int, C.c_int, ip, ip, ip],
         "falcon_gc_entries": [vp, i64p, ip, ip, ip, ip, dp, ip, dp],
+        "falcon_ghost_pull_table": [vp, ip, i64p, i64p],
         "falcon_set_background": [vp, C.c_int, dp],
         "falcon_background_polytrope": [vp, C.c_double, C.c_double],
         "falcon_get_background": [vp, C.c_int, dp],
@@ -197,6 +198,15 @@ class Shell:
         _check(self.L.falcon_gc_entries(self.h, C.byref(nn), _p(recv, C.c_int32), _p(donor, C.c_int32),
                                         _p(gijk, C.c_int32), _p(bijk, C.c_int32), _p(w), _p(yy, C.c_int32), _p(rot)))
         return dict(recv=recv, donor=donor, gijk=gijk, bijk=bijk, w=w, yy=yy, rot=rot)
+
+    def ghost_pull_table(self):
+        """FP32 step's pull table (nblocks, 27): the same-level block whose
+        interior holds each direction's ghosts, or -1; with the number of FP32
+        copy entries and of those the ghost fill still runs."""
+        pull = np.zeros((self.nblocks, 27), np.int32)
+        nc, nr = C.c_int64(), C.c_int64()
+        _check(self.L.falcon_ghost_pull_table(self.h, _p(pull, C.c_int32), C.byref(nc), C.byref(nr)))
+        return pull, nc.value, nr.value
 
     # ---- data
     def set_background(self, ib, bg):

@@ -163,6 +163,113 @@ void device_to_fortran(const double *src, double *dst, int NI, int NJ, int NK) {
                     dst[v * P + (size_t)I + (size_t)NI * (J + (size_t)NJ * K)] = src[v * P + ((size_t)I * NJ + J) * NK + K];
 }
 
+// FP32 buffers of the deviation path: same-panel ghost entries whose weights
+// lie within 1e-12 of 0 or 1 (same-level neighbours whose cell centres
+// coincide up to rounding) are plain copies from the nearest donor cell;
+// same-panel entries with only the r weight away from 0/1 interpolate along r.
+bool fp32_near(const GhostEntry &g) {
+    if (g.yy) return false;
+    for (int a = 0; a < 3; a++)
+        if (!(g.w[a] < 1e-12 || g.w[a] > 1.0 - 1e-12)) return false;
+    return true;
+}
+bool fp32_lerp_r(const GhostEntry &g) {
+    auto snapped = [](double w) { return w < 1e-12 || w > 1.0 - 1e-12; };
+    return !g.yy && !snapped(g.w[0]) && snapped(g.w[1]) && snapped(g.w[2]);
+}
+
+// (dst, src) of the copies and (dst, base) with the r weight of the radial
+// lerps, each sorted by destination; offsets block*8*P + padded cell.
+void fp32_copy_lerp(const falcon_handle *h, std::vector<int2> &copy32, std::vector<std::pair<int2, double>> &lerp32) {
+    const int NJ = h->NJ(), NK = h->NK(), ng = h->lay.tree.ng;
+    const long long P = (long long)h->NI() * NJ * NK, SI = (long long)NJ * NK, SJ = NK;
+    auto cell = [&](int i, int j, int k) -> long long {
+        return ((long long)(i + ng - 1) * NJ + (j + ng - 1)) * NK + (k + ng - 1);
+    };
+    copy32.clear();
+    lerp32.clear();
+    for (const GhostEntry &g : h->lay.ghosts) {
+        const int32_t d0 = (int32_t)((long long)g.recv_block * 8 * P + cell(g.gi, g.gj, g.gk));
+        const long long s0 = (long long)g.donor * 8 * P + cell(g.i0, g.j0, g.k0) + (g.w[1] > 0.5 ? SJ : 0) +
+                             (g.w[2] > 0.5 ? 1 : 0);
+        if (fp32_near(g)) copy32.push_back(make_int2(d0, (int32_t)(s0 + (g.w[0] > 0.5 ? SI : 0))));
+        else if (fp32_lerp_r(g)) lerp32.push_back({make_int2(d0, (int32_t)s0), g.w[0]});
+    }
+    std::sort(copy32.begin(), copy32.end(), [](const int2 &a, const int2 &b) { return a.x < b.x; });
+    std::sort(lerp32.begin(), lerp32.end(), [](const auto &a, const auto &b) { return a.first.x < b.first.x; });
+}
+
+// Pull table of the FP32 step.  A copy entry whose ghost (padded index g) and
+// donor cell (padded index p, interior) differ by 0 or +-n per axis is a
+// same-level translation; seen from the donor D the receiver lies in direction
+// dir = -(g - p)/n.  (D, dir) qualifies when all ghosts of the image of D's
+// band for dir (ng wide along every axis with dir != 0, n along the others)
+// are such entries with one receiving block R: then R reads its ghosts in
+// direction -dir from D, pull[R*27 + code(-dir)] = D (code(d) = (d0+1)*9 +
+// (d1+1)*3 + (d2+1); -1 elsewhere), and those entries leave the copies the
+// step runs (rest, in copy order).
+void build_pull_table(const falcon_handle *h, const std::vector<int2> &copy32, std::vector<int32_t> &pull,
+                      std::vector<int2> &rest) {
+    const Tree &t = h->lay.tree;
+    const int nb = t.nleaf, ng = t.ng, NJ = h->NJ(), NK = h->NK();
+    const long long P = (long long)h->NI() * NJ * NK, SI = (long long)NJ * NK, SJ = NK;
+    const int n3[3] = {t.ni, t.nj, t.nk};
+    const bool fits = n3[0] >= 2 * ng && n3[1] >= 2 * ng && n3[2] >= 2 * ng;
+    std::vector<int32_t> recv((size_t)nb * 27, -1), cnt((size_t)nb * 27, 0);
+    std::vector<char> bad((size_t)nb * 27, 0);
+    auto decode = [&](long long off, int &blk, int p[3]) {
+        blk = (int)(off / (8 * P));
+        const long long c = off - (long long)blk * 8 * P;
+        p[0] = (int)(c / SI);
+        p[1] = (int)((c / SJ) % NJ);
+        p[2] = (int)(c % SJ);
+    };
+    // donor-side direction key of a translation entry, or -1
+    auto key_of = [&](const int2 &e, int &R) -> long long {
+        int D, pg[3], pd[3], dir[3];
+        decode(e.x, R, pg);
+        decode(e.y, D, pd);
+        for (int a = 0; a < 3; a++) {
+            if (pd[a] < ng || pd[a] >= ng + n3[a]) return -1;
+            const int o = pg[a] - pd[a];
+            if (o == 0) dir[a] = 0;
+            else if (o == n3[a]) dir[a] = -1;
+            else if (o == -n3[a]) dir[a] = 1;
+            else return -1;
+        }
+        if (!dir[0] && !dir[1] && !dir[2]) return -1;
+        return (long long)D * 27 + (dir[0] + 1) * 9 + (dir[1] + 1) * 3 + (dir[2] + 1);
+    };
+    if (fits) {
+        for (const int2 &e : copy32) {
+            int R;
+            const long long k = key_of(e, R);
+            if (k < 0) continue;
+            if (recv[k] == -1) recv[k] = R;
+            else if (recv[k] != R) bad[k] = 1;
+            cnt[k]++;
+        }
+    }
+    std::vector<int32_t> by_donor((size_t)nb * 27, -1);
+    pull.assign((size_t)nb * 27, -1);
+    for (long long k = 0; k < (long long)nb * 27; k++) {
+        const int code = (int)(k % 27);
+        const int dir[3] = {code / 9 - 1, (code / 3) % 3 - 1, code % 3 - 1};
+        long long band = 1;
+        for (int a = 0; a < 3; a++) band *= dir[a] ? ng : n3[a];
+        if (fits && recv[k] >= 0 && !bad[k] && cnt[k] == band) {
+            by_donor[k] = recv[k];
+            pull[(size_t)recv[k] * 27 + (26 - code)] = (int32_t)(k / 27);
+        }
+    }
+    rest.clear();
+    for (const int2 &e : copy32) {
+        int R;
+        const long long k = fits ? key_of(e, R) : -1;
+        if (k < 0 || by_donor[k] != R) rest.push_back(e);
+    }
+}
+
 void upload_tables(falcon_handle *h) {
     const Tree &t = h->lay.tree;
     const int nb = t.nleaf, NI = h->NI(), NJ = h->NJ(), NK = h->NK(), ng = t.ng;
@@ -234,32 +341,11 @@ void upload_tables(falcon_handle *h) {
     if ((long long)nb * 8 * P >= (1ll << 31)) throw Error("state too large for 32-bit ghost offsets");
     // FP64 buffers: every entry weighted, same-panel entries first, then
     // Yin-Yang, each in target-list order.  FP32 buffers of the deviation path:
-    // same-panel entries whose weights lie within 1e-12 of 0 or 1 (same-level
-    // neighbours whose cell centres coincide up to rounding) become plain copies
-    // from the nearest donor cell, sorted by destination; the rest keep the
-    // weighted path in the same order.
-    const double eps = 1e-12;
-    auto near = [&](const GhostEntry &g) {
-        if (g.yy) return false;
-        for (int a = 0; a < 3; a++)
-            if (!(g.w[a] < eps || g.w[a] > 1.0 - eps)) return false;
-        return true;
-    };
-    auto snapped = [&](double w) { return w < eps || w > 1.0 - eps; };
-    auto lerp_r = [&](const GhostEntry &g) { return !g.yy && !snapped(g.w[0]) && snapped(g.w[1]) && snapped(g.w[2]); };
-    const long long SI = (long long)NJ * NK, SJ = NK;
+    // near copies and radial lerps (fp32_copy_lerp), the rest weighted in the
+    // same order.
     std::vector<int2> copy32;
     std::vector<std::pair<int2, double>> lerp32;
-    for (size_t e = 0; e < G.size(); e++) {
-        const GhostEntry &g = G[e];
-        const int32_t d0 = (int32_t)((long long)g.recv_block * 8 * P + cell(g.gi, g.gj, g.gk));
-        const long long s0 = (long long)g.donor * 8 * P + cell(g.i0, g.j0, g.k0) + (g.w[1] > 0.5 ? SJ : 0) +
-                             (g.w[2] > 0.5 ? 1 : 0);
-        if (near(g)) copy32.push_back(make_int2(d0, (int32_t)(s0 + (g.w[0] > 0.5 ? SI : 0))));
-        else if (lerp_r(g)) lerp32.push_back({make_int2(d0, (int32_t)s0), g.w[0]});
-    }
-    std::sort(copy32.begin(), copy32.end(), [](const int2 &a, const int2 &b) { return a.x < b.x; });
-    std::sort(lerp32.begin(), lerp32.end(), [](const auto &a, const auto &b) { return a.first.x < b.first.x; });
+    fp32_copy_lerp(h, copy32, lerp32);
     std::vector<int2> lerp_idx(lerp32.size());
     std::vector<double> lerp_w(lerp32.size());
     for (size_t q = 0; q < lerp32.size(); q++) { lerp_idx[q] = lerp32[q].first; lerp_w[q] = lerp32[q].second; }
@@ -293,7 +379,7 @@ void upload_tables(falcon_handle *h) {
                          0, (int)NG, yy_begin, nullptr, 0, nullptr, nullptr, 0};
     };
     h->gh64 = weighted([](const GhostEntry &) { return true; }, h->ghost64);
-    h->gh32 = weighted([&](const GhostEntry &g) { return !near(g) && !lerp_r(g); }, h->ghost32);
+    h->gh32 = weighted([&](const GhostEntry &g) { return !fp32_near(g) && !fp32_lerp_r(g); }, h->ghost32);
     up(h->ghost_copy32, copy32.data(), sizeof(int2) * copy32.size());
     up(h->ghost_lerp32[0], lerp_idx.data(), sizeof(int2) * lerp_idx.size());
     up(h->ghost_lerp32[1], lerp_w.data(), sizeof(double) * lerp_w.size());
@@ -303,69 +389,10 @@ void upload_tables(falcon_handle *h) {
     h->gh32.lerp_w = h->ghost_lerp32[1].as<double>();
     h->gh32.n_lerp = (int)lerp_idx.size();
 
-    // Pull table of the FP32 step.  A copy entry whose ghost (padded index g)
-    // and donor cell (padded index p, interior) differ by 0 or +-n per axis is
-    // a same-level translation; seen from the donor D the receiver lies in
-    // direction dir = -(g - p)/n.  (D, dir) qualifies when all ghosts of the
-    // image of D's band for dir (ng wide along every axis with dir != 0, n along
-    // the others) are such entries with one receiving block R: then R reads
-    // its ghosts in direction -dir from D (pull[R][-dir] = D) and those entries
-    // leave the copy table the step runs.
     {
-        const int n3[3] = {h->lay.tree.ni, h->lay.tree.nj, h->lay.tree.nk};
-        const bool fits = n3[0] >= 2 * ng && n3[1] >= 2 * ng && n3[2] >= 2 * ng;
-        std::vector<int32_t> recv((size_t)nb * 27, -1), cnt((size_t)nb * 27, 0);
-        std::vector<char> bad((size_t)nb * 27, 0);
-        auto decode = [&](long long off, int &blk, int p[3]) {
-            blk = (int)(off / (8 * P));
-            const long long c = off - (long long)blk * 8 * P;
-            p[0] = (int)(c / SI);
-            p[1] = (int)((c / SJ) % NJ);
-            p[2] = (int)(c % SJ);
-        };
-        // donor-side direction key of a translation entry, or -1
-        auto key_of = [&](const int2 &e, int &R) -> long long {
-            int D, pg[3], pd[3], dir[3];
-            decode(e.x, R, pg);
-            decode(e.y, D, pd);
-            for (int a = 0; a < 3; a++) {
-                if (pd[a] < ng || pd[a] >= ng + n3[a]) return -1;
-                const int o = pg[a] - pd[a];
-                if (o == 0) dir[a] = 0;
-                else if (o == n3[a]) dir[a] = -1;
-                else if (o == -n3[a]) dir[a] = 1;
-                else return -1;
-            }
-            if (!dir[0] && !dir[1] && !dir[2]) return -1;
-            return (long long)D * 27 + (dir[0] + 1) * 9 + (dir[1] + 1) * 3 + (dir[2] + 1);
-        };
-        if (fits) {
-            for (const int2 &e : copy32) {
-                int R;
-                const long long k = key_of(e, R);
-                if (k < 0) continue;
-                if (recv[k] == -1) recv[k] = R;
-                else if (recv[k] != R) bad[k] = 1;
-                cnt[k]++;
-            }
-        }
-        std::vector<int32_t> by_donor((size_t)nb * 27, -1), pull((size_t)nb * 27, -1);
-        for (long long k = 0; k < (long long)nb * 27; k++) {
-            const int code = (int)(k % 27);
-            const int dir[3] = {code / 9 - 1, (code / 3) % 3 - 1, code % 3 - 1};
-            long long band = 1;
-            for (int a = 0; a < 3; a++) band *= dir[a] ? ng : n3[a];
-            if (fits && recv[k] >= 0 && !bad[k] && cnt[k] == band) {
-                by_donor[k] = recv[k];
-                pull[(size_t)recv[k] * 27 + (26 - code)] = (int32_t)(k / 27);
-            }
-        }
+        std::vector<int32_t> pull;
         std::vector<int2> rest;
-        for (const int2 &e : copy32) {
-            int R;
-            const long long k = fits ? key_of(e, R) : -1;
-            if (k < 0 || by_donor[k] != R) rest.push_back(e);
-        }
+        build_pull_table(h, copy32, pull, rest);
         up(h->ghost_rest32, rest.data(), sizeof(int2) * rest.size());
         up(h->ghost_pull32, pull.data(), sizeof(int32_t) * pull.size());
         h->gh32.copy_rest = h->ghost_rest32.as<int2>();
@@ -755,6 +782,20 @@ int falcon_gc_target(falcon_handle_t h, int ib, int it, int *donor, int *ngc, in
                 ijk[g + 2 * n] = q / (NI * NJ) - ng + 1;
             }
         }
+    });
+}
+
+int falcon_ghost_pull_table(falcon_handle_t h, int32_t *pull, int64_t *n_copy, int64_t *n_rest) {
+    return guard([&] {
+        h->need_built();
+        std::vector<int2> copy32, rest;
+        std::vector<std::pair<int2, double>> lerp32;
+        std::vector<int32_t> tab;
+        fp32_copy_lerp(h, copy32, lerp32);
+        build_pull_table(h, copy32, tab, rest);
+        if (pull) std::copy(tab.begin(), tab.end(), pull);
+        if (n_copy) *n_copy = (int64_t)copy32.size();
+        if (n_rest) *n_rest = (int64_t)rest.size();
     });
 }
 

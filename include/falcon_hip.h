@@ -67,6 +67,12 @@ int falcon_gc_target(falcon_handle_t h, int ib, int it, int *donor, int *ngc, in
  * recv block, donor block, ghost ijk[3], base ijk[3], weights[3], yy flag, rot[2]. */
 int falcon_gc_entries(falcon_handle_t h, int64_t *n, int32_t *recv, int32_t *donor, int32_t *gijk, int32_t *bijk,
                       double *w, int32_t *yy, double *rot);
+/* FP32 step's pull table (host only, after falcon_build_tables): pull (NULL or
+ * nblocks*27 int32) receives, for block R and direction d of its ghosts (entry
+ * R*27 + (d0+1)*9 + (d1+1)*3 + (d2+1)), the same-level block whose interior the
+ * kernels read those ghosts from, or -1; n_copy: the FP32 copy entries, n_rest:
+ * those the ghost fill still runs.  No counterpart in the reference. */
+int falcon_ghost_pull_table(falcon_handle_t h, int32_t *pull, int64_t *n_copy, int64_t *n_rest);
 
 /* ---------------------------------------------------------------- data
  * Background profiles per block, replacing ModBlock_InitSSM's calls to

@@ -114,3 +114,50 @@ def test_entries_weights_and_bases_in_range():
     # counts conserved: every receiving block's entries equal its targets' nGC sum
     for ib in range(s.nblocks):
         assert np.sum(e["recv"] == ib) == sum(t[1].shape[1] for t in s.gc_targets(ib))
+
+
+def _pull_restated(lay, e):
+    """The FP32 pull table restated from the receiver's side: direction s of
+    block R (s_a = -1 / +1 for ghosts below / above the interior along axis a,
+    0 inside) names donor D when every ghost of R in that region is a near
+    copy (same panel, weights within 1e-12 of 0 or 1) from D's interior cell at
+    the ghost index minus s * n, and the region is complete."""
+    ng = lay.get("ng", 2)
+    n = np.array([lay["ni"], lay["nj"], lay["nk"]])
+    w = e["w"]
+    near = (e["yy"] == 0) & np.all((w < 1e-12) | (w > 1 - 1e-12), axis=1)
+    g = e["gijk"] + ng - 1
+    p = e["bijk"] + ng - 1 + (w > 0.5)
+    side = np.where(g < ng, -1, np.where(g >= ng + n, 1, 0))
+    ok = near & np.all(p == g - side * n, axis=1) & np.all((p >= ng) & (p < ng + n), axis=1)
+    code = (side[:, 0] + 1) * 9 + (side[:, 1] + 1) * 3 + (side[:, 2] + 1)
+    nb = int(e["recv"].max()) + 1
+    pull = -np.ones((nb, 27), np.int64)
+    fits = bool(np.all(n >= 2 * ng))
+    regions = {}
+    for q in range(len(g)):
+        regions.setdefault((int(e["recv"][q]), int(code[q])), []).append(q)
+    n_pulled = 0
+    for (R, c), qs in regions.items():
+        s = np.array([c // 9 - 1, (c // 3) % 3 - 1, c % 3 - 1])
+        size = int(np.prod(np.where(s != 0, ng, n)))
+        donors = {int(e["donor"][q]) for q in qs}
+        if fits and len(qs) == size and all(ok[q] for q in qs) and len(donors) == 1:
+            pull[R, c] = donors.pop()
+            n_pulled += len(qs)
+    return pull, int(near.sum()), int(near.sum()) - n_pulled
+
+
+@pytest.mark.parametrize("name", ["mixed", "cubic8"])
+def test_ghost_pull_table_restated(name):
+    """The library's pull table (built on the host from the donor side) equals
+    the receiver-side restatement, entry for entry, with the same copy and
+    left-over counts; the 16^3 cubic shell pulls most copies."""
+    lay = {"mixed": configs.small_mixed(), "cubic8": configs.cubic((16, 32, 96), 8)}[name]
+    s = Shell(lay)
+    s.build_tables()
+    pull, n_copy, n_rest = s.ghost_pull_table()
+    ref, ref_copy, ref_rest = _pull_restated(lay, s.gc_entries())
+    assert np.array_equal(pull, ref)
+    assert (n_copy, n_rest) == (ref_copy, ref_rest)
+    assert (pull >= 0).any() and n_rest < n_copy
