@@ -130,3 +130,24 @@ def test_fp32_ghost_pull_equals_full_exchange(name):
         s.advance(6, 0.5, graph=graph)
         runs.append(_states(s, s.nblocks))
     assert np.array_equal(runs[0], runs[1]) and np.array_equal(runs[0], runs[2])
+
+
+def test_fp32_deviation_error_after_100_steps():
+    """The precision target on the 24x48x144-per-panel shell in 8^3 blocks:
+    after 100 steps (each path with its own CFL dt) every variable of the
+    FP32-deviation path is within 1e-6 of the FP64 path relative to the
+    variable's maximum over the interior."""
+    print("[fp32 100 steps] layout=shell_24x48x144_b8", flush=True)
+    lay = configs.SHELL_24x48x144_B8
+    res = {}
+    for prec in ("fp64", "fp32"):
+        s = Shell(lay)
+        s.background_polytrope(20.0)
+        s.set_gc_all()
+        s.set_precision(prec)
+        s.init_synthetic(seed=12345, s_amp=1e-4, b_ratio=1e-3)
+        s.advance(100, 0.5, graph=True)
+        res[prec] = _states(s, s.nblocks)[:, :, 2:-2, 2:-2, 2:-2]
+    err = rel_err(res["fp32"], res["fp64"])
+    print("fp32 rel err per var after 100 steps:", err, flush=True)
+    assert np.all(err <= 1e-6), err
