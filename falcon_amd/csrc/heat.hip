@@ -728,7 +728,7 @@ struct HeatSolver::Impl {
         hipLaunchKernelGGL(k_hscalars, dim3((nsys + 63) / 64), dim3(64), 0, stream, what, nsys, (const double *)dots, sc,
                            (const int *)active, ncell);
     }
-    int pre = 2, post = 2, coarse_sweeps = 8;
+    int pre = 2, post = 2, coarse_sweeps = 2;
     double omega = 0.8;
     std::vector<double> dt;
     double dt_expl = 0.0;

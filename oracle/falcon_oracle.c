@@ -954,7 +954,7 @@ void orc_vecconv_coeffs(const double x[3], double out[2]) {
  * trilinear inter/extrapolation inside the donor found by FindNode (exact for
  * fields linear in r, theta, phi, whatever the two blocks' spacings; weights
  * within 1e-12 of 0 or 1 snapped to 0 or 1); V-cycle
- * with damped Jacobi (omega 0.8, 2+2 sweeps, 8 on the 1-cell level), summed
+ * with damped Jacobi (omega 0.8, 2+2 sweeps, 2 on the 1-cell level), summed
  * restriction, injection prolongation; flexible CG (FCG(1)).  Serial FP64.
  * Arrays: Fortran order (i fastest), padded (m+2)^3 with one ghost layer.
  * ====================================================================== */
@@ -1156,7 +1156,7 @@ static void h_vcycle(orc_heat *H, hwork *W, double inv_dt, const double *r0, dou
         const int m = L->m;
         const long P = (long)(m + 2) * (m + 2) * (m + 2), M3 = (long)m * m * m;
         memset(W[l].Z, 0, sizeof(double) * (size_t)nb * P);
-        int sweeps = (l == H->nlev - 1) ? 8 : 2;
+        int sweeps = 2;                 /* every level, the 1-cell one included */
         for (int s = 0; s < sweeps; s++) {
             h_exchange(L, W[l].Z);
             memcpy(W[l].Z2, W[l].Z, sizeof(double) * (size_t)nb * P);
