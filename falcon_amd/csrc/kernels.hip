@@ -406,7 +406,14 @@ __global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const TI
                                                    const double *__restrict__ U, double *__restrict__ out,
                                                    float *__restrict__ shadow, double *__restrict__ partial,
                                                    const double *__restrict__ dtp, const int32_t *__restrict__ pull) {
-    __shared__ T lds[2][8][PL][PL];
+    // one r-plane with its theta/phi halo; the second barrier of each plane
+    // orders its reuse, so it is not double-buffered
+    __shared__ T lds[8][PL][PL];
+    // theta and phi face fluxes of the plane, each computed once: fj[v][y][x]
+    // is the flux through the lower theta face of tile cell (y, x) (row TJa:
+    // the upper face of the last row), fk[v][y][x] through its lower phi face
+    __shared__ T fj[8][TILE + 1][TILE];
+    __shared__ T fk[8][TILE][TILE + 1];
     __shared__ double red[4];
     // offset of the cell read in place of a ghost in direction code c, relative
     // to the ghost's own offset (0 where the ghost itself is read); set before
@@ -443,7 +450,8 @@ __global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const TI
     T nxt[8], hn[8];
     // the upper r-face flux of plane i is the lower r-face flux of plane i+1:
     // same four values, same speed sum (speed at the upper cell first), so it
-    // is carried instead of recomputed
+    // is carried instead of recomputed.  The same holds for the theta and phi
+    // faces between neighbouring lanes, which go through fj / fk.
     T fr[8];
     bool have_fr = false;
 #pragma unroll
@@ -454,14 +462,30 @@ __global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const TI
         fr[v] = T(0);
     }
     const TIN *Ub = in + (long long)b * 8 * d.P;
+    // a ghost in a direction listed in the pull table is the same-level
+    // neighbour's interior cell, translated by n
+    auto at = [&](int v, int II, int JJ, int KK) -> T {
+        long long o = (long long)v * d.P + (long long)II * m.SI + (long long)JJ * m.SJ + KK;
+        if (PULL) {
+            const int c = (II < d.ng ? 0 : (II >= d.ng + d.ni ? 18 : 9)) +
+                          (JJ < d.ng ? 0 : (JJ >= d.ng + d.nj ? 6 : 3)) +
+                          (KK < d.ng ? 0 : (KK >= d.ng + d.nk ? 2 : 1));
+            o += pdel[c];
+        }
+        return (T)Ub[o];
+    };
+    // faces on the tile's lower edges: lanes 0-15 take the theta face below
+    // row 0 in column x = lane, lanes 16-31 the phi face left of column 0 in
+    // row y = lane - 16
+    const int lane = threadIdx.x;
+    const bool bact = lane < TILE ? lane < TKa : (lane < 2 * TILE && lane - TILE < TJa);
 
     for (int i = 0; i < d.ni; i++) {
         const int I = i + d.ng;
-        const int buf = i & 1;
 #pragma unroll
         for (int v = 0; v < 8; v++) {
-            if (m.act) lds[buf][v][ty + 2][tx + 2] = win[v][2];
-            if (m.hv) lds[buf][v][m.hly][m.hlx] = hn[v];
+            if (m.act) lds[v][ty + 2][tx + 2] = win[v][2];
+            if (m.hv) lds[v][m.hly][m.hlx] = hn[v];
         }
         const bool more = i + 1 < d.ni;
 #pragma unroll
@@ -470,35 +494,61 @@ __global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const TI
             hn[v] = (m.hv && more) ? m.halo(v, I + 1) : T(0);
         }
         __syncthreads();
+        const T csI2 = (T)t.cs[rb + I - 2];
+        auto Lv = [&](int v, int dj, int dk) -> T { return lds[v][ty + 2 + dj][tx + 2 + dk]; };
+        // U^n of this plane, loaded before the face exchange (FP32 path, MODE 0)
+        constexpr bool UEARLY = MODE == 0 && sizeof(T) == 4;
+        double ucur[8];
         if (m.act) {
             const int J = m.J, K = m.K;
-            // a ghost in a direction listed in the pull table is the
-            // same-level neighbour's interior cell, translated by n
-            auto at = [&](int v, int II, int JJ, int KK) -> T {
-                long long o = (long long)v * d.P + (long long)II * m.SI + (long long)JJ * m.SJ + KK;
-                if (PULL) {
-                    const int c = (II < d.ng ? 0 : (II >= d.ng + d.ni ? 18 : 9)) +
-                                  (JJ < d.ng ? 0 : (JJ >= d.ng + d.nj ? 6 : 3)) +
-                                  (KK < d.ng ? 0 : (KK >= d.ng + d.nk ? 2 : 1));
-                    o += pdel[c];
-                }
-                return (T)Ub[o];
-            };
+            if (UEARLY) {
+                const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + (long long)J * m.SJ + K;
+#pragma unroll
+                for (int v = 0; v < 8; v++) ucur[v] = U[base + (long long)v * d.P];
+            }
             // speeds |v_d| + c_s at the cells the reference reads (see above)
-            const T csI = (T)t.cs[rb + I], csIm = (T)t.cs[rb + I - 1], csIp = (T)t.cs[rb + I + 1], csI2 = (T)t.cs[rb + I - 2];
+            const T c20 = fabs(at(2, I - 2, J, K - 2)) + csI2;
+            const T c2p = fabs(at(2, I - 2, J + 1, K - 2)) + csI2;
+            const T c30 = fabs(at(3, I - 2, J - 2, K)) + csI2;
+            const T c3p = fabs(at(3, I - 2, J - 2, K + 1)) + csI2;
+#pragma unroll
+            for (int v = 0; v < 8; v++) {
+                // upper theta face (the lower one of the cell at row + 1)
+                fj[v][ty + 1][tx] = (face_flux(Lv(v, -1, 0), Lv(v, 0, 0), Lv(v, 1, 0), Lv(v, 2, 0)) * (c2p + c20)) * T(0.5);
+                // upper phi face
+                fk[v][ty][tx + 1] = (face_flux(Lv(v, 0, -1), Lv(v, 0, 0), Lv(v, 0, 1), Lv(v, 0, 2)) * (c3p + c30)) * T(0.5);
+            }
+        }
+        if (bact) {
+            // the lower face of the edge cell: its four-cell stencil in the LDS
+            // plane (start, step), its speed cells (J0, K0) on the upper side and
+            // (J1, K1) on the lower side
+            const bool bj = lane < TILE;
+            const int bx = bj ? lane : 0, by = bj ? 0 : lane - TILE;
+            const int bstart = bj ? bx + 2 : (by + 2) * PL, bstep = bj ? PL : 1;
+            const int bvar = bj ? 2 : 3;
+            const int bJ0 = bj ? j0 + d.ng : j0 + by + d.ng - 2, bK0 = bj ? k0 + bx + d.ng - 2 : k0 + d.ng;
+            const int bJ1 = bj ? bJ0 - 1 : bJ0, bK1 = bj ? bK0 : bK0 - 1;
+            const T s0 = fabs(at(bvar, I - 2, bJ0, bK0)) + csI2;
+            const T s1 = fabs(at(bvar, I - 2, bJ1, bK1)) + csI2;
+#pragma unroll
+            for (int v = 0; v < 8; v++) {
+                const T *pv = &lds[v][0][0] + bstart;
+                const T f = (face_flux(pv[0], pv[bstep], pv[2 * bstep], pv[3 * bstep]) * (s0 + s1)) * T(0.5);
+                if (bj) fj[v][0][bx] = f;
+                else fk[v][by][0] = f;
+            }
+        }
+        __syncthreads();
+        if (m.act) {
+            const int J = m.J, K = m.K;
+            const T csI = (T)t.cs[rb + I], csIm = (T)t.cs[rb + I - 1], csIp = (T)t.cs[rb + I + 1];
             const T c1m = fabs(at(1, I - 1, J - 2, K - 2)) + csIm;
             const T c10 = fabs(at(1, I, J - 2, K - 2)) + csI;
             const T c1p = fabs(at(1, I + 1, J - 2, K - 2)) + csIp;
-            const T c2m = fabs(at(2, I - 2, J - 1, K - 2)) + csI2;
-            const T c20 = fabs(at(2, I - 2, J, K - 2)) + csI2;
-            const T c2p = fabs(at(2, I - 2, J + 1, K - 2)) + csI2;
-            const T c3m = fabs(at(3, I - 2, J - 2, K - 1)) + csI2;
-            const T c30 = fabs(at(3, I - 2, J - 2, K)) + csI2;
-            const T c3p = fabs(at(3, I - 2, J - 2, K + 1)) + csI2;
             const T xic = (T)t.xi[rb + I];
             const T L2 = dxj * xic;
             const T L3 = (dxk * xic) * sintc;
-            auto Lv = [&](int v, int dj, int dk) -> T { return lds[buf][v][ty + 2 + dj][tx + 2 + dk]; };
             const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + (long long)J * m.SJ + K;
             const long long nn = (long long)d.ni * d.nj * d.nk;
             const long long rbase = (long long)b * 8 * nn + ((long long)i * d.nj + (j0 + ty)) * d.nk + (k0 + tx);
@@ -506,28 +556,16 @@ __global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const TI
 #pragma unroll
             for (int v = 0; v < 8; v++) {
                 T r = T(0);
-                {
-                    const T f0 = have_fr ? fr[v]
-                                         : (face_flux(win[v][0], win[v][1], win[v][2], win[v][3]) * (c10 + c1m)) * T(0.5);
-                    const T f1 = (face_flux(win[v][1], win[v][2], win[v][3], win[v][4]) * (c1p + c10)) * T(0.5);
-                    fr[v] = f1;
-                    r = r + (f0 - f1) / dxi;
-                }
-                {
-                    T a = Lv(v, -2, 0), bb = Lv(v, -1, 0), c = Lv(v, 0, 0), dd = Lv(v, 1, 0), e = Lv(v, 2, 0);
-                    T f0 = (face_flux(a, bb, c, dd) * (c20 + c2m)) * T(0.5);
-                    T f1 = (face_flux(bb, c, dd, e) * (c2p + c20)) * T(0.5);
-                    r = r + (f0 - f1) / L2;
-                }
-                {
-                    T a = Lv(v, 0, -2), bb = Lv(v, 0, -1), c = Lv(v, 0, 0), dd = Lv(v, 0, 1), e = Lv(v, 0, 2);
-                    T f0 = (face_flux(a, bb, c, dd) * (c30 + c3m)) * T(0.5);
-                    T f1 = (face_flux(bb, c, dd, e) * (c3p + c30)) * T(0.5);
-                    r = r + (f0 - f1) / L3;
-                }
+                const T f0 = have_fr ? fr[v]
+                                     : (face_flux(win[v][0], win[v][1], win[v][2], win[v][3]) * (c10 + c1m)) * T(0.5);
+                const T f1 = (face_flux(win[v][1], win[v][2], win[v][3], win[v][4]) * (c1p + c10)) * T(0.5);
+                fr[v] = f1;
+                r = r + (f0 - f1) / dxi;
+                r = r + (fj[v][ty][tx] - fj[v][ty + 1][tx]) / L2;
+                r = r + (fk[v][ty][tx] - fk[v][ty][tx + 1]) / L3;
                 if (MODE == 0) {
                     const long long q = base + (long long)v * d.P;
-                    const double x = U[q] + (0.5 * dt) * (double)r;
+                    const double x = (UEARLY ? ucur[v] : U[q]) + (0.5 * dt) * (double)r;
                     out[q] = x;
                     if (shadow) shadow[q] = (float)x;
                     if (v == 1) vv = x * x;
