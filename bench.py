@@ -2,13 +2,15 @@
 heat-solve seconds to 1e-10.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
-                    [--no-heat] [--no-cpu-baseline]
+                    [--legs rk,heat,learned] [--no-cpu-baseline] [--no-autotune]
 
 A "step" is one full RK4 step (4 RHS stages, artificial diffusion, 5 ghost
 fills) over every block of the workload; all blocks are resident in HBM when
 the timed region starts.  `value` is the FP32-deviation path's cell-updates/s
 on the 128x256x768-per-panel shell; the FP64 path, the stage kernel's
 roofline, the heat leg and the CPU baselines ride in the same JSON line.
+Before each precision is timed, the autotuner picks the (theta, phi) tile edge
+of the stage and diffusion marches from timings on the same state ("tiles").
 """
 import argparse
 import json
@@ -38,9 +40,12 @@ STAGE_BYTES = {"fp64": [192, 192, 192, 192], "fp32": [96, 96, 96, 192]}
 HBM_PEAK_GBS = 8000.0
 
 
-def run_precision(shell, prec, steps, warmup, cfl):
+def run_precision(shell, prec, steps, warmup, cfl, autotune=True):
     shell.set_precision(prec)
     shell.init_synthetic(seed=12345, s_amp=1e-4, b_ratio=1e-3)
+    # tile edges of the stage and diffusion marches, timed on this state
+    # (outside the timed region; results do not depend on the edges)
+    tune = shell.autotune(reps=5) if autotune else None
     shell.advance(warmup, cfl, graph=True)
     shell.synchronize()
     t0 = time.perf_counter()
@@ -54,6 +59,7 @@ def run_precision(shell, prec, steps, warmup, cfl):
     return {
         "cell_updates_per_s": cells * steps / el,
         "ms_per_step": 1e3 * el / steps,
+        "tiles": tune,
         "kernel_ms": prof,
         "stage_roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": achieved / HBM_PEAK_GBS, "traffic": None},
@@ -68,6 +74,7 @@ def main():
     ap.add_argument("--workload", default="shell_128x256x768_b16", choices=sorted(WORKLOADS))
     ap.add_argument("--cfl", type=float, default=0.5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the 16-edge march tiles instead of timing both")
     ap.add_argument("--learned-sweeps", default="2,2",
                     help="finest-level pre,post smoothing sweeps of the V-cycle under the learned correction")
     ap.add_argument("--learned-pairs", type=int, default=4000, help="solver-generated residual/error snapshots")
@@ -110,7 +117,7 @@ def main():
         print(f"# setup {args.workload}: {nblocks} blocks, {ncells} cells, {setup_s:.1f} s", file=sys.stderr, flush=True)
         res = {}
         for prec in ("fp64", "fp32"):
-            res[prec] = run_precision(shell, prec, args.steps, args.warmup, args.cfl)
+            res[prec] = run_precision(shell, prec, args.steps, args.warmup, args.cfl, not args.no_autotune)
             print(f"# {prec}: {res[prec]['cell_updates_per_s']:.4e} cell-updates/s, "
                   f"{res[prec]['ms_per_step']:.3f} ms/step, kernels {res[prec]['kernel_ms']}", file=sys.stderr,
                   flush=True)
@@ -159,11 +166,12 @@ def main():
         "config": {"workload": args.workload, "blocks": nblocks, "block": [lay["ni"], lay["nj"], lay["nk"]],
                    "interior_cells": ncells, "cfl": args.cfl},
         "fp64": ({"value": res["fp64"]["cell_updates_per_s"], "ms_per_step": res["fp64"]["ms_per_step"],
-                  "kernel_ms": res["fp64"]["kernel_ms"], "stage_roofline": res["fp64"]["stage_roofline"]}
-                 if res else None),
+                  "kernel_ms": res["fp64"]["kernel_ms"], "stage_roofline": res["fp64"]["stage_roofline"],
+                  "tiles": res["fp64"]["tiles"]} if res else None),
         "fp32_over_fp64": head["cell_updates_per_s"] / res["fp64"]["cell_updates_per_s"] if head else None,
         "fp32_error": res.get("fp32_error") if res else None,
         "kernel_ms": head["kernel_ms"] if head else None,
+        "tiles": head["tiles"] if head else None,
         "roofline": head["stage_roofline"] if head else None,
         "setup_s": setup_s,
         "cpu_baseline": cpu,

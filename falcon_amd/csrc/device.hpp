@@ -26,7 +26,7 @@ struct Dims {
     int ni, nj, nk, ng;
     int NI, NJ, NK;
     int nblocks;
-    int tiles_j, tiles_k;   // 16x16 (theta, phi) tiles per block
+    int tiles_j, tiles_k;   // 16x16 (theta, phi) tiles per block (k_dt_tile and the 16-edge marches)
     long long P;            // cells per variable per block (NI*NJ*NK)
 };
 
@@ -62,13 +62,15 @@ struct DevBoundary {
 
 // launchers (kernels.hip)
 // pull (null: none): the pull table of DevGhosts; ghosts of `in` in the
-// directions it lists are read from the neighbour's interior instead
+// directions it lists are read from the neighbour's interior instead.
+// tile: (theta, phi) tile edge of the march, 8 or 16 (results do not depend on
+// it); the dt partial minima are laid out per tile of the diffusion's edge.
 template <typename T, typename TIN, typename TOUT, typename TB>
 void launch_stage(const Dims &d, const DevTables &t, const TIN *in, const TB *U, TOUT *out, float *shadow,
-                  const double *dtp, double div, hipStream_t s, const int32_t *pull = nullptr);
+                  const double *dtp, double div, hipStream_t s, const int32_t *pull, int tile);
 template <typename T, typename TIN>
 void launch_diffusion(const Dims &d, const DevTables &t, const TIN *in, const double *U, double *out, float *shadow,
-                      double *partial, const double *dtp, hipStream_t s, const int32_t *pull = nullptr);
+                      double *partial, const double *dtp, hipStream_t s, const int32_t *pull, int tile);
 // every ghost entry of g
 template <typename TS>
 void launch_exchange(const Dims &d, const DevGhosts &g, TS *state, hipStream_t s);
@@ -79,13 +81,13 @@ template <typename TS>
 void launch_boundary(const Dims &d, const DevBoundary &b, TS *state, hipStream_t s, const int32_t *pull = nullptr);
 template <typename TS>
 void launch_copy_radial(const Dims &d, const DevBoundary &b, const TS *src, TS *dst, hipStream_t s);
-void launch_dt_partial(const Dims &d, const DevTables &t, const double *U, double *partial, hipStream_t s);
-void launch_dt_final(const Dims &d, const double *partial, double *dtp, double cfl, hipStream_t s);
+void launch_dt_partial(const Dims &d, const DevTables &t, const double *U, double *partial, hipStream_t s, int tile);
+void launch_dt_final(const Dims &d, const double *partial, double *dtp, double cfl, hipStream_t s, int tile);
 void launch_to_f32(const double *src, float *dst, long long n, hipStream_t s);
 // single-operator entry points used by parity tests: R only
 template <typename T, typename TIN>
-void launch_rhs_only(const Dims &d, const DevTables &t, const TIN *in, double *R, hipStream_t s);
+void launch_rhs_only(const Dims &d, const DevTables &t, const TIN *in, double *R, hipStream_t s, int tile);
 template <typename T>
-void launch_diffusion_only(const Dims &d, const DevTables &t, const double *U, double *R, hipStream_t s);
+void launch_diffusion_only(const Dims &d, const DevTables &t, const double *U, double *R, hipStream_t s, int tile);
 
 }  // namespace falcon

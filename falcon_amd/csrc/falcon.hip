@@ -104,6 +104,9 @@ struct falcon_handle {
     std::vector<int> *prof_kind = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_cur = -1, graph_prec = -1;
+    // (theta, phi) tile edge of the stage and diffusion marches (8 or 16), per
+    // precision (0 FP64, 1 FP32); falcon_autotune picks them by timing
+    int tile_stage[2] = {16, 16}, tile_diff[2] = {16, 16};
     std::unique_ptr<HeatSolver> heat;
     HeatParams heat_params;
     std::unique_ptr<CnnModel> cnn;
@@ -419,7 +422,10 @@ void alloc_state(falcon_handle *h) {
         }
     }
     if (!h->dtp.p) { h->dtp.alloc(sizeof(double)); HIPCHECK(hipMemset(h->dtp.p, 0, sizeof(double))); }
-    if (!h->partial.p) h->partial.alloc(sizeof(double) * std::max(1, h->dims.nblocks * h->dims.tiles_j * h->dims.tiles_k));
+    // one partial minimum per workgroup of the diffusion / k_dt_tile; the 8-edge
+    // tiling has the most
+    const long long nt8 = (long long)h->dims.nblocks * ((h->dims.nj + 7) / 8) * ((h->dims.nk + 7) / 8);
+    if (!h->partial.p) h->partial.alloc(sizeof(double) * std::max(1LL, nt8));
 }
 
 void alloc_stage(falcon_handle *h) {
@@ -471,7 +477,8 @@ void enqueue_step(falcon_handle *h, double cfl, int cur) {
     hipStream_t s = h->stream;
     double *U = h->U[cur].as<double>(), *Un = h->U[1 - cur].as<double>();
     double *dtp = h->dtp.as<double>(), *part = h->partial.as<double>();
-    timed(h, K_DT, [&] { launch_dt_final(d, part, dtp, cfl, s); });
+    const int ts = h->tile_stage[h->precision], td = h->tile_diff[h->precision];
+    timed(h, K_DT, [&] { launch_dt_final(d, part, dtp, cfl, s, td); });
     if (h->precision == 0) {
         double *S0 = h->S64[0].as<double>(), *S1 = h->S64[1].as<double>();
         const double *in[4] = {U, S0, S1, S0};
@@ -480,11 +487,11 @@ void enqueue_step(falcon_handle *h, double cfl, int cur) {
             timed(h, K_BC, [&] { launch_boundary<double>(d, h->bnd, const_cast<double *>(in[k - 1]), s); });
             timed(h, K_STAGE, [&] {
                 launch_stage<double, double, double, double>(d, t, in[k - 1], U, out[k - 1], nullptr, dtp,
-                                                             (double)(5.0f - (float)k), s);
+                                                             (double)(5.0f - (float)k), s, nullptr, ts);
             });
             timed(h, k < 4 ? K_XSTAGE : K_XSTATE, [&] { launch_exchange<double>(d, h->gh64, out[k - 1], s); });
         }
-        timed(h, K_DIFF, [&] { launch_diffusion<double, double>(d, t, U, U, Un, nullptr, part, dtp, s); });
+        timed(h, K_DIFF, [&] { launch_diffusion<double, double>(d, t, U, U, Un, nullptr, part, dtp, s, nullptr, td); });
         timed(h, K_OTHER, [&] { launch_copy_radial<double>(d, h->bnd, U, Un, s); });
         timed(h, K_XSTATE, [&] { launch_exchange<double>(d, h->gh64, Un, s); });
     } else {
@@ -506,16 +513,16 @@ void enqueue_step(falcon_handle *h, double cfl, int cur) {
             if (k < 4) {
                 // intermediate stage values are FP32 temporaries: base = FP32 copy of U^n
                 timed(h, K_STAGE, [&] {
-                    launch_stage<float, float, float, float>(d, t, in[k - 1], F, out[k - 1], nullptr, dtp, div, s, pull);
+                    launch_stage<float, float, float, float>(d, t, in[k - 1], F, out[k - 1], nullptr, dtp, div, s, pull, ts);
                 });
                 timed(h, K_XSTAGE, [&] { exchange(out[k - 1]); });
             } else {
                 timed(h, K_STAGE,
-                      [&] { launch_stage<float, float, double, double>(d, t, in[3], U, U, F, dtp, div, s, pull); });
+                      [&] { launch_stage<float, float, double, double>(d, t, in[3], U, U, F, dtp, div, s, pull, ts); });
                 timed(h, K_XSTATE, [&] { exchange(F); });
             }
         }
-        timed(h, K_DIFF, [&] { launch_diffusion<float, float>(d, t, F, U, Un, Fn, part, dtp, s, pull); });
+        timed(h, K_DIFF, [&] { launch_diffusion<float, float>(d, t, F, U, Un, Fn, part, dtp, s, pull, td); });
         timed(h, K_XSTATE, [&] { exchange(Fn); });
     }
 }
@@ -542,7 +549,8 @@ void prepare_steps(falcon_handle *h) {
         h->u_ghosts_valid = true;
     }
     if (!h->partials_valid) {
-        launch_dt_partial(d, h->dt, h->U[h->cur].as<double>(), h->partial.as<double>(), h->stream);
+        launch_dt_partial(d, h->dt, h->U[h->cur].as<double>(), h->partial.as<double>(), h->stream,
+                          h->tile_diff[h->precision]);
         h->partials_valid = true;
     }
 }
@@ -930,7 +938,108 @@ int falcon_set_precision(falcon_handle_t h, int mode) {
             refresh_u_ghosts(h);
             h->shadow_valid = false;
         }
+        // the partial minima are laid out per tile of the diffusion's edge
+        if (h->tile_diff[mode] != h->tile_diff[h->precision]) h->partials_valid = false;
         h->precision = mode;
+    });
+}
+
+// Tile edges of precision p's marches; a change drops that precision's graph
+// and, where the diffusion's edge changes, the partial minima laid out by it.
+static void apply_tiles(falcon_handle *h, int p, int ts, int td) {
+    if ((ts != 8 && ts != 16) || (td != 8 && td != 16)) throw Error("tile edges must be 8 or 16");
+    if (ts == h->tile_stage[p] && td == h->tile_diff[p]) return;
+    if (h->graph && h->graph_prec == p) {
+        HIPCHECK(hipGraphExecDestroy(h->graph));
+        h->graph = nullptr;
+    }
+    if (p == h->precision && td != h->tile_diff[p]) h->partials_valid = false;
+    h->tile_stage[p] = ts;
+    h->tile_diff[p] = td;
+}
+
+int falcon_set_tiles(falcon_handle_t h, int precision, int stage_tile, int diffusion_tile) {
+    return guard([&] {
+        if (precision != 0 && precision != 1) throw Error("precision must be 0 (FP64) or 1 (FP32 deviation)");
+        apply_tiles(h, precision, stage_tile, diffusion_tile);
+    });
+}
+
+int falcon_get_tiles(falcon_handle_t h, int precision, int *stage_tile, int *diffusion_tile) {
+    return guard([&] {
+        if (precision != 0 && precision != 1) throw Error("precision must be 0 (FP64) or 1 (FP32 deviation)");
+        if (stage_tile) *stage_tile = h->tile_stage[precision];
+        if (diffusion_tile) *diffusion_tile = h->tile_diff[precision];
+    });
+}
+
+// Autotuner of the march tile edges.  For the current precision, times the
+// stage kernel (intermediate-stage form) and the diffusion kernel with each
+// edge, reps launches after one untimed launch, on the current state, and
+// keeps the faster edge of each.  The timed launches write only the stage
+// buffer S[0], the next-state buffers and the partial minima, none of which
+// holds anything between steps (the partial minima are recomputed before the
+// next step), so the state and every later result are unchanged.
+// ms_out (optional): stage 16, stage 8, diffusion 16, diffusion 8 in ms.
+int falcon_autotune(falcon_handle_t h, int reps, int *tiles_out, double *ms_out) {
+    return guard([&] {
+        if (reps < 1 || reps > 1000) throw Error("reps must be in 1..1000");
+        h->need_tables();
+        ensure_device_tables(h);
+        alloc_stage(h);
+        prepare_steps(h);
+        const Dims &d = h->dims;
+        const DevTables &t = h->dt;
+        hipStream_t s = h->stream;
+        const int p = h->precision, cur = h->cur;
+        double *U = h->U[cur].as<double>(), *Un = h->U[1 - cur].as<double>();
+        double *dtp = h->dtp.as<double>(), *part = h->partial.as<double>();
+        const int32_t *pull = (p == 1 && h->ghost_pull) ? h->gh32.pull : nullptr;
+        // a time step for the timed launches to scale by
+        launch_dt_final(d, part, dtp, 0.5, s, h->tile_diff[p]);
+        hipEvent_t e0, e1;
+        HIPCHECK(hipEventCreate(&e0));
+        HIPCHECK(hipEventCreate(&e1));
+        auto time = [&](auto &&launch) -> double {
+            launch();
+            HIPCHECK(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; r++) launch();
+            HIPCHECK(hipEventRecord(e1, s));
+            HIPCHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+            return (double)ms / reps;
+        };
+        const int edges[2] = {16, 8};
+        double ms[4];
+        for (int q = 0; q < 2; q++) {
+            const int tl = edges[q];
+            if (p == 0) {
+                double *S0 = h->S64[0].as<double>();
+                ms[q] = time([&] {
+                    launch_stage<double, double, double, double>(d, t, U, U, S0, nullptr, dtp, 4.0, s, nullptr, tl);
+                });
+                ms[2 + q] = time([&] {
+                    launch_diffusion<double, double>(d, t, U, U, Un, nullptr, part, dtp, s, nullptr, tl);
+                });
+            } else {
+                float *F = h->F32[cur].as<float>(), *Fn = h->F32[1 - cur].as<float>(), *S0 = h->S32[0].as<float>();
+                ms[q] = time([&] {
+                    launch_stage<float, float, float, float>(d, t, F, F, S0, nullptr, dtp, 4.0, s, pull, tl);
+                });
+                ms[2 + q] = time([&] {
+                    launch_diffusion<float, float>(d, t, F, U, Un, Fn, part, dtp, s, pull, tl);
+                });
+            }
+        }
+        check_launch();
+        HIPCHECK(hipEventDestroy(e0));
+        HIPCHECK(hipEventDestroy(e1));
+        h->partials_valid = false;
+        const int ts = ms[1] < ms[0] ? 8 : 16, td = ms[3] < ms[2] ? 8 : 16;
+        apply_tiles(h, p, ts, td);
+        if (tiles_out) { tiles_out[0] = ts; tiles_out[1] = td; }
+        if (ms_out) for (int q = 0; q < 4; q++) ms_out[q] = ms[q];
     });
 }
 
@@ -965,8 +1074,9 @@ int falcon_compute_dt(falcon_handle_t h, double cfl, double *dt) {
     return guard([&] {
         h->need_tables();
         ensure_device_tables(h);
-        launch_dt_partial(h->dims, h->dt, h->U[h->cur].as<double>(), h->partial.as<double>(), h->stream);
-        launch_dt_final(h->dims, h->partial.as<double>(), h->dtp.as<double>(), cfl, h->stream);
+        const int td = h->tile_diff[h->precision];
+        launch_dt_partial(h->dims, h->dt, h->U[h->cur].as<double>(), h->partial.as<double>(), h->stream, td);
+        launch_dt_final(h->dims, h->partial.as<double>(), h->dtp.as<double>(), cfl, h->stream, td);
         h->partials_valid = true;
         check_launch();
         HIPCHECK(hipMemcpyAsync(dt, h->dtp.p, sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -1023,8 +1133,9 @@ int falcon_rhs(falcon_handle_t h, double *r_out) {
         refresh_u_ghosts(h);
         double *U = h->U[h->cur].as<double>();
         launch_boundary<double>(d, h->bnd, U, h->stream);
-        if (h->precision == 0) launch_rhs_only<double, double>(d, h->dt, U, h->rbuf.as<double>(), h->stream);
-        else launch_rhs_only<float, double>(d, h->dt, U, h->rbuf.as<double>(), h->stream);
+        const int ts = h->tile_stage[h->precision];
+        if (h->precision == 0) launch_rhs_only<double, double>(d, h->dt, U, h->rbuf.as<double>(), h->stream, ts);
+        else launch_rhs_only<float, double>(d, h->dt, U, h->rbuf.as<double>(), h->stream, ts);
         check_launch();
         std::vector<double> tmp(nn * 8 * d.nblocks);
         HIPCHECK(hipStreamSynchronize(h->stream));
@@ -1048,8 +1159,9 @@ int falcon_diffusion_rhs(falcon_handle_t h, double *r_out) {
         const size_t nn = (size_t)d.ni * d.nj * d.nk;
         h->rbuf.alloc(sizeof(double) * nn * 8 * d.nblocks);
         double *U = h->U[h->cur].as<double>();
-        if (h->precision == 0) launch_diffusion_only<double>(d, h->dt, U, h->rbuf.as<double>(), h->stream);
-        else launch_diffusion_only<float>(d, h->dt, U, h->rbuf.as<double>(), h->stream);
+        const int td = h->tile_diff[h->precision];
+        if (h->precision == 0) launch_diffusion_only<double>(d, h->dt, U, h->rbuf.as<double>(), h->stream, td);
+        else launch_diffusion_only<float>(d, h->dt, U, h->rbuf.as<double>(), h->stream, td);
         check_launch();
         std::vector<double> tmp(nn * 8 * d.nblocks);
         HIPCHECK(hipStreamSynchronize(h->stream));

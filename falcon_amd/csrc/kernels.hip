@@ -22,8 +22,12 @@
 
 namespace falcon {
 
-constexpr int TILE = 16;
-constexpr int PL = TILE + 4;   // LDS plane edge (tile + 2 halo each side)
+// Stage, diffusion and dt kernels take the (theta, phi) tile edge TL as a
+// template parameter: 16 (256 lanes, one 16x16 tile per 16^3 block face) or 8
+// (64 lanes).  Each cell's arithmetic is the same for either edge, so the
+// choice (made by timing both, falcon_autotune) never changes a result.
+template <int TL>
+constexpr int tile_pl() { return TL + 4; }   // LDS plane edge (tile + 2 halo each side)
 
 // ModDerivative_1st_O4_3D (3_6/ModDerivative.f90:16-19):
 // (1./12./dx)*( -f(+2) + 8.*f(+1) - 8.*f(-1) + f(-2) )
@@ -69,7 +73,7 @@ __device__ __forceinline__ T face_flux(T a, T b, T c, T d) {
 // index minus (di*ni, dj*nj, dk*nk); the radial ghost planes of the column are
 // direction (-1,0,0) / (+1,0,0), the theta/phi halo strips (0,dj,0) / (0,0,dk).
 // PULL = false compiles the plain reads (the pull table is ignored).
-template <typename T, typename TIN, bool PULL = false>
+template <typename T, typename TIN, bool PULL, int TL>
 struct Marcher {
     const TIN *blk;
     long long dlo, dhi;     // offset from blk of the column source of the radial ghost
@@ -99,20 +103,20 @@ struct Marcher {
             if (Dlo >= 0) dlo = (long long)(Dlo - b) * 8 * P + (long long)d.ni * SI;
             if (Dhi >= 0) dhi = (long long)(Dhi - b) * 8 * P - (long long)d.ni * SI;
         }
-        const int tx = threadIdx.x & (TILE - 1), ty = threadIdx.x / TILE;
+        const int tx = threadIdx.x & (TL - 1), ty = threadIdx.x / TL;
         act = ty < TJa && tx < TKa;
         J = j0 + ty + d.ng;
         K = k0 + tx + d.ng;
         const int h = threadIdx.x;
         hv = false; hly = 0; hlx = 0;
-        if (h < 4 * TILE) {
-            int r = h / TILE, kk = h % TILE;
+        if (h < 4 * TL) {
+            int r = h / TL, kk = h % TL;
             hly = r < 2 ? r : TJa + r;
             hlx = kk + 2;
             hv = kk < TKa;
-        } else if (h < 8 * TILE) {
-            int h2 = h - 4 * TILE;
-            int c = h2 / TILE, jj = h2 % TILE;
+        } else if (h < 8 * TL) {
+            int h2 = h - 4 * TL;
+            int c = h2 / TL, jj = h2 % TL;
             hlx = c < 2 ? c : TKa + c;
             hly = jj + 2;
             hv = jj < TJa;
@@ -290,22 +294,23 @@ __device__ __forceinline__ void cell_rhs(const T (&win)[8][5], LDSF L, const T x
 // MODE 1: write R (interior layout [block][var][i][j][k], FP64) for parity tests.
 // pull (null: none): ghost cells of `in` in the directions it lists are read
 // from the same-level neighbour's interior (Marcher).
-template <typename T, typename TIN, typename TOUT, typename TB, int MODE, bool SHADOW>
-__global__ void __launch_bounds__(256) k_stage(Dims d, DevTables t, const TIN *__restrict__ in,
+template <typename T, typename TIN, typename TOUT, typename TB, int MODE, bool SHADOW, int TL>
+__global__ void __launch_bounds__(TL * TL) k_stage(Dims d, DevTables t, const TIN *__restrict__ in,
                                                const TB *__restrict__ U, TOUT *__restrict__ out,
                                                float *__restrict__ shadow, const double *__restrict__ dtp,
                                                double div, const int32_t *__restrict__ pull) {
+    constexpr int PL = tile_pl<TL>();
     __shared__ T lds[2][8][PL][PL];
-    const int tpb = d.tiles_j * d.tiles_k;
+    const int ntk = (d.nk + TL - 1) / TL, tpb = ((d.nj + TL - 1) / TL) * ntk;
     const int b = blockIdx.x / tpb;
     const int trem = blockIdx.x - b * tpb;
-    const int tj = trem / d.tiles_k, tk = trem - tj * d.tiles_k;
-    const int j0 = tj * TILE, k0 = tk * TILE;
-    const int TJa = min(TILE, d.nj - j0), TKa = min(TILE, d.nk - k0);
-    const int tx = threadIdx.x & (TILE - 1), ty = threadIdx.x / TILE;
+    const int tj = trem / ntk, tk = trem - tj * ntk;
+    const int j0 = tj * TL, k0 = tk * TL;
+    const int TJa = min(TL, d.nj - j0), TKa = min(TL, d.nk - k0);
+    const int tx = threadIdx.x & (TL - 1), ty = threadIdx.x / TL;
 
     constexpr bool PULL = sizeof(T) == 4 && MODE == 0;
-    Marcher<T, TIN, PULL> m;
+    Marcher<T, TIN, PULL, TL> m;
     m.init(d, in, b, j0, k0, TJa, TKa, pull);
 
     const long long rb = (long long)b * d.NI, tb = (long long)b * d.NJ;
@@ -401,25 +406,26 @@ __global__ void __launch_bounds__(256) k_stage(Dims d, DevTables t, const TIN *_
 // null) receives the same value in FP32, and the workgroup's min(h/v_wave) of the
 // new state (the next step's ModTimeStep_Dynamo input) goes to partial[tile].
 // MODE 1: write R.  `in` is the stencil input (the state or its FP32 shadow).
-template <typename T, typename TIN, int MODE>
-__global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const TIN *__restrict__ in,
+template <typename T, typename TIN, int MODE, int TL>
+__global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, const TIN *__restrict__ in,
                                                    const double *__restrict__ U, double *__restrict__ out,
                                                    float *__restrict__ shadow, double *__restrict__ partial,
                                                    const double *__restrict__ dtp, const int32_t *__restrict__ pull) {
     // one r-plane with its theta/phi halo; the second barrier of each plane
     // orders its reuse, so it is not double-buffered
+    constexpr int PL = tile_pl<TL>();
     __shared__ T lds[8][PL][PL];
     // theta and phi face fluxes of the plane, each computed once: fj[v][y][x]
     // is the flux through the lower theta face of tile cell (y, x) (row TJa:
     // the upper face of the last row), fk[v][y][x] through its lower phi face
-    __shared__ T fj[8][TILE + 1][TILE];
-    __shared__ T fk[8][TILE][TILE + 1];
+    __shared__ T fj[8][TL + 1][TL];
+    __shared__ T fk[8][TL][TL + 1];
     __shared__ double red[4];
     // offset of the cell read in place of a ghost in direction code c, relative
     // to the ghost's own offset (0 where the ghost itself is read); set before
     // the first barrier of the march
     __shared__ long long pdel[27];
-    const int tpb = d.tiles_j * d.tiles_k;
+    const int ntk = (d.nk + TL - 1) / TL, tpb = ((d.nj + TL - 1) / TL) * ntk;
     const int b = blockIdx.x / tpb;
     if (sizeof(T) == 4 && MODE == 0 && threadIdx.x < 27) {
         const int c = threadIdx.x;
@@ -430,13 +436,13 @@ __global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const TI
                               (long long)dj * d.nj * d.NK - dk * d.nk;
     }
     const int trem = blockIdx.x - b * tpb;
-    const int tj = trem / d.tiles_k, tk = trem - tj * d.tiles_k;
-    const int j0 = tj * TILE, k0 = tk * TILE;
-    const int TJa = min(TILE, d.nj - j0), TKa = min(TILE, d.nk - k0);
-    const int tx = threadIdx.x & (TILE - 1), ty = threadIdx.x / TILE;
+    const int tj = trem / ntk, tk = trem - tj * ntk;
+    const int j0 = tj * TL, k0 = tk * TL;
+    const int TJa = min(TL, d.nj - j0), TKa = min(TL, d.nk - k0);
+    const int tx = threadIdx.x & (TL - 1), ty = threadIdx.x / TL;
 
     constexpr bool PULL = sizeof(T) == 4 && MODE == 0;
-    Marcher<T, TIN, PULL> m;
+    Marcher<T, TIN, PULL, TL> m;
     m.init(d, in, b, j0, k0, TJa, TKa, pull);
     const long long rb = (long long)b * d.NI, tb = (long long)b * d.NJ;
     const BlockMeta mt = t.meta[b];
@@ -474,11 +480,11 @@ __global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const TI
         }
         return (T)Ub[o];
     };
-    // faces on the tile's lower edges: lanes 0-15 take the theta face below
-    // row 0 in column x = lane, lanes 16-31 the phi face left of column 0 in
-    // row y = lane - 16
+    // faces on the tile's lower edges: lanes 0 .. TL-1 take the theta face
+    // below row 0 in column x = lane, lanes TL .. 2TL-1 the phi face left of
+    // column 0 in row y = lane - TL
     const int lane = threadIdx.x;
-    const bool bact = lane < TILE ? lane < TKa : (lane < 2 * TILE && lane - TILE < TJa);
+    const bool bact = lane < TL ? lane < TKa : (lane < 2 * TL && lane - TL < TJa);
 
     for (int i = 0; i < d.ni; i++) {
         const int I = i + d.ng;
@@ -523,8 +529,8 @@ __global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const TI
             // the lower face of the edge cell: its four-cell stencil in the LDS
             // plane (start, step), its speed cells (J0, K0) on the upper side and
             // (J1, K1) on the lower side
-            const bool bj = lane < TILE;
-            const int bx = bj ? lane : 0, by = bj ? 0 : lane - TILE;
+            const bool bj = lane < TL;
+            const int bx = bj ? lane : 0, by = bj ? 0 : lane - TL;
             const int bstart = bj ? bx + 2 : (by + 2) * PL, bstep = bj ? PL : 1;
             const int bvar = bj ? 2 : 3;
             const int bJ0 = bj ? j0 + d.ng : j0 + by + d.ng - 2, bK0 = bj ? k0 + bx + d.ng - 2 : k0 + d.ng;
@@ -597,7 +603,7 @@ __global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const TI
         }
     }
     if (MODE == 0 && partial) {
-        // min is exact in any order: wavefront shuffle, then across the 4 wavefronts
+        // min is exact in any order: wavefront shuffle, then across the wavefronts
         for (int off = 32; off > 0; off >>= 1) {
             double o = __shfl_down(hmin, off, 64);
             hmin = o < hmin ? o : hmin;
@@ -606,7 +612,7 @@ __global__ void __launch_bounds__(256) k_diffusion(Dims d, DevTables t, const TI
         __syncthreads();
         if (threadIdx.x == 0) {
             double r = red[0];
-            for (int q = 1; q < 4; q++) r = red[q] < r ? red[q] : r;
+            for (int q = 1; q < TL * TL / 64; q++) r = red[q] < r ? red[q] : r;
             partial[blockIdx.x] = r;
         }
     }
@@ -771,15 +777,16 @@ __global__ void __launch_bounds__(256) k_boundary(Dims d, DevBoundary bl, TS *__
 // ModTimeStep_Dynamo (3_6/ModTimeStep.f90:27) over one 16x16 (theta,phi) tile
 // column of a block, all r.  Used when the state was set from outside; during
 // stepping the diffusion kernel produces the same partials.
-__global__ void __launch_bounds__(256) k_dt_tile(Dims d, DevTables t, const double *__restrict__ U,
-                                                 double *__restrict__ partial) {
+template <int TL>
+__global__ void __launch_bounds__(TL * TL) k_dt_tile(Dims d, DevTables t, const double *__restrict__ U,
+                                                     double *__restrict__ partial) {
     __shared__ double red[4];
-    const int tpb = d.tiles_j * d.tiles_k;
+    const int ntk = (d.nk + TL - 1) / TL, tpb = ((d.nj + TL - 1) / TL) * ntk;
     const int b = blockIdx.x / tpb;
     const int trem = blockIdx.x - b * tpb;
-    const int tj = trem / d.tiles_k, tk = trem - tj * d.tiles_k;
-    const int tx = threadIdx.x & (TILE - 1), ty = threadIdx.x / TILE;
-    const int j = tj * TILE + ty, k = tk * TILE + tx;
+    const int tj = trem / ntk, tk = trem - tj * ntk;
+    const int tx = threadIdx.x & (TL - 1), ty = threadIdx.x / TL;
+    const int j = tj * TL + ty, k = tk * TL + tx;
     const long long rb = (long long)b * d.NI, tb = (long long)b * d.NJ;
     const BlockMeta mt = t.meta[b];
     const double *Ub = U + (long long)b * 8 * d.P;
@@ -811,7 +818,7 @@ __global__ void __launch_bounds__(256) k_dt_tile(Dims d, DevTables t, const doub
     __syncthreads();
     if (threadIdx.x == 0) {
         double r = red[0];
-        for (int q = 1; q < 4; q++) r = red[q] < r ? red[q] : r;
+        for (int q = 1; q < TL * TL / 64; q++) r = red[q] < r ? red[q] : r;
         partial[blockIdx.x] = r;
     }
 }
@@ -864,37 +871,68 @@ __global__ void k_to_f32(const double *__restrict__ s, float *__restrict__ d, lo
 
 // ---------------------------------------------------------------- launchers
 
-static inline unsigned tiles(const Dims &d) { return (unsigned)(d.nblocks * d.tiles_j * d.tiles_k); }
+// workgroups of a march over every block with tile edge TL
+static inline unsigned tiles(const Dims &d, int TL) {
+    return (unsigned)((long long)d.nblocks * ((d.nj + TL - 1) / TL) * ((d.nk + TL - 1) / TL));
+}
+
+static void check_tile(int tile) {
+    if (tile != 8 && tile != 16) throw std::runtime_error("tile edge must be 8 or 16");
+}
+
+template <int TL, typename T, typename TIN, typename TOUT, typename TB>
+static void stage_tl(const Dims &d, const DevTables &t, const TIN *in, const TB *U, TOUT *out, float *shadow,
+                     const double *dtp, double div, hipStream_t s, const int32_t *pull) {
+    if (shadow)
+        hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, true, TL>), dim3(tiles(d, TL)), dim3(TL * TL), 0, s, d, t, in, U,
+                           out, shadow, dtp, div, pull);
+    else
+        hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, false, TL>), dim3(tiles(d, TL)), dim3(TL * TL), 0, s, d, t, in, U,
+                           out, (float *)nullptr, dtp, div, pull);
+}
 
 template <typename T, typename TIN, typename TOUT, typename TB>
 void launch_stage(const Dims &d, const DevTables &t, const TIN *in, const TB *U, TOUT *out, float *shadow,
-                  const double *dtp, double div, hipStream_t s, const int32_t *pull) {
-    if (shadow)
-        hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, true>), dim3(tiles(d)), dim3(256), 0, s, d, t, in, U, out,
-                           shadow, dtp, div, pull);
-    else
-        hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, false>), dim3(tiles(d)), dim3(256), 0, s, d, t, in, U, out,
-                           (float *)nullptr, dtp, div, pull);
+                  const double *dtp, double div, hipStream_t s, const int32_t *pull, int tile) {
+    check_tile(tile);
+    if (tile == 16) stage_tl<16, T>(d, t, in, U, out, shadow, dtp, div, s, pull);
+    else stage_tl<8, T>(d, t, in, U, out, shadow, dtp, div, s, pull);
 }
 
 template <typename T, typename TIN>
-void launch_rhs_only(const Dims &d, const DevTables &t, const TIN *in, double *R, hipStream_t s) {
-    hipLaunchKernelGGL((k_stage<T, TIN, double, double, 1, false>), dim3(tiles(d)), dim3(256), 0, s, d, t, in,
-                       (const double *)nullptr, R, (float *)nullptr, (const double *)nullptr, 1.0,
-                       (const int32_t *)nullptr);
+void launch_rhs_only(const Dims &d, const DevTables &t, const TIN *in, double *R, hipStream_t s, int tile) {
+    check_tile(tile);
+    if (tile == 16)
+        hipLaunchKernelGGL((k_stage<T, TIN, double, double, 1, false, 16>), dim3(tiles(d, 16)), dim3(256), 0, s, d, t, in,
+                           (const double *)nullptr, R, (float *)nullptr, (const double *)nullptr, 1.0,
+                           (const int32_t *)nullptr);
+    else
+        hipLaunchKernelGGL((k_stage<T, TIN, double, double, 1, false, 8>), dim3(tiles(d, 8)), dim3(64), 0, s, d, t, in,
+                           (const double *)nullptr, R, (float *)nullptr, (const double *)nullptr, 1.0,
+                           (const int32_t *)nullptr);
 }
 
 template <typename T, typename TIN>
 void launch_diffusion(const Dims &d, const DevTables &t, const TIN *in, const double *U, double *out, float *shadow,
-                      double *partial, const double *dtp, hipStream_t s, const int32_t *pull) {
-    hipLaunchKernelGGL((k_diffusion<T, TIN, 0>), dim3(tiles(d)), dim3(256), 0, s, d, t, in, U, out, shadow, partial, dtp,
-                       pull);
+                      double *partial, const double *dtp, hipStream_t s, const int32_t *pull, int tile) {
+    check_tile(tile);
+    if (tile == 16)
+        hipLaunchKernelGGL((k_diffusion<T, TIN, 0, 16>), dim3(tiles(d, 16)), dim3(256), 0, s, d, t, in, U, out, shadow,
+                           partial, dtp, pull);
+    else
+        hipLaunchKernelGGL((k_diffusion<T, TIN, 0, 8>), dim3(tiles(d, 8)), dim3(64), 0, s, d, t, in, U, out, shadow,
+                           partial, dtp, pull);
 }
 
 template <typename T>
-void launch_diffusion_only(const Dims &d, const DevTables &t, const double *U, double *R, hipStream_t s) {
-    hipLaunchKernelGGL((k_diffusion<T, double, 1>), dim3(tiles(d)), dim3(256), 0, s, d, t, U, U, R, (float *)nullptr,
-                       (double *)nullptr, (const double *)nullptr, (const int32_t *)nullptr);
+void launch_diffusion_only(const Dims &d, const DevTables &t, const double *U, double *R, hipStream_t s, int tile) {
+    check_tile(tile);
+    if (tile == 16)
+        hipLaunchKernelGGL((k_diffusion<T, double, 1, 16>), dim3(tiles(d, 16)), dim3(256), 0, s, d, t, U, U, R,
+                           (float *)nullptr, (double *)nullptr, (const double *)nullptr, (const int32_t *)nullptr);
+    else
+        hipLaunchKernelGGL((k_diffusion<T, double, 1, 8>), dim3(tiles(d, 8)), dim3(64), 0, s, d, t, U, U, R,
+                           (float *)nullptr, (double *)nullptr, (const double *)nullptr, (const int32_t *)nullptr);
 }
 
 template <typename TS>
@@ -934,12 +972,17 @@ void launch_copy_radial(const Dims &d, const DevBoundary &b, const TS *src, TS *
     hipLaunchKernelGGL((k_copy_radial<TS>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, b, src, dst);
 }
 
-void launch_dt_partial(const Dims &d, const DevTables &t, const double *U, double *partial, hipStream_t s) {
-    hipLaunchKernelGGL(k_dt_tile, dim3(tiles(d)), dim3(256), 0, s, d, t, U, partial);
+// the partial minima are laid out as the diffusion with the same tile edge
+// leaves them, one per workgroup
+void launch_dt_partial(const Dims &d, const DevTables &t, const double *U, double *partial, hipStream_t s, int tile) {
+    check_tile(tile);
+    if (tile == 16) hipLaunchKernelGGL(k_dt_tile<16>, dim3(tiles(d, 16)), dim3(256), 0, s, d, t, U, partial);
+    else hipLaunchKernelGGL(k_dt_tile<8>, dim3(tiles(d, 8)), dim3(64), 0, s, d, t, U, partial);
 }
 
-void launch_dt_final(const Dims &d, const double *partial, double *dtp, double cfl, hipStream_t s) {
-    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, s, partial, (int)tiles(d), cfl, dtp);
+void launch_dt_final(const Dims &d, const double *partial, double *dtp, double cfl, hipStream_t s, int tile) {
+    check_tile(tile);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, s, partial, (int)tiles(d, tile), cfl, dtp);
 }
 
 void launch_to_f32(const double *src, float *dst, long long n, hipStream_t s) {
@@ -947,15 +990,15 @@ void launch_to_f32(const double *src, float *dst, long long n, hipStream_t s) {
 }
 
 // explicit instantiations: FP64 path, FP32-deviation path
-template void launch_stage<double, double, double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, const double *, double, hipStream_t, const int32_t *);
-template void launch_stage<float, float, float, float>(const Dims &, const DevTables &, const float *, const float *, float *, float *, const double *, double, hipStream_t, const int32_t *);
-template void launch_stage<float, float, double, double>(const Dims &, const DevTables &, const float *, const double *, double *, float *, const double *, double, hipStream_t, const int32_t *);
-template void launch_rhs_only<double, double>(const Dims &, const DevTables &, const double *, double *, hipStream_t);
-template void launch_rhs_only<float, double>(const Dims &, const DevTables &, const double *, double *, hipStream_t);
-template void launch_diffusion<double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, double *, const double *, hipStream_t, const int32_t *);
-template void launch_diffusion<float, float>(const Dims &, const DevTables &, const float *, const double *, double *, float *, double *, const double *, hipStream_t, const int32_t *);
-template void launch_diffusion_only<double>(const Dims &, const DevTables &, const double *, double *, hipStream_t);
-template void launch_diffusion_only<float>(const Dims &, const DevTables &, const double *, double *, hipStream_t);
+template void launch_stage<double, double, double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, const double *, double, hipStream_t, const int32_t *, int);
+template void launch_stage<float, float, float, float>(const Dims &, const DevTables &, const float *, const float *, float *, float *, const double *, double, hipStream_t, const int32_t *, int);
+template void launch_stage<float, float, double, double>(const Dims &, const DevTables &, const float *, const double *, double *, float *, const double *, double, hipStream_t, const int32_t *, int);
+template void launch_rhs_only<double, double>(const Dims &, const DevTables &, const double *, double *, hipStream_t, int);
+template void launch_rhs_only<float, double>(const Dims &, const DevTables &, const double *, double *, hipStream_t, int);
+template void launch_diffusion<double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, double *, const double *, hipStream_t, const int32_t *, int);
+template void launch_diffusion<float, float>(const Dims &, const DevTables &, const float *, const double *, double *, float *, double *, const double *, hipStream_t, const int32_t *, int);
+template void launch_diffusion_only<double>(const Dims &, const DevTables &, const double *, double *, hipStream_t, int);
+template void launch_diffusion_only<float>(const Dims &, const DevTables &, const double *, double *, hipStream_t, int);
 template void launch_exchange<double>(const Dims &, const DevGhosts &, double *, hipStream_t);
 template void launch_exchange<float>(const Dims &, const DevGhosts &, float *, hipStream_t);
 template void launch_exchange_rest<float>(const Dims &, const DevGhosts &, float *, hipStream_t);

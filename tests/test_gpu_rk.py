@@ -47,10 +47,15 @@ def test_exchange_bit_exact(name):
     assert np.array_equal(_states(s, o.nblocks), _states(o, o.nblocks))
 
 
+@pytest.mark.parametrize("tile", [16, 8])
 @pytest.mark.parametrize("name", list(LAYOUTS))
-def test_rhs_and_diffusion_bit_exact(name):
-    print(f"[rhs] layout={name}", flush=True)
+def test_rhs_and_diffusion_bit_exact(name, tile):
+    """RHS and diffusion of the FP64 path against the oracle with either
+    (theta, phi) tile edge of the march (partial tiles where a block's face is
+    not a multiple of the edge)."""
+    print(f"[rhs] layout={name} tile={tile}", flush=True)
     s, o = _pair(LAYOUTS[name], seed=2)
+    s.set_tiles(0, tile, tile)
     s.sendrecv_gc(0); o.exchange(0)
     r_gpu = s.rhs()
     r_orc = np.stack([o.rhs(ib) for ib in range(o.nblocks)])
@@ -60,10 +65,12 @@ def test_rhs_and_diffusion_bit_exact(name):
     assert np.array_equal(d_gpu, d_orc), rel_err(d_gpu, d_orc)
 
 
+@pytest.mark.parametrize("tiles", [(16, 16), (8, 8), (16, 8)])
 @pytest.mark.parametrize("name", ["mixed", "cubic8"])
-def test_steps_bit_exact(name):
-    print(f"[steps] layout={name}", flush=True)
+def test_steps_bit_exact(name, tiles):
+    print(f"[steps] layout={name} tiles={tiles}", flush=True)
     s, o = _pair(LAYOUTS[name], seed=3)
+    s.set_tiles(0, *tiles)
     s.sendrecv_gc(0); o.exchange(0)
     assert s.compute_dt(0.5) == o.dt(0.5)
     for step in range(3):
@@ -130,6 +137,51 @@ def test_fp32_ghost_pull_equals_full_exchange(name):
         s.advance(6, 0.5, graph=graph)
         runs.append(_states(s, s.nblocks))
     assert np.array_equal(runs[0], runs[1]) and np.array_equal(runs[0], runs[2])
+
+
+@pytest.mark.parametrize("name", ["mixed", "amr16"])
+def test_fp32_tiles_bit_identical(name):
+    """The FP32 step with 8-edge stage and/or diffusion tiles (eager and graph
+    replay) gives the same bits as with 16-edge tiles."""
+    print(f"[fp32 tiles] layout={name}", flush=True)
+    lay = PULL_LAYOUTS[name]
+    runs = []
+    for tiles, graph in (((16, 16), False), ((8, 8), False), ((8, 16), True), ((16, 8), True)):
+        s = Shell(lay)
+        s.set_gc_all()
+        s.background_polytrope(20.0)
+        s.init_synthetic(seed=17)
+        s.set_precision("fp32")
+        s.set_tiles(1, *tiles)
+        s.advance(5, 0.5, graph=graph)
+        runs.append(_states(s, s.nblocks))
+    for r in runs[1:]:
+        assert np.array_equal(runs[0], r)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_autotune_leaves_results_unchanged(prec):
+    """The autotuner picks an edge per kernel from its timings; stepping after
+    it (mid-run, graph replay) gives the same bits as stepping without it."""
+    print(f"[autotune] layout=amr16 prec={prec}", flush=True)
+    lay = PULL_LAYOUTS["amr16"]
+    runs = []
+    for tune in (False, True):
+        s = Shell(lay)
+        s.set_gc_all()
+        s.background_polytrope(20.0)
+        s.init_synthetic(seed=19)
+        s.set_precision(prec)
+        s.advance(3, 0.5, graph=True)
+        if tune:
+            r = s.autotune(reps=3)
+            print(r, flush=True)
+            assert r["stage"] in (8, 16) and r["diffusion"] in (8, 16)
+            assert all(v > 0 for k in r["ms"].values() for v in k.values())
+            assert s.get_tiles(1 if prec == "fp32" else 0) == (r["stage"], r["diffusion"])
+        s.advance(4, 0.5, graph=True)
+        runs.append(_states(s, s.nblocks))
+    assert np.array_equal(runs[0], runs[1])
 
 
 def test_fp32_deviation_error_after_100_steps():
