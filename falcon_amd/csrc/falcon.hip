@@ -995,11 +995,17 @@ int falcon_autotune(falcon_handle_t h, int reps, int *tiles_out, double *ms_out)
         double *U = h->U[cur].as<double>(), *Un = h->U[1 - cur].as<double>();
         double *dtp = h->dtp.as<double>(), *part = h->partial.as<double>();
         const int32_t *pull = (p == 1 && h->ghost_pull) ? h->gh32.pull : nullptr;
-        // a time step for the timed launches to scale by
+        // a time step for the timed launches to scale by; the diffusion launches
+        // then overwrite the partial minima
         launch_dt_final(d, part, dtp, 0.5, s, h->tile_diff[p]);
-        hipEvent_t e0, e1;
-        HIPCHECK(hipEventCreate(&e0));
-        HIPCHECK(hipEventCreate(&e1));
+        h->partials_valid = false;
+        // destroyed on every exit, an error included
+        struct Ev {
+            hipEvent_t e = nullptr;
+            Ev() { HIPCHECK(hipEventCreate(&e)); }
+            ~Ev() { if (e) (void)hipEventDestroy(e); }
+        } ev0, ev1;
+        hipEvent_t e0 = ev0.e, e1 = ev1.e;
         auto time = [&](auto &&launch) -> double {
             launch();
             HIPCHECK(hipEventRecord(e0, s));
@@ -1033,9 +1039,6 @@ int falcon_autotune(falcon_handle_t h, int reps, int *tiles_out, double *ms_out)
             }
         }
         check_launch();
-        HIPCHECK(hipEventDestroy(e0));
-        HIPCHECK(hipEventDestroy(e1));
-        h->partials_valid = false;
         const int ts = ms[1] < ms[0] ? 8 : 16, td = ms[3] < ms[2] ? 8 : 16;
         apply_tiles(h, p, ts, td);
         if (tiles_out) { tiles_out[0] = ts; tiles_out[1] = td; }
