@@ -56,8 +56,8 @@ def load():
         "falcon_init_synthetic": [vp, C.c_uint64, C.c_double, C.c_double],
         "falcon_set_precision": [vp, C.c_int],
         "falcon_set_ghost_pull": [vp, C.c_int],
-        "falcon_set_tiles": [vp, C.c_int, C.c_int, C.c_int],
-        "falcon_get_tiles": [vp, C.c_int, ip, ip],
+        "falcon_set_tiles": [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
+        "falcon_get_tiles": [vp, C.c_int, ip],
         "falcon_autotune": [vp, C.c_int, ip, dp],
         "falcon_sendrecv_gc": [vp, C.c_int],
         "falcon_advance_rk4": [vp, C.c_double, dp],
@@ -246,26 +246,33 @@ class Shell:
         ghost (off)."""
         _check(self.L.falcon_set_ghost_pull(self.h, 1 if on else 0))
 
-    def set_tiles(self, precision, stage, diffusion):
-        """(theta, phi) tile edges, 8 or 16, of the stage and diffusion marches
-        of one precision (0 FP64, 1 FP32 deviation); results do not depend on
-        them."""
-        _check(self.L.falcon_set_tiles(self.h, int(precision), int(stage), int(diffusion)))
+    def set_tiles(self, precision, stage, diffusion, stage_wgs=0, diffusion_wgs=0):
+        """Launch shapes of the stage and diffusion marches of one precision
+        (0 FP64, 1 FP32 deviation): (theta, phi) tile edge, 8 or 16, and
+        workgroups (0: one per tile; n > 0: n workgroups looping over the
+        tiles).  Results do not depend on them."""
+        _check(self.L.falcon_set_tiles(self.h, int(precision), int(stage), int(stage_wgs), int(diffusion),
+                                       int(diffusion_wgs)))
 
     def get_tiles(self, precision):
-        st, df = C.c_int32(0), C.c_int32(0)
-        _check(self.L.falcon_get_tiles(self.h, int(precision), C.byref(st), C.byref(df)))
-        return st.value, df.value
+        """(stage edge, stage workgroups, diffusion edge, diffusion workgroups)."""
+        sh = (C.c_int32 * 4)()
+        _check(self.L.falcon_get_tiles(self.h, int(precision), sh))
+        return tuple(sh)
+
+    SHAPES = ("16/tile", "16/persistent", "8/tile", "8/persistent")
 
     def autotune(self, reps=5):
-        """Time the stage and diffusion kernels of the current precision with
-        both tile edges on the current state and keep the faster edge of each.
-        Returns {"stage": edge, "diffusion": edge, "ms": {kernel: {edge: ms}}}."""
-        tiles = (C.c_int32 * 2)()
-        ms = (C.c_double * 4)()
-        _check(self.L.falcon_autotune(self.h, int(reps), tiles, ms))
-        return {"stage": tiles[0], "diffusion": tiles[1],
-                "ms": {"stage": {16: ms[0], 8: ms[1]}, "diffusion": {16: ms[2], 8: ms[3]}}}
+        """Time the stage and diffusion kernels of the current precision in
+        four launch shapes (tile edge 16 or 8, one workgroup per tile or the
+        persistent grid) on the current state and keep the fastest of each.
+        Returns {"stage": [edge, wgs], "diffusion": [edge, wgs],
+        "ms": {kernel: {shape: ms}}}."""
+        sh = (C.c_int32 * 4)()
+        ms = (C.c_double * 8)()
+        _check(self.L.falcon_autotune(self.h, int(reps), sh, ms))
+        return {"stage": [sh[0], sh[1]], "diffusion": [sh[2], sh[3]],
+                "ms": {"stage": dict(zip(self.SHAPES, ms[0:4])), "diffusion": dict(zip(self.SHAPES, ms[4:8]))}}
 
     def sendrecv_gc(self, which=0):
         _check(self.L.falcon_sendrecv_gc(self.h, which))

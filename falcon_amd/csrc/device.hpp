@@ -63,14 +63,19 @@ struct DevBoundary {
 // launchers (kernels.hip)
 // pull (null: none): the pull table of DevGhosts; ghosts of `in` in the
 // directions it lists are read from the neighbour's interior instead.
-// tile: (theta, phi) tile edge of the march, 8 or 16 (results do not depend on
-// it); the dt partial minima are laid out per tile of the diffusion's edge.
+// tile: (theta, phi) tile edge of the march, 8 or 16; wgs: 0 launches one
+// workgroup per tile, n > 0 launches n workgroups that loop over the tiles (a
+// persistent launch).  Results depend on neither; the dt partial minima are
+// laid out per tile of the diffusion's edge.
 template <typename T, typename TIN, typename TOUT, typename TB>
 void launch_stage(const Dims &d, const DevTables &t, const TIN *in, const TB *U, TOUT *out, float *shadow,
-                  const double *dtp, double div, hipStream_t s, const int32_t *pull, int tile);
+                  const double *dtp, double div, hipStream_t s, const int32_t *pull, int tile, int wgs);
 template <typename T, typename TIN>
 void launch_diffusion(const Dims &d, const DevTables &t, const TIN *in, const double *U, double *out, float *shadow,
-                      double *partial, const double *dtp, hipStream_t s, const int32_t *pull, int tile);
+                      double *partial, const double *dtp, hipStream_t s, const int32_t *pull, int tile, int wgs);
+// workgroups of the stage (kernel 0) or diffusion (1) march of a precision that
+// the device holds resident at once: the persistent grid
+int march_resident_wgs(int kernel, int precision, int tile);
 // every ghost entry of g
 template <typename TS>
 void launch_exchange(const Dims &d, const DevGhosts &g, TS *state, hipStream_t s);
@@ -86,8 +91,9 @@ void launch_dt_final(const Dims &d, const double *partial, double *dtp, double c
 void launch_to_f32(const double *src, float *dst, long long n, hipStream_t s);
 // single-operator entry points used by parity tests: R only
 template <typename T, typename TIN>
-void launch_rhs_only(const Dims &d, const DevTables &t, const TIN *in, double *R, hipStream_t s, int tile);
+void launch_rhs_only(const Dims &d, const DevTables &t, const TIN *in, double *R, hipStream_t s, int tile, int wgs);
 template <typename T>
-void launch_diffusion_only(const Dims &d, const DevTables &t, const double *U, double *R, hipStream_t s, int tile);
+void launch_diffusion_only(const Dims &d, const DevTables &t, const double *U, double *R, hipStream_t s, int tile,
+                           int wgs);
 
 }  // namespace falcon

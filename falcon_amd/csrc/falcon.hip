@@ -104,9 +104,12 @@ struct falcon_handle {
     std::vector<int> *prof_kind = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_cur = -1, graph_prec = -1;
-    // (theta, phi) tile edge of the stage and diffusion marches (8 or 16), per
-    // precision (0 FP64, 1 FP32); falcon_autotune picks them by timing
+    // launch shape of the stage and diffusion marches per precision (0 FP64,
+    // 1 FP32): (theta, phi) tile edge, 8 or 16, and workgroups (0: one per
+    // tile; n > 0: n workgroups looping over the tiles); falcon_autotune picks
+    // them by timing
     int tile_stage[2] = {16, 16}, tile_diff[2] = {16, 16};
+    int wgs_stage[2] = {0, 0}, wgs_diff[2] = {0, 0};
     std::unique_ptr<HeatSolver> heat;
     HeatParams heat_params;
     std::unique_ptr<CnnModel> cnn;
@@ -478,6 +481,7 @@ void enqueue_step(falcon_handle *h, double cfl, int cur) {
     double *U = h->U[cur].as<double>(), *Un = h->U[1 - cur].as<double>();
     double *dtp = h->dtp.as<double>(), *part = h->partial.as<double>();
     const int ts = h->tile_stage[h->precision], td = h->tile_diff[h->precision];
+    const int ws = h->wgs_stage[h->precision], wd = h->wgs_diff[h->precision];
     timed(h, K_DT, [&] { launch_dt_final(d, part, dtp, cfl, s, td); });
     if (h->precision == 0) {
         double *S0 = h->S64[0].as<double>(), *S1 = h->S64[1].as<double>();
@@ -487,11 +491,11 @@ void enqueue_step(falcon_handle *h, double cfl, int cur) {
             timed(h, K_BC, [&] { launch_boundary<double>(d, h->bnd, const_cast<double *>(in[k - 1]), s); });
             timed(h, K_STAGE, [&] {
                 launch_stage<double, double, double, double>(d, t, in[k - 1], U, out[k - 1], nullptr, dtp,
-                                                             (double)(5.0f - (float)k), s, nullptr, ts);
+                                                             (double)(5.0f - (float)k), s, nullptr, ts, ws);
             });
             timed(h, k < 4 ? K_XSTAGE : K_XSTATE, [&] { launch_exchange<double>(d, h->gh64, out[k - 1], s); });
         }
-        timed(h, K_DIFF, [&] { launch_diffusion<double, double>(d, t, U, U, Un, nullptr, part, dtp, s, nullptr, td); });
+        timed(h, K_DIFF, [&] { launch_diffusion<double, double>(d, t, U, U, Un, nullptr, part, dtp, s, nullptr, td, wd); });
         timed(h, K_OTHER, [&] { launch_copy_radial<double>(d, h->bnd, U, Un, s); });
         timed(h, K_XSTATE, [&] { launch_exchange<double>(d, h->gh64, Un, s); });
     } else {
@@ -513,16 +517,16 @@ void enqueue_step(falcon_handle *h, double cfl, int cur) {
             if (k < 4) {
                 // intermediate stage values are FP32 temporaries: base = FP32 copy of U^n
                 timed(h, K_STAGE, [&] {
-                    launch_stage<float, float, float, float>(d, t, in[k - 1], F, out[k - 1], nullptr, dtp, div, s, pull, ts);
+                    launch_stage<float, float, float, float>(d, t, in[k - 1], F, out[k - 1], nullptr, dtp, div, s, pull, ts, ws);
                 });
                 timed(h, K_XSTAGE, [&] { exchange(out[k - 1]); });
             } else {
                 timed(h, K_STAGE,
-                      [&] { launch_stage<float, float, double, double>(d, t, in[3], U, U, F, dtp, div, s, pull, ts); });
+                      [&] { launch_stage<float, float, double, double>(d, t, in[3], U, U, F, dtp, div, s, pull, ts, ws); });
                 timed(h, K_XSTATE, [&] { exchange(F); });
             }
         }
-        timed(h, K_DIFF, [&] { launch_diffusion<float, float>(d, t, F, U, Un, Fn, part, dtp, s, pull, td); });
+        timed(h, K_DIFF, [&] { launch_diffusion<float, float>(d, t, F, U, Un, Fn, part, dtp, s, pull, td, wd); });
         timed(h, K_XSTATE, [&] { exchange(Fn); });
     }
 }
@@ -944,11 +948,13 @@ int falcon_set_precision(falcon_handle_t h, int mode) {
     });
 }
 
-// Tile edges of precision p's marches; a change drops that precision's graph
-// and, where the diffusion's edge changes, the partial minima laid out by it.
-static void apply_tiles(falcon_handle *h, int p, int ts, int td) {
+// Launch shapes of precision p's marches; a change drops that precision's
+// graph and, where the diffusion's tile edge changes, the partial minima laid
+// out by it.
+static void apply_shape(falcon_handle *h, int p, int ts, int ws, int td, int wd) {
     if ((ts != 8 && ts != 16) || (td != 8 && td != 16)) throw Error("tile edges must be 8 or 16");
-    if (ts == h->tile_stage[p] && td == h->tile_diff[p]) return;
+    if (ws < 0 || wd < 0) throw Error("workgroup counts must be >= 0");
+    if (ts == h->tile_stage[p] && td == h->tile_diff[p] && ws == h->wgs_stage[p] && wd == h->wgs_diff[p]) return;
     if (h->graph && h->graph_prec == p) {
         HIPCHECK(hipGraphExecDestroy(h->graph));
         h->graph = nullptr;
@@ -956,32 +962,43 @@ static void apply_tiles(falcon_handle *h, int p, int ts, int td) {
     if (p == h->precision && td != h->tile_diff[p]) h->partials_valid = false;
     h->tile_stage[p] = ts;
     h->tile_diff[p] = td;
+    h->wgs_stage[p] = ws;
+    h->wgs_diff[p] = wd;
 }
 
-int falcon_set_tiles(falcon_handle_t h, int precision, int stage_tile, int diffusion_tile) {
+int falcon_set_tiles(falcon_handle_t h, int precision, int stage_tile, int stage_wgs, int diffusion_tile,
+                     int diffusion_wgs) {
     return guard([&] {
         if (precision != 0 && precision != 1) throw Error("precision must be 0 (FP64) or 1 (FP32 deviation)");
-        apply_tiles(h, precision, stage_tile, diffusion_tile);
+        apply_shape(h, precision, stage_tile, stage_wgs, diffusion_tile, diffusion_wgs);
     });
 }
 
-int falcon_get_tiles(falcon_handle_t h, int precision, int *stage_tile, int *diffusion_tile) {
+int falcon_get_tiles(falcon_handle_t h, int precision, int *shape) {
     return guard([&] {
         if (precision != 0 && precision != 1) throw Error("precision must be 0 (FP64) or 1 (FP32 deviation)");
-        if (stage_tile) *stage_tile = h->tile_stage[precision];
-        if (diffusion_tile) *diffusion_tile = h->tile_diff[precision];
+        if (!shape) throw Error("null output");
+        shape[0] = h->tile_stage[precision];
+        shape[1] = h->wgs_stage[precision];
+        shape[2] = h->tile_diff[precision];
+        shape[3] = h->wgs_diff[precision];
     });
 }
 
-// Autotuner of the march tile edges.  For the current precision, times the
-// stage kernel (intermediate-stage form) and the diffusion kernel with each
-// edge, reps launches after one untimed launch, on the current state, and
-// keeps the faster edge of each.  The timed launches write only the stage
-// buffer S[0], the next-state buffers and the partial minima, none of which
-// holds anything between steps (the partial minima are recomputed before the
-// next step), so the state and every later result are unchanged.
-// ms_out (optional): stage 16, stage 8, diffusion 16, diffusion 8 in ms.
-int falcon_autotune(falcon_handle_t h, int reps, int *tiles_out, double *ms_out) {
+// Autotuner of the march launch shapes.  For the current precision, times the
+// stage kernel (intermediate-stage form) and the diffusion kernel in four
+// shapes, tile edge 16 or 8 with one workgroup per tile or with the persistent
+// grid (the workgroups the device holds resident at once), reps launches after
+// one untimed launch, on the current state, and keeps the fastest shape of
+// each.  The timed launches write only the stage buffer S[0], the next-state
+// buffers and the partial minima, none of which holds anything between steps
+// (the partial minima are recomputed before the next step), so the state and
+// every later result are unchanged.
+// shape_out (optional): stage edge, stage workgroups, diffusion edge,
+// diffusion workgroups.  ms_out (optional): ms per launch of the stage in the
+// shapes (16, per tile), (16, persistent), (8, per tile), (8, persistent),
+// then the diffusion in the same order.
+int falcon_autotune(falcon_handle_t h, int reps, int *shape_out, double *ms_out) {
     return guard([&] {
         if (reps < 1 || reps > 1000) throw Error("reps must be in 1..1000");
         h->need_tables();
@@ -1016,33 +1033,46 @@ int falcon_autotune(falcon_handle_t h, int reps, int *tiles_out, double *ms_out)
             HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
             return (double)ms / reps;
         };
-        const int edges[2] = {16, 8};
-        double ms[4];
-        for (int q = 0; q < 2; q++) {
-            const int tl = edges[q];
+        // shape c: edge 16 (c < 2) or 8, one workgroup per tile (c even) or persistent
+        int edge[4], wst[4], wdi[4];
+        double ms[8];
+        for (int c = 0; c < 4; c++) {
+            edge[c] = c < 2 ? 16 : 8;
+            wst[c] = (c & 1) ? march_resident_wgs(0, p, edge[c]) : 0;
+            wdi[c] = (c & 1) ? march_resident_wgs(1, p, edge[c]) : 0;
+        }
+        for (int c = 0; c < 4; c++) {
+            const int tl = edge[c], ws = wst[c], wd = wdi[c];
             if (p == 0) {
                 double *S0 = h->S64[0].as<double>();
-                ms[q] = time([&] {
-                    launch_stage<double, double, double, double>(d, t, U, U, S0, nullptr, dtp, 4.0, s, nullptr, tl);
+                ms[c] = time([&] {
+                    launch_stage<double, double, double, double>(d, t, U, U, S0, nullptr, dtp, 4.0, s, nullptr, tl, ws);
                 });
-                ms[2 + q] = time([&] {
-                    launch_diffusion<double, double>(d, t, U, U, Un, nullptr, part, dtp, s, nullptr, tl);
+                ms[4 + c] = time([&] {
+                    launch_diffusion<double, double>(d, t, U, U, Un, nullptr, part, dtp, s, nullptr, tl, wd);
                 });
             } else {
                 float *F = h->F32[cur].as<float>(), *Fn = h->F32[1 - cur].as<float>(), *S0 = h->S32[0].as<float>();
-                ms[q] = time([&] {
-                    launch_stage<float, float, float, float>(d, t, F, F, S0, nullptr, dtp, 4.0, s, pull, tl);
+                ms[c] = time([&] {
+                    launch_stage<float, float, float, float>(d, t, F, F, S0, nullptr, dtp, 4.0, s, pull, tl, ws);
                 });
-                ms[2 + q] = time([&] {
-                    launch_diffusion<float, float>(d, t, F, U, Un, Fn, part, dtp, s, pull, tl);
+                ms[4 + c] = time([&] {
+                    launch_diffusion<float, float>(d, t, F, U, Un, Fn, part, dtp, s, pull, tl, wd);
                 });
             }
         }
         check_launch();
-        const int ts = ms[1] < ms[0] ? 8 : 16, td = ms[3] < ms[2] ? 8 : 16;
-        apply_tiles(h, p, ts, td);
-        if (tiles_out) { tiles_out[0] = ts; tiles_out[1] = td; }
-        if (ms_out) for (int q = 0; q < 4; q++) ms_out[q] = ms[q];
+        int bs = 0, bd = 0;
+        for (int c = 1; c < 4; c++) {
+            if (ms[c] < ms[bs]) bs = c;
+            if (ms[4 + c] < ms[4 + bd]) bd = c;
+        }
+        apply_shape(h, p, edge[bs], wst[bs], edge[bd], wdi[bd]);
+        if (shape_out) {
+            shape_out[0] = edge[bs]; shape_out[1] = wst[bs];
+            shape_out[2] = edge[bd]; shape_out[3] = wdi[bd];
+        }
+        if (ms_out) for (int q = 0; q < 8; q++) ms_out[q] = ms[q];
     });
 }
 
@@ -1136,9 +1166,9 @@ int falcon_rhs(falcon_handle_t h, double *r_out) {
         refresh_u_ghosts(h);
         double *U = h->U[h->cur].as<double>();
         launch_boundary<double>(d, h->bnd, U, h->stream);
-        const int ts = h->tile_stage[h->precision];
-        if (h->precision == 0) launch_rhs_only<double, double>(d, h->dt, U, h->rbuf.as<double>(), h->stream, ts);
-        else launch_rhs_only<float, double>(d, h->dt, U, h->rbuf.as<double>(), h->stream, ts);
+        const int ts = h->tile_stage[h->precision], ws = h->wgs_stage[h->precision];
+        if (h->precision == 0) launch_rhs_only<double, double>(d, h->dt, U, h->rbuf.as<double>(), h->stream, ts, ws);
+        else launch_rhs_only<float, double>(d, h->dt, U, h->rbuf.as<double>(), h->stream, ts, ws);
         check_launch();
         std::vector<double> tmp(nn * 8 * d.nblocks);
         HIPCHECK(hipStreamSynchronize(h->stream));
@@ -1162,9 +1192,9 @@ int falcon_diffusion_rhs(falcon_handle_t h, double *r_out) {
         const size_t nn = (size_t)d.ni * d.nj * d.nk;
         h->rbuf.alloc(sizeof(double) * nn * 8 * d.nblocks);
         double *U = h->U[h->cur].as<double>();
-        const int td = h->tile_diff[h->precision];
-        if (h->precision == 0) launch_diffusion_only<double>(d, h->dt, U, h->rbuf.as<double>(), h->stream, td);
-        else launch_diffusion_only<float>(d, h->dt, U, h->rbuf.as<double>(), h->stream, td);
+        const int td = h->tile_diff[h->precision], wd = h->wgs_diff[h->precision];
+        if (h->precision == 0) launch_diffusion_only<double>(d, h->dt, U, h->rbuf.as<double>(), h->stream, td, wd);
+        else launch_diffusion_only<float>(d, h->dt, U, h->rbuf.as<double>(), h->stream, td, wd);
         check_launch();
         std::vector<double> tmp(nn * 8 * d.nblocks);
         HIPCHECK(hipStreamSynchronize(h->stream));

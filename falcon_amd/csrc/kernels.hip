@@ -16,6 +16,7 @@
 // the reference.  The FP32 instantiation (T = float) is the deviation path: the
 // stencil arithmetic runs in FP32 while the stage accumulation into the FP64
 // state, the ghost interpolation weights and the time step stay FP64.
+#include <algorithm>
 #include <stdexcept>
 
 #include "device.hpp"
@@ -294,7 +295,7 @@ __device__ __forceinline__ void cell_rhs(const T (&win)[8][5], LDSF L, const T x
 // MODE 1: write R (interior layout [block][var][i][j][k], FP64) for parity tests.
 // pull (null: none): ghost cells of `in` in the directions it lists are read
 // from the same-level neighbour's interior (Marcher).
-template <typename T, typename TIN, typename TOUT, typename TB, int MODE, bool SHADOW, int TL>
+template <typename T, typename TIN, typename TOUT, typename TB, int MODE, bool SHADOW, int TL, bool PERSIST>
 __global__ void __launch_bounds__(TL * TL) k_stage(Dims d, DevTables t, const TIN *__restrict__ in,
                                                const TB *__restrict__ U, TOUT *__restrict__ out,
                                                float *__restrict__ shadow, const double *__restrict__ dtp,
@@ -302,96 +303,105 @@ __global__ void __launch_bounds__(TL * TL) k_stage(Dims d, DevTables t, const TI
     constexpr int PL = tile_pl<TL>();
     __shared__ T lds[2][8][PL][PL];
     const int ntk = (d.nk + TL - 1) / TL, tpb = ((d.nj + TL - 1) / TL) * ntk;
-    const int b = blockIdx.x / tpb;
-    const int trem = blockIdx.x - b * tpb;
-    const int tj = trem / ntk, tk = trem - tj * ntk;
-    const int j0 = tj * TL, k0 = tk * TL;
-    const int TJa = min(TL, d.nj - j0), TKa = min(TL, d.nk - k0);
-    const int tx = threadIdx.x & (TL - 1), ty = threadIdx.x / TL;
+    const int ntot = d.nblocks * tpb;
+    // PERSIST: a grid smaller than the tile count (a persistent launch) takes
+    // the next tile at a stride of the grid size; otherwise one pass, tile =
+    // the workgroup
+    for (int tile = blockIdx.x; tile < ntot; tile += gridDim.x) {
+        const int b = tile / tpb;
+        const int trem = tile - b * tpb;
+        const int tj = trem / ntk, tk = trem - tj * ntk;
+        const int j0 = tj * TL, k0 = tk * TL;
+        const int TJa = min(TL, d.nj - j0), TKa = min(TL, d.nk - k0);
+        const int tx = threadIdx.x & (TL - 1), ty = threadIdx.x / TL;
 
-    constexpr bool PULL = sizeof(T) == 4 && MODE == 0;
-    Marcher<T, TIN, PULL, TL> m;
-    m.init(d, in, b, j0, k0, TJa, TKa, pull);
+        constexpr bool PULL = sizeof(T) == 4 && MODE == 0;
+        Marcher<T, TIN, PULL, TL> m;
+        m.init(d, in, b, j0, k0, TJa, TKa, pull);
 
-    const long long rb = (long long)b * d.NI, tb = (long long)b * d.NJ;
-    const BlockMeta mt = t.meta[b];
-    const T cr = (T)mt.cr, ct = (T)mt.ct, cp = (T)mt.cp;
-    // theta rows are read only by active lanes (inactive lanes of a partial
-    // tile have J beyond the padded range)
-    T st[5];
-#pragma unroll
-    for (int q = 0; q < 5; q++) st[q] = m.act ? (T)t.sint[tb + m.J - 2 + q] : T(1);
-    const T sinv = m.act ? (T)t.sinv[tb + m.J] : T(1), cot = m.act ? (T)t.cot[tb + m.J] : T(0);
-    const double dt = MODE == 0 ? *dtp : 0.0;
+        const long long rb = (long long)b * d.NI, tb = (long long)b * d.NJ;
+        const BlockMeta mt = t.meta[b];
+        const T cr = (T)mt.cr, ct = (T)mt.ct, cp = (T)mt.cp;
+        // theta rows are read only by active lanes (inactive lanes of a partial
+        // tile have J beyond the padded range)
+        T st[5];
+    #pragma unroll
+        for (int q = 0; q < 5; q++) st[q] = m.act ? (T)t.sint[tb + m.J - 2 + q] : T(1);
+        const T sinv = m.act ? (T)t.sinv[tb + m.J] : T(1), cot = m.act ? (T)t.cot[tb + m.J] : T(0);
+        const double dt = MODE == 0 ? *dtp : 0.0;
 
-    T win[8][5];
-    T nxt[8], hn[8];
-#pragma unroll
-    for (int v = 0; v < 8; v++) {
-#pragma unroll
-        for (int q = 0; q < 5; q++) win[v][q] = m.act ? m.col(v, q) : T(0);
-        hn[v] = m.hv ? m.halo(v, d.ng) : T(0);
-    }
-
-    for (int i = 0; i < d.ni; i++) {
-        const int I = i + d.ng;
-        const int buf = i & 1;
-        // stage this plane: own column centre plus halo strips
-#pragma unroll
+        T win[8][5];
+        T nxt[8], hn[8];
+    #pragma unroll
         for (int v = 0; v < 8; v++) {
-            if (m.act) lds[buf][v][ty + 2][tx + 2] = win[v][2];
-            if (m.hv) lds[buf][v][m.hly][m.hlx] = hn[v];
+    #pragma unroll
+            for (int q = 0; q < 5; q++) win[v][q] = m.act ? m.col(v, q) : T(0);
+            hn[v] = m.hv ? m.halo(v, d.ng) : T(0);
         }
-        // prefetch the next window plane and the next halo
-        const bool more = i + 1 < d.ni;
-#pragma unroll
-        for (int v = 0; v < 8; v++) {
-            nxt[v] = (m.act && more) ? m.col(v, I + 3) : T(0);
-            hn[v] = (m.hv && more) ? m.halo(v, I + 1) : T(0);
-        }
-        __syncthreads();
-        if (m.act) {
-            T xr[5], rho0r[5], gpr[5], gqr[5];
-#pragma unroll
-            for (int q = 0; q < 5; q++) {
-                xr[q] = (T)t.xi[rb + I - 2 + q];
-                rho0r[q] = (T)t.rho0[rb + I - 2 + q];
-                gpr[q] = (T)t.gp[rb + I - 2 + q];
-                gqr[q] = (T)t.gq[rb + I - 2 + q];
+
+        for (int i = 0; i < d.ni; i++) {
+            const int I = i + d.ng;
+            const int buf = i & 1;
+            // stage this plane: own column centre plus halo strips
+    #pragma unroll
+            for (int v = 0; v < 8; v++) {
+                if (m.act) lds[buf][v][ty + 2][tx + 2] = win[v][2];
+                if (m.hv) lds[buf][v][m.hly][m.hlx] = hn[v];
             }
-            auto L = [&](int v, int dj, int dk) -> T { return lds[buf][v][ty + 2 + dj][tx + 2 + dk]; };
-            T R[8];
-            Balance64 bal;
-            if (sizeof(T) == 4) {
-                bal.gp = t.gp + rb + I - 2;
-                bal.gq = t.gq + rb + I - 2;
-                bal.cr = mt.cr;
-                bal.rho0inv = 1.0 / t.rho0[rb + I];
-                bal.gor = t.g_over_rho0[rb + I];
+            // prefetch the next window plane and the next halo
+            const bool more = i + 1 < d.ni;
+    #pragma unroll
+            for (int v = 0; v < 8; v++) {
+                nxt[v] = (m.act && more) ? m.col(v, I + 3) : T(0);
+                hn[v] = (m.hv && more) ? m.halo(v, I + 1) : T(0);
             }
-            cell_rhs<T>(win, L, xr, rho0r, gpr, gqr, st, (T)t.rinv[rb + I], (T)t.r2inv[rb + I], (T)t.xir2[rb + I],
-                        (T)t.heat[rb + I], (T)t.g_over_rho0[rb + I], sinv, cot, cr, ct, cp, R, bal);
-            if (MODE == 0) {
-                const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + (long long)m.J * m.SJ + m.K;
-#pragma unroll
-                for (int v = 0; v < 8; v++) {
-                    const long long q = base + (long long)v * d.P;
-                    const double x = (double)U[q] + (dt * (double)R[v]) / div;
-                    out[q] = (TOUT)x;
-                    if (SHADOW) shadow[q] = (float)x;
+            __syncthreads();
+            if (m.act) {
+                T xr[5], rho0r[5], gpr[5], gqr[5];
+    #pragma unroll
+                for (int q = 0; q < 5; q++) {
+                    xr[q] = (T)t.xi[rb + I - 2 + q];
+                    rho0r[q] = (T)t.rho0[rb + I - 2 + q];
+                    gpr[q] = (T)t.gp[rb + I - 2 + q];
+                    gqr[q] = (T)t.gq[rb + I - 2 + q];
                 }
-            } else {
-                const long long nn = (long long)d.ni * d.nj * d.nk;
-                const long long base = (long long)b * 8 * nn + ((long long)i * d.nj + (j0 + ty)) * d.nk + (k0 + tx);
-#pragma unroll
-                for (int v = 0; v < 8; v++) ((double *)out)[base + (long long)v * nn] = (double)R[v];
+                auto L = [&](int v, int dj, int dk) -> T { return lds[buf][v][ty + 2 + dj][tx + 2 + dk]; };
+                T R[8];
+                Balance64 bal;
+                if (sizeof(T) == 4) {
+                    bal.gp = t.gp + rb + I - 2;
+                    bal.gq = t.gq + rb + I - 2;
+                    bal.cr = mt.cr;
+                    bal.rho0inv = 1.0 / t.rho0[rb + I];
+                    bal.gor = t.g_over_rho0[rb + I];
+                }
+                cell_rhs<T>(win, L, xr, rho0r, gpr, gqr, st, (T)t.rinv[rb + I], (T)t.r2inv[rb + I], (T)t.xir2[rb + I],
+                            (T)t.heat[rb + I], (T)t.g_over_rho0[rb + I], sinv, cot, cr, ct, cp, R, bal);
+                if (MODE == 0) {
+                    const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + (long long)m.J * m.SJ + m.K;
+    #pragma unroll
+                    for (int v = 0; v < 8; v++) {
+                        const long long q = base + (long long)v * d.P;
+                        const double x = (double)U[q] + (dt * (double)R[v]) / div;
+                        out[q] = (TOUT)x;
+                        if (SHADOW) shadow[q] = (float)x;
+                    }
+                } else {
+                    const long long nn = (long long)d.ni * d.nj * d.nk;
+                    const long long base = (long long)b * 8 * nn + ((long long)i * d.nj + (j0 + ty)) * d.nk + (k0 + tx);
+    #pragma unroll
+                    for (int v = 0; v < 8; v++) ((double *)out)[base + (long long)v * nn] = (double)R[v];
+                }
+            }
+    #pragma unroll
+            for (int v = 0; v < 8; v++) {
+                win[v][0] = win[v][1]; win[v][1] = win[v][2]; win[v][2] = win[v][3]; win[v][3] = win[v][4];
+                win[v][4] = nxt[v];
             }
         }
-#pragma unroll
-        for (int v = 0; v < 8; v++) {
-            win[v][0] = win[v][1]; win[v][1] = win[v][2]; win[v][2] = win[v][3]; win[v][3] = win[v][4];
-            win[v][4] = nxt[v];
-        }
+        if (!PERSIST) break;
+        // the next tile's LDS writes wait for this tile's last reads
+        if (tile + (int)gridDim.x < ntot) __syncthreads();
     }
 }
 
@@ -406,7 +416,7 @@ __global__ void __launch_bounds__(TL * TL) k_stage(Dims d, DevTables t, const TI
 // null) receives the same value in FP32, and the workgroup's min(h/v_wave) of the
 // new state (the next step's ModTimeStep_Dynamo input) goes to partial[tile].
 // MODE 1: write R.  `in` is the stencil input (the state or its FP32 shadow).
-template <typename T, typename TIN, int MODE, int TL>
+template <typename T, typename TIN, int MODE, int TL, bool PERSIST>
 __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, const TIN *__restrict__ in,
                                                    const double *__restrict__ U, double *__restrict__ out,
                                                    float *__restrict__ shadow, double *__restrict__ partial,
@@ -426,195 +436,204 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
     // the first barrier of the march
     __shared__ long long pdel[27];
     const int ntk = (d.nk + TL - 1) / TL, tpb = ((d.nj + TL - 1) / TL) * ntk;
-    const int b = blockIdx.x / tpb;
-    if (sizeof(T) == 4 && MODE == 0 && threadIdx.x < 27) {
-        const int c = threadIdx.x;
-        const int D = pull ? pull[b * 27 + c] : -1;
-        const int di = c / 9 - 1, dj = (c / 3) % 3 - 1, dk = c % 3 - 1;
-        pdel[c] = D < 0 ? 0ll
-                        : (long long)(D - b) * 8 * d.P - (long long)di * d.ni * (d.NJ * d.NK) -
-                              (long long)dj * d.nj * d.NK - dk * d.nk;
-    }
-    const int trem = blockIdx.x - b * tpb;
-    const int tj = trem / ntk, tk = trem - tj * ntk;
-    const int j0 = tj * TL, k0 = tk * TL;
-    const int TJa = min(TL, d.nj - j0), TKa = min(TL, d.nk - k0);
-    const int tx = threadIdx.x & (TL - 1), ty = threadIdx.x / TL;
-
-    constexpr bool PULL = sizeof(T) == 4 && MODE == 0;
-    Marcher<T, TIN, PULL, TL> m;
-    m.init(d, in, b, j0, k0, TJa, TKa, pull);
-    const long long rb = (long long)b * d.NI, tb = (long long)b * d.NJ;
-    const BlockMeta mt = t.meta[b];
-    const T dxi = (T)mt.dxi, dxj = (T)mt.dxj, dxk = (T)mt.dxk;
-    double hmin = INFINITY;
-    const double sint64 = m.act ? t.sint[tb + m.J] : 1.0;
-    const T sintc = m.act ? (T)t.sint[tb + m.J] : T(1);
-    const double dt = MODE == 0 ? *dtp : 0.0;
-
-    T win[8][5];
-    T nxt[8], hn[8];
-    // the upper r-face flux of plane i is the lower r-face flux of plane i+1:
-    // same four values, same speed sum (speed at the upper cell first), so it
-    // is carried instead of recomputed.  The same holds for the theta and phi
-    // faces between neighbouring lanes, which go through fj / fk.
-    T fr[8];
-    bool have_fr = false;
-#pragma unroll
-    for (int v = 0; v < 8; v++) {
-#pragma unroll
-        for (int q = 0; q < 5; q++) win[v][q] = m.act ? m.col(v, q) : T(0);
-        hn[v] = m.hv ? m.halo(v, d.ng) : T(0);
-        fr[v] = T(0);
-    }
-    const TIN *Ub = in + (long long)b * 8 * d.P;
-    // a ghost in a direction listed in the pull table is the same-level
-    // neighbour's interior cell, translated by n
-    auto at = [&](int v, int II, int JJ, int KK) -> T {
-        long long o = (long long)v * d.P + (long long)II * m.SI + (long long)JJ * m.SJ + KK;
-        if (PULL) {
-            const int c = (II < d.ng ? 0 : (II >= d.ng + d.ni ? 18 : 9)) +
-                          (JJ < d.ng ? 0 : (JJ >= d.ng + d.nj ? 6 : 3)) +
-                          (KK < d.ng ? 0 : (KK >= d.ng + d.nk ? 2 : 1));
-            o += pdel[c];
+    const int ntot = d.nblocks * tpb;
+    // PERSIST: a grid smaller than the tile count (a persistent launch) takes
+    // the next tile at a stride of the grid size; otherwise one pass, tile =
+    // the workgroup
+    for (int tile = blockIdx.x; tile < ntot; tile += gridDim.x) {
+        const int b = tile / tpb;
+        if (sizeof(T) == 4 && MODE == 0 && threadIdx.x < 27) {
+            const int c = threadIdx.x;
+            const int D = pull ? pull[b * 27 + c] : -1;
+            const int di = c / 9 - 1, dj = (c / 3) % 3 - 1, dk = c % 3 - 1;
+            pdel[c] = D < 0 ? 0ll
+                            : (long long)(D - b) * 8 * d.P - (long long)di * d.ni * (d.NJ * d.NK) -
+                                  (long long)dj * d.nj * d.NK - dk * d.nk;
         }
-        return (T)Ub[o];
-    };
-    // faces on the tile's lower edges: lanes 0 .. TL-1 take the theta face
-    // below row 0 in column x = lane, lanes TL .. 2TL-1 the phi face left of
-    // column 0 in row y = lane - TL
-    const int lane = threadIdx.x;
-    const bool bact = lane < TL ? lane < TKa : (lane < 2 * TL && lane - TL < TJa);
+        const int trem = tile - b * tpb;
+        const int tj = trem / ntk, tk = trem - tj * ntk;
+        const int j0 = tj * TL, k0 = tk * TL;
+        const int TJa = min(TL, d.nj - j0), TKa = min(TL, d.nk - k0);
+        const int tx = threadIdx.x & (TL - 1), ty = threadIdx.x / TL;
 
-    for (int i = 0; i < d.ni; i++) {
-        const int I = i + d.ng;
-#pragma unroll
+        constexpr bool PULL = sizeof(T) == 4 && MODE == 0;
+        Marcher<T, TIN, PULL, TL> m;
+        m.init(d, in, b, j0, k0, TJa, TKa, pull);
+        const long long rb = (long long)b * d.NI, tb = (long long)b * d.NJ;
+        const BlockMeta mt = t.meta[b];
+        const T dxi = (T)mt.dxi, dxj = (T)mt.dxj, dxk = (T)mt.dxk;
+        double hmin = INFINITY;
+        const double sint64 = m.act ? t.sint[tb + m.J] : 1.0;
+        const T sintc = m.act ? (T)t.sint[tb + m.J] : T(1);
+        const double dt = MODE == 0 ? *dtp : 0.0;
+
+        T win[8][5];
+        T nxt[8], hn[8];
+        // the upper r-face flux of plane i is the lower r-face flux of plane i+1:
+        // same four values, same speed sum (speed at the upper cell first), so it
+        // is carried instead of recomputed.  The same holds for the theta and phi
+        // faces between neighbouring lanes, which go through fj / fk.
+        T fr[8];
+        bool have_fr = false;
+    #pragma unroll
         for (int v = 0; v < 8; v++) {
-            if (m.act) lds[v][ty + 2][tx + 2] = win[v][2];
-            if (m.hv) lds[v][m.hly][m.hlx] = hn[v];
+    #pragma unroll
+            for (int q = 0; q < 5; q++) win[v][q] = m.act ? m.col(v, q) : T(0);
+            hn[v] = m.hv ? m.halo(v, d.ng) : T(0);
+            fr[v] = T(0);
         }
-        const bool more = i + 1 < d.ni;
-#pragma unroll
-        for (int v = 0; v < 8; v++) {
-            nxt[v] = (m.act && more) ? m.col(v, I + 3) : T(0);
-            hn[v] = (m.hv && more) ? m.halo(v, I + 1) : T(0);
-        }
-        __syncthreads();
-        const T csI2 = (T)t.cs[rb + I - 2];
-        auto Lv = [&](int v, int dj, int dk) -> T { return lds[v][ty + 2 + dj][tx + 2 + dk]; };
-        // U^n of this plane, loaded before the face exchange (FP32 path, MODE 0)
-        constexpr bool UEARLY = MODE == 0 && sizeof(T) == 4;
-        double ucur[8];
-        if (m.act) {
-            const int J = m.J, K = m.K;
-            if (UEARLY) {
-                const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + (long long)J * m.SJ + K;
-#pragma unroll
-                for (int v = 0; v < 8; v++) ucur[v] = U[base + (long long)v * d.P];
+        const TIN *Ub = in + (long long)b * 8 * d.P;
+        // a ghost in a direction listed in the pull table is the same-level
+        // neighbour's interior cell, translated by n
+        auto at = [&](int v, int II, int JJ, int KK) -> T {
+            long long o = (long long)v * d.P + (long long)II * m.SI + (long long)JJ * m.SJ + KK;
+            if (PULL) {
+                const int c = (II < d.ng ? 0 : (II >= d.ng + d.ni ? 18 : 9)) +
+                              (JJ < d.ng ? 0 : (JJ >= d.ng + d.nj ? 6 : 3)) +
+                              (KK < d.ng ? 0 : (KK >= d.ng + d.nk ? 2 : 1));
+                o += pdel[c];
             }
-            // speeds |v_d| + c_s at the cells the reference reads (see above)
-            const T c20 = fabs(at(2, I - 2, J, K - 2)) + csI2;
-            const T c2p = fabs(at(2, I - 2, J + 1, K - 2)) + csI2;
-            const T c30 = fabs(at(3, I - 2, J - 2, K)) + csI2;
-            const T c3p = fabs(at(3, I - 2, J - 2, K + 1)) + csI2;
-#pragma unroll
+            return (T)Ub[o];
+        };
+        // faces on the tile's lower edges: lanes 0 .. TL-1 take the theta face
+        // below row 0 in column x = lane, lanes TL .. 2TL-1 the phi face left of
+        // column 0 in row y = lane - TL
+        const int lane = threadIdx.x;
+        const bool bact = lane < TL ? lane < TKa : (lane < 2 * TL && lane - TL < TJa);
+
+        for (int i = 0; i < d.ni; i++) {
+            const int I = i + d.ng;
+    #pragma unroll
             for (int v = 0; v < 8; v++) {
-                // upper theta face (the lower one of the cell at row + 1)
-                fj[v][ty + 1][tx] = (face_flux(Lv(v, -1, 0), Lv(v, 0, 0), Lv(v, 1, 0), Lv(v, 2, 0)) * (c2p + c20)) * T(0.5);
-                // upper phi face
-                fk[v][ty][tx + 1] = (face_flux(Lv(v, 0, -1), Lv(v, 0, 0), Lv(v, 0, 1), Lv(v, 0, 2)) * (c3p + c30)) * T(0.5);
+                if (m.act) lds[v][ty + 2][tx + 2] = win[v][2];
+                if (m.hv) lds[v][m.hly][m.hlx] = hn[v];
             }
-        }
-        if (bact) {
-            // the lower face of the edge cell: its four-cell stencil in the LDS
-            // plane (start, step), its speed cells (J0, K0) on the upper side and
-            // (J1, K1) on the lower side
-            const bool bj = lane < TL;
-            const int bx = bj ? lane : 0, by = bj ? 0 : lane - TL;
-            const int bstart = bj ? bx + 2 : (by + 2) * PL, bstep = bj ? PL : 1;
-            const int bvar = bj ? 2 : 3;
-            const int bJ0 = bj ? j0 + d.ng : j0 + by + d.ng - 2, bK0 = bj ? k0 + bx + d.ng - 2 : k0 + d.ng;
-            const int bJ1 = bj ? bJ0 - 1 : bJ0, bK1 = bj ? bK0 : bK0 - 1;
-            const T s0 = fabs(at(bvar, I - 2, bJ0, bK0)) + csI2;
-            const T s1 = fabs(at(bvar, I - 2, bJ1, bK1)) + csI2;
-#pragma unroll
+            const bool more = i + 1 < d.ni;
+    #pragma unroll
             for (int v = 0; v < 8; v++) {
-                const T *pv = &lds[v][0][0] + bstart;
-                const T f = (face_flux(pv[0], pv[bstep], pv[2 * bstep], pv[3 * bstep]) * (s0 + s1)) * T(0.5);
-                if (bj) fj[v][0][bx] = f;
-                else fk[v][by][0] = f;
+                nxt[v] = (m.act && more) ? m.col(v, I + 3) : T(0);
+                hn[v] = (m.hv && more) ? m.halo(v, I + 1) : T(0);
             }
-        }
-        __syncthreads();
-        if (m.act) {
-            const int J = m.J, K = m.K;
-            const T csI = (T)t.cs[rb + I], csIm = (T)t.cs[rb + I - 1], csIp = (T)t.cs[rb + I + 1];
-            const T c1m = fabs(at(1, I - 1, J - 2, K - 2)) + csIm;
-            const T c10 = fabs(at(1, I, J - 2, K - 2)) + csI;
-            const T c1p = fabs(at(1, I + 1, J - 2, K - 2)) + csIp;
-            const T xic = (T)t.xi[rb + I];
-            const T L2 = dxj * xic;
-            const T L3 = (dxk * xic) * sintc;
-            const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + (long long)J * m.SJ + K;
-            const long long nn = (long long)d.ni * d.nj * d.nk;
-            const long long rbase = (long long)b * 8 * nn + ((long long)i * d.nj + (j0 + ty)) * d.nk + (k0 + tx);
-            double vv = 0.0, bb2 = 0.0;   // |v|^2 and |B|^2 of the new state, summed in component order
-#pragma unroll
-            for (int v = 0; v < 8; v++) {
-                T r = T(0);
-                const T f0 = have_fr ? fr[v]
-                                     : (face_flux(win[v][0], win[v][1], win[v][2], win[v][3]) * (c10 + c1m)) * T(0.5);
-                const T f1 = (face_flux(win[v][1], win[v][2], win[v][3], win[v][4]) * (c1p + c10)) * T(0.5);
-                fr[v] = f1;
-                r = r + (f0 - f1) / dxi;
-                r = r + (fj[v][ty][tx] - fj[v][ty + 1][tx]) / L2;
-                r = r + (fk[v][ty][tx] - fk[v][ty][tx + 1]) / L3;
-                if (MODE == 0) {
-                    const long long q = base + (long long)v * d.P;
-                    const double x = (UEARLY ? ucur[v] : U[q]) + (0.5 * dt) * (double)r;
-                    out[q] = x;
-                    if (shadow) shadow[q] = (float)x;
-                    if (v == 1) vv = x * x;
-                    if (v == 2 || v == 3) vv = vv + x * x;
-                    if (v == 4) bb2 = x * x;
-                    if (v == 5 || v == 6) bb2 = bb2 + x * x;
-                } else {
-                    out[rbase + (long long)v * nn] = (double)r;
+            __syncthreads();
+            const T csI2 = (T)t.cs[rb + I - 2];
+            auto Lv = [&](int v, int dj, int dk) -> T { return lds[v][ty + 2 + dj][tx + 2 + dk]; };
+            // U^n of this plane, loaded before the face exchange (FP32 path, MODE 0)
+            constexpr bool UEARLY = MODE == 0 && sizeof(T) == 4;
+            double ucur[8];
+            if (m.act) {
+                const int J = m.J, K = m.K;
+                if (UEARLY) {
+                    const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + (long long)J * m.SJ + K;
+    #pragma unroll
+                    for (int v = 0; v < 8; v++) ucur[v] = U[base + (long long)v * d.P];
+                }
+                // speeds |v_d| + c_s at the cells the reference reads (see above)
+                const T c20 = fabs(at(2, I - 2, J, K - 2)) + csI2;
+                const T c2p = fabs(at(2, I - 2, J + 1, K - 2)) + csI2;
+                const T c30 = fabs(at(3, I - 2, J - 2, K)) + csI2;
+                const T c3p = fabs(at(3, I - 2, J - 2, K + 1)) + csI2;
+    #pragma unroll
+                for (int v = 0; v < 8; v++) {
+                    // upper theta face (the lower one of the cell at row + 1)
+                    fj[v][ty + 1][tx] = (face_flux(Lv(v, -1, 0), Lv(v, 0, 0), Lv(v, 1, 0), Lv(v, 2, 0)) * (c2p + c20)) * T(0.5);
+                    // upper phi face
+                    fk[v][ty][tx + 1] = (face_flux(Lv(v, 0, -1), Lv(v, 0, 0), Lv(v, 0, 1), Lv(v, 0, 2)) * (c3p + c30)) * T(0.5);
                 }
             }
-            if (MODE == 0) {
-                // ModWaveSpeed_Dynamo (3_6/ModWaveSpeed.f90:12-24) on the new state and
-                // h_LLL (3_6/ModBlock.f90:420-422), as k_dt_tile computes them
-                double w = t.cs[rb + I] + sqrt(vv);
-                w = w + sqrt(bb2) / t.sqrt_rho0[rb + I];
-                const double xi = t.xi[rb + I];
-                double h = mt.dxi, h2 = xi * mt.dxj, h3 = (xi * sint64) * mt.dxk;
-                h = h < h2 ? h : h2;
-                h = h < h3 ? h : h3;
-                const double x = h / w;
-                hmin = x < hmin ? x : hmin;
+            if (bact) {
+                // the lower face of the edge cell: its four-cell stencil in the LDS
+                // plane (start, step), its speed cells (J0, K0) on the upper side and
+                // (J1, K1) on the lower side
+                const bool bj = lane < TL;
+                const int bx = bj ? lane : 0, by = bj ? 0 : lane - TL;
+                const int bstart = bj ? bx + 2 : (by + 2) * PL, bstep = bj ? PL : 1;
+                const int bvar = bj ? 2 : 3;
+                const int bJ0 = bj ? j0 + d.ng : j0 + by + d.ng - 2, bK0 = bj ? k0 + bx + d.ng - 2 : k0 + d.ng;
+                const int bJ1 = bj ? bJ0 - 1 : bJ0, bK1 = bj ? bK0 : bK0 - 1;
+                const T s0 = fabs(at(bvar, I - 2, bJ0, bK0)) + csI2;
+                const T s1 = fabs(at(bvar, I - 2, bJ1, bK1)) + csI2;
+    #pragma unroll
+                for (int v = 0; v < 8; v++) {
+                    const T *pv = &lds[v][0][0] + bstart;
+                    const T f = (face_flux(pv[0], pv[bstep], pv[2 * bstep], pv[3 * bstep]) * (s0 + s1)) * T(0.5);
+                    if (bj) fj[v][0][bx] = f;
+                    else fk[v][by][0] = f;
+                }
+            }
+            __syncthreads();
+            if (m.act) {
+                const int J = m.J, K = m.K;
+                const T csI = (T)t.cs[rb + I], csIm = (T)t.cs[rb + I - 1], csIp = (T)t.cs[rb + I + 1];
+                const T c1m = fabs(at(1, I - 1, J - 2, K - 2)) + csIm;
+                const T c10 = fabs(at(1, I, J - 2, K - 2)) + csI;
+                const T c1p = fabs(at(1, I + 1, J - 2, K - 2)) + csIp;
+                const T xic = (T)t.xi[rb + I];
+                const T L2 = dxj * xic;
+                const T L3 = (dxk * xic) * sintc;
+                const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + (long long)J * m.SJ + K;
+                const long long nn = (long long)d.ni * d.nj * d.nk;
+                const long long rbase = (long long)b * 8 * nn + ((long long)i * d.nj + (j0 + ty)) * d.nk + (k0 + tx);
+                double vv = 0.0, bb2 = 0.0;   // |v|^2 and |B|^2 of the new state, summed in component order
+    #pragma unroll
+                for (int v = 0; v < 8; v++) {
+                    T r = T(0);
+                    const T f0 = have_fr ? fr[v]
+                                         : (face_flux(win[v][0], win[v][1], win[v][2], win[v][3]) * (c10 + c1m)) * T(0.5);
+                    const T f1 = (face_flux(win[v][1], win[v][2], win[v][3], win[v][4]) * (c1p + c10)) * T(0.5);
+                    fr[v] = f1;
+                    r = r + (f0 - f1) / dxi;
+                    r = r + (fj[v][ty][tx] - fj[v][ty + 1][tx]) / L2;
+                    r = r + (fk[v][ty][tx] - fk[v][ty][tx + 1]) / L3;
+                    if (MODE == 0) {
+                        const long long q = base + (long long)v * d.P;
+                        const double x = (UEARLY ? ucur[v] : U[q]) + (0.5 * dt) * (double)r;
+                        out[q] = x;
+                        if (shadow) shadow[q] = (float)x;
+                        if (v == 1) vv = x * x;
+                        if (v == 2 || v == 3) vv = vv + x * x;
+                        if (v == 4) bb2 = x * x;
+                        if (v == 5 || v == 6) bb2 = bb2 + x * x;
+                    } else {
+                        out[rbase + (long long)v * nn] = (double)r;
+                    }
+                }
+                if (MODE == 0) {
+                    // ModWaveSpeed_Dynamo (3_6/ModWaveSpeed.f90:12-24) on the new state and
+                    // h_LLL (3_6/ModBlock.f90:420-422), as k_dt_tile computes them
+                    double w = t.cs[rb + I] + sqrt(vv);
+                    w = w + sqrt(bb2) / t.sqrt_rho0[rb + I];
+                    const double xi = t.xi[rb + I];
+                    double h = mt.dxi, h2 = xi * mt.dxj, h3 = (xi * sint64) * mt.dxk;
+                    h = h < h2 ? h : h2;
+                    h = h < h3 ? h : h3;
+                    const double x = h / w;
+                    hmin = x < hmin ? x : hmin;
+                }
+            }
+            have_fr = true;
+    #pragma unroll
+            for (int v = 0; v < 8; v++) {
+                win[v][0] = win[v][1]; win[v][1] = win[v][2]; win[v][2] = win[v][3]; win[v][3] = win[v][4];
+                win[v][4] = nxt[v];
             }
         }
-        have_fr = true;
-#pragma unroll
-        for (int v = 0; v < 8; v++) {
-            win[v][0] = win[v][1]; win[v][1] = win[v][2]; win[v][2] = win[v][3]; win[v][3] = win[v][4];
-            win[v][4] = nxt[v];
+        if (MODE == 0 && partial) {
+            // min is exact in any order: wavefront shuffle, then across the wavefronts
+            for (int off = 32; off > 0; off >>= 1) {
+                double o = __shfl_down(hmin, off, 64);
+                hmin = o < hmin ? o : hmin;
+            }
+            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = hmin;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double r = red[0];
+                for (int q = 1; q < TL * TL / 64; q++) r = red[q] < r ? red[q] : r;
+                partial[tile] = r;
+            }
         }
-    }
-    if (MODE == 0 && partial) {
-        // min is exact in any order: wavefront shuffle, then across the wavefronts
-        for (int off = 32; off > 0; off >>= 1) {
-            double o = __shfl_down(hmin, off, 64);
-            hmin = o < hmin ? o : hmin;
-        }
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = hmin;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double r = red[0];
-            for (int q = 1; q < TL * TL / 64; q++) r = red[q] < r ? red[q] : r;
-            partial[blockIdx.x] = r;
-        }
+        if (!PERSIST) break;
+        // the next tile's LDS writes wait for this tile's last reads
+        if (tile + (int)gridDim.x < ntot) __syncthreads();
     }
 }
 
@@ -880,59 +899,120 @@ static void check_tile(int tile) {
     if (tile != 8 && tile != 16) throw std::runtime_error("tile edge must be 8 or 16");
 }
 
+// grid of a march: one workgroup per tile (wgs = 0) or wgs workgroups that
+// loop over the tiles (a persistent launch), never more than the tiles
+static inline dim3 march_grid(const Dims &d, int TL, int wgs) {
+    if (wgs < 0) throw std::runtime_error("workgroup count < 0");
+    const unsigned n = tiles(d, TL);
+    return dim3(wgs == 0 ? n : std::min<unsigned>(n, (unsigned)wgs));
+}
+
+template <typename K>
+static int resident(K kernel, int threads) {
+    int dev = 0, per_cu = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess)
+        throw std::runtime_error("occupancy query failed");
+    return std::max(1, per_cu) * prop.multiProcessorCount;
+}
+
+// workgroups of one march resident on the device at once (the persistent grid):
+// kernel 0 the stage (intermediate-stage form), 1 the diffusion; precision 0
+// FP64, 1 FP32
+int march_resident_wgs(int kernel, int precision, int tile) {
+    check_tile(tile);
+    if (kernel == 0 && precision == 0)
+        return tile == 16 ? resident(k_stage<double, double, double, double, 0, false, 16, true>, 256)
+                          : resident(k_stage<double, double, double, double, 0, false, 8, true>, 64);
+    if (kernel == 0)
+        return tile == 16 ? resident(k_stage<float, float, float, float, 0, false, 16, true>, 256)
+                          : resident(k_stage<float, float, float, float, 0, false, 8, true>, 64);
+    if (precision == 0)
+        return tile == 16 ? resident(k_diffusion<double, double, 0, 16, true>, 256)
+                          : resident(k_diffusion<double, double, 0, 8, true>, 64);
+    return tile == 16 ? resident(k_diffusion<float, float, 0, 16, true>, 256)
+                      : resident(k_diffusion<float, float, 0, 8, true>, 64);
+}
+
 template <int TL, typename T, typename TIN, typename TOUT, typename TB>
 static void stage_tl(const Dims &d, const DevTables &t, const TIN *in, const TB *U, TOUT *out, float *shadow,
-                     const double *dtp, double div, hipStream_t s, const int32_t *pull) {
-    if (shadow)
-        hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, true, TL>), dim3(tiles(d, TL)), dim3(TL * TL), 0, s, d, t, in, U,
-                           out, shadow, dtp, div, pull);
+                     const double *dtp, double div, hipStream_t s, const int32_t *pull, int wgs) {
+    const dim3 g = march_grid(d, TL, wgs), b(TL * TL);
+    if (shadow && wgs)
+        hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, true, TL, true>), g, b, 0, s, d, t, in, U, out, shadow, dtp, div,
+                           pull);
+    else if (shadow)
+        hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, true, TL, false>), g, b, 0, s, d, t, in, U, out, shadow, dtp, div,
+                           pull);
+    else if (wgs)
+        hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, false, TL, true>), g, b, 0, s, d, t, in, U, out,
+                           (float *)nullptr, dtp, div, pull);
     else
-        hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, false, TL>), dim3(tiles(d, TL)), dim3(TL * TL), 0, s, d, t, in, U,
-                           out, (float *)nullptr, dtp, div, pull);
+        hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, false, TL, false>), g, b, 0, s, d, t, in, U, out,
+                           (float *)nullptr, dtp, div, pull);
 }
 
 template <typename T, typename TIN, typename TOUT, typename TB>
 void launch_stage(const Dims &d, const DevTables &t, const TIN *in, const TB *U, TOUT *out, float *shadow,
-                  const double *dtp, double div, hipStream_t s, const int32_t *pull, int tile) {
+                  const double *dtp, double div, hipStream_t s, const int32_t *pull, int tile, int wgs) {
     check_tile(tile);
-    if (tile == 16) stage_tl<16, T>(d, t, in, U, out, shadow, dtp, div, s, pull);
-    else stage_tl<8, T>(d, t, in, U, out, shadow, dtp, div, s, pull);
+    if (tile == 16) stage_tl<16, T>(d, t, in, U, out, shadow, dtp, div, s, pull, wgs);
+    else stage_tl<8, T>(d, t, in, U, out, shadow, dtp, div, s, pull, wgs);
+}
+
+template <int TL, bool PERSIST, typename T, typename TIN>
+static void rhs_tl(const Dims &d, const DevTables &t, const TIN *in, double *R, hipStream_t s, int wgs) {
+    hipLaunchKernelGGL((k_stage<T, TIN, double, double, 1, false, TL, PERSIST>), march_grid(d, TL, wgs), dim3(TL * TL), 0,
+                       s, d, t, in, (const double *)nullptr, R, (float *)nullptr, (const double *)nullptr, 1.0,
+                       (const int32_t *)nullptr);
 }
 
 template <typename T, typename TIN>
-void launch_rhs_only(const Dims &d, const DevTables &t, const TIN *in, double *R, hipStream_t s, int tile) {
+void launch_rhs_only(const Dims &d, const DevTables &t, const TIN *in, double *R, hipStream_t s, int tile, int wgs) {
     check_tile(tile);
-    if (tile == 16)
-        hipLaunchKernelGGL((k_stage<T, TIN, double, double, 1, false, 16>), dim3(tiles(d, 16)), dim3(256), 0, s, d, t, in,
-                           (const double *)nullptr, R, (float *)nullptr, (const double *)nullptr, 1.0,
-                           (const int32_t *)nullptr);
-    else
-        hipLaunchKernelGGL((k_stage<T, TIN, double, double, 1, false, 8>), dim3(tiles(d, 8)), dim3(64), 0, s, d, t, in,
-                           (const double *)nullptr, R, (float *)nullptr, (const double *)nullptr, 1.0,
-                           (const int32_t *)nullptr);
+    if (tile == 16) {
+        if (wgs) rhs_tl<16, true, T>(d, t, in, R, s, wgs);
+        else rhs_tl<16, false, T>(d, t, in, R, s, wgs);
+    } else {
+        if (wgs) rhs_tl<8, true, T>(d, t, in, R, s, wgs);
+        else rhs_tl<8, false, T>(d, t, in, R, s, wgs);
+    }
+}
+
+template <int TL, bool PERSIST, int MODE, typename T, typename TIN>
+static void diffusion_tl(const Dims &d, const DevTables &t, const TIN *in, const double *U, double *out,
+                         float *shadow, double *partial, const double *dtp, hipStream_t s, const int32_t *pull,
+                         int wgs) {
+    hipLaunchKernelGGL((k_diffusion<T, TIN, MODE, TL, PERSIST>), march_grid(d, TL, wgs), dim3(TL * TL), 0, s, d, t, in,
+                       U, out, shadow, partial, dtp, pull);
+}
+
+template <int MODE, typename T, typename TIN>
+static void diffusion_any(const Dims &d, const DevTables &t, const TIN *in, const double *U, double *out,
+                          float *shadow, double *partial, const double *dtp, hipStream_t s, const int32_t *pull,
+                          int tile, int wgs) {
+    check_tile(tile);
+    if (tile == 16) {
+        if (wgs) diffusion_tl<16, true, MODE, T>(d, t, in, U, out, shadow, partial, dtp, s, pull, wgs);
+        else diffusion_tl<16, false, MODE, T>(d, t, in, U, out, shadow, partial, dtp, s, pull, wgs);
+    } else {
+        if (wgs) diffusion_tl<8, true, MODE, T>(d, t, in, U, out, shadow, partial, dtp, s, pull, wgs);
+        else diffusion_tl<8, false, MODE, T>(d, t, in, U, out, shadow, partial, dtp, s, pull, wgs);
+    }
 }
 
 template <typename T, typename TIN>
 void launch_diffusion(const Dims &d, const DevTables &t, const TIN *in, const double *U, double *out, float *shadow,
-                      double *partial, const double *dtp, hipStream_t s, const int32_t *pull, int tile) {
-    check_tile(tile);
-    if (tile == 16)
-        hipLaunchKernelGGL((k_diffusion<T, TIN, 0, 16>), dim3(tiles(d, 16)), dim3(256), 0, s, d, t, in, U, out, shadow,
-                           partial, dtp, pull);
-    else
-        hipLaunchKernelGGL((k_diffusion<T, TIN, 0, 8>), dim3(tiles(d, 8)), dim3(64), 0, s, d, t, in, U, out, shadow,
-                           partial, dtp, pull);
+                      double *partial, const double *dtp, hipStream_t s, const int32_t *pull, int tile, int wgs) {
+    diffusion_any<0, T>(d, t, in, U, out, shadow, partial, dtp, s, pull, tile, wgs);
 }
 
 template <typename T>
-void launch_diffusion_only(const Dims &d, const DevTables &t, const double *U, double *R, hipStream_t s, int tile) {
-    check_tile(tile);
-    if (tile == 16)
-        hipLaunchKernelGGL((k_diffusion<T, double, 1, 16>), dim3(tiles(d, 16)), dim3(256), 0, s, d, t, U, U, R,
-                           (float *)nullptr, (double *)nullptr, (const double *)nullptr, (const int32_t *)nullptr);
-    else
-        hipLaunchKernelGGL((k_diffusion<T, double, 1, 8>), dim3(tiles(d, 8)), dim3(64), 0, s, d, t, U, U, R,
-                           (float *)nullptr, (double *)nullptr, (const double *)nullptr, (const int32_t *)nullptr);
+void launch_diffusion_only(const Dims &d, const DevTables &t, const double *U, double *R, hipStream_t s, int tile,
+                           int wgs) {
+    diffusion_any<1, T>(d, t, U, U, R, (float *)nullptr, (double *)nullptr, (const double *)nullptr, s,
+                        (const int32_t *)nullptr, tile, wgs);
 }
 
 template <typename TS>
@@ -990,15 +1070,15 @@ void launch_to_f32(const double *src, float *dst, long long n, hipStream_t s) {
 }
 
 // explicit instantiations: FP64 path, FP32-deviation path
-template void launch_stage<double, double, double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, const double *, double, hipStream_t, const int32_t *, int);
-template void launch_stage<float, float, float, float>(const Dims &, const DevTables &, const float *, const float *, float *, float *, const double *, double, hipStream_t, const int32_t *, int);
-template void launch_stage<float, float, double, double>(const Dims &, const DevTables &, const float *, const double *, double *, float *, const double *, double, hipStream_t, const int32_t *, int);
-template void launch_rhs_only<double, double>(const Dims &, const DevTables &, const double *, double *, hipStream_t, int);
-template void launch_rhs_only<float, double>(const Dims &, const DevTables &, const double *, double *, hipStream_t, int);
-template void launch_diffusion<double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, double *, const double *, hipStream_t, const int32_t *, int);
-template void launch_diffusion<float, float>(const Dims &, const DevTables &, const float *, const double *, double *, float *, double *, const double *, hipStream_t, const int32_t *, int);
-template void launch_diffusion_only<double>(const Dims &, const DevTables &, const double *, double *, hipStream_t, int);
-template void launch_diffusion_only<float>(const Dims &, const DevTables &, const double *, double *, hipStream_t, int);
+template void launch_stage<double, double, double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, const double *, double, hipStream_t, const int32_t *, int, int);
+template void launch_stage<float, float, float, float>(const Dims &, const DevTables &, const float *, const float *, float *, float *, const double *, double, hipStream_t, const int32_t *, int, int);
+template void launch_stage<float, float, double, double>(const Dims &, const DevTables &, const float *, const double *, double *, float *, const double *, double, hipStream_t, const int32_t *, int, int);
+template void launch_rhs_only<double, double>(const Dims &, const DevTables &, const double *, double *, hipStream_t, int, int);
+template void launch_rhs_only<float, double>(const Dims &, const DevTables &, const double *, double *, hipStream_t, int, int);
+template void launch_diffusion<double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, double *, const double *, hipStream_t, const int32_t *, int, int);
+template void launch_diffusion<float, float>(const Dims &, const DevTables &, const float *, const double *, double *, float *, double *, const double *, hipStream_t, const int32_t *, int, int);
+template void launch_diffusion_only<double>(const Dims &, const DevTables &, const double *, double *, hipStream_t, int, int);
+template void launch_diffusion_only<float>(const Dims &, const DevTables &, const double *, double *, hipStream_t, int, int);
 template void launch_exchange<double>(const Dims &, const DevGhosts &, double *, hipStream_t);
 template void launch_exchange<float>(const Dims &, const DevGhosts &, float *, hipStream_t);
 template void launch_exchange_rest<float>(const Dims &, const DevGhosts &, float *, hipStream_t);

@@ -100,20 +100,27 @@ int falcon_set_precision(falcon_handle_t h, int mode);
  * ghost cells.  Both read the same values.  No counterpart in the reference
  * (its ghost fill is ModCommunication_SendRecvGC_new). */
 int falcon_set_ghost_pull(falcon_handle_t h, int on);
-/* Tile edges of the stage and diffusion marches for one precision (0 FP64,
- * 1 FP32 deviation): 16 (a 16x16 (theta, phi) tile of 256 lanes per
- * workgroup) or 8 (8x8, 64 lanes); both 16 when the handle is made.  Every
- * result is the same for either edge.  No counterpart in the reference (its
- * loops run over whole blocks). */
-int falcon_set_tiles(falcon_handle_t h, int precision, int stage_tile, int diffusion_tile);
-int falcon_get_tiles(falcon_handle_t h, int precision, int *stage_tile, int *diffusion_tile);
+/* Launch shapes of the stage and diffusion marches for one precision (0 FP64,
+ * 1 FP32 deviation).  tile: the (theta, phi) tile edge, 16 (16x16 lanes, 256
+ * per workgroup) or 8 (8x8, 64 lanes).  wgs: 0 launches one workgroup per
+ * tile; n > 0 launches n workgroups that loop over the tiles (a persistent
+ * launch).  16 and 0 for both when the handle is made.  Every result is the
+ * same in any shape.  shape[4] of get: stage tile, stage wgs, diffusion tile,
+ * diffusion wgs.  No counterpart in the reference (its loops run over whole
+ * blocks). */
+int falcon_set_tiles(falcon_handle_t h, int precision, int stage_tile, int stage_wgs, int diffusion_tile,
+                     int diffusion_wgs);
+int falcon_get_tiles(falcon_handle_t h, int precision, int *shape);
 /* Autotuner for the current precision: times the stage and diffusion kernels
- * with each tile edge (reps launches each, 1..1000) on the current state and
- * keeps the faster edge of each.  tiles_out[2] (optional): the edges chosen
- * for stage, diffusion; ms_out[4] (optional): ms per launch of stage 16,
- * stage 8, diffusion 16, diffusion 8.  The state and later results are
- * unchanged.  No counterpart in the reference. */
-int falcon_autotune(falcon_handle_t h, int reps, int *tiles_out, double *ms_out);
+ * (reps launches each, 1..1000) on the current state in four shapes, tile 16
+ * or 8, one workgroup per tile or the persistent grid (the workgroups the
+ * device holds resident at once), and keeps the fastest of each.
+ * shape_out[4] (optional): the shape chosen, as in falcon_get_tiles;
+ * ms_out[8] (optional): ms per launch of the stage in the shapes (16, per
+ * tile), (16, persistent), (8, per tile), (8, persistent), then of the
+ * diffusion in the same order.  The state and later results are unchanged.
+ * No counterpart in the reference. */
+int falcon_autotune(falcon_handle_t h, int reps, int *shape_out, double *ms_out);
 /* ModCommunication_SendRecvGC_new(Tree, if_rk) (3_6/ModCommunication.f90:655-795):
  * which = 0 fills the state's ghosts, 1 the current stage buffer's. */
 int falcon_sendrecv_gc(falcon_handle_t h, int which);

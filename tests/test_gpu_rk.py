@@ -47,15 +47,15 @@ def test_exchange_bit_exact(name):
     assert np.array_equal(_states(s, o.nblocks), _states(o, o.nblocks))
 
 
-@pytest.mark.parametrize("tile", [16, 8])
+@pytest.mark.parametrize("tile,wgs", [(16, 0), (8, 0), (16, 5)])
 @pytest.mark.parametrize("name", list(LAYOUTS))
-def test_rhs_and_diffusion_bit_exact(name, tile):
+def test_rhs_and_diffusion_bit_exact(name, tile, wgs):
     """RHS and diffusion of the FP64 path against the oracle with either
     (theta, phi) tile edge of the march (partial tiles where a block's face is
-    not a multiple of the edge)."""
-    print(f"[rhs] layout={name} tile={tile}", flush=True)
+    not a multiple of the edge), and with 5 workgroups looping over every tile."""
+    print(f"[rhs] layout={name} tile={tile} wgs={wgs}", flush=True)
     s, o = _pair(LAYOUTS[name], seed=2)
-    s.set_tiles(0, tile, tile)
+    s.set_tiles(0, tile, tile, wgs, wgs)
     s.sendrecv_gc(0); o.exchange(0)
     r_gpu = s.rhs()
     r_orc = np.stack([o.rhs(ib) for ib in range(o.nblocks)])
@@ -65,12 +65,12 @@ def test_rhs_and_diffusion_bit_exact(name, tile):
     assert np.array_equal(d_gpu, d_orc), rel_err(d_gpu, d_orc)
 
 
-@pytest.mark.parametrize("tiles", [(16, 16), (8, 8), (16, 8)])
+@pytest.mark.parametrize("shape", [(16, 16, 0, 0), (8, 8, 0, 0), (16, 8, 7, 3)])
 @pytest.mark.parametrize("name", ["mixed", "cubic8"])
-def test_steps_bit_exact(name, tiles):
-    print(f"[steps] layout={name} tiles={tiles}", flush=True)
+def test_steps_bit_exact(name, shape):
+    print(f"[steps] layout={name} shape={shape}", flush=True)
     s, o = _pair(LAYOUTS[name], seed=3)
-    s.set_tiles(0, *tiles)
+    s.set_tiles(0, *shape)
     s.sendrecv_gc(0); o.exchange(0)
     assert s.compute_dt(0.5) == o.dt(0.5)
     for step in range(3):
@@ -141,12 +141,14 @@ def test_fp32_ghost_pull_equals_full_exchange(name):
 
 @pytest.mark.parametrize("name", ["mixed", "amr16"])
 def test_fp32_tiles_bit_identical(name):
-    """The FP32 step with 8-edge stage and/or diffusion tiles (eager and graph
-    replay) gives the same bits as with 16-edge tiles."""
+    """The FP32 step with 8-edge stage and/or diffusion tiles, and with a few
+    workgroups looping over the tiles (eager and graph replay), gives the same
+    bits as with 16-edge tiles and one workgroup per tile."""
     print(f"[fp32 tiles] layout={name}", flush=True)
     lay = PULL_LAYOUTS[name]
     runs = []
-    for tiles, graph in (((16, 16), False), ((8, 8), False), ((8, 16), True), ((16, 8), True)):
+    for tiles, graph in (((16, 16, 0, 0), False), ((8, 8, 0, 0), False), ((8, 16, 0, 0), True),
+                         ((16, 8, 0, 0), True), ((16, 16, 3, 11), True), ((8, 8, 13, 2), False)):
         s = Shell(lay)
         s.set_gc_all()
         s.background_polytrope(20.0)
@@ -176,9 +178,10 @@ def test_autotune_leaves_results_unchanged(prec):
         if tune:
             r = s.autotune(reps=3)
             print(r, flush=True)
-            assert r["stage"] in (8, 16) and r["diffusion"] in (8, 16)
+            assert r["stage"][0] in (8, 16) and r["diffusion"][0] in (8, 16)
+            assert r["stage"][1] >= 0 and r["diffusion"][1] >= 0
             assert all(v > 0 for k in r["ms"].values() for v in k.values())
-            assert s.get_tiles(1 if prec == "fp32" else 0) == (r["stage"], r["diffusion"])
+            assert s.get_tiles(1 if prec == "fp32" else 0) == tuple(r["stage"] + r["diffusion"])
         s.advance(4, 0.5, graph=True)
         runs.append(_states(s, s.nblocks))
     assert np.array_equal(runs[0], runs[1])
