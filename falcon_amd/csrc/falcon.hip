@@ -986,18 +986,18 @@ int falcon_get_tiles(falcon_handle_t h, int precision, int *shape) {
 }
 
 // Autotuner of the march launch shapes.  For the current precision, times the
-// stage kernel (intermediate-stage form) and the diffusion kernel in four
+// stage kernel (a step's four stage launches) and the diffusion kernel in four
 // shapes, tile edge 16 or 8 with one workgroup per tile or with the persistent
 // grid (the workgroups the device holds resident at once), reps launches after
-// one untimed launch, on the current state, and keeps the fastest shape of
-// each.  The timed launches write only the stage buffer S[0], the next-state
-// buffers and the partial minima, none of which holds anything between steps
+// one untimed launch, twice over the shapes, on the current state, and keeps
+// the fastest shape of each.  The timed launches write only the stage buffer
+// S[0], the next-state buffers and the partial minima, none of which holds anything between steps
 // (the partial minima are recomputed before the next step), so the state and
 // every later result are unchanged.
 // shape_out (optional): stage edge, stage workgroups, diffusion edge,
-// diffusion workgroups.  ms_out (optional): ms per launch of the stage in the
-// shapes (16, per tile), (16, persistent), (8, per tile), (8, persistent),
-// then the diffusion in the same order.
+// diffusion workgroups.  ms_out (optional): ms per stage launch (a step's four
+// averaged) in the shapes (16, per tile), (16, persistent), (8, per tile),
+// (8, persistent), then ms per diffusion launch in the same order.
 int falcon_autotune(falcon_handle_t h, int reps, int *shape_out, double *ms_out) {
     return guard([&] {
         if (reps < 1 || reps > 1000) throw Error("reps must be in 1..1000");
@@ -1041,24 +1041,41 @@ int falcon_autotune(falcon_handle_t h, int reps, int *shape_out, double *ms_out)
             wst[c] = (c & 1) ? march_resident_wgs(0, p, edge[c]) : 0;
             wdi[c] = (c & 1) ? march_resident_wgs(1, p, edge[c]) : 0;
         }
-        for (int c = 0; c < 4; c++) {
-            const int tl = edge[c], ws = wst[c], wd = wdi[c];
-            if (p == 0) {
-                double *S0 = h->S64[0].as<double>();
-                ms[c] = time([&] {
-                    launch_stage<double, double, double, double>(d, t, U, U, S0, nullptr, dtp, 4.0, s, nullptr, tl, ws);
-                });
-                ms[4 + c] = time([&] {
-                    launch_diffusion<double, double>(d, t, U, U, Un, nullptr, part, dtp, s, nullptr, tl, wd);
-                });
-            } else {
-                float *F = h->F32[cur].as<float>(), *Fn = h->F32[1 - cur].as<float>(), *S0 = h->S32[0].as<float>();
-                ms[c] = time([&] {
-                    launch_stage<float, float, float, float>(d, t, F, F, S0, nullptr, dtp, 4.0, s, pull, tl, ws);
-                });
-                ms[4 + c] = time([&] {
-                    launch_diffusion<float, float>(d, t, F, U, Un, Fn, part, dtp, s, pull, tl, wd);
-                });
+        // the stage is timed as a step runs it: three intermediate stages and the
+        // final one (for the FP32 path its FP64 form, writing the next-state
+        // buffers); ms is per stage launch.  Two sweeps over the shapes, the
+        // faster time of each kept, so the order of the shapes does not bias them.
+        for (int q = 0; q < 8; q++) ms[q] = 1e30;
+        for (int sweep = 0; sweep < 2; sweep++) {
+            for (int c = 0; c < 4; c++) {
+                const int tl = edge[c], ws = wst[c], wd = wdi[c];
+                double a, b;
+                if (p == 0) {
+                    double *S0 = h->S64[0].as<double>();
+                    a = time([&] {
+                        for (int k = 1; k <= 3; k++)
+                            launch_stage<double, double, double, double>(d, t, U, U, S0, nullptr, dtp, 5.0 - k, s,
+                                                                         nullptr, tl, ws);
+                        launch_stage<double, double, double, double>(d, t, U, U, Un, nullptr, dtp, 1.0, s, nullptr, tl,
+                                                                     ws);
+                    }) / 4;
+                    b = time([&] {
+                        launch_diffusion<double, double>(d, t, U, U, Un, nullptr, part, dtp, s, nullptr, tl, wd);
+                    });
+                } else {
+                    float *F = h->F32[cur].as<float>(), *Fn = h->F32[1 - cur].as<float>(), *S0 = h->S32[0].as<float>();
+                    a = time([&] {
+                        for (int k = 1; k <= 3; k++)
+                            launch_stage<float, float, float, float>(d, t, F, F, S0, nullptr, dtp, 5.0 - k, s, pull, tl,
+                                                                     ws);
+                        launch_stage<float, float, double, double>(d, t, F, U, Un, Fn, dtp, 1.0, s, pull, tl, ws);
+                    }) / 4;
+                    b = time([&] {
+                        launch_diffusion<float, float>(d, t, F, U, Un, Fn, part, dtp, s, pull, tl, wd);
+                    });
+                }
+                ms[c] = std::min(ms[c], a);
+                ms[4 + c] = std::min(ms[4 + c], b);
             }
         }
         check_launch();

@@ -111,14 +111,15 @@ int falcon_set_ghost_pull(falcon_handle_t h, int on);
 int falcon_set_tiles(falcon_handle_t h, int precision, int stage_tile, int stage_wgs, int diffusion_tile,
                      int diffusion_wgs);
 int falcon_get_tiles(falcon_handle_t h, int precision, int *shape);
-/* Autotuner for the current precision: times the stage and diffusion kernels
- * (reps launches each, 1..1000) on the current state in four shapes, tile 16
- * or 8, one workgroup per tile or the persistent grid (the workgroups the
- * device holds resident at once), and keeps the fastest of each.
+/* Autotuner for the current precision: times the stage kernel (a step's four
+ * stage launches) and the diffusion kernel, reps times each (1..1000), twice
+ * over four shapes on the current state: tile 16 or 8, one workgroup per tile
+ * or the persistent grid (the workgroups the device holds resident at once).
+ * Keeps the fastest of each.
  * shape_out[4] (optional): the shape chosen, as in falcon_get_tiles;
- * ms_out[8] (optional): ms per launch of the stage in the shapes (16, per
- * tile), (16, persistent), (8, per tile), (8, persistent), then of the
- * diffusion in the same order.  The state and later results are unchanged.
+ * ms_out[8] (optional): ms per stage launch (a step's four averaged) in the
+ * shapes (16, per tile), (16, persistent), (8, per tile), (8, persistent),
+ * then ms per diffusion launch in the same order.  The state and later results are unchanged.
  * No counterpart in the reference. */
 int falcon_autotune(falcon_handle_t h, int reps, int *shape_out, double *ms_out);
 /* ModCommunication_SendRecvGC_new(Tree, if_rk) (3_6/ModCommunication.f90:655-795):
