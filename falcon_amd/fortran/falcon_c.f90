@@ -63,6 +63,15 @@ module falcon_c
             integer(c_int), value           :: mode
         end function falcon_set_precision
 
+        ! launch-shape autotuner of the current precision; shape_out (4 ints) and
+        ! ms_out (8 doubles) may be c_null_ptr
+        integer(c_int) function falcon_autotune(h, reps, shape_out, ms_out) bind(c, name='falcon_autotune')
+            import :: c_int, c_ptr
+            type(c_ptr), value              :: h
+            integer(c_int), value           :: reps
+            type(c_ptr), value              :: shape_out, ms_out
+        end function falcon_autotune
+
         integer(c_int) function falcon_sendrecv_gc(h, which) bind(c, name='falcon_sendrecv_gc')
             import :: c_int, c_ptr
             type(c_ptr), value              :: h
