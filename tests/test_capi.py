@@ -45,3 +45,20 @@ def test_calls_out_of_order_raise():
         s.gc_iblocks(s.nblocks)
     with pytest.raises(FalconError, match="set_gc_all"):
         s.get_state(0)
+
+
+def test_launch_shapes_are_validated_and_kept_per_precision():
+    """Host side of the launch-shape setter: defaults, per-precision storage,
+    and the errors for an edge other than 8 or 16 or a negative workgroup count
+    (no device call)."""
+    s = Shell(configs.small_mixed())
+    assert s.get_tiles(0) == (16, 0, 16, 0) and s.get_tiles(1) == (16, 0, 16, 0)
+    s.set_tiles(1, 8, 16, 700, 0)
+    assert s.get_tiles(1) == (8, 700, 16, 0) and s.get_tiles(0) == (16, 0, 16, 0)
+    with pytest.raises(FalconError, match="tile edges must be 8 or 16"):
+        s.set_tiles(0, 12, 16)
+    with pytest.raises(FalconError, match="workgroup counts must be >= 0"):
+        s.set_tiles(0, 16, 16, -1, 0)
+    with pytest.raises(FalconError, match="precision must be"):
+        s.set_tiles(2, 16, 16)
+    assert s.get_tiles(0) == (16, 0, 16, 0)
