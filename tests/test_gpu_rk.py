@@ -81,8 +81,11 @@ def test_steps_bit_exact(name, shape):
     assert np.array_equal(a, b), rel_err(a, b)
 
 
-def test_graph_replay_matches_eager():
-    print("[graph] layout=cubic8", flush=True)
+@pytest.mark.parametrize("shape", [(16, 16, 0, 0), (8, 16, 5, 9)])
+def test_graph_replay_matches_eager(shape):
+    """FP64 steps replayed from the captured graph equal eager steps, also with
+    8-edge stage tiles and looping (persistent) grids of 5 and 9 workgroups."""
+    print(f"[graph] layout=cubic8 shape={shape}", flush=True)
     lay = LAYOUTS["cubic8"]
     runs = []
     for graph in (False, True):
@@ -90,6 +93,7 @@ def test_graph_replay_matches_eager():
         s.set_gc_all()
         s.background_polytrope(20.0)
         s.init_synthetic(seed=7)
+        s.set_tiles(0, *shape)
         s.advance(6, 0.5, graph=graph)
         runs.append(_states(s, s.nblocks))
     assert np.array_equal(runs[0], runs[1])
