@@ -6,7 +6,7 @@
 module ModAdvance
     use iso_c_binding
     use falcon_c,       only: falcon_advance_rk4
-    use ModFalconShim,  only: falcon_h, falcon_check
+    use ModFalconShim,  only: falcon_h, falcon_check, falcon_push_pending
     use ModParameters,  only: CFL
     use ModYinYangTree, only: YYTree
     implicit none
@@ -16,6 +16,7 @@ contains
         logical, intent(in)                 :: if_all_same_sizes    ! every block is 16^3-alike on the GPU
         real(8)                             :: dt_global
         real(c_double)                      :: dt
+        call falcon_push_pending(Tree)
         call falcon_check(falcon_advance_rk4(falcon_h, real(CFL, c_double), dt))
         dt_global = dt
     end subroutine ModAdvance_rk4

@@ -5,12 +5,12 @@
 module ModCommunication
     use iso_c_binding
     use falcon_c,       only: falcon_sendrecv_gc
-    use ModFalconShim,  only: falcon_h, falcon_check, falcon_attach
+    use ModFalconShim,  only: falcon_h, falcon_check, falcon_attach, falcon_push_pending
     use ModYinYangTree, only: YYTree
     implicit none
 contains
     ! 3_6/ModCommunication.f90:22-64: donors, targets, sources; here also the
-    ! upload of tree, background and state
+    ! upload of tree and background (the state follows at the first fill)
     subroutine ModCommunication_SetGCAll(Tree)
         type(YYTree), intent(inout), target :: Tree
         call falcon_attach(Tree)
@@ -20,6 +20,7 @@ contains
     subroutine ModCommunication_SendRecvGC_new(Tree, if_rk)
         type(YYTree), intent(inout), target :: Tree
         logical, intent(in)                 :: if_rk
+        call falcon_push_pending(Tree)
         call falcon_check(falcon_sendrecv_gc(falcon_h, merge(1_c_int, 0_c_int, if_rk)))
     end subroutine ModCommunication_SendRecvGC_new
 end module ModCommunication

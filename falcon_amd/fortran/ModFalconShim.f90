@@ -2,6 +2,11 @@
 ! (3_6/ModYinYangTree.f90:55-81) and BlockType data; libfalcon_hip.so keeps every
 ! block resident in HBM.  Host primitive arrays are stale between
 ! falcon_sync_to_host calls, which the driver makes before ModCheck/ModSave.
+!
+! The reference driver writes the initial state on the host after
+! ModCommunication_SetGCAll (3_6/test1.f90:133,140), so attaching at SetGCAll
+! only marks the host state as not yet on the device; the first ghost fill or
+! step after it copies the state up (falcon_push_pending).
 module ModFalconShim
     use iso_c_binding
     use falcon_c
@@ -10,9 +15,11 @@ module ModFalconShim
     use ModBlock,       only: BlockType
     implicit none
     private
-    public :: falcon_h, falcon_check, falcon_attach, falcon_push_state, falcon_sync_to_host
+    public :: falcon_h, falcon_check, falcon_attach, falcon_push_state, falcon_push_pending, falcon_sync_to_host
 
     type(c_ptr), save   :: falcon_h = c_null_ptr
+    ! the host state is newer than the device state (set by falcon_attach)
+    logical, save       :: state_pending = .false.
 
 contains
 
@@ -106,10 +113,20 @@ contains
             call falcon_check(falcon_set_background(falcon_h, int(B%iBlock - 1, c_int), bg))
         end do
         call falcon_check(falcon_set_gc_all(falcon_h))
-        call falcon_push_state(Tree)
+        state_pending = .true.
     end subroutine falcon_attach
 
-    ! host -> HBM: rho1, vr, vt, vp, br, bt, bp, s1 of every block (psi stays on the host)
+    ! Copy the host state up if it changed since falcon_attach (the driver's
+    ! ModInitiation_DoAll, 3_6/test1.f90:140); called by the drop-in entry points.
+    subroutine falcon_push_pending(Tree)
+        type(YYTree), intent(inout), target :: Tree
+        if (.not. state_pending) return
+        call falcon_push_state(Tree)
+        state_pending = .false.
+    end subroutine falcon_push_pending
+
+    ! host -> HBM: rho1, vr, vt, vp, br, bt, bp, s1 of every block (psi stays on
+    ! the host); a driver that changes primitive_IV on the host later calls it
     subroutine falcon_push_state(Tree)
         type(YYTree), intent(inout), target :: Tree
         integer                         :: ib

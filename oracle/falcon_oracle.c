@@ -62,6 +62,11 @@ struct orc_ctx {
     onode yin, yang;
     int nleaf, nleaf_yin;
     oblock *b;
+    /* pin of the step composition (orc_set_linear_rhs): when lin_on, orc_step
+     * takes R = lin_lambda * (stage input) for the stages and R = lin_mu * U
+     * for the diffusion, in place of ModEquation_Dynamo / ModDiffusion */
+    int lin_on;
+    double lin_lambda, lin_mu;
 };
 
 #define PAD(c, i, j, k) ((size_t)((i) + (c)->ng - 1) + (size_t)(c)->NI * ((size_t)((j) + (c)->ng - 1) + (size_t)(c)->NJ * (size_t)((k) + (c)->ng - 1)))
@@ -903,8 +908,14 @@ double orc_step(orc_ctx *c, double cfl) {
         for (int ib = 0; ib < c->nleaf; ib++) {
             oblock *b = &c->b[ib];
             double *in = rk > 1 ? b->urk : b->u;
-            boundary(c, b, in);
-            equation(c, b, in, b->r);
+            if (c->lin_on) {
+                for (int v = 0; v < 8; v++)
+                    for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) for (int i = 1; i <= ni; i++)
+                        b->r[INT(c, i, j, k) + nn * v] = c->lin_lambda * in[PAD(c, i, j, k) + np * v];
+            } else {
+                boundary(c, b, in);
+                equation(c, b, in, b->r);
+            }
             double div = (double)(5.0f - (float)rk);
             double *out = rk < 4 ? b->urk : b->u;
             for (int v = 0; v < 8; v++)
@@ -919,7 +930,13 @@ double orc_step(orc_ctx *c, double cfl) {
     for (int ib = 0; ib < c->nleaf; ib++) {
         oblock *b = &c->b[ib];
         memset(b->r, 0, sizeof(double) * nn * 8);
-        diffusion(c, b, b->u, 2, b->r);
+        if (c->lin_on) {
+            for (int v = 0; v < 8; v++)
+                for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) for (int i = 1; i <= ni; i++)
+                    b->r[INT(c, i, j, k) + nn * v] = c->lin_mu * b->u[PAD(c, i, j, k) + np * v];
+        } else {
+            diffusion(c, b, b->u, 2, b->r);
+        }
         for (int v = 0; v < 8; v++)
             for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) for (int i = 1; i <= ni; i++) {
                 size_t q = PAD(c, i, j, k) + np * v;
@@ -928,6 +945,12 @@ double orc_step(orc_ctx *c, double cfl) {
     }
     orc_exchange(c, 0);
     return dt;
+}
+
+void orc_set_linear_rhs(orc_ctx *c, int on, double lambda, double mu) {
+    c->lin_on = on;
+    c->lin_lambda = lambda;
+    c->lin_mu = mu;
 }
 
 /* Yin-Yang point map and the VecConv coefficients (sin p sin p', cos p / sin t')

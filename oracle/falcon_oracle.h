@@ -57,6 +57,12 @@ void orc_rhs(orc_ctx *c, int ib, double *r_out);    /* BC + ModEquation_Dynamo o
 void orc_diffusion(orc_ctx *c, int ib, double *r_out); /* ModDiffusion_Aritificial_1(h=2) on U */
 double orc_dt(orc_ctx *c, double cfl);               /* ModWaveSpeed + ModTimeStep */
 double orc_step(orc_ctx *c, double cfl);             /* one ModAdvance_rk4; returns dt */
+/* Pin of orc_step's composition: on != 0 replaces the stage right-hand side by
+ * lambda * (stage input) and the diffusion's by mu * U (interior, every
+ * variable), leaving the stage buffers, coefficients, exchanges and dt as they
+ * are; one step then multiplies each interior value by
+ * (1 + z + z^2/2 + z^3/6 + z^4/24) (1 + mu dt/2), z = lambda dt. */
+void orc_set_linear_rhs(orc_ctx *c, int on, double lambda, double mu);
 void orc_set_threads(int n);
 void orc_coordconv(const double in[3], double out[3]);
 void orc_vecconv_coeffs(const double x[3], double out[2]);

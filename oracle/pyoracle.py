@@ -55,6 +55,7 @@ def lib():
         L.orc_dt.restype = C.c_double
         L.orc_step.argtypes = [vp, C.c_double]
         L.orc_step.restype = C.c_double
+        L.orc_set_linear_rhs.argtypes = [vp, C.c_int, C.c_double, C.c_double]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_heat_create.restype = vp
         L.orc_heat_create.argtypes = [vp, C.c_double, C.c_double]
@@ -187,6 +188,11 @@ class Oracle:
 
     def step(self, cfl):
         return self.L.orc_step(self.h, cfl)
+
+    def set_linear_rhs(self, on, lam=0.0, mu=0.0):
+        """Pin hook: the step's stage right-hand side becomes lam * (stage input)
+        and the diffusion's mu * U (see orc_set_linear_rhs)."""
+        self.L.orc_set_linear_rhs(self.h, 1 if on else 0, float(lam), float(mu))
 
 
 class HeatOracle:

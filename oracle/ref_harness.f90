@@ -1,64 +1,60 @@
 ! Test-infrastructure driver for the reference Fortran (never part of the product).
 !
-! Builds the Yin-Yang tree, ghost tables and blocks with the reference's own
-! modules (compiled in place from 3_6/ by oracle/Makefile), loads a background
-! and initial state written by tests/golden/make_golden.py, and advances the
-! state with the reference's physics routines.  Two reference files need MPI and
-! are not compiled; their single-rank control flow is restated here:
-!   * harness_exchange  follows ModCommunication_SendRecvGC_new
-!                       (3_6/ModCommunication.f90:655-795, local-rank branch)
-!                       and the local branch of SetGC_sources (:219-300);
-!   * harness_rk4       follows ModAdvance_rk4 (3_6/ModAdvance.f90:18-112) with
-!                       MPI_AllReduce(MIN) over one rank = identity.
-! Background replication follows ModBlock_InitSSM (3_6/ModBlock.f90:251-264);
+! Builds the Yin-Yang tree and blocks with the reference's own modules (compiled
+! in place from 3_6/ by oracle/Makefile), loads a background and initial state
+! written by tests/golden/make_golden.py, and runs the reference's own driver
+! sequence (3_6/test1.f90:110-148 and :56-58):
+!   ModCommunication_SetGCAll, state, two ModCommunication_SendRecvGC_new(.false.),
+!   then ModAdvance_rk4 per step,
+! with the reference's ModCommunication.f90 and ModAdvance.f90 themselves, over
+! MPI.  Run under `mpirun -n N` the blocks split over ranks exactly as
+! ModAllocation_Init assigns them; the output file is the same for every N.
+! The background replication follows ModBlock_InitSSM (3_6/ModBlock.f90:251-264);
 ! the 1-D profiles come from the input file because the FreeEOS table that
 ! ModStratification_new reads is not in the reference tree.
 !
-! Built with -DFALCON_DROPIN (oracle/Makefile target `dropin`), the same
-! driver calls the Fortran shims of falcon_amd/fortran (ModCommunication_SetGCAll,
-! ModCommunication_SendRecvGC_new, ModAdvance_rk4) instead, stepping on the GPU.
+! Built with -DFALCON_DROPIN (oracle/Makefile target `dropin`), the same driver
+! links the Fortran shims of falcon_amd/fortran in place of the reference's
+! ModCommunication / ModAdvance, stepping on the GPU (one rank).
 !
-! Usage: falcon_ref_harness <input.bin> <output.bin>
+! Usage: [mpirun -n N] falcon_ref_harness <input.bin> <output.bin>
 program falcon_ref_harness
+    use MPI
     use ModParameters,  only:   ni,nj,nk,ng,nvar,iGeometry,iEquation,MpiSize,MpiRank,&
                                 if_involve_B,Artificial_heating_ratio,CFL,rLevelInitial,&
                                 nAMRs,AMRs,DivB_option
-    use ModBlock,       only:   BlockType,GC_target,ModBlock_InitGrid,ModBlock_InitPrimitives
+    use ModControl,     only:   nMaxBlocksPerRank
+    use ModBlock,       only:   BlockType,ModBlock_InitGrid,ModBlock_InitPrimitives
     use ModYinYangTree, only:   YYTree,YinYangTree_InitTree,YinYangTree_LabelNodes,&
                                 YinYangTree_GetLocalBlocksInfo,YinYangTree_if_top_bottom
     use ModAllocation,  only:   ModAllocation_Init,nLocalLeafNodes,iLocalLeafNodes
     use ModAMR,         only:   ModAMR_set_grid
-    use ModGC,          only:   ModGC_FindGC_Single,ModGC_GetGC_Targets_single
-    use ModMath,        only:   ModMath_1D3D_interpolate_1D1D
-    use ModYinYang,     only:   ModYinYang_CoordConv_1D,ModYinYang_VecConv_1D
     use ModEquation,    only:   ModEquation_Dynamo
     use ModDiffusion,   only:   ModDiffusion_Aritificial_1
     use ModBoundary,    only:   ModBoundary_Dynamo_primitives
-    use ModWaveSpeed,   only:   ModWaveSpeed_Dynamo
-    use ModTimeStep,    only:   ModTimeStep_Dynamo
-#ifdef FALCON_DROPIN
-    ! drop-in build: the shims of falcon_amd/fortran in place of the reference's
-    ! ModAdvance / ModCommunication, stepping on the GPU
     use ModCommunication, only: ModCommunication_SetGCAll,ModCommunication_SendRecvGC_new
     use ModAdvance,       only: ModAdvance_rk4
+#ifdef FALCON_DROPIN
     use ModFalconShim,    only: falcon_sync_to_host
 #endif
     implicit none
 
     type(YYTree),target         ::  Tree
     character(len=512)          ::  fin,fout
-    integer                     ::  ui,uo,iamr,iflag(3),iglob,ib,nsteps,ndump,mode,istep,idump
+    integer                     ::  ui,iamr,iflag(3),iglob,ib,jb,nsteps,ndump,mode,istep,idump,ierr
     integer,allocatable         ::  dump_steps(:)
-    real(8)                     ::  r_range_cgs(2),rng(3,2),dt
+    real(8)                     ::  r_range_cgs(2),rng(3,2),dt,t0,t1
     real(8),allocatable         ::  rtp_ranges(:,:,:),bg(:,:),st(:,:,:,:)
-    integer                     ::  nstate,npad_i,npad_j,npad_k,v,ig,it
-    integer(8)                  ::  clk0,clk1,clk_rate
+    integer                     ::  nstate,v
     type(BlockType),pointer     ::  B
+
+    call MPI_INIT(ierr)
+    call MPI_Comm_size(MPI_COMM_WORLD,MpiSize,ierr)
+    call MPI_Comm_rank(MPI_COMM_WORLD,MpiRank,ierr)
 
     call get_command_argument(1,fin)
     call get_command_argument(2,fout)
-    open(newunit=ui,file=trim(fin),access='stream',form='unformatted',status='old')
-    open(newunit=uo,file=trim(fout),access='stream',form='unformatted',status='replace')
+    open(newunit=ui,file=trim(fin),access='stream',form='unformatted',status='old',action='read')
 
     read(ui) ni,nj,nk,ng,nstate
     read(ui) r_range_cgs
@@ -78,8 +74,7 @@ program falcon_ref_harness
     if (ndump>0) read(ui) dump_steps(1:ndump)
     read(ui) mode
 
-    ! Global parameters as the reference's #GRID/#EQUATION/#MPI setup leaves them.
-    MpiSize=1; MpiRank=0
+    ! Global parameters as the reference's #GRID/#EQUATION setup leaves them.
     iGeometry=1; iEquation=1; if_involve_B=.true.; DivB_option=0
     Artificial_heating_ratio=1.0d0
     nvar=9
@@ -96,6 +91,10 @@ program falcon_ref_harness
     call YinYangTree_LabelNodes(Tree,Tree%Yang)
     Tree%NumLeafNodes_YinYang(2)=Tree%NumLeafNodes-Tree%NumLeafNodes_YinYang(1)
     call ModAllocation_Init(Tree%NumLeafNodes)
+    ! the message tags of ModCommunication scale with nMaxBlocksPerRank
+    ! (3_6/ModCommunication.f90:947); raised, as a #MPI parameter would, to
+    ! hold every rank's share (3_6/test1.f90:122-130)
+    nMaxBlocksPerRank=max(nMaxBlocksPerRank,(Tree%NumLeafNodes+MpiSize-1)/MpiSize)
     allocate(rtp_ranges(3,2,nLocalLeafNodes))
     allocate(Tree%LocalBlocks(nLocalLeafNodes))
     allocate(Tree%iBlocks(nLocalLeafNodes))
@@ -105,7 +104,6 @@ program falcon_ref_harness
     call YinYangTree_GetLocalBlocksInfo(Tree,Tree%Yin ,rtp_ranges,ib)
     call YinYangTree_GetLocalBlocksInfo(Tree,Tree%Yang,rtp_ranges,ib)
 
-    npad_i=ni+2*ng; npad_j=nj+2*ng; npad_k=nk+2*ng
     allocate(bg(-ng+1:ni+ng,8),st(-ng+1:ni+ng,-ng+1:nj+ng,-ng+1:nk+ng,nstate))
 
     do ib=1,Tree%nLocalBlocks
@@ -124,71 +122,60 @@ program falcon_ref_harness
         call YinYangTree_if_top_bottom(B,Tree%r_range_CGS)
     end do
 
-    ! Tables (ModCommunication_SetGCAll, 3_6/ModCommunication.f90:22-64, single rank)
-    do ib=1,Tree%nLocalBlocks
-        call ModGC_FindGC_Single(Tree,Tree%LocalBlocks(ib))
-    end do
-    do ib=1,Tree%nLocalBlocks
-        call ModGC_GetGC_Targets_single(Tree,Tree%LocalBlocks(ib))
-    end do
-    call harness_set_sources(Tree)
-
-    ! ---- output: layout and tables
-    write(uo) Tree%nLocalBlocks,Tree%NumLeafNodes_YinYang(1)
-    do ib=1,Tree%nLocalBlocks
-        B=>Tree%LocalBlocks(ib)
-        write(uo) B%xijk_range
-        write(uo) merge(1,0,B%if_yin),merge(1,0,B%if_top),merge(1,0,B%if_bottom)
-        write(uo) B%xi_I,B%xj_I,B%xk_I
-    end do
-    do ib=1,Tree%nLocalBlocks
-        write(uo) Tree%LocalBlocks(ib)%GC_iBlocks_III
-    end do
-    do ib=1,Tree%nLocalBlocks
-        B=>Tree%LocalBlocks(ib)
-        write(uo) B%nGC_targets
-        do it=1,B%nGC_targets
-            write(uo) B%GC_targets(it)%iBlock,B%GC_targets(it)%nGC
-            write(uo) B%GC_targets(it)%ijk_list
-            write(uo) B%GC_targets(it)%xijk_list
-        end do
-    end do
-    do ib=1,Tree%nLocalBlocks
-        B=>Tree%LocalBlocks(ib)
-        write(uo) B%nGC_sources
-        do it=1,B%nGC_sources
-            write(uo) B%GC_sources(it)%iBlock,B%GC_sources(it)%nGC
-            write(uo) B%GC_sources(it)%ijk_list
-            write(uo) B%GC_sources(it)%xijk_list
-        end do
-    end do
-
+    ! 1-D background of every local block: read before ModCommunication_SetGCAll,
+    ! as YinYangTree_SetAll's ModBlock_InitSSM sets it before test1.f90:133
     if (mode>=2) then
-        ! ---- input: per block 1-D background (8 profiles) and padded state
-        do ib=1,Tree%nLocalBlocks
-            B=>Tree%LocalBlocks(ib)
+        do jb=1,Tree%NumLeafNodes
             read(ui) bg
             read(ui) st
-            call harness_background(B,bg)
-            B%primitive_IV=0.0d0
-            do v=1,min(nstate,8)
-                B%primitive_IV(:,:,:,v)=st(:,:,:,v)
-            end do
-            B%primitive_rk_IV=B%primitive_IV
-            B%primitive=>B%primitive_IV
+            ib=jb-Tree%iBlocks(1)+1
+            if (ib>=1 .and. ib<=Tree%nLocalBlocks) call harness_background(Tree%LocalBlocks(ib),bg)
+        end do
+    end if
+
+    ! Tables (3_6/test1.f90:133)
+    call ModCommunication_SetGCAll(Tree)
+
+    ! ---- output: layout and tables
+    if (MpiRank==0) then
+        block
+            integer :: uo
+            open(newunit=uo,file=trim(fout),access='stream',form='unformatted',status='replace')
+            write(uo) Tree%NumLeafNodes,Tree%NumLeafNodes_YinYang(1)
+            close(uo)
+        end block
+    end if
+    call emit(1,0,0.0d0)
+    call emit(2,0,0.0d0)
+    call emit(3,0,0.0d0)
+    call emit(4,0,0.0d0)
+
+    if (mode>=2) then
+        ! initial state of every local block: ModInitiation_DoAll's place (test1.f90:140)
+        rewind(ui)
+        call skip_header()
+        do jb=1,Tree%NumLeafNodes
+            read(ui) bg
+            read(ui) st
+            ib=jb-Tree%iBlocks(1)+1
+            if (ib>=1 .and. ib<=Tree%nLocalBlocks) then
+                B=>Tree%LocalBlocks(ib)
+                B%primitive_IV=0.0d0
+                do v=1,min(nstate,8)
+                    B%primitive_IV(:,:,:,v)=st(:,:,:,v)
+                end do
+                B%primitive_rk_IV=B%primitive_IV
+                B%primitive=>B%primitive_IV
+            end if
         end do
 
-        ! initial ghost fill as 3_6/test1.f90:133,143-144
+        ! initial ghost fill (3_6/test1.f90:143-144)
+        call ModCommunication_SendRecvGC_new(Tree,.false.)
+        call ModCommunication_SendRecvGC_new(Tree,.false.)
 #ifdef FALCON_DROPIN
-        call ModCommunication_SetGCAll(Tree)
-        call ModCommunication_SendRecvGC_new(Tree,.false.)
-        call ModCommunication_SendRecvGC_new(Tree,.false.)
         call falcon_sync_to_host(Tree)
-#else
-        call harness_exchange(Tree,.false.)
-        call harness_exchange(Tree,.false.)
 #endif
-        call harness_dump(Tree,uo,0,0.0d0)
+        call emit(5,0,0.0d0)
 
         ! single-operator dumps on the exchanged initial state
         do ib=1,Tree%nLocalBlocks
@@ -197,14 +184,14 @@ program falcon_ref_harness
             call ModBoundary_Dynamo_primitives(B)
             B%EQN_update_R_IV=0.0d0
             call ModEquation_Dynamo(B)
-            write(uo) B%EQN_update_R_IV(:,:,:,1:8)
         end do
+        call emit(6,0,0.0d0)
         do ib=1,Tree%nLocalBlocks
             B=>Tree%LocalBlocks(ib)
             B%EQN_update_R_IV=0.0d0
             call ModDiffusion_Aritificial_1(B,B%EQN_update_R_IV,2,.false.)
-            write(uo) B%EQN_update_R_IV(:,:,:,1:8)
         end do
+        call emit(6,0,0.0d0)
         ! restore the state the stepping starts from (BCs above touched ghosts,
         ! which ModAdvance_rk4 overwrites at stage 1 anyway)
         do ib=1,Tree%nLocalBlocks
@@ -212,30 +199,98 @@ program falcon_ref_harness
         end do
 
         idump=1
-        call system_clock(clk0,clk_rate)
+        call MPI_BARRIER(MPI_COMM_WORLD,ierr)
+        t0=MPI_Wtime()
         do istep=1,nsteps
-#ifdef FALCON_DROPIN
+            ! 3_6/test1.f90:58
             call ModAdvance_rk4(Tree,.true.,dt)
-#else
-            call harness_rk4(Tree,dt)
-#endif
             if (idump<=ndump) then
                 if (dump_steps(idump)==istep) then
+                    ! the dumps stay outside the timed loop
+                    t1=MPI_Wtime()
 #ifdef FALCON_DROPIN
                     call falcon_sync_to_host(Tree)
 #endif
-                    call harness_dump(Tree,uo,istep,dt)
+                    call emit(5,istep,dt)
                     idump=idump+1
+                    t0=t0+(MPI_Wtime()-t1)
                 end if
             end if
         end do
-        call system_clock(clk1)
+        call MPI_BARRIER(MPI_COMM_WORLD,ierr)
+        t1=MPI_Wtime()
         ! wall time of the stepping loop (the CPU baseline reads this line)
-        write(*,'(a,i0,a,es16.9)') 'harness_steps ',nsteps,' seconds ',real(clk1-clk0,8)/real(clk_rate,8)
+        if (MpiRank==0) write(*,'(a,i0,a,es16.9,a,i0)') 'harness_steps ',nsteps,' seconds ',t1-t0,' ranks ',MpiSize
     end if
-    close(ui); close(uo)
+    close(ui)
+    call MPI_FINALIZE(ierr)
 
 contains
+
+    subroutine skip_header()
+        integer :: h5(5),k,n1,n2,n3,md,ia(4)
+        real(8) :: r2(2),r6(6),c
+        read(ui) h5
+        read(ui) r2
+        read(ui) n1
+        do k=1,n1
+            read(ui) ia,r6
+        end do
+        read(ui) c
+        read(ui) n2
+        read(ui) n3
+        if (n3>0) read(ui) dump_steps(1:n3)
+        read(ui) md
+    end subroutine skip_header
+
+    ! One section of the output, written rank after rank (each rank's blocks are a
+    ! contiguous, ordered range of leaf labels, ModAllocation.f90:25-61), so the
+    ! file lists every block in label order whatever the rank count.
+    ! what: 1 layout, 2 GC_iBlocks_III, 3 GC_targets, 4 GC_sources, 5 state dump
+    ! (step, dt, primitive_IV), 6 EQN_update_R_IV.
+    subroutine emit(what,istep,dt)
+        integer,intent(in)          ::  what,istep
+        real(8),intent(in)          ::  dt
+        integer                     ::  r,uo,ib,it
+        type(BlockType),pointer     ::  B
+        do r=0,MpiSize-1
+            if (r==MpiRank) then
+                open(newunit=uo,file=trim(fout),access='stream',form='unformatted',status='old',position='append')
+                if (what==5 .and. MpiRank==0) write(uo) istep,dt
+                do ib=1,Tree%nLocalBlocks
+                    B=>Tree%LocalBlocks(ib)
+                    select case(what)
+                    case(1)
+                        write(uo) B%xijk_range
+                        write(uo) merge(1,0,B%if_yin),merge(1,0,B%if_top),merge(1,0,B%if_bottom)
+                        write(uo) B%xi_I,B%xj_I,B%xk_I
+                    case(2)
+                        write(uo) B%GC_iBlocks_III
+                    case(3)
+                        write(uo) B%nGC_targets
+                        do it=1,B%nGC_targets
+                            write(uo) B%GC_targets(it)%iBlock,B%GC_targets(it)%nGC
+                            write(uo) B%GC_targets(it)%ijk_list
+                            write(uo) B%GC_targets(it)%xijk_list
+                        end do
+                    case(4)
+                        write(uo) B%nGC_sources
+                        do it=1,B%nGC_sources
+                            write(uo) B%GC_sources(it)%iBlock,B%GC_sources(it)%nGC
+                            write(uo) B%GC_sources(it)%ijk_list
+                            write(uo) B%GC_sources(it)%xijk_list
+                        end do
+                    case(5)
+                        write(uo) B%primitive_IV(:,:,:,1:8)
+                    case(6)
+                        write(uo) B%EQN_update_R_IV(:,:,:,1:8)
+                    end select
+                end do
+                close(uo)
+            end if
+            call MPI_BARRIER(MPI_COMM_WORLD,ierr)
+        end do
+    end subroutine emit
 
     ! Background replication exactly as ModBlock_InitSSM (3_6/ModBlock.f90:252-264).
     ! bg columns: g, rho0, p0, T0, gamma1, gamma3, heat, xi_rsst.
@@ -274,145 +329,5 @@ contains
             end do
         end do
     end subroutine harness_background
-
-    ! Local branch of ModCommunication_GetnGC_sources / SetGC_sources
-    ! (3_6/ModCommunication.f90:66-247): every receiving block's target list is
-    ! appended to the donor's GC_sources, with coordinates moved into the donor's
-    ! panel frame by ModYinYang_CoordConv_1D when the panels differ.
-    subroutine harness_set_sources(Tree)
-        type(YYTree),target         ::  Tree
-        type(BlockType),pointer     ::  Bt,Bs
-        type(GC_target),pointer     ::  tg,sc
-        integer,allocatable         ::  cnt(:)
-        integer                     ::  ib,it
-
-        allocate(cnt(Tree%NumLeafNodes)); cnt=0
-        do ib=1,Tree%nLocalBlocks
-            Bt=>Tree%LocalBlocks(ib)
-            do it=1,Bt%nGC_targets
-                cnt(Bt%GC_targets(it)%iBlock)=cnt(Bt%GC_targets(it)%iBlock)+1
-            end do
-        end do
-        do ib=1,Tree%nLocalBlocks
-            allocate(Tree%LocalBlocks(ib)%GC_sources(cnt(Tree%LocalBlocks(ib)%iBlock)))
-            Tree%LocalBlocks(ib)%nGC_sources=0
-        end do
-        do ib=1,Tree%nLocalBlocks
-            Bt=>Tree%LocalBlocks(ib)
-            do it=1,Bt%nGC_targets
-                tg=>Bt%GC_targets(it)
-                Bs=>Tree%LocalBlocks(tg%iBlock)
-                Bs%nGC_sources=Bs%nGC_sources+1
-                sc=>Bs%GC_sources(Bs%nGC_sources)
-                sc%iRank=0
-                sc%iBlock=Bt%iBlock
-                sc%if_yin=Bt%iBlock.le.Tree%NumLeafNodes_YinYang(1)
-                sc%nGC=tg%nGC
-                allocate(sc%xijk_list(sc%nGC,3),sc%ijk_list(sc%nGC,3),sc%primitive_list(sc%nGC,nvar))
-                sc%ijk_list=tg%ijk_list
-                if (sc%if_yin .eqv. Bs%if_yin) then
-                    sc%xijk_list=tg%xijk_list
-                else
-                    sc%xijk_list=ModYinYang_CoordConv_1D(tg%xijk_list,tg%nGC)
-                end if
-            end do
-        end do
-    end subroutine harness_set_sources
-
-    ! ModCommunication_SendRecvGC_new (3_6/ModCommunication.f90:655-795) on one rank.
-    subroutine harness_exchange(Tree,if_rk)
-        type(YYTree),target         ::  Tree
-        logical,intent(in)          ::  if_rk
-        type(BlockType),pointer     ::  Bs,Bt
-        type(GC_target),pointer     ::  sc
-        real(8),pointer             ::  ps(:,:,:,:),pt(:,:,:,:)
-        integer                     ::  ib,is,ig
-
-        do ib=1,Tree%nLocalBlocks
-            Bs=>Tree%LocalBlocks(ib)
-            if (if_rk) then
-                ps=>Bs%primitive_rk_IV
-            else
-                ps=>Bs%primitive_IV
-            end if
-            do is=1,Bs%nGC_sources
-                sc=>Bs%GC_sources(is)
-                call ModMath_1D3D_interpolate_1D1D(ps,nvar,ni,nj,nk,ng,&
-                    Bs%xi_I,Bs%xj_I,Bs%xk_I,sc%nGC,sc%xijk_list,sc%primitive_list)
-                if (sc%if_yin .neqv. Bs%if_yin) then
-                    sc%primitive_list(:,Bs%br_:Bs%bp_)=ModYinYang_VecConv_1D(sc%xijk_list,&
-                        sc%primitive_list(:,Bs%br_:Bs%bp_),sc%nGC)
-                    sc%primitive_list(:,Bs%vr_:Bs%vp_)=ModYinYang_VecConv_1D(sc%xijk_list,&
-                        sc%primitive_list(:,Bs%vr_:Bs%vp_),sc%nGC)
-                end if
-                Bt=>Tree%LocalBlocks(sc%iBlock)
-                if (if_rk) then
-                    pt=>Bt%primitive_rk_IV
-                else
-                    pt=>Bt%primitive_IV
-                end if
-                do ig=1,sc%nGC
-                    pt(sc%ijk_list(ig,1),sc%ijk_list(ig,2),sc%ijk_list(ig,3),:)=sc%primitive_list(ig,:)
-                end do
-            end do
-        end do
-    end subroutine harness_exchange
-
-    ! ModAdvance_rk4 (3_6/ModAdvance.f90:18-112), iEquation=1, one rank.
-    subroutine harness_rk4(Tree,dt_global)
-        type(YYTree),target         ::  Tree
-        real(8),intent(out)         ::  dt_global
-        type(BlockType),pointer     ::  B
-        integer                     ::  ib,rk_index
-        logical                     ::  if_rk_input,if_rk_output
-        real(8)                     ::  dt_local
-
-        do ib=1,Tree%nLocalBlocks
-            call ModWaveSpeed_Dynamo(Tree%LocalBlocks(ib))
-        end do
-        call ModTimeStep_Dynamo(Tree,CFL,dt_local)
-        dt_global=dt_local
-        do rk_index=1,4
-            do ib=1,Tree%nLocalBlocks
-                B=>Tree%LocalBlocks(ib)
-                if_rk_input=(rk_index>1)
-                if_rk_output=(rk_index<4)
-                if (if_rk_input) then
-                    B%primitive=>B%primitive_rk_IV
-                else
-                    B%primitive=>B%primitive_IV
-                end if
-                call ModBoundary_Dynamo_primitives(B)
-                call ModEquation_Dynamo(B)
-                if (if_rk_output) then
-                    B%primitive_rk_IV(1:ni,1:nj,1:nk,:)=&
-                        B%primitive_IV(1:ni,1:nj,1:nk,:)+dt_global*B%EQN_update_R_IV/(5.0-rk_index)
-                else
-                    B%primitive_IV(1:ni,1:nj,1:nk,:)=&
-                        B%primitive_IV(1:ni,1:nj,1:nk,:)+dt_global*B%EQN_update_R_IV/(5.0-rk_index)
-                end if
-            end do
-            call harness_exchange(Tree,if_rk_output)
-        end do
-        do ib=1,Tree%nLocalBlocks
-            B=>Tree%LocalBlocks(ib)
-            B%EQN_update_R_IV=0.0
-            call ModDiffusion_Aritificial_1(B,B%EQN_update_R_IV,2,.false.)
-            B%primitive_IV(1:ni,1:nj,1:nk,:)=&
-                B%primitive_IV(1:ni,1:nj,1:nk,:)+0.5*dt_global*B%EQN_update_R_IV
-        end do
-        call harness_exchange(Tree,.false.)
-    end subroutine harness_rk4
-
-    subroutine harness_dump(Tree,uo,istep,dt)
-        type(YYTree),target         ::  Tree
-        integer,intent(in)          ::  uo,istep
-        real(8),intent(in)          ::  dt
-        integer                     ::  ib
-        write(uo) istep,dt
-        do ib=1,Tree%nLocalBlocks
-            write(uo) Tree%LocalBlocks(ib)%primitive_IV(:,:,:,1:8)
-        end do
-    end subroutine harness_dump
 
 end program falcon_ref_harness

@@ -134,9 +134,11 @@ def read_output(path, layout, mode, ndumps):
     return out
 
 
-def run_harness(layout, cfl=0.5, nsteps=0, dumps=(), mode=1, blocks_bg=None, blocks_state=None, exe=None):
+def run_harness(layout, cfl=0.5, nsteps=0, dumps=(), mode=1, blocks_bg=None, blocks_state=None, exe=None,
+                launcher=()):
+    """Run the harness (launcher: e.g. ["mpirun", "-n", "2"]) and read its output."""
     with tempfile.TemporaryDirectory() as td:
         fi, fo = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         write_input(fi, layout, cfl, nsteps, list(dumps), mode, blocks_bg, blocks_state)
-        subprocess.run([exe or HARNESS, fi, fo], check=True)
+        subprocess.run(list(launcher) + [exe or HARNESS, fi, fo], check=True, stdout=subprocess.DEVNULL)
         return read_output(fo, layout, mode, len(dumps))

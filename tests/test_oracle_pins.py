@@ -177,3 +177,27 @@ def test_yinyang_map_involution_and_rotation_orthogonal():
         assert np.allclose([-a[0], a[2], a[1]], b, atol=1e-12)
         L.orc_vecconv_coeffs(_p(p), _p(co))
         assert abs(co[0] ** 2 + co[1] ** 2 - 1.0) < 1e-11
+
+
+@pytest.mark.parametrize("lam_dt,mu_dt", [(-0.7, 0.0), (0.3, 0.0), (-1.9, -0.4), (0.05, 0.8)])
+def test_step_composition_is_classical_rk4(lam_dt, mu_dt):
+    """With the stage right-hand side R = lambda * (stage input) and the
+    diffusion's R = mu * U, one orc_step multiplies every interior value by
+    (1 + z + z^2/2 + z^3/6 + z^4/24)(1 + mu dt/2), z = lambda dt: the low-storage
+    coefficients 1/(5-k) (3_6/ModAdvance.f90:89-92), the stage inputs (U for
+    stage 1, the stage buffer after), the accumulation base U^n and the diffusion
+    update U += 0.5 dt R (:104-107).  A wrong stage input gives 1 + z, a wrong
+    coefficient another polynomial, a diffusion read of the stage buffer
+    another factor."""
+    o = _setup(lambda r, t, p: {0: 1e-3 * np.sin(3 * t) * np.cos(p), 1: 1e2 * (r / 7e10 - 0.85),
+                                3: 50.0 + t, 5: 0.5 * np.cos(p), 7: 1e3 * (1 + np.sin(t + p))})
+    dt = o.dt(0.5)
+    o.set_linear_rhs(True, lam_dt / dt, mu_dt / dt)
+    before = [o.get_state(ib) for ib in range(o.nblocks)]
+    assert o.step(0.5) == dt
+    z = lam_dt
+    amp = (1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24) * (1 + mu_dt / 2)
+    for ib in range(o.nblocks):
+        a, b = _interior(o, o.get_state(ib)), _interior(o, before[ib])
+        assert np.allclose(a, amp * b, rtol=1e-13, atol=0), (ib, np.max(np.abs(a - amp * b)))
+    o.set_linear_rhs(False)

@@ -96,3 +96,46 @@ def test_live_reference_fresh_seed():
     assert dt == out["dumps"][1][1]
     for ib in range(o.nblocks):
         assert np.array_equal(o.get_state(ib), out["dumps"][1][2][ib])
+
+
+MPIRUN = "/opt/conda/bin/mpirun"
+
+
+@pytest.mark.skipif(not (os.path.exists(ref_io.HARNESS) and os.path.exists(MPIRUN)),
+                    reason="flang/MPICH build of the reference not present")
+@pytest.mark.parametrize("ranks", [2, 3])
+def test_reference_under_mpirun_matches_one_rank(ranks):
+    """The reference driver with its own ModCommunication/ModAdvance under
+    `mpirun -n N`: blocks split over ranks by ModAllocation_Init, remote ghosts
+    sent and received over MPI.  Tables, the initial two-fill state (3_6/test1.f90:
+    143-144), the RHS, the diffusion and the first step's dt equal the one-rank
+    run's bit for bit.
+
+    The states after a step are not compared: the reference's receive path
+    copies each received message into the targets' lists right after posting
+    MPI_IRECV (3_6/ModCommunication.f90:731-739), before MPI_WAITALL (:764), so
+    a remote ghost gets the message of the previous fill.  The initial fills
+    are two calls on one state, so they come out right; every fill during a
+    step on more than one rank uses the previous fill's remote values.  The
+    one-rank run (every donor local) and this library (one process per GPU,
+    every block resident) have no remote ghosts."""
+    c = make_golden.CASES["mixed"]
+    o = pyoracle.Oracle(c["layout"])
+    bgs, sts = make_golden.inputs(c, lambda ib: o.centres(ib)[0], o.pad, o.nblocks)
+    one = ref_io.run_harness(c["layout"], cfl=0.5, nsteps=1, dumps=[1], mode=2, blocks_bg=bgs, blocks_state=sts)
+    many = ref_io.run_harness(c["layout"], cfl=0.5, nsteps=1, dumps=[1], mode=2, blocks_bg=bgs, blocks_state=sts,
+                              launcher=[MPIRUN, "-n", str(ranks)])
+    assert one["nblocks"] == many["nblocks"]
+    for a, b in zip(one["blocks"], many["blocks"]):
+        assert np.array_equal(a["range"], b["range"]) and np.array_equal(a["gcib"], b["gcib"])
+        assert [(t[0], t[1].tolist(), t[2].tolist()) for t in a["targets"]] == \
+               [(t[0], t[1].tolist(), t[2].tolist()) for t in b["targets"]]
+        # remote sources are appended in arrival order (MPI_ANY_SOURCE): same set
+        key = lambda t: (t[0], t[1].tobytes(), t[2].tobytes())
+        assert sorted(map(key, a["sources"])) == sorted(map(key, b["sources"]))
+    for x, y in zip(one["dumps"][0][2], many["dumps"][0][2]):
+        assert np.array_equal(x, y)
+    for lab in ("rhs", "diff"):
+        for x, y in zip(one[lab], many[lab]):
+            assert np.array_equal(x, y), lab
+    assert one["dumps"][1][1] == many["dumps"][1][1]
