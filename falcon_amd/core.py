@@ -63,6 +63,9 @@ def load():
         "falcon_advance_rk4": [vp, C.c_double, dp],
         "falcon_advance": [vp, C.c_int, C.c_double, C.c_int, dp],
         "falcon_compute_dt": [vp, C.c_double, dp],
+        "falcon_set_fixed_dt": [vp, C.c_double],
+        "falcon_check_state": [vp, C.c_int, i64p, ip],
+        "falcon_state_diff": [vp, vp, C.c_double, dp, dp, i64p, i64p],
         "falcon_rhs": [vp, dp],
         "falcon_diffusion_rhs": [vp, dp],
         "falcon_profile_step": [vp, C.c_int, C.c_double, dp],
@@ -87,6 +90,7 @@ def load():
                                         C.c_double, C.c_uint64, dp, i64p, dp],
         "falcon_heat_attach_corrector": [vp],
         "falcon_heat_set_learned_sweeps": [vp, C.c_int, C.c_int],
+        "falcon_heat_set_coarse": [vp, C.c_double, C.c_int],
         "falcon_heat_set_graph": [vp, C.c_int],
         "falcon_heat_set_learned_blockstep": [vp, C.c_int],
         "falcon_heat_set_learned_iters": [vp, C.c_int],
@@ -292,6 +296,37 @@ class Shell:
         _check(self.L.falcon_compute_dt(self.h, cfl, C.byref(dt)))
         return dt.value
 
+    def check_state(self, which=0):
+        """ModCheck_primitive on the device: (NaN count, first NaN as (block,
+        I, J, K padded 0-based, variable) or None); which 0 = state, 1 = stage buffer."""
+        n = C.c_int64()
+        w = np.zeros(5, np.int32)
+        _check(self.L.falcon_check_state(self.h, int(which), C.byref(n), _p(w, C.c_int32)))
+        return n.value, (tuple(int(x) for x in w) if n.value else None)
+
+    def set_fixed_dt(self, dt):
+        """dt > 0: later steps use this dt instead of the CFL dt; 0 restores it."""
+        _check(self.L.falcon_set_fixed_dt(self.h, float(dt)))
+
+    def state_diff(self, ref, rel_tol=1e-6):
+        """Interior difference to another Shell's state (same layout), per
+        variable: max|a-b|, max|ref|, cells above rel_tol*max|ref| and the
+        (block, i, j, k) of the largest difference."""
+        md, mr = np.zeros(8), np.zeros(8)
+        na, am = np.zeros(8, np.int64), np.zeros(8, np.int64)
+        _check(self.L.falcon_state_diff(self.h, ref.h, float(rel_tol), _p(md), _p(mr), _p(na, C.c_int64),
+                                        _p(am, C.c_int64)))
+        where = []
+        for x in am.tolist():
+            if x < 0:
+                where.append(None)
+                continue
+            k = x % self.nk; x //= self.nk
+            j = x % self.nj; x //= self.nj
+            i = x % self.ni; b = x // self.ni
+            where.append((int(b), int(i), int(j), int(k)))
+        return {"max_diff": md, "max_ref": mr, "n_above": na, "argmax": where}
+
     def rhs(self):
         out = np.zeros((self.nblocks, 8, self.nk, self.nj, self.ni))
         _check(self.L.falcon_rhs(self.h, _p(out)))
@@ -407,6 +442,10 @@ class HeatSolver:
         """Build the first n search directions of a learned solve with the learned
         correction and the rest with the plain V-cycle (0: all of them)."""
         _check(self.L.falcon_heat_set_learned_iters(self.s.h, int(n)))
+
+    def set_coarse(self, tol, max_iter):
+        """Stop of the V-cycle's coarsest-level CG (relative tol, iteration cap)."""
+        _check(self.L.falcon_heat_set_coarse(self.s.h, float(tol), int(max_iter)))
 
     def set_learned_sweeps(self, pre0, post0):
         """Finest-level smoothing sweeps of the V-cycle used with the learned correction."""

@@ -10,6 +10,7 @@ namespace falcon {
 struct BlockMeta {
     double cr, ct, cp;      // (double)(1.0f/12.0f)/dx
     double dxi, dxj, dxk;
+    int flags;              // bit 0: bottom block (inner radial boundary), bit 1: top block
 };
 
 // Geometry and background tables on the device, one row per block.
@@ -70,9 +71,12 @@ struct DevBoundary {
 template <typename T, typename TIN, typename TOUT, typename TB>
 void launch_stage(const Dims &d, const DevTables &t, const TIN *in, const TB *U, TOUT *out, float *shadow,
                   const double *dtp, double div, hipStream_t s, const int32_t *pull, int tile, int wgs);
+// Uold: U^n for the FP32 path, whose diffusion updates U (= out) in place;
+// null for the FP64 path (U != out)
 template <typename T, typename TIN>
 void launch_diffusion(const Dims &d, const DevTables &t, const TIN *in, const double *U, double *out, float *shadow,
-                      double *partial, const double *dtp, hipStream_t s, const int32_t *pull, int tile, int wgs);
+                      double *partial, const double *dtp, hipStream_t s, const int32_t *pull, int tile, int wgs,
+                      const double *Uold = nullptr);
 // workgroups of the stage (kernel 0) or diffusion (1) march of a precision that
 // the device holds resident at once: the persistent grid
 int march_resident_wgs(int kernel, int precision, int tile);
@@ -89,6 +93,15 @@ void launch_copy_radial(const Dims &d, const DevBoundary &b, const TS *src, TS *
 void launch_dt_partial(const Dims &d, const DevTables &t, const double *U, double *partial, hipStream_t s, int tile);
 void launch_dt_final(const Dims &d, const double *partial, double *dtp, double cfl, hipStream_t s, int tile);
 void launch_to_f32(const double *src, float *dst, long long n, hipStream_t s);
+void launch_set_scalar(double *p, double v, hipStream_t s);
+// NaN scan of a padded state (see k_nan_scan): w[0] counts, w[1] first position;
+// w starts as {0, ~0}
+template <typename TS>
+void launch_nan_scan(const Dims &d, const TS *st, unsigned long long *w, hipStream_t s);
+// interior difference of two states (see k_state_diff): pass 0 then pass 1 on
+// the same 32 words w, whose first 8 and last 8 start at 0 and at ~0
+void launch_state_diff(const Dims &d, const double *a, const double *b, double tol, unsigned long long *w, int pass,
+                       hipStream_t s);
 // single-operator entry points used by parity tests: R only
 template <typename T, typename TIN>
 void launch_rhs_only(const Dims &d, const DevTables &t, const TIN *in, double *R, hipStream_t s, int tile, int wgs);

@@ -80,6 +80,7 @@ struct DevBuf {
 struct falcon_handle {
     Layout lay;
     bool labelled = false, built = false, tables = false;
+    bool bg_dirty = false;    // a background changed after the device tables went up
     std::vector<double> bg;   // per block 8 x NI
     std::vector<char> bg_set;
     int precision = 0;
@@ -94,6 +95,8 @@ struct falcon_handle {
     bool ghost_pull = true;              // FP32 step: kernels read same-level ghost copies in place
 
     DevBuf U[2], F32[2], S64[2], S32[2], dtp, partial, rbuf;
+    DevBuf diffbuf;          // falcon_state_diff's reduction words
+    DevBuf scanbuf;          // falcon_check_state's count and first position
     int cur = 0;             // index of the current state buffer in U[]
     // validity of derived device data
     bool partials_valid = false;   // partial[] holds min(h/v_wave) of U[cur]
@@ -114,6 +117,8 @@ struct falcon_handle {
     HeatParams heat_params;
     std::unique_ptr<CnnModel> cnn;
     double graph_cfl = -1;
+    // dt > 0: every step uses it in place of the CFL dt (falcon_set_fixed_dt)
+    double fixed_dt = 0, graph_fixed_dt = 0;
 
     ~falcon_handle() {
         if (graph) (void)hipGraphExecDestroy(graph);
@@ -121,7 +126,7 @@ struct falcon_handle {
                          &ghost32[1], &ghost32[2], &ghost32[3], &ghost_copy32, &ghost_lerp32[0], &ghost_lerp32[1],
                          &bnd_list, &ghost_rest32, &ghost_pull32,
 
-                         &U[0], &U[1], &F32[0], &F32[1], &S64[0], &S64[1], &S32[0], &S32[1], &dtp, &partial, &rbuf};
+                         &U[0], &U[1], &F32[0], &F32[1], &S64[0], &S64[1], &S32[0], &S32[1], &dtp, &partial, &rbuf, &diffbuf, &scanbuf};
         for (DevBuf *b : all) b->free();
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -276,38 +281,63 @@ void build_pull_table(const falcon_handle *h, const std::vector<int2> &copy32, s
     }
 }
 
-void upload_tables(falcon_handle *h) {
+constexpr int N_RADIAL_ROWS = 11, N_ANGULAR_ROWS = 3;
+
+// Radial rows of every block from its 1-D background, as [row][block][I]:
+// the 3_6/ModBlock.f90:255-262 products, then the per-term products of
+// ModEquation.
+std::vector<double> radial_rows(const falcon_handle *h) {
     const Tree &t = h->lay.tree;
-    const int nb = t.nleaf, NI = h->NI(), NJ = h->NJ(), NK = h->NK(), ng = t.ng;
-    for (int b = 0; b < nb; b++)
-        if (!h->bg_set[b]) throw Error("background not set for block " + std::to_string(b));
-    // per-block constants
-    std::vector<BlockMeta> meta(nb);
-    const int NR = 11, NA = 3;
-    std::vector<double> rad((size_t)nb * NR * NI), ang((size_t)nb * NA * NJ);
+    const int nb = t.nleaf, NI = h->NI(), NR = N_RADIAL_ROWS;
+    std::vector<double> radT((size_t)nb * NR * NI);
     for (int b = 0; b < nb; b++) {
+        if (!h->bg_set[b]) throw Error("background not set for block " + std::to_string(b));
         const BlockGeom &g = h->lay.geom[b];
-        const double c12 = (double)(1.0f / 12.0f);
-        meta[b] = {c12 / g.dxi, c12 / g.dxj, c12 / g.dxk, g.dxi, g.dxj, g.dxk};
         const double *B = h->bg.data() + (size_t)b * 8 * NI;
         for (int I = 0; I < NI; I++) {
             const double gg = B[I], rho0 = B[I + NI], p0 = B[I + 2 * NI], T0 = B[I + 3 * NI];
             const double g1 = B[I + 4 * NI], g3 = B[I + 5 * NI], heat = B[I + 6 * NI], xr = B[I + 7 * NI];
-            // 3_6/ModBlock.f90:255-262 products, then the per-term products of ModEquation
             const double p0r = p0 / rho0, rho0T0 = rho0 * T0, gm1 = g3 - 1.0, x = g.xi[I];
-            double *row = rad.data() + (size_t)b * NR * NI;
-            row[0 * NI + I] = x;
-            row[1 * NI + I] = 1.0 / x;
-            row[2 * NI + I] = 1.0 / (x * x);
-            row[3 * NI + I] = rho0;
-            row[4 * NI + I] = g1 * p0r;
-            row[5 * NI + I] = gm1 * rho0T0;
-            row[6 * NI + I] = gg / rho0;
-            row[7 * NI + I] = -1.0 / (xr * xr);
-            row[8 * NI + I] = (1.0 * (heat + 0.0)) / rho0T0;
-            row[9 * NI + I] = (1.0 / xr) * std::sqrt(g1 * p0r);
-            row[10 * NI + I] = std::sqrt(rho0);
+            auto row = [&](int a) -> double & { return radT[((size_t)a * nb + b) * NI + I]; };
+            row(0) = x;
+            row(1) = 1.0 / x;
+            row(2) = 1.0 / (x * x);
+            row(3) = rho0;
+            row(4) = g1 * p0r;
+            row(5) = gm1 * rho0T0;
+            row(6) = gg / rho0;
+            row(7) = -1.0 / (xr * xr);
+            row(8) = (1.0 * (heat + 0.0)) / rho0T0;
+            row(9) = (1.0 / xr) * std::sqrt(g1 * p0r);
+            row(10) = std::sqrt(rho0);
         }
+    }
+    return radT;
+}
+
+// A background set after the tables went up: the radial rows are rewritten in
+// place (same buffer, so captured graphs stay valid) and the dt partial minima,
+// which depend on the sound speed, are recomputed before the next step.
+void upload_radial(falcon_handle *h) {
+    const std::vector<double> radT = radial_rows(h);
+    if (h->radial.bytes != sizeof(double) * radT.size()) throw Error("radial table size changed");
+    if (h->stream) HIPCHECK(hipStreamSynchronize(h->stream));
+    HIPCHECK(hipMemcpy(h->radial.p, radT.data(), sizeof(double) * radT.size(), hipMemcpyHostToDevice));
+    h->partials_valid = false;
+    h->shadow_valid = false;
+}
+
+void upload_tables(falcon_handle *h) {
+    const Tree &t = h->lay.tree;
+    const int nb = t.nleaf, NI = h->NI(), NJ = h->NJ(), NK = h->NK(), ng = t.ng;
+    // per-block constants
+    std::vector<BlockMeta> meta(nb);
+    const int NA = N_ANGULAR_ROWS;
+    std::vector<double> radT = radial_rows(h), ang((size_t)nb * NA * NJ);
+    for (int b = 0; b < nb; b++) {
+        const BlockGeom &g = h->lay.geom[b];
+        const double c12 = (double)(1.0f / 12.0f);
+        meta[b] = {c12 / g.dxi, c12 / g.dxj, c12 / g.dxk, g.dxi, g.dxj, g.dxk, (g.bottom ? 1 : 0) | (g.top ? 2 : 0)};
         double *arow = ang.data() + (size_t)b * NA * NJ;
         for (int J = 0; J < NJ; J++) {
             const double s = std::sin(g.xj[J]);
@@ -316,14 +346,11 @@ void upload_tables(falcon_handle *h) {
             arow[2 * NJ + J] = std::cos(g.xj[J]) / s;
         }
     }
-    // reorganise radial/angular rows as [array][block][index] for simple pointers
-    std::vector<double> radT(rad.size()), angT(ang.size());
-    for (int b = 0; b < nb; b++) {
-        for (int a = 0; a < NR; a++)
-            std::memcpy(&radT[((size_t)a * nb + b) * NI], &rad[((size_t)b * NR + a) * NI], sizeof(double) * NI);
+    // angular rows as [array][block][index] for simple pointers
+    std::vector<double> angT(ang.size());
+    for (int b = 0; b < nb; b++)
         for (int a = 0; a < NA; a++)
             std::memcpy(&angT[((size_t)a * nb + b) * NJ], &ang[((size_t)b * NA + a) * NJ], sizeof(double) * NJ);
-    }
     h->meta.alloc(sizeof(BlockMeta) * nb);
     h->radial.alloc(sizeof(double) * radT.size());
     h->angular.alloc(sizeof(double) * angT.size());
@@ -482,7 +509,10 @@ void enqueue_step(falcon_handle *h, double cfl, int cur) {
     double *dtp = h->dtp.as<double>(), *part = h->partial.as<double>();
     const int ts = h->tile_stage[h->precision], td = h->tile_diff[h->precision];
     const int ws = h->wgs_stage[h->precision], wd = h->wgs_diff[h->precision];
-    timed(h, K_DT, [&] { launch_dt_final(d, part, dtp, cfl, s, td); });
+    timed(h, K_DT, [&] {
+        if (h->fixed_dt > 0) launch_set_scalar(dtp, h->fixed_dt, s);
+        else launch_dt_final(d, part, dtp, cfl, s, td);
+    });
     if (h->precision == 0) {
         double *S0 = h->S64[0].as<double>(), *S1 = h->S64[1].as<double>();
         const double *in[4] = {U, S0, S1, S0};
@@ -521,12 +551,13 @@ void enqueue_step(falcon_handle *h, double cfl, int cur) {
                 });
                 timed(h, K_XSTAGE, [&] { exchange(out[k - 1]); });
             } else {
+                // U^n stays in U: the diffusion's boundary faces read it
                 timed(h, K_STAGE,
-                      [&] { launch_stage<float, float, double, double>(d, t, in[3], U, U, F, dtp, div, s, pull, ts, ws); });
+                      [&] { launch_stage<float, float, double, double>(d, t, in[3], U, Un, F, dtp, div, s, pull, ts, ws); });
                 timed(h, K_XSTATE, [&] { exchange(F); });
             }
         }
-        timed(h, K_DIFF, [&] { launch_diffusion<float, float>(d, t, F, U, Un, Fn, part, dtp, s, pull, td, wd); });
+        timed(h, K_DIFF, [&] { launch_diffusion<float, float>(d, t, F, Un, Un, Fn, part, dtp, s, pull, td, wd, U); });
         timed(h, K_XSTATE, [&] { exchange(Fn); });
     }
 }
@@ -838,6 +869,7 @@ int falcon_set_background(falcon_handle_t h, int ib, const double *bg) {
         const int NI = h->NI();
         std::memcpy(h->bg.data() + (size_t)ib * 8 * NI, bg, sizeof(double) * 8 * NI);
         h->bg_set[ib] = 1;
+        if (h->dt.meta) h->bg_dirty = true;
     });
 }
 
@@ -871,11 +903,14 @@ int falcon_background_polytrope(falcon_handle_t h, double contrast, double rho_t
             }
             h->bg_set[b] = 1;
         }
+        if (h->dt.meta) h->bg_dirty = true;
     });
 }
 
 static void ensure_device_tables(falcon_handle_t h) {
     if (!h->dt.meta) upload_tables(h);
+    else if (h->bg_dirty) upload_radial(h);
+    h->bg_dirty = false;
 }
 
 int falcon_set_state(falcon_handle_t h, int ib, const double *prim) {
@@ -1071,7 +1106,7 @@ int falcon_autotune(falcon_handle_t h, int reps, int *shape_out, double *ms_out)
                         launch_stage<float, float, double, double>(d, t, F, U, Un, Fn, dtp, 1.0, s, pull, tl, ws);
                     }) / 4;
                     b = time([&] {
-                        launch_diffusion<float, float>(d, t, F, U, Un, Fn, part, dtp, s, pull, tl, wd);
+                        launch_diffusion<float, float>(d, t, F, Un, Un, Fn, part, dtp, s, pull, tl, wd, U);
                     });
                 }
                 ms[c] = std::min(ms[c], a);
@@ -1143,7 +1178,8 @@ int falcon_advance(falcon_handle_t h, int nsteps, double cfl, int use_graph, dou
         if (nsteps == 0) return;
         prepare_steps(h);
         if (use_graph && nsteps >= 2) {
-            if (!h->graph || h->graph_cur != h->cur || h->graph_prec != h->precision || h->graph_cfl != cfl) {
+            if (!h->graph || h->graph_cur != h->cur || h->graph_prec != h->precision || h->graph_cfl != cfl ||
+                h->graph_fixed_dt != h->fixed_dt) {
                 if (h->graph) { HIPCHECK(hipGraphExecDestroy(h->graph)); h->graph = nullptr; }
                 hipGraph_t g;
                 HIPCHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
@@ -1153,6 +1189,7 @@ int falcon_advance(falcon_handle_t h, int nsteps, double cfl, int use_graph, dou
                 HIPCHECK(hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0));
                 HIPCHECK(hipGraphDestroy(g));
                 h->graph_cur = h->cur; h->graph_prec = h->precision; h->graph_cfl = cfl;
+                h->graph_fixed_dt = h->fixed_dt;
             }
             int pairs = nsteps / 2;
             for (int p = 0; p < pairs; p++) HIPCHECK(hipGraphLaunch(h->graph, h->stream));
@@ -1166,6 +1203,84 @@ int falcon_advance(falcon_handle_t h, int nsteps, double cfl, int use_graph, dou
             HIPCHECK(hipMemcpyAsync(dt_last, h->dtp.p, sizeof(double), hipMemcpyDeviceToHost, h->stream));
         }
         HIPCHECK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int falcon_check_state(falcon_handle_t h, int which, int64_t *n_nan, int32_t *where) {
+    return guard([&] {
+        h->need_tables();
+        if (which != 0 && which != 1) throw Error("falcon_check_state: which must be 0 (state) or 1 (stage buffer)");
+        if (!n_nan) throw Error("falcon_check_state: null count");
+        if (!h->scanbuf.p) h->scanbuf.alloc(2 * sizeof(unsigned long long));
+        unsigned long long init[2] = {0ull, ~0ull}, out[2];
+        unsigned long long *w = h->scanbuf.as<unsigned long long>();
+        const Dims &d = h->dims;
+        if (which == 0 && h->dt.meta) refresh_u_ghosts(h);
+        HIPCHECK(hipMemcpyAsync(w, init, sizeof(init), hipMemcpyHostToDevice, h->stream));
+        if (which == 0) {
+            launch_nan_scan<double>(d, h->U[h->cur].as<double>(), w, h->stream);
+        } else if (h->precision == 0) {
+            if (!h->S64[0].p) throw Error("falcon_check_state: no stage buffer yet");
+            launch_nan_scan<double>(d, h->S64[0].as<double>(), w, h->stream);
+        } else {
+            if (!h->S32[0].p) throw Error("falcon_check_state: no stage buffer yet");
+            launch_nan_scan<float>(d, h->S32[0].as<float>(), w, h->stream);
+        }
+        check_launch();
+        HIPCHECK(hipMemcpyAsync(out, w, sizeof(out), hipMemcpyDeviceToHost, h->stream));
+        HIPCHECK(hipStreamSynchronize(h->stream));
+        *n_nan = (int64_t)out[0];
+        if (where) {
+            for (int q = 0; q < 5; q++) where[q] = -1;
+            if (out[0]) {
+                unsigned long long k = out[1];
+                where[4] = (int32_t)(k % 8); k /= 8;
+                where[1] = (int32_t)(k % d.NI); k /= d.NI;
+                where[2] = (int32_t)(k % d.NJ); k /= d.NJ;
+                where[3] = (int32_t)(k % d.NK); k /= d.NK;
+                where[0] = (int32_t)k;
+            }
+        }
+    });
+}
+
+int falcon_set_fixed_dt(falcon_handle_t h, double dt) {
+    return guard([&] {
+        if (!(dt >= 0) || !std::isfinite(dt)) throw Error("fixed dt must be finite and >= 0");
+        h->fixed_dt = dt;
+    });
+}
+
+int falcon_state_diff(falcon_handle_t a, falcon_handle_t b, double rel_tol, double *max_diff, double *max_ref,
+                      int64_t *n_above, int64_t *argmax) {
+    return guard([&] {
+        a->need_tables();
+        b->need_tables();
+        const Dims &d = a->dims, &e = b->dims;
+        if (d.ni != e.ni || d.nj != e.nj || d.nk != e.nk || d.ng != e.ng || d.nblocks != e.nblocks)
+            throw Error("falcon_state_diff: the two handles have different layouts");
+        HIPCHECK(hipStreamSynchronize(b->stream));
+        if (!a->diffbuf.p) a->diffbuf.alloc(32 * sizeof(unsigned long long));
+        unsigned long long init[32];
+        for (int q = 0; q < 32; q++) init[q] = q < 24 ? 0ull : ~0ull;
+        unsigned long long *w = a->diffbuf.as<unsigned long long>();
+        HIPCHECK(hipMemcpyAsync(w, init, sizeof(init), hipMemcpyHostToDevice, a->stream));
+        const double *pa = a->U[a->cur].as<double>(), *pb = b->U[b->cur].as<double>();
+        launch_state_diff(d, pa, pb, rel_tol, w, 0, a->stream);
+        launch_state_diff(d, pa, pb, rel_tol, w, 1, a->stream);
+        check_launch();
+        unsigned long long out[32];
+        HIPCHECK(hipMemcpyAsync(out, w, sizeof(out), hipMemcpyDeviceToHost, a->stream));
+        HIPCHECK(hipStreamSynchronize(a->stream));
+        for (int v = 0; v < 8; v++) {
+            double x, y;
+            std::memcpy(&x, &out[v], 8);
+            std::memcpy(&y, &out[8 + v], 8);
+            if (max_diff) max_diff[v] = x;
+            if (max_ref) max_ref[v] = y;
+            if (n_above) n_above[v] = (int64_t)out[16 + v];
+            if (argmax) argmax[v] = out[24 + v] == ~0ull ? -1 : (int64_t)out[24 + v];
+        }
     });
 }
 
@@ -1521,6 +1636,10 @@ int falcon_cnn_set_loss(falcon_handle_t h, double h1_alpha, double w_r, double w
 
 int falcon_heat_set_learned_iters(falcon_handle_t h, int n) {
     return guard([&] { need_heat(h).set_learned_iters(n); });
+}
+
+int falcon_heat_set_coarse(falcon_handle_t h, double tol, int max_iter) {
+    return guard([&] { need_heat(h).set_coarse(tol, max_iter); });
 }
 
 int falcon_heat_set_learned_sweeps(falcon_handle_t h, int pre0, int post0) {

@@ -8,10 +8,13 @@
 // Laplacian (3_5/ModPFSS.f90:68-84), kappa(r) = kappa0 (r/r_ref)^-3, insulated
 // radial boundaries.  Multigrid levels per block m = 16, 8, 4, 2, 1 with face
 // ghosts at every level (the HC exchange of 3_5/ModCommunication.f90:1497-1639,
-// here table-driven), summed restriction of the residual, injection
-// prolongation, damped Jacobi (ModMultigrid_Jacobi_Mz_r, 3_5/ModMultigrid.f90:304-325,
-// with damping).  Krylov: flexible CG (FCG(1)), everything FP64; dot products
-// are deterministic two-pass reductions (wavefront shuffle, LDS, second pass).
+// here table-driven), summed restriction of the residual, linear
+// prolongation from the coarse cell and its face neighbours, damped Jacobi
+// (ModMultigrid_Jacobi_Mz_r, 3_5/ModMultigrid.f90:304-325, with damping), and
+// on the 1-cell level a Jacobi-preconditioned CG over every block of a system
+// (ModLinearSolver_JCB_CG, 3_5/ModLinearSolver.f90:86-180, as a method).
+// Krylov: flexible CG (FCG(1)), everything FP64; dot products are
+// deterministic two-pass reductions (wavefront shuffle, LDS, second pass).
 //
 // Layout: every vector holds interior cells only, [system][block][i][j][k].
 // Ghost values are never stored: a stencil kernel evaluates the face ghost of
@@ -19,8 +22,8 @@
 // so a Jacobi sweep is one launch and reads each vector once.  Coefficients
 // are [block][i][j] tables (independent of phi).  Fusions: the first sweep
 // from zero is omega r/diag; the residual is restricted in the kernel that
-// forms it; the first post-smoothing sweep reads x + P(coarse) on the fly;
-// dot products are accumulated by the kernels producing their operands.
+// forms it; dot products are accumulated by the kernels producing their
+// operands.
 //
 // All kernels carry a system index in blockIdx.y: nsys independent systems
 // (same grid, own dt and right-hand side) are solved in the same launches, and
@@ -75,24 +78,10 @@ __device__ __forceinline__ Cell cell_of(const HLev &L, long long c) {
     return r;
 }
 
-// interior index of the coarse parent of fine cell c (edge 2^lm -> 2^(lm-1))
-__device__ __forceinline__ long long parent_of(int lm, long long c) {
-    const int m = 1 << lm, lc = lm - 1;
-    const int b = (int)(c >> (3 * lm));
-    const int q = (int)(c & ((1LL << (3 * lm)) - 1));
-    const int i = q >> (2 * lm), j = (q >> lm) & (m - 1), k = q & (m - 1);
-    return ((long long)b << (3 * lc)) + ((i >> 1) << (2 * lc)) + ((j >> 1) << lc) + (k >> 1);
-}
-
-// the field a stencil reads: x, or x + P xc (injection of the coarse level)
-template <bool PROL>
+// the field a stencil reads
 struct Field {
-    const double *x, *xc;
-    int lm;
-    __device__ __forceinline__ double operator()(long long c) const {
-        if (PROL) return x[c] + xc[parent_of(lm, c)];
-        return x[c];
-    }
+    const double *x;
+    __device__ __forceinline__ double operator()(long long c) const { return x[c]; }
 };
 
 __device__ __forceinline__ double diag_at(const HLev &L, const Cell &q, double inv_dt) {
@@ -139,8 +128,7 @@ __device__ __forceinline__ int xcd_group(int w, int nwg) {
 
 // trilinear inter/extrapolation inside the donor block from base cell c, in
 // the term order of ModMath.f90:283-291 (weights may lie in [-0.5, 1.5]); f0 = X(c)
-template <bool PROL>
-__device__ __forceinline__ double trilinear(const HLev &L, const Field<PROL> &X, long long c, int wid, double f0) {
+__device__ __forceinline__ double trilinear(const HLev &L, const Field &X, long long c, int wid, double f0) {
     const double wi = L.gw[3 * wid], wj = L.gw[3 * wid + 1], wk = L.gw[3 * wid + 2];
     const long long S1 = L.m, SI = (long long)L.m * L.m;
     const double f1 = X(c + SI), f2 = X(c + S1), f3 = X(c + 1), f4 = X(c + SI + S1), f5 = X(c + S1 + 1),
@@ -235,8 +223,8 @@ __device__ __forceinline__ double dpp_d(double v) {
 
 // phases 1-2: col = the thread's x column, a0 / a1 its columns of in0 / in1,
 // sm = tile + ghosts + coefficients.  Ends with a barrier.
-template <int LM, bool PROL, bool IN0, bool IN1>
-__device__ __forceinline__ void load_tile(const HLev &L, const Field<PROL> &F, int b0, bool own, long long cb,
+template <int LM, bool IN0, bool IN1>
+__device__ __forceinline__ void load_tile(const HLev &L, const Field &F, int b0, bool own, long long cb,
                                           const double *__restrict__ in0, const double *__restrict__ in1, long long so,
                                           double (&col)[1 << LM], double (&a0)[1 << LM], double (&a1)[1 << LM],
                                           double *__restrict__ sm) {
@@ -342,11 +330,11 @@ __device__ __forceinline__ double tile_apply(const double *__restrict__ sm, cons
 }
 
 // One operator pass over every cell of a level: op.cell at each cell, dot
-// products per workgroup into partial (when given).  PROL: the field is x + P xc.
-template <int LM, bool PROL, class OP>
-__global__ void __launch_bounds__(256) k_hmarch(HLev L, const double *__restrict__ x, const double *__restrict__ xc,
-                                                const double *__restrict__ inv_dt, const int *__restrict__ active,
-                                                long long istride, long long cstride, OP op, double *__restrict__ partial) {
+// products per workgroup into partial (when given).
+template <int LM, class OP>
+__global__ void __launch_bounds__(256) k_hmarch(HLev L, const double *__restrict__ x, const double *__restrict__ inv_dt,
+                                                const int *__restrict__ active, long long istride, OP op,
+                                                double *__restrict__ partial) {
     constexpr int m = 1 << LM, mm = m * m, bpw = 256 >> (2 * LM);
     __shared__ double sm[TileLds<LM>::SIZE];
     const int s = blockIdx.y;
@@ -355,11 +343,11 @@ __global__ void __launch_bounds__(256) k_hmarch(HLev L, const double *__restrict
     const int lb = tid >> (2 * LM), j = (tid >> LM) & (m - 1), k = tid & (m - 1);
     const int b0 = xcd_group(blockIdx.x, gridDim.x) * bpw, b = b0 + lb;
     const bool own = b < L.nb;
-    const Field<PROL> F{x + s * istride, PROL ? xc + s * cstride : nullptr, LM};
+    const Field F{x + s * istride};
     const long long so = s * istride;
     const long long cb = ((long long)b << (3 * LM)) + (j << LM) + k;
     double col[m], a0[m], a1[m];
-    load_tile<LM, PROL, OP::IN0, OP::IN1>(L, F, b0, own, cb, op.in0, op.in1, so, col, a0, a1, sm);
+    load_tile<LM, OP::IN0, OP::IN1>(L, F, b0, own, cb, op.in0, op.in1, so, col, a0, a1, sm);
     double v[OP::NP] = {};
     const double idt = inv_dt[s];
 #pragma unroll
@@ -389,11 +377,11 @@ __global__ void __launch_bounds__(256) k_hresrestrict(HLev L, const double *__re
     const int lb = tid >> (2 * LM), j = (tid >> LM) & (m - 1), k = tid & (m - 1);
     const int b0 = xcd_group(blockIdx.x, gridDim.x) * bpw, b = b0 + lb;
     const bool own = b < L.nb;
-    const Field<false> F{x + s * istride, nullptr, LM};
+    const Field F{x + s * istride};
     const long long so = s * istride;
     const long long cb = ((long long)b << (3 * LM)) + (j << LM) + k;
     double col[m], a0[m], a1[m];
-    load_tile<LM, false, true, false>(L, F, b0, own, cb, r, nullptr, so, col, a0, a1, sm);
+    load_tile<LM, true, false>(L, F, b0, own, cb, r, nullptr, so, col, a0, a1, sm);
     const double idt = inv_dt[s];
     const bool out = own && !(j & 1) && !(k & 1);
     double *R = rc + s * cstride + ((long long)b << (3 * LC)) + ((j >> 1) << LC) + (k >> 1);
@@ -425,14 +413,138 @@ __global__ void __launch_bounds__(256) k_hjacobi0(HLev L, const double *__restri
     y[io] = (omega * r[io]) / diag_at(L, cell_of(L, c), inv_dt[s]);
 }
 
-// prolongation by injection, stored: fine x += coarse x of the parent cell
-__global__ void __launch_bounds__(256) k_hprolong(HLev L, double *__restrict__ zf, const double *__restrict__ zc,
-                                                  const int *__restrict__ active, long long fstride, long long cstride) {
+// Linear prolongation, stored: fine cell (i, j, k) of level LM adds
+//   z0 + (zi - z0)/4 + (zj - z0)/4 + (zk - z0)/4
+// with z0 its parent on the coarse level and zi, zj, zk the parent's face
+// neighbours on the fine cell's side in each axis (cell-centred linear
+// interpolation from face neighbours only, exact for fields linear in the
+// block's index space).  A neighbour outside the coarse block is the coarse
+// level's face ghost (copy or trilinear from the donor table); at an insulated
+// radial face (no donor) it is z0 (zero gradient).
+template <int LM>
+__global__ void __launch_bounds__(256) k_hprolong_lin(HLev Lc, double *__restrict__ zf, const double *__restrict__ zc,
+                                                      const int *__restrict__ active, long long fstride,
+                                                      long long cstride) {
+    constexpr int m = 1 << LM, LC = LM - 1, mc = 1 << LC, mmc = mc * mc;
     const int s = blockIdx.y;
     if (!active[s]) return;
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= (long long)L.nb * L.M3) return;
-    zf[s * fstride + c] += zc[s * cstride + parent_of(L.lm, c)];
+    if (c >= ((long long)Lc.nb << (3 * LM))) return;
+    const int b = (int)(c >> (3 * LM));
+    const int q = (int)(c & ((1 << (3 * LM)) - 1));
+    const int i = q >> (2 * LM), j = (q >> LM) & (m - 1), k = q & (m - 1);
+    const int ic = i >> 1, jc = j >> 1, kc = k >> 1;
+    const int si = (i & 1) ? 1 : -1, sj = (j & 1) ? 1 : -1, sk = (k & 1) ? 1 : -1;
+    const double *Z = zc + s * cstride;
+    const Field F{Z};
+    const long long cb = ((long long)b << (3 * LC)) + (ic << (2 * LC)) + (jc << LC) + kc;
+    const double z0 = Z[cb];
+    const int2 *E = Lc.gsw + (long long)b * 6 * mmc;
+    auto ghost = [&](int face, int idx) -> double {
+        const int2 e = E[face * mmc + idx];
+        if (e.x < 0) return z0;
+        const double f0 = Z[e.x];
+        return e.y < 0 ? f0 : trilinear(Lc, F, e.x, e.y, f0);
+    };
+    const int in = ic + si, jn = jc + sj, kn = kc + sk;
+    const double zi = (in >= 0 && in < mc) ? Z[cb + si * mmc] : ghost(si < 0 ? 0 : 1, jc * mc + kc);
+    const double zj = (jn >= 0 && jn < mc) ? Z[cb + sj * mc] : ghost(sj < 0 ? 2 : 3, ic * mc + kc);
+    const double zk = (kn >= 0 && kn < mc) ? Z[cb + sk] : ghost(sk < 0 ? 4 : 5, ic * mc + jc);
+    double v = z0 + 0.25 * (zi - z0);
+    v = v + 0.25 * (zj - z0);
+    v = v + 0.25 * (zk - z0);
+    const long long io = s * fstride + c;
+    zf[io] = zf[io] + v;
+}
+
+// Sum over one workgroup of 1024 threads: every wavefront reduces by halving
+// shifts, then thread 0 adds the 16 wavefront sums in order; all threads get
+// the result.
+__device__ __forceinline__ double wg1024_sum(double a, double *red) {
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = red[0];
+        for (int w = 1; w < 16; w++) t = t + red[w];
+        red[16] = t;
+    }
+    __syncthreads();
+    return red[16];
+}
+
+// Coarsest level (one cell per block): Jacobi-preconditioned CG from x = 0,
+// ModLinearSolver_JCB_CG (3_5/ModLinearSolver.f90:86-180) as a method, with
+// the Jacobi diagonal as preconditioner and a relative stop r.z <= tol2 (r.z)_0
+// or max_iter iterations.  One workgroup per system holds every row
+// (thread t: blocks t, t + 1024, ...); r, p and A p live in work[s][3][nb].
+// A p per row is the level's operator with face ghosts copied from the donor
+// blocks' single cells (0 at insulated faces, whose coefficient is 0).
+__global__ void __launch_bounds__(1024) k_hcoarse_cg(HLev L, const double *__restrict__ rhs, double *__restrict__ x,
+                                                     double *__restrict__ work, const double *__restrict__ inv_dt,
+                                                     const int *__restrict__ active, long long stride, double tol2,
+                                                     int max_iter) {
+    __shared__ double red[17];
+    const int s = blockIdx.x;
+    if (!active[s]) return;
+    const int nb = L.nb, t = threadIdx.x;
+    const double idt = inv_dt[s];
+    double *X = x + s * stride;
+    const double *B = rhs + s * stride;
+    double *R = work + (long long)s * 3 * nb, *Pv = R + nb, *AP = Pv + nb;
+    double acc = 0.0;
+    for (int b = t; b < nb; b += 1024) {
+        const double dg = L.V[b] * idt + L.off[b];
+        const double r = B[b];
+        R[b] = r;
+        const double z = r / dg;
+        Pv[b] = z;
+        X[b] = 0.0;
+        acc += r * z;
+    }
+    double rz = wg1024_sum(acc, red);
+    const double stop = tol2 * rz;
+    for (int it = 0; it < max_iter && rz > stop; it++) {
+        acc = 0.0;
+        for (int b = t; b < nb; b += 1024) {
+            const int2 *E = L.gsw + (long long)b * 6;
+            double g[6];
+#pragma unroll
+            for (int f = 0; f < 6; f++) {
+                const int2 e = E[f];
+                g[f] = e.x >= 0 ? Pv[e.x] : 0.0;
+            }
+            const double cim = L.Ai[2 * b], cip = L.Ai[2 * b + 1], cjm = L.Aj[2 * b], cjp = L.Aj[2 * b + 1];
+            const double ck = L.Ak[b];
+            const double diag = L.V[b] * idt + L.off[b];
+            double nbv = cim * g[0] + cip * g[1];
+            nbv = nbv + cjm * g[2] + cjp * g[3];
+            nbv = nbv + ck * g[4] + ck * g[5];
+            const double ap = diag * Pv[b] - nbv;
+            AP[b] = ap;
+            acc += Pv[b] * ap;
+        }
+        const double pAp = wg1024_sum(acc, red);
+        if (!(pAp > 0.0)) break;
+        const double alpha = rz / pAp;
+        acc = 0.0;
+        for (int b = t; b < nb; b += 1024) {
+            const double dg = L.V[b] * idt + L.off[b];
+            X[b] = X[b] + alpha * Pv[b];
+            const double r = R[b] - alpha * AP[b];
+            R[b] = r;
+            acc += r * (r / dg);
+        }
+        const double rzn = wg1024_sum(acc, red);
+        const double beta = rzn / rz;
+        rz = rzn;
+        for (int b = t; b < nb; b += 1024) {
+            const double dg = L.V[b] * idt + L.off[b];
+            Pv[b] = R[b] / dg + beta * Pv[b];
+        }
+        __syncthreads();   // every p before the next product reads its neighbours
+    }
 }
 
 __global__ void __launch_bounds__(256) k_hzero(double *__restrict__ a, long long n) {
@@ -728,7 +840,11 @@ struct HeatSolver::Impl {
         hipLaunchKernelGGL(k_hscalars, dim3((nsys + 63) / 64), dim3(64), 0, stream, what, nsys, (const double *)dots, sc,
                            (const int *)active, ncell);
     }
-    int pre = 2, post = 2, coarse_sweeps = 2;
+    int pre = 2, post = 2;
+    // coarsest level: Jacobi-preconditioned CG to r.z <= coarse_tol^2 (r.z)_0
+    double coarse_tol = 1.0e-2;
+    int coarse_max = 50;
+    double *coarse_work = nullptr;   // [system][r, p, Ap][nb]
     double omega = 0.8;
     std::vector<double> dt;
     double dt_expl = 0.0;
@@ -763,19 +879,19 @@ struct HeatSolver::Impl {
         std::vector<int> one(nsys, 1);
         HCHECK(hipMemcpy(active, one.data(), sizeof(int) * nsys, hipMemcpyHostToDevice));
     }
-    template <bool PROL, class OP>
-    void launch_march(int l, const double *x_, const double *xc, long long cs, const OP &op, double *part) {
+    template <class OP>
+    void launch_march(int l, const double *x_, const OP &op, double *part) {
         const dim3 g(march_wgs(l), (unsigned)nsys);
         const HLev &D = L[l];
         const double *idt = inv_dt;
         const int *ac = active;
         const long long is = ist(l);
         switch (D.lm) {
-            case 0: hipLaunchKernelGGL((k_hmarch<0, PROL, OP>), g, dim3(256), 0, stream, D, x_, xc, idt, ac, is, cs, op, part); break;
-            case 1: hipLaunchKernelGGL((k_hmarch<1, PROL, OP>), g, dim3(256), 0, stream, D, x_, xc, idt, ac, is, cs, op, part); break;
-            case 2: hipLaunchKernelGGL((k_hmarch<2, PROL, OP>), g, dim3(256), 0, stream, D, x_, xc, idt, ac, is, cs, op, part); break;
-            case 3: hipLaunchKernelGGL((k_hmarch<3, PROL, OP>), g, dim3(256), 0, stream, D, x_, xc, idt, ac, is, cs, op, part); break;
-            case 4: hipLaunchKernelGGL((k_hmarch<4, PROL, OP>), g, dim3(256), 0, stream, D, x_, xc, idt, ac, is, cs, op, part); break;
+            case 0: hipLaunchKernelGGL((k_hmarch<0, OP>), g, dim3(256), 0, stream, D, x_, idt, ac, is, op, part); break;
+            case 1: hipLaunchKernelGGL((k_hmarch<1, OP>), g, dim3(256), 0, stream, D, x_, idt, ac, is, op, part); break;
+            case 2: hipLaunchKernelGGL((k_hmarch<2, OP>), g, dim3(256), 0, stream, D, x_, idt, ac, is, op, part); break;
+            case 3: hipLaunchKernelGGL((k_hmarch<3, OP>), g, dim3(256), 0, stream, D, x_, idt, ac, is, op, part); break;
+            case 4: hipLaunchKernelGGL((k_hmarch<4, OP>), g, dim3(256), 0, stream, D, x_, idt, ac, is, op, part); break;
             default: throw std::runtime_error("heat blocks are at most 16^3");
         }
     }
@@ -795,23 +911,14 @@ struct HeatSolver::Impl {
     }
     // y = A x (and per-workgroup partial sums of x.y, x.r when r is given)
     void apply0(const double *x_, double *y, const double *r_dot) {
-        if (r_dot) launch_march<false>(0, x_, nullptr, 0, OpApply<true>{y, r_dot, nullptr}, partial);
-        else launch_march<false>(0, x_, nullptr, 0, OpApply<false>{y, nullptr, nullptr}, nullptr);
+        if (r_dot) launch_march(0, x_, OpApply<true>{y, r_dot, nullptr}, partial);
+        else launch_march(0, x_, OpApply<false>{y, nullptr, nullptr}, nullptr);
     }
-    // one damped Jacobi sweep Z[l] -> Z2[l] (swapped); prol: the field is
-    // Z[l] + P Z[l+1]; dq: accumulate <new z, dq> (level 0)
-    void sweep(int l, bool prol, const double *dq) {
-        const double *zc = Z[l + 1 < nlev ? l + 1 : l];
-        const long long cs = ist(l + 1 < nlev ? l + 1 : l);
-        if (dq) {
-            const OpJacobi<true> op{Z2[l], R[l], dq, omega};
-            if (prol) launch_march<true>(l, Z[l], zc, cs, op, partial);
-            else launch_march<false>(l, Z[l], nullptr, 0, op, partial);
-        } else {
-            const OpJacobi<false> op{Z2[l], R[l], nullptr, omega};
-            if (prol) launch_march<true>(l, Z[l], zc, cs, op, nullptr);
-            else launch_march<false>(l, Z[l], nullptr, 0, op, nullptr);
-        }
+    // one damped Jacobi sweep Z[l] -> Z2[l] (swapped); dq: accumulate
+    // <new z, dq> (level 0)
+    void sweep(int l, const double *dq) {
+        if (dq) launch_march(l, Z[l], OpJacobi<true>{Z2[l], R[l], dq, omega}, partial);
+        else launch_march(l, Z[l], OpJacobi<false>{Z2[l], R[l], nullptr, omega}, nullptr);
         std::swap(Z[l], Z2[l]);
     }
     // first sweep from zero: Z[l] = omega R[l] / diag
@@ -820,33 +927,56 @@ struct HeatSolver::Impl {
                            (const double *)inv_dt, (const int *)active, ist(l), omega);
     }
 
+    // zf (level l) += linear prolongation of level l+1
+    void prolong(int l) {
+        const HLev &Lc = L[l + 1];
+        const dim3 g = grid_cells(l);
+        double *zf = Z[l];
+        const double *zc = Z[l + 1];
+        const int *ac = active;
+        const long long fs = ist(l), cs = ist(l + 1);
+        switch (L[l].lm) {
+            case 1: hipLaunchKernelGGL(k_hprolong_lin<1>, g, dim3(256), 0, stream, Lc, zf, zc, ac, fs, cs); break;
+            case 2: hipLaunchKernelGGL(k_hprolong_lin<2>, g, dim3(256), 0, stream, Lc, zf, zc, ac, fs, cs); break;
+            case 3: hipLaunchKernelGGL(k_hprolong_lin<3>, g, dim3(256), 0, stream, Lc, zf, zc, ac, fs, cs); break;
+            case 4: hipLaunchKernelGGL(k_hprolong_lin<4>, g, dim3(256), 0, stream, Lc, zf, zc, ac, fs, cs); break;
+            default: throw std::runtime_error("no prolongation onto 1-cell blocks");
+        }
+    }
+    // coarsest level: Z[l] = CG solve of its operator with right-hand side R[l]
+    void coarse_solve(int l) {
+        hipLaunchKernelGGL(k_hcoarse_cg, dim3((unsigned)nsys), dim3(1024), 0, stream, L[l], (const double *)R[l], Z[l],
+                           coarse_work, (const double *)inv_dt, (const int *)active, ist(l),
+                           coarse_tol * coarse_tol, coarse_max);
+    }
+
     // V-cycle: z0 (level 0) = M r0.  pre0 / post0: smoothing sweeps on the
-    // finest level (others use pre / post).  dq: accumulate <z0, dq> into the
-    // partial sums; returns the number of workgroup partials to sum (0 without dq).
+    // finest level (others use pre / post); the coarsest level is solved by
+    // CG.  dq: accumulate <z0, dq> into the partial sums; returns the number of
+    // workgroup partials to sum (0 without dq).
     unsigned vcycle(const double *r0, double *&z0, int pre0, int post0, const double *dq) {
         R[0] = const_cast<double *>(r0);
         for (int l = 0; l < nlev; l++) {
-            const int sweeps = (l == nlev - 1) ? coarse_sweeps : (l == 0 ? pre0 : pre);
+            if (l == nlev - 1) {
+                coarse_solve(l);
+                break;
+            }
+            const int sweeps = l == 0 ? pre0 : pre;
             if (sweeps == 0) {
                 zero(Z[l], ist(l) * nsys);
             } else {
                 sweep0(l);
-                for (int s = 1; s < sweeps; s++) sweep(l, false, nullptr);
+                for (int s = 1; s < sweeps; s++) sweep(l, nullptr);
             }
-            if (l == nlev - 1) break;
             restrict_residual(l);
         }
         unsigned nwg = 0;
         for (int l = nlev - 2; l >= 0; l--) {
+            prolong(l);
             const int sweeps = l == 0 ? post0 : post;
-            if (sweeps == 0) {
-                hipLaunchKernelGGL(k_hprolong, grid_cells(l), dim3(256), 0, stream, L[l], Z[l], (const double *)Z[l + 1],
-                                   (const int *)active, ist(l), ist(l + 1));
-                continue;
-            }
             for (int s = 0; s < sweeps; s++) {
                 const bool last0 = l == 0 && s == sweeps - 1 && dq;
-                sweep(l, s == 0, last0 ? dq : nullptr);
+                sweep(l, last0 ? dq : nullptr);
                 if (last0) nwg = march_wgs(0);
             }
         }
@@ -915,6 +1045,14 @@ void HeatSolver::setup(const Layout &lay, const HeatParams &hp, int nsys, int nt
         I.Z2[l] = I.dalloc(I.ist(l) * nsys);
         I.R[l] = l ? I.dalloc(I.ist(l) * nsys) : nullptr;
     }
+    {
+        // the coarsest level's face ghosts are copies of the donors' single cells
+        const HeatLevel &H = I.host[I.nlev - 1];
+        if (H.m != 1) throw std::runtime_error("the coarsest heat level must have one cell per block");
+        for (int32_t w : H.g_wid)
+            if (w >= 0) throw std::runtime_error("weighted face ghost on the 1-cell level");
+    }
+    I.coarse_work = I.dalloc((size_t)nsys * 3 * I.nb);
     I.x = I.dalloc(I.ist(0) * nsys);
     I.r = I.dalloc(I.ist(0) * nsys);
     I.q = I.dalloc(I.ist(0) * nsys);
@@ -956,6 +1094,12 @@ void HeatSolver::set_learned_blockstep(bool on) {
 void HeatSolver::set_learned_iters(int n) {
     if (n < 0) throw std::runtime_error("the number of learned directions must be >= 0");
     d->learned_iters = n;
+}
+void HeatSolver::set_coarse(double tol, int max_iter) {
+    if (!(tol >= 0.0) || max_iter < 0) throw std::runtime_error("coarse CG: tol >= 0 and max_iter >= 0");
+    d->coarse_tol = tol;
+    d->coarse_max = max_iter;
+    d->drop_graphs();
 }
 void HeatSolver::set_learned_sweeps(int pre0, int post0) {
     if (pre0 < 0 || post0 < 0 || pre0 > 8 || post0 > 8) throw std::runtime_error("sweeps must be in [0, 8]");
@@ -1028,7 +1172,7 @@ void HeatSolver::Impl::ensure_learned() {
 // r' = r - A z, network input r'/diag and its per-system scale (root mean
 // square) in sc[3]
 void HeatSolver::Impl::features(double *z) {
-    launch_march<false>(0, z, nullptr, 0, OpFeat{rp, xin, r, nullptr}, partial);
+    launch_march(0, z, OpFeat{rp, xin, r, nullptr}, partial);
     final_dots(1, march_wgs(0));
     scalars(3, (double)ist(0));
 }
@@ -1037,7 +1181,7 @@ void HeatSolver::Impl::features(double *z) {
 // unless first, beta = <z + a c, q>/gamma of the direction update (sc[1])
 void HeatSolver::Impl::learned_correction(bool first) {
     corr->infer_heat(xin, sc + 3, kSc, nb, ist(0), cc, (long long)nsys * nb, active, stream);
-    launch_march<false>(0, cc, nullptr, 0, OpApplyDot{rp, q}, partial);
+    launch_march(0, cc, OpApplyDot{rp, q}, partial);
     if (blockstep) {
         const unsigned nwg = march_wgs(0);
         hipLaunchKernelGGL(k_hblockstep, dim3((nwg + 255) / 256, (unsigned)nsys), dim3(256), 0, stream, (int)nwg, nb,

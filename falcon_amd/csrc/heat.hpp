@@ -74,6 +74,9 @@ class HeatSolver {
     // finest-level smoothing sweeps of the V-cycle used with the learned
     // correction (recording and learned solves); the plain method keeps 2 / 2
     void set_learned_sweeps(int pre0, int post0);
+    // coarsest-level CG of the V-cycle: stop at r.z <= tol^2 (r.z)_0 or after
+    // max_iter iterations (defaults 1e-2, 50)
+    void set_coarse(double tol, int max_iter);
     // plain and learned solves replay one captured FCG iteration (default) or
     // launch the same kernels eagerly
     void set_graph(bool on);

@@ -407,6 +407,11 @@ __global__ void __launch_bounds__(TL * TL) k_stage(Dims d, DevTables t, const TI
 
 // ---------------------------------------------------------------- diffusion
 
+// Radial mirror parities of ModBoundary_Dynamo_primitives (3_6/ModBoundary.f90:
+// 16-52) for rho1, vr, vt, vp, br, bt, bp, s1: bottom (inner) and top (outer).
+__constant__ double kParBottom[8] = {1.0, -1.0, 1.0, 1.0, -1.0, 1.0, 1.0, 1.0};
+__constant__ double kParTop[8] = {1.0, -1.0, 1.0, 1.0, 1.0, -1.0, -1.0, 1.0};
+
 // ModDiffusion_Aritificial_1 (3_6/ModDiffusion.f90:9-150), h = 2, all 8 vars.
 // The characteristic speed c of each face is read where the reference reads it:
 // the section c(1:ni+1,:,:) (:63) spans the padded transverse range while flux
@@ -416,11 +421,21 @@ __global__ void __launch_bounds__(TL * TL) k_stage(Dims d, DevTables t, const TI
 // null) receives the same value in FP32, and the workgroup's min(h/v_wave) of the
 // new state (the next step's ModTimeStep_Dynamo input) goes to partial[tile].
 // MODE 1: write R.  `in` is the stencil input (the state or its FP32 shadow).
+// FP32 path (T = float, MODE 0): U and out are the same buffer, updated in
+// place, and Uold is U^n.  The radial ghosts of the input are the mirror of
+// U^n that the first stage's boundary call left (the reference applies
+// boundaries only to each stage's input), while the interior is the state
+// after the fourth stage; at the domain's radial boundary faces the limiter
+// therefore divides by the one-step change of the first cell, which FP32
+// cannot resolve.  There the face flux is formed in FP64 from U^n (mirrored
+// with the parities of ModBoundary_Dynamo_primitives, 3_6/ModBoundary.f90:
+// 16-52) and the FP64 state, as the FP64 path forms it.
+
 template <typename T, typename TIN, int MODE, int TL, bool PERSIST>
 __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, const TIN *__restrict__ in,
-                                                   const double *__restrict__ U, double *__restrict__ out,
-                                                   float *__restrict__ shadow, double *__restrict__ partial,
-                                                   const double *__restrict__ dtp, const int32_t *__restrict__ pull) {
+                                                   const double *U, double *out, float *__restrict__ shadow,
+                                                   double *__restrict__ partial, const double *__restrict__ dtp,
+                                                   const int32_t *__restrict__ pull, const double *__restrict__ Uold) {
     // one r-plane with its theta/phi halo; the second barrier of each plane
     // orders its reuse, so it is not double-buffered
     constexpr int PL = tile_pl<TL>();
@@ -475,6 +490,11 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
         // faces between neighbouring lanes, which go through fj / fk.
         T fr[8];
         bool have_fr = false;
+        // FP32 path: U before the diffusion on the previous plane (the top
+        // boundary face reads it after that plane has been updated in place)
+        constexpr bool BFACE = MODE == 0 && sizeof(T) == 4;
+        double uprev[8];
+        const bool bot = BFACE && (mt.flags & 1), topb = BFACE && (mt.flags & 2);
     #pragma unroll
         for (int v = 0; v < 8; v++) {
     #pragma unroll
@@ -577,9 +597,23 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
     #pragma unroll
                 for (int v = 0; v < 8; v++) {
                     T r = T(0);
-                    const T f0 = have_fr ? fr[v]
-                                         : (face_flux(win[v][0], win[v][1], win[v][2], win[v][3]) * (c10 + c1m)) * T(0.5);
-                    const T f1 = (face_flux(win[v][1], win[v][2], win[v][3], win[v][4]) * (c1p + c10)) * T(0.5);
+                    T f0 = have_fr ? fr[v]
+                                   : (face_flux(win[v][0], win[v][1], win[v][2], win[v][3]) * (c10 + c1m)) * T(0.5);
+                    T f1 = (face_flux(win[v][1], win[v][2], win[v][3], win[v][4]) * (c1p + c10)) * T(0.5);
+                    if (BFACE && bot && i == 0) {
+                        // lower face of the first cell: ghosts 0 and -1 mirror cells 1 and 2 of U^n
+                        const long long q = base + (long long)v * d.P;
+                        const double p = kParBottom[v];
+                        const double g0 = p * Uold[q], gm1 = p * Uold[q + m.SI];
+                        f0 = (T)((face_flux<double>(gm1, g0, ucur[v], U[q + m.SI]) * (double)(c10 + c1m)) * 0.5);
+                    }
+                    if (BFACE && topb && i == d.ni - 1) {
+                        // upper face of the last cell: ghosts ni+1 and ni+2 mirror cells ni and ni-1 of U^n
+                        const long long q = base + (long long)v * d.P;
+                        const double p = kParTop[v];
+                        const double g1 = p * Uold[q], g2 = p * Uold[q - m.SI];
+                        f1 = (T)((face_flux<double>(uprev[v], ucur[v], g1, g2) * (double)(c1p + c10)) * 0.5);
+                    }
                     fr[v] = f1;
                     r = r + (f0 - f1) / dxi;
                     r = r + (fj[v][ty][tx] - fj[v][ty + 1][tx]) / L2;
@@ -615,6 +649,7 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
             for (int v = 0; v < 8; v++) {
                 win[v][0] = win[v][1]; win[v][1] = win[v][2]; win[v][2] = win[v][3]; win[v][3] = win[v][4];
                 win[v][4] = nxt[v];
+                if (BFACE) uprev[v] = ucur[v];
             }
         }
         if (MODE == 0 && partial) {
@@ -888,6 +923,74 @@ __global__ void k_to_f32(const double *__restrict__ s, float *__restrict__ d, lo
     for (; i < n; i += stride) d[i] = (float)s[i];
 }
 
+__global__ void k_set_scalar(double *__restrict__ p, double v) {
+    if (threadIdx.x == 0) p[0] = v;
+}
+
+// Interior difference of two states of one layout, one workgroup per (block,
+// variable).  PASS 0: w[v] = max|a-b|, w[8+v] = max|b| (as the bit patterns of
+// non-negative doubles, which order like the values).  PASS 1: w[16+v] = cells
+// with |a-b| > tol*max|b|, w[24+v] = the smallest interior index
+// ((b*ni + i)*nj + j)*nk + k holding max|a-b|.
+template <int PASS>
+__global__ void __launch_bounds__(256) k_state_diff(Dims d, const double *__restrict__ a, const double *__restrict__ b,
+                                                    double tol, unsigned long long *__restrict__ w) {
+    const int blk = blockIdx.x >> 3, v = blockIdx.x & 7;
+    const long long n3 = (long long)d.ni * d.nj * d.nk;
+    const long long base = ((long long)blk * 8 + v) * d.P;
+    double m0 = 0.0, m1 = 0.0;
+    unsigned long long cnt = 0, first = ~0ull;
+    const double dmax = PASS == 1 ? __longlong_as_double((long long)w[v]) : 0.0;
+    const double lim = PASS == 1 ? tol * __longlong_as_double((long long)w[8 + v]) : 0.0;
+    for (long long q = threadIdx.x; q < n3; q += blockDim.x) {
+        const int k = (int)(q % d.nk), j = (int)((q / d.nk) % d.nj), i = (int)(q / ((long long)d.nk * d.nj));
+        const long long o = base + ((long long)(i + d.ng) * d.NJ + (j + d.ng)) * d.NK + (k + d.ng);
+        const double x = fabs(a[o] - b[o]), y = fabs(b[o]);
+        if (PASS == 0) {
+            m0 = x > m0 ? x : m0;
+            m1 = y > m1 ? y : m1;
+        } else {
+            cnt += x > lim ? 1 : 0;
+            if (x == dmax) {
+                const unsigned long long id = (unsigned long long)((((long long)blk * d.ni + i) * d.nj + j) * d.nk + k);
+                first = id < first ? id : first;
+            }
+        }
+    }
+    if (PASS == 0) {
+        atomicMax(&w[v], (unsigned long long)__double_as_longlong(m0));
+        atomicMax(&w[8 + v], (unsigned long long)__double_as_longlong(m1));
+    } else {
+        if (cnt) atomicAdd(&w[16 + v], cnt);
+        if (first != ~0ull) atomicMin(&w[24 + v], first);
+    }
+}
+
+// ModCheck_primitive (3_6/ModCheck.f90:12-71) on a padded state of any
+// precision: w[0] += NaN count, w[1] = min over NaN cells of the reference's
+// scan position ((b*NK + K)*NJ + J)*NI*8 + I*8 + v (blocks in order, then
+// phi, theta, r, variable innermost), one workgroup per (block, variable).
+template <typename TS>
+__global__ void __launch_bounds__(256) k_nan_scan(Dims d, const TS *__restrict__ st, unsigned long long *__restrict__ w) {
+    const int blk = blockIdx.x >> 3, v = blockIdx.x & 7;
+    const TS *p = st + ((long long)blk * 8 + v) * d.P;
+    unsigned long long cnt = 0, first = ~0ull;
+    for (long long q = threadIdx.x; q < d.P; q += blockDim.x) {
+        const TS x = p[q];
+        if (x != x) {
+            const int K = (int)(q % d.NK), J = (int)((q / d.NK) % d.NJ), I = (int)(q / ((long long)d.NK * d.NJ));
+            const unsigned long long key =
+                ((((unsigned long long)blk * d.NK + K) * d.NJ + J) * d.NI + I) * 8ull + (unsigned long long)v;
+            cnt++;
+            first = key < first ? key : first;
+        }
+    }
+    if (cnt) {
+        atomicAdd(&w[0], cnt);
+        atomicMin(&w[1], first);
+    }
+}
+
 // ---------------------------------------------------------------- launchers
 
 // workgroups of a march over every block with tile edge TL
@@ -983,36 +1086,39 @@ void launch_rhs_only(const Dims &d, const DevTables &t, const TIN *in, double *R
 template <int TL, bool PERSIST, int MODE, typename T, typename TIN>
 static void diffusion_tl(const Dims &d, const DevTables &t, const TIN *in, const double *U, double *out,
                          float *shadow, double *partial, const double *dtp, hipStream_t s, const int32_t *pull,
-                         int wgs) {
+                         int wgs, const double *Uold) {
     hipLaunchKernelGGL((k_diffusion<T, TIN, MODE, TL, PERSIST>), march_grid(d, TL, wgs), dim3(TL * TL), 0, s, d, t, in,
-                       U, out, shadow, partial, dtp, pull);
+                       U, out, shadow, partial, dtp, pull, Uold);
 }
 
 template <int MODE, typename T, typename TIN>
 static void diffusion_any(const Dims &d, const DevTables &t, const TIN *in, const double *U, double *out,
                           float *shadow, double *partial, const double *dtp, hipStream_t s, const int32_t *pull,
-                          int tile, int wgs) {
+                          int tile, int wgs, const double *Uold) {
     check_tile(tile);
     if (tile == 16) {
-        if (wgs) diffusion_tl<16, true, MODE, T>(d, t, in, U, out, shadow, partial, dtp, s, pull, wgs);
-        else diffusion_tl<16, false, MODE, T>(d, t, in, U, out, shadow, partial, dtp, s, pull, wgs);
+        if (wgs) diffusion_tl<16, true, MODE, T>(d, t, in, U, out, shadow, partial, dtp, s, pull, wgs, Uold);
+        else diffusion_tl<16, false, MODE, T>(d, t, in, U, out, shadow, partial, dtp, s, pull, wgs, Uold);
     } else {
-        if (wgs) diffusion_tl<8, true, MODE, T>(d, t, in, U, out, shadow, partial, dtp, s, pull, wgs);
-        else diffusion_tl<8, false, MODE, T>(d, t, in, U, out, shadow, partial, dtp, s, pull, wgs);
+        if (wgs) diffusion_tl<8, true, MODE, T>(d, t, in, U, out, shadow, partial, dtp, s, pull, wgs, Uold);
+        else diffusion_tl<8, false, MODE, T>(d, t, in, U, out, shadow, partial, dtp, s, pull, wgs, Uold);
     }
 }
 
 template <typename T, typename TIN>
 void launch_diffusion(const Dims &d, const DevTables &t, const TIN *in, const double *U, double *out, float *shadow,
-                      double *partial, const double *dtp, hipStream_t s, const int32_t *pull, int tile, int wgs) {
-    diffusion_any<0, T>(d, t, in, U, out, shadow, partial, dtp, s, pull, tile, wgs);
+                      double *partial, const double *dtp, hipStream_t s, const int32_t *pull, int tile, int wgs,
+                      const double *Uold) {
+    if (sizeof(T) == 4 && (U != out || !Uold || d.ni < 2))
+        throw std::runtime_error("FP32 diffusion: updates the state in place, needs U^n and ni >= 2");
+    diffusion_any<0, T>(d, t, in, U, out, shadow, partial, dtp, s, pull, tile, wgs, Uold);
 }
 
 template <typename T>
 void launch_diffusion_only(const Dims &d, const DevTables &t, const double *U, double *R, hipStream_t s, int tile,
                            int wgs) {
     diffusion_any<1, T>(d, t, U, U, R, (float *)nullptr, (double *)nullptr, (const double *)nullptr, s,
-                        (const int32_t *)nullptr, tile, wgs);
+                        (const int32_t *)nullptr, tile, wgs, (const double *)nullptr);
 }
 
 template <typename TS>
@@ -1065,6 +1171,24 @@ void launch_dt_final(const Dims &d, const double *partial, double *dtp, double c
     hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, s, partial, (int)tiles(d, tile), cfl, dtp);
 }
 
+void launch_set_scalar(double *p, double v, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(64), 0, s, p, v);
+}
+
+void launch_state_diff(const Dims &d, const double *a, const double *b, double tol, unsigned long long *w, int pass,
+                       hipStream_t s) {
+    const dim3 grid((unsigned)d.nblocks * 8);
+    if (pass == 0) hipLaunchKernelGGL(k_state_diff<0>, grid, dim3(256), 0, s, d, a, b, tol, w);
+    else hipLaunchKernelGGL(k_state_diff<1>, grid, dim3(256), 0, s, d, a, b, tol, w);
+}
+
+template <typename TS>
+void launch_nan_scan(const Dims &d, const TS *st, unsigned long long *w, hipStream_t s) {
+    hipLaunchKernelGGL(k_nan_scan<TS>, dim3((unsigned)d.nblocks * 8), dim3(256), 0, s, d, st, w);
+}
+template void launch_nan_scan<double>(const Dims &, const double *, unsigned long long *, hipStream_t);
+template void launch_nan_scan<float>(const Dims &, const float *, unsigned long long *, hipStream_t);
+
 void launch_to_f32(const double *src, float *dst, long long n, hipStream_t s) {
     hipLaunchKernelGGL(k_to_f32, dim3(4096), dim3(256), 0, s, src, dst, n);
 }
@@ -1075,8 +1199,8 @@ template void launch_stage<float, float, float, float>(const Dims &, const DevTa
 template void launch_stage<float, float, double, double>(const Dims &, const DevTables &, const float *, const double *, double *, float *, const double *, double, hipStream_t, const int32_t *, int, int);
 template void launch_rhs_only<double, double>(const Dims &, const DevTables &, const double *, double *, hipStream_t, int, int);
 template void launch_rhs_only<float, double>(const Dims &, const DevTables &, const double *, double *, hipStream_t, int, int);
-template void launch_diffusion<double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, double *, const double *, hipStream_t, const int32_t *, int, int);
-template void launch_diffusion<float, float>(const Dims &, const DevTables &, const float *, const double *, double *, float *, double *, const double *, hipStream_t, const int32_t *, int, int);
+template void launch_diffusion<double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, double *, const double *, hipStream_t, const int32_t *, int, int, const double *);
+template void launch_diffusion<float, float>(const Dims &, const DevTables &, const float *, const double *, double *, float *, double *, const double *, hipStream_t, const int32_t *, int, int, const double *);
 template void launch_diffusion_only<double>(const Dims &, const DevTables &, const double *, double *, hipStream_t, int, int);
 template void launch_diffusion_only<float>(const Dims &, const DevTables &, const double *, double *, hipStream_t, int, int);
 template void launch_exchange<double>(const Dims &, const DevGhosts &, double *, hipStream_t);

@@ -10,9 +10,10 @@
 module ModFalconShim
     use iso_c_binding
     use falcon_c
-    use ModParameters,  only: ni, nj, nk, ng
+    use ModParameters,  only: ni, nj, nk, ng, nvar
     use ModYinYangTree, only: YYTree, TreeNode
-    use ModBlock,       only: BlockType
+    use ModBlock,       only: BlockType, GC_target
+    use ModAllocation,  only: ModAllocation_GetRank
     implicit none
     private
     public :: falcon_h, falcon_check, falcon_attach, falcon_push_state, falcon_push_pending, falcon_sync_to_host
@@ -113,8 +114,53 @@ contains
             call falcon_check(falcon_set_background(falcon_h, int(B%iBlock - 1, c_int), bg))
         end do
         call falcon_check(falcon_set_gc_all(falcon_h))
+        call falcon_host_tables(Tree)
         state_pending = .true.
     end subroutine falcon_attach
+
+    ! The host-side tables that ModGC_FindGC_Single and ModGC_GetGC_Targets_single
+    ! leave in every block (3_6/ModGC.f90:25-147), from the library's own (the
+    ! same integers): GC_iBlocks_III and GC_targets with their ghost indices and
+    ! receiver-frame cell centres.  The device exchange needs no GC_sources.
+    subroutine falcon_host_tables(Tree)
+        type(YYTree), intent(inout), target :: Tree
+        type(BlockType), pointer        :: B
+        type(GC_target), pointer        :: T
+        integer(c_int32_t), allocatable :: gcib(:)
+        integer(c_int32_t), allocatable, target :: ijk(:)
+        integer(c_int)                  :: ntg, donor, ngc
+        integer                         :: ib, it, g
+        allocate(gcib((ni + 2 * ng) * (nj + 2 * ng) * (nk + 2 * ng)))
+        do ib = 1, Tree%nLocalBlocks
+            B => Tree%LocalBlocks(ib)
+            call falcon_check(falcon_gc_iblocks(falcon_h, int(B%iBlock - 1, c_int), gcib))
+            B%GC_iBlocks_III = reshape(gcib, shape(B%GC_iBlocks_III))
+            call falcon_check(falcon_gc_ntargets(falcon_h, int(B%iBlock - 1, c_int), ntg))
+            if (allocated(B%GC_targets)) deallocate(B%GC_targets)
+            allocate(B%GC_targets(ntg))
+            B%nGC_targets = ntg
+            B%nGC_sources = 0
+            do it = 1, ntg
+                T => B%GC_targets(it)
+                call falcon_check(falcon_gc_target(falcon_h, int(B%iBlock - 1, c_int), int(it - 1, c_int), donor, &
+                                                   ngc, c_null_ptr))
+                allocate(ijk(3 * ngc))
+                call falcon_check(falcon_gc_target(falcon_h, int(B%iBlock - 1, c_int), int(it - 1, c_int), donor, &
+                                                   ngc, c_loc(ijk)))
+                T%iBlock = donor
+                T%if_yin = donor <= Tree%NumLeafNodes_YinYang(1)
+                T%iRank = ModAllocation_GetRank(donor)
+                T%nGC = ngc
+                allocate(T%ijk_list(ngc, 3), T%xijk_list(ngc, 3), T%primitive_list(ngc, nvar))
+                T%ijk_list = reshape(ijk, [int(ngc), 3])
+                do g = 1, ngc
+                    T%xijk_list(g, :) = [B%xi_I(T%ijk_list(g, 1)), B%xj_I(T%ijk_list(g, 2)), B%xk_I(T%ijk_list(g, 3))]
+                end do
+                T%primitive_list = 0.0d0
+                deallocate(ijk)
+            end do
+        end do
+    end subroutine falcon_host_tables
 
     ! Copy the host state up if it changed since falcon_attach (the driver's
     ! ModInitiation_DoAll, 3_6/test1.f90:140); called by the drop-in entry points.

@@ -43,6 +43,29 @@ module falcon_c
             type(c_ptr), value              :: h
         end function falcon_set_gc_all
 
+        integer(c_int) function falcon_gc_iblocks(h, ib, gcib) bind(c, name='falcon_gc_iblocks')
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value              :: h
+            integer(c_int), value           :: ib
+            integer(c_int32_t), intent(out) :: gcib(*)
+        end function falcon_gc_iblocks
+
+        integer(c_int) function falcon_gc_ntargets(h, ib, n) bind(c, name='falcon_gc_ntargets')
+            import :: c_int, c_ptr
+            type(c_ptr), value              :: h
+            integer(c_int), value           :: ib
+            integer(c_int), intent(out)     :: n
+        end function falcon_gc_ntargets
+
+        ! ijk: c_null_ptr to query ngc, else c_loc of 3*ngc integers
+        integer(c_int) function falcon_gc_target(h, ib, it, donor, ngc, ijk) bind(c, name='falcon_gc_target')
+            import :: c_int, c_ptr
+            type(c_ptr), value              :: h
+            integer(c_int), value           :: ib, it
+            integer(c_int), intent(out)     :: donor, ngc
+            type(c_ptr), value              :: ijk
+        end function falcon_gc_target
+
         integer(c_int) function falcon_set_state(h, ib, prim) bind(c, name='falcon_set_state')
             import :: c_int, c_double, c_ptr
             type(c_ptr), value              :: h
@@ -56,6 +79,14 @@ module falcon_c
             integer(c_int), value           :: ib
             real(c_double), intent(out)     :: prim(*)
         end function falcon_get_state
+
+        integer(c_int) function falcon_check_state(h, which, n_nan, where) bind(c, name='falcon_check_state')
+            import :: c_int, c_int32_t, c_int64_t, c_ptr
+            type(c_ptr), value              :: h
+            integer(c_int), value           :: which
+            integer(c_int64_t), intent(out) :: n_nan
+            integer(c_int32_t), intent(out) :: where(5)
+        end function falcon_check_state
 
         integer(c_int) function falcon_set_precision(h, mode) bind(c, name='falcon_set_precision')
             import :: c_int, c_ptr

@@ -130,6 +130,27 @@ int falcon_advance_rk4(falcon_handle_t h, double cfl, double *dt_out);
 /* nsteps steps without host synchronisation; use_graph captures two steps in a
  * hipGraph and replays it.  dt_last = dt of the last step. */
 int falcon_advance(falcon_handle_t h, int nsteps, double cfl, int use_graph, double *dt_last);
+/* ModCheck_primitive (3_6/ModCheck.f90:12-71) on the device: which = 0 scans
+ * the state (primitive_IV), 1 the current stage buffer (primitive_rk_IV), every
+ * padded cell and variable.  n_nan = NaN count; where[5] (optional) = block
+ * (0-based), I, J, K (padded, 0-based: index - (-ng+1)) and variable (0-based)
+ * of the first NaN in the reference's scan order (blocks in order, then phi,
+ * theta, r, variable innermost), or -1s.  The Fortran shim turns n_nan > 0
+ * into the reference's message and `stop 1` (its MPI_Abort). */
+int falcon_check_state(falcon_handle_t h, int which, int64_t *n_nan, int32_t *where);
+/* dt > 0: every later step uses this dt in place of CFL * min(h / v_wave)
+ * (ModTimeStep_Dynamo, 3_6/ModTimeStep.f90:10-30); 0 restores the CFL dt.  For
+ * stepping two precisions with one dt sequence.  No counterpart in the
+ * reference. */
+int falcon_set_fixed_dt(falcon_handle_t h, double dt);
+/* Interior difference of the current states of two handles with the same
+ * layout on one device, per variable v (rho1, vr, ..., s1): max_diff[v] =
+ * max|a - b|, max_ref[v] = max|b|, n_above[v] = cells with |a - b| >
+ * rel_tol * max|b|, argmax[v] = smallest interior index ((blk*ni + i)*nj +
+ * j)*nk + k (0-based) where |a - b| = max_diff[v].  Any output may be NULL.
+ * No counterpart in the reference (comparison of two runs). */
+int falcon_state_diff(falcon_handle_t a, falcon_handle_t b, double rel_tol, double *max_diff, double *max_ref,
+                      int64_t *n_above, int64_t *argmax);
 /* ModWaveSpeed_Dynamo + ModTimeStep_Dynamo (3_6/ModWaveSpeed.f90, ModTimeStep.f90). */
 int falcon_compute_dt(falcon_handle_t h, double cfl, double *dt);
 /* Single operators for parity: R of ModEquation_Dynamo after ModBoundary on the
@@ -186,10 +207,16 @@ int falcon_heat_set_learned_iters(falcon_handle_t h, int n);
  * of the output c leaves.  No counterpart in the reference (learned V-cycle
  * correction). */
 int falcon_cnn_set_loss(falcon_handle_t h, double h1_alpha, double w_r, double w_t, double w_p, int scale_free);
+/* Stop of the coarsest-level CG: r.z <= tol^2 (r.z)_0 or max_iter iterations
+ * (defaults 1e-2 and 50; the reference's JCB_CG stops at an absolute r.z). */
+int falcon_heat_set_coarse(falcon_handle_t h, double tol, int max_iter);
 /* y = A x for all systems (parity tests) */
 int falcon_heat_apply(falcon_handle_t h, const double *x, double *y);
 /* z = M r: one V-cycle of the preconditioner for all systems, zero initial
- * guess (ModMultigrid_Vcycle, 3_5/ModMultigrid.f90:338-376, as a method) */
+ * guess (ModMultigrid_Vcycle, 3_5/ModMultigrid.f90:338-376, as a method):
+ * damped Jacobi on the per-block levels, summed restriction, linear
+ * prolongation, and on the 1-cell-per-block level a Jacobi-preconditioned CG
+ * over the whole system (ModLinearSolver_JCB_CG, 3_5/ModLinearSolver.f90:86-180) */
 int falcon_heat_vcycle(falcon_handle_t h, const double *r, double *z);
 
 /* ---------------------------------------------------------------- learned correction

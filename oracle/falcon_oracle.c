@@ -995,7 +995,13 @@ struct orc_heat {
     int nlev, nb;
     hlev L[8];
     double dt_expl;
+    double coarse_tol;      /* coarsest-level CG: stop at r.z <= tol^2 (r.z)_0 */
+    int coarse_max;         /* ... or after this many iterations */
 };
+
+/* Coarsest-level CG defaults (the product's, heat.hip) */
+#define HEAT_COARSE_TOL 1.0e-2
+#define HEAT_COARSE_MAX 50
 
 #define HP(m, i, j, k) ((long)(i) + (long)((m) + 2) * ((long)(j) + (long)((m) + 2) * (long)(k)))  /* padded, 0..m+1 */
 
@@ -1109,6 +1115,8 @@ orc_heat *orc_heat_create(orc_ctx *c, double kappa0, double r_ref) {
     }
     H->nlev = l;
     H->dt_expl = 1.0 / worst;
+    H->coarse_tol = HEAT_COARSE_TOL;
+    H->coarse_max = HEAT_COARSE_MAX;
     return H;
 }
 
@@ -1170,6 +1178,99 @@ static void h_apply(const hlev *L, int nb, double inv_dt, const double *x, const
 }
 
 typedef struct { double *Z, *Z2, *R, *RES; } hwork;
+static void h_pad(int m, int nb, const double *in, double *out);
+
+/* Sum of per-row values in the order of the product's one-workgroup reduction
+ * (heat.hip, k_hcoarse_cg): 1024 threads, thread t adds rows t, t + 1024, ...
+ * from 0.0; each 64-lane wavefront reduces by halving shifts (lane l takes
+ * l + 32, then l + 16, ...); the 16 wavefront sums are added in order. */
+static double wg_sum(const double *v, long n) {
+    double acc[1024], red[16];
+    for (int t = 0; t < 1024; t++) {
+        double a = 0.0;
+        for (long q = t; q < n; q += 1024) a += v[q];
+        acc[t] = a;
+    }
+    for (int w = 0; w < 16; w++) {
+        double *b = acc + 64 * w;
+        for (int off = 32; off > 0; off >>= 1)
+            for (int l = 0; l < off; l++) b[l] = b[l] + b[l + off];
+        red[w] = b[0];
+    }
+    double t = red[0];
+    for (int w = 1; w < 16; w++) t = t + red[w];
+    return t;
+}
+
+/* Coarsest level (one cell per block): Jacobi-preconditioned CG from x = 0,
+ * ModLinearSolver_JCB_CG (3_5/ModLinearSolver.f90:86-180) as a method, with
+ * the Jacobi diagonal as preconditioner and a relative stop on r.z. */
+static void h_coarse_cg(orc_heat *H, hwork *W, int l, double inv_dt) {
+    const hlev *L = &H->L[l];
+    const int nb = H->nb;
+    const long P = 27;          /* (m + 2)^3 at m = 1 */
+    const long pc = HP(1, 1, 1, 1);
+    double *x = W[l].Z;
+    double *r = malloc(sizeof(double) * nb), *pp = calloc((size_t)nb * P, sizeof(double));
+    double *Ap = malloc(sizeof(double) * nb), *dg = malloc(sizeof(double) * nb), *t = malloc(sizeof(double) * nb);
+    memset(x, 0, sizeof(double) * (size_t)nb * P);
+    for (int b = 0; b < nb; b++) {
+        dg[b] = L->V[b] * inv_dt + L->off[b];
+        r[b] = W[l].R[b];
+        const double z = r[b] / dg[b];
+        pp[(long)b * P + pc] = z;
+        t[b] = r[b] * z;
+    }
+    double rz = wg_sum(t, nb);
+    const double stop = (H->coarse_tol * H->coarse_tol) * rz;
+    for (int it = 0; it < H->coarse_max && rz > stop; it++) {
+        h_exchange(L, pp);
+        h_apply(L, nb, inv_dt, pp, NULL, Ap, 0, 0.0);
+        for (int b = 0; b < nb; b++) t[b] = pp[(long)b * P + pc] * Ap[b];
+        const double pAp = wg_sum(t, nb);
+        if (!(pAp > 0.0)) break;
+        const double alpha = rz / pAp;
+        for (int b = 0; b < nb; b++) {
+            x[(long)b * P + pc] = x[(long)b * P + pc] + alpha * pp[(long)b * P + pc];
+            r[b] = r[b] - alpha * Ap[b];
+            t[b] = r[b] * (r[b] / dg[b]);
+        }
+        const double rzn = wg_sum(t, nb);
+        const double beta = rzn / rz;
+        rz = rzn;
+        for (int b = 0; b < nb; b++) pp[(long)b * P + pc] = r[b] / dg[b] + beta * pp[(long)b * P + pc];
+    }
+    free(r); free(pp); free(Ap); free(dg); free(t);
+}
+
+/* Linear prolongation, stored: every fine cell adds zc + 1/4 of the coarse
+ * differences toward its three face neighbours on its side of the parent
+ * (cell-centred linear interpolation from face neighbours only; exact for
+ * fields linear in the block's index space).  Coarse ghosts come from the
+ * coarse level's face-ghost fill; at an insulated radial face (no donor) the
+ * neighbour is the parent itself (zero gradient). */
+static void h_prolong_linear(const orc_heat *H, int l, const double *Zc_all, double *Zf_all) {
+    const int nb = H->nb, m = H->L[l].m, mc = m / 2;
+    const long P = (long)(m + 2) * (m + 2) * (m + 2), Pc = (long)(mc + 2) * (mc + 2) * (mc + 2);
+    for (int b = 0; b < nb; b++) {
+        const oblock *ob = &H->c->b[b];
+        const double *Zc = Zc_all + (long)b * Pc;
+        for (int k = 0; k < m; k++) for (int j = 0; j < m; j++) for (int i = 0; i < m; i++) {
+            const int ic = i / 2 + 1, jc = j / 2 + 1, kc = k / 2 + 1;
+            const int si = (i & 1) ? 1 : -1, sj = (j & 1) ? 1 : -1, sk = (k & 1) ? 1 : -1;
+            const double z0 = Zc[HP(mc, ic, jc, kc)];
+            const int in = ic + si;
+            double zi = Zc[HP(mc, in, jc, kc)];
+            if ((in == 0 && ob->bottom) || (in == mc + 1 && ob->top)) zi = z0;
+            const double zj = Zc[HP(mc, ic, jc + sj, kc)], zk = Zc[HP(mc, ic, jc, kc + sk)];
+            double v = z0 + 0.25 * (zi - z0);
+            v = v + 0.25 * (zj - z0);
+            v = v + 0.25 * (zk - z0);
+            double *zf = &Zf_all[(long)b * P + HP(m, i + 1, j + 1, k + 1)];
+            *zf = *zf + v;
+        }
+    }
+}
 
 static void h_vcycle(orc_heat *H, hwork *W, double inv_dt, const double *r0, double *z0_out) {
     const int nb = H->nb;
@@ -1178,15 +1279,17 @@ static void h_vcycle(orc_heat *H, hwork *W, double inv_dt, const double *r0, dou
         const hlev *L = &H->L[l];
         const int m = L->m;
         const long P = (long)(m + 2) * (m + 2) * (m + 2), M3 = (long)m * m * m;
+        if (l == H->nlev - 1) {
+            h_coarse_cg(H, W, l, inv_dt);
+            break;
+        }
         memset(W[l].Z, 0, sizeof(double) * (size_t)nb * P);
-        int sweeps = 2;                 /* every level, the 1-cell one included */
-        for (int s = 0; s < sweeps; s++) {
+        for (int s = 0; s < 2; s++) {
             h_exchange(L, W[l].Z);
             memcpy(W[l].Z2, W[l].Z, sizeof(double) * (size_t)nb * P);
             h_apply(L, nb, inv_dt, W[l].Z, W[l].R, W[l].Z2, 2, 0.8);
             double *t = W[l].Z; W[l].Z = W[l].Z2; W[l].Z2 = t;
         }
-        if (l == H->nlev - 1) break;
         h_exchange(L, W[l].Z);
         h_apply(L, nb, inv_dt, W[l].Z, W[l].R, W[l].RES, 1, 0.0);
         const int mc = m / 2;
@@ -1201,11 +1304,10 @@ static void h_vcycle(orc_heat *H, hwork *W, double inv_dt, const double *r0, dou
     }
     for (int l = H->nlev - 2; l >= 0; l--) {
         const hlev *L = &H->L[l];
-        const int m = L->m, mc = m / 2;
-        const long P = (long)(m + 2) * (m + 2) * (m + 2), Pc = (long)(mc + 2) * (mc + 2) * (mc + 2);
-        for (int b = 0; b < nb; b++)
-            for (int k = 0; k < m; k++) for (int j = 0; j < m; j++) for (int i = 0; i < m; i++)
-                W[l].Z[(long)b * P + HP(m, i + 1, j + 1, k + 1)] += W[l + 1].Z[(long)b * Pc + HP(mc, i / 2 + 1, j / 2 + 1, k / 2 + 1)];
+        const int m = L->m;
+        const long P = (long)(m + 2) * (m + 2) * (m + 2);
+        h_exchange(&H->L[l + 1], W[l + 1].Z);
+        h_prolong_linear(H, l, W[l + 1].Z, W[l].Z);
         for (int s = 0; s < 2; s++) {
             h_exchange(L, W[l].Z);
             memcpy(W[l].Z2, W[l].Z, sizeof(double) * (size_t)nb * P);
@@ -1215,6 +1317,29 @@ static void h_vcycle(orc_heat *H, hwork *W, double inv_dt, const double *r0, dou
     }
     const int m = H->L[0].m;
     memcpy(z0_out, W[0].Z, sizeof(double) * (size_t)nb * (m + 2) * (m + 2) * (m + 2));
+}
+
+/* zf += P zc between level l+1 and level l (interior arrays, Fortran order
+ * per block): the V-cycle's prolongation with the coarse level's ghost fill */
+void orc_heat_prolong(orc_heat *H, int l, const double *zc, double *zf) {
+    const int nb = H->nb, m = H->L[l].m, mc = m / 2;
+    const long P = (long)(m + 2) * (m + 2) * (m + 2), Pc = (long)(mc + 2) * (mc + 2) * (mc + 2);
+    const long M3 = (long)m * m * m, Mc = (long)mc * mc * mc;
+    double *cp = malloc(sizeof(double) * (size_t)nb * Pc), *fp = malloc(sizeof(double) * (size_t)nb * P);
+    h_pad(mc, nb, zc, cp);
+    h_pad(m, nb, zf, fp);
+    h_exchange(&H->L[l + 1], cp);
+    h_prolong_linear(H, l, cp, fp);
+    for (int b = 0; b < nb; b++)
+        for (int k = 0; k < m; k++) for (int j = 0; j < m; j++) for (int i = 0; i < m; i++)
+            zf[(long)b * M3 + i + (long)m * (j + (long)m * k)] = fp[(long)b * P + HP(m, i + 1, j + 1, k + 1)];
+    (void)Mc;
+    free(cp); free(fp);
+}
+
+void orc_heat_set_coarse(orc_heat *H, double tol, int max_iter) {
+    H->coarse_tol = tol;
+    H->coarse_max = max_iter;
 }
 
 /* pad/unpad helpers (interior Fortran order per block) */

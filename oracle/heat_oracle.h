@@ -14,5 +14,9 @@ void orc_heat_apply(orc_heat *H, double dt, const double *x, double *y);
 void orc_heat_vcycle(orc_heat *H, double dt, const double *r, double *z);
 /* Jacobi diagonal V/dt + sum of face coefficients, interior Fortran order */
 void orc_heat_diag(orc_heat *H, double dt, double *d);
+/* coarsest-level CG of the V-cycle: relative stop tol on r.z^(1/2), iteration cap */
+void orc_heat_set_coarse(orc_heat *H, double tol, int max_iter);
+/* zf += P zc, the V-cycle's prolongation from level l+1 to level l (test hook) */
+void orc_heat_prolong(orc_heat *H, int l, const double *zc, double *zf);
 int orc_heat_solve(orc_heat *H, double dt, const double *Tn, double tol, int max_iter, double *x_out, double *relres);
 #endif

@@ -67,6 +67,8 @@ def lib():
         L.orc_heat_vcycle.argtypes = [vp, C.c_double, dp, dp]
         L.orc_heat_diag.argtypes = [vp, C.c_double, dp]
         L.orc_heat_solve.argtypes = [vp, C.c_double, dp, C.c_double, C.c_int, dp, dp]
+        L.orc_heat_set_coarse.argtypes = [vp, C.c_double, C.c_int]
+        L.orc_heat_prolong.argtypes = [vp, C.c_int, dp, dp]
         fp = C.POINTER(C.c_float)
         L.cnn_round_bf16.restype = C.c_float
         L.cnn_round_bf16.argtypes = [C.c_float]
@@ -226,6 +228,17 @@ class HeatOracle:
         y = np.zeros(self.shape)
         self.L.orc_heat_apply(self.h, dt, _p(x, C.c_double), _p(y, C.c_double))
         return y
+
+    def prolong(self, level, zc, zf):
+        """zf + P zc from level+1 (edge n/2^(level+1)) to level (test hook)."""
+        zc = np.ascontiguousarray(zc, dtype=np.float64)
+        out = np.array(zf, dtype=np.float64, copy=True, order="C")
+        self.L.orc_heat_prolong(self.h, int(level), _p(zc, C.c_double), _p(out, C.c_double))
+        return out
+
+    def set_coarse(self, tol, max_iter):
+        """Coarsest-level CG of the V-cycle: relative stop tol, iteration cap."""
+        self.L.orc_heat_set_coarse(self.h, float(tol), int(max_iter))
 
     def vcycle(self, dt, r):
         r = np.ascontiguousarray(r, dtype=np.float64)

@@ -15,7 +15,7 @@
 !
 ! Built with -DFALCON_DROPIN (oracle/Makefile target `dropin`), the same driver
 ! links the Fortran shims of falcon_amd/fortran in place of the reference's
-! ModCommunication / ModAdvance, stepping on the GPU (one rank).
+! ModCommunication / ModAdvance / ModCheck, stepping on the GPU (one rank).
 !
 ! Usage: [mpirun -n N] falcon_ref_harness <input.bin> <output.bin>
 program falcon_ref_harness
@@ -34,6 +34,7 @@ program falcon_ref_harness
     use ModBoundary,    only:   ModBoundary_Dynamo_primitives
     use ModCommunication, only: ModCommunication_SetGCAll,ModCommunication_SendRecvGC_new
     use ModAdvance,       only: ModAdvance_rk4
+    use ModCheck,         only: ModCheck_primitive
 #ifdef FALCON_DROPIN
     use ModFalconShim,    only: falcon_sync_to_host
 #endif
@@ -202,8 +203,9 @@ program falcon_ref_harness
         call MPI_BARRIER(MPI_COMM_WORLD,ierr)
         t0=MPI_Wtime()
         do istep=1,nsteps
-            ! 3_6/test1.f90:58
+            ! 3_6/test1.f90:58,61 (DoCheck)
             call ModAdvance_rk4(Tree,.true.,dt)
+            call ModCheck_primitive(Tree,.false.)
             if (idump<=ndump) then
                 if (dump_steps(idump)==istep) then
                     ! the dumps stay outside the timed loop

@@ -115,8 +115,9 @@ def test_fp32_deviation_error_vs_fp64():
     err = rel_err(res["fp32"], res["fp64"])
     print("fp32 rel err per var after 20 steps:", err)
     # tolerance: 1e-6 relative to each variable's max after 100 steps is the
-    # target for the 50M-cell shell; 20 steps here must already be within it.
-    assert np.all(err[1:] <= 1e-6), err
+    # target for the 50M-cell shell; 20 steps here must already be within it,
+    # for every variable
+    assert np.all(err <= 1e-6), err
 
 
 PULL_LAYOUTS = dict(LAYOUTS, amr16=configs.cubic_random_amr((32, 64, 192), 16))
@@ -210,3 +211,59 @@ def test_fp32_deviation_error_after_100_steps():
     err = rel_err(res["fp32"], res["fp64"])
     print("fp32 rel err per var after 100 steps:", err, flush=True)
     assert np.all(err <= 1e-6), err
+
+
+def test_background_change_after_upload():
+    """A background set after the device tables are up (here after a first
+    ghost fill) reaches the RHS, the diffusion and dt: they equal the oracle's
+    with the new background."""
+    print("[background] layout=mixed", flush=True)
+    s, o = _pair(LAYOUTS["mixed"], seed=21)
+    s.sendrecv_gc(0); o.exchange(0)
+    s.compute_dt(0.5)
+    import ref_io
+    for ib in range(o.nblocks):
+        bg = ref_io.synthetic_background(o.centres(ib)[0], seed_shift=0.3)
+        bg[1] *= 1.7
+        s.set_background(ib, bg); o.set_background(ib, bg)
+    assert s.compute_dt(0.5) == o.dt(0.5)
+    r_gpu = s.rhs()
+    r_orc = np.stack([o.rhs(ib) for ib in range(o.nblocks)])
+    assert np.array_equal(r_gpu, r_orc), rel_err(r_gpu, r_orc)
+    d_gpu = s.diffusion_rhs()
+    d_orc = np.stack([o.diffusion(ib) for ib in range(o.nblocks)])
+    assert np.array_equal(d_gpu, d_orc), rel_err(d_gpu, d_orc)
+
+
+def test_nan_scan_finds_the_first_nan_in_reference_order():
+    """falcon_check_state counts NaNs over every padded cell and variable and
+    reports the first one in ModCheck_primitive's scan order (blocks, then phi,
+    theta, r, variable innermost)."""
+    print("[nan scan] layout=cubic8", flush=True)
+    s, o = _pair(LAYOUTS["cubic8"], seed=23)
+    assert s.check_state(0) == (0, None)
+    nb = o.nblocks
+    st = s.get_state(5)                       # numpy (8, NK, NJ, NI)
+    st[6, 7, 0, 3] = np.nan                   # block 5, K=7, J=0 (ghost row), I=3, var 6
+    st[2, 7, 0, 9] = np.nan                   # same K, J; larger I: later in the scan
+    s.set_state(5, st)
+    st = s.get_state(nb - 1)
+    st[0, 0, 0, 0] = np.nan                   # a later block
+    s.set_state(nb - 1, st)
+    n, where = s.check_state(0)
+    assert n == 3 and where == (5, 3, 0, 7, 6)
+    # FP32 path: a NaN in an interior cell spreads through a step, into the
+    # stage buffer too (the ghosts above are refilled by the first fill)
+    st = s.get_state(7)
+    st[1, 5, 6, 7] = np.nan
+    s.set_state(7, st)
+    s.set_precision("fp32")
+    s.advance(1, 0.5, graph=False)
+    assert s.check_state(0)[0] > 3 and s.check_state(1)[0] > 0
+    s2 = Shell(LAYOUTS["cubic8"])
+    s2.set_gc_all()
+    s2.background_polytrope(20.0)
+    s2.init_synthetic(seed=3)
+    s2.set_precision("fp32")
+    s2.advance(2, 0.5, graph=True)
+    assert s2.check_state(0) == (0, None) and s2.check_state(1) == (0, None)

@@ -10,7 +10,9 @@ oracle is pinned by mathematics rather than by reference vectors:
     (whose angular terms must cancel against the radial ones), kappa = kappa0
     (r/r_ref)^-3;
   * the FCG solution equals a dense direct solve of the assembled matrix;
-  * the V-cycle is linear and contracts the residual.
+  * the V-cycle is linear (with its coarsest-level CG run to convergence) and
+    contracts the residual; its linear prolongation is exact for coarse fields
+    linear in the block's index space.
 """
 import numpy as np
 import pytest
@@ -168,11 +170,71 @@ def test_solve_zero_rhs():
 
 
 def test_vcycle_linear_and_contracting():
+    """The V-cycle's only nonlinear part is the coarsest level's CG (a Krylov
+    method); run to convergence it is the linear coarse inverse, and the cycle
+    is linear.  With the default stop (1e-2) it still contracts."""
     o, H = _setup((16, 32, 96), 8, refined=True)
     dt = 100 * H.dt_explicit
     rng = np.random.default_rng(5)
     r1, r2 = rng.standard_normal(H.shape), rng.standard_normal(H.shape)
+    H.set_coarse(1e-15, 500)
     z1, z2, z12 = H.vcycle(dt, r1), H.vcycle(dt, r2), H.vcycle(dt, r1 + 2.0 * r2)
     assert np.max(np.abs(z12 - (z1 + 2.0 * z2))) <= 1e-12 * np.max(np.abs(z12))
-    for r, z in ((r1, z1), (r2, z2)):
+    H.set_coarse(1e-2, 50)
+    for r in (r1, r2):
+        z = H.vcycle(dt, r)
         assert np.linalg.norm(r - H.apply(dt, z)) < 0.5 * np.linalg.norm(r)
+
+
+def test_coarse_cg_solves_the_coarsest_level():
+    """A V-cycle on a layout of 1-cell blocks is the coarsest level alone: its
+    CG run to a tight stop solves the 1-cell-per-block operator (A z = r to
+    1e-8; the Yin-Yang overlap makes the operator slightly unsymmetric, so CG
+    stalls near there), and the default stop leaves about 1e-2."""
+    o, H = _setup((4, 8, 24), 1)
+    assert H.levels == 1
+    dt = 10.0 * H.dt_explicit
+    r = np.random.default_rng(6).standard_normal(H.shape)
+    H.set_coarse(1e-15, 200)
+    z = H.vcycle(dt, r)
+    assert np.linalg.norm(r - H.apply(dt, z)) <= 1e-8 * np.linalg.norm(r)
+    H.set_coarse(1e-2, 50)
+    z = H.vcycle(dt, r)
+    assert 1e-4 < np.linalg.norm(r - H.apply(dt, z)) / np.linalg.norm(r) < 2e-2
+
+
+def test_linear_prolongation_exact_on_linear_fields():
+    """zf + P zc with zc linear in (theta, phi) at the coarse cell centres gives
+    the same linear function at the fine centres, wherever the three coarse
+    face neighbours a fine cell reads are in its own panel's same-level blocks
+    (Yin blocks away from the panel's theta/phi edges; theta/phi spacing is
+    uniform in a panel).  Checked between every pair of levels."""
+    o, H = _setup((16, 32, 96), 8)
+    R, T, P, info = _cells(o)
+    yin = np.array([fl[0] for _, fl in info], bool)
+    lo = np.min([rng[:3] for rng, _ in info], axis=0)
+    hi = np.max([rng[3:] for rng, _ in info], axis=0)
+    f = lambda t, p: 0.7 - 1.3 * t + 0.4 * p
+    n = o.ni
+    for level in range(H.levels - 1):
+        m, mc = n >> level, n >> (level + 1)
+        tc = np.zeros((o.nblocks, mc, mc, mc)); pc = np.zeros_like(tc)
+        tf = np.zeros((o.nblocks, m, m, m)); pf = np.zeros_like(tf)
+        ok = np.zeros(tf.shape, bool)
+        for ib, (rng, fl) in enumerate(info):
+            ct = rng[1] + (np.arange(mc) + 0.5) * (rng[4] - rng[1]) / mc
+            cp = rng[2] + (np.arange(mc) + 0.5) * (rng[5] - rng[2]) / mc
+            ft = rng[1] + (np.arange(m) + 0.5) * (rng[4] - rng[1]) / m
+            fp = rng[2] + (np.arange(m) + 0.5) * (rng[5] - rng[2]) / m
+            tc[ib] = ct[None, :, None]; pc[ib] = cp[:, None, None]
+            tf[ib] = ft[None, :, None]; pf[ib] = fp[:, None, None]
+            if yin[ib]:
+                okj = ~(((np.arange(m) < 2) & np.isclose(rng[1], lo[1])) | ((np.arange(m) >= m - 2) & np.isclose(rng[4], hi[1])))
+                okk = ~(((np.arange(m) < 2) & np.isclose(rng[2], lo[2])) | ((np.arange(m) >= m - 2) & np.isclose(rng[5], hi[2])))
+                ok[ib] = okk[:, None, None] & okj[None, :, None]
+        got = H.prolong(level, f(tc, pc), np.zeros(tf.shape))
+        assert ok.sum() > 0
+        assert np.max(np.abs(got - f(tf, pf))[ok]) < 1e-12, level
+        # and it adds to the fine field
+        base = np.random.default_rng(level).standard_normal(tf.shape)
+        assert np.allclose(H.prolong(level, f(tc, pc), base) - base, got, rtol=0, atol=1e-12)
