@@ -65,6 +65,7 @@ def load():
         "falcon_compute_dt": [vp, C.c_double, dp],
         "falcon_set_fixed_dt": [vp, C.c_double],
         "falcon_check_state": [vp, C.c_int, i64p, ip],
+        "falcon_compute_dt_noalloc": [vp, C.c_double, dp],
         "falcon_state_diff": [vp, vp, C.c_double, dp, dp, i64p, i64p],
         "falcon_rhs": [vp, dp],
         "falcon_diffusion_rhs": [vp, dp],
@@ -294,6 +295,12 @@ class Shell:
     def compute_dt(self, cfl=0.5):
         dt = C.c_double()
         _check(self.L.falcon_compute_dt(self.h, cfl, C.byref(dt)))
+        return dt.value
+
+    def compute_dt_noalloc(self, cfl=0.5):
+        """compute_dt through the library's static scratch (no device allocation)."""
+        dt = C.c_double()
+        _check(self.L.falcon_compute_dt_noalloc(self.h, cfl, C.byref(dt)))
         return dt.value
 
     def check_state(self, which=0):

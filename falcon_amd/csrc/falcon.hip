@@ -21,6 +21,13 @@
 #include "heat.hpp"
 #include "layout.hpp"
 
+// Scratch of falcon_compute_dt_noalloc, in device memory the library never
+// allocates: the dt partial minima (one per tile of the diffusion's edge, at
+// most DT_NOALLOC_MAX_PARTIALS) followed by the resulting dt.  Each GPU holds its
+// own copy until the process ends; every call on one GPU shares it.
+constexpr int DT_NOALLOC_MAX_PARTIALS = 1 << 14;
+static __device__ __align__(256) double scratch[DT_NOALLOC_MAX_PARTIALS + 32];
+
 namespace falcon {
 void synthetic_state(const Layout &lay, const std::vector<double> &bg, uint64_t seed, double s_amp, double b_ratio,
                      int nthreads, std::vector<double> &state);
@@ -51,6 +58,23 @@ int guard(F &&f) {
         g_err = e.what();
         return -1;
     }
+}
+
+// Device address of `scratch` per GPU, looked up once per device at set-up
+// (falcon_set_gc_all) with that device current; host code uses these, never the
+// array's name.
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[MAX_DEVICES] = {};
+
+void scratch_setup(int dev) {
+    if (dev < 0 || dev >= MAX_DEVICES) throw Error("device id out of range for the dt scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return;
+    void *p = nullptr;
+    HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error("hipGetSymbolAddress returned no address for the dt scratch");
+    g_scratch_ptr[dev] = (double *)p;
 }
 
 int host_threads() {
@@ -84,6 +108,7 @@ struct falcon_handle {
     std::vector<double> bg;   // per block 8 x NI
     std::vector<char> bg_set;
     int precision = 0;
+    int device = -1;           // the GPU current at falcon_set_gc_all (the stream's device)
     hipStream_t stream = nullptr;
     Dims dims{};
     DevTables dt{};
@@ -782,7 +807,11 @@ int falcon_set_gc_all(falcon_handle_t h) {
         d.tiles_j = (t.nj + 15) / 16;
         d.tiles_k = (t.nk + 15) / 16;
         d.P = (long long)d.NI * d.NJ * d.NK;
-        if (!h->stream) HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        if (!h->stream) {
+            HIPCHECK(hipGetDevice(&h->device));
+            HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+            scratch_setup(h->device);
+        }
         alloc_state(h);
         h->tables = true;
     });
@@ -1202,6 +1231,38 @@ int falcon_advance(falcon_handle_t h, int nsteps, double cfl, int use_graph, dou
         if (dt_last) {
             HIPCHECK(hipMemcpyAsync(dt_last, h->dtp.p, sizeof(double), hipMemcpyDeviceToHost, h->stream));
         }
+        HIPCHECK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int falcon_compute_dt_noalloc(falcon_handle_t h, double cfl, double *dt) {
+    return guard([&] {
+        h->need_tables();
+        if (!dt) throw Error("falcon_compute_dt_noalloc: null dt");
+        if (!h->dt.meta)
+            throw Error("falcon_compute_dt_noalloc: the device tables are not uploaded yet (a first "
+                        "falcon_compute_dt, falcon_sendrecv_gc or step uploads them)");
+        if (h->bg_dirty) {   // rewrites the existing radial rows in place, allocates nothing
+            upload_radial(h);
+            h->bg_dirty = false;
+        }
+        int dev = -1;
+        HIPCHECK(hipGetDevice(&dev));
+        if (dev != h->device) throw Error("falcon_compute_dt_noalloc: call it with the handle's GPU current");
+        double *base = g_scratch_ptr[h->device];
+        if (!base) throw Error("falcon_compute_dt_noalloc: no dt scratch on this GPU");
+        const Dims &d = h->dims;
+        const int td = h->tile_diff[h->precision];
+        const long long n = (long long)d.nblocks * ((d.nj + td - 1) / td) * ((d.nk + td - 1) / td);
+        if (n > DT_NOALLOC_MAX_PARTIALS)
+            throw Error("falcon_compute_dt_noalloc: " + std::to_string(n) + " partial minima exceed its limit of " +
+                        std::to_string(DT_NOALLOC_MAX_PARTIALS) + " (falcon_compute_dt has none)");
+        // every partial is written by the tile pass before the final pass reads it
+        double *part = base, *dtp = base + DT_NOALLOC_MAX_PARTIALS;
+        launch_dt_partial(d, h->dt, h->U[h->cur].as<double>(), part, h->stream, td);
+        launch_dt_final(d, part, dtp, cfl, h->stream, td);
+        check_launch();
+        HIPCHECK(hipMemcpyAsync(dt, dtp, sizeof(double), hipMemcpyDeviceToHost, h->stream));
         HIPCHECK(hipStreamSynchronize(h->stream));
     });
 }

@@ -153,6 +153,19 @@ int falcon_state_diff(falcon_handle_t a, falcon_handle_t b, double rel_tol, doub
                       int64_t *n_above, int64_t *argmax);
 /* ModWaveSpeed_Dynamo + ModTimeStep_Dynamo (3_6/ModWaveSpeed.f90, ModTimeStep.f90). */
 int falcon_compute_dt(falcon_handle_t h, double cfl, double *dt);
+/* falcon_compute_dt without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight stepping loops).  Same result as falcon_compute_dt, bit for bit, but its
+ * partial minima and dt live in a fixed static device array of the library (one
+ * per GPU, held until the process ends) instead of the handle's buffers, and it
+ * calls no hipMalloc/hipFree.  Limits: at most 16384 partial minima (nblocks x
+ * the diffusion's tiles per block: the 128x256x768-per-panel shell in 16^3
+ * blocks needs 12288 at tile edge 16), else an error; the handle's device
+ * tables must already be on the GPU (after a first falcon_compute_dt,
+ * falcon_sendrecv_gc or step), else an error; call it with the handle's GPU
+ * current.  Calls on one GPU share the array, so calls that overlap in time on
+ * one GPU (two host threads, two handles) race. */
+int falcon_compute_dt_noalloc(falcon_handle_t h, double cfl, double *dt);
 /* Single operators for parity: R of ModEquation_Dynamo after ModBoundary on the
  * state (3_6/ModEquation.f90:24-37), and of ModDiffusion_Aritificial_1 (h=2).
  * Out: per block (ni,nj,nk,8) Fortran order, blocks consecutive. */
