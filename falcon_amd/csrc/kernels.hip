@@ -150,34 +150,26 @@ struct Marcher {
 
 // ---------------------------------------------------------------- stage
 
-// FP64 inputs of the FP32 path's pressure and gravity terms: gp and gq at the
-// five r-offsets of the column, the radial stencil coefficient, 1/rho0 and
-// g/rho0 of the cell.
-struct Balance64 {
-    const double *gp, *gq;
-    double cr, rho0inv, gor;
-};
-
-// RHS of ModEquation_Dynamo (3_6/ModEquation.f90:24-162) for one cell.
-// win[v][q]: variable v at r-offset q-2 of this lane's column; L(v,dj,dk): LDS plane.
-// With T = float the pressure perturbation p1 = gp rho' + gq s' and the sum of
-// the radial pressure gradient and gravity are formed in FP64 from FP64
-// coefficients (bal): once the pressure perturbation has relaxed both are
-// small differences of large terms, whose FP32 rounding would dominate the
-// velocity error.  T = double keeps the reference's operation order.
-template <typename T, typename LDSF>
-__device__ __forceinline__ void cell_rhs(const T (&win)[8][5], LDSF L, const T xr[5], const T rho0r[5], const T gpr[5],
+// RHS of ModEquation_Dynamo (3_6/ModEquation.f90:24-162) for one cell, in
+// the reference's operation order, computed in T (FP64 on both paths).
+// W(v, q): variable v at r-offset q-2 of this lane's column, read from the
+// register window `win` of storage type TW (the FP32 path keeps its stage
+// values in FP32 and widens them here); L(v,dj,dk): the LDS plane, widened the
+// same way.
+template <typename T, typename TW, typename LDSF>
+__device__ __forceinline__ void cell_rhs(const TW (&win)[8][5], LDSF L, const T xr[5], const T rho0r[5], const T gpr[5],
                                          const T gqr[5], const T st[5], T rinv, T r2inv, T xir2, T heat, T gor,
-                                         T sinv, T cot, T cr, T ct, T cp, T R[8], const Balance64 &bal) {
+                                         T sinv, T cot, T cr, T ct, T cp, T R[8]) {
+    auto W = [&](int v, int q) -> T { return (T)win[v][q]; };
     const T rho0c = rho0r[2];
-    const T vr = win[1][2], vt = win[2][2], vp = win[3][2];
-    const T br = win[4][2], bt = win[5][2], bp = win[6][2];
+    const T vr = W(1, 2), vt = W(2, 2), vp = W(3, 2);
+    const T br = W(4, 2), bt = W(5, 2), bp = W(6, 2);
 
     // Mass conservation (:50-61) with ModSpherical_div (:104-159)
     {
         T w[5];
 #pragma unroll
-        for (int q = 0; q < 5; q++) w[q] = (rho0r[q] * win[1][q]) * (xr[q] * xr[q]);
+        for (int q = 0; q < 5; q++) w[q] = (rho0r[q] * W(1, q)) * (xr[q] * xr[q]);
         T t1 = d4(w[0], w[1], w[3], w[4], cr) * r2inv;
         T t2 = d4((rho0c * L(2, -2, 0)) * st[0], (rho0c * L(2, -1, 0)) * st[1], (rho0c * L(2, 1, 0)) * st[3],
                   (rho0c * L(2, 2, 0)) * st[4], ct);
@@ -189,7 +181,7 @@ __device__ __forceinline__ void cell_rhs(const T (&win)[8][5], LDSF L, const T x
     }
     // A_dot_Grad_f (ModSpherical.f90:241-279) with A = v at the centre
     auto adg = [&](int f) -> T {
-        T o = vr * d4(win[f][0], win[f][1], win[f][3], win[f][4], cr);
+        T o = vr * d4(W(f, 0), W(f, 1), W(f, 3), W(f, 4), cr);
         T tmp = vt * d4(L(f, -2, 0), L(f, -1, 0), L(f, 1, 0), L(f, 2, 0), ct);
         o = o + tmp * rinv;
         tmp = (vp * d4(L(f, 0, -2), L(f, 0, -1), L(f, 0, 1), L(f, 0, 2), cp)) * rinv;
@@ -212,24 +204,10 @@ __device__ __forceinline__ void cell_rhs(const T (&win)[8][5], LDSF L, const T x
         R[1] = -a1; R[2] = -a2; R[3] = -a3;
     }
     // Pressure gradient (:79-92): p1 (:40-46), ModSpherical_Grad_f (:201-239)
-    double prad = 0.0;   // FP32 path: radial pressure gradient plus gravity, FP64
-    if constexpr (sizeof(T) == 4) {
-        double pr[5];
-#pragma unroll
-        for (int q = 0; q < 5; q++) pr[q] = bal.gp[q] * (double)win[0][q] + bal.gq[q] * (double)win[7][q];
-        auto p1L = [&](int dj, int dk) -> double {
-            return bal.gp[2] * (double)L(0, dj, dk) + bal.gq[2] * (double)L(7, dj, dk);
-        };
-        const double t1 = d4(pr[0], pr[1], pr[3], pr[4], bal.cr);
-        const double t2 = d4(p1L(-2, 0), p1L(-1, 0), p1L(1, 0), p1L(2, 0), (double)ct);
-        const double t3 = d4(p1L(0, -2), p1L(0, -1), p1L(0, 1), p1L(0, 2), (double)cp);
-        prad = (-t1) * bal.rho0inv - bal.gor * (double)win[0][2];
-        R[2] = R[2] - (T)((t2 * (double)rinv) * bal.rho0inv);
-        R[3] = R[3] - (T)(((t3 * (double)rinv) * (double)sinv) * bal.rho0inv);
-    } else {
+    {
         T pr[5];
 #pragma unroll
-        for (int q = 0; q < 5; q++) pr[q] = gpr[q] * win[0][q] + gqr[q] * win[7][q];
+        for (int q = 0; q < 5; q++) pr[q] = gpr[q] * W(0, q) + gqr[q] * W(7, q);
         auto p1L = [&](int dj, int dk) -> T { return gpr[2] * L(0, dj, dk) + gqr[2] * L(7, dj, dk); };
         T t1 = d4(pr[0], pr[1], pr[3], pr[4], cr);
         T t2 = d4(p1L(-2, 0), p1L(-1, 0), p1L(1, 0), p1L(2, 0), ct);
@@ -240,14 +218,13 @@ __device__ __forceinline__ void cell_rhs(const T (&win)[8][5], LDSF L, const T x
         R[2] = R[2] - g2 / rho0c;
         R[3] = R[3] - g3 / rho0c;
     }
-    (void)prad;
     // Lorentz force (:97-116): curl(B) (ModSpherical.f90:38-102) x B / rho0
     {
         T o1 = d4(L(6, -2, 0) * st[0], L(6, -1, 0) * st[1], L(6, 1, 0) * st[3], L(6, 2, 0) * st[4], ct) -
                d4(L(5, 0, -2), L(5, 0, -1), L(5, 0, 1), L(5, 0, 2), cp);
         T o2 = d4(L(4, 0, -2) / st[2], L(4, 0, -1) / st[2], L(4, 0, 1) / st[2], L(4, 0, 2) / st[2], cp) -
-               d4(win[6][0] * xr[0], win[6][1] * xr[1], win[6][3] * xr[3], win[6][4] * xr[4], cr);
-        T o3 = d4(win[5][0] * xr[0], win[5][1] * xr[1], win[5][3] * xr[3], win[5][4] * xr[4], cr) -
+               d4(W(6, 0) * xr[0], W(6, 1) * xr[1], W(6, 3) * xr[3], W(6, 4) * xr[4], cr);
+        T o3 = d4(W(5, 0) * xr[0], W(5, 1) * xr[1], W(5, 3) * xr[3], W(5, 4) * xr[4], cr) -
                d4(L(4, -2, 0), L(4, -1, 0), L(4, 1, 0), L(4, 2, 0), ct);
         o1 = o1 * rinv; o2 = o2 * rinv; o3 = o3 * rinv;
         o1 = o1 * sinv;
@@ -259,8 +236,7 @@ __device__ __forceinline__ void cell_rhs(const T (&win)[8][5], LDSF L, const T x
         R[3] = R[3] + c3 / rho0c;
     }
     // Gravity (:121-127)
-    if constexpr (sizeof(T) == 4) R[1] = (T)((double)R[1] + prad);
-    else R[1] = R[1] - gor * win[0][2];
+    R[1] = R[1] - gor * W(0, 2);
     // Entropy advection and heating (:132-151)
     R[7] = -adg(7);
     R[7] = R[7] + heat;
@@ -272,8 +248,8 @@ __device__ __forceinline__ void cell_rhs(const T (&win)[8][5], LDSF L, const T x
         T e2r[5], e3r[5];
 #pragma unroll
         for (int q = 0; q < 5; q++) {
-            e2r[q] = win[3][q] * win[4][q] - win[1][q] * win[6][q];
-            e3r[q] = win[1][q] * win[5][q] - win[2][q] * win[4][q];
+            e2r[q] = W(3, q) * W(4, q) - W(1, q) * W(6, q);
+            e3r[q] = W(1, q) * W(5, q) - W(2, q) * W(4, q);
         }
         T o1 = d4(e3L(-2, 0) * st[0], e3L(-1, 0) * st[1], e3L(1, 0) * st[3], e3L(2, 0) * st[4], ct) -
                d4(e2L(0, -2), e2L(0, -1), e2L(0, 1), e2L(0, 2), cp);
@@ -287,6 +263,12 @@ __device__ __forceinline__ void cell_rhs(const T (&win)[8][5], LDSF L, const T x
     }
 }
 
+// T is the storage type of the stage input (the LDS planes and the register
+// window); the right-hand side is computed in FP64 for either.  Stage values
+// stored in FP32 are the FP32 path's only rounding: FP32 arithmetic on the
+// stencils and FP32 copies of the metric and background coefficients were
+// each measured to put 1e-6 of error into the 100-step FP32-vs-FP64 state on
+// the 50 M-cell shell, FP32 storage alone 5e-7.
 // MODE 0: out = U + (dt*R)/div on the interior (3_6/ModAdvance.f90:87-93), the
 //         sum formed in FP64 from the base U of type TB (the FP64 state, or for
 //         the FP32 path's intermediate stages its FP32 copy: those stage values
@@ -321,13 +303,15 @@ __global__ void __launch_bounds__(TL * TL) k_stage(Dims d, DevTables t, const TI
 
         const long long rb = (long long)b * d.NI, tb = (long long)b * d.NJ;
         const BlockMeta mt = t.meta[b];
-        const T cr = (T)mt.cr, ct = (T)mt.ct, cp = (T)mt.cp;
+        // the right-hand side is computed in FP64 whatever the storage type T:
+        // coefficients from the FP64 tables, stage values widened on use
+        const double cr = mt.cr, ct = mt.ct, cp = mt.cp;
         // theta rows are read only by active lanes (inactive lanes of a partial
         // tile have J beyond the padded range)
-        T st[5];
+        double st[5];
     #pragma unroll
-        for (int q = 0; q < 5; q++) st[q] = m.act ? (T)t.sint[tb + m.J - 2 + q] : T(1);
-        const T sinv = m.act ? (T)t.sinv[tb + m.J] : T(1), cot = m.act ? (T)t.cot[tb + m.J] : T(0);
+        for (int q = 0; q < 5; q++) st[q] = m.act ? t.sint[tb + m.J - 2 + q] : 1.0;
+        const double sinv = m.act ? t.sinv[tb + m.J] : 1.0, cot = m.act ? t.cot[tb + m.J] : 0.0;
         const double dt = MODE == 0 ? *dtp : 0.0;
 
         T win[8][5];
@@ -357,32 +341,24 @@ __global__ void __launch_bounds__(TL * TL) k_stage(Dims d, DevTables t, const TI
             }
             __syncthreads();
             if (m.act) {
-                T xr[5], rho0r[5], gpr[5], gqr[5];
+                double xr[5], rho0r[5], gpr[5], gqr[5];
     #pragma unroll
                 for (int q = 0; q < 5; q++) {
-                    xr[q] = (T)t.xi[rb + I - 2 + q];
-                    rho0r[q] = (T)t.rho0[rb + I - 2 + q];
-                    gpr[q] = (T)t.gp[rb + I - 2 + q];
-                    gqr[q] = (T)t.gq[rb + I - 2 + q];
+                    xr[q] = t.xi[rb + I - 2 + q];
+                    rho0r[q] = t.rho0[rb + I - 2 + q];
+                    gpr[q] = t.gp[rb + I - 2 + q];
+                    gqr[q] = t.gq[rb + I - 2 + q];
                 }
-                auto L = [&](int v, int dj, int dk) -> T { return lds[buf][v][ty + 2 + dj][tx + 2 + dk]; };
-                T R[8];
-                Balance64 bal;
-                if (sizeof(T) == 4) {
-                    bal.gp = t.gp + rb + I - 2;
-                    bal.gq = t.gq + rb + I - 2;
-                    bal.cr = mt.cr;
-                    bal.rho0inv = 1.0 / t.rho0[rb + I];
-                    bal.gor = t.g_over_rho0[rb + I];
-                }
-                cell_rhs<T>(win, L, xr, rho0r, gpr, gqr, st, (T)t.rinv[rb + I], (T)t.r2inv[rb + I], (T)t.xir2[rb + I],
-                            (T)t.heat[rb + I], (T)t.g_over_rho0[rb + I], sinv, cot, cr, ct, cp, R, bal);
+                auto L = [&](int v, int dj, int dk) -> double { return (double)lds[buf][v][ty + 2 + dj][tx + 2 + dk]; };
+                double R[8];
+                cell_rhs<double, T>(win, L, xr, rho0r, gpr, gqr, st, t.rinv[rb + I], t.r2inv[rb + I], t.xir2[rb + I],
+                                    t.heat[rb + I], t.g_over_rho0[rb + I], sinv, cot, cr, ct, cp, R);
                 if (MODE == 0) {
                     const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + (long long)m.J * m.SJ + m.K;
     #pragma unroll
                     for (int v = 0; v < 8; v++) {
                         const long long q = base + (long long)v * d.P;
-                        const double x = (double)U[q] + (dt * (double)R[v]) / div;
+                        const double x = (double)U[q] + (dt * R[v]) / div;
                         out[q] = (TOUT)x;
                         if (SHADOW) shadow[q] = (float)x;
                     }
@@ -427,9 +403,10 @@ __constant__ double kParTop[8] = {1.0, -1.0, 1.0, 1.0, 1.0, -1.0, -1.0, 1.0};
 // boundaries only to each stage's input), while the interior is the state
 // after the fourth stage; at the domain's radial boundary faces the limiter
 // therefore divides by the one-step change of the first cell, which FP32
-// cannot resolve.  There the face flux is formed in FP64 from U^n (mirrored
+// storage cannot resolve.  There the face flux is formed from U^n (mirrored
 // with the parities of ModBoundary_Dynamo_primitives, 3_6/ModBoundary.f90:
-// 16-52) and the FP64 state, as the FP64 path forms it.
+// 16-52) and the FP64 state, as the FP64 path forms it.  Fluxes, speeds and
+// the update are computed in FP64 for either storage type T.
 
 template <typename T, typename TIN, int MODE, int TL, bool PERSIST>
 __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, const TIN *__restrict__ in,
@@ -443,8 +420,8 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
     // theta and phi face fluxes of the plane, each computed once: fj[v][y][x]
     // is the flux through the lower theta face of tile cell (y, x) (row TJa:
     // the upper face of the last row), fk[v][y][x] through its lower phi face
-    __shared__ T fj[8][TL + 1][TL];
-    __shared__ T fk[8][TL][TL + 1];
+    __shared__ double fj[8][TL + 1][TL];
+    __shared__ double fk[8][TL][TL + 1];
     __shared__ double red[4];
     // offset of the cell read in place of a ghost in direction code c, relative
     // to the ghost's own offset (0 where the ghost itself is read); set before
@@ -476,10 +453,10 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
         m.init(d, in, b, j0, k0, TJa, TKa, pull);
         const long long rb = (long long)b * d.NI, tb = (long long)b * d.NJ;
         const BlockMeta mt = t.meta[b];
-        const T dxi = (T)mt.dxi, dxj = (T)mt.dxj, dxk = (T)mt.dxk;
+        // fluxes and speeds are computed in FP64 whatever the storage type T
+        const double dxi = mt.dxi, dxj = mt.dxj, dxk = mt.dxk;
         double hmin = INFINITY;
         const double sint64 = m.act ? t.sint[tb + m.J] : 1.0;
-        const T sintc = m.act ? (T)t.sint[tb + m.J] : T(1);
         const double dt = MODE == 0 ? *dtp : 0.0;
 
         T win[8][5];
@@ -488,7 +465,7 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
         // same four values, same speed sum (speed at the upper cell first), so it
         // is carried instead of recomputed.  The same holds for the theta and phi
         // faces between neighbouring lanes, which go through fj / fk.
-        T fr[8];
+        double fr[8];
         bool have_fr = false;
         // FP32 path: U before the diffusion on the previous plane (the top
         // boundary face reads it after that plane has been updated in place)
@@ -500,12 +477,12 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
     #pragma unroll
             for (int q = 0; q < 5; q++) win[v][q] = m.act ? m.col(v, q) : T(0);
             hn[v] = m.hv ? m.halo(v, d.ng) : T(0);
-            fr[v] = T(0);
+            fr[v] = 0.0;
         }
         const TIN *Ub = in + (long long)b * 8 * d.P;
         // a ghost in a direction listed in the pull table is the same-level
         // neighbour's interior cell, translated by n
-        auto at = [&](int v, int II, int JJ, int KK) -> T {
+        auto at = [&](int v, int II, int JJ, int KK) -> double {
             long long o = (long long)v * d.P + (long long)II * m.SI + (long long)JJ * m.SJ + KK;
             if (PULL) {
                 const int c = (II < d.ng ? 0 : (II >= d.ng + d.ni ? 18 : 9)) +
@@ -513,7 +490,7 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
                               (KK < d.ng ? 0 : (KK >= d.ng + d.nk ? 2 : 1));
                 o += pdel[c];
             }
-            return (T)Ub[o];
+            return (double)Ub[o];
         };
         // faces on the tile's lower edges: lanes 0 .. TL-1 take the theta face
         // below row 0 in column x = lane, lanes TL .. 2TL-1 the phi face left of
@@ -535,8 +512,8 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
                 hn[v] = (m.hv && more) ? m.halo(v, I + 1) : T(0);
             }
             __syncthreads();
-            const T csI2 = (T)t.cs[rb + I - 2];
-            auto Lv = [&](int v, int dj, int dk) -> T { return lds[v][ty + 2 + dj][tx + 2 + dk]; };
+            const double csI2 = t.cs[rb + I - 2];
+            auto Lv = [&](int v, int dj, int dk) -> double { return (double)lds[v][ty + 2 + dj][tx + 2 + dk]; };
             // U^n of this plane, loaded before the face exchange (FP32 path, MODE 0)
             constexpr bool UEARLY = MODE == 0 && sizeof(T) == 4;
             double ucur[8];
@@ -548,16 +525,16 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
                     for (int v = 0; v < 8; v++) ucur[v] = U[base + (long long)v * d.P];
                 }
                 // speeds |v_d| + c_s at the cells the reference reads (see above)
-                const T c20 = fabs(at(2, I - 2, J, K - 2)) + csI2;
-                const T c2p = fabs(at(2, I - 2, J + 1, K - 2)) + csI2;
-                const T c30 = fabs(at(3, I - 2, J - 2, K)) + csI2;
-                const T c3p = fabs(at(3, I - 2, J - 2, K + 1)) + csI2;
+                const double c20 = fabs(at(2, I - 2, J, K - 2)) + csI2;
+                const double c2p = fabs(at(2, I - 2, J + 1, K - 2)) + csI2;
+                const double c30 = fabs(at(3, I - 2, J - 2, K)) + csI2;
+                const double c3p = fabs(at(3, I - 2, J - 2, K + 1)) + csI2;
     #pragma unroll
                 for (int v = 0; v < 8; v++) {
                     // upper theta face (the lower one of the cell at row + 1)
-                    fj[v][ty + 1][tx] = (face_flux(Lv(v, -1, 0), Lv(v, 0, 0), Lv(v, 1, 0), Lv(v, 2, 0)) * (c2p + c20)) * T(0.5);
+                    fj[v][ty + 1][tx] = (face_flux(Lv(v, -1, 0), Lv(v, 0, 0), Lv(v, 1, 0), Lv(v, 2, 0)) * (c2p + c20)) * 0.5;
                     // upper phi face
-                    fk[v][ty][tx + 1] = (face_flux(Lv(v, 0, -1), Lv(v, 0, 0), Lv(v, 0, 1), Lv(v, 0, 2)) * (c3p + c30)) * T(0.5);
+                    fk[v][ty][tx + 1] = (face_flux(Lv(v, 0, -1), Lv(v, 0, 0), Lv(v, 0, 1), Lv(v, 0, 2)) * (c3p + c30)) * 0.5;
                 }
             }
             if (bact) {
@@ -570,12 +547,12 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
                 const int bvar = bj ? 2 : 3;
                 const int bJ0 = bj ? j0 + d.ng : j0 + by + d.ng - 2, bK0 = bj ? k0 + bx + d.ng - 2 : k0 + d.ng;
                 const int bJ1 = bj ? bJ0 - 1 : bJ0, bK1 = bj ? bK0 : bK0 - 1;
-                const T s0 = fabs(at(bvar, I - 2, bJ0, bK0)) + csI2;
-                const T s1 = fabs(at(bvar, I - 2, bJ1, bK1)) + csI2;
+                const double s0 = fabs(at(bvar, I - 2, bJ0, bK0)) + csI2;
+                const double s1 = fabs(at(bvar, I - 2, bJ1, bK1)) + csI2;
     #pragma unroll
                 for (int v = 0; v < 8; v++) {
                     const T *pv = &lds[v][0][0] + bstart;
-                    const T f = (face_flux(pv[0], pv[bstep], pv[2 * bstep], pv[3 * bstep]) * (s0 + s1)) * T(0.5);
+                    const double f = (face_flux<double>(pv[0], pv[bstep], pv[2 * bstep], pv[3 * bstep]) * (s0 + s1)) * 0.5;
                     if (bj) fj[v][0][bx] = f;
                     else fk[v][by][0] = f;
                 }
@@ -583,36 +560,36 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
             __syncthreads();
             if (m.act) {
                 const int J = m.J, K = m.K;
-                const T csI = (T)t.cs[rb + I], csIm = (T)t.cs[rb + I - 1], csIp = (T)t.cs[rb + I + 1];
-                const T c1m = fabs(at(1, I - 1, J - 2, K - 2)) + csIm;
-                const T c10 = fabs(at(1, I, J - 2, K - 2)) + csI;
-                const T c1p = fabs(at(1, I + 1, J - 2, K - 2)) + csIp;
-                const T xic = (T)t.xi[rb + I];
-                const T L2 = dxj * xic;
-                const T L3 = (dxk * xic) * sintc;
+                const double csI = t.cs[rb + I], csIm = t.cs[rb + I - 1], csIp = t.cs[rb + I + 1];
+                const double c1m = fabs(at(1, I - 1, J - 2, K - 2)) + csIm;
+                const double c10 = fabs(at(1, I, J - 2, K - 2)) + csI;
+                const double c1p = fabs(at(1, I + 1, J - 2, K - 2)) + csIp;
+                const double xic = t.xi[rb + I];
+                const double L2 = dxj * xic;
+                const double L3 = (dxk * xic) * sint64;
                 const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + (long long)J * m.SJ + K;
                 const long long nn = (long long)d.ni * d.nj * d.nk;
                 const long long rbase = (long long)b * 8 * nn + ((long long)i * d.nj + (j0 + ty)) * d.nk + (k0 + tx);
                 double vv = 0.0, bb2 = 0.0;   // |v|^2 and |B|^2 of the new state, summed in component order
     #pragma unroll
                 for (int v = 0; v < 8; v++) {
-                    T r = T(0);
-                    T f0 = have_fr ? fr[v]
-                                   : (face_flux(win[v][0], win[v][1], win[v][2], win[v][3]) * (c10 + c1m)) * T(0.5);
-                    T f1 = (face_flux(win[v][1], win[v][2], win[v][3], win[v][4]) * (c1p + c10)) * T(0.5);
+                    double r = 0.0;
+                    double f0 = have_fr ? fr[v]
+                                        : (face_flux<double>(win[v][0], win[v][1], win[v][2], win[v][3]) * (c10 + c1m)) * 0.5;
+                    double f1 = (face_flux<double>(win[v][1], win[v][2], win[v][3], win[v][4]) * (c1p + c10)) * 0.5;
                     if (BFACE && bot && i == 0) {
                         // lower face of the first cell: ghosts 0 and -1 mirror cells 1 and 2 of U^n
                         const long long q = base + (long long)v * d.P;
                         const double p = kParBottom[v];
                         const double g0 = p * Uold[q], gm1 = p * Uold[q + m.SI];
-                        f0 = (T)((face_flux<double>(gm1, g0, ucur[v], U[q + m.SI]) * (double)(c10 + c1m)) * 0.5);
+                        f0 = (face_flux<double>(gm1, g0, ucur[v], U[q + m.SI]) * (c10 + c1m)) * 0.5;
                     }
                     if (BFACE && topb && i == d.ni - 1) {
                         // upper face of the last cell: ghosts ni+1 and ni+2 mirror cells ni and ni-1 of U^n
                         const long long q = base + (long long)v * d.P;
                         const double p = kParTop[v];
                         const double g1 = p * Uold[q], g2 = p * Uold[q - m.SI];
-                        f1 = (T)((face_flux<double>(uprev[v], ucur[v], g1, g2) * (double)(c1p + c10)) * 0.5);
+                        f1 = (face_flux<double>(uprev[v], ucur[v], g1, g2) * (c1p + c10)) * 0.5;
                     }
                     fr[v] = f1;
                     r = r + (f0 - f1) / dxi;
@@ -620,7 +597,7 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
                     r = r + (fk[v][ty][tx] - fk[v][ty][tx + 1]) / L3;
                     if (MODE == 0) {
                         const long long q = base + (long long)v * d.P;
-                        const double x = (UEARLY ? ucur[v] : U[q]) + (0.5 * dt) * (double)r;
+                        const double x = (UEARLY ? ucur[v] : U[q]) + (0.5 * dt) * r;
                         out[q] = x;
                         if (shadow) shadow[q] = (float)x;
                         if (v == 1) vv = x * x;
@@ -628,7 +605,7 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
                         if (v == 4) bb2 = x * x;
                         if (v == 5 || v == 6) bb2 = bb2 + x * x;
                     } else {
-                        out[rbase + (long long)v * nn] = (double)r;
+                        out[rbase + (long long)v * nn] = r;
                     }
                 }
                 if (MODE == 0) {
