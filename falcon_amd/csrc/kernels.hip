@@ -403,10 +403,12 @@ __constant__ double kParTop[8] = {1.0, -1.0, 1.0, 1.0, 1.0, -1.0, -1.0, 1.0};
 // boundaries only to each stage's input), while the interior is the state
 // after the fourth stage; at the domain's radial boundary faces the limiter
 // therefore divides by the one-step change of the first cell, which FP32
-// storage cannot resolve.  There the face flux is formed from U^n (mirrored
+// cannot resolve.  There the face flux is formed in FP64 from U^n (mirrored
 // with the parities of ModBoundary_Dynamo_primitives, 3_6/ModBoundary.f90:
-// 16-52) and the FP64 state, as the FP64 path forms it.  Fluxes, speeds and
-// the update are computed in FP64 for either storage type T.
+// 16-52) and the FP64 state, as the FP64 path forms it.  Elsewhere the FP32
+// path's fluxes are computed in FP32: on the 50 M-cell shell FP64 fluxes
+// changed the 100-step FP32-vs-FP64 difference only from 5.2e-7 to 5.5e-7
+// (largest variable) and cost 0.8 ms per step.
 
 template <typename T, typename TIN, int MODE, int TL, bool PERSIST>
 __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, const TIN *__restrict__ in,
@@ -420,8 +422,8 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
     // theta and phi face fluxes of the plane, each computed once: fj[v][y][x]
     // is the flux through the lower theta face of tile cell (y, x) (row TJa:
     // the upper face of the last row), fk[v][y][x] through its lower phi face
-    __shared__ double fj[8][TL + 1][TL];
-    __shared__ double fk[8][TL][TL + 1];
+    __shared__ T fj[8][TL + 1][TL];
+    __shared__ T fk[8][TL][TL + 1];
     __shared__ double red[4];
     // offset of the cell read in place of a ghost in direction code c, relative
     // to the ghost's own offset (0 where the ghost itself is read); set before
@@ -453,10 +455,10 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
         m.init(d, in, b, j0, k0, TJa, TKa, pull);
         const long long rb = (long long)b * d.NI, tb = (long long)b * d.NJ;
         const BlockMeta mt = t.meta[b];
-        // fluxes and speeds are computed in FP64 whatever the storage type T
-        const double dxi = mt.dxi, dxj = mt.dxj, dxk = mt.dxk;
+        const T dxi = (T)mt.dxi, dxj = (T)mt.dxj, dxk = (T)mt.dxk;
         double hmin = INFINITY;
         const double sint64 = m.act ? t.sint[tb + m.J] : 1.0;
+        const T sintc = m.act ? (T)t.sint[tb + m.J] : T(1);
         const double dt = MODE == 0 ? *dtp : 0.0;
 
         T win[8][5];
@@ -465,7 +467,7 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
         // same four values, same speed sum (speed at the upper cell first), so it
         // is carried instead of recomputed.  The same holds for the theta and phi
         // faces between neighbouring lanes, which go through fj / fk.
-        double fr[8];
+        T fr[8];
         bool have_fr = false;
         // FP32 path: U before the diffusion on the previous plane (the top
         // boundary face reads it after that plane has been updated in place)
@@ -477,12 +479,12 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
     #pragma unroll
             for (int q = 0; q < 5; q++) win[v][q] = m.act ? m.col(v, q) : T(0);
             hn[v] = m.hv ? m.halo(v, d.ng) : T(0);
-            fr[v] = 0.0;
+            fr[v] = T(0);
         }
         const TIN *Ub = in + (long long)b * 8 * d.P;
         // a ghost in a direction listed in the pull table is the same-level
         // neighbour's interior cell, translated by n
-        auto at = [&](int v, int II, int JJ, int KK) -> double {
+        auto at = [&](int v, int II, int JJ, int KK) -> T {
             long long o = (long long)v * d.P + (long long)II * m.SI + (long long)JJ * m.SJ + KK;
             if (PULL) {
                 const int c = (II < d.ng ? 0 : (II >= d.ng + d.ni ? 18 : 9)) +
@@ -490,7 +492,7 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
                               (KK < d.ng ? 0 : (KK >= d.ng + d.nk ? 2 : 1));
                 o += pdel[c];
             }
-            return (double)Ub[o];
+            return (T)Ub[o];
         };
         // faces on the tile's lower edges: lanes 0 .. TL-1 take the theta face
         // below row 0 in column x = lane, lanes TL .. 2TL-1 the phi face left of
@@ -512,8 +514,8 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
                 hn[v] = (m.hv && more) ? m.halo(v, I + 1) : T(0);
             }
             __syncthreads();
-            const double csI2 = t.cs[rb + I - 2];
-            auto Lv = [&](int v, int dj, int dk) -> double { return (double)lds[v][ty + 2 + dj][tx + 2 + dk]; };
+            const T csI2 = (T)t.cs[rb + I - 2];
+            auto Lv = [&](int v, int dj, int dk) -> T { return lds[v][ty + 2 + dj][tx + 2 + dk]; };
             // U^n of this plane, loaded before the face exchange (FP32 path, MODE 0)
             constexpr bool UEARLY = MODE == 0 && sizeof(T) == 4;
             double ucur[8];
@@ -525,16 +527,16 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
                     for (int v = 0; v < 8; v++) ucur[v] = U[base + (long long)v * d.P];
                 }
                 // speeds |v_d| + c_s at the cells the reference reads (see above)
-                const double c20 = fabs(at(2, I - 2, J, K - 2)) + csI2;
-                const double c2p = fabs(at(2, I - 2, J + 1, K - 2)) + csI2;
-                const double c30 = fabs(at(3, I - 2, J - 2, K)) + csI2;
-                const double c3p = fabs(at(3, I - 2, J - 2, K + 1)) + csI2;
+                const T c20 = fabs(at(2, I - 2, J, K - 2)) + csI2;
+                const T c2p = fabs(at(2, I - 2, J + 1, K - 2)) + csI2;
+                const T c30 = fabs(at(3, I - 2, J - 2, K)) + csI2;
+                const T c3p = fabs(at(3, I - 2, J - 2, K + 1)) + csI2;
     #pragma unroll
                 for (int v = 0; v < 8; v++) {
                     // upper theta face (the lower one of the cell at row + 1)
-                    fj[v][ty + 1][tx] = (face_flux(Lv(v, -1, 0), Lv(v, 0, 0), Lv(v, 1, 0), Lv(v, 2, 0)) * (c2p + c20)) * 0.5;
+                    fj[v][ty + 1][tx] = (face_flux(Lv(v, -1, 0), Lv(v, 0, 0), Lv(v, 1, 0), Lv(v, 2, 0)) * (c2p + c20)) * T(0.5);
                     // upper phi face
-                    fk[v][ty][tx + 1] = (face_flux(Lv(v, 0, -1), Lv(v, 0, 0), Lv(v, 0, 1), Lv(v, 0, 2)) * (c3p + c30)) * 0.5;
+                    fk[v][ty][tx + 1] = (face_flux(Lv(v, 0, -1), Lv(v, 0, 0), Lv(v, 0, 1), Lv(v, 0, 2)) * (c3p + c30)) * T(0.5);
                 }
             }
             if (bact) {
@@ -547,12 +549,12 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
                 const int bvar = bj ? 2 : 3;
                 const int bJ0 = bj ? j0 + d.ng : j0 + by + d.ng - 2, bK0 = bj ? k0 + bx + d.ng - 2 : k0 + d.ng;
                 const int bJ1 = bj ? bJ0 - 1 : bJ0, bK1 = bj ? bK0 : bK0 - 1;
-                const double s0 = fabs(at(bvar, I - 2, bJ0, bK0)) + csI2;
-                const double s1 = fabs(at(bvar, I - 2, bJ1, bK1)) + csI2;
+                const T s0 = fabs(at(bvar, I - 2, bJ0, bK0)) + csI2;
+                const T s1 = fabs(at(bvar, I - 2, bJ1, bK1)) + csI2;
     #pragma unroll
                 for (int v = 0; v < 8; v++) {
                     const T *pv = &lds[v][0][0] + bstart;
-                    const double f = (face_flux<double>(pv[0], pv[bstep], pv[2 * bstep], pv[3 * bstep]) * (s0 + s1)) * 0.5;
+                    const T f = (face_flux(pv[0], pv[bstep], pv[2 * bstep], pv[3 * bstep]) * (s0 + s1)) * T(0.5);
                     if (bj) fj[v][0][bx] = f;
                     else fk[v][by][0] = f;
                 }
@@ -560,36 +562,36 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
             __syncthreads();
             if (m.act) {
                 const int J = m.J, K = m.K;
-                const double csI = t.cs[rb + I], csIm = t.cs[rb + I - 1], csIp = t.cs[rb + I + 1];
-                const double c1m = fabs(at(1, I - 1, J - 2, K - 2)) + csIm;
-                const double c10 = fabs(at(1, I, J - 2, K - 2)) + csI;
-                const double c1p = fabs(at(1, I + 1, J - 2, K - 2)) + csIp;
-                const double xic = t.xi[rb + I];
-                const double L2 = dxj * xic;
-                const double L3 = (dxk * xic) * sint64;
+                const T csI = (T)t.cs[rb + I], csIm = (T)t.cs[rb + I - 1], csIp = (T)t.cs[rb + I + 1];
+                const T c1m = fabs(at(1, I - 1, J - 2, K - 2)) + csIm;
+                const T c10 = fabs(at(1, I, J - 2, K - 2)) + csI;
+                const T c1p = fabs(at(1, I + 1, J - 2, K - 2)) + csIp;
+                const T xic = (T)t.xi[rb + I];
+                const T L2 = dxj * xic;
+                const T L3 = (dxk * xic) * sintc;
                 const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + (long long)J * m.SJ + K;
                 const long long nn = (long long)d.ni * d.nj * d.nk;
                 const long long rbase = (long long)b * 8 * nn + ((long long)i * d.nj + (j0 + ty)) * d.nk + (k0 + tx);
                 double vv = 0.0, bb2 = 0.0;   // |v|^2 and |B|^2 of the new state, summed in component order
     #pragma unroll
                 for (int v = 0; v < 8; v++) {
-                    double r = 0.0;
-                    double f0 = have_fr ? fr[v]
-                                        : (face_flux<double>(win[v][0], win[v][1], win[v][2], win[v][3]) * (c10 + c1m)) * 0.5;
-                    double f1 = (face_flux<double>(win[v][1], win[v][2], win[v][3], win[v][4]) * (c1p + c10)) * 0.5;
+                    T r = T(0);
+                    T f0 = have_fr ? fr[v]
+                                   : (face_flux(win[v][0], win[v][1], win[v][2], win[v][3]) * (c10 + c1m)) * T(0.5);
+                    T f1 = (face_flux(win[v][1], win[v][2], win[v][3], win[v][4]) * (c1p + c10)) * T(0.5);
                     if (BFACE && bot && i == 0) {
                         // lower face of the first cell: ghosts 0 and -1 mirror cells 1 and 2 of U^n
                         const long long q = base + (long long)v * d.P;
                         const double p = kParBottom[v];
                         const double g0 = p * Uold[q], gm1 = p * Uold[q + m.SI];
-                        f0 = (face_flux<double>(gm1, g0, ucur[v], U[q + m.SI]) * (c10 + c1m)) * 0.5;
+                        f0 = (T)((face_flux<double>(gm1, g0, ucur[v], U[q + m.SI]) * (double)(c10 + c1m)) * 0.5);
                     }
                     if (BFACE && topb && i == d.ni - 1) {
                         // upper face of the last cell: ghosts ni+1 and ni+2 mirror cells ni and ni-1 of U^n
                         const long long q = base + (long long)v * d.P;
                         const double p = kParTop[v];
                         const double g1 = p * Uold[q], g2 = p * Uold[q - m.SI];
-                        f1 = (face_flux<double>(uprev[v], ucur[v], g1, g2) * (c1p + c10)) * 0.5;
+                        f1 = (T)((face_flux<double>(uprev[v], ucur[v], g1, g2) * (double)(c1p + c10)) * 0.5);
                     }
                     fr[v] = f1;
                     r = r + (f0 - f1) / dxi;
@@ -597,7 +599,7 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
                     r = r + (fk[v][ty][tx] - fk[v][ty][tx + 1]) / L3;
                     if (MODE == 0) {
                         const long long q = base + (long long)v * d.P;
-                        const double x = (UEARLY ? ucur[v] : U[q]) + (0.5 * dt) * r;
+                        const double x = (UEARLY ? ucur[v] : U[q]) + (0.5 * dt) * (double)r;
                         out[q] = x;
                         if (shadow) shadow[q] = (float)x;
                         if (v == 1) vv = x * x;
@@ -605,7 +607,7 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
                         if (v == 4) bb2 = x * x;
                         if (v == 5 || v == 6) bb2 = bb2 + x * x;
                     } else {
-                        out[rbase + (long long)v * nn] = r;
+                        out[rbase + (long long)v * nn] = (double)r;
                     }
                 }
                 if (MODE == 0) {

@@ -367,3 +367,34 @@ def test_fp32_error_on_the_50m_cell_shell():
                 assert np.all(rel <= 1e-6) and np.all(above == 0), (n, name, rel)
     for s in (ref, own, shared):
         s.close()
+
+
+def test_fp64_1000_steps_on_the_64x128x384_shell_match_the_oracle():
+    """FP64 path on the 64x128x384-per-panel shell in the reference's own block
+    layout (128 blocks of 64x16x48): per-block SHA-256 of the state after 1,
+    100 and 1000 steps equal the committed digests of the C oracle, which the
+    reference harness reproduces bit for bit at steps 1 and 100
+    (tests/golden/make_cfg2_digests.py)."""
+    import json
+    import os
+    import make_cfg2_digests as cfg2
+    print("[fp64 1000 steps] layout=64x16x48 blocks x128", flush=True)
+    with open(cfg2.OUT) as f:
+        want = json.load(f)
+    s = Shell(cfg2.LAYOUT)
+    assert s.nblocks == 128
+    s.set_gc_all()
+    bgs, sts = cfg2.inputs(lambda ib: s.block_info(ib)[2][0], s.pad, s.nblocks)
+    for ib in range(s.nblocks):
+        s.set_background(ib, bgs[ib])
+        s.set_state(ib, sts[ib])
+    n = 0
+    for target in cfg2.STEPS:
+        dt = s.advance(target - n, cfg2.CFL, graph=True)
+        n = target
+        got = [cfg2.digest(s.get_state(ib)) for ib in range(s.nblocks)]
+        bad = [ib for ib in range(s.nblocks) if got[ib] != want["digests"][str(n)][ib]]
+        print(f"step {n}: dt {dt:.17g}, blocks differing: {len(bad)}", flush=True)
+        assert dt == want["dt"][str(n)]
+        assert not bad, bad[:10]
+    s.close()
