@@ -18,6 +18,10 @@
 ! ModCommunication / ModAdvance / ModCheck, stepping on the GPU (one rank).
 !
 ! Usage: [mpirun -n N] falcon_ref_harness <input.bin> <output.bin>
+! Input modes: 1 tables only; 2 tables, dumps and steps; 3 steps only (timing:
+! no output beyond the file header and the stepping loop's wall time); 4 the
+! reference's ModSave_AllCells output of the exchanged initial state, written
+! into the working directory (AllCells_00000000/), and no steps.
 program falcon_ref_harness
     use MPI
     use ModParameters,  only:   ni,nj,nk,ng,nvar,iGeometry,iEquation,MpiSize,MpiRank,&
@@ -35,6 +39,7 @@ program falcon_ref_harness
     use ModCommunication, only: ModCommunication_SetGCAll,ModCommunication_SendRecvGC_new
     use ModAdvance,       only: ModAdvance_rk4
     use ModCheck,         only: ModCheck_primitive
+    use ModSavePlot,      only: ModSave_AllCells
 #ifdef FALCON_DROPIN
     use ModFalconShim,    only: falcon_sync_to_host
 #endif
@@ -177,8 +182,13 @@ program falcon_ref_harness
         call falcon_sync_to_host(Tree)
 #endif
         call emit(5,0,0.0d0)
+        if (mode==4) then
+            call ModSave_AllCells(Tree,0,77)
+            nsteps=0
+        end if
 
         ! single-operator dumps on the exchanged initial state
+        if (mode==2) then
         do ib=1,Tree%nLocalBlocks
             B=>Tree%LocalBlocks(ib)
             B%primitive=>B%primitive_IV
@@ -198,6 +208,7 @@ program falcon_ref_harness
         do ib=1,Tree%nLocalBlocks
             Tree%LocalBlocks(ib)%EQN_update_R_IV=0.0d0
         end do
+        end if
 
         idump=1
         call MPI_BARRIER(MPI_COMM_WORLD,ierr)
@@ -255,6 +266,7 @@ contains
         real(8),intent(in)          ::  dt
         integer                     ::  r,uo,ib,it
         type(BlockType),pointer     ::  B
+        if (mode==3) return
         do r=0,MpiSize-1
             if (r==MpiRank) then
                 open(newunit=uo,file=trim(fout),access='stream',form='unformatted',status='old',position='append')

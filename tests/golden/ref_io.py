@@ -134,9 +134,20 @@ def read_output(path, layout, mode, ndumps):
     return out
 
 
+def raise_stack_limit():
+    """The reference keeps whole-block temporaries on the stack (several MB per
+    array for 64x16x48 blocks): raise this process's soft stack limit to its
+    hard limit, which the harness processes it starts inherit."""
+    import resource
+    soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+    if soft != hard:
+        resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
+
 def run_harness(layout, cfl=0.5, nsteps=0, dumps=(), mode=1, blocks_bg=None, blocks_state=None, exe=None,
                 launcher=()):
     """Run the harness (launcher: e.g. ["mpirun", "-n", "2"]) and read its output."""
+    raise_stack_limit()
     with tempfile.TemporaryDirectory() as td:
         fi, fo = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         write_input(fi, layout, cfl, nsteps, list(dumps), mode, blocks_bg, blocks_state)
