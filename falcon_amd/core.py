@@ -65,6 +65,8 @@ def load():
         "falcon_compute_dt": [vp, C.c_double, dp],
         "falcon_set_fixed_dt": [vp, C.c_double],
         "falcon_check_state": [vp, C.c_int, i64p, ip],
+        "falcon_save_allcells": [vp, C.c_int, C.c_char_p, C.c_int],
+        "falcon_format_es16": [dp, C.c_int, C.c_char_p],
         "falcon_compute_dt_noalloc": [vp, C.c_double, dp],
         "falcon_state_diff": [vp, vp, C.c_double, dp, dp, i64p, i64p],
         "falcon_rhs": [vp, dp],
@@ -311,6 +313,13 @@ class Shell:
         _check(self.L.falcon_check_state(self.h, int(which), C.byref(n), _p(w, C.c_int32)))
         return n.value, (tuple(int(x) for x in w) if n.value else None)
 
+    def save_allcells(self, step, outdir, merged=False):
+        """ModSave_AllCells output of the HBM state: outdir/AllCells_<step>/ with
+        info.dat and Block_<label>.dat files (merged: merged.dat, the Tecplot
+        file of merge_AllCells.py); returns that directory."""
+        _check(self.L.falcon_save_allcells(self.h, int(step), os.fsencode(outdir), 1 if merged else 0))
+        return os.path.join(outdir, "AllCells_%08d" % int(step))
+
     def set_fixed_dt(self, dt):
         """dt > 0: later steps use this dt instead of the CFL dt; 0 restores it."""
         _check(self.L.falcon_set_fixed_dt(self.h, float(dt)))
@@ -361,6 +370,14 @@ class Shell:
     @property
     def interior_cells(self):
         return self.nblocks * self.ni * self.nj * self.nk
+
+
+def format_es16(values):
+    """Fortran ES16.8 fields of the values (the AllCells text edit), one string."""
+    v = np.ascontiguousarray(np.atleast_1d(values), dtype=np.float64)
+    buf = C.create_string_buffer(16 * v.size + 1)
+    _check(load().falcon_format_es16(_p(v), int(v.size), buf))
+    return buf.value.decode()
 
 
 class HeatSolver:

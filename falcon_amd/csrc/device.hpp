@@ -102,6 +102,20 @@ void launch_nan_scan(const Dims &d, const TS *st, unsigned long long *w, hipStre
 // the same 32 words w, whose first 8 and last 8 start at 0 and at ~0
 void launch_state_diff(const Dims &d, const double *a, const double *b, double tol, unsigned long long *w, int pass,
                        hipStream_t s);
+// one Fourier mode of the synthetic vector potential: B contribution
+// a cos(k.x + ph), a = k x (random amplitude)
+struct SynthMode {
+    double k[3], a[3], ph;
+};
+// seeded synthetic initial state on the interior (k_synthetic); rng: 6 per
+// block, rows: c_p, sqrt(rho0), c_s (NI each) per block; wgmax: one double
+// per 256 interior cells
+void launch_synthetic(const Dims &d, const double *rng, const int32_t *yin, const double *rows, const SynthMode *modes,
+                      int nm, unsigned long long seed, double s_amp, double b_ratio, double *U, double *wgmax,
+                      hipStream_t s);
+// corner values of the AllCells output for blocks [b0, b0 + nb) (k_allcells)
+void launch_allcells(const Dims &d, const double *U, const double *geo, const int32_t *yin, int b0, int nb,
+                     double rsun, double *out, hipStream_t s);
 // single-operator entry points used by parity tests: R only
 template <typename T, typename TIN>
 void launch_rhs_only(const Dims &d, const DevTables &t, const TIN *in, double *R, hipStream_t s, int tile, int wgs);

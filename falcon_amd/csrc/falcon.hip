@@ -7,6 +7,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <filesystem>
+#include <fstream>
 #include <memory>
 #include <mutex>
 #include <random>
@@ -20,6 +22,7 @@
 #include "cnn.hpp"
 #include "heat.hpp"
 #include "layout.hpp"
+#include "output.hpp"
 
 // Scratch of falcon_compute_dt_noalloc, in device memory the library never
 // allocates: the dt partial minima (one per tile of the diffusion's edge, at
@@ -29,8 +32,7 @@ constexpr int DT_NOALLOC_MAX_PARTIALS = 1 << 14;
 static __device__ __align__(256) double scratch[DT_NOALLOC_MAX_PARTIALS + 32];
 
 namespace falcon {
-void synthetic_state(const Layout &lay, const std::vector<double> &bg, uint64_t seed, double s_amp, double b_ratio,
-                     int nthreads, std::vector<double> &state);
+void synthetic_modes(const Tree &t, uint64_t seed, std::vector<double> &modes);
 }
 
 using namespace falcon;
@@ -975,22 +977,51 @@ int falcon_init_synthetic(falcon_handle_t h, uint64_t seed, double s_amp, double
     return guard([&] {
         h->need_tables();
         ensure_device_tables(h);
-        std::vector<double> st;
-        synthetic_state(h->lay, h->bg, seed, s_amp, b_ratio, host_threads(), st);
-        const int nb = h->dims.nblocks;
-        const size_t P = (size_t)h->dims.P;
-        std::vector<double> dev((size_t)nb * 8 * P);
-        std::vector<std::thread> th;
-        int nt = std::min(host_threads(), nb);
-        for (int w = 0; w < nt; w++)
-            th.emplace_back([&, w]() {
-                for (int b = w; b < nb; b += nt)
-                    fortran_to_device(st.data() + (size_t)b * 8 * P, dev.data() + (size_t)b * 8 * P, h->NI(), h->NJ(), h->NK());
-            });
-        for (auto &x : th) x.join();
-        HIPCHECK(hipMemcpy(h->U[h->cur].p, dev.data(), sizeof(double) * dev.size(), hipMemcpyHostToDevice));
-        launch_exchange<double>(h->dims, h->gh64, h->U[h->cur].as<double>(), h->stream);
-        launch_exchange<double>(h->dims, h->gh64, h->U[h->cur].as<double>(), h->stream);
+        const Dims &d = h->dims;
+        const int nb = d.nblocks, NI = h->NI();
+        // per block: its ranges, Yin flag and the rows c_p = Gamma3/(Gamma3-1)
+        // p0/(rho0 T0), sqrt(rho0), c_s = sqrt(Gamma1 p0/rho0) of its background
+        std::vector<double> rng((size_t)6 * nb), rows((size_t)3 * nb * NI), modes;
+        std::vector<int32_t> yin(nb);
+        for (int b = 0; b < nb; b++) {
+            const BlockGeom &g = h->lay.geom[b];
+            for (int a = 0; a < 3; a++) { rng[6 * b + 2 * a] = g.rng[a][0]; rng[6 * b + 2 * a + 1] = g.rng[a][1]; }
+            yin[b] = g.yin;
+            const double *B = h->bg.data() + (size_t)b * 8 * NI;
+            double *r = rows.data() + (size_t)3 * b * NI;
+            for (int I = 0; I < NI; I++) {
+                const double rho0 = B[I + NI], p0 = B[I + 2 * NI], T0 = B[I + 3 * NI];
+                const double g1 = B[I + 4 * NI], g3 = B[I + 5 * NI];
+                r[I] = g3 / (g3 - 1.0) * p0 / (rho0 * T0);
+                r[NI + I] = std::sqrt(rho0);
+                r[2 * NI + I] = std::sqrt(g1 * p0 / rho0);
+            }
+        }
+        synthetic_modes(h->lay.tree, seed, modes);
+        const int nm = (int)modes.size() / 7;
+        std::vector<SynthMode> sm(nm);
+        for (int m = 0; m < nm; m++) {
+            for (int a = 0; a < 3; a++) { sm[m].k[a] = modes[7 * m + a]; sm[m].a[a] = modes[7 * m + 3 + a]; }
+            sm[m].ph = modes[7 * m + 6];
+        }
+        const long long ncell = (long long)nb * d.ni * d.nj * d.nk;
+        DevBuf drng, dyin, drows, dmodes, dmax;
+        drng.alloc(sizeof(double) * rng.size());
+        dyin.alloc(sizeof(int32_t) * yin.size());
+        drows.alloc(sizeof(double) * rows.size());
+        dmodes.alloc(sizeof(SynthMode) * nm);
+        dmax.alloc(sizeof(double) * (size_t)((ncell + 255) / 256));
+        HIPCHECK(hipMemcpy(drng.p, rng.data(), drng.bytes, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(dyin.p, yin.data(), dyin.bytes, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(drows.p, rows.data(), drows.bytes, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(dmodes.p, sm.data(), dmodes.bytes, hipMemcpyHostToDevice));
+        double *U = h->U[h->cur].as<double>();
+        HIPCHECK(hipMemsetAsync(U, 0, sizeof(double) * (size_t)nb * 8 * d.P, h->stream));
+        launch_synthetic(d, drng.as<double>(), dyin.as<int32_t>(), drows.as<double>(), dmodes.as<SynthMode>(), nm, seed,
+                         s_amp, b_ratio, U, dmax.as<double>(), h->stream);
+        // the reference driver's two initial fills (3_6/test1.f90:143-144)
+        launch_exchange<double>(d, h->gh64, U, h->stream);
+        launch_exchange<double>(d, h->gh64, U, h->stream);
         h->partials_valid = false;
         h->shadow_valid = false;
         h->u_ghosts_valid = true;
@@ -1302,6 +1333,92 @@ int falcon_check_state(falcon_handle_t h, int which, int64_t *n_nan, int32_t *wh
                 where[0] = (int32_t)k;
             }
         }
+    });
+}
+
+int falcon_save_allcells(falcon_handle_t h, int istep, const char *outdir, int merged) {
+    return guard([&] {
+        h->need_tables();
+        if (!h->dt.meta) throw Error("falcon_save_allcells: call falcon_set_gc_all and set a state first");
+        if (istep < 0 || istep > 99999999) throw Error("falcon_save_allcells: step must be in [0, 99999999]");
+        if (!outdir) throw Error("falcon_save_allcells: null directory");
+        ensure_device_tables(h);
+        refresh_u_ghosts(h);
+        const Dims &d = h->dims;
+        const int nb = d.nblocks;
+        AllCellsSpec spec{d.ni, d.nj, d.nk, 12, nb,
+                          "VARIABLES = \"X [Rs]\", \"Y [Rs]\", \"Z [Rs]\", \"Rho1\", \"Vx\", \"Vy\", \"Vz\", "
+                          "\"Bx\", \"By\", \"Bz\", \"S1\", \"PHI\""};
+        std::vector<double> geo;
+        std::vector<int32_t> yin;
+        allcells_geometry(h->lay, geo, yin);
+        DevBuf dgeo, dyin, dout;
+        dgeo.alloc(sizeof(double) * geo.size());
+        dyin.alloc(sizeof(int32_t) * yin.size());
+        HIPCHECK(hipMemcpy(dgeo.p, geo.data(), dgeo.bytes, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(dyin.p, yin.data(), dyin.bytes, hipMemcpyHostToDevice));
+        const size_t per = (size_t)(d.ni + 1) * (d.nj + 1) * (d.nk + 1) * 12;
+        const int chunk = (int)std::max<size_t>(1, std::min<size_t>(nb, ((size_t)256 << 20) / (per * sizeof(double))));
+        dout.alloc(sizeof(double) * per * chunk);
+        std::vector<double> host(per * chunk);
+        char stepname[16];
+        std::snprintf(stepname, sizeof(stepname), "%08d", istep);
+        const std::filesystem::path dir = std::filesystem::path(outdir) / (std::string("AllCells_") + stepname);
+        std::filesystem::create_directories(dir);
+        auto put = [&](const std::filesystem::path &p, const std::string &text, bool append) {
+            std::ofstream f(p, append ? std::ios::binary | std::ios::app : std::ios::binary | std::ios::trunc);
+            f.write(text.data(), (std::streamsize)text.size());
+            if (!f) throw Error("falcon_save_allcells: cannot write " + p.string());
+        };
+        put(dir / "info.dat", allcells_info(spec), false);
+        const std::filesystem::path mpath = dir / "merged.dat";
+        if (merged) {
+            const long long ncorner = (long long)nb * (d.ni + 1) * (d.nj + 1) * (d.nk + 1);
+            const long long ncell = (long long)nb * d.ni * d.nj * d.nk;
+            put(mpath, "TITLE = \"AllCells\"\n" + spec.variables + "\nZONE N=" + std::to_string(ncorner) +
+                           ", E=" + std::to_string(ncell) + ", F=FEPOINT, ET=BRICK\n", false);
+        }
+        const int nt = std::max(1, std::min(host_threads(), 16));
+        for (int b0 = 0; b0 < nb; b0 += chunk) {
+            const int n = std::min(chunk, nb - b0);
+            launch_allcells(d, h->U[h->cur].as<double>(), dgeo.as<double>(), dyin.as<int32_t>(), b0, n,
+                            (double)6.955e10f, dout.as<double>(), h->stream);
+            check_launch();
+            HIPCHECK(hipMemcpyAsync(host.data(), dout.p, sizeof(double) * per * n, hipMemcpyDeviceToHost, h->stream));
+            HIPCHECK(hipStreamSynchronize(h->stream));
+            std::vector<std::string> text(n);
+            std::vector<std::thread> th;
+            for (int w = 0; w < std::min(nt, n); w++)
+                th.emplace_back([&, w]() {
+                    for (int q = w; q < n; q += nt) allcells_block_text(spec, host.data() + per * q, text[q]);
+                });
+            for (auto &x : th) x.join();
+            for (int q = 0; q < n; q++) {
+                if (merged) {
+                    put(mpath, text[q], true);
+                } else {
+                    char name[32];
+                    std::snprintf(name, sizeof(name), "Block_%08d.dat", b0 + q + 1);
+                    put(dir / name, text[q], false);
+                }
+            }
+        }
+        if (merged) {
+            for (int b0 = 0; b0 < nb; b0 += 256) {
+                std::string conn;
+                allcells_connectivity(spec, b0, std::min(nb, b0 + 256), conn);
+                put(mpath, conn, true);
+            }
+            put(dir / ".merged", "", false);
+        }
+    });
+}
+
+int falcon_format_es16(const double *x, int n, char *out) {
+    return guard([&] {
+        if (n < 0 || (n > 0 && (!x || !out))) throw Error("falcon_format_es16: bad arguments");
+        for (int q = 0; q < n; q++) format_es16(x[q], out + 16 * q);
+        out[16 * n] = '\0';
     });
 }
 

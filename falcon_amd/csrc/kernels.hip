@@ -1172,6 +1172,190 @@ void launch_to_f32(const double *src, float *dst, long long n, hipStream_t s) {
     hipLaunchKernelGGL(k_to_f32, dim3(4096), dim3(256), 0, s, src, dst, n);
 }
 
+// ---------------------------------------------------------------- initial state
+
+// Seeded synthetic initial state on the interior of every block (the
+// configured runs' initial condition; no reference counterpart beyond
+// 3_6/ModInitiation.f90:14-188, which sets other fields): rho1 = v = 0;
+// s1 = s_amp * c_p * U(-1,1) with U from a splitmix64 hash of (seed, block,
+// cell); B = curl A of a smooth random vector potential, the sum over modes
+// of (k x a) cos(k.x + phase) evaluated in global Cartesian coordinates (Yang
+// positions and components through the involution (x, y, z) -> (-x, z, y) of
+// ModYinYang_CoordConv, 3_6/ModYinYang.f90:8-17), before its scaling.
+// rows per block (3 x NI): c_p, sqrt(rho0), c_s.  wgmax[workgroup]: the
+// largest |B| / (sqrt(rho0) c_s) of its cells, for the scale.
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__global__ void __launch_bounds__(256) k_synthetic(Dims d, const double *__restrict__ rng,
+                                                   const int32_t *__restrict__ yin, const double *__restrict__ rows,
+                                                   const SynthMode *__restrict__ modes, int nm, unsigned long long seed,
+                                                   double s_amp, double *__restrict__ U, double *__restrict__ wgmax) {
+    __shared__ double red[4];
+    const long long n3 = (long long)d.ni * d.nj * d.nk;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double ratio = 0.0;
+    if (t < (long long)d.nblocks * n3) {
+        const int b = (int)(t / n3);
+        const long long c = t - (long long)b * n3;
+        // Fortran indices, i fastest
+        const int i = (int)(c % d.ni) + 1, j = (int)((c / d.ni) % d.nj) + 1, k = (int)(c / ((long long)d.ni * d.nj)) + 1;
+        const double *rg = rng + 6 * (long long)b;
+        auto centre = [](double lo, double hi, int n, int f) {
+            return (lo * (double)((float)(n - f) + 0.5f) + hi * (double)((float)f - 0.5f)) / (double)n;
+        };
+        const double r = centre(rg[0], rg[1], d.ni, i), th = centre(rg[2], rg[3], d.nj, j);
+        const double ph = centre(rg[4], rg[5], d.nk, k);
+        const double sth = sin(th), cth = cos(th), sph = sin(ph), cph = cos(ph);
+        const bool y = yin[b] != 0;
+        const double p0 = r * sth * cph, p1 = r * sth * sph, p2 = r * cth;
+        const double x0 = y ? p0 : -p0, x1 = y ? p1 : p2, x2 = y ? p2 : p1;
+        double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+        for (int m = 0; m < nm; m++) {
+            const SynthMode &M = modes[m];
+            const double cc = cos(M.k[0] * x0 + M.k[1] * x1 + M.k[2] * x2 + M.ph);
+            g0 += M.a[0] * cc; g1 += M.a[1] * cc; g2 += M.a[2] * cc;
+        }
+        const double l0 = y ? g0 : -g0, l1 = y ? g1 : g2, l2 = y ? g2 : g1;
+        const double Br = l0 * (sth * cph) + l1 * (sth * sph) + l2 * cth;
+        const double Bt = l0 * (cth * cph) + l1 * (cth * sph) + l2 * (-sth);
+        const double Bp = l0 * (-sph) + l1 * cph + l2 * 0.0;
+        const int I = i + d.ng - 1;
+        const double *rw = rows + 3 * (long long)b * d.NI;
+        const double cp = rw[I], srho = rw[d.NI + I], cs = rw[2 * d.NI + I];
+        ratio = sqrt(Br * Br + Bt * Bt + Bp * Bp) / srho / cs;
+        const unsigned long long id = ((unsigned long long)b << 32) ^ (unsigned long long)((i * 131 + j) * 1031 + k);
+        const double u = (double)(splitmix64(seed ^ (id * 0x2545F4914F6CDD1Dull)) >> 11) * (1.0 / 9007199254740992.0);
+        const long long q = (long long)b * 8 * d.P + ((long long)I * d.NJ + (j + d.ng - 1)) * d.NK + (k + d.ng - 1);
+        U[q + 4 * d.P] = Br;
+        U[q + 5 * d.P] = Bt;
+        U[q + 6 * d.P] = Bp;
+        U[q + 7 * d.P] = s_amp * cp * (2.0 * u - 1.0);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_down(ratio, off, 64);
+        ratio = o > ratio ? o : ratio;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ratio;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mx = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) mx = red[w] > mx ? red[w] : mx;
+        wgmax[blockIdx.x] = mx;
+    }
+}
+
+// scale = b_ratio / max over the workgroup maxima (0 where the field is 0),
+// then B *= scale on every interior cell
+__global__ void __launch_bounds__(256) k_synthetic_scale(Dims d, const double *__restrict__ wgmax, int nwg,
+                                                         double b_ratio, double *__restrict__ U) {
+    __shared__ double sc;
+    if (threadIdx.x == 0) {
+        double mx = 0.0;
+        for (int w = 0; w < nwg; w++) mx = wgmax[w] > mx ? wgmax[w] : mx;
+        sc = mx > 0.0 ? b_ratio / mx : 0.0;
+    }
+    __syncthreads();
+    const double scale = sc;
+    const long long n3 = (long long)d.ni * d.nj * d.nk;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < (long long)d.nblocks * n3;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int b = (int)(t / n3);
+        const long long c = t - (long long)b * n3;
+        const int i = (int)(c % d.ni), j = (int)((c / d.ni) % d.nj), k = (int)(c / ((long long)d.ni * d.nj));
+        const long long q = (long long)b * 8 * d.P + ((long long)(i + d.ng) * d.NJ + (j + d.ng)) * d.NK + (k + d.ng);
+        for (int v = 4; v < 7; v++) U[q + v * d.P] *= scale;
+    }
+}
+
+void launch_synthetic(const Dims &d, const double *rng, const int32_t *yin, const double *rows, const SynthMode *modes,
+                      int nm, unsigned long long seed, double s_amp, double b_ratio, double *U, double *wgmax,
+                      hipStream_t s) {
+    const long long n = (long long)d.nblocks * d.ni * d.nj * d.nk;
+    const int nwg = (int)((n + 255) / 256);
+    hipLaunchKernelGGL(k_synthetic, dim3((unsigned)nwg), dim3(256), 0, s, d, rng, yin, rows, modes, nm, seed, s_amp, U,
+                       wgmax);
+    hipLaunchKernelGGL(k_synthetic_scale, dim3(2048), dim3(256), 0, s, d, wgmax, nwg, b_ratio, U);
+}
+
+// ---------------------------------------------------------------- output
+
+// Corner values of ModSave_AllCells (3_6/ModSavePlot.f90:585-640) for blocks
+// [b0, b0 + nb): one thread per corner (i, j, k), i fastest, writing the
+// line's 12 numbers to out[(b - b0)][corner][12]: the corner's Cartesian
+// position in solar radii, then the 9 variables of the reference's state
+// averaged over the 8 surrounding cells (ghosts included, summed in the
+// reference's order), with v and B turned to Cartesian components; psi is 0
+// (the state holds no psi: no term writes it with DivB_option = 0).
+// geo per block: face radii xi_F (ni+1), then sin and cos of the theta faces
+// (nj+1 each) and of the phi faces (nk+1 each), from the host's libm.
+__global__ void __launch_bounds__(256) k_allcells(Dims d, const double *__restrict__ U,
+                                                  const double *__restrict__ geo, const int32_t *__restrict__ yin,
+                                                  int b0, int nb, double rsun, double *__restrict__ out) {
+    const int ci = d.ni + 1, cj = d.nj + 1, ck = d.nk + 1;
+    const long long nc = (long long)ci * cj * ck;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nb * nc) return;
+    const int lb = (int)(t / nc), b = b0 + lb;
+    const long long c = t - (long long)lb * nc;
+    const int i = (int)(c % ci) + 1, j = (int)((c / ci) % cj) + 1, k = (int)(c / ((long long)ci * cj)) + 1;
+    const long long G = ci + 2 * cj + 2 * ck;
+    const double *g = geo + (long long)b * G;
+    const double r = g[i - 1];
+    const double sth = g[ci + j - 1], cth = g[ci + cj + j - 1];
+    const double sph = g[ci + 2 * cj + k - 1], cph = g[ci + 2 * cj + ck + k - 1];
+    // padded index of Fortran index f: f + ng - 1
+    const int I = i + d.ng - 1, J = j + d.ng - 1, K = k + d.ng - 1;
+    auto at = [&](int v, int II, int JJ, int KK) -> double {
+        return U[(long long)b * 8 * d.P + (long long)v * d.P + ((long long)II * d.NJ + JJ) * d.NK + KK];
+    };
+    double o[12];
+    const bool y = yin[b] != 0;
+    if (y) {
+        o[0] = (r * sth) * cph; o[1] = (r * sth) * sph; o[2] = r * cth;
+    } else {
+        o[0] = -((r * sth) * cph); o[1] = r * cth; o[2] = (r * sth) * sph;
+    }
+    o[0] = o[0] / rsun; o[1] = o[1] / rsun; o[2] = o[2] / rsun;
+    #pragma unroll
+    for (int v = 0; v < 8; v++) {
+        double s = at(v, I - 1, J - 1, K - 1) + at(v, I, J - 1, K - 1);
+        s = s + at(v, I - 1, J, K - 1);
+        s = s + at(v, I, J, K - 1);
+        s = s + at(v, I - 1, J - 1, K);
+        s = s + at(v, I, J - 1, K);
+        s = s + at(v, I - 1, J, K);
+        s = s + at(v, I, J, K);
+        o[3 + v] = 0.125 * s;
+    }
+    o[11] = 0.0;
+    // (vr, vt, vp) and (br, bt, bp) to Cartesian (:614-637); Yang: (-x, z, y)
+    #pragma unroll
+    for (int q = 0; q < 2; q++) {
+        double *w = o + 4 + 3 * q;
+        const double x0 = ((w[0] * sth) * cph + (w[1] * cth) * cph) - w[2] * sph;
+        const double y0 = ((w[0] * sth) * sph + (w[1] * cth) * sph) + w[2] * cph;
+        const double z0 = w[0] * cth - w[1] * sth;
+        if (y) { w[0] = x0; w[1] = y0; w[2] = z0; }
+        else { w[0] = -x0; w[1] = z0; w[2] = y0; }
+    }
+    double *dst = out + t * 12;
+    #pragma unroll
+    for (int q = 0; q < 12; q++) dst[q] = o[q];
+}
+
+void launch_allcells(const Dims &d, const double *U, const double *geo, const int32_t *yin, int b0, int nb,
+                     double rsun, double *out, hipStream_t s) {
+    const long long n = (long long)nb * (d.ni + 1) * (d.nj + 1) * (d.nk + 1);
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_allcells, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, U, geo, yin, b0, nb, rsun,
+                       out);
+}
+
 // explicit instantiations: FP64 path, FP32-deviation path
 template void launch_stage<double, double, double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, const double *, double, hipStream_t, const int32_t *, int, int);
 template void launch_stage<float, float, float, float>(const Dims &, const DevTables &, const float *, const float *, float *, float *, const double *, double, hipStream_t, const int32_t *, int, int);

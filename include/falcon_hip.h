@@ -138,6 +138,20 @@ int falcon_advance(falcon_handle_t h, int nsteps, double cfl, int use_graph, dou
  * theta, r, variable innermost), or -1s.  The Fortran shim turns n_nan > 0
  * into the reference's message and `stop 1` (its MPI_Abort). */
 int falcon_check_state(falcon_handle_t h, int which, int64_t *n_nan, int32_t *where);
+/* ModSave_AllCells (3_6/ModSavePlot.f90:520-650): writes
+ * <outdir>/AllCells_<8-digit step>/info.dat and one Block_<8-digit label>.dat
+ * per block, each line the corner's Cartesian position in solar radii and the
+ * 9 variables averaged over its 8 surrounding cells (v, B in Cartesian
+ * components, psi = 0), in Fortran ES16.8 fields, byte for byte as the
+ * reference's writer.  The corner values are formed on the device from the
+ * HBM state (ghosts refreshed first); the host formats and writes.  merged =
+ * 1 writes merged.dat (the Tecplot FEPOINT BRICK file of
+ * 3_6/merge_AllCells.py) and .merged in place of the block files. */
+int falcon_save_allcells(falcon_handle_t h, int istep, const char *outdir, int merged);
+/* n values as Fortran ES16.8 fields: out receives 16*n characters and a
+ * terminating NUL.  The edit the AllCells writer uses; no counterpart in the
+ * reference beyond Fortran's own formatted output. */
+int falcon_format_es16(const double *x, int n, char *out);
 /* dt > 0: every later step uses this dt in place of CFL * min(h / v_wave)
  * (ModTimeStep_Dynamo, 3_6/ModTimeStep.f90:10-30); 0 restores the CFL dt.  For
  * stepping two precisions with one dt sequence.  No counterpart in the

@@ -9,7 +9,7 @@ import pytest
 
 import pyoracle
 from conftest import random_inputs, rel_err
-from falcon_amd import Shell, configs
+from falcon_amd import R_SUN, Shell, configs
 
 pytestmark = pytest.mark.gpu
 
@@ -400,3 +400,29 @@ def test_fp64_1000_steps_on_the_64x128x384_shell_match_the_oracle():
         assert dt == want["dt"][str(n)]
         assert not bad, bad[:10]
     s.close()
+
+
+@pytest.mark.parametrize("name", ["cubic8", "mixed"])
+def test_synthetic_state_on_device_matches_the_restatement(name):
+    """k_synthetic against oracle/synthetic_oracle.py on the same block
+    centres and background rows: rho' = v = 0 and s' bit for bit (integer
+    hash, correctly rounded products), B to 1e-12 of its largest magnitude
+    (device and numpy sin/cos differ in the last bits)."""
+    import synthetic_oracle
+    print(f"[synthetic vs restatement] layout={name}", flush=True)
+    s = Shell(LAYOUTS[name])
+    s.background_polytrope(20.0)
+    s.set_gc_all()
+    s.init_synthetic(seed=77, s_amp=1e-4, b_ratio=1e-3)
+    blocks = []
+    for ib in range(s.nblocks):
+        _, fl, (xi, xj, xk) = s.block_info(ib)
+        blocks.append((xi, xj, xk, int(fl[0]), s.get_background(ib)))
+    want = synthetic_oracle.state(blocks, LAYOUTS[name]["r_rsun"][1] * R_SUN, 77, 1e-4, 1e-3)
+    got = [s.get_state(ib)[:, 2:-2, 2:-2, 2:-2] for ib in range(s.nblocks)]
+    s.close()
+    bmax = max(float(np.abs(w[4:7]).max()) for w in want)
+    for ib in range(len(got)):
+        assert np.all(got[ib][0:4] == 0.0)
+        assert np.array_equal(got[ib][7], want[ib][7]), ib
+        assert np.max(np.abs(got[ib][4:7] - want[ib][4:7])) <= 1e-12 * bmax, ib
