@@ -30,6 +30,12 @@ namespace falcon {
 template <int TL>
 constexpr int tile_pl() { return TL + 4; }   // LDS plane edge (tile + 2 halo each side)
 
+// radial coefficient rows the stage kernel stages in LDS, and the largest padded
+// radial extent NI a march accepts (the launchers reject larger blocks)
+constexpr int kStageRows = 10;
+
+constexpr int kMaxNI = 132;
+
 // ModDerivative_1st_O4_3D (3_6/ModDerivative.f90:16-19):
 // (1./12./dx)*( -f(+2) + 8.*f(+1) - 8.*f(-1) + f(-2) )
 template <typename T>
@@ -84,9 +90,10 @@ struct Marcher {
     int SI, SJ;
     int ilo, ihi;           // first interior plane, first plane above the interior
     int J, K;           // padded column of this lane
+    long long coff;     // J*SJ + K, clamped into the block for lanes outside a partial tile
     bool act;
     bool hv;
-    long long hoff;     // halo point offset within a plane
+    long long hoff;     // halo point offset within a plane (a valid cell for lanes without a halo point)
     int hly, hlx;
 
     __device__ __forceinline__ void init(const Dims &d, const TIN *in, int b, int j0, int k0, int TJa, int TKa,
@@ -108,6 +115,9 @@ struct Marcher {
         act = ty < TJa && tx < TKa;
         J = j0 + ty + d.ng;
         K = k0 + tx + d.ng;
+        // loads of inactive lanes read the tile's first cell, so every load can be
+        // issued without a branch; their values are never used
+        coff = act ? (long long)J * SJ + K : (long long)(j0 + d.ng) * SJ + (k0 + d.ng);
         const int h = threadIdx.x;
         hv = false; hly = 0; hlx = 0;
         if (h < 4 * TL) {
@@ -122,7 +132,7 @@ struct Marcher {
             hly = jj + 2;
             hv = jj < TJa;
         }
-        hoff = (long long)(j0 + hly) * SJ + (k0 + hlx);
+        hoff = hv ? (long long)(j0 + hly) * SJ + (k0 + hlx) : coff;
         if (PULL && pull && hv) {
             const int HJ = j0 + hly, HK = k0 + hlx;   // padded halo cell
             const int dj = HJ < d.ng ? -1 : (HJ >= d.ng + d.nj ? 1 : 0);
@@ -136,9 +146,9 @@ struct Marcher {
             }
         }
     }
-    // I is the same for every lane of the workgroup
+    // I is the same for every lane of the workgroup, 0 <= I < NI
     __device__ __forceinline__ T col(int v, int I) const {
-        long long o = (long long)v * P + (long long)I * SI + (long long)J * SJ + K;
+        long long o = (long long)v * P + (long long)I * SI + coff;
         if (PULL) o += (I < ilo ? dlo : 0ll) + (I >= ihi ? dhi : 0ll);
         return (T)blk[o];
     }
@@ -156,12 +166,22 @@ struct Marcher {
 // register window `win` of storage type TW (the FP32 path keeps its stage
 // values in FP32 and widens them here); L(v,dj,dk): the LDS plane, widened the
 // same way.
-template <typename T, typename TW, typename LDSF>
+// RECIP (FP32 path only): the divisions by rho0 and by sin(theta) become
+// products with irho0 = 1/rho0 and sinv = 1/sin(theta), the latter applied
+// after the phi derivative instead of to each of its four points.  Each is a
+// rounding-level change in FP64 (relative 1e-16, far below the FP32 storage
+// rounding of the stage values) and removes 14 FP64 divisions per cell.
+template <typename T, typename TW, bool RECIP, typename LDSF>
 __device__ __forceinline__ void cell_rhs(const TW (&win)[8][5], LDSF L, const T xr[5], const T rho0r[5], const T gpr[5],
                                          const T gqr[5], const T st[5], T rinv, T r2inv, T xir2, T heat, T gor,
-                                         T sinv, T cot, T cr, T ct, T cp, T R[8]) {
+                                         T sinv, T cot, T cr, T ct, T cp, T irho0, T R[8]) {
     auto W = [&](int v, int q) -> T { return (T)win[v][q]; };
     const T rho0c = rho0r[2];
+    auto over_rho0 = [&](T x) -> T { return RECIP ? x * irho0 : x / rho0c; };
+    // d4 of f/sin(theta) along phi (sin(theta) is constant along phi)
+    auto d4_over_st = [&](T f0, T f1, T f3, T f4) -> T {
+        return RECIP ? d4(f0, f1, f3, f4, cp) * sinv : d4(f0 / st[2], f1 / st[2], f3 / st[2], f4 / st[2], cp);
+    };
     const T vr = W(1, 2), vt = W(2, 2), vp = W(3, 2);
     const T br = W(4, 2), bt = W(5, 2), bp = W(6, 2);
 
@@ -214,15 +234,15 @@ __device__ __forceinline__ void cell_rhs(const TW (&win)[8][5], LDSF L, const T 
         T t3 = d4(p1L(0, -2), p1L(0, -1), p1L(0, 1), p1L(0, 2), cp);
         T g2 = t2 * rinv;
         T g3 = (t3 * rinv) * sinv;
-        R[1] = R[1] - t1 / rho0c;
-        R[2] = R[2] - g2 / rho0c;
-        R[3] = R[3] - g3 / rho0c;
+        R[1] = R[1] - over_rho0(t1);
+        R[2] = R[2] - over_rho0(g2);
+        R[3] = R[3] - over_rho0(g3);
     }
     // Lorentz force (:97-116): curl(B) (ModSpherical.f90:38-102) x B / rho0
     {
         T o1 = d4(L(6, -2, 0) * st[0], L(6, -1, 0) * st[1], L(6, 1, 0) * st[3], L(6, 2, 0) * st[4], ct) -
                d4(L(5, 0, -2), L(5, 0, -1), L(5, 0, 1), L(5, 0, 2), cp);
-        T o2 = d4(L(4, 0, -2) / st[2], L(4, 0, -1) / st[2], L(4, 0, 1) / st[2], L(4, 0, 2) / st[2], cp) -
+        T o2 = d4_over_st(L(4, 0, -2), L(4, 0, -1), L(4, 0, 1), L(4, 0, 2)) -
                d4(W(6, 0) * xr[0], W(6, 1) * xr[1], W(6, 3) * xr[3], W(6, 4) * xr[4], cr);
         T o3 = d4(W(5, 0) * xr[0], W(5, 1) * xr[1], W(5, 3) * xr[3], W(5, 4) * xr[4], cr) -
                d4(L(4, -2, 0), L(4, -1, 0), L(4, 1, 0), L(4, 2, 0), ct);
@@ -231,9 +251,9 @@ __device__ __forceinline__ void cell_rhs(const TW (&win)[8][5], LDSF L, const T 
         T c1 = o2 * bp - o3 * bt;
         T c2 = o3 * br - o1 * bp;
         T c3 = o1 * bt - o2 * br;
-        R[1] = R[1] + c1 / rho0c;
-        R[2] = R[2] + c2 / rho0c;
-        R[3] = R[3] + c3 / rho0c;
+        R[1] = R[1] + over_rho0(c1);
+        R[2] = R[2] + over_rho0(c2);
+        R[3] = R[3] + over_rho0(c3);
     }
     // Gravity (:121-127)
     R[1] = R[1] - gor * W(0, 2);
@@ -253,7 +273,7 @@ __device__ __forceinline__ void cell_rhs(const TW (&win)[8][5], LDSF L, const T 
         }
         T o1 = d4(e3L(-2, 0) * st[0], e3L(-1, 0) * st[1], e3L(1, 0) * st[3], e3L(2, 0) * st[4], ct) -
                d4(e2L(0, -2), e2L(0, -1), e2L(0, 1), e2L(0, 2), cp);
-        T o2 = d4(e1L(0, -2) / st[2], e1L(0, -1) / st[2], e1L(0, 1) / st[2], e1L(0, 2) / st[2], cp) -
+        T o2 = d4_over_st(e1L(0, -2), e1L(0, -1), e1L(0, 1), e1L(0, 2)) -
                d4(e3r[0] * xr[0], e3r[1] * xr[1], e3r[3] * xr[3], e3r[4] * xr[4], cr);
         T o3 = d4(e2r[0] * xr[0], e2r[1] * xr[1], e2r[3] * xr[3], e2r[4] * xr[4], cr) -
                d4(e1L(-2, 0), e1L(-1, 0), e1L(1, 0), e1L(2, 0), ct);
@@ -284,6 +304,17 @@ __global__ void __launch_bounds__(TL * TL) k_stage(Dims d, DevTables t, const TI
                                                double div, const int32_t *__restrict__ pull) {
     constexpr int PL = tile_pl<TL>();
     __shared__ T lds[2][8][PL][PL];
+    // LEAN (FP32 storage): the block's radial coefficient rows (xi, rho0, gp,
+    // gq, rinv, r2inv, xir2, heat, g/rho0, and 1/rho0) are staged in LDS once
+    // per tile, U of the plane is loaded at the top of the plane, and the next
+    // plane's loads are issued unconditionally from clamped addresses.  The
+    // plane loop then waits on no global load in the middle of its arithmetic
+    // (a load issued after the prefetch would wait for the prefetch too), which
+    // took the FP32 stage from 2.99 to 2.78 ms per launch on the 50 M-cell
+    // shell.  The FP64 instantiation keeps the table loads: with the LDS rows
+    // its register demand exceeds two wavefronts per SIMD.
+    constexpr bool LEAN = sizeof(T) == 4;
+    __shared__ double crow[kStageRows][kMaxNI];
     const int ntk = (d.nk + TL - 1) / TL, tpb = ((d.nj + TL - 1) / TL) * ntk;
     const int ntot = d.nblocks * tpb;
     // PERSIST: a grid smaller than the tile count (a persistent launch) takes
@@ -302,6 +333,16 @@ __global__ void __launch_bounds__(TL * TL) k_stage(Dims d, DevTables t, const TI
         m.init(d, in, b, j0, k0, TJa, TKa, pull);
 
         const long long rb = (long long)b * d.NI, tb = (long long)b * d.NJ;
+        if (LEAN) {
+    #pragma unroll
+            for (int r = 0; r < 9; r++) {
+                const double *src = r == 0 ? t.xi : r == 1 ? t.rho0 : r == 2 ? t.gp : r == 3 ? t.gq : r == 4 ? t.rinv
+                                  : r == 5 ? t.r2inv : r == 6 ? t.xir2 : r == 7 ? t.heat : t.g_over_rho0;
+                for (int I = threadIdx.x; I < d.NI; I += TL * TL) crow[r][I] = src[rb + I];
+            }
+            // 1/rho0 for the reciprocal form of the FP32 path's right-hand side
+            for (int I = threadIdx.x; I < d.NI; I += TL * TL) crow[9][I] = 1.0 / t.rho0[rb + I];
+        }
         const BlockMeta mt = t.meta[b];
         // the right-hand side is computed in FP64 whatever the storage type T:
         // coefficients from the FP64 tables, stage values widened on use
@@ -313,52 +354,73 @@ __global__ void __launch_bounds__(TL * TL) k_stage(Dims d, DevTables t, const TI
         for (int q = 0; q < 5; q++) st[q] = m.act ? t.sint[tb + m.J - 2 + q] : 1.0;
         const double sinv = m.act ? t.sinv[tb + m.J] : 1.0, cot = m.act ? t.cot[tb + m.J] : 0.0;
         const double dt = MODE == 0 ? *dtp : 0.0;
+        // the FP32 path's reciprocal form (cell_rhs): the stage increment
+        // (dt*R)/(5-k) as a product with 1/(5-k)
+        constexpr bool RECIP = LEAN;
+        const double rdiv = 1.0 / div;
 
         T win[8][5];
         T nxt[8], hn[8];
     #pragma unroll
         for (int v = 0; v < 8; v++) {
     #pragma unroll
-            for (int q = 0; q < 5; q++) win[v][q] = m.act ? m.col(v, q) : T(0);
-            hn[v] = m.hv ? m.halo(v, d.ng) : T(0);
+            for (int q = 0; q < 5; q++) win[v][q] = m.col(v, q);
+            hn[v] = m.halo(v, d.ng);
         }
+        const long long ubase = (long long)b * 8 * d.P + m.coff;
 
         for (int i = 0; i < d.ni; i++) {
             const int I = i + d.ng;
             const int buf = i & 1;
+            // U of this plane, issued first: the plane's arithmetic covers its latency
+            TB ub[8];
+            if (MODE == 0 && LEAN) {
+    #pragma unroll
+                for (int v = 0; v < 8; v++) ub[v] = U[ubase + (long long)v * d.P + (long long)I * m.SI];
+            }
             // stage this plane: own column centre plus halo strips
     #pragma unroll
             for (int v = 0; v < 8; v++) {
                 if (m.act) lds[buf][v][ty + 2][tx + 2] = win[v][2];
                 if (m.hv) lds[buf][v][m.hly][m.hlx] = hn[v];
             }
-            // prefetch the next window plane and the next halo
+            // prefetch the next window plane and the next halo (on the last plane
+            // the clamped loads are discarded)
+            const int In = min(I + 3, d.NI - 1), Ih = min(I + 1, d.NI - 1);
             const bool more = i + 1 < d.ni;
     #pragma unroll
             for (int v = 0; v < 8; v++) {
-                nxt[v] = (m.act && more) ? m.col(v, I + 3) : T(0);
-                hn[v] = (m.hv && more) ? m.halo(v, I + 1) : T(0);
+                nxt[v] = (LEAN || (m.act && more)) ? m.col(v, In) : T(0);
+                hn[v] = (LEAN || (m.hv && more)) ? m.halo(v, Ih) : T(0);
             }
             __syncthreads();
             if (m.act) {
+                // coefficient row r at plane I: LDS (FP32 path) or the table itself
+                auto coef = [&](int r, int II) -> double {
+                    if (LEAN) return crow[r][II];
+                    const double *src = r == 0 ? t.xi : r == 1 ? t.rho0 : r == 2 ? t.gp : r == 3 ? t.gq : r == 4 ? t.rinv
+                                      : r == 5 ? t.r2inv : r == 6 ? t.xir2 : r == 7 ? t.heat : t.g_over_rho0;
+                    return r == 9 ? 1.0 / t.rho0[rb + II] : src[rb + II];
+                };
                 double xr[5], rho0r[5], gpr[5], gqr[5];
     #pragma unroll
                 for (int q = 0; q < 5; q++) {
-                    xr[q] = t.xi[rb + I - 2 + q];
-                    rho0r[q] = t.rho0[rb + I - 2 + q];
-                    gpr[q] = t.gp[rb + I - 2 + q];
-                    gqr[q] = t.gq[rb + I - 2 + q];
+                    xr[q] = coef(0, I - 2 + q);
+                    rho0r[q] = coef(1, I - 2 + q);
+                    gpr[q] = coef(2, I - 2 + q);
+                    gqr[q] = coef(3, I - 2 + q);
                 }
                 auto L = [&](int v, int dj, int dk) -> double { return (double)lds[buf][v][ty + 2 + dj][tx + 2 + dk]; };
                 double R[8];
-                cell_rhs<double, T>(win, L, xr, rho0r, gpr, gqr, st, t.rinv[rb + I], t.r2inv[rb + I], t.xir2[rb + I],
-                                    t.heat[rb + I], t.g_over_rho0[rb + I], sinv, cot, cr, ct, cp, R);
+                cell_rhs<double, T, RECIP>(win, L, xr, rho0r, gpr, gqr, st, coef(4, I), coef(5, I), coef(6, I),
+                                           coef(7, I), coef(8, I), sinv, cot, cr, ct, cp, RECIP ? coef(9, I) : 1.0, R);
                 if (MODE == 0) {
                     const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + (long long)m.J * m.SJ + m.K;
     #pragma unroll
                     for (int v = 0; v < 8; v++) {
                         const long long q = base + (long long)v * d.P;
-                        const double x = (double)U[q] + (dt * R[v]) / div;
+                        const double inc = RECIP ? (dt * R[v]) * rdiv : (dt * R[v]) / div;
+                        const double x = (double)(LEAN ? ub[v] : U[q]) + inc;
                         out[q] = (TOUT)x;
                         if (SHADOW) shadow[q] = (float)x;
                     }
@@ -429,6 +491,13 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
     // to the ghost's own offset (0 where the ghost itself is read); set before
     // the first barrier of the march
     __shared__ long long pdel[27];
+    // LEAN (FP32 path, MODE 0): the block's radial rows c_s, xi, sqrt(rho0) are
+    // staged in LDS once per tile, and every global load a plane needs besides
+    // the next plane's window and halo (U^n of the plane, the shifted speed
+    // cells) is issued at the top of the plane, before that prefetch, so no
+    // wait in the plane's arithmetic includes the prefetch
+    constexpr bool LEAN = sizeof(T) == 4 && MODE == 0;
+    __shared__ double drow[3][kMaxNI];
     const int ntk = (d.nk + TL - 1) / TL, tpb = ((d.nj + TL - 1) / TL) * ntk;
     const int ntot = d.nblocks * tpb;
     // PERSIST: a grid smaller than the tile count (a persistent launch) takes
@@ -454,6 +523,17 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
         Marcher<T, TIN, PULL, TL> m;
         m.init(d, in, b, j0, k0, TJa, TKa, pull);
         const long long rb = (long long)b * d.NI, tb = (long long)b * d.NJ;
+        if (LEAN) {
+            for (int I = threadIdx.x; I < d.NI; I += TL * TL) {
+                drow[0][I] = t.cs[rb + I];
+                drow[1][I] = t.xi[rb + I];
+                drow[2][I] = t.sqrt_rho0[rb + I];
+            }
+        }
+        // radial rows at plane I: LDS (LEAN) or the tables
+        auto cs_at = [&](int I) -> double { return LEAN ? drow[0][I] : t.cs[rb + I]; };
+        auto xi_at = [&](int I) -> double { return LEAN ? drow[1][I] : t.xi[rb + I]; };
+        auto sr_at = [&](int I) -> double { return LEAN ? drow[2][I] : t.sqrt_rho0[rb + I]; };
         const BlockMeta mt = t.meta[b];
         const T dxi = (T)mt.dxi, dxj = (T)mt.dxj, dxk = (T)mt.dxk;
         double hmin = INFINITY;
@@ -477,8 +557,8 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
     #pragma unroll
         for (int v = 0; v < 8; v++) {
     #pragma unroll
-            for (int q = 0; q < 5; q++) win[v][q] = m.act ? m.col(v, q) : T(0);
-            hn[v] = m.hv ? m.halo(v, d.ng) : T(0);
+            for (int q = 0; q < 5; q++) win[v][q] = (LEAN || m.act) ? m.col(v, q) : T(0);
+            hn[v] = (LEAN || m.hv) ? m.halo(v, d.ng) : T(0);
             fr[v] = T(0);
         }
         const TIN *Ub = in + (long long)b * 8 * d.P;
@@ -496,41 +576,64 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
         };
         // faces on the tile's lower edges: lanes 0 .. TL-1 take the theta face
         // below row 0 in column x = lane, lanes TL .. 2TL-1 the phi face left of
-        // column 0 in row y = lane - TL
+        // column 0 in row y = lane - TL.  Such a face's four-cell stencil lies in
+        // the LDS plane at (bstart, step bstep); its speed cells are (bJ0, bK0) on
+        // the upper side and (bJ1, bK1) on the lower side.
         const int lane = threadIdx.x;
         const bool bact = lane < TL ? lane < TKa : (lane < 2 * TL && lane - TL < TJa);
+        const bool bj = lane < TL;
+        const int bx = bj ? lane : 0, by = bj ? 0 : lane - TL;
+        const int bstart = bj ? bx + 2 : (by + 2) * PL, bstep = bj ? PL : 1;
+        const int bvar = bj ? 2 : 3;
+        const int bJ0 = bj ? j0 + d.ng : j0 + by + d.ng - 2, bK0 = bj ? k0 + bx + d.ng - 2 : k0 + d.ng;
+        const int bJ1 = bj ? bJ0 - 1 : bJ0, bK1 = bj ? bK0 : bK0 - 1;
 
         for (int i = 0; i < d.ni; i++) {
             const int I = i + d.ng;
+            // LEAN: U^n of the plane and |v_d| at the shifted speed cells, issued
+            // before the prefetch (inactive lanes read at the tile's first column)
+            double ucur[8];
+            T a20 = T(0), a2p = T(0), a30 = T(0), a3p = T(0), a1m = T(0), a10 = T(0), a1p = T(0), as0 = T(0),
+              as1 = T(0);
+            if (LEAN) {
+                const int J = m.act ? m.J : j0 + d.ng, K = m.act ? m.K : k0 + d.ng;
+                const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + m.coff;
+    #pragma unroll
+                for (int v = 0; v < 8; v++) ucur[v] = U[base + (long long)v * d.P];
+                a20 = fabs(at(2, I - 2, J, K - 2));
+                a2p = fabs(at(2, I - 2, J + 1, K - 2));
+                a30 = fabs(at(3, I - 2, J - 2, K));
+                a3p = fabs(at(3, I - 2, J - 2, K + 1));
+                a1m = fabs(at(1, I - 1, J - 2, K - 2));
+                a10 = fabs(at(1, I, J - 2, K - 2));
+                a1p = fabs(at(1, I + 1, J - 2, K - 2));
+                if (bact) {
+                    as0 = fabs(at(bvar, I - 2, bJ0, bK0));
+                    as1 = fabs(at(bvar, I - 2, bJ1, bK1));
+                }
+            }
     #pragma unroll
             for (int v = 0; v < 8; v++) {
                 if (m.act) lds[v][ty + 2][tx + 2] = win[v][2];
                 if (m.hv) lds[v][m.hly][m.hlx] = hn[v];
             }
             const bool more = i + 1 < d.ni;
+            const int In = min(I + 3, d.NI - 1), Ih = min(I + 1, d.NI - 1);
     #pragma unroll
             for (int v = 0; v < 8; v++) {
-                nxt[v] = (m.act && more) ? m.col(v, I + 3) : T(0);
-                hn[v] = (m.hv && more) ? m.halo(v, I + 1) : T(0);
+                nxt[v] = (LEAN || (m.act && more)) ? m.col(v, In) : T(0);
+                hn[v] = (LEAN || (m.hv && more)) ? m.halo(v, Ih) : T(0);
             }
             __syncthreads();
-            const T csI2 = (T)t.cs[rb + I - 2];
+            const T csI2 = (T)cs_at(I - 2);
             auto Lv = [&](int v, int dj, int dk) -> T { return lds[v][ty + 2 + dj][tx + 2 + dk]; };
-            // U^n of this plane, loaded before the face exchange (FP32 path, MODE 0)
-            constexpr bool UEARLY = MODE == 0 && sizeof(T) == 4;
-            double ucur[8];
             if (m.act) {
                 const int J = m.J, K = m.K;
-                if (UEARLY) {
-                    const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + (long long)J * m.SJ + K;
-    #pragma unroll
-                    for (int v = 0; v < 8; v++) ucur[v] = U[base + (long long)v * d.P];
-                }
                 // speeds |v_d| + c_s at the cells the reference reads (see above)
-                const T c20 = fabs(at(2, I - 2, J, K - 2)) + csI2;
-                const T c2p = fabs(at(2, I - 2, J + 1, K - 2)) + csI2;
-                const T c30 = fabs(at(3, I - 2, J - 2, K)) + csI2;
-                const T c3p = fabs(at(3, I - 2, J - 2, K + 1)) + csI2;
+                const T c20 = (LEAN ? a20 : fabs(at(2, I - 2, J, K - 2))) + csI2;
+                const T c2p = (LEAN ? a2p : fabs(at(2, I - 2, J + 1, K - 2))) + csI2;
+                const T c30 = (LEAN ? a30 : fabs(at(3, I - 2, J - 2, K))) + csI2;
+                const T c3p = (LEAN ? a3p : fabs(at(3, I - 2, J - 2, K + 1))) + csI2;
     #pragma unroll
                 for (int v = 0; v < 8; v++) {
                     // upper theta face (the lower one of the cell at row + 1)
@@ -540,17 +643,9 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
                 }
             }
             if (bact) {
-                // the lower face of the edge cell: its four-cell stencil in the LDS
-                // plane (start, step), its speed cells (J0, K0) on the upper side and
-                // (J1, K1) on the lower side
-                const bool bj = lane < TL;
-                const int bx = bj ? lane : 0, by = bj ? 0 : lane - TL;
-                const int bstart = bj ? bx + 2 : (by + 2) * PL, bstep = bj ? PL : 1;
-                const int bvar = bj ? 2 : 3;
-                const int bJ0 = bj ? j0 + d.ng : j0 + by + d.ng - 2, bK0 = bj ? k0 + bx + d.ng - 2 : k0 + d.ng;
-                const int bJ1 = bj ? bJ0 - 1 : bJ0, bK1 = bj ? bK0 : bK0 - 1;
-                const T s0 = fabs(at(bvar, I - 2, bJ0, bK0)) + csI2;
-                const T s1 = fabs(at(bvar, I - 2, bJ1, bK1)) + csI2;
+                // the lower face of the edge cell
+                const T s0 = (LEAN ? as0 : fabs(at(bvar, I - 2, bJ0, bK0))) + csI2;
+                const T s1 = (LEAN ? as1 : fabs(at(bvar, I - 2, bJ1, bK1))) + csI2;
     #pragma unroll
                 for (int v = 0; v < 8; v++) {
                     const T *pv = &lds[v][0][0] + bstart;
@@ -562,11 +657,11 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
             __syncthreads();
             if (m.act) {
                 const int J = m.J, K = m.K;
-                const T csI = (T)t.cs[rb + I], csIm = (T)t.cs[rb + I - 1], csIp = (T)t.cs[rb + I + 1];
-                const T c1m = fabs(at(1, I - 1, J - 2, K - 2)) + csIm;
-                const T c10 = fabs(at(1, I, J - 2, K - 2)) + csI;
-                const T c1p = fabs(at(1, I + 1, J - 2, K - 2)) + csIp;
-                const T xic = (T)t.xi[rb + I];
+                const T csI = (T)cs_at(I), csIm = (T)cs_at(I - 1), csIp = (T)cs_at(I + 1);
+                const T c1m = (LEAN ? a1m : fabs(at(1, I - 1, J - 2, K - 2))) + csIm;
+                const T c10 = (LEAN ? a10 : fabs(at(1, I, J - 2, K - 2))) + csI;
+                const T c1p = (LEAN ? a1p : fabs(at(1, I + 1, J - 2, K - 2))) + csIp;
+                const T xic = (T)xi_at(I);
                 const T L2 = dxj * xic;
                 const T L3 = (dxk * xic) * sintc;
                 const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + (long long)J * m.SJ + K;
@@ -599,7 +694,7 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
                     r = r + (fk[v][ty][tx] - fk[v][ty][tx + 1]) / L3;
                     if (MODE == 0) {
                         const long long q = base + (long long)v * d.P;
-                        const double x = (UEARLY ? ucur[v] : U[q]) + (0.5 * dt) * (double)r;
+                        const double x = (LEAN ? ucur[v] : U[q]) + (0.5 * dt) * (double)r;
                         out[q] = x;
                         if (shadow) shadow[q] = (float)x;
                         if (v == 1) vv = x * x;
@@ -613,9 +708,9 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
                 if (MODE == 0) {
                     // ModWaveSpeed_Dynamo (3_6/ModWaveSpeed.f90:12-24) on the new state and
                     // h_LLL (3_6/ModBlock.f90:420-422), as k_dt_tile computes them
-                    double w = t.cs[rb + I] + sqrt(vv);
-                    w = w + sqrt(bb2) / t.sqrt_rho0[rb + I];
-                    const double xi = t.xi[rb + I];
+                    double w = cs_at(I) + sqrt(vv);
+                    w = w + sqrt(bb2) / sr_at(I);
+                    const double xi = xi_at(I);
                     double h = mt.dxi, h2 = xi * mt.dxj, h3 = (xi * sint64) * mt.dxk;
                     h = h < h2 ? h : h2;
                     h = h < h3 ? h : h3;
@@ -981,6 +1076,10 @@ static void check_tile(int tile) {
     if (tile != 8 && tile != 16) throw std::runtime_error("tile edge must be 8 or 16");
 }
 
+static void check_stage_ni(const Dims &d) {
+    if (d.NI > kMaxNI) throw std::runtime_error("stage march: more than 128 radial cells per block");
+}
+
 // grid of a march: one workgroup per tile (wgs = 0) or wgs workgroups that
 // loop over the tiles (a persistent launch), never more than the tiles
 static inline dim3 march_grid(const Dims &d, int TL, int wgs) {
@@ -1039,6 +1138,7 @@ template <typename T, typename TIN, typename TOUT, typename TB>
 void launch_stage(const Dims &d, const DevTables &t, const TIN *in, const TB *U, TOUT *out, float *shadow,
                   const double *dtp, double div, hipStream_t s, const int32_t *pull, int tile, int wgs) {
     check_tile(tile);
+    check_stage_ni(d);
     if (tile == 16) stage_tl<16, T>(d, t, in, U, out, shadow, dtp, div, s, pull, wgs);
     else stage_tl<8, T>(d, t, in, U, out, shadow, dtp, div, s, pull, wgs);
 }
@@ -1053,6 +1153,7 @@ static void rhs_tl(const Dims &d, const DevTables &t, const TIN *in, double *R, 
 template <typename T, typename TIN>
 void launch_rhs_only(const Dims &d, const DevTables &t, const TIN *in, double *R, hipStream_t s, int tile, int wgs) {
     check_tile(tile);
+    check_stage_ni(d);
     if (tile == 16) {
         if (wgs) rhs_tl<16, true, T>(d, t, in, R, s, wgs);
         else rhs_tl<16, false, T>(d, t, in, R, s, wgs);
