@@ -183,22 +183,22 @@ struct falcon_handle {
 
 namespace {
 
-// host Fortran (i fastest) <-> device [v][I][J][K] (K fastest) for one block
-void fortran_to_device(const double *src, double *dst, int NI, int NJ, int NK) {
-    const size_t P = (size_t)NI * NJ * NK;
+// host Fortran (i fastest) <-> device block variables (cell_off) for one block
+void fortran_to_device(const double *src, double *dst, const Dims &d) {
+    const size_t P = (size_t)d.P;
     for (int v = 0; v < 8; v++)
-        for (int K = 0; K < NK; K++)
-            for (int J = 0; J < NJ; J++)
-                for (int I = 0; I < NI; I++)
-                    dst[v * P + ((size_t)I * NJ + J) * NK + K] = src[v * P + (size_t)I + (size_t)NI * (J + (size_t)NJ * K)];
+        for (int K = 0; K < d.NK; K++)
+            for (int J = 0; J < d.NJ; J++)
+                for (int I = 0; I < d.NI; I++)
+                    dst[v * P + cell_off(d, I, J, K)] = src[v * P + (size_t)I + (size_t)d.NI * (J + (size_t)d.NJ * K)];
 }
-void device_to_fortran(const double *src, double *dst, int NI, int NJ, int NK) {
-    const size_t P = (size_t)NI * NJ * NK;
+void device_to_fortran(const double *src, double *dst, const Dims &d) {
+    const size_t P = (size_t)d.P;
     for (int v = 0; v < 8; v++)
-        for (int K = 0; K < NK; K++)
-            for (int J = 0; J < NJ; J++)
-                for (int I = 0; I < NI; I++)
-                    dst[v * P + (size_t)I + (size_t)NI * (J + (size_t)NJ * K)] = src[v * P + ((size_t)I * NJ + J) * NK + K];
+        for (int K = 0; K < d.NK; K++)
+            for (int J = 0; J < d.NJ; J++)
+                for (int I = 0; I < d.NI; I++)
+                    dst[v * P + (size_t)I + (size_t)d.NI * (J + (size_t)d.NJ * K)] = src[v * P + cell_off(d, I, J, K)];
 }
 
 // FP32 buffers of the deviation path: same-panel ghost entries whose weights
@@ -219,18 +219,20 @@ bool fp32_lerp_r(const GhostEntry &g) {
 // (dst, src) of the copies and (dst, base) with the r weight of the radial
 // lerps, each sorted by destination; offsets block*8*P + padded cell.
 void fp32_copy_lerp(const falcon_handle *h, std::vector<int2> &copy32, std::vector<std::pair<int2, double>> &lerp32) {
-    const int NJ = h->NJ(), NK = h->NK(), ng = h->lay.tree.ng;
-    const long long P = (long long)h->NI() * NJ * NK, SI = (long long)NJ * NK, SJ = NK;
-    auto cell = [&](int i, int j, int k) -> long long {
-        return ((long long)(i + ng - 1) * NJ + (j + ng - 1)) * NK + (k + ng - 1);
-    };
+    const Dims &d = h->dims;
+    const int ng = d.ng;
+    // Fortran (1-based interior) index -> cell offset
+    auto cell = [&](int i, int j, int k) -> long long { return cell_off(d, i + ng - 1, j + ng - 1, k + ng - 1); };
     copy32.clear();
     lerp32.clear();
     for (const GhostEntry &g : h->lay.ghosts) {
-        const int32_t d0 = (int32_t)((long long)g.recv_block * 8 * P + cell(g.gi, g.gj, g.gk));
-        const long long s0 = (long long)g.donor * 8 * P + cell(g.i0, g.j0, g.k0) + (g.w[1] > 0.5 ? SJ : 0) +
-                             (g.w[2] > 0.5 ? 1 : 0);
-        if (fp32_near(g)) copy32.push_back(make_int2(d0, (int32_t)(s0 + (g.w[0] > 0.5 ? SI : 0))));
+        const int32_t d0 = (int32_t)((long long)g.recv_block * 8 * d.P + cell(g.gi, g.gj, g.gk));
+        // nearest donor cell along theta and phi; along r too for a copy
+        const int jn = g.j0 + (g.w[1] > 0.5 ? 1 : 0), kn = g.k0 + (g.w[2] > 0.5 ? 1 : 0);
+        const long long s0 = (long long)g.donor * 8 * d.P + cell(g.i0, jn, kn);
+        if (fp32_near(g))
+            copy32.push_back(make_int2(d0, (int32_t)((long long)g.donor * 8 * d.P +
+                                                     cell(g.i0 + (g.w[0] > 0.5 ? 1 : 0), jn, kn))));
         else if (fp32_lerp_r(g)) lerp32.push_back({make_int2(d0, (int32_t)s0), g.w[0]});
     }
     std::sort(copy32.begin(), copy32.end(), [](const int2 &a, const int2 &b) { return a.x < b.x; });
@@ -249,18 +251,16 @@ void fp32_copy_lerp(const falcon_handle *h, std::vector<int2> &copy32, std::vect
 void build_pull_table(const falcon_handle *h, const std::vector<int2> &copy32, std::vector<int32_t> &pull,
                       std::vector<int2> &rest) {
     const Tree &t = h->lay.tree;
-    const int nb = t.nleaf, ng = t.ng, NJ = h->NJ(), NK = h->NK();
-    const long long P = (long long)h->NI() * NJ * NK, SI = (long long)NJ * NK, SJ = NK;
+    const Dims &d = h->dims;
+    const int nb = t.nleaf, ng = t.ng;
+    const long long P = d.P;
     const int n3[3] = {t.ni, t.nj, t.nk};
     const bool fits = n3[0] >= 2 * ng && n3[1] >= 2 * ng && n3[2] >= 2 * ng;
     std::vector<int32_t> recv((size_t)nb * 27, -1), cnt((size_t)nb * 27, 0);
     std::vector<char> bad((size_t)nb * 27, 0);
     auto decode = [&](long long off, int &blk, int p[3]) {
         blk = (int)(off / (8 * P));
-        const long long c = off - (long long)blk * 8 * P;
-        p[0] = (int)(c / SI);
-        p[1] = (int)((c / SJ) % NJ);
-        p[2] = (int)(c % SJ);
+        cell_ijk(d, off - (long long)blk * 8 * P, p[0], p[1], p[2]);
     };
     // donor-side direction key of a translation entry, or -1
     auto key_of = [&](const int2 &e, int &R) -> long long {
@@ -394,10 +394,8 @@ void upload_tables(falcon_handle *h) {
 
     // ghost entries: same-panel first, then Yin-Yang, each in target-list order
     const auto &G = h->lay.ghosts;
-    const long long P = (long long)NI * NJ * NK;
-    auto cell = [&](int i, int j, int k) -> long long {
-        return ((long long)(i + ng - 1) * NJ + (j + ng - 1)) * NK + (k + ng - 1);
-    };
+    const long long P = h->dims.P;
+    auto cell = [&](int i, int j, int k) -> long long { return cell_off(h->dims, i + ng - 1, j + ng - 1, k + ng - 1); };
     if ((long long)nb * 8 * P >= (1ll << 31)) throw Error("state too large for 32-bit ghost offsets");
     // FP64 buffers: every entry weighted, same-panel entries first, then
     // Yin-Yang, each in target-list order.  FP32 buffers of the deviation path:
@@ -794,6 +792,8 @@ int falcon_build_tables(falcon_handle_t h) {
         Layout &L = h->lay;
         L.keep_gcib = L.tree.nleaf <= 4096;
         L.build(host_threads());
+        const Tree &t = L.tree;
+        h->dims = make_dims(t.ni, t.nj, t.nk, t.ng, t.nleaf);
         h->built = true;
     });
 }
@@ -801,14 +801,6 @@ int falcon_build_tables(falcon_handle_t h) {
 int falcon_set_gc_all(falcon_handle_t h) {
     if (falcon_build_tables(h) != 0) return -1;
     return guard([&] {
-        const Tree &t = h->lay.tree;
-        Dims &d = h->dims;
-        d.ni = t.ni; d.nj = t.nj; d.nk = t.nk; d.ng = t.ng;
-        d.NI = h->NI(); d.NJ = h->NJ(); d.NK = h->NK();
-        d.nblocks = t.nleaf;
-        d.tiles_j = (t.nj + 15) / 16;
-        d.tiles_k = (t.nk + 15) / 16;
-        d.P = (long long)d.NI * d.NJ * d.NK;
         if (!h->stream) {
             HIPCHECK(hipGetDevice(&h->device));
             HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
@@ -950,7 +942,7 @@ int falcon_set_state(falcon_handle_t h, int ib, const double *prim) {
         h->check_block(ib);
         const size_t P = (size_t)h->dims.P;
         std::vector<double> tmp(8 * P);
-        fortran_to_device(prim, tmp.data(), h->NI(), h->NJ(), h->NK());
+        fortran_to_device(prim, tmp.data(), h->dims);
         HIPCHECK(hipStreamSynchronize(h->stream));
         HIPCHECK(hipMemcpy(h->U[h->cur].as<double>() + (size_t)ib * 8 * P, tmp.data(), sizeof(double) * 8 * P,
                            hipMemcpyHostToDevice));
@@ -969,7 +961,7 @@ int falcon_get_state(falcon_handle_t h, int ib, double *prim) {
         HIPCHECK(hipStreamSynchronize(h->stream));
         HIPCHECK(hipMemcpy(tmp.data(), h->U[h->cur].as<double>() + (size_t)ib * 8 * P, sizeof(double) * 8 * P,
                            hipMemcpyDeviceToHost));
-        device_to_fortran(tmp.data(), prim, h->NI(), h->NJ(), h->NK());
+        device_to_fortran(tmp.data(), prim, h->dims);
     });
 }
 

@@ -30,6 +30,20 @@ namespace falcon {
 template <int TL>
 constexpr int tile_pl() { return TL + 4; }   // LDS plane edge (tile + 2 halo each side)
 
+// First tile of workgroup w of a one-workgroup-per-tile launch of n tiles.
+// The dispatcher deals workgroups to the 8 XCDs in turn (w mod 8), and each
+// XCD has its own L2.  Giving XCD x the contiguous tile range [x*n/8, ...)
+// in dispatch order puts phi-adjacent blocks (consecutive ids) on the same XCD
+// at the same time, so the halo each reads from the other is an L2 hit.
+__device__ __forceinline__ int xcd_tile(int w, int n) {
+#ifdef XCD_REMAP
+    const int q = n >> 3, r = n & 7, x = w & 7, k = w >> 3;
+    return x * q + min(x, r) + k;
+#else
+    return w;
+#endif
+}
+
 // radial coefficient rows the stage kernel stages in LDS, and the largest padded
 // radial extent NI a march accepts (the launchers reject larger blocks)
 constexpr int kStageRows = 10;
@@ -87,20 +101,21 @@ struct Marcher {
                             // planes below / above the block, plane shift included
     const TIN *hblk;        // halo strip source
     long long P;
-    int SI, SJ;
+    int SI, SJ;             // main-part plane and row pitch (cell_off)
     int ilo, ihi;           // first interior plane, first plane above the interior
     int J, K;           // padded column of this lane
     long long coff;     // J*SJ + K, clamped into the block for lanes outside a partial tile
     bool act;
     bool hv;
-    long long hoff;     // halo point offset within a plane (a valid cell for lanes without a halo point)
+    long long hoff;     // cell_off of the halo point at plane 0 (a valid cell for lanes without one)
+    int hsi;            // plane pitch of the halo point's part (main or side)
     int hly, hlx;
 
     __device__ __forceinline__ void init(const Dims &d, const TIN *in, int b, int j0, int k0, int TJa, int TKa,
                          const int32_t *pull = nullptr) {
         P = d.P;
-        SI = d.NJ * d.NK;
-        SJ = d.NK;
+        SI = (int)d.SI;
+        SJ = d.SJ;
         blk = in + (long long)b * 8 * P;
         ilo = d.ng;
         ihi = d.ng + d.ni;
@@ -117,7 +132,7 @@ struct Marcher {
         K = k0 + tx + d.ng;
         // loads of inactive lanes read the tile's first cell, so every load can be
         // issued without a branch; their values are never used
-        coff = act ? (long long)J * SJ + K : (long long)(j0 + d.ng) * SJ + (k0 + d.ng);
+        coff = act ? (long long)J * SJ + (K - d.ng) : (long long)(j0 + d.ng) * SJ + k0;
         const int h = threadIdx.x;
         hv = false; hly = 0; hlx = 0;
         if (h < 4 * TL) {
@@ -132,16 +147,20 @@ struct Marcher {
             hly = jj + 2;
             hv = jj < TJa;
         }
-        hoff = hv ? (long long)(j0 + hly) * SJ + (k0 + hlx) : coff;
+        const int HJ = j0 + hly, HK = k0 + hlx;   // padded halo cell
+        const bool hmain = HK >= d.ng && HK < d.ng + d.nk;
+        hoff = hv ? cell_off(d, 0, HJ, HK) : coff;
+        hsi = (!hv || hmain) ? SI : d.NJ * 2 * d.ng;
         if (PULL && pull && hv) {
-            const int HJ = j0 + hly, HK = k0 + hlx;   // padded halo cell
             const int dj = HJ < d.ng ? -1 : (HJ >= d.ng + d.nj ? 1 : 0);
             const int dk = HK < d.ng ? -1 : (HK >= d.ng + d.nk ? 1 : 0);
             if (dj | dk) {
                 const int D = pull[b * 27 + 9 + (dj + 1) * 3 + (dk + 1)];
                 if (D >= 0) {
+                    // the neighbour's interior cell: always in its main part
                     hblk = in + (long long)D * 8 * P;
-                    hoff = (long long)(HJ - dj * d.nj) * SJ + (HK - dk * d.nk);
+                    hoff = cell_off(d, 0, HJ - dj * d.nj, HK - dk * d.nk);
+                    hsi = SI;
                 }
             }
         }
@@ -154,7 +173,7 @@ struct Marcher {
     }
     // interior planes only
     __device__ __forceinline__ T halo(int v, int I) const {
-        return (T)(PULL ? hblk : blk)[(long long)v * P + (long long)I * SI + hoff];
+        return (T)(PULL ? hblk : blk)[(long long)v * P + (long long)I * hsi + hoff];
     }
 };
 
@@ -320,7 +339,7 @@ __global__ void __launch_bounds__(TL * TL) k_stage(Dims d, DevTables t, const TI
     // PERSIST: a grid smaller than the tile count (a persistent launch) takes
     // the next tile at a stride of the grid size; otherwise one pass, tile =
     // the workgroup
-    for (int tile = blockIdx.x; tile < ntot; tile += gridDim.x) {
+    for (int tile = PERSIST ? (int)blockIdx.x : xcd_tile(blockIdx.x, ntot); tile < ntot; tile += gridDim.x) {
         const int b = tile / tpb;
         const int trem = tile - b * tpb;
         const int tj = trem / ntk, tk = trem - tj * ntk;
@@ -415,7 +434,7 @@ __global__ void __launch_bounds__(TL * TL) k_stage(Dims d, DevTables t, const TI
                 cell_rhs<double, T, RECIP>(win, L, xr, rho0r, gpr, gqr, st, coef(4, I), coef(5, I), coef(6, I),
                                            coef(7, I), coef(8, I), sinv, cot, cr, ct, cp, RECIP ? coef(9, I) : 1.0, R);
                 if (MODE == 0) {
-                    const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + (long long)m.J * m.SJ + m.K;
+                    const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + m.coff;
     #pragma unroll
                     for (int v = 0; v < 8; v++) {
                         const long long q = base + (long long)v * d.P;
@@ -487,10 +506,10 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
     __shared__ T fj[8][TL + 1][TL];
     __shared__ T fk[8][TL][TL + 1];
     __shared__ double red[4];
-    // offset of the cell read in place of a ghost in direction code c, relative
-    // to the ghost's own offset (0 where the ghost itself is read); set before
-    // the first barrier of the march
-    __shared__ long long pdel[27];
+    // block read in place of this block's ghosts in direction code c (the
+    // pull table's row, -1 where the ghost itself is read); set before the
+    // first barrier of the march
+    __shared__ int pdir[27];
     // LEAN (FP32 path, MODE 0): the block's radial rows c_s, xi, sqrt(rho0) are
     // staged in LDS once per tile, and every global load a plane needs besides
     // the next plane's window and halo (U^n of the plane, the shifted speed
@@ -503,16 +522,9 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
     // PERSIST: a grid smaller than the tile count (a persistent launch) takes
     // the next tile at a stride of the grid size; otherwise one pass, tile =
     // the workgroup
-    for (int tile = blockIdx.x; tile < ntot; tile += gridDim.x) {
+    for (int tile = PERSIST ? (int)blockIdx.x : xcd_tile(blockIdx.x, ntot); tile < ntot; tile += gridDim.x) {
         const int b = tile / tpb;
-        if (sizeof(T) == 4 && MODE == 0 && threadIdx.x < 27) {
-            const int c = threadIdx.x;
-            const int D = pull ? pull[b * 27 + c] : -1;
-            const int di = c / 9 - 1, dj = (c / 3) % 3 - 1, dk = c % 3 - 1;
-            pdel[c] = D < 0 ? 0ll
-                            : (long long)(D - b) * 8 * d.P - (long long)di * d.ni * (d.NJ * d.NK) -
-                                  (long long)dj * d.nj * d.NK - dk * d.nk;
-        }
+        if (sizeof(T) == 4 && MODE == 0 && threadIdx.x < 27) pdir[threadIdx.x] = pull ? pull[b * 27 + threadIdx.x] : -1;
         const int trem = tile - b * tpb;
         const int tj = trem / ntk, tk = trem - tj * ntk;
         const int j0 = tj * TL, k0 = tk * TL;
@@ -561,18 +573,21 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
             hn[v] = (LEAN || m.hv) ? m.halo(v, d.ng) : T(0);
             fr[v] = T(0);
         }
-        const TIN *Ub = in + (long long)b * 8 * d.P;
         // a ghost in a direction listed in the pull table is the same-level
         // neighbour's interior cell, translated by n
         auto at = [&](int v, int II, int JJ, int KK) -> T {
-            long long o = (long long)v * d.P + (long long)II * m.SI + (long long)JJ * m.SJ + KK;
+            int B = b;
             if (PULL) {
-                const int c = (II < d.ng ? 0 : (II >= d.ng + d.ni ? 18 : 9)) +
-                              (JJ < d.ng ? 0 : (JJ >= d.ng + d.nj ? 6 : 3)) +
-                              (KK < d.ng ? 0 : (KK >= d.ng + d.nk ? 2 : 1));
-                o += pdel[c];
+                const int di = II < d.ng ? -1 : (II >= d.ng + d.ni ? 1 : 0);
+                const int dj = JJ < d.ng ? -1 : (JJ >= d.ng + d.nj ? 1 : 0);
+                const int dk = KK < d.ng ? -1 : (KK >= d.ng + d.nk ? 1 : 0);
+                const int D = pdir[(di + 1) * 9 + (dj + 1) * 3 + (dk + 1)];
+                if ((di | dj | dk) && D >= 0) {
+                    B = D;
+                    II -= di * d.ni; JJ -= dj * d.nj; KK -= dk * d.nk;
+                }
             }
-            return (T)Ub[o];
+            return (T)in[((long long)B * 8 + v) * d.P + cell_off(d, II, JJ, KK)];
         };
         // faces on the tile's lower edges: lanes 0 .. TL-1 take the theta face
         // below row 0 in column x = lane, lanes TL .. 2TL-1 the phi face left of
@@ -664,7 +679,7 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
                 const T xic = (T)xi_at(I);
                 const T L2 = dxj * xic;
                 const T L3 = (dxk * xic) * sintc;
-                const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + (long long)J * m.SJ + K;
+                const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + m.coff;
                 const long long nn = (long long)d.ni * d.nj * d.nk;
                 const long long rbase = (long long)b * 8 * nn + ((long long)i * d.nj + (j0 + ty)) * d.nk + (k0 + tx);
                 double vv = 0.0, bb2 = 0.0;   // |v|^2 and |B|^2 of the new state, summed in component order
@@ -769,7 +784,8 @@ __global__ void __launch_bounds__(256) k_exchange(Dims d, DevGhosts g, TS *__res
     const int e = g.first + (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (e >= g.n_total) return;
     const long long P = d.P;
-    const int SI = d.NJ * d.NK, SJ = d.NK;
+    // the base cell is interior along phi, so its corners are in the main part
+    const int SI = (int)d.SI, SJ = d.SJ;
     const int dst = g.dst[e], src = g.src[e];
     const double wi = g.w[3 * (long long)e], wj = g.w[3 * (long long)e + 1], wk = g.w[3 * (long long)e + 2];
     const bool i0 = !SKIP || wi != 1.0, i1 = !SKIP || wi != 0.0, j0 = !SKIP || wj != 1.0, j1 = !SKIP || wj != 0.0,
@@ -842,7 +858,7 @@ __global__ void __launch_bounds__(256) k_exchange_lerp(Dims d, DevGhosts g, TS *
     const int2 c = g.lerp[e];
     const double wi = g.lerp_w[e];
     const long long P = d.P;
-    const int SI = d.NJ * d.NK;
+    const int SI = (int)d.SI;   // the donor cell is interior along phi (main part)
     double val[8];
 #pragma unroll
     for (int v = 0; v < 8; v++) {
@@ -877,25 +893,27 @@ __global__ void __launch_bounds__(256) k_boundary(Dims d, DevBoundary bl, TS *__
     else { dI = g; sI = 2 * d.ng - 1 - g; }
     // signs: top (+,-,+,+,+,-,-,+), bottom (+,-,+,+,-,+,+,+)
     const unsigned neg = side ? 0x62u : 0x12u;   // bit v set -> negate variable v
+    // (J, K) of this thread's cell, enumerated row by row over the padded plane
+    const int J = (int)(jk / d.NK), K = (int)(jk - (long long)J * d.NK);
     TS *base = st + (long long)b * 8 * d.P;
     const TS *src = base;
-    long long sjk = jk;
+    long long soff = cell_off(d, sI, J, K);
     if (pull) {
-        const int J = (int)(jk / d.NK), K = (int)(jk - (long long)J * d.NK);
         const int dj = J < d.ng ? -1 : (J >= d.ng + d.nj ? 1 : 0);
         const int dk = K < d.ng ? -1 : (K >= d.ng + d.nk ? 1 : 0);
         if (dj | dk) {
             const int D = pull[b * 27 + 9 + (dj + 1) * 3 + (dk + 1)];
             if (D >= 0) {
                 src = st + (long long)D * 8 * d.P;
-                sjk = (long long)(J - dj * d.nj) * d.NK + (K - dk * d.nk);
+                soff = cell_off(d, sI, J - dj * d.nj, K - dk * d.nk);
             }
         }
     }
+    const long long doff = cell_off(d, dI, J, K);
 #pragma unroll
     for (int v = 0; v < 8; v++) {
-        TS x = src[(long long)v * d.P + (long long)sI * plane + sjk];
-        base[(long long)v * d.P + (long long)dI * plane + jk] = ((neg >> v) & 1u) ? -x : x;
+        TS x = src[(long long)v * d.P + soff];
+        base[(long long)v * d.P + doff] = ((neg >> v) & 1u) ? -x : x;
     }
 }
 
@@ -918,14 +936,13 @@ __global__ void __launch_bounds__(TL * TL) k_dt_tile(Dims d, DevTables t, const 
     const long long rb = (long long)b * d.NI, tb = (long long)b * d.NJ;
     const BlockMeta mt = t.meta[b];
     const double *Ub = U + (long long)b * 8 * d.P;
-    const int SI = d.NJ * d.NK, SJ = d.NK;
     double mn = INFINITY;
     if (j < d.nj && k < d.nk) {
         const int J = j + d.ng, K = k + d.ng;
         const double st = t.sint[tb + J];
         for (int i = 0; i < d.ni; i++) {
             const int I = i + d.ng;
-            const long long q = (long long)I * SI + (long long)J * SJ + K;
+            const long long q = cell_off(d, I, J, K);
             const double vr = Ub[q + d.P], vt = Ub[q + 2 * d.P], vp = Ub[q + 3 * d.P];
             const double br = Ub[q + 4 * d.P], bt = Ub[q + 5 * d.P], bp = Ub[q + 6 * d.P];
             double w = t.cs[rb + I] + sqrt(vr * vr + vt * vt + vp * vp);
@@ -986,7 +1003,8 @@ __global__ void __launch_bounds__(256) k_copy_radial(Dims d, DevBoundary bl, con
     const long long jk = rem - (long long)g * plane;
     const int b = bl.list[e] >> 1, side = bl.list[e] & 1;
     const int I = side ? d.ni + d.ng + g : g;
-    const long long base = (long long)b * 8 * d.P + (long long)I * plane + jk;
+    const int J = (int)(jk / d.NK), K = (int)(jk - (long long)J * d.NK);
+    const long long base = (long long)b * 8 * d.P + cell_off(d, I, J, K);
 #pragma unroll
     for (int v = 0; v < 8; v++) dst[base + (long long)v * d.P] = src[base + (long long)v * d.P];
 }
@@ -1018,7 +1036,7 @@ __global__ void __launch_bounds__(256) k_state_diff(Dims d, const double *__rest
     const double lim = PASS == 1 ? tol * __longlong_as_double((long long)w[8 + v]) : 0.0;
     for (long long q = threadIdx.x; q < n3; q += blockDim.x) {
         const int k = (int)(q % d.nk), j = (int)((q / d.nk) % d.nj), i = (int)(q / ((long long)d.nk * d.nj));
-        const long long o = base + ((long long)(i + d.ng) * d.NJ + (j + d.ng)) * d.NK + (k + d.ng);
+        const long long o = base + cell_off(d, i + d.ng, j + d.ng, k + d.ng);
         const double x = fabs(a[o] - b[o]), y = fabs(b[o]);
         if (PASS == 0) {
             m0 = x > m0 ? x : m0;
@@ -1052,7 +1070,8 @@ __global__ void __launch_bounds__(256) k_nan_scan(Dims d, const TS *__restrict__
     for (long long q = threadIdx.x; q < d.P; q += blockDim.x) {
         const TS x = p[q];
         if (x != x) {
-            const int K = (int)(q % d.NK), J = (int)((q / d.NK) % d.NJ), I = (int)(q / ((long long)d.NK * d.NJ));
+            int I, J, K;
+            cell_ijk(d, q, I, J, K);
             const unsigned long long key =
                 ((((unsigned long long)blk * d.NK + K) * d.NJ + J) * d.NI + I) * 8ull + (unsigned long long)v;
             cnt++;
@@ -1331,7 +1350,7 @@ __global__ void __launch_bounds__(256) k_synthetic(Dims d, const double *__restr
         ratio = sqrt(Br * Br + Bt * Bt + Bp * Bp) / srho / cs;
         const unsigned long long id = ((unsigned long long)b << 32) ^ (unsigned long long)((i * 131 + j) * 1031 + k);
         const double u = (double)(splitmix64(seed ^ (id * 0x2545F4914F6CDD1Dull)) >> 11) * (1.0 / 9007199254740992.0);
-        const long long q = (long long)b * 8 * d.P + ((long long)I * d.NJ + (j + d.ng - 1)) * d.NK + (k + d.ng - 1);
+        const long long q = (long long)b * 8 * d.P + cell_off(d, I, j + d.ng - 1, k + d.ng - 1);
         U[q + 4 * d.P] = Br;
         U[q + 5 * d.P] = Bt;
         U[q + 6 * d.P] = Bp;
@@ -1368,7 +1387,7 @@ __global__ void __launch_bounds__(256) k_synthetic_scale(Dims d, const double *_
         const int b = (int)(t / n3);
         const long long c = t - (long long)b * n3;
         const int i = (int)(c % d.ni), j = (int)((c / d.ni) % d.nj), k = (int)(c / ((long long)d.ni * d.nj));
-        const long long q = (long long)b * 8 * d.P + ((long long)(i + d.ng) * d.NJ + (j + d.ng)) * d.NK + (k + d.ng);
+        const long long q = (long long)b * 8 * d.P + cell_off(d, i + d.ng, j + d.ng, k + d.ng);
         for (int v = 4; v < 7; v++) U[q + v * d.P] *= scale;
     }
 }
@@ -1412,7 +1431,7 @@ __global__ void __launch_bounds__(256) k_allcells(Dims d, const double *__restri
     // padded index of Fortran index f: f + ng - 1
     const int I = i + d.ng - 1, J = j + d.ng - 1, K = k + d.ng - 1;
     auto at = [&](int v, int II, int JJ, int KK) -> double {
-        return U[(long long)b * 8 * d.P + (long long)v * d.P + ((long long)II * d.NJ + JJ) * d.NK + KK];
+        return U[(long long)b * 8 * d.P + (long long)v * d.P + cell_off(d, II, JJ, KK)];
     };
     double o[12];
     const bool y = yin[b] != 0;
