@@ -87,6 +87,22 @@ __device__ __forceinline__ T face_flux(T a, T b, T c, T d) {
     return ((-T(0.5)) * ph) * num;
 }
 
+// The same face from the two cells' minmod slopes: sY at y (value b), sX at
+// x (value c).  The reference forms every cell's slope once per direction
+// (ModLinReconstruct_minmod over the whole array) and both faces of a cell use
+// it; face_flux recomputes it per face.  RCP (FP32 path): the limiter ratio as
+// a product with the hardware reciprocal of (c - b), whose 1 ulp error is far
+// below the FP32 storage rounding; 0 and +-0 denominators give the same
+// infinities and NaN as the quotient, so the NaN -> 0 rule still applies.
+template <typename T, bool RCP>
+__device__ __forceinline__ T face_flux_s(T b, T c, T sY, T sX) {
+    T num = ((c - sX * T(0.5)) - b) - sY * T(0.5);
+    T q = RCP ? num * (T)__builtin_amdgcn_rcpf((float)(c - b)) : num / (c - b);
+    T ph = T(1) + T(2) * (q - T(1));
+    ph = ph > T(0) ? ph : T(0);
+    return ((-T(0.5)) * ph) * num;
+}
+
 // Shared r-marching skeleton: loads the column window and the halo'd plane.
 // With a pull table (the row of block b: entry (di+1)*9+(dj+1)*3+(dk+1) is the
 // same-level neighbour whose interior holds this block's ghosts in direction
@@ -517,6 +533,11 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
     // wait in the plane's arithmetic includes the prefetch
     constexpr bool LEAN = sizeof(T) == 4 && MODE == 0;
     __shared__ double drow[3][kMaxNI];
+    // LEAN: the theta and phi minmod slopes of the plane's cells, each formed
+    // once and read by both faces of the cell: sj[v][y+1][x] for tile rows
+    // y = -1 .. TJa, sk[v][y][x+1] for tile columns x = -1 .. TKa
+    __shared__ T sj[LEAN ? 8 : 1][TL + 2][TL];
+    __shared__ T sk[LEAN ? 8 : 1][TL][TL + 2];
     const int ntk = (d.nk + TL - 1) / TL, tpb = ((d.nj + TL - 1) / TL) * ntk;
     const int ntot = d.nblocks * tpb;
     // PERSIST: a grid smaller than the tile count (a persistent launch) takes
@@ -548,6 +569,7 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
         auto sr_at = [&](int I) -> double { return LEAN ? drow[2][I] : t.sqrt_rho0[rb + I]; };
         const BlockMeta mt = t.meta[b];
         const T dxi = (T)mt.dxi, dxj = (T)mt.dxj, dxk = (T)mt.dxk;
+        const T rdxi = LEAN ? (T)__builtin_amdgcn_rcpf((float)dxi) : T(0);
         double hmin = INFINITY;
         const double sint64 = m.act ? t.sint[tb + m.J] : 1.0;
         const T sintc = m.act ? (T)t.sint[tb + m.J] : T(1);
@@ -560,6 +582,7 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
         // is carried instead of recomputed.  The same holds for the theta and phi
         // faces between neighbouring lanes, which go through fj / fk.
         T fr[8];
+        T sr[8];   // LEAN: radial slope of the cell above this plane's (carried)
         bool have_fr = false;
         // FP32 path: U before the diffusion on the previous plane (the top
         // boundary face reads it after that plane has been updated in place)
@@ -649,12 +672,53 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
                 const T c2p = (LEAN ? a2p : fabs(at(2, I - 2, J + 1, K - 2))) + csI2;
                 const T c30 = (LEAN ? a30 : fabs(at(3, I - 2, J - 2, K))) + csI2;
                 const T c3p = (LEAN ? a3p : fabs(at(3, I - 2, J - 2, K + 1))) + csI2;
+                if (LEAN) {
+                    // this cell's slopes; its faces are formed after the barrier
     #pragma unroll
-                for (int v = 0; v < 8; v++) {
-                    // upper theta face (the lower one of the cell at row + 1)
-                    fj[v][ty + 1][tx] = (face_flux(Lv(v, -1, 0), Lv(v, 0, 0), Lv(v, 1, 0), Lv(v, 2, 0)) * (c2p + c20)) * T(0.5);
-                    // upper phi face
-                    fk[v][ty][tx + 1] = (face_flux(Lv(v, 0, -1), Lv(v, 0, 0), Lv(v, 0, 1), Lv(v, 0, 2)) * (c3p + c30)) * T(0.5);
+                    for (int v = 0; v < 8; v++) {
+                        sj[v][ty + 1][tx] = minmod(Lv(v, -1, 0), Lv(v, 0, 0), Lv(v, 1, 0));
+                        sk[v][ty][tx + 1] = minmod(Lv(v, 0, -1), Lv(v, 0, 0), Lv(v, 0, 1));
+                    }
+                } else {
+    #pragma unroll
+                    for (int v = 0; v < 8; v++) {
+                        // upper theta face (the lower one of the cell at row + 1)
+                        fj[v][ty + 1][tx] =
+                            (face_flux(Lv(v, -1, 0), Lv(v, 0, 0), Lv(v, 1, 0), Lv(v, 2, 0)) * (c2p + c20)) * T(0.5);
+                        // upper phi face
+                        fk[v][ty][tx + 1] =
+                            (face_flux(Lv(v, 0, -1), Lv(v, 0, 0), Lv(v, 0, 1), Lv(v, 0, 2)) * (c3p + c30)) * T(0.5);
+                    }
+                }
+            }
+            if (LEAN) {
+                // slopes of the halo cells next to the tile: rows -1 and TJa (theta),
+                // columns -1 and TKa (phi), lanes 0 .. 4*TL-1
+                const int q = lane / TL, x = lane - q * TL;
+                if (q < 4 && x < (q < 2 ? TKa : TJa)) {
+                    // LDS plane index of the slope cell and the step to its neighbours
+                    const int y = q == 0 ? -1 : (q == 1 ? TJa : x), xx = q < 2 ? x : (q == 2 ? -1 : TKa);
+                    const int st = q < 2 ? PL : 1;
+    #pragma unroll
+                    for (int v = 0; v < 8; v++) {
+                        const T *pc = &lds[v][y + 2][xx + 2];
+                        const T sl = minmod(pc[-st], pc[0], pc[st]);
+                        if (q < 2) sj[v][y + 1][x] = sl;
+                        else sk[v][x][xx + 1] = sl;
+                    }
+                }
+                __syncthreads();
+                if (m.act) {
+                    const T c20 = a20 + csI2, c2p = a2p + csI2, c30 = a30 + csI2, c3p = a3p + csI2;
+    #pragma unroll
+                    for (int v = 0; v < 8; v++) {
+                        fj[v][ty + 1][tx] =
+                            (face_flux_s<T, true>(Lv(v, 0, 0), Lv(v, 1, 0), sj[v][ty + 1][tx], sj[v][ty + 2][tx]) *
+                             (c2p + c20)) * T(0.5);
+                        fk[v][ty][tx + 1] =
+                            (face_flux_s<T, true>(Lv(v, 0, 0), Lv(v, 0, 1), sk[v][ty][tx + 1], sk[v][ty][tx + 2]) *
+                             (c3p + c30)) * T(0.5);
+                    }
                 }
             }
             if (bact) {
@@ -664,7 +728,13 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
     #pragma unroll
                 for (int v = 0; v < 8; v++) {
                     const T *pv = &lds[v][0][0] + bstart;
-                    const T f = (face_flux(pv[0], pv[bstep], pv[2 * bstep], pv[3 * bstep]) * (s0 + s1)) * T(0.5);
+                    T f;
+                    if (LEAN)
+                        f = face_flux_s<T, true>(pv[bstep], pv[2 * bstep], bj ? sj[v][0][bx] : sk[v][by][0],
+                                                 bj ? sj[v][1][bx] : sk[v][by][1]);
+                    else
+                        f = face_flux(pv[0], pv[bstep], pv[2 * bstep], pv[3 * bstep]);
+                    f = (f * (s0 + s1)) * T(0.5);
                     if (bj) fj[v][0][bx] = f;
                     else fk[v][by][0] = f;
                 }
@@ -679,6 +749,9 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
                 const T xic = (T)xi_at(I);
                 const T L2 = dxj * xic;
                 const T L3 = (dxk * xic) * sintc;
+                // LEAN: the face sums scaled by hardware reciprocals of the lengths
+                const T rL2 = LEAN ? (T)__builtin_amdgcn_rcpf((float)L2) : T(0);
+                const T rL3 = LEAN ? (T)__builtin_amdgcn_rcpf((float)L3) : T(0);
                 const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + m.coff;
                 const long long nn = (long long)d.ni * d.nj * d.nk;
                 const long long rbase = (long long)b * 8 * nn + ((long long)i * d.nj + (j0 + ty)) * d.nk + (k0 + tx);
@@ -686,9 +759,23 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
     #pragma unroll
                 for (int v = 0; v < 8; v++) {
                     T r = T(0);
-                    T f0 = have_fr ? fr[v]
-                                   : (face_flux(win[v][0], win[v][1], win[v][2], win[v][3]) * (c10 + c1m)) * T(0.5);
-                    T f1 = (face_flux(win[v][1], win[v][2], win[v][3], win[v][4]) * (c1p + c10)) * T(0.5);
+                    T f0, f1;
+                    if (LEAN) {
+                        // radial slopes of this cell and the one above; the latter is
+                        // carried to the next plane
+                        const T sc = have_fr ? sr[v] : minmod(win[v][1], win[v][2], win[v][3]);
+                        const T su = minmod(win[v][2], win[v][3], win[v][4]);
+                        f0 = have_fr ? fr[v]
+                                     : (face_flux_s<T, true>(win[v][1], win[v][2],
+                                                             minmod(win[v][0], win[v][1], win[v][2]), sc) *
+                                        (c10 + c1m)) * T(0.5);
+                        f1 = (face_flux_s<T, true>(win[v][2], win[v][3], sc, su) * (c1p + c10)) * T(0.5);
+                        sr[v] = su;
+                    } else {
+                        f0 = have_fr ? fr[v]
+                                     : (face_flux(win[v][0], win[v][1], win[v][2], win[v][3]) * (c10 + c1m)) * T(0.5);
+                        f1 = (face_flux(win[v][1], win[v][2], win[v][3], win[v][4]) * (c1p + c10)) * T(0.5);
+                    }
                     if (BFACE && bot && i == 0) {
                         // lower face of the first cell: ghosts 0 and -1 mirror cells 1 and 2 of U^n
                         const long long q = base + (long long)v * d.P;
@@ -704,9 +791,15 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
                         f1 = (T)((face_flux<double>(uprev[v], ucur[v], g1, g2) * (double)(c1p + c10)) * 0.5);
                     }
                     fr[v] = f1;
-                    r = r + (f0 - f1) / dxi;
-                    r = r + (fj[v][ty][tx] - fj[v][ty + 1][tx]) / L2;
-                    r = r + (fk[v][ty][tx] - fk[v][ty][tx + 1]) / L3;
+                    if (LEAN) {
+                        r = r + (f0 - f1) * rdxi;
+                        r = r + (fj[v][ty][tx] - fj[v][ty + 1][tx]) * rL2;
+                        r = r + (fk[v][ty][tx] - fk[v][ty][tx + 1]) * rL3;
+                    } else {
+                        r = r + (f0 - f1) / dxi;
+                        r = r + (fj[v][ty][tx] - fj[v][ty + 1][tx]) / L2;
+                        r = r + (fk[v][ty][tx] - fk[v][ty][tx + 1]) / L3;
+                    }
                     if (MODE == 0) {
                         const long long q = base + (long long)v * d.P;
                         const double x = (LEAN ? ucur[v] : U[q]) + (0.5 * dt) * (double)r;
