@@ -367,10 +367,11 @@ void upload_tables(falcon_handle *h) {
         meta[b] = {c12 / g.dxi, c12 / g.dxj, c12 / g.dxk, g.dxi, g.dxj, g.dxk, (g.bottom ? 1 : 0) | (g.top ? 2 : 0)};
         double *arow = ang.data() + (size_t)b * NA * NJ;
         for (int J = 0; J < NJ; J++) {
-            const double s = std::sin(g.xj[J]);
+            double s, cot;
+            sin_and_cot(g.xj[J], s, cot);
             arow[J] = s;
             arow[NJ + J] = 1.0 / s;
-            arow[2 * NJ + J] = std::cos(g.xj[J]) / s;
+            arow[2 * NJ + J] = cot;
         }
     }
     // angular rows as [array][block][index] for simple pointers

@@ -15,12 +15,24 @@ namespace falcon {
 static const float kDpiF = 3.1415926535f;
 static const float kRsunF = 6.955e10f;
 
+// sin and cos of one angle come from one sincos call, as the reference's
+// build evaluates the pairs it writes (LLVM fuses them into glibc's sincos,
+// whose cos can differ from cos by one ulp)
 void coordconv(const double in[3], double out[3]) {
-    double st = std::sin(in[1]), sp = std::sin(in[2]), cp = std::cos(in[2]), ct = std::cos(in[1]);
+    double st, ct, sp, cp;
+    ::sincos(in[1], &st, &ct);
+    ::sincos(in[2], &sp, &cp);
     out[0] = in[0];
     out[1] = std::acos(st * sp);
     double a = std::acos(-((st * cp) / std::sqrt((st * st) * (cp * cp) + ct * ct)));
     out[2] = std::copysign(std::fabs(a), ct);  // SIGN(a, cos t)
+}
+
+void sin_and_cot(double t, double &s, double &cot) {
+    s = std::sin(t);
+    double ss, cc;
+    ::sincos(t, &ss, &cc);
+    cot = cc / ss;
 }
 
 void Tree::init(int ni_, int nj_, int nk_, int ng_, double r_lo_cgs, double r_hi_cgs) {
@@ -303,8 +315,10 @@ void Layout::build(int nthreads) {
                     // ModYinYang_VecConv_1D (3_6/ModYinYang.f90:48-61) coefficients
                     double o[3];
                     coordconv(x, o);
-                    e.rot[0] = std::sin(x[2]) * std::sin(o[2]);
-                    e.rot[1] = std::cos(x[2]) / std::sin(o[1]);
+                    double sp, cp;
+                    ::sincos(x[2], &sp, &cp);
+                    e.rot[0] = sp * std::sin(o[2]);
+                    e.rot[1] = cp / std::sin(o[1]);
                 }
                 out.push_back(e);
             }

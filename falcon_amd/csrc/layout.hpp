@@ -111,6 +111,10 @@ void build_heat_levels(const Layout &lay, const HeatParams &hp, int nthreads, st
 
 // Yin<->Yang point map (3_6/ModYinYang.f90:8-17); same glibc calls as the reference.
 void coordconv(const double in[3], double out[3]);
+// sin(t) (a plain libm call, as the reference's operators take it) and
+// cos(t)/sin(t) with both from one sincos call (the reference's cotant,
+// 3_6/ModSpherical.f90:296, which its build evaluates that way)
+void sin_and_cot(double t, double &s, double &cot);
 
 // n=1.5 spherical polytrope background at radius r (cm): writes g, rho0, p0,
 // T0, gamma1, gamma3, heat, xi.  rho(r_lo)/rho(r_hi) = contrast.

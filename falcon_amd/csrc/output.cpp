@@ -69,16 +69,10 @@ void allcells_geometry(const Layout &lay, std::vector<double> &geo, std::vector<
         yin[b] = g.yin;
         for (int i = 1; i <= ni + 1; i++) o[i - 1] = face(g.rng[0], ni, i);
         double *st = o + ni + 1, *ct = st + nj + 1, *sp = ct + nj + 1, *cp = sp + nk + 1;
-        for (int j = 1; j <= nj + 1; j++) {
-            const double th = face(g.rng[1], nj, j);
-            st[j - 1] = std::sin(th);
-            ct[j - 1] = std::cos(th);
-        }
-        for (int k = 1; k <= nk + 1; k++) {
-            const double ph = face(g.rng[2], nk, k);
-            sp[k - 1] = std::sin(ph);
-            cp[k - 1] = std::cos(ph);
-        }
+        // sth = sin(theta); cth = cos(theta) (3_6/ModSavePlot.f90:601-602): one
+        // sincos call per angle, as the reference's build evaluates the pair
+        for (int j = 1; j <= nj + 1; j++) ::sincos(face(g.rng[1], nj, j), &st[j - 1], &ct[j - 1]);
+        for (int k = 1; k <= nk + 1; k++) ::sincos(face(g.rng[2], nk, k), &sp[k - 1], &cp[k - 1]);
     }
 }
 

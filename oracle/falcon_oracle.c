@@ -6,6 +6,7 @@
  * reproduced so results can be compared with the flang build of the reference.
  * Compiled with -ffp-contract=off: no fused multiply-add anywhere.
  */
+#define _GNU_SOURCE
 #include "falcon_oracle.h"
 #include <math.h>
 #include <stdlib.h>
@@ -217,9 +218,15 @@ static void collect(orc_ctx *c, onode *n) {
     } else for (int i = 0; i < n->nch; i++) collect(c, &n->ch[i]);
 }
 
-/* ModYinYang_CoordConv_0D (3_6/ModYinYang.f90:8-17) */
+/* ModYinYang_CoordConv_0D (3_6/ModYinYang.f90:8-17).  The reference takes sin
+ * and cos of the same angle, which flang (like any LLVM compiler) evaluates as
+ * one glibc sincos call; sincos can differ from cos by one ulp (theta =
+ * 1.0983302628155798: cos 0x3FDD2016D70B5643, sincos 0x3FDD2016D70B5642), so
+ * the pairs are taken from sincos here as well. */
 static void coordconv(const double in[3], double out[3]) {
-    double st = sin(in[1]), sp = sin(in[2]), cp = cos(in[2]), ct = cos(in[1]);
+    double st, ct, sp, cp;
+    sincos(in[1], &st, &ct);
+    sincos(in[2], &sp, &cp);
     out[0] = in[0];
     out[1] = acos(st * sp);
     double a = acos(-((st * cp) / sqrt((st * st) * (cp * cp) + ct * ct)));
@@ -509,11 +516,14 @@ static void interp_point(const orc_ctx *c, const oblock *s, const double *f, con
     }
 }
 
-/* ModYinYang_VecConv_1D (3_6/ModYinYang.f90:48-61) for one point */
+/* ModYinYang_VecConv_1D (3_6/ModYinYang.f90:48-61) for one point; sin and
+ * cos of the input phi as one sincos call (see coordconv) */
 static void vecconv(const double rtp[3], double v[3]) {
     double o[3];
     coordconv(rtp, o);
-    double sp = sin(rtp[2]), spo = sin(o[2]), cp = cos(rtp[2]), sto = sin(o[1]);
+    double sp, cp;
+    sincos(rtp[2], &sp, &cp);
+    double spo = sin(o[2]), sto = sin(o[1]);
     double v2 = -((sp * spo) * v[1]) - (cp / sto) * v[2];
     double v3 = -((sp * spo) * v[2]) + (cp / sto) * v[1];
     v[1] = v2; v[2] = v3;
@@ -719,7 +729,10 @@ static void equation(const orc_ctx *c, const oblock *b, const double *p, double 
         size_t q = INT(c, i, j, k), pq = PAD(c, i, j, k);
         double rinv = 1.0 / b->xi[i + ng - 1];
         double xj = b->xj[j + ng - 1];
-        double cot = cos(xj) / sin(xj);
+        /* cotant=cos(t)/sin(t) (ModSpherical.f90:296): one sincos call */
+        double sxj, cxj;
+        sincos(xj, &sxj, &cxj);
+        double cot = cxj / sxj;
         double A2 = P[VT][pq], A3 = P[VP][pq], B1 = P[VR][pq], B2 = P[VT][pq], B3 = P[VP][pq];
         double tr = (-A2) * B2 - A3 * B3;
         t1[q] = t1[q] + tr * rinv;
@@ -959,7 +972,9 @@ void orc_coordconv(const double in[3], double out[3]) { coordconv(in, out); }
 void orc_vecconv_coeffs(const double x[3], double out[2]) {
     double o[3];
     coordconv(x, o);
-    double sp = sin(x[2]), spo = sin(o[2]), cp = cos(x[2]), sto = sin(o[1]);
+    double sp, cp;
+    sincos(x[2], &sp, &cp);
+    double spo = sin(o[2]), sto = sin(o[1]);
     out[0] = sp * spo;
     out[1] = cp / sto;
 }
