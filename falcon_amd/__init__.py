@@ -3,5 +3,5 @@
 Compute runs in libfalcon_hip.so (hand-written HIP for gfx950, built in-tree by
 `python -m falcon_amd.build`); this package is the host-side driver.
 """
-from .core import Cnn, FalconError, HeatSolver, Shell, format_es16, load, R_SUN  # noqa: F401
+from .core import Cnn, FalconError, HeatSolver, Pfss, Shell, format_es16, load, R_SUN  # noqa: F401
 from . import configs  # noqa: F401

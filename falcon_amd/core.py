@@ -83,6 +83,14 @@ def load():
         "falcon_heat_get_T": [vp, C.c_int, dp],
         "falcon_heat_apply": [vp, dp, dp],
         "falcon_heat_vcycle": [vp, dp, dp],
+        "falcon_pfss_setup": [vp],
+        "falcon_pfss_bottom_blocks": [vp, ip, ip],
+        "falcon_pfss_bottom_areas": [vp, dp],
+        "falcon_pfss_set_magnetogram": [vp, dp],
+        "falcon_pfss_dipole_magnetogram": [vp, C.c_int, dp, dp, dp, dp],
+        "falcon_pfss_solve": [vp, C.c_double, C.c_int, ip, dp, dp],
+        "falcon_pfss_get_phi": [vp, dp],
+        "falcon_pfss_field": [vp, dp],
         "falcon_cnn_init": [vp, C.c_uint64],
         "falcon_cnn_set_params": [vp, fp],
         "falcon_cnn_get_params": [vp, fp],
@@ -378,6 +386,74 @@ def format_es16(values):
     buf = C.create_string_buffer(16 * v.size + 1)
     _check(load().falcon_format_es16(_p(v), int(v.size), buf))
     return buf.value.decode()
+
+
+class Pfss:
+    """The PFSS potential field (3_5/ModPFSS.f90) on a Shell's blocks (cubic,
+    power-of-two edge): the Laplacian with the flux Br S of a magnetogram
+    through the bottom faces and phi = 0 on the top face, solved by the
+    MG-preconditioned flexible CG; B = -grad(phi).  Magnetograms are arrays
+    (nbottom, n, n) in the C view of Fortran (j, k) order (j fastest), one per
+    bottom block (`bottom`, block order); phi (nblocks, n, n, n) and B
+    (nblocks, 3, n, n, n) per block in the C view of Fortran order."""
+
+    def __init__(self, shell):
+        self.s = shell
+        self.L = shell.L
+        _check(self.L.falcon_pfss_setup(shell.h))
+        nb = C.c_int32()
+        _check(self.L.falcon_pfss_bottom_blocks(shell.h, None, C.byref(nb)))
+        blocks = np.zeros(nb.value, np.int32)
+        _check(self.L.falcon_pfss_bottom_blocks(shell.h, _p(blocks, C.c_int32), C.byref(nb)))
+        self.bottom = blocks
+        n = shell.ni
+        self.n = n
+        self.shape = (shell.nblocks, n, n, n)
+        self.mshape = (len(blocks), n, n)
+
+    def areas(self):
+        a = np.zeros(self.mshape)
+        _check(self.L.falcon_pfss_bottom_areas(self.s.h, _p(a)))
+        return a
+
+    def dipole_magnetogram(self, xyz, q, l):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        l = np.ascontiguousarray(l, dtype=np.float64).reshape(-1, 3)
+        br = np.zeros(self.mshape)
+        _check(self.L.falcon_pfss_dipole_magnetogram(self.s.h, len(q), _p(xyz), _p(q), _p(l), _p(br)))
+        return br
+
+    def set_magnetogram(self, br):
+        br = np.ascontiguousarray(br, dtype=np.float64).reshape(self.mshape)
+        _check(self.L.falcon_pfss_set_magnetogram(self.s.h, _p(br)))
+
+    def solve(self, tol=1e-10, max_iter=500):
+        it, rel, sec = C.c_int32(), C.c_double(), C.c_double()
+        _check(self.L.falcon_pfss_solve(self.s.h, tol, max_iter, C.byref(it), C.byref(rel), C.byref(sec)))
+        return it.value, rel.value, sec.value
+
+    def phi(self):
+        out = np.zeros(self.shape)
+        _check(self.L.falcon_pfss_get_phi(self.s.h, _p(out)))
+        return out
+
+    def field(self):
+        out = np.zeros((self.shape[0], 3) + self.shape[1:])
+        _check(self.L.falcon_pfss_field(self.s.h, _p(out)))
+        return out
+
+    def apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape((1,) + self.shape)
+        y = np.zeros((1,) + self.shape)
+        _check(self.L.falcon_heat_apply(self.s.h, _p(x), _p(y)))
+        return y[0]
+
+    def vcycle(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64).reshape((1,) + self.shape)
+        z = np.zeros((1,) + self.shape)
+        _check(self.L.falcon_heat_vcycle(self.s.h, _p(r), _p(z)))
+        return z[0]
 
 
 class HeatSolver:

@@ -142,6 +142,11 @@ struct falcon_handle {
     int wgs_stage[2] = {0, 0}, wgs_diff[2] = {0, 0};
     std::unique_ptr<HeatSolver> heat;
     HeatParams heat_params;
+    // PFSS form of the implicit solver (falcon_pfss_setup): its tables and
+    // the magnetogram of the right-hand side (per bottom block, j fastest)
+    bool pfss_on = false;
+    PfssTables pfss;
+    std::vector<double> pfss_br;
     std::unique_ptr<CnnModel> cnn;
     double graph_cfl = -1;
     // dt > 0: every step uses it in place of the CFL dt (falcon_set_fixed_dt)
@@ -1578,10 +1583,119 @@ int falcon_heat_setup(falcon_handle_t h, double kappa0, double r_ref_rsun, int n
     return guard([&] {
         if (nsys < 1) throw Error("nsys must be >= 1");
         if (!(kappa0 > 0)) throw Error("kappa0 must be positive");
+        h->heat_params = HeatParams{};
         h->heat_params.kappa0 = kappa0;
         h->heat_params.r_ref = r_ref_rsun * (double)6.955e10f;
         h->heat.reset(new HeatSolver());
         h->heat->setup(h->lay, h->heat_params, nsys, host_threads());
+        h->pfss_on = false;
+    });
+}
+
+// ---------------------------------------------------------------- PFSS potential field
+
+int falcon_pfss_setup(falcon_handle_t h) {
+    if (falcon_build_tables(h) != 0) return -1;
+    return guard([&] {
+        HeatParams hp;
+        hp.kappa0 = 1.0;
+        hp.r_ref = 1.0;
+        hp.kappa_power = 0;
+        hp.top_dirichlet = true;
+        h->heat_params = hp;
+        h->heat.reset(new HeatSolver());
+        h->heat->setup(h->lay, hp, 1, host_threads());
+        h->heat->set_dt({INFINITY});   // 1/dt = 0: no V/dt term
+        build_pfss_tables(h->lay, h->pfss);
+        h->pfss_br.assign(h->pfss.bottom.size() * (size_t)h->pfss.m * h->pfss.m, 0.0);
+        h->heat->set_rhs(std::vector<double>(h->heat->cells(), 0.0));
+        h->pfss_on = true;
+    });
+}
+
+static HeatSolver &need_pfss(falcon_handle_t h) {
+    if (!h->heat || !h->pfss_on) throw Error("falcon_pfss_setup has not been called");
+    return *h->heat;
+}
+
+int falcon_pfss_bottom_blocks(falcon_handle_t h, int32_t *blocks, int *n) {
+    return guard([&] {
+        need_pfss(h);
+        if (n) *n = (int)h->pfss.bottom.size();
+        if (blocks) std::copy(h->pfss.bottom.begin(), h->pfss.bottom.end(), blocks);
+    });
+}
+
+int falcon_pfss_set_magnetogram(falcon_handle_t h, const double *br) {
+    return guard([&] {
+        HeatSolver &H = need_pfss(h);
+        const PfssTables &t = h->pfss;
+        const int m = t.m;
+        const long long mm = (long long)m * m;
+        h->pfss_br.assign(br, br + t.bottom.size() * mm);
+        // b = Br S on the first radial cell of each bottom block (3_5/ModPFSS.f90:43-44)
+        std::vector<double> b(H.cells(), 0.0);
+        for (size_t q = 0; q < t.bottom.size(); q++)
+            for (int k = 0; k < m; k++)
+                for (int j = 0; j < m; j++) {
+                    const long long f = (long long)q * mm + (long long)k * m + j;
+                    b[(long long)t.bottom[q] * mm * m + (long long)j * m + k] = br[f] * t.area[f];
+                }
+        H.set_rhs(b);
+    });
+}
+
+int falcon_pfss_bottom_areas(falcon_handle_t h, double *area) {
+    return guard([&] {
+        need_pfss(h);
+        std::copy(h->pfss.area.begin(), h->pfss.area.end(), area);
+    });
+}
+
+int falcon_pfss_dipole_magnetogram(falcon_handle_t h, int n, const double *xyz, const double *q, const double *l,
+                                   double *br) {
+    return guard([&] {
+        need_pfss(h);
+        if (n < 0) throw Error("the number of dipoles must be >= 0");
+        std::vector<double> out;
+        dipole_magnetogram(h->lay, h->pfss, n, xyz, q, l, out);
+        std::copy(out.begin(), out.end(), br);
+    });
+}
+
+int falcon_pfss_solve(falcon_handle_t h, double tol, int max_iter, int32_t *iters, double *relres, double *seconds) {
+    return guard([&] {
+        HeatSolver &H = need_pfss(h);
+        HeatResult r = H.solve(tol, max_iter, false);
+        if (iters) *iters = r.iters[0];
+        if (relres) *relres = r.relres[0];
+        if (seconds) *seconds = r.seconds;
+    });
+}
+
+int falcon_pfss_get_phi(falcon_handle_t h, double *phi) {
+    return guard([&] {
+        HeatSolver &H = need_pfss(h);
+        std::vector<double> dev;
+        H.get_solution(0, dev);
+        heat_to_host(dev.data(), phi, H.m(), (long long)dev.size() / ((long long)H.m() * H.m() * H.m()));
+    });
+}
+
+int falcon_pfss_field(falcon_handle_t h, double *B) {
+    return guard([&] {
+        HeatSolver &H = need_pfss(h);
+        const PfssTables &t = h->pfss;
+        const int m = t.m, nb = h->lay.tree.nleaf;
+        const long long mm = (long long)m * m;
+        std::vector<double> bot((size_t)nb * mm, 0.0), dev;
+        for (size_t q = 0; q < t.bottom.size(); q++)
+            for (int k = 0; k < m; k++)
+                for (int j = 0; j < m; j++)
+                    bot[(long long)t.bottom[q] * mm + (long long)j * m + k] =
+                        h->pfss_br[(long long)q * mm + (long long)k * m + j] * t.dr[t.bottom[q]];
+        H.pfss_field(bot, t.geo, dev);
+        heat_to_host(dev.data(), B, m, 3LL * nb);
     });
 }
 

@@ -718,6 +718,54 @@ __global__ void __launch_bounds__(256) k_hrhs(HLev L, const double *__restrict__
     bvec[s * istride + c] = L.V[t2] * inv_dt[s] * T[s * istride + c];
 }
 
+// PFSS field B = -grad(phi) of the finest level's solution x (system 0) at
+// every cell: the second-order gradient of ModSpherical_Grad_f_O2
+// (3_6/ModSpherical.f90:161-199: (1./2./dx)*(f(+1) - f(-1)), then 1/r and
+// 1/sin(theta)) with the sign 3_5/ModPFSS.f90:141 states.  Neighbours across a
+// block face are the operator's face ghosts; at the domain's radial boundaries
+// the ghosts of the boundary conditions: bottom phi_1 + bot (= Br dr, the
+// insulated face carrying the flux Br), top -phi_m (phi = 0 on the face).
+// geo per block: 1/r (m), 1/sin(theta) (m), (1./2./dx) of r, theta, phi;
+// B: [block][3][i][j][k].
+__global__ void __launch_bounds__(256) k_pfss_field(HLev L, const double *__restrict__ x, const double *__restrict__ bot,
+                                                    const double *__restrict__ geo, double *__restrict__ B) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (long long)L.nb * L.M3) return;
+    const Cell q = cell_of(L, c);
+    const int m = L.m;
+    const long long mm = (long long)m * m, cb = (long long)q.b * L.M3;
+    const Field F{x};
+    // the face ghost of face f (i-, i+, j-, j+, k-, k+) at in-face position (t1, t2)
+    auto ghost = [&](int f, int t1, int t2) -> double {
+        const int2 e = L.gsw[((long long)q.b * 6 + f) * mm + (long long)t1 * m + t2];
+        if (e.x >= 0) {
+            const double f0 = F(e.x);
+            return e.y >= 0 ? trilinear(L, F, e.x, e.y, f0) : f0;
+        }
+        // no donor: a radial boundary face of the domain
+        return f == 0 ? F(cb + ((long long)q.j * m + q.k)) + bot[(long long)q.b * mm + (long long)q.j * m + q.k]
+                      : -F(cb + ((long long)(m - 1) * m + q.j) * m + q.k);
+    };
+    const double xim = q.i > 0 ? F(c - mm) : ghost(0, q.j, q.k);
+    const double xip = q.i < m - 1 ? F(c + mm) : ghost(1, q.j, q.k);
+    const double xjm = q.j > 0 ? F(c - m) : ghost(2, q.i, q.k);
+    const double xjp = q.j < m - 1 ? F(c + m) : ghost(3, q.i, q.k);
+    const double xkm = q.k > 0 ? F(c - 1) : ghost(4, q.i, q.j);
+    const double xkp = q.k < m - 1 ? F(c + 1) : ghost(5, q.i, q.j);
+    const double *G = geo + (long long)q.b * (2 * m + 3);
+    const double rinv = G[q.i], sinv = G[m + q.j];
+    const double gr = G[2 * m] * (xip - xim);
+    double gt = G[2 * m + 1] * (xjp - xjm);
+    gt = gt * rinv;
+    double gp = G[2 * m + 2] * (xkp - xkm);
+    gp = gp * rinv;
+    gp = gp * sinv;
+    const long long o = (long long)q.b * 3 * L.M3 + (c - cb);
+    B[o] = -gr;
+    B[o + L.M3] = -gt;
+    B[o + 2 * L.M3] = -gp;
+}
+
 __global__ void __launch_bounds__(256) k_hcopy(const double *__restrict__ a, double *__restrict__ b, long long n) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = a[i];
@@ -1014,7 +1062,7 @@ void HeatSolver::setup(const Layout &lay, const HeatParams &hp, int nsys, int nt
                     const double th = (nd.rng[1][0] * (n - j + 0.5) + nd.rng[1][1] * (j - 0.5)) / n;
                     const double h2 = r * dth, h3 = r * std::sin(th) * dph;
                     const double x = r / hp.r_ref;
-                    const double kap = hp.kappa0 / (x * x * x);
+                    const double kap = hp.kappa_power == 0 ? hp.kappa0 : hp.kappa0 / (x * x * x);
                     const double v = 2.0 * kap * (1.0 / (dr * dr) + 1.0 / (h2 * h2) + 1.0 / (h3 * h3));
                     worst = v > worst ? v : worst;
                 }
@@ -1126,6 +1174,35 @@ void HeatSolver::set_rhs_from_T(const std::vector<double> &T) {
     hipLaunchKernelGGL(k_hrhs, I.grid_cells(0), dim3(256), 0, I.stream, I.L[0], (const double *)I.tmp, I.b,
                        (const double *)I.inv_dt, I.ist(0));
     HCHECK(hipStreamSynchronize(I.stream));
+}
+
+void HeatSolver::set_rhs(const std::vector<double> &b) {
+    Impl &I = *d;
+    if ((long long)b.size() != I.ist(0) * I.nsys) throw std::runtime_error("b has the wrong size");
+    HCHECK(hipMemcpy(I.b, b.data(), sizeof(double) * b.size(), hipMemcpyHostToDevice));
+}
+
+void HeatSolver::pfss_field(const std::vector<double> &bot, const std::vector<double> &geo, std::vector<double> &B) {
+    Impl &I = *d;
+    const long long n = I.ist(0);
+    if ((long long)bot.size() != (long long)I.nb * I.L[0].m * I.L[0].m ||
+        (long long)geo.size() != (long long)I.nb * (2 * I.L[0].m + 3))
+        throw std::runtime_error("pfss_field: table sizes do not match the layout");
+    double *dbot = nullptr, *dgeo = nullptr, *dB = nullptr;
+    HCHECK(hipMalloc(&dbot, sizeof(double) * bot.size()));
+    HCHECK(hipMalloc(&dgeo, sizeof(double) * geo.size()));
+    HCHECK(hipMalloc(&dB, sizeof(double) * 3 * n));
+    HCHECK(hipMemcpy(dbot, bot.data(), sizeof(double) * bot.size(), hipMemcpyHostToDevice));
+    HCHECK(hipMemcpy(dgeo, geo.data(), sizeof(double) * geo.size(), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_pfss_field, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, I.stream, I.L[0],
+                       (const double *)I.x, (const double *)dbot, (const double *)dgeo, dB);
+    HCHECK(hipGetLastError());
+    B.resize(3 * n);
+    HCHECK(hipStreamSynchronize(I.stream));
+    HCHECK(hipMemcpy(B.data(), dB, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    HCHECK(hipFree(dbot));
+    HCHECK(hipFree(dgeo));
+    HCHECK(hipFree(dB));
 }
 
 void HeatSolver::apply(const std::vector<double> &x, std::vector<double> &y) {

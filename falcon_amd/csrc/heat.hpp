@@ -61,6 +61,11 @@ class HeatSolver {
     double explicit_dt() const;              // 1 / max(2 kappa sum_d 1/h_d^2)
     void set_dt(const std::vector<double> &dt);
     void set_rhs_from_T(const std::vector<double> &T);   // system-major, per block [i][j][k]
+    void set_rhs(const std::vector<double> &b);          // the right-hand side itself, same layout
+    // B = -grad(x) of system 0 at every finest-level cell, [block][3][i][j][k]
+    // (PFSS field; bot: per block and bottom-face cell (j, k) the ghost offset
+    // Br dr of the domain's bottom, geo: see build_pfss_tables)
+    void pfss_field(const std::vector<double> &bot, const std::vector<double> &geo, std::vector<double> &B);
     void apply(const std::vector<double> &x, std::vector<double> &y);
     // z = M r, one V-cycle, system-major interior vectors (parity tests)
     void vcycle_host(const std::vector<double> &r, std::vector<double> &z);

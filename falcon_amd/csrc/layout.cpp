@@ -395,6 +395,73 @@ LevelGeom level_geom(const Node &n, int m) {
 }
 }  // namespace
 
+void build_pfss_tables(const Layout &lay, PfssTables &pt) {
+    const Tree &t = lay.tree;
+    const int m = t.ni, nb = t.nleaf;
+    pt.m = m;
+    pt.bottom.clear();
+    pt.area.clear();
+    pt.dr.assign(nb, 0.0);
+    pt.geo.assign((size_t)nb * (2 * m + 3), 0.0);
+    for (int b = 0; b < nb; b++) {
+        const LevelGeom g = level_geom(t.nodes[t.leaf_node[b]], m);
+        pt.dr[b] = g.dx[0];
+        double *G = &pt.geo[(size_t)b * (2 * m + 3)];
+        for (int i = 0; i < m; i++) G[i] = 1.0 / g.xc[0][i + 1];
+        for (int j = 0; j < m; j++) G[m + j] = 1.0 / std::sin(g.xc[1][j + 1]);
+        for (int a = 0; a < 3; a++) G[2 * m + a] = (double)(1.0f / 2.0f) / g.dx[a];
+        if (!lay.geom[b].bottom) continue;
+        pt.bottom.push_back(b);
+        const double rf = g.xf[0][0], dph = g.dx[2];
+        for (int k = 0; k < m; k++)
+            for (int j = 0; j < m; j++)
+                pt.area.push_back(rf * rf * (std::cos(g.xf[1][j]) - std::cos(g.xf[1][j + 1])) * dph);
+    }
+}
+
+void dipole_magnetogram(const Layout &lay, const PfssTables &pt, int n, const double *xyz, const double *q,
+                        const double *l, std::vector<double> &br) {
+    const Tree &t = lay.tree;
+    const int m = pt.m;
+    br.assign(pt.bottom.size() * (size_t)m * m, 0.0);
+    for (size_t bb = 0; bb < pt.bottom.size(); bb++) {
+        const int b = pt.bottom[bb];
+        const LevelGeom g = level_geom(t.nodes[t.leaf_node[b]], m);
+        const double rF = g.xf[0][0];
+        for (int j = 0; j < m; j++)
+            for (int k = 0; k < m; k++) {
+                // rtp_face_center = [xi_F(1), xj_F(j), xk_I(k)]: theta at the face
+                // xj_F(j) and phi at the centre, as the reference writes it; Yin
+                // points in the other panel's frame
+                double rtp[3] = {rF, g.xf[1][j], g.xc[2][k + 1]};
+                if (g.yin) {
+                    double o[3];
+                    coordconv(rtp, o);
+                    rtp[0] = o[0]; rtp[1] = o[1]; rtp[2] = o[2];
+                }
+                double sth, cth, sph, cph;
+                ::sincos(rtp[1], &sth, &cth);
+                ::sincos(rtp[2], &sph, &cph);
+                const double f[3] = {rtp[0] * (cph * sth), rtp[0] * (sph * sth), rtp[0] * cth};
+                double o = 0.0;
+                for (int d = 0; d < n; d++) {
+                    double dp[3], dn[3];
+                    for (int a = 0; a < 3; a++) {
+                        dp[a] = f[a] - (xyz[3 * d + a] + l[3 * d + a] / 2.0);
+                        dn[a] = f[a] - (xyz[3 * d + a] - l[3 * d + a] / 2.0);
+                    }
+                    const double rp = std::sqrt(dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2]);
+                    const double rn = std::sqrt(dn[0] * dn[0] + dn[1] * dn[1] + dn[2] * dn[2]);
+                    double sp = 0.0, sn = 0.0;
+                    for (int a = 0; a < 3; a++) { sp = sp + dp[a] * f[a] / rF; sn = sn + dn[a] * f[a] / rF; }
+                    // the radial component of the two poles' Coulomb fields
+                    o = (o + q[d] * sp / (rp * rp * rp)) - q[d] * sn / (rn * rn * rn);
+                }
+                br[bb * (size_t)m * m + (size_t)k * m + j] = o;
+            }
+    }
+}
+
 void build_heat_levels(const Layout &lay, const HeatParams &hp, int nthreads, std::vector<HeatLevel> &levels) {
     const Tree &t = lay.tree;
     if (t.ni != t.nj || t.ni != t.nk) throw std::runtime_error("heat levels need cubic blocks");
@@ -415,7 +482,12 @@ void build_heat_levels(const Layout &lay, const HeatParams &hp, int nthreads, st
         L.g_wid.assign((size_t)nb * 6 * M2, -1);
         std::vector<std::vector<double>> gw(nb);
         std::vector<std::vector<int32_t>> gwslot(nb);     // ghost ordinal of each weighted entry
-        auto kap = [&](double r) { double x = r / hp.r_ref; return hp.kappa0 / (x * x * x); };
+        if (hp.kappa_power != -3 && hp.kappa_power != 0) throw std::runtime_error("kappa_power must be -3 or 0");
+        auto kap = [&](double r) {
+            if (hp.kappa_power == 0) return hp.kappa0;
+            double x = r / hp.r_ref;
+            return hp.kappa0 / (x * x * x);
+        };
         auto work = [&](int b) {
             const Node &nd = t.nodes[t.leaf_node[b]];
             LevelGeom g = level_geom(nd, m);
@@ -429,7 +501,9 @@ void build_heat_levels(const Layout &lay, const HeatParams &hp, int nthreads, st
                 for (int j = 0; j < m; j++) {
                     double S = rf[i] * rf[i] * (std::cos(tf[j]) - std::cos(tf[j + 1])) * dph;
                     double c = kap(rf[i]) * S / g.dx[0];
-                    if ((i == 0 && bottom) || (i == m && top)) c = 0.0;   // insulated radial boundary
+                    // insulated radial boundary (the PFSS bottom too: its flux is the
+                    // right-hand side); the top is insulated or Dirichlet
+                    if ((i == 0 && bottom) || (i == m && top && !hp.top_dirichlet)) c = 0.0;
                     Ai[i * m + j] = c;
                 }
             for (int i = 0; i < m; i++)
@@ -450,6 +524,9 @@ void build_heat_levels(const Layout &lay, const HeatParams &hp, int nthreads, st
                                    (std::cos(tf[j]) - std::cos(tf[j + 1])) * dph;
                     off[i * m + j] = Ai[i * m + j] + Ai[(i + 1) * m + j] + Aj[i * (m + 1) + j] + Aj[i * (m + 1) + j + 1] +
                                      Ak[i * m + j] + Ak[i * m + j];
+                    // Dirichlet top face: Aii = Aii - Ai(top) with Ai(top) = -S/D
+                    // (3_5/ModPFSS.f90:88-91), i.e. the face once more
+                    if (hp.top_dirichlet && top && i == m - 1) off[i * m + j] = off[i * m + j] + Ai[m * m + j];
                 }
             // face ghosts in face order i-, i+, j-, j+, k-, k+; radial domain boundary
             // faces (no donor) keep src = -1 and a zero coefficient

@@ -102,12 +102,39 @@ struct HeatLevel {
 };
 
 struct HeatParams {
-    double kappa0 = 1.0;     // kappa(r) = kappa0 * (r / r_ref)^-3
+    double kappa0 = 1.0;     // kappa(r) = kappa0 * (r / r_ref)^kappa_power
     double r_ref = 0.0;      // cm
+    int kappa_power = -3;    // -3 (heat conduction) or 0 (uniform: the PFSS Laplacian)
+    // top radial boundary: insulated (false) or Dirichlet value 0 on the face
+    // (true, the PFSS source surface of 3_5/ModPFSS.f90:88-91: the face
+    // coefficient enters the diagonal twice, as a mirror ghost -x would, and
+    // couples to nothing)
+    bool top_dirichlet = false;
 };
 
 // Builds levels m = n, n/2, ..., 1 for cubic blocks (ni = nj = nk = n, power of 2).
 void build_heat_levels(const Layout &lay, const HeatParams &hp, int nthreads, std::vector<HeatLevel> &levels);
+
+// Host tables of the PFSS potential field on the finest heat level (edge m):
+// bottom: the blocks at the domain's inner radius, in block order;
+// area: for each of them the areas r_F^2 (cos t_j - cos t_j+1) dphi of its
+// bottom faces (j fastest, 3_5/ModMultigrid.f90 Si_FLL(1,:,:));
+// dr: every block's radial spacing;
+// geo: per block 1/r and 1/sin(theta) of its cell centres (m each), then the
+// second-order derivative factors (1./2./dx) of r, theta, phi
+// (3_6/ModDerivative.f90:46-56, 3_6/ModSpherical.f90:161-199).
+struct PfssTables {
+    int m = 0;
+    std::vector<int> bottom;
+    std::vector<double> area, dr, geo;
+};
+void build_pfss_tables(const Layout &lay, PfssTables &t);
+// ModMagnetogram_dipole_magnetogram_ALL (3_5/ModMagnetogram.f90:13-80): Br of
+// n dipoles (centre xyz, pole charge q, pole separation l, cm) at the bottom
+// faces of the bottom blocks, (j, k) with j fastest, in FP64 (the reference
+// mixes default-real and real(8) operands); br holds nbottom * m * m values.
+void dipole_magnetogram(const Layout &lay, const PfssTables &t, int n, const double *xyz, const double *q,
+                        const double *l, std::vector<double> &br);
 
 // Yin<->Yang point map (3_6/ModYinYang.f90:8-17); same glibc calls as the reference.
 void coordconv(const double in[3], double out[3]);

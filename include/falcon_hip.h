@@ -246,6 +246,41 @@ int falcon_heat_apply(falcon_handle_t h, const double *x, double *y);
  * over the whole system (ModLinearSolver_JCB_CG, 3_5/ModLinearSolver.f90:86-180) */
 int falcon_heat_vcycle(falcon_handle_t h, const double *r, double *z);
 
+/* ---------------------------------------------------------------- PFSS potential field
+ * ModPFSS_setup / ModPFSS_solve (3_5/ModPFSS.f90:21-148) on the solver above:
+ * the finite-volume Laplacian (uniform kappa = 1, no V/dt term: 1/dt = 0),
+ * insulated bottom faces whose flux b = Br S is the right-hand side (:38-44),
+ * and the value 0 on the top face, the source surface (:88-91: the face's
+ * coefficient enters the diagonal twice and couples to nothing).  The solve is
+ * the flexible CG with the multigrid V-cycle (:116), FP64, one system.  The
+ * setup replaces any heat setup of the handle: falcon_heat_apply and
+ * falcon_heat_vcycle then apply the PFSS operator.  Cubic power-of-two blocks.
+ * Magnetograms: for each bottom block (falcon_pfss_bottom_blocks, block
+ * order) Br at its nj x nk bottom faces, j fastest. */
+int falcon_pfss_setup(falcon_handle_t h);
+/* the blocks at the domain's inner radius, in block order, and their number */
+int falcon_pfss_bottom_blocks(falcon_handle_t h, int32_t *blocks, int *n);
+/* the areas r_F^2 (cos t_j - cos t_j+1) dphi of those blocks' bottom faces
+ * (Si_FLL(1,:,:), 3_5/ModMultigrid.f90:150-233), in magnetogram order */
+int falcon_pfss_bottom_areas(falcon_handle_t h, double *area);
+/* right-hand side b = Br S from a magnetogram (3_5/ModPFSS.f90:38-44) */
+int falcon_pfss_set_magnetogram(falcon_handle_t h, const double *br);
+/* ModMagnetogram_dipole_magnetogram_ALL (3_5/ModMagnetogram.f90:13-80): Br of
+ * n dipoles (centres xyz[3n], pole charges q[n], pole separations l[3n], cm)
+ * at the bottom faces, written to br (a magnetogram) */
+int falcon_pfss_dipole_magnetogram(falcon_handle_t h, int n, const double *xyz, const double *q, const double *l,
+                                   double *br);
+/* ModPFSS_solve with i_option = 1 (3_5/ModPFSS.f90:102-116): to ||b - A phi||/||b|| <= tol */
+int falcon_pfss_solve(falcon_handle_t h, double tol, int max_iter, int32_t *iters, double *relres, double *seconds);
+/* phi: nblocks x (n,n,n) Fortran order per block */
+int falcon_pfss_get_phi(falcon_handle_t h, double *phi);
+/* B = -grad(phi) (3_5/ModPFSS.f90:138-145 with ModSpherical_Grad_f_O2,
+ * 3_6/ModSpherical.f90:161-199), second order, at every cell: nblocks x 3 x
+ * (n,n,n) Fortran order (r, theta, phi components).  Neighbours across block
+ * faces are the operator's face ghosts; at the bottom phi_0 = phi_1 + Br dr,
+ * at the top phi_{n+1} = -phi_n. */
+int falcon_pfss_field(falcon_handle_t h, double *B);
+
 /* ---------------------------------------------------------------- learned correction
  * No counterpart in the reference.  A 1 -> 16 -> 16 -> 1 network of 3x3x3
  * convolutions over 16^3 block images (zero padding, ReLU after layers 1-2),

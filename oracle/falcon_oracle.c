@@ -1028,7 +1028,20 @@ static void lev_centres(const oblock *b, int m, double *xc[3], double *xf[3]) {
         }
 }
 
-orc_heat *orc_heat_create(orc_ctx *c, double kappa0, double r_ref) {
+/* kappa(r) = kappa0 (r/r_ref)^kpow, kpow -3 (heat conduction) or 0 (uniform:
+ * the PFSS Laplacian of 3_5/ModPFSS.f90:68-84) */
+static double h_kappa(double r, double kappa0, double r_ref, int kpow) {
+    if (kpow == 0) return kappa0;
+    double x = r / r_ref;
+    return kappa0 / (x * x * x);
+}
+
+orc_heat *orc_heat_create(orc_ctx *c, double kappa0, double r_ref) { return orc_heat_create_opt(c, kappa0, r_ref, -3, 0); }
+
+/* top_dirichlet: the top radial face holds the value 0 (3_5/ModPFSS.f90:88-91:
+ * Aii = Aii - Ai(top), Ai(top) = 0), i.e. its coefficient enters the diagonal
+ * twice and couples to nothing; otherwise the top is insulated */
+orc_heat *orc_heat_create_opt(orc_ctx *c, double kappa0, double r_ref, int kpow, int top_dirichlet) {
     orc_heat *H = calloc(1, sizeof(orc_heat));
     H->c = c; H->nb = c->nleaf;
     int n = c->ni;
@@ -1062,19 +1075,19 @@ orc_heat *orc_heat_create(orc_ctx *c, double kappa0, double r_ref) {
                    *Ak = L->Ak + (size_t)ib * (m + 1) * m * m;
             /* face index along its own axis first: Ai[(fi*m + j)*m + k], Aj[(i*(m+1)+fj)*m + k], Ak[(i*m+j)*(m+1)+fk] */
             for (int i = 0; i <= m; i++) for (int j = 0; j < m; j++) for (int k = 0; k < m; k++) {
-                double x = xf[0][i] / r_ref, kap = kappa0 / (x * x * x);
+                double kap = h_kappa(xf[0][i], kappa0, r_ref, kpow);
                 double S = xf[0][i] * xf[0][i] * (cos(xf[1][j]) - cos(xf[1][j + 1])) * dx[2];
                 double cf = kap * S / dx[0];
-                if ((i == 0 && b->bottom) || (i == m && b->top)) cf = 0.0;
+                if ((i == 0 && b->bottom) || (i == m && b->top && !top_dirichlet)) cf = 0.0;
                 Ai[((long)i * m + j) * m + k] = cf;
             }
             for (int i = 0; i < m; i++) for (int j = 0; j <= m; j++) for (int k = 0; k < m; k++) {
-                double x = xc[0][i + 1] / r_ref, kap = kappa0 / (x * x * x);
+                double kap = h_kappa(xc[0][i + 1], kappa0, r_ref, kpow);
                 double S = 0.5 * (xf[0][i + 1] * xf[0][i + 1] - xf[0][i] * xf[0][i]) * dx[2] * sin(xf[1][j]);
                 Aj[((long)i * (m + 1) + j) * m + k] = kap * S / (xc[0][i + 1] * dx[1]);
             }
             for (int i = 0; i < m; i++) for (int j = 0; j < m; j++) for (int k = 0; k <= m; k++) {
-                double x = xc[0][i + 1] / r_ref, kap = kappa0 / (x * x * x);
+                double kap = h_kappa(xc[0][i + 1], kappa0, r_ref, kpow);
                 double S = 0.5 * (xf[0][i + 1] * xf[0][i + 1] - xf[0][i] * xf[0][i]) * (xf[1][j + 1] - xf[1][j]);
                 Ak[((long)i * m + j) * (m + 1) + k] = kap * S / (xc[0][i + 1] * dx[2] * sin(xc[1][j + 1]));
             }
@@ -1085,9 +1098,10 @@ orc_heat *orc_heat_create(orc_ctx *c, double kappa0, double r_ref) {
                 off[q] = Ai[((long)i * m + j) * m + k] + Ai[((long)(i + 1) * m + j) * m + k] +
                          Aj[((long)i * (m + 1) + j) * m + k] + Aj[((long)i * (m + 1) + j + 1) * m + k] +
                          Ak[((long)i * m + j) * (m + 1) + k] + Ak[((long)i * m + j) * (m + 1) + k + 1];
+                if (top_dirichlet && b->top && i == m - 1) off[q] = off[q] + Ai[((long)m * m + j) * m + k];
                 if (l == 0) {
                     double r = xc[0][i + 1], th = xc[1][j + 1];
-                    double xx = r / r_ref, kap = kappa0 / (xx * xx * xx);
+                    double kap = h_kappa(r, kappa0, r_ref, kpow);
                     double h2 = r * dx[1], h3 = r * sin(th) * dx[2];
                     double v = 2.0 * kap * (1.0 / (dx[0] * dx[0]) + 1.0 / (h2 * h2) + 1.0 / (h3 * h3));
                     if (v > worst) worst = v;
@@ -1376,6 +1390,56 @@ void orc_heat_apply(orc_heat *H, double dt, const double *x, double *y) {
     free(xp);
 }
 
+/* B = -grad(phi) at every interior cell, the gradient of
+ * ModSpherical_Grad_f_O2 (3_6/ModSpherical.f90:161-199, second-order centred
+ * differences (1./2./dx)*(f(+1) - f(-1)), then 1/r and 1/sin(theta)) with the
+ * minus sign that 3_5/ModPFSS.f90:141 states (B = -grad Phi): with the flux
+ * right-hand side b = Br S of the bottom faces, -dPhi/dr = Br there.  Face
+ * ghosts as the operator's; at the domain's radial boundaries, the ghosts that
+ * the boundary conditions give: bottom Phi_0 = Phi_1 + Br dr, top (Phi = 0 on
+ * the face) Phi_{m+1} = -Phi_m.  br: for each bottom block in block order,
+ * Br at its bottom faces (j, k), j fastest.  B: nblocks x 3 x (m,m,m). */
+void orc_pfss_field(orc_heat *H, const double *phi, const double *br, double *B) {
+    const hlev *L = &H->L[0];
+    const int m = L->m, nb = H->nb;
+    const long M3 = (long)m * m * m, P = (long)(m + 2) * (m + 2) * (m + 2);
+    double *xp = malloc(sizeof(double) * (size_t)nb * P);
+    h_pad(m, nb, phi, xp);
+    h_exchange(L, xp);
+    double *xc[3], *xf[3];
+    for (int a = 0; a < 3; a++) { xc[a] = malloc(sizeof(double) * (m + 2)); xf[a] = malloc(sizeof(double) * (m + 1)); }
+    int nbot = 0;
+    for (int ib = 0; ib < nb; ib++) {
+        const oblock *b = &H->c->b[ib];
+        double *X = xp + (long)ib * P;
+        lev_centres(b, m, xc, xf);
+        double dx[3];
+        for (int a = 0; a < 3; a++) dx[a] = (b->rng[a][1] - b->rng[a][0]) / (double)m;
+        if (b->bottom) {
+            for (int k = 1; k <= m; k++) for (int j = 1; j <= m; j++)
+                X[HP(m, 0, j, k)] = X[HP(m, 1, j, k)] + br[(long)nbot * m * m + (long)(k - 1) * m + (j - 1)] * dx[0];
+            nbot++;
+        }
+        if (b->top)
+            for (int k = 1; k <= m; k++) for (int j = 1; j <= m; j++) X[HP(m, m + 1, j, k)] = -X[HP(m, m, j, k)];
+        const double cr = (double)(1.0f / 2.0f) / dx[0], ct = (double)(1.0f / 2.0f) / dx[1], cp = (double)(1.0f / 2.0f) / dx[2];
+        double *Br = B + (long)ib * 3 * M3, *Bt = Br + M3, *Bp = Bt + M3;
+        for (int k = 1; k <= m; k++) for (int j = 1; j <= m; j++) for (int i = 1; i <= m; i++) {
+            const long q = (long)(i - 1) + (long)m * ((long)(j - 1) + (long)m * (k - 1));
+            const double rinv = 1.0 / xc[0][i], sinv = 1.0 / sin(xc[1][j]);
+            const double gr = cr * (X[HP(m, i + 1, j, k)] - X[HP(m, i - 1, j, k)]);
+            double gt = ct * (X[HP(m, i, j + 1, k)] - X[HP(m, i, j - 1, k)]);
+            gt = gt * rinv;
+            double gp = cp * (X[HP(m, i, j, k + 1)] - X[HP(m, i, j, k - 1)]);
+            gp = gp * rinv;
+            gp = gp * sinv;
+            Br[q] = -gr; Bt[q] = -gt; Bp[q] = -gp;
+        }
+    }
+    for (int a = 0; a < 3; a++) { free(xc[a]); free(xf[a]); }
+    free(xp);
+}
+
 static void h_work_alloc(orc_heat *H, hwork *W) {
     for (int l = 0; l < H->nlev; l++) {
         int ml = H->L[l].m;
@@ -1427,13 +1491,25 @@ static double h_dot(int m, int nb, const double *a, int apad, const double *b, i
  * Returns iterations; x_out interior (Fortran order per block). */
 int orc_heat_solve(orc_heat *H, double dt, const double *Tn, double tol, int max_iter, double *x_out, double *relres) {
     const int nb = H->nb, m = H->L[0].m;
+    const long N = (long)nb * m * m * m;
+    const double inv_dt = 1.0 / dt;
+    double *b = malloc(sizeof(double) * N);
+    for (long c = 0; c < N; c++) b[c] = H->L[0].V[c] * inv_dt * Tn[c];
+    int it = orc_heat_solve_b(H, dt, b, tol, max_iter, x_out, relres);
+    free(b);
+    return it;
+}
+
+/* A x = b0 from x = 0 by the same flexible CG */
+int orc_heat_solve_b(orc_heat *H, double dt, const double *b0, double tol, int max_iter, double *x_out, double *relres) {
+    const int nb = H->nb, m = H->L[0].m;
     const long M3 = (long)m * m * m, P = (long)(m + 2) * (m + 2) * (m + 2), N = (long)nb * M3;
     const double inv_dt = 1.0 / dt;
     hwork W[8];
     h_work_alloc(H, W);
     double *b = malloc(sizeof(double) * N), *r = malloc(sizeof(double) * N), *x = calloc(N, sizeof(double));
     double *q = malloc(sizeof(double) * N), *p = calloc((size_t)nb * P, sizeof(double)), *z = malloc(sizeof(double) * nb * P);
-    for (long c = 0; c < N; c++) { b[c] = H->L[0].V[c] * inv_dt * Tn[c]; r[c] = b[c]; }
+    for (long c = 0; c < N; c++) { b[c] = b0[c]; r[c] = b[c]; }
     double bn = h_dot(m, nb, b, 0, b, 0);
     if (bn == 0.0) {  /* b = 0: the solution is 0 */
         memset(x_out, 0, sizeof(double) * N);

@@ -69,6 +69,10 @@ def lib():
         L.orc_heat_solve.argtypes = [vp, C.c_double, dp, C.c_double, C.c_int, dp, dp]
         L.orc_heat_set_coarse.argtypes = [vp, C.c_double, C.c_int]
         L.orc_heat_prolong.argtypes = [vp, C.c_int, dp, dp]
+        L.orc_heat_create_opt.restype = vp
+        L.orc_heat_create_opt.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_int]
+        L.orc_heat_solve_b.argtypes = [vp, C.c_double, dp, C.c_double, C.c_int, dp, dp]
+        L.orc_pfss_field.argtypes = [vp, dp, dp, dp]
         fp = C.POINTER(C.c_float)
         L.cnn_round_bf16.restype = C.c_float
         L.cnn_round_bf16.argtypes = [C.c_float]
@@ -202,10 +206,11 @@ class HeatOracle:
     (cubic blocks).  Interior arrays are (nblocks, n, n, n) in C view of Fortran
     (i fastest), i.e. numpy shape (nblocks, nk, nj, ni)."""
 
-    def __init__(self, orc, kappa0, r_ref_rsun):
+    def __init__(self, orc, kappa0, r_ref_rsun, kappa_power=-3, top_dirichlet=False):
         self.L = lib()
         self.o = orc
-        self.h = self.L.orc_heat_create(orc.h, kappa0, r_ref_rsun * float(np.float32(6.955e10)))
+        self.h = self.L.orc_heat_create_opt(orc.h, kappa0, r_ref_rsun * float(np.float32(6.955e10)), int(kappa_power),
+                                            int(bool(top_dirichlet)))
         self.n = orc.ni
         self.shape = (orc.nblocks, orc.nk, orc.nj, orc.ni)
 
@@ -257,6 +262,44 @@ class HeatOracle:
         rel = np.zeros(1)
         it = self.L.orc_heat_solve(self.h, dt, _p(Tn, C.c_double), tol, max_iter, _p(x, C.c_double), _p(rel, C.c_double))
         return x, it, float(rel[0])
+
+    def solve_b(self, dt, b, tol=1e-10, max_iter=500):
+        """A x = b from x = 0 (dt = inf: no V/dt term)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.shape)
+        rel = np.zeros(1)
+        it = self.L.orc_heat_solve_b(self.h, dt, _p(b, C.c_double), tol, max_iter, _p(x, C.c_double), _p(rel, C.c_double))
+        return x, it, float(rel[0])
+
+
+class PfssOracle(HeatOracle):
+    """The PFSS potential field of 3_5/ModPFSS.f90 as the oracle restates it:
+    the Laplacian (uniform kappa, no V/dt term), insulated bottom faces whose
+    flux b = Br S is the right-hand side, the value 0 on the top face (the
+    source surface); B = -grad(phi), second order.  br: (nbottom, nk, nj), the
+    bottom blocks in block order."""
+
+    def __init__(self, orc):
+        super().__init__(orc, 1.0, 1.0, kappa_power=0, top_dirichlet=True)
+        self.bottom = [ib for ib in range(orc.nblocks) if orc.block_info(ib)[1][2]]
+
+    def rhs(self, br, area):
+        """b = Br S on the first radial cell of every bottom block (area: the
+        bottom faces' areas, same shape as br)."""
+        b = np.zeros(self.shape)
+        for q, ib in enumerate(self.bottom):
+            b[ib, :, :, 0] = br[q] * area[q]
+        return b
+
+    def solve_br(self, br, area, tol=1e-10, max_iter=500):
+        return self.solve_b(float("inf"), self.rhs(br, area), tol, max_iter)
+
+    def field(self, phi, br):
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        br = np.ascontiguousarray(br, dtype=np.float64)
+        B = np.zeros((self.o.nblocks, 3) + self.shape[1:])
+        self.L.orc_pfss_field(self.h, _p(phi, C.c_double), _p(br, C.c_double), _p(B, C.c_double))
+        return B
 
 
 CNN_NPARAM = 7809
