@@ -1699,14 +1699,19 @@ int falcon_pfss_field(falcon_handle_t h, double *B) {
     });
 }
 
-static HeatSolver &need_heat(falcon_handle_t h) {
+// any: the operator-level calls (info, apply, V-cycle, graph and coarse-solve
+// settings) also act on the solver falcon_pfss_setup made; the heat-only calls
+// (dt, T, solve, training) refuse it
+static HeatSolver &need_heat(falcon_handle_t h, bool any = false) {
     if (!h->heat) throw Error("falcon_heat_setup has not been called");
+    if (h->pfss_on && !any)
+        throw Error("the handle's solver is set up for PFSS (falcon_pfss_setup); call falcon_heat_setup first");
     return *h->heat;
 }
 
 int falcon_heat_info(falcon_handle_t h, int *levels, int64_t *cells, double *dt_explicit) {
     return guard([&] {
-        HeatSolver &H = need_heat(h);
+        HeatSolver &H = need_heat(h, true);
         if (levels) *levels = H.levels();
         if (cells) *cells = H.cells();
         if (dt_explicit) *dt_explicit = H.explicit_dt();
@@ -1776,7 +1781,7 @@ int falcon_heat_get_T(falcon_handle_t h, int sys, double *T) {
 
 int falcon_heat_apply(falcon_handle_t h, const double *x, double *y) {
     return guard([&] {
-        HeatSolver &H = need_heat(h);
+        HeatSolver &H = need_heat(h, true);
         const long long n = H.cells() * H.nsys();
         const long long nblk = n / ((long long)H.m() * H.m() * H.m());
         std::vector<double> xd(n), yd;
@@ -1788,7 +1793,7 @@ int falcon_heat_apply(falcon_handle_t h, const double *x, double *y) {
 
 int falcon_heat_vcycle(falcon_handle_t h, const double *r, double *z) {
     return guard([&] {
-        HeatSolver &H = need_heat(h);
+        HeatSolver &H = need_heat(h, true);
         const long long n = H.cells() * H.nsys();
         const long long nblk = n / ((long long)H.m() * H.m() * H.m());
         std::vector<double> rd(n), zd;
@@ -1902,7 +1907,7 @@ int falcon_heat_train_corrector(falcon_handle_t h, int npairs, int blocks_per_pa
 }
 
 int falcon_heat_set_graph(falcon_handle_t h, int on) {
-    return guard([&] { need_heat(h).set_graph(on != 0); });
+    return guard([&] { need_heat(h, true).set_graph(on != 0); });
 }
 
 int falcon_heat_set_learned_blockstep(falcon_handle_t h, int on) {
@@ -1924,7 +1929,7 @@ int falcon_heat_set_learned_iters(falcon_handle_t h, int n) {
 }
 
 int falcon_heat_set_coarse(falcon_handle_t h, double tol, int max_iter) {
-    return guard([&] { need_heat(h).set_coarse(tol, max_iter); });
+    return guard([&] { need_heat(h, true).set_coarse(tol, max_iter); });
 }
 
 int falcon_heat_set_learned_sweeps(falcon_handle_t h, int pre0, int post0) {

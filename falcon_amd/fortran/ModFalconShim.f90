@@ -17,10 +17,14 @@ module ModFalconShim
     implicit none
     private
     public :: falcon_h, falcon_check, falcon_attach, falcon_push_state, falcon_push_pending, falcon_sync_to_host
+    public :: falcon_solver
 
     type(c_ptr), save   :: falcon_h = c_null_ptr
     ! the host state is newer than the device state (set by falcon_attach)
     logical, save       :: state_pending = .false.
+    ! the implicit solver the handle holds: 0 none, 1 heat conduction
+    ! (ModHeatConduction), 2 PFSS (ModPFSS); each set-up replaces the other
+    integer, save       :: falcon_solver = 0
 
 contains
 

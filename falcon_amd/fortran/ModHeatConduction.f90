@@ -5,13 +5,12 @@
 module ModHeatConduction
     use iso_c_binding
     use falcon_c
-    use ModFalconShim,  only: falcon_h, falcon_check
+    use ModFalconShim,  only: falcon_h, falcon_check, falcon_solver
     use ModParameters,  only: ni
     use ModYinYangTree, only: YYTree
     implicit none
     real(8), save   :: heat_kappa0 = 1.0d0          ! cgs
     real(8), save   :: heat_r_ref_rsun = 0.7d0
-    logical, save   :: heat_ready = .false.
 contains
     ! T(ni,ni,ni,nLocalBlocks): T' at the old time level in, the new level out
     ! (interior cells, cubic blocks); iters = FCG iterations.
@@ -23,9 +22,9 @@ contains
         integer, intent(out)                :: iters
         real(c_double)                      :: dts(1), relres(1), seconds
         integer(c_int)                      :: it(1)
-        if (.not. heat_ready) then
+        if (falcon_solver /= 1) then
             call falcon_check(falcon_heat_setup(falcon_h, heat_kappa0, heat_r_ref_rsun, 1_c_int))
-            heat_ready = .true.
+            falcon_solver = 1
         end if
         dts(1) = dt
         call falcon_check(falcon_heat_set_dt(falcon_h, dts))

@@ -152,6 +152,46 @@ module falcon_c
             real(c_double), intent(out)     :: T(*)
         end function falcon_heat_get_T
 
+        integer(c_int) function falcon_pfss_setup(h) bind(c, name='falcon_pfss_setup')
+            import :: c_int, c_ptr
+            type(c_ptr), value              :: h
+        end function falcon_pfss_setup
+
+        integer(c_int) function falcon_pfss_bottom_blocks(h, blocks, n) bind(c, name='falcon_pfss_bottom_blocks')
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value              :: h
+            integer(c_int32_t), intent(out) :: blocks(*)
+            integer(c_int), intent(out)     :: n
+        end function falcon_pfss_bottom_blocks
+
+        integer(c_int) function falcon_pfss_set_magnetogram(h, br) bind(c, name='falcon_pfss_set_magnetogram')
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value              :: h
+            real(c_double), intent(in)      :: br(*)
+        end function falcon_pfss_set_magnetogram
+
+        integer(c_int) function falcon_pfss_solve(h, tol, max_iter, iters, relres, seconds) &
+                bind(c, name='falcon_pfss_solve')
+            import :: c_int, c_int32_t, c_double, c_ptr
+            type(c_ptr), value              :: h
+            real(c_double), value           :: tol
+            integer(c_int), value           :: max_iter
+            integer(c_int32_t), intent(out) :: iters
+            real(c_double), intent(out)     :: relres, seconds
+        end function falcon_pfss_solve
+
+        integer(c_int) function falcon_pfss_get_phi(h, phi) bind(c, name='falcon_pfss_get_phi')
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value              :: h
+            real(c_double), intent(out)     :: phi(*)
+        end function falcon_pfss_get_phi
+
+        integer(c_int) function falcon_pfss_field(h, B) bind(c, name='falcon_pfss_field')
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value              :: h
+            real(c_double), intent(out)     :: B(*)
+        end function falcon_pfss_field
+
         integer(c_int) function falcon_last_error(buf, len) bind(c, name='falcon_last_error')
             import :: c_int, c_char
             character(kind=c_char), intent(out) :: buf(*)

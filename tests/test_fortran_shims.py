@@ -21,5 +21,5 @@ def test_shims_compile_against_reference_modules(tmp_path):
     shutil.copytree(src, work, ignore=shutil.ignore_patterns("_build"))
     r = subprocess.run(["make", "-s", "-C", str(work), f"REFMODS={REFMODS}"], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
-    for m in ("modadvance.mod", "modcommunication.mod", "modheatconduction.mod", "falcon_c.mod"):
+    for m in ("modadvance.mod", "modcommunication.mod", "modheatconduction.mod", "modpfss.mod", "falcon_c.mod"):
         assert (work / "_build" / m).exists(), m
