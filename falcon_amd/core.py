@@ -53,6 +53,9 @@ def load():
         "falcon_get_background": [vp, C.c_int, dp],
         "falcon_set_state": [vp, C.c_int, dp],
         "falcon_get_state": [vp, C.c_int, dp],
+        "falcon_set_divb": [vp, C.c_int],
+        "falcon_set_psi": [vp, C.c_int, dp],
+        "falcon_get_psi": [vp, C.c_int, dp],
         "falcon_init_synthetic": [vp, C.c_uint64, C.c_double, C.c_double],
         "falcon_set_precision": [vp, C.c_int],
         "falcon_set_ghost_pull": [vp, C.c_int],
@@ -246,6 +249,23 @@ class Shell:
     def get_state(self, ib):
         out = np.zeros((8,) + self.pad[::-1])
         _check(self.L.falcon_get_state(self.h, ib, _p(out)))
+        return out
+
+    def set_divb(self, option):
+        """DivB_option (3_6/ModParameters.f90:70): 0 none, 1 GLM divergence
+        cleaning with psi as a ninth stepped variable (FP64 path only)."""
+        _check(self.L.falcon_set_divb(self.h, int(option)))
+
+    def set_psi(self, ib, psi):
+        """padded psi of block ib, (NK, NJ, NI) (the C view of Fortran order)"""
+        psi = np.ascontiguousarray(psi, dtype=np.float64)
+        if psi.shape != tuple(self.pad[::-1]):
+            raise ValueError("psi must have shape %s" % (tuple(self.pad[::-1]),))
+        _check(self.L.falcon_set_psi(self.h, ib, _p(psi)))
+
+    def get_psi(self, ib):
+        out = np.zeros(self.pad[::-1])
+        _check(self.L.falcon_get_psi(self.h, ib, _p(out)))
         return out
 
     def init_synthetic(self, seed=12345, s_amp=1e-4, b_ratio=1e-3):

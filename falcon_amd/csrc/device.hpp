@@ -116,9 +116,22 @@ struct DevBoundary {
 // workgroup per tile, n > 0 launches n workgroups that loop over the tiles (a
 // persistent launch).  Results depend on neither; the dt partial minima are
 // laid out per tile of the diffusion's edge.
+// glm (FP64 only; null: off): grad(psi) per interior cell, [block][3][ni*nj*nk],
+// subtracted from R(B) before the update (GLM divergence cleaning)
 template <typename T, typename TIN, typename TOUT, typename TB>
 void launch_stage(const Dims &d, const DevTables &t, const TIN *in, const TB *U, TOUT *out, float *shadow,
-                  const double *dtp, double div, hipStream_t s, const int32_t *pull, int tile, int wgs);
+                  const double *dtp, double div, hipStream_t s, const int32_t *pull, int tile, int wgs,
+                  const double *glm = nullptr);
+// GLM divergence cleaning (FP64 path).  psi buffers are [block][P] in the
+// cell_off layout.  Stage: psi_out = psi_n + dt R(psi)/div from the stage
+// input (state `in`, psi_in) and the step's state Un, and G = grad(psi_in);
+// diffusion: psi_new = psi + 0.5 dt R_diff(psi) with the speeds of U;
+// exchange: psi's ghosts from the FP64 ghost tables.
+void launch_glm_stage(const Dims &d, const DevTables &t, const double *in, const double *Un, const double *psi_in,
+                      const double *psi_n, double *psi_out, double *G, const double *dtp, double div, hipStream_t s);
+void launch_glm_diffusion(const Dims &d, const DevTables &t, const double *U, const double *psi, double *psi_new,
+                          const double *dtp, hipStream_t s);
+void launch_exchange_psi(const Dims &d, const DevGhosts &g, double *psi, hipStream_t s);
 // Uold: U^n for the FP32 path, whose diffusion updates U (= out) in place;
 // null for the FP64 path (U != out)
 template <typename T, typename TIN>

@@ -122,6 +122,11 @@ struct falcon_handle {
     bool ghost_pull = true;              // FP32 step: kernels read same-level ghost copies in place
 
     DevBuf U[2], F32[2], S64[2], S32[2], dtp, partial, rbuf;
+    // GLM divergence cleaning (falcon_set_divb, FP64 path): psi beside U[] and
+    // the stage buffers, one padded variable per block, and grad(psi) of the
+    // current stage
+    int divb = 0;
+    DevBuf psi[2], psiS[2], glmG;
     DevBuf diffbuf;          // falcon_state_diff's reduction words
     DevBuf scanbuf;          // falcon_check_state's count and first position
     int cur = 0;             // index of the current state buffer in U[]
@@ -133,7 +138,7 @@ struct falcon_handle {
     std::vector<hipEvent_t> *prof_ev = nullptr;
     std::vector<int> *prof_kind = nullptr;
     hipGraphExec_t graph = nullptr;
-    int graph_cur = -1, graph_prec = -1;
+    int graph_cur = -1, graph_prec = -1, graph_divb = -1;
     // launch shape of the stage and diffusion marches per precision (0 FP64,
     // 1 FP32): (theta, phi) tile edge, 8 or 16, and workgroups (0: one per
     // tile; n > 0: n workgroups looping over the tiles); falcon_autotune picks
@@ -158,7 +163,8 @@ struct falcon_handle {
                          &ghost32[1], &ghost32[2], &ghost32[3], &ghost_copy32, &ghost_lerp32[0], &ghost_lerp32[1],
                          &bnd_list, &ghost_rest32, &ghost_pull32,
 
-                         &U[0], &U[1], &F32[0], &F32[1], &S64[0], &S64[1], &S32[0], &S32[1], &dtp, &partial, &rbuf, &diffbuf, &scanbuf};
+                         &U[0], &U[1], &F32[0], &F32[1], &S64[0], &S64[1], &S32[0], &S32[1], &dtp, &partial, &rbuf, &diffbuf, &scanbuf,
+                         &psi[0], &psi[1], &psiS[0], &psiS[1], &glmG};
         for (DevBuf *b : all) b->free();
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -548,17 +554,37 @@ void enqueue_step(falcon_handle *h, double cfl, int cur) {
         double *S0 = h->S64[0].as<double>(), *S1 = h->S64[1].as<double>();
         const double *in[4] = {U, S0, S1, S0};
         double *out[4] = {S0, S1, S0, U};
+        // GLM: psi follows the same buffer roles; its radial boundary ghosts
+        // are never written (ModBoundary sets none), so every psi buffer
+        // keeps the ones falcon_set_psi gave it
+        const bool glm = h->divb == 1;
+        double *P = glm ? h->psi[cur].as<double>() : nullptr, *Pn = glm ? h->psi[1 - cur].as<double>() : nullptr;
+        double *PS0 = glm ? h->psiS[0].as<double>() : nullptr, *PS1 = glm ? h->psiS[1].as<double>() : nullptr;
+        double *G = glm ? h->glmG.as<double>() : nullptr;
+        const double *pin[4] = {P, PS0, PS1, PS0};
+        double *pout[4] = {PS0, PS1, PS0, P};
         for (int k = 1; k <= 4; k++) {
+            const double div = (double)(5.0f - (float)k);
             timed(h, K_BC, [&] { launch_boundary<double>(d, h->bnd, const_cast<double *>(in[k - 1]), s); });
+            if (glm)
+                timed(h, K_OTHER,
+                      [&] { launch_glm_stage(d, t, in[k - 1], U, pin[k - 1], P, pout[k - 1], G, dtp, div, s); });
             timed(h, K_STAGE, [&] {
-                launch_stage<double, double, double, double>(d, t, in[k - 1], U, out[k - 1], nullptr, dtp,
-                                                             (double)(5.0f - (float)k), s, nullptr, ts, ws);
+                launch_stage<double, double, double, double>(d, t, in[k - 1], U, out[k - 1], nullptr, dtp, div, s,
+                                                             nullptr, ts, ws, G);
             });
-            timed(h, k < 4 ? K_XSTAGE : K_XSTATE, [&] { launch_exchange<double>(d, h->gh64, out[k - 1], s); });
+            timed(h, k < 4 ? K_XSTAGE : K_XSTATE, [&] {
+                launch_exchange<double>(d, h->gh64, out[k - 1], s);
+                if (glm) launch_exchange_psi(d, h->gh64, pout[k - 1], s);
+            });
         }
         timed(h, K_DIFF, [&] { launch_diffusion<double, double>(d, t, U, U, Un, nullptr, part, dtp, s, nullptr, td, wd); });
+        if (glm) timed(h, K_OTHER, [&] { launch_glm_diffusion(d, t, U, P, Pn, dtp, s); });
         timed(h, K_OTHER, [&] { launch_copy_radial<double>(d, h->bnd, U, Un, s); });
-        timed(h, K_XSTATE, [&] { launch_exchange<double>(d, h->gh64, Un, s); });
+        timed(h, K_XSTATE, [&] {
+            launch_exchange<double>(d, h->gh64, Un, s);
+            if (glm) launch_exchange_psi(d, h->gh64, Pn, s);
+        });
     } else {
         float *S0 = h->S32[0].as<float>(), *S1 = h->S32[1].as<float>();
         float *F = h->F32[cur].as<float>(), *Fn = h->F32[1 - cur].as<float>();
@@ -971,6 +997,62 @@ int falcon_get_state(falcon_handle_t h, int ib, double *prim) {
     });
 }
 
+int falcon_set_divb(falcon_handle_t h, int option) {
+    return guard([&] {
+        if (option != 0 && option != 1) throw Error("DivB option must be 0 (none) or 1 (GLM)");
+        h->need_tables();
+        if (option == 1 && h->precision != 0) throw Error("GLM divergence cleaning runs on the FP64 path only");
+        if (option == 1 && !h->psi[0].p) {
+            const Dims &d = h->dims;
+            const size_t n = (size_t)d.nblocks * d.P, ng = (size_t)d.nblocks * 3 * d.ni * d.nj * d.nk;
+            for (DevBuf *b : {&h->psi[0], &h->psi[1], &h->psiS[0], &h->psiS[1]}) {
+                b->alloc(n * sizeof(double));
+                HIPCHECK(hipMemset(b->p, 0, n * sizeof(double)));
+            }
+            h->glmG.alloc(ng * sizeof(double));
+            HIPCHECK(hipMemset(h->glmG.p, 0, ng * sizeof(double)));
+        }
+        h->divb = option;
+    });
+}
+
+int falcon_set_psi(falcon_handle_t h, int ib, const double *psi) {
+    return guard([&] {
+        h->need_tables();
+        h->check_block(ib);
+        if (h->divb != 1) throw Error("psi is stepped only with GLM divergence cleaning (falcon_set_divb(h, 1))");
+        const Dims &d = h->dims;
+        std::vector<double> tmp((size_t)d.P);
+        for (int K = 0; K < d.NK; K++)
+            for (int J = 0; J < d.NJ; J++)
+                for (int I = 0; I < d.NI; I++)
+                    tmp[cell_off(d, I, J, K)] = psi[(size_t)I + (size_t)d.NI * (J + (size_t)d.NJ * K)];
+        HIPCHECK(hipStreamSynchronize(h->stream));
+        // every psi buffer: the reference starts primitive_rk_IV as a copy of
+        // primitive_IV, and no kernel writes psi's radial boundary ghosts
+        for (DevBuf *b : {&h->psi[0], &h->psi[1], &h->psiS[0], &h->psiS[1]})
+            HIPCHECK(hipMemcpy(b->as<double>() + (size_t)ib * d.P, tmp.data(), sizeof(double) * d.P,
+                               hipMemcpyHostToDevice));
+    });
+}
+
+int falcon_get_psi(falcon_handle_t h, int ib, double *psi) {
+    return guard([&] {
+        h->need_tables();
+        h->check_block(ib);
+        if (h->divb != 1) throw Error("psi is stepped only with GLM divergence cleaning (falcon_set_divb(h, 1))");
+        const Dims &d = h->dims;
+        std::vector<double> tmp((size_t)d.P);
+        HIPCHECK(hipStreamSynchronize(h->stream));
+        HIPCHECK(hipMemcpy(tmp.data(), h->psi[h->cur].as<double>() + (size_t)ib * d.P, sizeof(double) * d.P,
+                           hipMemcpyDeviceToHost));
+        for (int K = 0; K < d.NK; K++)
+            for (int J = 0; J < d.NJ; J++)
+                for (int I = 0; I < d.NI; I++)
+                    psi[(size_t)I + (size_t)d.NI * (J + (size_t)d.NJ * K)] = tmp[cell_off(d, I, J, K)];
+    });
+}
+
 int falcon_init_synthetic(falcon_handle_t h, uint64_t seed, double s_amp, double b_ratio) {
     return guard([&] {
         h->need_tables();
@@ -1031,6 +1113,7 @@ int falcon_init_synthetic(falcon_handle_t h, uint64_t seed, double s_amp, double
 int falcon_set_precision(falcon_handle_t h, int mode) {
     return guard([&] {
         if (mode != 0 && mode != 1) throw Error("precision must be 0 (FP64) or 1 (FP32 deviation)");
+        if (mode == 1 && h->divb) throw Error("GLM divergence cleaning runs on the FP64 path only");
         if (mode != h->precision && h->dt.meta) {
             refresh_u_ghosts(h);
             h->shadow_valid = false;
@@ -1202,11 +1285,13 @@ int falcon_sendrecv_gc(falcon_handle_t h, int which) {
         ensure_device_tables(h);
         if (which == 0) {
             launch_exchange<double>(h->dims, h->gh64, h->U[h->cur].as<double>(), h->stream);
+            if (h->divb) launch_exchange_psi(h->dims, h->gh64, h->psi[h->cur].as<double>(), h->stream);
             h->shadow_valid = false;
         } else {
             alloc_stage(h);
             if (h->precision == 0) launch_exchange<double>(h->dims, h->gh64, h->S64[0].as<double>(), h->stream);
             else launch_exchange<float>(h->dims, h->gh32, h->S32[0].as<float>(), h->stream);
+            if (h->divb) launch_exchange_psi(h->dims, h->gh64, h->psiS[0].as<double>(), h->stream);
         }
         check_launch();
         HIPCHECK(hipStreamSynchronize(h->stream));
@@ -1237,7 +1322,7 @@ int falcon_advance(falcon_handle_t h, int nsteps, double cfl, int use_graph, dou
         prepare_steps(h);
         if (use_graph && nsteps >= 2) {
             if (!h->graph || h->graph_cur != h->cur || h->graph_prec != h->precision || h->graph_cfl != cfl ||
-                h->graph_fixed_dt != h->fixed_dt) {
+                h->graph_fixed_dt != h->fixed_dt || h->graph_divb != h->divb) {
                 if (h->graph) { HIPCHECK(hipGraphExecDestroy(h->graph)); h->graph = nullptr; }
                 hipGraph_t g;
                 HIPCHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
@@ -1246,7 +1331,7 @@ int falcon_advance(falcon_handle_t h, int nsteps, double cfl, int use_graph, dou
                 HIPCHECK(hipStreamEndCapture(h->stream, &g));
                 HIPCHECK(hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0));
                 HIPCHECK(hipGraphDestroy(g));
-                h->graph_cur = h->cur; h->graph_prec = h->precision; h->graph_cfl = cfl;
+                h->graph_cur = h->cur; h->graph_prec = h->precision; h->graph_cfl = cfl; h->graph_divb = h->divb;
                 h->graph_fixed_dt = h->fixed_dt;
             }
             int pairs = nsteps / 2;

@@ -334,7 +334,8 @@ template <typename T, typename TIN, typename TOUT, typename TB, int MODE, bool S
 __global__ void __launch_bounds__(TL * TL) k_stage(Dims d, DevTables t, const TIN *__restrict__ in,
                                                const TB *__restrict__ U, TOUT *__restrict__ out,
                                                float *__restrict__ shadow, const double *__restrict__ dtp,
-                                               double div, const int32_t *__restrict__ pull) {
+                                               double div, const int32_t *__restrict__ pull,
+                                               const double *__restrict__ glm) {
     constexpr int PL = tile_pl<TL>();
     __shared__ T lds[2][8][PL][PL];
     // LEAN (FP32 storage): the block's radial coefficient rows (xi, rho0, gp,
@@ -448,6 +449,17 @@ __global__ void __launch_bounds__(TL * TL) k_stage(Dims d, DevTables t, const TI
                 cell_rhs<double, T, RECIP>(win, L, xr, rho0r, gpr, gqr, st, coef(4, I), coef(5, I), coef(6, I),
                                            coef(7, I), coef(8, I), sinv, cot, cr, ct, cp, RECIP ? coef(9, I) : 1.0, R);
                 if (MODE == 0) {
+                    if constexpr (sizeof(T) == 8) {
+                        if (glm) {
+                            // ModDivB_GLM (3_6/ModDivB.f90:27-28): R(B) = R(B) - grad(psi)
+                            const long long nn = (long long)d.ni * d.nj * d.nk;
+                            const long long gb =
+                                (long long)b * 3 * nn + ((long long)i * d.nj + (j0 + ty)) * d.nk + (k0 + tx);
+                            R[4] = R[4] - glm[gb];
+                            R[5] = R[5] - glm[gb + nn];
+                            R[6] = R[6] - glm[gb + 2 * nn];
+                        }
+                    }
                     const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + m.coff;
     #pragma unroll
                     for (int v = 0; v < 8; v++) {
@@ -1228,36 +1240,38 @@ int march_resident_wgs(int kernel, int precision, int tile) {
 
 template <int TL, typename T, typename TIN, typename TOUT, typename TB>
 static void stage_tl(const Dims &d, const DevTables &t, const TIN *in, const TB *U, TOUT *out, float *shadow,
-                     const double *dtp, double div, hipStream_t s, const int32_t *pull, int wgs) {
+                     const double *dtp, double div, hipStream_t s, const int32_t *pull, int wgs, const double *glm) {
     const dim3 g = march_grid(d, TL, wgs), b(TL * TL);
     if (shadow && wgs)
         hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, true, TL, true>), g, b, 0, s, d, t, in, U, out, shadow, dtp, div,
-                           pull);
+                           pull, glm);
     else if (shadow)
         hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, true, TL, false>), g, b, 0, s, d, t, in, U, out, shadow, dtp, div,
-                           pull);
+                           pull, glm);
     else if (wgs)
         hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, false, TL, true>), g, b, 0, s, d, t, in, U, out,
-                           (float *)nullptr, dtp, div, pull);
+                           (float *)nullptr, dtp, div, pull, glm);
     else
         hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, false, TL, false>), g, b, 0, s, d, t, in, U, out,
-                           (float *)nullptr, dtp, div, pull);
+                           (float *)nullptr, dtp, div, pull, glm);
 }
 
 template <typename T, typename TIN, typename TOUT, typename TB>
 void launch_stage(const Dims &d, const DevTables &t, const TIN *in, const TB *U, TOUT *out, float *shadow,
-                  const double *dtp, double div, hipStream_t s, const int32_t *pull, int tile, int wgs) {
+                  const double *dtp, double div, hipStream_t s, const int32_t *pull, int tile, int wgs,
+                  const double *glm) {
     check_tile(tile);
     check_stage_ni(d);
-    if (tile == 16) stage_tl<16, T>(d, t, in, U, out, shadow, dtp, div, s, pull, wgs);
-    else stage_tl<8, T>(d, t, in, U, out, shadow, dtp, div, s, pull, wgs);
+    if (glm && sizeof(T) != 8) throw std::runtime_error("GLM divergence cleaning runs on the FP64 path only");
+    if (tile == 16) stage_tl<16, T>(d, t, in, U, out, shadow, dtp, div, s, pull, wgs, glm);
+    else stage_tl<8, T>(d, t, in, U, out, shadow, dtp, div, s, pull, wgs, glm);
 }
 
 template <int TL, bool PERSIST, typename T, typename TIN>
 static void rhs_tl(const Dims &d, const DevTables &t, const TIN *in, double *R, hipStream_t s, int wgs) {
     hipLaunchKernelGGL((k_stage<T, TIN, double, double, 1, false, TL, PERSIST>), march_grid(d, TL, wgs), dim3(TL * TL), 0,
                        s, d, t, in, (const double *)nullptr, R, (float *)nullptr, (const double *)nullptr, 1.0,
-                       (const int32_t *)nullptr);
+                       (const int32_t *)nullptr, (const double *)nullptr);
 }
 
 template <typename T, typename TIN>
@@ -1567,10 +1581,213 @@ void launch_allcells(const Dims &d, const double *U, const double *geo, const in
                        out);
 }
 
+// ---------------------------------------------------------------- GLM divergence cleaning
+
+// psi (the ninth variable of the reference's MHD blocks, 3_6/ModBlock.f90:163)
+// is one padded variable per block, [block][P] in the cell_off layout; the
+// ghost tables address [block][8][P] state buffers, so an entry's offset o
+// maps to (o / 8P) P + o mod 8P.
+__device__ __forceinline__ long long psi_off(const Dims &d, long long o) {
+    const long long P8 = 8 * d.P, b = o / P8;
+    return b * d.P + (o - b * P8);
+}
+
+// interior cell c of all blocks (phi fastest): block, interior and padded indices
+struct GlmCell {
+    int b, i, j, k, I, J, K;
+};
+__device__ __forceinline__ GlmCell glm_cell(const Dims &d, long long c) {
+    const long long nn = (long long)d.ni * d.nj * d.nk;
+    GlmCell g;
+    g.b = (int)(c / nn);
+    long long r = c - (long long)g.b * nn;
+    g.k = (int)(r % d.nk);
+    r /= d.nk;
+    g.j = (int)(r % d.nj);
+    g.i = (int)(r / d.nj);
+    g.I = g.i + d.ng; g.J = g.j + d.ng; g.K = g.k + d.ng;
+    return g;
+}
+
+// ModDivB_GLM (3_6/ModDivB.f90:13-38) at one RK stage, one lane per interior
+// cell, on the stage input `in` (radial boundaries applied) and psi_in:
+//   div B    ModSpherical_div (3_6/ModSpherical.f90:104-159): A_r r^2 along r
+//            times 1/r^2, plus (A_t sin(t) along theta + A_p along phi) times
+//            1/sin(t), then 1/r;
+//   grad psi ModSpherical_Grad_f (:201-239);
+//   R(psi) = -c_h^2 div B - (c_h / h_LLL) psi, c_h = v_wave of the step's
+//            state Un (ModWaveSpeed_Dynamo at the step's start, as k_dt_tile).
+// psi_out = psi_n + dt R(psi) / div (3_6/ModAdvance.f90:88-93), and grad psi
+// goes to G ([block][3][interior cells]) for the stage kernel to subtract
+// from R(B).  psi_out may be psi_n (the fourth stage): each lane reads psi_n
+// at its own cell only.
+__global__ void __launch_bounds__(256) k_glm_stage(Dims d, DevTables t, const double *__restrict__ in,
+                                                   const double *__restrict__ Un, const double *__restrict__ psi_in,
+                                                   const double *psi_n, double *psi_out, double *__restrict__ G,
+                                                   const double *__restrict__ dtp, double div) {
+    const long long nn = (long long)d.ni * d.nj * d.nk;
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nn * d.nblocks) return;
+    const GlmCell g = glm_cell(d, c);
+    const int I = g.I, J = g.J, K = g.K;
+    const BlockMeta mt = t.meta[g.b];
+    const long long rb = (long long)g.b * d.NI, tb = (long long)g.b * d.NJ;
+    const double *S = in + (long long)g.b * 8 * d.P;
+    const double *Ps = psi_in + (long long)g.b * d.P;
+    auto B = [&](int v, int II, int JJ, int KK) -> double { return S[(long long)v * d.P + cell_off(d, II, JJ, KK)]; };
+    auto a1 = [&](int II) -> double {
+        const double x = t.xi[rb + II];
+        return B(4, II, J, K) * (x * x);
+    };
+    auto a2 = [&](int JJ) -> double { return B(5, I, JJ, K) * t.sint[tb + JJ]; };
+    auto a3 = [&](int KK) -> double { return B(6, I, J, KK); };
+    const double rinv = t.rinv[rb + I], sinv = t.sinv[tb + J];
+    const double dr = d4(a1(I - 2), a1(I - 1), a1(I + 1), a1(I + 2), mt.cr) * t.r2inv[rb + I];
+    double d23 = d4(a2(J - 2), a2(J - 1), a2(J + 1), a2(J + 2), mt.ct);
+    d23 = d23 + d4(a3(K - 2), a3(K - 1), a3(K + 1), a3(K + 2), mt.cp);
+    d23 = d23 * sinv;
+    d23 = d23 * rinv;
+    const double divb = dr + d23;
+    auto p = [&](int II, int JJ, int KK) -> double { return Ps[cell_off(d, II, JJ, KK)]; };
+    const double g1 = d4(p(I - 2, J, K), p(I - 1, J, K), p(I + 1, J, K), p(I + 2, J, K), mt.cr);
+    const double g2 = d4(p(I, J - 2, K), p(I, J - 1, K), p(I, J + 1, K), p(I, J + 2, K), mt.ct) * rinv;
+    double g3 = d4(p(I, J, K - 2), p(I, J, K - 1), p(I, J, K + 1), p(I, J, K + 2), mt.cp) * rinv;
+    g3 = g3 * sinv;
+    const long long q = cell_off(d, I, J, K);
+    const double *Ub = Un + (long long)g.b * 8 * d.P + q;
+    const double vr = Ub[d.P], vt = Ub[2 * d.P], vp = Ub[3 * d.P];
+    const double br = Ub[4 * d.P], bt = Ub[5 * d.P], bp = Ub[6 * d.P];
+    double w = t.cs[rb + I] + sqrt(vr * vr + vt * vt + vp * vp);
+    w = w + sqrt(br * br + bt * bt + bp * bp) / t.sqrt_rho0[rb + I];
+    const double xi = t.xi[rb + I];
+    double h = mt.dxi, h2 = xi * mt.dxj, h3 = (xi * t.sint[tb + J]) * mt.dxk;
+    h = h < h2 ? h : h2;
+    h = h < h3 ? h : h3;
+    const double rpsi = -((w * w) * divb) - (w / h) * Ps[q];
+    const long long pq = (long long)g.b * d.P + q;
+    psi_out[pq] = psi_n[pq] + (*dtp * rpsi) / div;
+    const long long gb = (long long)g.b * 3 * nn + ((long long)g.i * d.nj + g.j) * d.nk + g.k;
+    G[gb] = g1;
+    G[gb + nn] = g2;
+    G[gb + 2 * nn] = g3;
+}
+
+// The artificial diffusion of psi after the fourth stage
+// (3_6/ModDiffusion.f90:9-150, h = 2, the ninth variable; 3_6/ModAdvance.f90:
+// 101-108): the faces of k_diffusion's FP64 instantiation, the speeds of the
+// state U where the reference reads them (transverse index - ng), and
+// psi_new = psi + (0.5 dt) R.
+__global__ void __launch_bounds__(256) k_glm_diffusion(Dims d, DevTables t, const double *__restrict__ U,
+                                                       const double *__restrict__ psi, double *__restrict__ psi_new,
+                                                       const double *__restrict__ dtp) {
+    const long long nn = (long long)d.ni * d.nj * d.nk;
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nn * d.nblocks) return;
+    const GlmCell g = glm_cell(d, c);
+    const int I = g.I, J = g.J, K = g.K;
+    const BlockMeta mt = t.meta[g.b];
+    const long long rb = (long long)g.b * d.NI, tb = (long long)g.b * d.NJ;
+    const double *Sb = U + (long long)g.b * 8 * d.P;
+    const double *Pb = psi + (long long)g.b * d.P;
+    const double *cs = t.cs + rb;
+    auto ps = [&](int II, int JJ, int KK) -> double { return Pb[cell_off(d, II, JJ, KK)]; };
+    auto sp = [&](int v, int II, int JJ, int KK) -> double {
+        return fabs(Sb[(long long)v * d.P + cell_off(d, II, JJ, KK)]);
+    };
+    const double c1m = sp(1, I - 1, J - 2, K - 2) + cs[I - 1];
+    const double c10 = sp(1, I, J - 2, K - 2) + cs[I];
+    const double c1p = sp(1, I + 1, J - 2, K - 2) + cs[I + 1];
+    const double f0 = (face_flux(ps(I - 2, J, K), ps(I - 1, J, K), ps(I, J, K), ps(I + 1, J, K)) * (c10 + c1m)) * 0.5;
+    const double f1 = (face_flux(ps(I - 1, J, K), ps(I, J, K), ps(I + 1, J, K), ps(I + 2, J, K)) * (c1p + c10)) * 0.5;
+    const double csI2 = cs[I - 2];
+    const double c2m = sp(2, I - 2, J - 1, K - 2) + csI2, c20 = sp(2, I - 2, J, K - 2) + csI2,
+                 c2p = sp(2, I - 2, J + 1, K - 2) + csI2;
+    const double fj0 = (face_flux(ps(I, J - 2, K), ps(I, J - 1, K), ps(I, J, K), ps(I, J + 1, K)) * (c20 + c2m)) * 0.5;
+    const double fj1 = (face_flux(ps(I, J - 1, K), ps(I, J, K), ps(I, J + 1, K), ps(I, J + 2, K)) * (c2p + c20)) * 0.5;
+    const double c3m = sp(3, I - 2, J - 2, K - 1) + csI2, c30 = sp(3, I - 2, J - 2, K) + csI2,
+                 c3p = sp(3, I - 2, J - 2, K + 1) + csI2;
+    const double fk0 = (face_flux(ps(I, J, K - 2), ps(I, J, K - 1), ps(I, J, K), ps(I, J, K + 1)) * (c30 + c3m)) * 0.5;
+    const double fk1 = (face_flux(ps(I, J, K - 1), ps(I, J, K), ps(I, J, K + 1), ps(I, J, K + 2)) * (c3p + c30)) * 0.5;
+    const double xic = t.xi[rb + I];
+    const double L2 = mt.dxj * xic;
+    const double L3 = (mt.dxk * xic) * t.sint[tb + J];
+    double r = 0.0;
+    r = r + (f0 - f1) / mt.dxi;
+    r = r + (fj0 - fj1) / L2;
+    r = r + (fk0 - fk1) / L3;
+    const long long pq = (long long)g.b * d.P + cell_off(d, I, J, K);
+    psi_new[pq] = psi[pq] + (0.5 * *dtp) * r;
+}
+
+// psi's ghost fill from the FP64 ghost tables: plain copies, then the
+// weighted entries (zero-weight corners skipped, as k_exchange's FP64
+// instantiation); psi is a scalar, so Yin-Yang entries are not rotated.
+__global__ void __launch_bounds__(256) k_exchange_copy_psi(Dims d, const int2 *__restrict__ copy, int n,
+                                                           double *__restrict__ psi) {
+    const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= n) return;
+    const int2 c = copy[e];
+    psi[psi_off(d, c.x)] = psi[psi_off(d, c.y)] + 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_exchange_psi(Dims d, DevGhosts g, double *__restrict__ psi) {
+    const int e = g.first + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= g.n_total) return;
+    const int SI = (int)d.SI, SJ = d.SJ;
+    const double wi = g.w[3 * (long long)e], wj = g.w[3 * (long long)e + 1], wk = g.w[3 * (long long)e + 2];
+    const bool i0 = wi != 1.0, i1 = wi != 0.0, j0 = wj != 1.0, j1 = wj != 0.0, k0 = wk != 1.0, k1 = wk != 0.0;
+    const double *F = psi + psi_off(d, g.src[e]);
+    const double f000 = (i0 && j0 && k0) ? F[0] : 0.0;
+    const double f100 = (i1 && j0 && k0) ? F[SI] : 0.0;
+    const double f010 = (i0 && j1 && k0) ? F[SJ] : 0.0;
+    const double f001 = (i0 && j0 && k1) ? F[1] : 0.0;
+    const double f110 = (i1 && j1 && k0) ? F[SI + SJ] : 0.0;
+    const double f011 = (i0 && j1 && k1) ? F[SJ + 1] : 0.0;
+    const double f101 = (i1 && j0 && k1) ? F[SI + 1] : 0.0;
+    const double f111 = (i1 && j1 && k1) ? F[SI + SJ + 1] : 0.0;
+    double v = f000 * (1.0 - wi) * (1.0 - wj) * (1.0 - wk);
+    v = v + f100 * (wi) * (1.0 - wj) * (1.0 - wk);
+    v = v + f010 * (1.0 - wi) * (wj) * (1.0 - wk);
+    v = v + f001 * (1.0 - wi) * (1.0 - wj) * (wk);
+    v = v + f110 * (wi) * (wj) * (1.0 - wk);
+    v = v + f011 * (1.0 - wi) * (wj) * (wk);
+    v = v + f101 * (wi) * (1.0 - wj) * (wk);
+    v = v + f111 * (wi) * (wj) * (wk);
+    psi[psi_off(d, g.dst[e])] = v;
+}
+
+static unsigned glm_grid(const Dims &d) {
+    const long long n = (long long)d.ni * d.nj * d.nk * d.nblocks;
+    return (unsigned)((n + 255) / 256);
+}
+
+void launch_glm_stage(const Dims &d, const DevTables &t, const double *in, const double *Un, const double *psi_in,
+                      const double *psi_n, double *psi_out, double *G, const double *dtp, double div, hipStream_t s) {
+    if (psi_in == psi_out) throw std::runtime_error("GLM stage: psi_in and psi_out must differ");
+    hipLaunchKernelGGL(k_glm_stage, dim3(glm_grid(d)), dim3(256), 0, s, d, t, in, Un, psi_in, psi_n, psi_out, G, dtp,
+                       div);
+}
+
+void launch_glm_diffusion(const Dims &d, const DevTables &t, const double *U, const double *psi, double *psi_new,
+                          const double *dtp, hipStream_t s) {
+    if (psi == psi_new) throw std::runtime_error("GLM diffusion: psi and psi_new must differ");
+    hipLaunchKernelGGL(k_glm_diffusion, dim3(glm_grid(d)), dim3(256), 0, s, d, t, U, psi, psi_new, dtp);
+}
+
+void launch_exchange_psi(const Dims &d, const DevGhosts &g, double *psi, hipStream_t s) {
+    if (g.n_lerp > 0) throw std::runtime_error("psi exchange: the FP64 ghost tables have no r-only entries");
+    if (g.n_copy > 0)
+        hipLaunchKernelGGL(k_exchange_copy_psi, dim3((unsigned)((g.n_copy + 255) / 256)), dim3(256), 0, s, d, g.copy,
+                           g.n_copy, psi);
+    if (g.n_total > g.first)
+        hipLaunchKernelGGL(k_exchange_psi, dim3((unsigned)((g.n_total - g.first + 255) / 256)), dim3(256), 0, s, d, g,
+                           psi);
+}
+
 // explicit instantiations: FP64 path, FP32-deviation path
-template void launch_stage<double, double, double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, const double *, double, hipStream_t, const int32_t *, int, int);
-template void launch_stage<float, float, float, float>(const Dims &, const DevTables &, const float *, const float *, float *, float *, const double *, double, hipStream_t, const int32_t *, int, int);
-template void launch_stage<float, float, double, double>(const Dims &, const DevTables &, const float *, const double *, double *, float *, const double *, double, hipStream_t, const int32_t *, int, int);
+template void launch_stage<double, double, double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, const double *, double, hipStream_t, const int32_t *, int, int, const double *);
+template void launch_stage<float, float, float, float>(const Dims &, const DevTables &, const float *, const float *, float *, float *, const double *, double, hipStream_t, const int32_t *, int, int, const double *);
+template void launch_stage<float, float, double, double>(const Dims &, const DevTables &, const float *, const double *, double *, float *, const double *, double, hipStream_t, const int32_t *, int, int, const double *);
 template void launch_rhs_only<double, double>(const Dims &, const DevTables &, const double *, double *, hipStream_t, int, int);
 template void launch_rhs_only<float, double>(const Dims &, const DevTables &, const double *, double *, hipStream_t, int, int);
 template void launch_diffusion<double, double>(const Dims &, const DevTables &, const double *, const double *, double *, float *, double *, const double *, hipStream_t, const int32_t *, int, int, const double *);

@@ -10,7 +10,7 @@
 module ModFalconShim
     use iso_c_binding
     use falcon_c
-    use ModParameters,  only: ni, nj, nk, ng, nvar
+    use ModParameters,  only: ni, nj, nk, ng, nvar, DivB_option
     use ModYinYangTree, only: YYTree, TreeNode
     use ModBlock,       only: BlockType, GC_target
     use ModAllocation,  only: ModAllocation_GetRank
@@ -118,6 +118,8 @@ contains
             call falcon_check(falcon_set_background(falcon_h, int(B%iBlock - 1, c_int), bg))
         end do
         call falcon_check(falcon_set_gc_all(falcon_h))
+        ! GLM divergence cleaning (3_6/ModAdvance.f90:82-84): psi is stepped on the device too
+        if (DivB_option == 1) call falcon_check(falcon_set_divb(falcon_h, 1_c_int))
         call falcon_host_tables(Tree)
         state_pending = .true.
     end subroutine falcon_attach
@@ -175,14 +177,18 @@ contains
         state_pending = .false.
     end subroutine falcon_push_pending
 
-    ! host -> HBM: rho1, vr, vt, vp, br, bt, bp, s1 of every block (psi stays on
-    ! the host); a driver that changes primitive_IV on the host later calls it
+    ! host -> HBM: rho1, vr, vt, vp, br, bt, bp, s1 of every block, and psi with
+    ! GLM (DivB_option=1; otherwise psi stays on the host, where no term writes
+    ! it); a driver that changes primitive_IV on the host later calls it
     subroutine falcon_push_state(Tree)
         type(YYTree), intent(inout), target :: Tree
         integer                         :: ib
+        type(BlockType), pointer        :: B
         do ib = 1, Tree%nLocalBlocks
-            call falcon_check(falcon_set_state(falcon_h, int(Tree%LocalBlocks(ib)%iBlock - 1, c_int), &
-                                               Tree%LocalBlocks(ib)%primitive_IV(:, :, :, 1:8)))
+            B => Tree%LocalBlocks(ib)
+            call falcon_check(falcon_set_state(falcon_h, int(B%iBlock - 1, c_int), B%primitive_IV(:, :, :, 1:8)))
+            if (DivB_option == 1) &
+                call falcon_check(falcon_set_psi(falcon_h, int(B%iBlock - 1, c_int), B%primitive_IV(:, :, :, B%psi_)))
         end do
     end subroutine falcon_push_state
 
@@ -195,6 +201,10 @@ contains
         do ib = 1, Tree%nLocalBlocks
             call falcon_check(falcon_get_state(falcon_h, int(Tree%LocalBlocks(ib)%iBlock - 1, c_int), prim))
             Tree%LocalBlocks(ib)%primitive_IV(:, :, :, 1:8) = prim
+            if (DivB_option == 1) then
+                call falcon_check(falcon_get_psi(falcon_h, int(Tree%LocalBlocks(ib)%iBlock - 1, c_int), prim(:, :, :, 1)))
+                Tree%LocalBlocks(ib)%primitive_IV(:, :, :, Tree%LocalBlocks(ib)%psi_) = prim(:, :, :, 1)
+            end if
         end do
     end subroutine falcon_sync_to_host
 end module ModFalconShim

@@ -152,6 +152,26 @@ module falcon_c
             real(c_double), intent(out)     :: T(*)
         end function falcon_heat_get_T
 
+        integer(c_int) function falcon_set_divb(h, option) bind(c, name='falcon_set_divb')
+            import :: c_int, c_ptr
+            type(c_ptr), value              :: h
+            integer(c_int), value           :: option
+        end function falcon_set_divb
+
+        integer(c_int) function falcon_set_psi(h, ib, psi) bind(c, name='falcon_set_psi')
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value              :: h
+            integer(c_int), value           :: ib
+            real(c_double), intent(in)      :: psi(*)
+        end function falcon_set_psi
+
+        integer(c_int) function falcon_get_psi(h, ib, psi) bind(c, name='falcon_get_psi')
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value              :: h
+            integer(c_int), value           :: ib
+            real(c_double), intent(out)     :: psi(*)
+        end function falcon_get_psi
+
         integer(c_int) function falcon_pfss_setup(h) bind(c, name='falcon_pfss_setup')
             import :: c_int, c_ptr
             type(c_ptr), value              :: h

@@ -84,6 +84,19 @@ int falcon_get_background(falcon_handle_t h, int ib, double *bg);
 int falcon_background_polytrope(falcon_handle_t h, double contrast, double rho_top);
 int falcon_set_state(falcon_handle_t h, int ib, const double *prim);
 int falcon_get_state(falcon_handle_t h, int ib, double *prim);
+
+/* GLM divergence cleaning: DivB_option of 3_6/ModParameters.f90:70, 1 = GLM
+ * (ModDivB_GLM, 3_6/ModDivB.f90:13-38, called from 3_6/ModAdvance.f90:82-84).
+ * With option 1 psi, the ninth variable of the reference's MHD blocks
+ * (3_6/ModBlock.f90:163), is stepped, exchanged and diffused with the state;
+ * FP64 path only (falcon_set_precision(h, 1) is refused, and the reverse).
+ * psi is set and read per block as one padded array in Fortran order
+ * (-ng+1:ni+ng, -ng+1:nj+ng, -ng+1:nk+ng), i.e. primitive_IV(:,:,:,psi_);
+ * falcon_set_psi writes the stage buffers too, as the reference's
+ * primitive_rk_IV starts as a copy of primitive_IV. */
+int falcon_set_divb(falcon_handle_t h, int option);
+int falcon_set_psi(falcon_handle_t h, int ib, const double *psi);
+int falcon_get_psi(falcon_handle_t h, int ib, double *psi);
 /* Seeded synthetic initial state on the interior: rho1 = v = 0, s1 = s_amp*c_p*U(-1,1),
  * B = curl of a seeded smooth vector potential scaled to max|B|/sqrt(rho0) = b_ratio*c_s.
  * Ghosts are then filled by two exchanges (3_6/test1.f90:143-144). */

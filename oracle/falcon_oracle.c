@@ -52,8 +52,9 @@ typedef struct {
     int nsc; osource *sc;
     /* background 1-D (padded in r) */
     double *gamma1, *gm1, *rho0, *g_over_rho0, *p0_over_rho0, *rho0T0, *heat, *xirsst;
-    double *u, *urk;        /* padded x 8 */
-    double *r;              /* interior x 8 */
+    double *u, *urk;        /* padded x 9 (the ninth, psi, used with GLM only) */
+    double *r;              /* interior x 9 */
+    double *vw;             /* interior v_wave of the step (GLM only) */
 } oblock;
 
 struct orc_ctx {
@@ -68,6 +69,8 @@ struct orc_ctx {
      * for the diffusion, in place of ModEquation_Dynamo / ModDiffusion */
     int lin_on;
     double lin_lambda, lin_mu;
+    /* variables stepped: 8, or 9 with GLM divergence cleaning (psi last) */
+    int nv;
 };
 
 #define PAD(c, i, j, k) ((size_t)((i) + (c)->ng - 1) + (size_t)(c)->NI * ((size_t)((j) + (c)->ng - 1) + (size_t)(c)->NJ * (size_t)((k) + (c)->ng - 1)))
@@ -87,6 +90,7 @@ static void node_init(onode *n, double rng[3][2], int yin) {
 orc_ctx *orc_create(int ni, int nj, int nk, int ng, double r0_rsun, double r1_rsun) {
     orc_ctx *c = calloc(1, sizeof(orc_ctx));
     c->ni = ni; c->nj = nj; c->nk = nk; c->ng = ng;
+    c->nv = 8;
     c->NI = ni + 2 * ng; c->NJ = nj + 2 * ng; c->NK = nk + 2 * ng;
     /* r_range = r_range_Rsun*R_sun__CGS (3_6/ModReadParameters.f90:71) */
     c->r_range[0] = r0_rsun * (double)RSUN_F;
@@ -383,9 +387,10 @@ int orc_finalize(orc_ctx *c) {
     for (int ib = 0; ib < c->nleaf; ib++) {
         oblock *b = &c->b[ib];
         init_grid(c, b);
-        b->u = calloc(np * ORC_NVAR, sizeof(double));
-        b->urk = calloc(np * ORC_NVAR, sizeof(double));
-        b->r = calloc(nint * ORC_NVAR, sizeof(double));
+        b->u = calloc(np * (ORC_NVAR + 1), sizeof(double));
+        b->urk = calloc(np * (ORC_NVAR + 1), sizeof(double));
+        b->r = calloc(nint * (ORC_NVAR + 1), sizeof(double));
+        b->vw = calloc(nint, sizeof(double));
     }
     #pragma omp parallel for num_threads(g_threads) schedule(dynamic)
     for (int ib = 0; ib < c->nleaf; ib++) find_gc(c, &c->b[ib]);
@@ -403,7 +408,7 @@ void orc_destroy(orc_ctx *c) {
         free(b->tg); free(b->sc);
         free(b->gamma1); free(b->gm1); free(b->rho0); free(b->g_over_rho0); free(b->p0_over_rho0);
         free(b->rho0T0); free(b->heat); free(b->xirsst);
-        free(b->u); free(b->urk); free(b->r);
+        free(b->u); free(b->urk); free(b->r); free(b->vw);
     }
     free(c->b);
     node_free(&c->yin); node_free(&c->yang);
@@ -488,7 +493,7 @@ void orc_get_state(const orc_ctx *c, int ib, double *u) {
 /* ------------------------------------------------------------------ exchange */
 
 /* ModMath_1D3D_interpolate_1D1D (3_6/ModMath.f90:250-293) for one point, all vars */
-static void interp_point(const orc_ctx *c, const oblock *s, const double *f, const double x[3], double out[ORC_NVAR]) {
+static void interp_point(const orc_ctx *c, const oblock *s, const double *f, const double x[3], double out[ORC_NVAR + 1]) {
     int ng = c->ng;
     double dxi = s->xi[ng + 1] - s->xi[ng], dxj = s->xj[ng + 1] - s->xj[ng], dxk = s->xk[ng + 1] - s->xk[ng];
     double base = (double)((float)(-ng) + 1.0f);
@@ -502,7 +507,7 @@ static void interp_point(const orc_ctx *c, const oblock *s, const double *f, con
     int i0 = (int)floor(pi_), j0 = (int)floor(pj), k0 = (int)floor(pk);
     double wi = pi_ - (double)i0, wj = pj - (double)j0, wk = pk - (double)k0;
     size_t nv = (size_t)c->NI * c->NJ * c->NK;
-    for (int v = 0; v < ORC_NVAR; v++) {
+    for (int v = 0; v < c->nv; v++) {
         const double *F = f + nv * v;
         double t = F[PAD(c, i0, j0, k0)] * (1.0 - wi) * (1.0 - wj) * (1.0 - wk);
         t = t + F[PAD(c, i0 + 1, j0, k0)] * (wi) * (1.0 - wj) * (1.0 - wk);
@@ -544,7 +549,7 @@ void orc_exchange(orc_ctx *c, int which) {
             size_t nv = (size_t)c->NI * c->NJ * c->NK;
             for (int g = 0; g < sc->ngc; g++) {
                 double x[3] = {sc->xijk[g], sc->xijk[g + sc->ngc], sc->xijk[g + 2 * sc->ngc]};
-                double val[ORC_NVAR];
+                double val[ORC_NVAR + 1];
                 interp_point(c, s, f, x, val);
                 if (sc->recv_yin != s->yin) {
                     double bv[3] = {val[4], val[5], val[6]}, vv[3] = {val[1], val[2], val[3]};
@@ -554,7 +559,7 @@ void orc_exchange(orc_ctx *c, int which) {
                     val[1] = vv[0]; val[2] = vv[1]; val[3] = vv[2];
                 }
                 size_t q = PAD(c, sc->ijk[g], sc->ijk[g + sc->ngc], sc->ijk[g + 2 * sc->ngc]);
-                for (int v = 0; v < ORC_NVAR; v++) ft[q + nv * v] = val[v];
+                for (int v = 0; v < c->nv; v++) ft[q + nv * v] = val[v];
             }
         }
     }
@@ -562,7 +567,7 @@ void orc_exchange(orc_ctx *c, int which) {
 
 /* ------------------------------------------------------------------ physics */
 
-enum { RHO1 = 0, VR = 1, VT = 2, VP = 3, BR = 4, BT = 5, BP = 6, S1 = 7 };
+enum { RHO1 = 0, VR = 1, VT = 2, VP = 3, BR = 4, BT = 5, BP = 6, S1 = 7, PSI = 8 };
 
 /* ModBoundary_Dynamo_primitives (3_6/ModBoundary.f90:10-55) */
 static void boundary(const orc_ctx *c, const oblock *b, double *p) {
@@ -824,9 +829,10 @@ static double minmod(double ul, double u, double ur) {
     return (half + 0.5) * copysign(mn, a);
 }
 
-/* ModDiffusion_Aritificial_1 (3_6/ModDiffusion.f90:9-150), h integer, all 8 vars;
- * MAX(0.0,NaN) pinned to 0.0. R accumulates (caller zeroes it). */
-static void diffusion(const orc_ctx *c, const oblock *b, const double *p, int h, double *R) {
+/* ModDiffusion_Aritificial_1 (3_6/ModDiffusion.f90:9-150), h integer, the
+ * first nvar variables; MAX(0.0,NaN) pinned to 0.0. R accumulates (caller
+ * zeroes it). */
+static void diffusion(const orc_ctx *c, const oblock *b, const double *p, int h, double *R, int nvar) {
     int ni = c->ni, nj = c->nj, nk = c->nk, ng = c->ng;
     size_t np = (size_t)c->NI * c->NJ * c->NK, nn = (size_t)ni * nj * nk;
     double *cs = malloc(sizeof(double) * c->NI);
@@ -837,7 +843,7 @@ static void diffusion(const orc_ctx *c, const oblock *b, const double *p, int h,
         const double *vel = p + np * dir;     /* primitive(:,:,:,direction1+1) */
         int nf = n[dir - 1] + 1;
         (void)nf;
-        for (int v = 0; v < 8; v++) {
+        for (int v = 0; v < nvar; v++) {
             const double *u = p + np * v;
             for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) for (int i = 1; i <= ni; i++) {
                 double fl[2];
@@ -882,7 +888,7 @@ void orc_rhs(orc_ctx *c, int ib, double *r_out) {
 void orc_diffusion(orc_ctx *c, int ib, double *r_out) {
     oblock *b = &c->b[ib];
     memset(r_out, 0, sizeof(double) * (size_t)c->ni * c->nj * c->nk * 8);
-    diffusion(c, b, b->u, 2, r_out);
+    diffusion(c, b, b->u, 2, r_out, 8);
 }
 
 /* ModWaveSpeed_Dynamo + ModTimeStep_Dynamo (3_6/ModWaveSpeed.f90:8-27,
@@ -902,6 +908,7 @@ double orc_dt(orc_ctx *c, double cfl) {
             double br = b->u[q + np * BR], bt = b->u[q + np * BT], bp = b->u[q + np * BP];
             double w = (1.0 / b->xirsst[ii]) * sqrt(b->gamma1[ii] * b->p0_over_rho0[ii]) + sqrt(vr * vr + vt * vt + vp * vp);
             w = w + sqrt(br * br + bt * bt + bp * bp) / sqrt(b->rho0[ii]);
+            b->vw[INT(c, i, j, k)] = w;
             double x = b->h[INT(c, i, j, k)] / w;
             if (first || x < mn) { mn = x; first = 0; }
         }
@@ -909,6 +916,59 @@ double orc_dt(orc_ctx *c, double cfl) {
         dt = dt < dt1 ? dt : dt1;
     }
     return dt;
+}
+
+/* ModDivB_GLM (3_6/ModDivB.f90:13-38) on the padded stage input p, after
+ * ModEquation_Dynamo: R(B) = R(B) - grad(psi) and R(psi) = -c_h^2 div(B) -
+ * (c_h / h) psi, c_h = v_wave of the step (ModWaveSpeed at the step's start),
+ * h = h_LLL.  div is ModSpherical_div (3_6/ModSpherical.f90:104-159), grad
+ * ModSpherical_Grad_f (:201-239), each in the reference's operation order. */
+static void glm(const orc_ctx *c, const oblock *b, const double *p, double *R) {
+    int ni = c->ni, nj = c->nj, nk = c->nk, ng = c->ng;
+    size_t np = (size_t)c->NI * c->NJ * c->NK, nn = (size_t)ni * nj * nk;
+    const double *Br = p + np * BR, *Bt = p + np * BT, *Bp = p + np * BP, *psi = p + np * PSI;
+    double *w = malloc(sizeof(double) * np);
+    double *dv = malloc(sizeof(double) * nn), *d23 = malloc(sizeof(double) * nn), *d = malloc(sizeof(double) * nn);
+    double *g1 = malloc(sizeof(double) * nn), *g2 = malloc(sizeof(double) * nn), *g3 = malloc(sizeof(double) * nn);
+    /* div: A_scaled(1) = A_r r**2, A_scaled(2) = A_t sin(t) */
+    for (size_t q = 0; q < np; q++) {
+        int i = (int)(q % c->NI);
+        w[q] = Br[q] * (b->xi[i] * b->xi[i]);
+    }
+    deriv(c, w, b->dxi, 1, dv);
+    for (size_t q = 0; q < np; q++) {
+        int j = (int)((q / c->NI) % c->NJ);
+        w[q] = Bt[q] * sin(b->xj[j]);
+    }
+    deriv(c, w, b->dxj, 2, d23);
+    deriv(c, Bp, b->dxk, 3, d);
+    for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) for (int i = 1; i <= ni; i++) {
+        size_t q = INT(c, i, j, k);
+        double r = b->xi[i + ng - 1];
+        double r2_inverse = 1.0 / (r * r), r_inverse = 1.0 / r, sint_inverse = 1.0 / sin(b->xj[j + ng - 1]);
+        double a = dv[q] * r2_inverse;
+        double t = d23[q] + d[q];
+        t = t * sint_inverse;
+        t = t * r_inverse;
+        dv[q] = a + t;
+    }
+    /* grad psi */
+    deriv(c, psi, b->dxi, 1, g1);
+    deriv(c, psi, b->dxj, 2, g2);
+    deriv(c, psi, b->dxk, 3, g3);
+    for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) for (int i = 1; i <= ni; i++) {
+        size_t q = INT(c, i, j, k);
+        double r_inverse = 1.0 / b->xi[i + ng - 1], sint_inverse = 1.0 / sin(b->xj[j + ng - 1]);
+        g2[q] = g2[q] * r_inverse;
+        g3[q] = g3[q] * r_inverse;
+        g3[q] = g3[q] * sint_inverse;
+        R[q + nn * BR] = R[q + nn * BR] - g1[q];
+        R[q + nn * BT] = R[q + nn * BT] - g2[q];
+        R[q + nn * BP] = R[q + nn * BP] - g3[q];
+        double vw = b->vw[q];
+        R[q + nn * PSI] = -((vw * vw) * dv[q]) - (vw / b->h[q]) * psi[PAD(c, i, j, k)];
+    }
+    free(w); free(dv); free(d23); free(d); free(g1); free(g2); free(g3);
 }
 
 /* ModAdvance_rk4 (3_6/ModAdvance.f90:18-112), one rank */
@@ -928,10 +988,11 @@ double orc_step(orc_ctx *c, double cfl) {
             } else {
                 boundary(c, b, in);
                 equation(c, b, in, b->r);
+                if (c->nv > 8) glm(c, b, in, b->r);
             }
             double div = (double)(5.0f - (float)rk);
             double *out = rk < 4 ? b->urk : b->u;
-            for (int v = 0; v < 8; v++)
+            for (int v = 0; v < c->nv; v++)
                 for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) for (int i = 1; i <= ni; i++) {
                     size_t q = PAD(c, i, j, k) + np * v;
                     out[q] = b->u[q] + (dt * b->r[INT(c, i, j, k) + nn * v]) / div;
@@ -942,15 +1003,15 @@ double orc_step(orc_ctx *c, double cfl) {
     #pragma omp parallel for num_threads(g_threads) schedule(dynamic)
     for (int ib = 0; ib < c->nleaf; ib++) {
         oblock *b = &c->b[ib];
-        memset(b->r, 0, sizeof(double) * nn * 8);
+        memset(b->r, 0, sizeof(double) * nn * c->nv);
         if (c->lin_on) {
             for (int v = 0; v < 8; v++)
                 for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) for (int i = 1; i <= ni; i++)
                     b->r[INT(c, i, j, k) + nn * v] = c->lin_mu * b->u[PAD(c, i, j, k) + np * v];
         } else {
-            diffusion(c, b, b->u, 2, b->r);
+            diffusion(c, b, b->u, 2, b->r, c->nv);
         }
-        for (int v = 0; v < 8; v++)
+        for (int v = 0; v < c->nv; v++)
             for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) for (int i = 1; i <= ni; i++) {
                 size_t q = PAD(c, i, j, k) + np * v;
                 b->u[q] = b->u[q] + (0.5 * dt) * b->r[INT(c, i, j, k) + nn * v];
@@ -958,6 +1019,19 @@ double orc_step(orc_ctx *c, double cfl) {
     }
     orc_exchange(c, 0);
     return dt;
+}
+
+void orc_set_glm(orc_ctx *c, int on) { c->nv = on ? 9 : 8; }
+
+void orc_set_psi(orc_ctx *c, int ib, const double *psi) {
+    size_t np = (size_t)c->NI * c->NJ * c->NK;
+    memcpy(c->b[ib].u + np * PSI, psi, sizeof(double) * np);
+    memcpy(c->b[ib].urk + np * PSI, psi, sizeof(double) * np);
+}
+
+void orc_get_psi(const orc_ctx *c, int ib, double *psi) {
+    size_t np = (size_t)c->NI * c->NJ * c->NK;
+    memcpy(psi, c->b[ib].u + np * PSI, sizeof(double) * np);
 }
 
 void orc_set_linear_rhs(orc_ctx *c, int on, double lambda, double mu) {

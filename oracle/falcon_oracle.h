@@ -63,6 +63,12 @@ double orc_step(orc_ctx *c, double cfl);             /* one ModAdvance_rk4; retu
  * are; one step then multiplies each interior value by
  * (1 + z + z^2/2 + z^3/6 + z^4/24) (1 + mu dt/2), z = lambda dt. */
 void orc_set_linear_rhs(orc_ctx *c, int on, double lambda, double mu);
+/* GLM divergence cleaning (3_6/ModDivB.f90:13-38, DivB_option=1): on != 0
+ * makes psi a ninth variable of the state, stepped, exchanged and diffused
+ * with the others; psi is set and read as one padded array per block. */
+void orc_set_glm(orc_ctx *c, int on);
+void orc_set_psi(orc_ctx *c, int ib, const double *psi_padded);
+void orc_get_psi(const orc_ctx *c, int ib, double *psi_padded);
 void orc_set_threads(int n);
 void orc_coordconv(const double in[3], double out[3]);
 void orc_vecconv_coeffs(const double x[3], double out[2]);

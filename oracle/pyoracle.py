@@ -56,6 +56,9 @@ def lib():
         L.orc_step.argtypes = [vp, C.c_double]
         L.orc_step.restype = C.c_double
         L.orc_set_linear_rhs.argtypes = [vp, C.c_int, C.c_double, C.c_double]
+        L.orc_set_glm.argtypes = [vp, C.c_int]
+        L.orc_set_psi.argtypes = [vp, C.c_int, dp]
+        L.orc_get_psi.argtypes = [vp, C.c_int, dp]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_heat_create.restype = vp
         L.orc_heat_create.argtypes = [vp, C.c_double, C.c_double]
@@ -174,6 +177,23 @@ class Oracle:
     def get_state(self, ib):
         out = np.zeros((8,) + self.pad[::-1])
         self.L.orc_get_state(self.h, ib, _p(out, C.c_double))
+        return out
+
+    def set_glm(self, on):
+        """GLM divergence cleaning (3_6/ModDivB.f90): psi becomes a ninth
+        stepped variable."""
+        self.L.orc_set_glm(self.h, 1 if on else 0)
+
+    def set_psi(self, ib, psi):
+        """padded psi of block ib, (NK, NJ, NI); written to the state and the
+        stage buffer, ghosts included"""
+        psi = np.ascontiguousarray(psi, dtype=np.float64)
+        assert psi.shape == tuple(self.pad[::-1])
+        self.L.orc_set_psi(self.h, ib, _p(psi, C.c_double))
+
+    def get_psi(self, ib):
+        out = np.zeros(self.pad[::-1])
+        self.L.orc_get_psi(self.h, ib, _p(out, C.c_double))
         return out
 
     def exchange(self, which=0):

@@ -18,6 +18,9 @@
 ! ModCommunication / ModAdvance / ModCheck, stepping on the GPU (one rank).
 !
 ! Usage: [mpirun -n N] falcon_ref_harness <input.bin> <output.bin>
+! The input's state count is 8, or 9 for a run with GLM divergence cleaning
+! (DivB_option=1, 3_6/ModAdvance.f90:82-84): psi is the ninth variable of the
+! initial state and of the state dumps.
 ! Input modes: 1 tables only; 2 tables, dumps and steps; 3 steps only (timing:
 ! no output beyond the file header and the stepping loop's wall time); 4 the
 ! reference's ModSave_AllCells output of the exchanged initial state, written
@@ -81,7 +84,7 @@ program falcon_ref_harness
     read(ui) mode
 
     ! Global parameters as the reference's #GRID/#EQUATION setup leaves them.
-    iGeometry=1; iEquation=1; if_involve_B=.true.; DivB_option=0
+    iGeometry=1; iEquation=1; if_involve_B=.true.; DivB_option=merge(1,0,nstate==9)
     Artificial_heating_ratio=1.0d0
     nvar=9
     rLevelInitial=0
@@ -167,7 +170,7 @@ program falcon_ref_harness
             if (ib>=1 .and. ib<=Tree%nLocalBlocks) then
                 B=>Tree%LocalBlocks(ib)
                 B%primitive_IV=0.0d0
-                do v=1,min(nstate,8)
+                do v=1,min(nstate,9)
                     B%primitive_IV(:,:,:,v)=st(:,:,:,v)
                 end do
                 B%primitive_rk_IV=B%primitive_IV
@@ -295,7 +298,7 @@ contains
                             write(uo) B%GC_sources(it)%xijk_list
                         end do
                     case(5)
-                        write(uo) B%primitive_IV(:,:,:,1:8)
+                        write(uo) B%primitive_IV(:,:,:,1:min(nstate,9))
                     case(6)
                         write(uo) B%EQN_update_R_IV(:,:,:,1:8)
                     end select

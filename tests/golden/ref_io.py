@@ -40,15 +40,18 @@ def synthetic_background(xi, seed_shift=0.0):
 
 
 def synthetic_state(rng, pad, amp):
-    """Random padded state (8, NK, NJ, NI), every variable nonzero; uniform
-    draws scaled per variable (uniform doubles are stable across numpy versions)."""
-    s = 2.0 * rng.random((8,) + tuple(pad[::-1])) - 1.0
+    """Random padded state (len(amp), NK, NJ, NI), every variable nonzero;
+    uniform draws scaled per variable (uniform doubles are stable across numpy
+    versions)."""
+    s = 2.0 * rng.random((len(amp),) + tuple(pad[::-1])) - 1.0
     return s * np.asarray(amp, dtype=np.float64)[:, None, None, None]
 
 
-def write_input(path, layout, cfl, nsteps, dumps, mode, blocks_bg=None, blocks_state=None):
+def write_input(path, layout, cfl, nsteps, dumps, mode, blocks_bg=None, blocks_state=None, nvar=8):
+    """nvar 9: psi is the ninth variable and the harness runs with GLM
+    divergence cleaning (DivB_option=1)."""
     with open(path, "wb") as f:
-        f.write(struct.pack("<5i", layout["ni"], layout["nj"], layout["nk"], layout["ng"], 8))
+        f.write(struct.pack("<5i", layout["ni"], layout["nj"], layout["nk"], layout["ng"], nvar))
         f.write(struct.pack("<2d", layout["r_rsun"][0] * R_SUN, layout["r_rsun"][1] * R_SUN))
         amrs = [op for op in layout.get("ops", []) if op[0] == "amr"]
         if len(amrs) != len(layout.get("ops", [])):
@@ -65,7 +68,7 @@ def write_input(path, layout, cfl, nsteps, dumps, mode, blocks_bg=None, blocks_s
         f.write(struct.pack("<i", mode))
         if mode >= 2:
             for bg, st in zip(blocks_bg, blocks_state):
-                # harness reads bg(-ng+1:ni+ng, 8) then st(NI,NJ,NK,8): C order (8,NI) / (8,NK,NJ,NI)
+                # harness reads bg(-ng+1:ni+ng, 8) then st(NI,NJ,NK,nvar): C order (8,NI) / (nvar,NK,NJ,NI)
                 f.write(np.ascontiguousarray(bg, dtype="<f8").tobytes())
                 f.write(np.ascontiguousarray(st, dtype="<f8").tobytes())
 
@@ -86,7 +89,7 @@ class _Reader:
         return v
 
 
-def read_output(path, layout, mode, ndumps):
+def read_output(path, layout, mode, ndumps, nvar=8):
     ni, nj, nk, ng = layout["ni"], layout["nj"], layout["nk"], layout["ng"]
     NI, NJ, NK = ni + 2 * ng, nj + 2 * ng, nk + 2 * ng
     r = _Reader(open(path, "rb").read())
@@ -121,7 +124,7 @@ def read_output(path, layout, mode, ndumps):
         dumps = []
 
         def read_states():
-            return [r.f(8 * NI * NJ * NK).reshape(8, NK, NJ, NI).copy() for _ in range(nb)]
+            return [r.f(nvar * NI * NJ * NK).reshape(nvar, NK, NJ, NI).copy() for _ in range(nb)]
 
         step = int(r.i()[0]); dt = float(r.f()[0])
         dumps.append((step, dt, read_states()))
@@ -150,6 +153,7 @@ def run_harness(layout, cfl=0.5, nsteps=0, dumps=(), mode=1, blocks_bg=None, blo
     raise_stack_limit()
     with tempfile.TemporaryDirectory() as td:
         fi, fo = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
-        write_input(fi, layout, cfl, nsteps, list(dumps), mode, blocks_bg, blocks_state)
+        nvar = len(blocks_state[0]) if blocks_state else 8
+        write_input(fi, layout, cfl, nsteps, list(dumps), mode, blocks_bg, blocks_state, nvar)
         subprocess.run(list(launcher) + [exe or HARNESS, fi, fo], check=True, stdout=subprocess.DEVNULL)
-        return read_output(fo, layout, mode, len(dumps))
+        return read_output(fo, layout, mode, len(dumps), nvar)

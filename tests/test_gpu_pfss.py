@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import pyoracle
-from falcon_amd import Pfss, Shell, configs
+from falcon_amd import HeatSolver, Pfss, Shell, configs
 
 pytestmark = pytest.mark.gpu
 
@@ -149,3 +149,15 @@ def test_monopole_converges_at_second_order_on_the_gpu():
     order = np.log2(errs[0] / errs[1])
     print(errs, order, flush=True)
     assert order[0] > 1.9 and order[1] > 1.8
+
+
+def test_heat_only_calls_refuse_the_pfss_solver():
+    """The handle holds one implicit solver: after falcon_pfss_setup the heat
+    solve must fail loudly rather than solve the PFSS system; a heat set-up
+    afterwards replaces it."""
+    s = Shell(LAYOUTS["single8"])
+    Pfss(s)
+    assert s.L.falcon_heat_solve(s.h, 1e-10, 10, 0, None, None, None) != 0
+    hs = HeatSolver(s, 1.0, 0.7, 1)
+    hs.set_dt(np.array([100.0 * hs.dt_explicit]))
+    assert s.L.falcon_pfss_solve(s.h, 1e-10, 10, None, None, None) != 0
