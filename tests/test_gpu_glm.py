@@ -95,9 +95,9 @@ def test_glm_steps_bit_exact_vs_oracle(name, shape, graph):
 
 
 def test_glm_changes_the_field_and_is_off_by_default():
-    """With psi = 0 and a field of nonzero divergence, one step with GLM
-    differs from one without in B and psi only through the cleaning terms;
-    with GLM off psi is refused and the state equals the 8-variable oracle."""
+    """With psi = 0 and a field of nonzero divergence, psi grows in one step
+    with GLM and B departs from the step without it (grad psi enters from the
+    second stage on); with GLM off psi is refused."""
     lay = configs.small_mixed()
     o = pyoracle.Oracle(lay)
     bgs, sts = random_inputs(o.nblocks, lambda ib: o.centres(ib)[0], o.pad, 6)
