@@ -35,8 +35,8 @@ constexpr int tile_pl() { return TL + 4; }   // LDS plane edge (tile + 2 halo ea
 // has its own L2.  Giving XCD x the contiguous tile range [x*n/8, ...) in
 // dispatch order puts phi-adjacent blocks (consecutive ids) on the same XCD at
 // the same time, so the halo each reads from the other is an L2 hit: on the
-// 50 M-cell shell the FP32 stage reads 84 instead of 143 bytes per cell
-// (stages 1-3) and the FP32 diffusion 158 instead of 220.
+// 50 M-cell shell the FP32 stage reads 68 instead of 134 bytes per cell from
+// HBM (stages 1-3) and the FP32 diffusion 158 instead of 220.
 __device__ __forceinline__ int xcd_tile(int w, int n) {
     const int q = n >> 3, r = n & 7, x = w & 7, k = w >> 3;
     return x * q + min(x, r) + k;
