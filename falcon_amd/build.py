@@ -16,7 +16,7 @@ OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libfalcon_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["layout.cpp", "synthetic.cpp", "output.cpp", "kernels.hip", "falcon.hip", "heat.hip", "cnn.hip"]
+SOURCES = ["layout.cpp", "synthetic.cpp", "output.cpp", "kernels.hip", "tables.hip", "falcon.hip", "heat.hip", "cnn.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

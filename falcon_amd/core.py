@@ -54,6 +54,7 @@ def load():
         "falcon_set_state": [vp, C.c_int, dp],
         "falcon_get_state": [vp, C.c_int, dp],
         "falcon_set_divb": [vp, C.c_int],
+        "falcon_set_device_tables": [vp, C.c_int],
         "falcon_set_psi": [vp, C.c_int, dp],
         "falcon_get_psi": [vp, C.c_int, dp],
         "falcon_init_synthetic": [vp, C.c_uint64, C.c_double, C.c_double],
@@ -190,6 +191,11 @@ class Shell:
     def set_gc_all(self):
         _check(self.L.falcon_set_gc_all(self.h))
         self.on_device = True
+
+    def set_device_tables(self, on=True):
+        """build the ghost tables of set_gc_all on the GPU (donor search,
+        targets, interpolation data) instead of the host"""
+        _check(self.L.falcon_set_device_tables(self.h, 1 if on else 0))
 
     def gc_iblocks(self, ib):
         out = np.zeros(self.pad[::-1], dtype=np.int32)

@@ -5,6 +5,12 @@
 
 namespace falcon {
 
+struct Layout;
+// Layout::build's ghost tables (donors, targets, entries, and GC_iBlocks when
+// the layout keeps them) built on the device (tables.hip); equal to the host
+// build bit for bit
+void build_ghosts_device(Layout &lay, int nthreads, hipStream_t s);
+
 // Per-block constants, FP64 (3_6/ModBlock.f90:315-317 and the derivative
 // coefficient (1./12./dx) of 3_6/ModDerivative.f90:16,22,28).
 struct BlockMeta {

@@ -120,6 +120,7 @@ struct falcon_handle {
     DevBuf meta, radial, angular, ghost64[4], ghost32[4], ghost_copy32, ghost_lerp32[2], bnd_list;   // ghost tables
     DevBuf ghost_rest32, ghost_pull32;   // copies not read in place, pull table (FP32 path)
     bool ghost_pull = true;              // FP32 step: kernels read same-level ghost copies in place
+    bool device_tables = false;          // falcon_set_gc_all builds the ghost tables on the device
 
     DevBuf U[2], F32[2], S64[2], S32[2], dtp, partial, rbuf;
     // GLM divergence cleaning (falcon_set_divb, FP64 path): psi beside U[] and
@@ -830,14 +831,32 @@ int falcon_build_tables(falcon_handle_t h) {
     });
 }
 
-int falcon_set_gc_all(falcon_handle_t h) {
-    if (falcon_build_tables(h) != 0) return -1;
+int falcon_set_device_tables(falcon_handle_t h, int on) {
     return guard([&] {
+        if (h->built) throw Error("the ghost tables are already built");
+        h->device_tables = on != 0;
+    });
+}
+
+int falcon_set_gc_all(falcon_handle_t h) {
+    int rc = guard([&] {
         if (!h->stream) {
             HIPCHECK(hipGetDevice(&h->device));
             HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
             scratch_setup(h->device);
         }
+        if (h->device_tables && !h->built) {
+            h->need_label();
+            Layout &L = h->lay;
+            L.keep_gcib = L.tree.nleaf <= 4096;
+            build_ghosts_device(L, host_threads(), h->stream);
+            const Tree &t = L.tree;
+            h->dims = make_dims(t.ni, t.nj, t.nk, t.ng, t.nleaf);
+            h->built = true;
+        }
+    });
+    if (rc != 0 || falcon_build_tables(h) != 0) return -1;
+    return guard([&] {
         alloc_state(h);
         h->tables = true;
     });

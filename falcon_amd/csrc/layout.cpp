@@ -248,13 +248,49 @@ static void init_geom(const Tree &t, const Node &n, BlockGeom &g) {
     g.bottom = g.xi[0] < t.r_range[0];
 }
 
-void Layout::build(int nthreads) {
+void Layout::build_geom() {
     if (tree.nleaf == 0) tree.label();
+    const int nb = tree.nleaf;
+    geom.assign(nb, BlockGeom());
+    for (int b = 0; b < nb; b++) init_geom(tree, tree.nodes[tree.leaf_node[b]], geom[b]);
+}
+
+void conv_rows(const Layout &lay, int nthreads, std::vector<double> &rows) {
+    const int nb = lay.tree.nleaf, ng = lay.tree.ng;
+    const int NJ = lay.tree.nj + 2 * ng, NK = lay.tree.nk + 2 * ng;
+    rows.assign((size_t)nb * NJ * NK * 4, 0.0);
+    auto work = [&](int b) {
+        const BlockGeom &g = lay.geom[b];
+        for (int J = 0; J < NJ; J++)
+            for (int K = 0; K < NK; K++) {
+                // as Tree::find and the Yin-Yang entries of build() form them:
+                // the point in the other frame, then the rotation of that
+                // (donor-frame) point, whose CoordConv leads back
+                const double x[3] = {g.xi[0], g.xj[J], g.xk[K]};
+                double o[3], o2[3];
+                coordconv(x, o);
+                coordconv(o, o2);
+                double sp, cp;
+                ::sincos(o[2], &sp, &cp);
+                double *r = &rows[(((size_t)b * NJ + J) * NK + K) * 4];
+                r[0] = o[1];
+                r[1] = o[2];
+                r[2] = sp * std::sin(o2[2]);
+                r[3] = cp / std::sin(o2[1]);
+            }
+    };
+    std::vector<std::thread> th;
+    const int nt = std::max(1, std::min(nthreads, nb));
+    for (int t = 0; t < nt; t++)
+        th.emplace_back([&, t]() { for (int b = t; b < nb; b += nt) work(b); });
+    for (auto &x : th) x.join();
+}
+
+void Layout::build(int nthreads) {
+    build_geom();
     const int nb = tree.nleaf;
     const int ni = tree.ni, nj = tree.nj, nk = tree.nk, ng = tree.ng;
     const int NI = ni + 2 * ng, NJ = nj + 2 * ng, NK = nk + 2 * ng;
-    geom.assign(nb, BlockGeom());
-    for (int b = 0; b < nb; b++) init_geom(tree, tree.nodes[tree.leaf_node[b]], geom[b]);
     targets.assign(nb, {});
     if (keep_gcib) gcib.assign(nb, {});
     std::vector<std::vector<GhostEntry>> per(nb);

@@ -73,8 +73,20 @@ struct Layout {
     bool keep_gcib = false;
 
     int nblocks() const { return tree.nleaf; }
+    // per-block geometry of every leaf (labels the tree if needed)
+    void build_geom();
+    // geometry and ghost tables on the host
     void build(int nthreads);
 };
+
+// Transcendental rows of the device table builder, per block [J][K] over the
+// padded (theta, phi) centres, 4 doubles: theta' and phi' of CoordConv
+// (3_6/ModYinYang.f90:29-46, the point in the other panel's frame) and the
+// VecConv coefficients (:48-61) of that donor-frame point,
+// sin(phi') sin(phi''), cos(phi') / sin(theta'') with (theta'', phi'') its
+// CoordConv.  The device evaluates the search and the weights from them with
+// +, -, *, / and floor only, so its tables equal the host's bit for bit.
+void conv_rows(const Layout &lay, int nthreads, std::vector<double> &rows);
 
 // ---------------------------------------------------------------- heat levels
 // One multigrid level of the heat operator: every block holds m^3 cells.  The

@@ -54,6 +54,13 @@ int falcon_block_info(falcon_handle_t h, int ib, double *rng6, int *flags3, doub
  * GC targets, interpolation bases/weights and Yin-Yang rotations, built on the
  * host and uploaded once.  Also allocates every block's state in HBM. */
 int falcon_set_gc_all(falcon_handle_t h);
+/* Build the ghost tables of falcon_set_gc_all on the device (on != 0) or on
+ * the host (0, the default); before the tables are built.  The device build
+ * runs the donor search of every ghost cell (YinYangTree_FindNode,
+ * 3_6/ModYinYangTree.f90:449-505), the target grouping (ModGC_GetGC_Targets_
+ * single, 3_6/ModGC.f90:53-147) and the interpolation data on the GPU; the
+ * tables equal the host build's bit for bit, after any refinement. */
+int falcon_set_device_tables(falcon_handle_t h, int on);
 /* Host half of falcon_set_gc_all (no device calls): tables only. */
 int falcon_build_tables(falcon_handle_t h);
 /* GC_iBlocks_III of block ib (padded, Fortran order; -777 = outside the domain,
