@@ -1221,16 +1221,19 @@ static int resident(K kernel, int threads) {
 }
 
 // workgroups of one march resident on the device at once (the persistent grid):
-// kernel 0 the stage (intermediate-stage form), 1 the diffusion; precision 0
-// FP64, 1 FP32
+// kernel 0 the stage, 1 the diffusion; precision 0 FP64, 1 FP32.  The FP32
+// path launches the stage in two builds (stages 1-3 and the final stage with
+// its FP64 output and FP32 copy) on one grid size: the smaller of their counts.
 int march_resident_wgs(int kernel, int precision, int tile) {
     check_tile(tile);
     if (kernel == 0 && precision == 0)
         return tile == 16 ? resident(k_stage<double, double, double, double, 0, false, 16, true>, 256)
                           : resident(k_stage<double, double, double, double, 0, false, 8, true>, 64);
     if (kernel == 0)
-        return tile == 16 ? resident(k_stage<float, float, float, float, 0, false, 16, true>, 256)
-                          : resident(k_stage<float, float, float, float, 0, false, 8, true>, 64);
+        return tile == 16 ? std::min(resident(k_stage<float, float, float, float, 0, false, 16, true>, 256),
+                                     resident(k_stage<float, float, double, double, 0, true, 16, true>, 256))
+                          : std::min(resident(k_stage<float, float, float, float, 0, false, 8, true>, 64),
+                                     resident(k_stage<float, float, double, double, 0, true, 8, true>, 64));
     if (precision == 0)
         return tile == 16 ? resident(k_diffusion<double, double, 0, 16, true>, 256)
                           : resident(k_diffusion<double, double, 0, 8, true>, 64);
