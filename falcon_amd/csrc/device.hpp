@@ -165,6 +165,8 @@ void launch_set_scalar(double *p, double v, hipStream_t s);
 // w starts as {0, ~0}
 template <typename TS>
 void launch_nan_scan(const Dims &d, const TS *st, unsigned long long *w, hipStream_t s);
+// the same scan of psi ([block][P]) as variable 8 (GLM divergence cleaning)
+void launch_nan_scan_psi(const Dims &d, const double *psi, unsigned long long *w, hipStream_t s);
 // interior difference of two states (see k_state_diff): pass 0 then pass 1 on
 // the same 32 words w, whose first 8 and last 8 start at 0 and at ~0
 void launch_state_diff(const Dims &d, const double *a, const double *b, double tol, unsigned long long *w, int pass,

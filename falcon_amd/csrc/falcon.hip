@@ -1413,9 +1413,11 @@ int falcon_check_state(falcon_handle_t h, int which, int64_t *n_nan, int32_t *wh
         HIPCHECK(hipMemcpyAsync(w, init, sizeof(init), hipMemcpyHostToDevice, h->stream));
         if (which == 0) {
             launch_nan_scan<double>(d, h->U[h->cur].as<double>(), w, h->stream);
+            if (h->divb) launch_nan_scan_psi(d, h->psi[h->cur].as<double>(), w, h->stream);
         } else if (h->precision == 0) {
             if (!h->S64[0].p) throw Error("falcon_check_state: no stage buffer yet");
             launch_nan_scan<double>(d, h->S64[0].as<double>(), w, h->stream);
+            if (h->divb) launch_nan_scan_psi(d, h->psiS[0].as<double>(), w, h->stream);
         } else {
             if (!h->S32[0].p) throw Error("falcon_check_state: no stage buffer yet");
             launch_nan_scan<float>(d, h->S32[0].as<float>(), w, h->stream);
@@ -1428,7 +1430,7 @@ int falcon_check_state(falcon_handle_t h, int which, int64_t *n_nan, int32_t *wh
             for (int q = 0; q < 5; q++) where[q] = -1;
             if (out[0]) {
                 unsigned long long k = out[1];
-                where[4] = (int32_t)(k % 8); k /= 8;
+                where[4] = (int32_t)(k % 9); k /= 9;
                 where[1] = (int32_t)(k % d.NI); k /= d.NI;
                 where[2] = (int32_t)(k % d.NJ); k /= d.NJ;
                 where[3] = (int32_t)(k % d.NK); k /= d.NK;

@@ -1175,8 +1175,32 @@ __global__ void __launch_bounds__(256) k_nan_scan(Dims d, const TS *__restrict__
         if (x != x) {
             int I, J, K;
             cell_ijk(d, q, I, J, K);
+            // variable innermost, with room for psi (variable 8) after the eight
             const unsigned long long key =
-                ((((unsigned long long)blk * d.NK + K) * d.NJ + J) * d.NI + I) * 8ull + (unsigned long long)v;
+                ((((unsigned long long)blk * d.NK + K) * d.NJ + J) * d.NI + I) * 9ull + (unsigned long long)v;
+            cnt++;
+            first = key < first ? key : first;
+        }
+    }
+    if (cnt) {
+        atomicAdd(&w[0], cnt);
+        atomicMin(&w[1], first);
+    }
+}
+
+// psi of GLM divergence cleaning ([block][P]), as the ninth variable of the
+// same scan order
+__global__ void __launch_bounds__(256) k_nan_scan_psi(Dims d, const double *__restrict__ psi,
+                                                      unsigned long long *__restrict__ w) {
+    const int blk = blockIdx.x;
+    const double *p = psi + (long long)blk * d.P;
+    unsigned long long cnt = 0, first = ~0ull;
+    for (long long q = threadIdx.x; q < d.P; q += blockDim.x) {
+        const double x = p[q];
+        if (x != x) {
+            int I, J, K;
+            cell_ijk(d, q, I, J, K);
+            const unsigned long long key = ((((unsigned long long)blk * d.NK + K) * d.NJ + J) * d.NI + I) * 9ull + 8ull;
             cnt++;
             first = key < first ? key : first;
         }
@@ -1392,6 +1416,9 @@ void launch_state_diff(const Dims &d, const double *a, const double *b, double t
 template <typename TS>
 void launch_nan_scan(const Dims &d, const TS *st, unsigned long long *w, hipStream_t s) {
     hipLaunchKernelGGL(k_nan_scan<TS>, dim3((unsigned)d.nblocks * 8), dim3(256), 0, s, d, st, w);
+}
+void launch_nan_scan_psi(const Dims &d, const double *psi, unsigned long long *w, hipStream_t s) {
+    hipLaunchKernelGGL(k_nan_scan_psi, dim3((unsigned)d.nblocks), dim3(256), 0, s, d, psi, w);
 }
 template void launch_nan_scan<double>(const Dims &, const double *, unsigned long long *, hipStream_t);
 template void launch_nan_scan<float>(const Dims &, const float *, unsigned long long *, hipStream_t);

@@ -153,9 +153,10 @@ int falcon_advance(falcon_handle_t h, int nsteps, double cfl, int use_graph, dou
 /* ModCheck_primitive (3_6/ModCheck.f90:12-71) on the device: which = 0 scans
  * the state (primitive_IV), 1 the current stage buffer (primitive_rk_IV), every
  * padded cell and variable.  n_nan = NaN count; where[5] (optional) = block
- * (0-based), I, J, K (padded, 0-based: index - (-ng+1)) and variable (0-based)
- * of the first NaN in the reference's scan order (blocks in order, then phi,
- * theta, r, variable innermost), or -1s.  The Fortran shim turns n_nan > 0
+ * (0-based), I, J, K (padded, 0-based: index - (-ng+1)) and variable (0-based;
+ * 8 is psi, scanned with GLM divergence cleaning on) of the first NaN in the
+ * reference's scan order (blocks in order, then phi, theta, r, variable
+ * innermost), or -1s.  The Fortran shim turns n_nan > 0
  * into the reference's message and `stop 1` (its MPI_Abort). */
 int falcon_check_state(falcon_handle_t h, int which, int64_t *n_nan, int32_t *where);
 /* ModSave_AllCells (3_6/ModSavePlot.f90:520-650): writes

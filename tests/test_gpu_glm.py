@@ -130,3 +130,26 @@ def test_glm_runs_on_the_fp64_path_only():
     s.set_precision(1)
     with pytest.raises(FalconError):
         s.set_divb(1)
+
+
+def test_nan_scan_includes_psi():
+    """With GLM on, falcon_check_state scans psi as the ninth variable of
+    ModCheck_primitive's order (variable innermost: psi after the eight)."""
+    lay = configs.small_mixed()
+    o = pyoracle.Oracle(lay)
+    bgs, sts = random_inputs(o.nblocks, lambda ib: o.centres(ib)[0], o.pad, 8)
+    zeros = [np.zeros(tuple(o.pad[::-1])) for _ in range(o.nblocks)]
+    s = Shell(lay)
+    _setup(s, bgs, sts, zeros, True)
+    assert s.check_state(0) == (0, None)
+    psi = np.zeros(tuple(o.pad[::-1]))
+    psi[5, 4, 1] = np.nan                     # block 3, K=5, J=4, I=1: first in the scan
+    s.set_psi(3, psi)
+    st = s.get_state(3)
+    st[7, 5, 4, 2] = np.nan                   # same K, J, next I, variable 7
+    s.set_state(3, st)
+    assert s.check_state(0) == (2, (3, 1, 4, 5, 8))
+    psi[5, 4, 1] = 0.0
+    psi[5, 4, 2] = np.nan                     # the same cell as the state's NaN: after variable 7
+    s.set_psi(3, psi)
+    assert s.check_state(0) == (2, (3, 2, 4, 5, 7))
