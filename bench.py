@@ -23,7 +23,7 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
-from falcon_amd import HeatSolver, Shell, configs  # noqa: E402
+from falcon_amd import HeatSolver, Pfss, Shell, configs  # noqa: E402
 
 WORKLOADS = {
     "shell_128x256x768_b16": configs.SHELL_128x256x768_B16,
@@ -89,8 +89,9 @@ def main():
                     help="1: train on the scale-free form of the gradient-weighted loss")
     ap.add_argument("--learned-iters", type=int, default=0,
                     help="directions built with the learned correction before the plain V-cycle takes over (0: all)")
-    ap.add_argument("--legs", default="rk,heat,learned",
-                    help="comma list of legs: rk (RK4 FP64/FP32), heat (MG-PCG), learned (learned FCG vs MG-PCG)")
+    ap.add_argument("--legs", default="rk,heat,learned,pfss,glm",
+                    help="comma list of legs: rk (RK4 FP64/FP32), heat (MG-PCG), learned (learned FCG vs MG-PCG), "
+                         "pfss (potential-field solve), glm (FP64 step with GLM divergence cleaning)")
     ap.add_argument("--error-steps", type=int, default=100,
                     help="also step the FP64 and FP32 paths this many steps from the same state and report the "
                          "FP32 path's per-variable difference (0: skip)")
@@ -145,6 +146,16 @@ def main():
         heat = heat or {}
         heat["learned_fcg"] = learned
 
+    extras = {}
+    if "pfss" in legs:
+        extras["pfss"] = run_pfss()
+        print(f"# pfss: {json.dumps(extras['pfss'])}", file=sys.stderr, flush=True)
+    if "glm" in legs:
+        extras["glm"] = run_glm(args.workload, args.steps, args.warmup, args.cfl)
+        if res:
+            extras["glm"]["fp64_ms_per_step_without"] = res["fp64"]["ms_per_step"]
+        print(f"# glm: {json.dumps(extras['glm'])}", file=sys.stderr, flush=True)
+
     cpu = None
     if "rk" in legs and not args.no_cpu_baseline:
         cpu = run_cpu_baseline()
@@ -176,6 +187,7 @@ def main():
         "setup_s": setup_s,
         "cpu_baseline": cpu,
         "heat": heat,
+        "extras": extras or None,
     }
     print(json.dumps(out))
 
@@ -373,6 +385,49 @@ def run_heat(cpu_baseline=True, repeats=3):
         out["cpu_baseline"] = {"value": el, "unit": "s", "cores": 1, "kind": "port", "iterations": ito,
                                "sample": "oracle FP64 MG-preconditioned FCG, same system, one solve to 1e-10"}
     return out
+
+
+def run_pfss(repeats=3):
+    """PFSS potential field (3_5/ModPFSS.f90) on the 64x128x384-per-panel shell
+    (1536 blocks of 16^3): magnetogram of a centred dipole plus an offset one,
+    MG-preconditioned FCG to 1e-10, then B = -grad(phi)."""
+    s = Shell(configs.SHELL_64x128x384_B16)
+    pf = Pfss(s)
+    r0 = 0.7 * float(np.float32(6.955e10))
+    xyz = np.array([[0.0, 0.0, 0.0], [0.3 * r0, -0.2 * r0, 0.5 * r0]])
+    q = np.array([1.0, -0.4])
+    l = np.array([[0.0, 0.0, 0.05 * r0], [0.02 * r0, 0.03 * r0, 0.0]])
+    pf.set_magnetogram(pf.dipole_magnetogram(xyz, q, l))
+    pf.solve(1e-10, 500)                    # warm-up (first launches)
+    runs = [pf.solve(1e-10, 500) for _ in range(repeats)]
+    secs = sorted(r[2] for r in runs)
+    t0 = time.perf_counter()
+    pf.field()
+    field_s = time.perf_counter() - t0
+    s.close()
+    return {"workload": "pfss_shell_64x128x384_b16", "cells": int(np.prod(pf.shape)), "tol": 1e-10,
+            "seconds": secs[len(secs) // 2], "iterations": runs[0][0], "relres": runs[0][1],
+            "field_seconds_with_transfer": field_s}
+
+
+def run_glm(workload, steps, warmup, cfl):
+    """FP64 step with GLM divergence cleaning (DivB_option=1) on the RK
+    workload: the same seeded state, psi = 0 at the start."""
+    s = Shell(WORKLOADS[workload])
+    s.background_polytrope(20.0)
+    s.set_gc_all()
+    s.set_precision("fp64")
+    s.set_divb(1)
+    s.init_synthetic(seed=12345, s_amp=1e-4, b_ratio=1e-3)
+    s.advance(warmup, cfl, graph=True)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.advance(steps, cfl, graph=True)
+    s.synchronize()
+    el = time.perf_counter() - t0
+    cells = s.interior_cells
+    s.close()
+    return {"workload": workload, "cell_updates_per_s": cells * steps / el, "ms_per_step": 1e3 * el / steps}
 
 
 LEARNED_WORKLOAD = "heat_amr_32x64x192_b16_x32"
