@@ -330,7 +330,10 @@ __device__ __forceinline__ void cell_rhs(const TW (&win)[8][5], LDSF L, const T 
 // MODE 1: write R (interior layout [block][var][i][j][k], FP64) for parity tests.
 // pull (null: none): ghost cells of `in` in the directions it lists are read
 // from the same-level neighbour's interior (Marcher).
-template <typename T, typename TIN, typename TOUT, typename TB, int MODE, bool SHADOW, int TL, bool PERSIST>
+// GLM (FP64 only): grad(psi) from `glm` is subtracted from R(B) (GLM
+// divergence cleaning); a separate instantiation, so the plain one is unchanged
+template <typename T, typename TIN, typename TOUT, typename TB, int MODE, bool SHADOW, int TL, bool PERSIST,
+          bool GLM = false>
 __global__ void __launch_bounds__(TL * TL) k_stage(Dims d, DevTables t, const TIN *__restrict__ in,
                                                const TB *__restrict__ U, TOUT *__restrict__ out,
                                                float *__restrict__ shadow, const double *__restrict__ dtp,
@@ -449,16 +452,14 @@ __global__ void __launch_bounds__(TL * TL) k_stage(Dims d, DevTables t, const TI
                 cell_rhs<double, T, RECIP>(win, L, xr, rho0r, gpr, gqr, st, coef(4, I), coef(5, I), coef(6, I),
                                            coef(7, I), coef(8, I), sinv, cot, cr, ct, cp, RECIP ? coef(9, I) : 1.0, R);
                 if (MODE == 0) {
-                    if constexpr (sizeof(T) == 8) {
-                        if (glm) {
-                            // ModDivB_GLM (3_6/ModDivB.f90:27-28): R(B) = R(B) - grad(psi)
-                            const long long nn = (long long)d.ni * d.nj * d.nk;
-                            const long long gb =
-                                (long long)b * 3 * nn + ((long long)i * d.nj + (j0 + ty)) * d.nk + (k0 + tx);
-                            R[4] = R[4] - glm[gb];
-                            R[5] = R[5] - glm[gb + nn];
-                            R[6] = R[6] - glm[gb + 2 * nn];
-                        }
+                    if constexpr (GLM && sizeof(T) == 8) {
+                        // ModDivB_GLM (3_6/ModDivB.f90:27-28): R(B) = R(B) - grad(psi)
+                        const long long nn = (long long)d.ni * d.nj * d.nk;
+                        const long long gb =
+                            (long long)b * 3 * nn + ((long long)i * d.nj + (j0 + ty)) * d.nk + (k0 + tx);
+                        R[4] = R[4] - glm[gb];
+                        R[5] = R[5] - glm[gb + nn];
+                        R[6] = R[6] - glm[gb + 2 * nn];
                     }
                     const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + m.coff;
     #pragma unroll
@@ -1269,6 +1270,18 @@ template <int TL, typename T, typename TIN, typename TOUT, typename TB>
 static void stage_tl(const Dims &d, const DevTables &t, const TIN *in, const TB *U, TOUT *out, float *shadow,
                      const double *dtp, double div, hipStream_t s, const int32_t *pull, int wgs, const double *glm) {
     const dim3 g = march_grid(d, TL, wgs), b(TL * TL);
+    if constexpr (sizeof(T) == 8) {
+        if (glm) {
+            if (shadow) throw std::runtime_error("GLM stage: no FP32 copy");
+            if (wgs)
+                hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, false, TL, true, true>), g, b, 0, s, d, t, in, U, out,
+                                   (float *)nullptr, dtp, div, pull, glm);
+            else
+                hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, false, TL, false, true>), g, b, 0, s, d, t, in, U, out,
+                                   (float *)nullptr, dtp, div, pull, glm);
+            return;
+        }
+    }
     if (shadow && wgs)
         hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, true, TL, true>), g, b, 0, s, d, t, in, U, out, shadow, dtp, div,
                            pull, glm);
@@ -1664,26 +1677,28 @@ __global__ void __launch_bounds__(256) k_glm_stage(Dims d, DevTables t, const do
     const long long rb = (long long)g.b * d.NI, tb = (long long)g.b * d.NJ;
     const double *S = in + (long long)g.b * 8 * d.P;
     const double *Ps = psi_in + (long long)g.b * d.P;
-    auto B = [&](int v, int II, int JJ, int KK) -> double { return S[(long long)v * d.P + cell_off(d, II, JJ, KK)]; };
-    auto a1 = [&](int II) -> double {
-        const double x = t.xi[rb + II];
-        return B(4, II, J, K) * (x * x);
+    // K is interior: the cell and its r and theta neighbours are in the main
+    // part (offsets q +- SI, q +- SJ); only the phi neighbours may be ghosts
+    const long long q = cell_off(d, I, J, K), SI = d.SI, SJ = d.SJ;
+    const long long km2 = cell_off(d, I, J, K - 2), km1 = cell_off(d, I, J, K - 1), kp1 = cell_off(d, I, J, K + 1),
+                    kp2 = cell_off(d, I, J, K + 2);
+    const double *Br = S + 4 * d.P, *Bt = S + 5 * d.P, *Bp = S + 6 * d.P;
+    auto a1 = [&](int di) -> double {
+        const double x = t.xi[rb + I + di];
+        return Br[q + di * SI] * (x * x);
     };
-    auto a2 = [&](int JJ) -> double { return B(5, I, JJ, K) * t.sint[tb + JJ]; };
-    auto a3 = [&](int KK) -> double { return B(6, I, J, KK); };
+    auto a2 = [&](int dj) -> double { return Bt[q + dj * SJ] * t.sint[tb + J + dj]; };
     const double rinv = t.rinv[rb + I], sinv = t.sinv[tb + J];
-    const double dr = d4(a1(I - 2), a1(I - 1), a1(I + 1), a1(I + 2), mt.cr) * t.r2inv[rb + I];
-    double d23 = d4(a2(J - 2), a2(J - 1), a2(J + 1), a2(J + 2), mt.ct);
-    d23 = d23 + d4(a3(K - 2), a3(K - 1), a3(K + 1), a3(K + 2), mt.cp);
+    const double dr = d4(a1(-2), a1(-1), a1(1), a1(2), mt.cr) * t.r2inv[rb + I];
+    double d23 = d4(a2(-2), a2(-1), a2(1), a2(2), mt.ct);
+    d23 = d23 + d4(Bp[km2], Bp[km1], Bp[kp1], Bp[kp2], mt.cp);
     d23 = d23 * sinv;
     d23 = d23 * rinv;
     const double divb = dr + d23;
-    auto p = [&](int II, int JJ, int KK) -> double { return Ps[cell_off(d, II, JJ, KK)]; };
-    const double g1 = d4(p(I - 2, J, K), p(I - 1, J, K), p(I + 1, J, K), p(I + 2, J, K), mt.cr);
-    const double g2 = d4(p(I, J - 2, K), p(I, J - 1, K), p(I, J + 1, K), p(I, J + 2, K), mt.ct) * rinv;
-    double g3 = d4(p(I, J, K - 2), p(I, J, K - 1), p(I, J, K + 1), p(I, J, K + 2), mt.cp) * rinv;
+    const double g1 = d4(Ps[q - 2 * SI], Ps[q - SI], Ps[q + SI], Ps[q + 2 * SI], mt.cr);
+    const double g2 = d4(Ps[q - 2 * SJ], Ps[q - SJ], Ps[q + SJ], Ps[q + 2 * SJ], mt.ct) * rinv;
+    double g3 = d4(Ps[km2], Ps[km1], Ps[kp1], Ps[kp2], mt.cp) * rinv;
     g3 = g3 * sinv;
-    const long long q = cell_off(d, I, J, K);
     const double *Ub = Un + (long long)g.b * 8 * d.P + q;
     const double vr = Ub[d.P], vt = Ub[2 * d.P], vp = Ub[3 * d.P];
     const double br = Ub[4 * d.P], bt = Ub[5 * d.P], bp = Ub[6 * d.P];
@@ -1720,24 +1735,32 @@ __global__ void __launch_bounds__(256) k_glm_diffusion(Dims d, DevTables t, cons
     const double *Sb = U + (long long)g.b * 8 * d.P;
     const double *Pb = psi + (long long)g.b * d.P;
     const double *cs = t.cs + rb;
-    auto ps = [&](int II, int JJ, int KK) -> double { return Pb[cell_off(d, II, JJ, KK)]; };
-    auto sp = [&](int v, int II, int JJ, int KK) -> double {
-        return fabs(Sb[(long long)v * d.P + cell_off(d, II, JJ, KK)]);
-    };
-    const double c1m = sp(1, I - 1, J - 2, K - 2) + cs[I - 1];
-    const double c10 = sp(1, I, J - 2, K - 2) + cs[I];
-    const double c1p = sp(1, I + 1, J - 2, K - 2) + cs[I + 1];
-    const double f0 = (face_flux(ps(I - 2, J, K), ps(I - 1, J, K), ps(I, J, K), ps(I + 1, J, K)) * (c10 + c1m)) * 0.5;
-    const double f1 = (face_flux(ps(I - 1, J, K), ps(I, J, K), ps(I + 1, J, K), ps(I + 2, J, K)) * (c1p + c10)) * 0.5;
+    // the cell and its r and theta neighbours are in the main part; the phi
+    // neighbours and the speed cells at K - 2 may be ghosts
+    const long long q = cell_off(d, I, J, K), SI = d.SI, SJ = d.SJ;
+    const long long km2 = cell_off(d, I, J, K - 2), km1 = cell_off(d, I, J, K - 1), kp1 = cell_off(d, I, J, K + 1),
+                    kp2 = cell_off(d, I, J, K + 2);
+    const double *V1 = Sb + d.P, *V2 = Sb + 2 * d.P, *V3 = Sb + 3 * d.P;
+    // speed cells (transverse index - ng): (I', J - 2, K - 2), (I - 2, J', K - 2),
+    // (I - 2, J - 2, K'); K - 2 lies in the side part below the interior, whose
+    // plane and row pitches are NJ 2ng and 2ng
+    const long long s1 = cell_off(d, I, J - 2, K - 2), s2 = cell_off(d, I - 2, J, K - 2);
+    const bool kmain = K - 2 >= d.ng;
+    const long long pI = kmain ? SI : (long long)d.NJ * 2 * d.ng, pJ = kmain ? SJ : 2 * d.ng;
+    const double c1m = fabs(V1[s1 - pI]) + cs[I - 1];
+    const double c10 = fabs(V1[s1]) + cs[I];
+    const double c1p = fabs(V1[s1 + pI]) + cs[I + 1];
+    const double p0 = Pb[q];
+    const double f0 = (face_flux(Pb[q - 2 * SI], Pb[q - SI], p0, Pb[q + SI]) * (c10 + c1m)) * 0.5;
+    const double f1 = (face_flux(Pb[q - SI], p0, Pb[q + SI], Pb[q + 2 * SI]) * (c1p + c10)) * 0.5;
     const double csI2 = cs[I - 2];
-    const double c2m = sp(2, I - 2, J - 1, K - 2) + csI2, c20 = sp(2, I - 2, J, K - 2) + csI2,
-                 c2p = sp(2, I - 2, J + 1, K - 2) + csI2;
-    const double fj0 = (face_flux(ps(I, J - 2, K), ps(I, J - 1, K), ps(I, J, K), ps(I, J + 1, K)) * (c20 + c2m)) * 0.5;
-    const double fj1 = (face_flux(ps(I, J - 1, K), ps(I, J, K), ps(I, J + 1, K), ps(I, J + 2, K)) * (c2p + c20)) * 0.5;
-    const double c3m = sp(3, I - 2, J - 2, K - 1) + csI2, c30 = sp(3, I - 2, J - 2, K) + csI2,
-                 c3p = sp(3, I - 2, J - 2, K + 1) + csI2;
-    const double fk0 = (face_flux(ps(I, J, K - 2), ps(I, J, K - 1), ps(I, J, K), ps(I, J, K + 1)) * (c30 + c3m)) * 0.5;
-    const double fk1 = (face_flux(ps(I, J, K - 1), ps(I, J, K), ps(I, J, K + 1), ps(I, J, K + 2)) * (c3p + c30)) * 0.5;
+    const double c2m = fabs(V2[s2 - pJ]) + csI2, c20 = fabs(V2[s2]) + csI2, c2p = fabs(V2[s2 + pJ]) + csI2;
+    const double fj0 = (face_flux(Pb[q - 2 * SJ], Pb[q - SJ], p0, Pb[q + SJ]) * (c20 + c2m)) * 0.5;
+    const double fj1 = (face_flux(Pb[q - SJ], p0, Pb[q + SJ], Pb[q + 2 * SJ]) * (c2p + c20)) * 0.5;
+    const double c3m = fabs(V3[cell_off(d, I - 2, J - 2, K - 1)]) + csI2, c30 = fabs(V3[cell_off(d, I - 2, J - 2, K)]) + csI2,
+                 c3p = fabs(V3[cell_off(d, I - 2, J - 2, K + 1)]) + csI2;
+    const double fk0 = (face_flux(Pb[km2], Pb[km1], p0, Pb[kp1]) * (c30 + c3m)) * 0.5;
+    const double fk1 = (face_flux(Pb[km1], p0, Pb[kp1], Pb[kp2]) * (c3p + c30)) * 0.5;
     const double xic = t.xi[rb + I];
     const double L2 = mt.dxj * xic;
     const double L3 = (mt.dxk * xic) * t.sint[tb + J];
@@ -1745,8 +1768,7 @@ __global__ void __launch_bounds__(256) k_glm_diffusion(Dims d, DevTables t, cons
     r = r + (f0 - f1) / mt.dxi;
     r = r + (fj0 - fj1) / L2;
     r = r + (fk0 - fk1) / L3;
-    const long long pq = (long long)g.b * d.P + cell_off(d, I, J, K);
-    psi_new[pq] = psi[pq] + (0.5 * *dtp) * r;
+    psi_new[(long long)g.b * d.P + q] = p0 + (0.5 * *dtp) * r;
 }
 
 // psi's ghost fill from the FP64 ghost tables: plain copies, then the
