@@ -132,12 +132,13 @@ void launch_stage(const Dims &d, const DevTables &t, const TIN *in, const TB *U,
 // cell_off layout.  Stage: psi_out = psi_n + dt R(psi)/div from the stage
 // input (state `in`, psi_in) and the step's state Un, and G = grad(psi_in);
 // diffusion: psi_new = psi + 0.5 dt R_diff(psi) with the speeds of U;
-// exchange: psi's ghosts from the FP64 ghost tables.
+// ghost fill: the FP64 state's and psi's ghosts in one pass over the FP64
+// ghost tables (psi a scalar: no Yin-Yang rotation).
 void launch_glm_stage(const Dims &d, const DevTables &t, const double *in, const double *Un, const double *psi_in,
                       const double *psi_n, double *psi_out, double *G, const double *dtp, double div, hipStream_t s);
 void launch_glm_diffusion(const Dims &d, const DevTables &t, const double *U, const double *psi, double *psi_new,
                           const double *dtp, hipStream_t s);
-void launch_exchange_psi(const Dims &d, const DevGhosts &g, double *psi, hipStream_t s);
+void launch_exchange_glm(const Dims &d, const DevGhosts &g, double *state, double *psi, hipStream_t s);
 // Uold: U^n for the FP32 path, whose diffusion updates U (= out) in place;
 // null for the FP64 path (U != out)
 template <typename T, typename TIN>

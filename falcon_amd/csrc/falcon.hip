@@ -575,16 +575,16 @@ void enqueue_step(falcon_handle *h, double cfl, int cur) {
                                                              nullptr, ts, ws, G);
             });
             timed(h, k < 4 ? K_XSTAGE : K_XSTATE, [&] {
-                launch_exchange<double>(d, h->gh64, out[k - 1], s);
-                if (glm) launch_exchange_psi(d, h->gh64, pout[k - 1], s);
+                if (glm) launch_exchange_glm(d, h->gh64, out[k - 1], pout[k - 1], s);
+                else launch_exchange<double>(d, h->gh64, out[k - 1], s);
             });
         }
         timed(h, K_DIFF, [&] { launch_diffusion<double, double>(d, t, U, U, Un, nullptr, part, dtp, s, nullptr, td, wd); });
         if (glm) timed(h, K_OTHER, [&] { launch_glm_diffusion(d, t, U, P, Pn, dtp, s); });
         timed(h, K_OTHER, [&] { launch_copy_radial<double>(d, h->bnd, U, Un, s); });
         timed(h, K_XSTATE, [&] {
-            launch_exchange<double>(d, h->gh64, Un, s);
-            if (glm) launch_exchange_psi(d, h->gh64, Pn, s);
+            if (glm) launch_exchange_glm(d, h->gh64, Un, Pn, s);
+            else launch_exchange<double>(d, h->gh64, Un, s);
         });
     } else {
         float *S0 = h->S32[0].as<float>(), *S1 = h->S32[1].as<float>();
@@ -1303,14 +1303,16 @@ int falcon_sendrecv_gc(falcon_handle_t h, int which) {
         h->need_tables();
         ensure_device_tables(h);
         if (which == 0) {
-            launch_exchange<double>(h->dims, h->gh64, h->U[h->cur].as<double>(), h->stream);
-            if (h->divb) launch_exchange_psi(h->dims, h->gh64, h->psi[h->cur].as<double>(), h->stream);
+            if (h->divb)
+                launch_exchange_glm(h->dims, h->gh64, h->U[h->cur].as<double>(), h->psi[h->cur].as<double>(), h->stream);
+            else launch_exchange<double>(h->dims, h->gh64, h->U[h->cur].as<double>(), h->stream);
             h->shadow_valid = false;
         } else {
             alloc_stage(h);
-            if (h->precision == 0) launch_exchange<double>(h->dims, h->gh64, h->S64[0].as<double>(), h->stream);
+            if (h->precision == 0 && h->divb)
+                launch_exchange_glm(h->dims, h->gh64, h->S64[0].as<double>(), h->psiS[0].as<double>(), h->stream);
+            else if (h->precision == 0) launch_exchange<double>(h->dims, h->gh64, h->S64[0].as<double>(), h->stream);
             else launch_exchange<float>(h->dims, h->gh32, h->S32[0].as<float>(), h->stream);
-            if (h->divb) launch_exchange_psi(h->dims, h->gh64, h->psiS[0].as<double>(), h->stream);
         }
         check_launch();
         HIPCHECK(hipStreamSynchronize(h->stream));

@@ -42,6 +42,15 @@ __device__ __forceinline__ int xcd_tile(int w, int n) {
     return x * q + min(x, r) + k;
 }
 
+// psi (the ninth variable of the reference's MHD blocks, 3_6/ModBlock.f90:163)
+// is one padded variable per block, [block][P] in the cell_off layout; the
+// ghost tables address [block][8][P] state buffers, so an entry's offset o
+// maps to (o / 8P) P + o mod 8P.
+__device__ __forceinline__ long long psi_off(const Dims &d, long long o) {
+    const long long P8 = 8 * d.P, b = o / P8;
+    return b * d.P + (o - b * P8);
+}
+
 // radial coefficient rows the stage kernel stages in LDS, and the largest padded
 // radial extent NI a march accepts (the launchers reject larger blocks)
 constexpr int kStageRows = 10;
@@ -883,8 +892,10 @@ __global__ void __launch_bounds__(TL * TL) k_diffusion(Dims d, DevTables t, cons
 // ghost cell that the ghost pull leaves unwritten; the FP32 buffers are zeroed
 // when allocated and written with finite values only, so that term is f*0 = +-0
 // as well.
-template <typename TS, bool SKIP>
-__global__ void __launch_bounds__(256) k_exchange(Dims d, DevGhosts g, TS *__restrict__ st) {
+// PSI: also psi of GLM divergence cleaning ([block][P], FP64), a scalar
+template <typename TS, bool SKIP, bool PSI = false>
+__global__ void __launch_bounds__(256) k_exchange(Dims d, DevGhosts g, TS *__restrict__ st,
+                                                  double *__restrict__ psi = nullptr) {
     const int e = g.first + (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (e >= g.n_total) return;
     const long long P = d.P;
@@ -928,6 +939,26 @@ __global__ void __launch_bounds__(256) k_exchange(Dims d, DevGhosts g, TS *__res
     }
 #pragma unroll
     for (int v = 0; v < 8; v++) st[dst + (long long)v * P] = (TS)val[v];
+    if constexpr (PSI) {
+        const double *F = psi + psi_off(d, src);
+        const double f000 = (i0 && j0 && k0) ? F[0] : 0.0;
+        const double f100 = (i1 && j0 && k0) ? F[SI] : 0.0;
+        const double f010 = (i0 && j1 && k0) ? F[SJ] : 0.0;
+        const double f001 = (i0 && j0 && k1) ? F[1] : 0.0;
+        const double f110 = (i1 && j1 && k0) ? F[SI + SJ] : 0.0;
+        const double f011 = (i0 && j1 && k1) ? F[SJ + 1] : 0.0;
+        const double f101 = (i1 && j0 && k1) ? F[SI + 1] : 0.0;
+        const double f111 = (i1 && j1 && k1) ? F[SI + SJ + 1] : 0.0;
+        double t = f000 * (1.0 - wi) * (1.0 - wj) * (1.0 - wk);
+        t = t + f100 * (wi) * (1.0 - wj) * (1.0 - wk);
+        t = t + f010 * (1.0 - wi) * (wj) * (1.0 - wk);
+        t = t + f001 * (1.0 - wi) * (1.0 - wj) * (wk);
+        t = t + f110 * (wi) * (wj) * (1.0 - wk);
+        t = t + f011 * (1.0 - wi) * (wj) * (wk);
+        t = t + f101 * (wi) * (1.0 - wj) * (wk);
+        t = t + f111 * (wi) * (wj) * (wk);
+        psi[psi_off(d, dst)] = t;
+    }
 }
 
 // Ghost entries filled by a plain copy, sorted by destination so consecutive
@@ -938,9 +969,9 @@ __global__ void __launch_bounds__(256) k_exchange(Dims d, DevGhosts g, TS *__res
 // also include entries whose weights lie within 1e-12 of 0 or 1 (same-level
 // neighbours whose cell centres coincide up to rounding), copied from the
 // nearest donor cell.
-template <typename TS>
+template <typename TS, bool PSI = false>
 __global__ void __launch_bounds__(256) k_exchange_copy(Dims d, const int2 *__restrict__ copy, int n,
-                                                       TS *__restrict__ st) {
+                                                       TS *__restrict__ st, double *__restrict__ psi = nullptr) {
     const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (e >= n) return;
     const int2 c = copy[e];
@@ -950,6 +981,7 @@ __global__ void __launch_bounds__(256) k_exchange_copy(Dims d, const int2 *__res
     for (int v = 0; v < 8; v++) val[v] = st[c.y + (long long)v * P];
 #pragma unroll
     for (int v = 0; v < 8; v++) st[c.x + (long long)v * P] = val[v] + TS(0);
+    if constexpr (PSI) psi[psi_off(d, c.x)] = psi[psi_off(d, c.y)] + 0.0;
 }
 
 // FP32 buffers: same-panel entries whose theta/phi weights lie within 1e-12 of
@@ -1626,14 +1658,6 @@ void launch_allcells(const Dims &d, const double *U, const double *geo, const in
 
 // ---------------------------------------------------------------- GLM divergence cleaning
 
-// psi (the ninth variable of the reference's MHD blocks, 3_6/ModBlock.f90:163)
-// is one padded variable per block, [block][P] in the cell_off layout; the
-// ghost tables address [block][8][P] state buffers, so an entry's offset o
-// maps to (o / 8P) P + o mod 8P.
-__device__ __forceinline__ long long psi_off(const Dims &d, long long o) {
-    const long long P8 = 8 * d.P, b = o / P8;
-    return b * d.P + (o - b * P8);
-}
 
 // interior cell c of all blocks (phi fastest): block, interior and padded indices
 struct GlmCell {
@@ -1771,43 +1795,6 @@ __global__ void __launch_bounds__(256) k_glm_diffusion(Dims d, DevTables t, cons
     psi_new[(long long)g.b * d.P + q] = p0 + (0.5 * *dtp) * r;
 }
 
-// psi's ghost fill from the FP64 ghost tables: plain copies, then the
-// weighted entries (zero-weight corners skipped, as k_exchange's FP64
-// instantiation); psi is a scalar, so Yin-Yang entries are not rotated.
-__global__ void __launch_bounds__(256) k_exchange_copy_psi(Dims d, const int2 *__restrict__ copy, int n,
-                                                           double *__restrict__ psi) {
-    const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (e >= n) return;
-    const int2 c = copy[e];
-    psi[psi_off(d, c.x)] = psi[psi_off(d, c.y)] + 0.0;
-}
-
-__global__ void __launch_bounds__(256) k_exchange_psi(Dims d, DevGhosts g, double *__restrict__ psi) {
-    const int e = g.first + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (e >= g.n_total) return;
-    const int SI = (int)d.SI, SJ = d.SJ;
-    const double wi = g.w[3 * (long long)e], wj = g.w[3 * (long long)e + 1], wk = g.w[3 * (long long)e + 2];
-    const bool i0 = wi != 1.0, i1 = wi != 0.0, j0 = wj != 1.0, j1 = wj != 0.0, k0 = wk != 1.0, k1 = wk != 0.0;
-    const double *F = psi + psi_off(d, g.src[e]);
-    const double f000 = (i0 && j0 && k0) ? F[0] : 0.0;
-    const double f100 = (i1 && j0 && k0) ? F[SI] : 0.0;
-    const double f010 = (i0 && j1 && k0) ? F[SJ] : 0.0;
-    const double f001 = (i0 && j0 && k1) ? F[1] : 0.0;
-    const double f110 = (i1 && j1 && k0) ? F[SI + SJ] : 0.0;
-    const double f011 = (i0 && j1 && k1) ? F[SJ + 1] : 0.0;
-    const double f101 = (i1 && j0 && k1) ? F[SI + 1] : 0.0;
-    const double f111 = (i1 && j1 && k1) ? F[SI + SJ + 1] : 0.0;
-    double v = f000 * (1.0 - wi) * (1.0 - wj) * (1.0 - wk);
-    v = v + f100 * (wi) * (1.0 - wj) * (1.0 - wk);
-    v = v + f010 * (1.0 - wi) * (wj) * (1.0 - wk);
-    v = v + f001 * (1.0 - wi) * (1.0 - wj) * (wk);
-    v = v + f110 * (wi) * (wj) * (1.0 - wk);
-    v = v + f011 * (1.0 - wi) * (wj) * (wk);
-    v = v + f101 * (wi) * (1.0 - wj) * (wk);
-    v = v + f111 * (wi) * (wj) * (wk);
-    psi[psi_off(d, g.dst[e])] = v;
-}
-
 static unsigned glm_grid(const Dims &d) {
     const long long n = (long long)d.ni * d.nj * d.nk * d.nblocks;
     return (unsigned)((n + 255) / 256);
@@ -1826,14 +1813,14 @@ void launch_glm_diffusion(const Dims &d, const DevTables &t, const double *U, co
     hipLaunchKernelGGL(k_glm_diffusion, dim3(glm_grid(d)), dim3(256), 0, s, d, t, U, psi, psi_new, dtp);
 }
 
-void launch_exchange_psi(const Dims &d, const DevGhosts &g, double *psi, hipStream_t s) {
-    if (g.n_lerp > 0) throw std::runtime_error("psi exchange: the FP64 ghost tables have no r-only entries");
+void launch_exchange_glm(const Dims &d, const DevGhosts &g, double *state, double *psi, hipStream_t s) {
+    if (g.n_lerp > 0) throw std::runtime_error("GLM ghost fill: the FP64 ghost tables have no r-only entries");
     if (g.n_copy > 0)
-        hipLaunchKernelGGL(k_exchange_copy_psi, dim3((unsigned)((g.n_copy + 255) / 256)), dim3(256), 0, s, d, g.copy,
-                           g.n_copy, psi);
+        hipLaunchKernelGGL((k_exchange_copy<double, true>), dim3((unsigned)((g.n_copy + 255) / 256)), dim3(256), 0, s, d,
+                           g.copy, g.n_copy, state, psi);
     if (g.n_total > g.first)
-        hipLaunchKernelGGL(k_exchange_psi, dim3((unsigned)((g.n_total - g.first + 255) / 256)), dim3(256), 0, s, d, g,
-                           psi);
+        hipLaunchKernelGGL((k_exchange<double, true, true>), dim3((unsigned)((g.n_total - g.first + 255) / 256)),
+                           dim3(256), 0, s, d, g, state, psi);
 }
 
 // explicit instantiations: FP64 path, FP32-deviation path
