@@ -390,6 +390,8 @@ def test_fp64_1000_steps_on_the_64x128x384_shell_match_the_oracle():
     for ib in range(s.nblocks):
         s.set_background(ib, bgs[ib])
         s.set_state(ib, sts[ib])
+    s.sendrecv_gc(0)      # the driver's two ghost fills (3_6/test1.f90:143-144)
+    s.sendrecv_gc(0)
     n = 0
     for target in cfg2.STEPS:
         dt = s.advance(target - n, cfg2.CFL, graph=True)
