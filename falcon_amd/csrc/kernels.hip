@@ -340,10 +340,11 @@ __device__ __forceinline__ void cell_rhs(const TW (&win)[8][5], LDSF L, const T 
 // pull (null: none): ghost cells of `in` in the directions it lists are read
 // from the same-level neighbour's interior (Marcher).
 // GLM (FP64 only): grad(psi) from `glm` is subtracted from R(B) (GLM
-// divergence cleaning); a separate instantiation, so the plain one is unchanged
+// divergence cleaning); a separate instantiation, so the plain one is unchanged.
+// The body is shared by two kernels, k_stage and k_stage_w2 (below).
 template <typename T, typename TIN, typename TOUT, typename TB, int MODE, bool SHADOW, int TL, bool PERSIST,
           bool GLM = false>
-__global__ void __launch_bounds__(TL * TL) k_stage(Dims d, DevTables t, const TIN *__restrict__ in,
+__device__ __forceinline__ void stage_body(Dims d, DevTables t, const TIN *__restrict__ in,
                                                const TB *__restrict__ U, TOUT *__restrict__ out,
                                                float *__restrict__ shadow, const double *__restrict__ dtp,
                                                double div, const int32_t *__restrict__ pull,
@@ -496,6 +497,31 @@ __global__ void __launch_bounds__(TL * TL) k_stage(Dims d, DevTables t, const TI
         // the next tile's LDS writes wait for this tile's last reads
         if (tile + (int)gridDim.x < ntot) __syncthreads();
     }
+}
+
+template <typename T, typename TIN, typename TOUT, typename TB, int MODE, bool SHADOW, int TL, bool PERSIST,
+          bool GLM = false>
+__global__ void __launch_bounds__(TL * TL) k_stage(Dims d, DevTables t, const TIN *__restrict__ in,
+                                               const TB *__restrict__ U, TOUT *__restrict__ out,
+                                               float *__restrict__ shadow, const double *__restrict__ dtp,
+                                               double div, const int32_t *__restrict__ pull,
+                                               const double *__restrict__ glm) {
+    stage_body<T, TIN, TOUT, TB, MODE, SHADOW, TL, PERSIST, GLM>(d, t, in, U, out, shadow, dtp, div, pull, glm);
+}
+
+// The FP64 step's stage (16-edge tiles, one workgroup per tile) held to two
+// wavefronts per SIMD: left to itself the compiler gives it 256 VGPRs plus 2
+// AGPRs, one wavefront per SIMD; held to two it spills two values (12 bytes
+// per lane) and runs in 3.06 instead of 3.94 ms on the 50 M-cell shell.  The
+// other builds either fit two wavefronts already (the FP32 stages) or would
+// spill tens of registers.
+template <typename T, typename TIN, typename TOUT, typename TB, int MODE, bool SHADOW, int TL, bool PERSIST,
+          bool GLM = false>
+__global__ void __launch_bounds__(TL * TL) __attribute__((amdgpu_waves_per_eu(2)))
+k_stage_w2(Dims d, DevTables t, const TIN *__restrict__ in, const TB *__restrict__ U, TOUT *__restrict__ out,
+           float *__restrict__ shadow, const double *__restrict__ dtp, double div,
+           const int32_t *__restrict__ pull, const double *__restrict__ glm) {
+    stage_body<T, TIN, TOUT, TB, MODE, SHADOW, TL, PERSIST, GLM>(d, t, in, U, out, shadow, dtp, div, pull, glm);
 }
 
 // ---------------------------------------------------------------- diffusion
@@ -1308,9 +1334,17 @@ static void stage_tl(const Dims &d, const DevTables &t, const TIN *in, const TB 
             if (wgs)
                 hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, false, TL, true, true>), g, b, 0, s, d, t, in, U, out,
                                    (float *)nullptr, dtp, div, pull, glm);
+            else if (TL == 16)
+                hipLaunchKernelGGL((k_stage_w2<T, TIN, TOUT, TB, 0, false, TL, false, true>), g, b, 0, s, d, t, in, U,
+                                   out, (float *)nullptr, dtp, div, pull, glm);
             else
                 hipLaunchKernelGGL((k_stage<T, TIN, TOUT, TB, 0, false, TL, false, true>), g, b, 0, s, d, t, in, U, out,
                                    (float *)nullptr, dtp, div, pull, glm);
+            return;
+        }
+        if (!shadow && !wgs && TL == 16) {
+            hipLaunchKernelGGL((k_stage_w2<T, TIN, TOUT, TB, 0, false, TL, false>), g, b, 0, s, d, t, in, U, out,
+                               (float *)nullptr, dtp, div, pull, glm);
             return;
         }
     }
