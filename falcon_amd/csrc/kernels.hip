@@ -935,14 +935,33 @@ __global__ void __launch_bounds__(256) k_exchange(Dims d, DevGhosts g, TS *__res
 #pragma unroll
     for (int v = 0; v < 8; v++) {
         const TS *F = st + src + (long long)v * P;
-        const double f000 = (i0 && j0 && k0) ? (double)F[0] : 0.0;
-        const double f100 = (i1 && j0 && k0) ? (double)F[SI] : 0.0;
-        const double f010 = (i0 && j1 && k0) ? (double)F[SJ] : 0.0;
-        const double f001 = (i0 && j0 && k1) ? (double)F[1] : 0.0;
-        const double f110 = (i1 && j1 && k0) ? (double)F[SI + SJ] : 0.0;
-        const double f011 = (i0 && j1 && k1) ? (double)F[SJ + 1] : 0.0;
-        const double f101 = (i1 && j0 && k1) ? (double)F[SI + 1] : 0.0;
-        const double f111 = (i1 && j1 && k1) ? (double)F[SI + SJ + 1] : 0.0;
+        double f000, f100, f010, f001, f110, f011, f101, f111;
+        if constexpr (sizeof(TS) == 8 && SKIP) {
+            // the two phi corners of each (i, j) pair are adjacent: one 16-byte
+            // load per pair (the phi neighbour of a clamped base at the last
+            // interior cell is read in bounds and dropped with its zero weight)
+            auto ld2 = [](const TS *p) -> double2 {
+                double2 r;
+                __builtin_memcpy(&r, p, sizeof(r));
+                return r;
+            };
+            const double2 z = make_double2(0.0, 0.0);
+            const double2 a00 = (i0 && j0) ? ld2(F) : z, a10 = (i1 && j0) ? ld2(F + SI) : z;
+            const double2 a01 = (i0 && j1) ? ld2(F + SJ) : z, a11 = (i1 && j1) ? ld2(F + SI + SJ) : z;
+            f000 = k0 ? a00.x : 0.0; f001 = k1 ? a00.y : 0.0;
+            f100 = k0 ? a10.x : 0.0; f101 = k1 ? a10.y : 0.0;
+            f010 = k0 ? a01.x : 0.0; f011 = k1 ? a01.y : 0.0;
+            f110 = k0 ? a11.x : 0.0; f111 = k1 ? a11.y : 0.0;
+        } else {
+            f000 = (i0 && j0 && k0) ? (double)F[0] : 0.0;
+            f100 = (i1 && j0 && k0) ? (double)F[SI] : 0.0;
+            f010 = (i0 && j1 && k0) ? (double)F[SJ] : 0.0;
+            f001 = (i0 && j0 && k1) ? (double)F[1] : 0.0;
+            f110 = (i1 && j1 && k0) ? (double)F[SI + SJ] : 0.0;
+            f011 = (i0 && j1 && k1) ? (double)F[SJ + 1] : 0.0;
+            f101 = (i1 && j0 && k1) ? (double)F[SI + 1] : 0.0;
+            f111 = (i1 && j1 && k1) ? (double)F[SI + SJ + 1] : 0.0;
+        }
         double t = f000 * (1.0 - wi) * (1.0 - wj) * (1.0 - wk);
         t = t + f100 * (wi) * (1.0 - wj) * (1.0 - wk);
         t = t + f010 * (1.0 - wi) * (wj) * (1.0 - wk);
