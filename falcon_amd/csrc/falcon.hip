@@ -1294,6 +1294,17 @@ int falcon_set_ghost_pull(falcon_handle_t h, int on) {
             HIPCHECK(hipGraphExecDestroy(h->graph));
             h->graph = nullptr;
         }
+        if (on && !h->ghost_pull) {
+            // With the pull on, the ghosts that the kernels read in place are
+            // never written again, yet the FP32 Yin-Yang gather reads some of
+            // them as zero-weight corners (f * 0).  Whatever a run without the
+            // pull left there (a diverged run's NaN included) is cleared: the
+            // FP32 buffers restart from zeros and the shadow of U is rebuilt.
+            if (h->stream) HIPCHECK(hipStreamSynchronize(h->stream));
+            for (DevBuf *b : {&h->S32[0], &h->S32[1], &h->F32[0], &h->F32[1]})
+                if (b->p) HIPCHECK(hipMemset(b->p, 0, b->bytes));
+            h->shadow_valid = false;
+        }
         h->ghost_pull = on != 0;
     });
 }

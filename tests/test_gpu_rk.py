@@ -144,6 +144,40 @@ def test_fp32_ghost_pull_equals_full_exchange(name):
     assert np.array_equal(runs[0], runs[1]) and np.array_equal(runs[0], runs[2])
 
 
+def test_ghost_pull_switched_on_after_a_diverged_run():
+    """A run without the ghost pull that went NaN leaves NaN in every ghost of
+    the FP32 buffers; after a clean state is set and the pull switched on, the
+    steps equal a fresh handle's (the ghosts the pull skips are cleared)."""
+    print("[fp32 pull after NaN] layout=mixed", flush=True)
+    lay = PULL_LAYOUTS["mixed"]
+
+    def fresh():
+        s = Shell(lay)
+        s.set_gc_all()
+        s.background_polytrope(20.0)
+        s.init_synthetic(seed=13)
+        s.set_precision("fp32")
+        return s
+    ref = fresh()
+    clean = _states(ref, ref.nblocks)
+    ref.set_ghost_pull(True)
+    ref.advance(3, 0.5, graph=False)
+    want = _states(ref, ref.nblocks)
+    s = fresh()
+    s.set_ghost_pull(False)
+    st = clean[3].copy()
+    st[1, 5, 5, 5] = np.nan
+    s.set_state(3, st)
+    s.advance(3, 0.5, graph=False)
+    assert s.check_state(1)[0] > 0
+    for ib in range(s.nblocks):
+        s.set_state(ib, clean[ib])
+    s.set_ghost_pull(True)
+    s.advance(3, 0.5, graph=False)
+    got = _states(s, s.nblocks)
+    assert np.all(np.isfinite(got)) and np.array_equal(got, want)
+
+
 @pytest.mark.parametrize("name", ["mixed", "amr16"])
 def test_fp32_tiles_bit_identical(name):
     """The FP32 step with 8-edge stage and/or diffusion tiles, and with a few
