@@ -129,12 +129,15 @@ void launch_stage(const Dims &d, const DevTables &t, const TIN *in, const TB *U,
                   const double *dtp, double div, hipStream_t s, const int32_t *pull, int tile, int wgs,
                   const double *glm = nullptr);
 // GLM divergence cleaning (FP64 path).  psi buffers are [block][P] in the
-// cell_off layout.  Stage: psi_out = psi_n + dt R(psi)/div from the stage
-// input (state `in`, psi_in) and the step's state Un, and G = grad(psi_in);
+// cell_off layout.  Speed (once per step): Ch = c_h and c_h / h_LLL of the
+// step's state U, [block][2][interior cells].  Stage: psi_out = psi_n +
+// dt R(psi)/div from the stage input (state `in`, psi_in) and Ch, and
+// G = grad(psi_in);
 // diffusion: psi_new = psi + 0.5 dt R_diff(psi) with the speeds of U;
 // ghost fill: the FP64 state's and psi's ghosts in one pass over the FP64
 // ghost tables (psi a scalar: no Yin-Yang rotation).
-void launch_glm_stage(const Dims &d, const DevTables &t, const double *in, const double *Un, const double *psi_in,
+void launch_glm_speed(const Dims &d, const DevTables &t, const double *U, double *Ch, hipStream_t s);
+void launch_glm_stage(const Dims &d, const DevTables &t, const double *in, const double *Ch, const double *psi_in,
                       const double *psi_n, double *psi_out, double *G, const double *dtp, double div, hipStream_t s);
 void launch_glm_diffusion(const Dims &d, const DevTables &t, const double *U, const double *psi, double *psi_new,
                           const double *dtp, hipStream_t s);

@@ -127,7 +127,7 @@ struct falcon_handle {
     // the stage buffers, one padded variable per block, and grad(psi) of the
     // current stage
     int divb = 0;
-    DevBuf psi[2], psiS[2], glmG;
+    DevBuf psi[2], psiS[2], glmG, glmC;
     DevBuf diffbuf;          // falcon_state_diff's reduction words
     DevBuf scanbuf;          // falcon_check_state's count and first position
     int cur = 0;             // index of the current state buffer in U[]
@@ -165,7 +165,7 @@ struct falcon_handle {
                          &bnd_list, &ghost_rest32, &ghost_pull32,
 
                          &U[0], &U[1], &F32[0], &F32[1], &S64[0], &S64[1], &S32[0], &S32[1], &dtp, &partial, &rbuf, &diffbuf, &scanbuf,
-                         &psi[0], &psi[1], &psiS[0], &psiS[1], &glmG};
+                         &psi[0], &psi[1], &psiS[0], &psiS[1], &glmG, &glmC};
         for (DevBuf *b : all) b->free();
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -561,15 +561,16 @@ void enqueue_step(falcon_handle *h, double cfl, int cur) {
         const bool glm = h->divb == 1;
         double *P = glm ? h->psi[cur].as<double>() : nullptr, *Pn = glm ? h->psi[1 - cur].as<double>() : nullptr;
         double *PS0 = glm ? h->psiS[0].as<double>() : nullptr, *PS1 = glm ? h->psiS[1].as<double>() : nullptr;
-        double *G = glm ? h->glmG.as<double>() : nullptr;
+        double *G = glm ? h->glmG.as<double>() : nullptr, *Ch = glm ? h->glmC.as<double>() : nullptr;
         const double *pin[4] = {P, PS0, PS1, PS0};
         double *pout[4] = {PS0, PS1, PS0, P};
+        if (glm) timed(h, K_OTHER, [&] { launch_glm_speed(d, t, U, Ch, s); });
         for (int k = 1; k <= 4; k++) {
             const double div = (double)(5.0f - (float)k);
             timed(h, K_BC, [&] { launch_boundary<double>(d, h->bnd, const_cast<double *>(in[k - 1]), s); });
             if (glm)
                 timed(h, K_OTHER,
-                      [&] { launch_glm_stage(d, t, in[k - 1], U, pin[k - 1], P, pout[k - 1], G, dtp, div, s); });
+                      [&] { launch_glm_stage(d, t, in[k - 1], Ch, pin[k - 1], P, pout[k - 1], G, dtp, div, s); });
             timed(h, K_STAGE, [&] {
                 launch_stage<double, double, double, double>(d, t, in[k - 1], U, out[k - 1], nullptr, dtp, div, s,
                                                              nullptr, ts, ws, G);
@@ -1030,6 +1031,8 @@ int falcon_set_divb(falcon_handle_t h, int option) {
             }
             h->glmG.alloc(ng * sizeof(double));
             HIPCHECK(hipMemset(h->glmG.p, 0, ng * sizeof(double)));
+            h->glmC.alloc(ng / 3 * 2 * sizeof(double));
+            HIPCHECK(hipMemset(h->glmC.p, 0, ng / 3 * 2 * sizeof(double)));
         }
         h->divb = option;
     });

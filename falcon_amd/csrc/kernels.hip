@@ -1736,13 +1736,13 @@ __device__ __forceinline__ GlmCell glm_cell(const Dims &d, long long c) {
 //            1/sin(t), then 1/r;
 //   grad psi ModSpherical_Grad_f (:201-239);
 //   R(psi) = -c_h^2 div B - (c_h / h_LLL) psi, c_h = v_wave of the step's
-//            state Un (ModWaveSpeed_Dynamo at the step's start, as k_dt_tile).
+//            state (ModWaveSpeed_Dynamo at the step's start), from Ch.
 // psi_out = psi_n + dt R(psi) / div (3_6/ModAdvance.f90:88-93), and grad psi
 // goes to G ([block][3][interior cells]) for the stage kernel to subtract
 // from R(B).  psi_out may be psi_n (the fourth stage): each lane reads psi_n
 // at its own cell only.
 __global__ void __launch_bounds__(256) k_glm_stage(Dims d, DevTables t, const double *__restrict__ in,
-                                                   const double *__restrict__ Un, const double *__restrict__ psi_in,
+                                                   const double *__restrict__ Ch, const double *__restrict__ psi_in,
                                                    const double *psi_n, double *psi_out, double *__restrict__ G,
                                                    const double *__restrict__ dtp, double div) {
     const long long nn = (long long)d.ni * d.nj * d.nk;
@@ -1776,7 +1776,32 @@ __global__ void __launch_bounds__(256) k_glm_stage(Dims d, DevTables t, const do
     const double g2 = d4(Ps[q - 2 * SJ], Ps[q - SJ], Ps[q + SJ], Ps[q + 2 * SJ], mt.ct) * rinv;
     double g3 = d4(Ps[km2], Ps[km1], Ps[kp1], Ps[kp2], mt.cp) * rinv;
     g3 = g3 * sinv;
-    const double *Ub = Un + (long long)g.b * 8 * d.P + q;
+    // c_h and c_h / h_LLL of the step (k_glm_speed)
+    const long long gb = (long long)g.b * 3 * nn + ((long long)g.i * d.nj + g.j) * d.nk + g.k;
+    const long long cb = (long long)g.b * 2 * nn + ((long long)g.i * d.nj + g.j) * d.nk + g.k;
+    const double w = Ch[cb], wh = Ch[cb + nn];
+    const double rpsi = -((w * w) * divb) - wh * Ps[q];
+    const long long pq = (long long)g.b * d.P + q;
+    psi_out[pq] = psi_n[pq] + (*dtp * rpsi) / div;
+    G[gb] = g1;
+    G[gb + nn] = g2;
+    G[gb + 2 * nn] = g3;
+}
+
+// c_h = v_wave and c_h / h_LLL of every interior cell of the step's state U
+// (ModWaveSpeed_Dynamo, 3_6/ModWaveSpeed.f90:8-27, and h_LLL,
+// 3_6/ModBlock.f90:420-422, as k_dt_tile forms them), once per step for the
+// four GLM stages: Ch[block][2][interior cells].
+__global__ void __launch_bounds__(256) k_glm_speed(Dims d, DevTables t, const double *__restrict__ U,
+                                                   double *__restrict__ Ch) {
+    const long long nn = (long long)d.ni * d.nj * d.nk;
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nn * d.nblocks) return;
+    const GlmCell g = glm_cell(d, c);
+    const int I = g.I, J = g.J, K = g.K;
+    const BlockMeta mt = t.meta[g.b];
+    const long long rb = (long long)g.b * d.NI, tb = (long long)g.b * d.NJ;
+    const double *Ub = U + (long long)g.b * 8 * d.P + cell_off(d, I, J, K);
     const double vr = Ub[d.P], vt = Ub[2 * d.P], vp = Ub[3 * d.P];
     const double br = Ub[4 * d.P], bt = Ub[5 * d.P], bp = Ub[6 * d.P];
     double w = t.cs[rb + I] + sqrt(vr * vr + vt * vt + vp * vp);
@@ -1785,13 +1810,9 @@ __global__ void __launch_bounds__(256) k_glm_stage(Dims d, DevTables t, const do
     double h = mt.dxi, h2 = xi * mt.dxj, h3 = (xi * t.sint[tb + J]) * mt.dxk;
     h = h < h2 ? h : h2;
     h = h < h3 ? h : h3;
-    const double rpsi = -((w * w) * divb) - (w / h) * Ps[q];
-    const long long pq = (long long)g.b * d.P + q;
-    psi_out[pq] = psi_n[pq] + (*dtp * rpsi) / div;
-    const long long gb = (long long)g.b * 3 * nn + ((long long)g.i * d.nj + g.j) * d.nk + g.k;
-    G[gb] = g1;
-    G[gb + nn] = g2;
-    G[gb + 2 * nn] = g3;
+    const long long cb = (long long)g.b * 2 * nn + ((long long)g.i * d.nj + g.j) * d.nk + g.k;
+    Ch[cb] = w;
+    Ch[cb + nn] = w / h;
 }
 
 // The artificial diffusion of psi after the fourth stage
@@ -1853,10 +1874,14 @@ static unsigned glm_grid(const Dims &d) {
     return (unsigned)((n + 255) / 256);
 }
 
-void launch_glm_stage(const Dims &d, const DevTables &t, const double *in, const double *Un, const double *psi_in,
+void launch_glm_speed(const Dims &d, const DevTables &t, const double *U, double *Ch, hipStream_t s) {
+    hipLaunchKernelGGL(k_glm_speed, dim3(glm_grid(d)), dim3(256), 0, s, d, t, U, Ch);
+}
+
+void launch_glm_stage(const Dims &d, const DevTables &t, const double *in, const double *Ch, const double *psi_in,
                       const double *psi_n, double *psi_out, double *G, const double *dtp, double div, hipStream_t s) {
     if (psi_in == psi_out) throw std::runtime_error("GLM stage: psi_in and psi_out must differ");
-    hipLaunchKernelGGL(k_glm_stage, dim3(glm_grid(d)), dim3(256), 0, s, d, t, in, Un, psi_in, psi_n, psi_out, G, dtp,
+    hipLaunchKernelGGL(k_glm_stage, dim3(glm_grid(d)), dim3(256), 0, s, d, t, in, Ch, psi_in, psi_n, psi_out, G, dtp,
                        div);
 }
 
