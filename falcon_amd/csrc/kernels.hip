@@ -424,6 +424,9 @@ __device__ __forceinline__ void stage_body(Dims d, DevTables t, const TIN *__res
             hn[v] = m.halo(v, d.ng);
         }
         const long long ubase = (long long)b * 8 * d.P + m.coff;
+        // G ([block][3][interior cells]) at this lane's column; plane i at + i nj nk
+        const long long gnn = (long long)d.ni * d.nj * d.nk;
+        const double *gcol = GLM ? glm + (b * 3 * gnn + ((long long)(j0 + ty) * d.nk + (k0 + tx))) : nullptr;
 
         for (int i = 0; i < d.ni; i++) {
             const int I = i + d.ng;
@@ -473,12 +476,10 @@ __device__ __forceinline__ void stage_body(Dims d, DevTables t, const TIN *__res
                 if (MODE == 0) {
                     if constexpr (GLM && sizeof(T) == 8) {
                         // ModDivB_GLM (3_6/ModDivB.f90:27-28): R(B) = R(B) - grad(psi)
-                        const long long nn = (long long)d.ni * d.nj * d.nk;
-                        const long long gb =
-                            (long long)b * 3 * nn + ((long long)i * d.nj + (j0 + ty)) * d.nk + (k0 + tx);
-                        R[4] = R[4] - glm[gb];
-                        R[5] = R[5] - glm[gb + nn];
-                        R[6] = R[6] - glm[gb + 2 * nn];
+                        const double *gi = gcol + i * (d.nj * d.nk);
+                        R[4] = R[4] - gi[0];
+                        R[5] = R[5] - gi[gnn];
+                        R[6] = R[6] - gi[2 * gnn];
                     }
                     const long long base = (long long)b * 8 * d.P + (long long)I * m.SI + m.coff;
     #pragma unroll
